@@ -1,0 +1,145 @@
+/*
+ * pf_oracle.h -- CPU restatement of the reference's sparse two-phase
+ * Phase-Field-FLIP time step (lsfs, JangHyun-bin/adaptive-pf-flip).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it.  The product (adaptive-pf-flip_amd/) never links or calls it.
+ *
+ * Every function follows a reference function cited as file:line (paths are
+ * relative to the reference repository root).  The grid scalar is selectable
+ * per grid:
+ *   faithful = 1 : every grid store is rounded through float exactly where the
+ *                  reference stores into its float block pool (reference mode);
+ *   faithful = 0 : grid values are kept in double (the FP64 field oracle).
+ * Particles are double, as in src/particles/particles3d_tp.h:4-35.
+ *
+ * Parity pinning of each function (what checks it, see tests/):
+ *   orc_phi / orc_beta ............ Eq.7 endpoints + monotonicity (test_phasefield.cpp:4-13)
+ *                                   and closed-form values; vs oracle/_ref.
+ *   orc_calibrate_rho_tilde0 ...... brute-force lattice sum in the test; vs oracle/_ref.
+ *   orc_p2g ....................... momentum 4.0 (test_sparse_transfer3d_tp.cpp:11-29), phase
+ *                                   separation (:31-58), partition of unity, brute-force
+ *                                   active sets, bit-exact vs oracle/_ref.
+ *   orc_mark_cells ................ brute-force integer truncation; vs oracle/_ref.
+ *   orc_apply_gravity/_walls ...... closed form; vs oracle/_ref.
+ *   orc_project ................... div < 1e-4 and 8 p-blocks (test_sparse_vc3d.cpp:8-54),
+ *                                   c_div 0.125 (:56-100), dense direct solve on tiny grids,
+ *                                   vs oracle/_ref.
+ *   orc_g2p ....................... 13 / 5 (test_sparse_transfer3d_tp.cpp:60-78); vs oracle/_ref.
+ *   orc_advect .................... clamps 7.5 / 0.5 (:123-161), RK3 closed form (:163-180).
+ *   orc_timestep .................. 0.05 / 0.2 (test_sparse_sim3d_tp.cpp:85-107).
+ *   orc_step ...................... vs oracle/_ref SparseSim3DTP::step on identical state.
+ */
+#ifndef PF_ORACLE_H
+#define PF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_U = 0, ORC_V, ORC_W, ORC_MU, ORC_MV, ORC_MW, ORC_P, ORC_MK, ORC_NFIELDS };
+enum { ORC_AIR = 0, ORC_FLUID = 1, ORC_SOLID = 2 };
+enum { ORC_STOP_REF_MAXABS = 0, ORC_STOP_REL2 = 1 };
+
+typedef struct orc_phase { double rho_l, rho_g, alpha_phi, rho_tilde_0; } orc_phase;
+
+typedef struct orc_solve_stats {
+  int32_t iterations, pinned, converged, breakdown;
+  int64_t fluid_cells;
+  int64_t pinned_cell;           /* linear cell id of the pinned cell or -1 */
+  double initial_max_residual;   /* |r0|_inf */
+  double final_max_residual;     /* |r|_inf of the recursive residual at exit */
+  double true_rel_residual;      /* ||b - A x||_2 / ||b||_2 recomputed in FP64 at exit */
+  double true_max_residual;      /* ||b - A x||_inf recomputed in FP64 at exit */
+} orc_solve_stats;
+
+typedef struct orc_escape_stats {
+  int32_t clamped_liquid, clamped_gas, x_lo, x_hi, y_lo, y_hi, z_lo, z_hi;
+} orc_escape_stats;
+
+typedef struct orc_timestep_stats {
+  double requested_dt, effective_dt, max_particle_speed, cfl_limit_dt;
+  int32_t limited;
+} orc_timestep_stats;
+
+typedef struct orc_grid orc_grid;
+
+/* grid lifecycle and access (src/grid/sparse_block_grid3d.h, sparse_mac_grid3d.h) */
+orc_grid* orc_grid_create(int nx, int ny, int nz, double dx, double ox, double oy, double oz,
+                          int faithful);
+void orc_grid_destroy(orc_grid* g);
+orc_grid* orc_grid_clone(const orc_grid* g);
+void orc_field_dims(const orc_grid* g, int field, int32_t out_dims[3]);
+double orc_get(const orc_grid* g, int field, int i, int j, int k);
+void orc_set(orc_grid* g, int field, int i, int j, int k, double v);
+void orc_field_dense(const orc_grid* g, int field, double* out);
+int64_t orc_active_blocks(const orc_grid* g, int field, int32_t* out);
+void orc_clear_field(orc_grid* g, int field);
+
+/* physics (src/physics/phasefield.h, src/transfer/transfer3d_tp.cpp) */
+double orc_phi(double raw, const orc_phase* pp);
+double orc_beta(double phi, const orc_phase* pp);
+double orc_calibrate_rho_tilde0(const orc_phase* pp, double Vp);
+
+/* step stages (src/driver/sparse_sim3d_tp.cpp, src/driver/sparse_ops3d_tp.cpp) */
+void orc_mark_cells(orc_grid* g, int64_t n, const double* x, const double* y, const double* z);
+void orc_p2g(orc_grid* g, int64_t n, const double* x, const double* y, const double* z,
+             const double* u, const double* v, const double* w, const uint8_t* type,
+             const double* vol, const orc_phase* pp, double Vp);
+void orc_apply_gravity(orc_grid* g, double dt, double gravity);
+void orc_apply_walls(orc_grid* g);
+void orc_project(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, double cg_tol,
+                 double div_correction, int stop_rule, orc_solve_stats* st);
+void orc_g2p(const orc_grid* g, const orc_grid* saved, int64_t n, const double* x,
+             const double* y, const double* z, double* u, double* v, double* w,
+             const uint8_t* type, double aL, double aG);
+void orc_advect(const orc_grid* g, int64_t n, double* x, double* y, double* z,
+                const uint8_t* type, double dt, int order, orc_escape_stats* es,
+                int32_t* escaped_idx, int64_t* n_escaped);
+void orc_timestep(int64_t n, const double* u, const double* v, const double* w, double dx,
+                  double requested_dt, int enabled, double cfl, double min_dt,
+                  orc_timestep_stats* out);
+
+/* whole step (SparseSim3DTP::step, sparse_sim3d_tp.cpp:854-884), default flags:
+ * no adaptivity, no secondary particles, no surface tension. */
+typedef struct orc_step_cfg {
+  double dt, gravity, Vp, alpha_liquid, alpha_gas;
+  int32_t adaptive_timestep, advection_order, cg_iters, stop_rule;
+  double adaptive_cfl, adaptive_min_dt, cg_tol;
+  int32_t c_div_volume_correction, pad0;
+  double c_div_strength, liquid_volume_target;
+  orc_phase phase;
+} orc_step_cfg;
+
+typedef struct orc_step_stats {
+  orc_timestep_stats ts;
+  orc_solve_stats solve;
+  orc_escape_stats escape;
+  double liquid_volume, c_div;
+  double ms_mark, ms_p2g, ms_save_gravity_walls, ms_project, ms_g2p, ms_advect, ms_total;
+} orc_step_stats;
+
+void orc_step(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u, double* v,
+              double* w, const uint8_t* type, const double* vol, const orc_step_cfg* cfg,
+              orc_step_stats* st);
+
+/* seeded synthetic scenes (shared construction with the HIP generator) */
+enum { ORC_SCENE_RT = 0, ORC_SCENE_BUBBLE = 1, ORC_SCENE_DAMBREAK = 2 };
+int64_t orc_scene_count(int kind, int nx, int ny, int nz);
+int64_t orc_scene_fill(int kind, int nx, int ny, int nz, double dx, uint64_t seed,
+                       float* x, float* y, float* z, uint8_t* type);
+
+/* dense two-phase P2G (src/transfer/transfer3d_tp.cpp:37-58), the reference's own
+ * equivalence oracle for the sparse splat.  Outputs dense face arrays. */
+void orc_dense_p2g(int nx, int ny, int nz, double dx, int64_t n, const double* x,
+                   const double* y, const double* z, const double* u, const double* v,
+                   const double* w, const uint8_t* type, const double* vol,
+                   const orc_phase* pp, double Vp, double* U, double* V, double* W, double* MU,
+                   double* MV, double* MW);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
