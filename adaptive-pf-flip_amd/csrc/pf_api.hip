@@ -1,0 +1,876 @@
+// pf_api.hip -- the C ABI of include/pf_gpu.h: context, memory, stage sequencing.
+//
+// Built as one translation unit with the kernel files so every launch is a direct
+// call (no relocatable device code).  All work is stream-ordered on the context's
+// stream; the only host synchronisations are the ones a result needs: the
+// fluid-block count before the solve, the refinement loop's residual check, the
+// convergence flag every `check_every` inner iterations, and downloads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pf_internal.h"
+#include "pf_util.hip"
+#include "pf_bin.hip"
+#include "pf_grid.hip"
+#include "pf_p2g.hip"
+#include "pf_project.hip"
+#include "pf_g2p.hip"
+
+using namespace pf;
+
+#define HIP_TRY(c, expr)                                                         \
+  do {                                                                           \
+    hipError_t _e = (expr);                                                      \
+    if (_e != hipSuccess) {                                                      \
+      (c)->err = std::string(#expr) + ": " + hipGetErrorString(_e);              \
+      return PF_ERR_HIP;                                                         \
+    }                                                                            \
+  } while (0)
+
+#define LAUNCH_CHECK(c) HIP_TRY(c, hipGetLastError())
+
+namespace {
+
+int fail(pf_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, long long n) {
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) return e;
+  return hipMemset(*p, 0, sizeof(T) * (size_t)n);
+}
+
+unsigned grid_for(long long work, long long per_block) {
+  long long g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+unsigned list_grid(long long nlist) {
+  long long g = (nlist + 3) / 4;
+  if (g < 1) g = 1;
+  if (g > kMaxPartials) g = kMaxPartials;
+  return (unsigned)g;
+}
+
+const Dims& dims_of(const pf_ctx* c, int field) {
+  switch (field) {
+    case PF_FIELD_U: case PF_FIELD_MU: case PF_FIELD_SAVED_U: return c->du;
+    case PF_FIELD_V: case PF_FIELD_MV: case PF_FIELD_SAVED_V: return c->dv;
+    case PF_FIELD_W: case PF_FIELD_MW: case PF_FIELD_SAVED_W: return c->dw;
+    default: return c->dc;
+  }
+}
+
+int field_slot(int field) {
+  switch (field) {
+    case PF_FIELD_U: return F_U;
+    case PF_FIELD_V: return F_V;
+    case PF_FIELD_W: return F_W;
+    case PF_FIELD_MU: return F_MU;
+    case PF_FIELD_MV: return F_MV;
+    case PF_FIELD_MW: return F_MW;
+    case PF_FIELD_P: return F_P;
+    case PF_FIELD_MARKER: return F_MK;
+    case PF_FIELD_SAVED_U: return F_SU;
+    case PF_FIELD_SAVED_V: return F_SV;
+    case PF_FIELD_SAVED_W: return F_SW;
+    default: return -1;
+  }
+}
+
+bool is_pow2(double dx) {
+  int e;
+  double m = std::frexp(dx, &e);
+  return m == 0.5;
+}
+
+PhaseK phase_k(const pf_phase* ph) {
+  PhaseK k;
+  k.rho_l = ph->rho_l;
+  k.rho_g = ph->rho_g;
+  // etaPhi(pp) * pp.rho_g * pp.rho_tilde_0, phasefield.h:10-11
+  k.rmin = std::log(ph->rho_l / ph->rho_g) * ph->rho_g * ph->rho_tilde_0;
+  k.den = ph->alpha_phi * ph->rho_tilde_0 * ph->rho_l;
+  return k;
+}
+
+int grow_particles(pf_ctx* c, long long n, bool vol) {
+  if (n <= c->pcap && (!vol || c->pa.vol)) return PF_OK;
+  long long cap = std::max(n, c->pcap);
+  PBuf* bufs[2] = {&c->pa, &c->pb};
+  for (PBuf* b : bufs) {
+    if (n > c->pcap) {
+      hipFree(b->x); hipFree(b->y); hipFree(b->z); hipFree(b->u); hipFree(b->v); hipFree(b->w);
+      hipFree(b->type); hipFree(b->id); hipFree(b->vol);
+      *b = PBuf{};
+      HIP_TRY(c, dalloc(&b->x, cap)); HIP_TRY(c, dalloc(&b->y, cap)); HIP_TRY(c, dalloc(&b->z, cap));
+      HIP_TRY(c, dalloc(&b->u, cap)); HIP_TRY(c, dalloc(&b->v, cap)); HIP_TRY(c, dalloc(&b->w, cap));
+      HIP_TRY(c, dalloc(&b->type, cap)); HIP_TRY(c, dalloc(&b->id, cap));
+    }
+    if (vol && !b->vol) HIP_TRY(c, dalloc(&b->vol, cap));
+  }
+  if (n > c->pcap) {
+    hipFree(c->key); hipFree(c->rank);
+    HIP_TRY(c, dalloc(&c->key, cap));
+    HIP_TRY(c, dalloc(&c->rank, cap));
+    c->pcap = cap;
+  }
+  return PF_OK;
+}
+
+// counting sort of particles by block-major cell index + per-colour source lists
+int ensure_binned(pf_ctx* c) {
+  if (c->binned) return PF_OK;
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  HIP_TRY(c, fill_u32(c->stream, c->cell_count, 0u, ncell));
+  if (c->np > 0) {
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa, c->geom,
+                       c->dc, c->key, c->rank, c->cell_count);
+    LAUNCH_CHECK(c);
+  }
+  HIP_TRY(c, scan_u32(*c, c->cell_count, c->cell_start, ncell));
+  if (c->np > 0) {
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa, c->pb,
+                       c->has_vol, c->key, c->rank, c->cell_start);
+    LAUNCH_CHECK(c);
+    std::swap(c->pa, c->pb);
+  }
+  HIP_TRY(c, fill_u32(c->stream, c->color_cnt, 0u, 8));
+  hipLaunchKernelGGL(k_color_lists, dim3(grid_for(c->dc.nblocks, 256)), dim3(256), 0, c->stream, c->dc,
+                     c->cell_start, c->color_cnt, c->color_list, c->color_cap);
+  LAUNCH_CHECK(c);
+  c->binned = true;
+  return PF_OK;
+}
+
+int run_mark(pf_ctx* c) {
+  int st = ensure_binned(c);
+  if (st) return st;
+  hipLaunchKernelGGL(k_mark, dim3(grid_for(c->dc.nblocks, 4)), dim3(256), 0, c->stream, c->dc, c->geom,
+                     c->cell_start, c->marker, c->act[F_MK]);
+  LAUNCH_CHECK(c);
+  return PF_OK;
+}
+
+int run_face_ops(pf_ctx* c, int ops, float dtg) {
+  const Dims* d[3] = {&c->du, &c->dv, &c->dw};
+  for (int ax = 0; ax < 3; ++ax) {
+    dim3 grid(grid_for(d[ax]->nblocks, 4));
+    float* vel = c->f[F_U + ax];
+    const float* mass = c->f[F_MU + ax];
+    float* sv = c->f[F_SU + ax];
+    uint8_t* av = c->act[F_U + ax];
+    uint8_t* as = c->act[F_SU + ax];
+    if (ax == 0)
+      hipLaunchKernelGGL(k_face_ops<0>, grid, dim3(256), 0, c->stream, *d[ax], c->geom, vel, mass, sv, av, as, ops, dtg);
+    else if (ax == 1)
+      hipLaunchKernelGGL(k_face_ops<1>, grid, dim3(256), 0, c->stream, *d[ax], c->geom, vel, mass, sv, av, as, ops, dtg);
+    else
+      hipLaunchKernelGGL(k_face_ops<2>, grid, dim3(256), 0, c->stream, *d[ax], c->geom, vel, mass, sv, av, as, ops, dtg);
+    LAUNCH_CHECK(c);
+  }
+  return PF_OK;
+}
+
+int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp) {
+  int st = ensure_binned(c);
+  if (st) return st;
+  const Dims* d[3] = {&c->du, &c->dv, &c->dw};
+  for (int ax = 0; ax < 3; ++ax) {
+    HIP_TRY(c, fill_f32(c->stream, c->f[F_U + ax], 0.f, d[ax]->nblocks * kBlockVol));
+    HIP_TRY(c, fill_f32(c->stream, c->f[F_MU + ax], 0.f, d[ax]->nblocks * kBlockVol));
+    HIP_TRY(c, fill_u8(c->stream, c->act[F_U + ax], 0, d[ax]->nblocks));
+    HIP_TRY(c, fill_u8(c->stream, c->act[F_MU + ax], 0, d[ax]->nblocks));
+  }
+  P2GArgs a;
+  a.g = c->geom;
+  a.dc = c->dc;
+  for (int ax = 0; ax < 3; ++ax) {
+    a.df[ax] = *d[ax];
+    a.mom[ax] = c->f[F_U + ax];
+    a.mass[ax] = c->f[F_MU + ax];
+    a.act_vel[ax] = c->act[F_U + ax];
+    a.act_mass[ax] = c->act[F_MU + ax];
+  }
+  a.start = c->cell_start;
+  a.P = c->pa;
+  a.has_vol = c->has_vol;
+  a.pow2 = is_pow2(c->geom.dx);
+  a.inv_dx = 1.0 / c->geom.dx;
+  a.mp_l = (float)(ph->rho_l * Vp);
+  a.mp_g = (float)(ph->rho_g * Vp);
+  if (c->np == 0) return PF_OK;
+  for (int col = 0; col < 8; ++col) {
+    a.list = c->color_list + (long long)col * c->color_cap;
+    a.count = c->color_cnt + col;
+    hipLaunchKernelGGL(k_p2g_color, dim3((unsigned)std::max<long long>(c->color_cap, 1)), dim3(64), 0, c->stream, a);
+    LAUNCH_CHECK(c);
+  }
+  return PF_OK;
+}
+
+int read_small(pf_ctx* c) {
+  HIP_TRY(c, hipMemcpyAsync(c->red_host, c->red, sizeof(double) * 16, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(c->u32_host, c->u32_dev, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+void record(pf_ctx* c, bool timing, int slot) {
+  if (timing) hipEventRecord(c->ev[slot], c->stream);
+}
+
+int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cfg, double cdiv,
+                pf_solve_stats* out, bool timing, int ev_assemble, int ev_solve, int ev_vel) {
+  pf_solve_stats S;
+  std::memset(&S, 0, sizeof(S));
+  S.pinned_cell = -1;
+  const Geom& g = c->geom;
+  const long long nb = c->dc.nblocks;
+  // 1. blocks of the previous solve return to zero; the pressure field is cleared
+  if (c->npblk_prev > 0) {
+    ZeroArgs z{};
+    z.list = c->pblk_prev;
+    z.nlist = c->npblk_prev;
+    float* f32[12] = {c->f[F_P], c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->cd32, c->cx32, c->cy32, c->cz32, c->invd32};
+    for (int q = 0; q < 12; ++q) z.f32[q] = f32[q];
+    z.nf32 = 12;
+    double* f64[7] = {c->b64, c->x64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64};
+    for (int q = 0; q < 7; ++q) z.f64[q] = f64[q];
+    z.nf64 = 7;
+    hipLaunchKernelGGL(k_zero_blocks, dim3(grid_for(c->npblk_prev, 4)), dim3(256), 0, c->stream, z);
+    LAUNCH_CHECK(c);
+  }
+  HIP_TRY(c, fill_u8(c->stream, c->act[F_P], 0, nb));
+  // 2. sorted list of blocks holding FLUID cells
+  uint8_t* flag8 = reinterpret_cast<uint8_t*>(c->blk_flag32);
+  hipLaunchKernelGGL(k_fluid_flags, dim3(grid_for(nb, 4)), dim3(256), 0, c->stream, c->dc, c->marker, flag8);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, compact_flags(*c, flag8, nb, c->pblk, c->u32_dev + 2));
+  HIP_TRY(c, fill_u32(c->stream, c->u32_dev, 0u, 1));
+  HIP_TRY(c, fill_u32(c->stream, c->u32_dev + 1, 0xffffffffu, 1));
+  HIP_TRY(c, fill_u32(c->stream, c->u32_dev + 4, 0u, 1));
+  HIP_TRY(c, hipMemcpyAsync(c->u32_host, c->u32_dev, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const long long npb = c->u32_host[2];
+  c->npblk = npb;
+  if (npb == 0) {
+    std::swap(c->pblk, c->pblk_prev);
+    c->npblk_prev = 0;
+    c->npblk = 0;
+    S.converged = 1;
+    record(c, timing, ev_assemble);
+    record(c, timing, ev_solve);
+    record(c, timing, ev_vel);
+    if (out) *out = S;
+    return PF_OK;
+  }
+  ProjArgs pa;
+  pa.g = g; pa.dc = c->dc; pa.du = c->du; pa.dv = c->dv; pa.dw = c->dw;
+  pa.marker = c->marker; pa.list = c->pblk; pa.nlist = npb;
+  pa.ph = phase_k(ph);
+  pa.scale = dt / (g.dx * g.dx);
+  pa.cdiv = cdiv;
+  pa.pin = c->u32_dev;
+  hipLaunchKernelGGL(k_pin_detect, dim3(grid_for(npb, 4)), dim3(256), 0, c->stream, pa);
+  LAUNCH_CHECK(c);
+  AsmOut ao;
+  ao.b64 = c->b64; ao.cd64 = c->cd64; ao.cx64 = c->cx64; ao.cy64 = c->cy64; ao.cz64 = c->cz64;
+  ao.cd32 = c->cd32; ao.cx32 = c->cx32; ao.cy32 = c->cy32; ao.cz32 = c->cz32; ao.invd32 = c->invd32;
+  ao.u = c->f[F_U]; ao.v = c->f[F_V]; ao.w = c->f[F_W];
+  ao.mu = c->f[F_MU]; ao.mv = c->f[F_MV]; ao.mw = c->f[F_MW];
+  ao.part = c->partial;
+  const unsigned G = list_grid(npb);
+  hipLaunchKernelGGL(k_assemble, dim3(G), dim3(256), 0, c->stream, pa, ao);
+  LAUNCH_CHECK(c);
+  hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 1, 0u, c->red + 2);
+  LAUNCH_CHECK(c);
+  record(c, timing, ev_assemble);
+
+  // 3. FP64 iterative refinement around the FP32 Jacobi PCG
+  VecArgs va;
+  va.dc = c->dc; va.list = c->pblk; va.nlist = npb; va.sc = c->sc; va.part = c->partial;
+  hipLaunchKernelGGL(k_ir_start, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->f[F_P], c->cd64, cfg->warm_start);
+  LAUNCH_CHECK(c);
+  auto residual = [&]() -> int {
+    hipLaunchKernelGGL(k_residual64, dim3(G), dim3(256), 0, c->stream, va, c->b64, c->x64, c->r64, c->cd64, c->cx64,
+                       c->cy64, c->cz64);
+    LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 2, 2u, c->red);
+    LAUNCH_CHECK(c);
+    return read_small(c);
+  };
+  int st = residual();
+  if (st) return st;
+  const double bnorm2 = c->red_host[2];
+  double rr = c->red_host[0], rmax = c->red_host[1];
+  S.rhs_norm2 = bnorm2;
+  S.initial_max_residual = rmax;
+  S.pinned = c->u32_host[0] == 0;
+  S.pinned_cell = S.pinned ? (int64_t)c->u32_host[1] : -1;
+  const int max_iters = std::max(cfg->max_iters, 0);
+  const int max_outer = cfg->max_outer > 0 ? cfg->max_outer : 50;
+  const int check = cfg->check_every > 0 ? cfg->check_every : 8;
+  const double inner = cfg->inner_rel_tol > 0 ? cfg->inner_rel_tol : 1e-4;
+  auto met = [&]() {
+    if (cfg->stop_rule == PF_STOP_REL2) return bnorm2 == 0.0 || std::sqrt(rr) <= cfg->tol * std::sqrt(bnorm2);
+    return rmax < cfg->tol;
+  };
+  int total = 0, outer = 0;
+  bool converged = met();
+  while (!converged && total < max_iters && outer < max_outer && rmax > 0.0 && std::isfinite(rmax)) {
+    const double s = 1.0 / rmax;
+    hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(256), 0, c->stream, va, c->r64, s, c->x32, c->r32, c->z32, c->pA,
+                       c->invd32);
+    LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, inner * inner,
+                       max_iters - total, 64);
+    LAUNCH_CHECK(c);
+    float* pold = c->pA;
+    float* pnew = c->pB;
+    for (;;) {
+      for (int q = 0; q < check; ++q) {
+        hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(256), 0, c->stream, va, c->z32, pold, pnew, c->Ap, c->cd32,
+                           c->cx32, c->cy32, c->cz32);
+        hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
+        hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, c->stream, va, c->x32, c->r32, c->z32, pnew, c->Ap,
+                           c->invd32);
+        hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
+        std::swap(pold, pnew);
+      }
+      LAUNCH_CHECK(c);
+      HIP_TRY(c, hipMemcpyAsync(c->done_host, &c->sc->iter, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      if (c->done_host[1]) break;  // {iter, done, breakdown, stalled}
+    }
+    const int it = c->done_host[0];
+    if (c->done_host[2]) S.breakdown = 1;
+    total += it;
+    ++outer;
+    hipLaunchKernelGGL(k_ir_accum, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->x32, rmax);
+    LAUNCH_CHECK(c);
+    st = residual();
+    if (st) return st;
+    rr = c->red_host[0];
+    rmax = c->red_host[1];
+    converged = met();
+    if (it == 0) break;
+  }
+  S.iterations = total;
+  S.outer_iterations = outer;
+  S.converged = converged ? 1 : 0;
+  S.final_max_residual = rmax;
+  S.true_max_residual = rmax;
+  S.true_rel_residual = bnorm2 > 0 ? std::sqrt(rr / bnorm2) : 0.0;
+  S.fluid_cells = c->u32_host[4];
+  record(c, timing, ev_solve);
+
+  // 4. pressure store and face velocity update
+  hipLaunchKernelGGL(k_store_pressure, dim3(G), dim3(256), 0, c->stream, va, c->marker, c->x64, c->f[F_P],
+                     c->act[F_P]);
+  LAUNCH_CHECK(c);
+  VelArgs vl;
+  vl.g = g; vl.dc = c->dc; vl.df[0] = c->du; vl.df[1] = c->dv; vl.df[2] = c->dw;
+  vl.marker = c->marker; vl.p = c->f[F_P];
+  for (int ax = 0; ax < 3; ++ax) {
+    vl.vel[ax] = c->f[F_U + ax];
+    vl.mass[ax] = c->f[F_MU + ax];
+    vl.act_vel[ax] = c->act[F_U + ax];
+  }
+  vl.ph = pa.ph;
+  vl.s = dt / g.dx;
+  vl.list = c->pblk;
+  vl.nlist = npb;
+  hipLaunchKernelGGL(k_velocity_update, dim3(grid_for(npb, 4)), dim3(256), 0, c->stream, vl);
+  LAUNCH_CHECK(c);
+  record(c, timing, ev_vel);
+  std::swap(c->pblk, c->pblk_prev);
+  c->npblk_prev = npb;
+  if (out) *out = S;
+  return PF_OK;
+}
+
+int run_g2p_advect(pf_ctx* c, bool g2p, bool adv, double aL, double aG, double dt, int order) {
+  if (c->np == 0) return PF_OK;
+  G2PArgs a;
+  a.g = c->geom;
+  a.df[0] = c->du; a.df[1] = c->dv; a.df[2] = c->dw;
+  for (int ax = 0; ax < 3; ++ax) {
+    a.vel[ax] = c->f[F_U + ax];
+    a.saved[ax] = c->f[F_SU + ax];
+  }
+  a.P = c->pa;
+  a.n = c->np;
+  a.inv_dx = (float)(1.0 / c->geom.dx);
+  a.ox = (float)c->geom.ox; a.oy = (float)c->geom.oy; a.oz = (float)c->geom.oz;
+  const Geom& g = c->geom;
+  a.lo[0] = (float)(g.ox + 0.5 * g.dx); a.hi[0] = (float)(g.ox + (g.nx - 0.5) * g.dx);
+  a.lo[1] = (float)(g.oy + 0.5 * g.dx); a.hi[1] = (float)(g.oy + (g.ny - 0.5) * g.dx);
+  a.lo[2] = (float)(g.oz + 0.5 * g.dx); a.hi[2] = (float)(g.oz + (g.nz - 0.5) * g.dx);
+  a.aL = (float)aL; a.aG = (float)aG; a.dt = (float)dt;
+  a.order = order;
+  a.do_g2p = g2p; a.do_advect = adv;
+  a.esc = c->esc_dev;
+  a.vmax_bits = reinterpret_cast<uint32_t*>(c->vmax_dev);
+  hipLaunchKernelGGL(k_g2p_advect, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, a);
+  LAUNCH_CHECK(c);
+  if (adv) c->binned = false;
+  return PF_OK;
+}
+
+// calibrateRhoTilde0, src/transfer/transfer3d_tp.cpp:29-36: raw u-face mass at
+// (4,4,4) of an 8^3 lattice with 8 liquid particles per cell, over rho_l.
+double calibrate_host(const pf_phase* pp, double Vp) {
+  const int n = 8;
+  std::vector<double> M((size_t)(n + 1) * n * n, 0.0);
+  const double mp = pp->rho_l * Vp;
+  auto kern = [](double d2) {
+    double q = d2 / (1.5 * 1.5), t = 1.0 - q;
+    return t > 0.0 ? t * t * t : 0.0;
+  };
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i)
+        for (int s = 0; s < 8; ++s) {
+          double gx = i + 0.25 + 0.5 * (s & 1), gy = j + 0.25 + 0.5 * ((s >> 1) & 1) - 0.5,
+                 gz = k + 0.25 + 0.5 * ((s >> 2) & 1) - 0.5;
+          int i0 = (int)std::floor(gx), j0 = (int)std::floor(gy), k0 = (int)std::floor(gz);
+          double wsum = 0.0;
+          for (int dk = -2; dk <= 3; ++dk)
+            for (int dj = -2; dj <= 3; ++dj)
+              for (int di = -2; di <= 3; ++di) {
+                int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0 || ii >= n + 1 || jj < 0 || jj >= n || kk < 0 || kk >= n) continue;
+                double dx = gx - ii, dy = gy - jj, dz = gz - kk;
+                wsum += kern(dx * dx + dy * dy + dz * dz);
+              }
+          if (wsum <= 0.0) continue;
+          for (int dk = -2; dk <= 3; ++dk)
+            for (int dj = -2; dj <= 3; ++dj)
+              for (int di = -2; di <= 3; ++di) {
+                int ii = i0 + di, jj = j0 + dj, kk = k0 + dk;
+                if (ii < 0 || ii >= n + 1 || jj < 0 || jj >= n || kk < 0 || kk >= n) continue;
+                double dx = gx - ii, dy = gy - jj, dz = gz - kk;
+                double w = kern(dx * dx + dy * dy + dz * dz) / wsum;
+                if (w <= 0.0) continue;
+                M[ii + (size_t)(n + 1) * (jj + (size_t)n * kk)] += w * mp;
+              }
+        }
+  double raw = M[4 + (size_t)(n + 1) * (4 + (size_t)n * 4)];
+  return raw > 0 ? raw / pp->rho_l : 1.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_abi_version(void) { return PF_ABI_VERSION; }
+
+const char* pf_last_error(const pf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* pf_stream(pf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int pf_synchronize(pf_ctx* c) {
+  if (!c) return PF_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+int pf_create(pf_ctx** out, const pf_grid_desc* d) {
+  if (!out || !d) return PF_ERR_ARG;
+  *out = nullptr;
+  if (d->nx < 3 || d->ny < 3 || d->nz < 3 || !(d->dx > 0.0)) return PF_ERR_ARG;
+  long long ncell_est = (long long)d->nx * d->ny * d->nz;
+  if (ncell_est > (1LL << 31)) return PF_ERR_ARG;  // cell keys and ranks are 32-bit
+  pf_ctx* c = new pf_ctx();
+  HIP_TRY(c, hipGetDevice(&c->device));
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->geom = Geom{d->nx, d->ny, d->nz, d->dx, d->ox, d->oy, d->oz};
+  c->du = make_dims(d->nx + 1, d->ny, d->nz);
+  c->dv = make_dims(d->nx, d->ny + 1, d->nz);
+  c->dw = make_dims(d->nx, d->ny, d->nz + 1);
+  c->dc = make_dims(d->nx, d->ny, d->nz);
+  const Dims* fd[F_N] = {&c->du, &c->dv, &c->dw, &c->du, &c->dv, &c->dw, &c->dc, &c->dc, &c->du, &c->dv, &c->dw};
+  for (int f = 0; f < F_N; ++f) {
+    if (f != F_MK) HIP_TRY(c, dalloc(&c->f[f], fd[f]->nblocks * kBlockVol));
+    HIP_TRY(c, dalloc(&c->act[f], fd[f]->nblocks));
+  }
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  HIP_TRY(c, dalloc(&c->marker, ncell));
+  HIP_TRY(c, dalloc(&c->cell_count, ncell));
+  HIP_TRY(c, dalloc(&c->cell_start, ncell + 1));
+  c->color_cap = (long long)((c->dc.nbx + 1) / 2) * ((c->dc.nby + 1) / 2) * ((c->dc.nbz + 1) / 2);
+  HIP_TRY(c, dalloc(&c->color_list, 8 * c->color_cap));
+  HIP_TRY(c, dalloc(&c->color_cnt, 8));
+  long long maxblk = std::max(std::max(c->du.nblocks, c->dv.nblocks), std::max(c->dw.nblocks, c->dc.nblocks));
+  HIP_TRY(c, dalloc(&c->pblk, maxblk));
+  HIP_TRY(c, dalloc(&c->pblk_prev, maxblk));
+  HIP_TRY(c, dalloc(&c->blk_flag32, maxblk));
+  HIP_TRY(c, dalloc(&c->blk_scan, maxblk + 1));
+  float** f32[] = {&c->x32, &c->r32, &c->z32, &c->pA, &c->pB, &c->Ap, &c->cd32, &c->cx32, &c->cy32, &c->cz32, &c->invd32};
+  for (float** p : f32) HIP_TRY(c, dalloc(p, ncell));
+  double** f64[] = {&c->b64, &c->x64, &c->r64, &c->cd64, &c->cx64, &c->cy64, &c->cz64};
+  for (double** p : f64) HIP_TRY(c, dalloc(p, ncell));
+  HIP_TRY(c, dalloc(&c->sc, 1));
+  HIP_TRY(c, dalloc(&c->partial, (long long)kMaxPartials * 4));
+  HIP_TRY(c, dalloc(&c->red, 16));
+  HIP_TRY(c, hipHostMalloc((void**)&c->red_host, sizeof(double) * 16));
+  HIP_TRY(c, hipHostMalloc((void**)&c->done_host, sizeof(int) * 16));
+  HIP_TRY(c, dalloc(&c->u32_dev, 16));
+  HIP_TRY(c, hipHostMalloc((void**)&c->u32_host, sizeof(uint32_t) * 16));
+  HIP_TRY(c, dalloc(&c->esc_dev, 8));
+  HIP_TRY(c, dalloc(&c->vmax_dev, 1));
+  c->scan_tile_cap = (std::max(ncell, maxblk) + 4095) / 4096 + 1;
+  HIP_TRY(c, dalloc(&c->scan_tiles, c->scan_tile_cap));
+  long long maxnodes = maxblk * kBlockVol;
+  HIP_TRY(c, dalloc(&c->stage_f, maxnodes));
+  c->stage_cap = maxnodes;
+  for (int e = 0; e <= PF_NSTAGES; ++e) HIP_TRY(c, hipEventCreate(&c->ev[e]));
+  // cell starts must be valid before the first binning (empty particle set)
+  c->binned = false;
+  *out = c;
+  return PF_OK;
+}
+
+int pf_destroy(pf_ctx* c) {
+  if (!c) return PF_OK;
+  hipStreamSynchronize(c->stream);
+  for (int f = 0; f < F_N; ++f) { hipFree(c->f[f]); hipFree(c->act[f]); }
+  hipFree(c->marker); hipFree(c->cell_count); hipFree(c->cell_start);
+  hipFree(c->color_list); hipFree(c->color_cnt); hipFree(c->pblk); hipFree(c->pblk_prev);
+  hipFree(c->blk_flag32); hipFree(c->blk_scan);
+  PBuf* bufs[2] = {&c->pa, &c->pb};
+  for (PBuf* b : bufs) {
+    hipFree(b->x); hipFree(b->y); hipFree(b->z); hipFree(b->u); hipFree(b->v); hipFree(b->w);
+    hipFree(b->type); hipFree(b->id); hipFree(b->vol);
+  }
+  hipFree(c->key); hipFree(c->rank);
+  float* f32[] = {c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->cd32, c->cx32, c->cy32, c->cz32, c->invd32};
+  for (float* p : f32) hipFree(p);
+  double* f64[] = {c->b64, c->x64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64};
+  for (double* p : f64) hipFree(p);
+  hipFree(c->sc); hipFree(c->partial); hipFree(c->red);
+  hipHostFree(c->red_host); hipHostFree(c->done_host); hipHostFree(c->u32_host);
+  hipFree(c->u32_dev); hipFree(c->esc_dev); hipFree(c->vmax_dev);
+  hipFree(c->scan_tiles); hipFree(c->stage_f);
+  for (int e = 0; e <= PF_NSTAGES; ++e)
+    if (c->ev[e]) hipEventDestroy(c->ev[e]);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return PF_OK;
+}
+
+int pf_particles_upload(pf_ctx* c, int64_t n, const float* x, const float* y, const float* z, const float* u,
+                        const float* v, const float* w, const uint8_t* type, const float* volume) {
+  if (!c || n < 0) return PF_ERR_ARG;
+  if (n > 0 && (!x || !y || !z || !u || !v || !w || !type)) return fail(c, PF_ERR_ARG, "null particle array");
+  if (n >= (1LL << 32)) return fail(c, PF_ERR_ARG, "particle count exceeds 32-bit ids");
+  int st = grow_particles(c, n, volume != nullptr);
+  if (st) return st;
+  c->np = n;
+  c->has_vol = volume != nullptr;
+  if (n > 0) {
+    size_t fb = sizeof(float) * (size_t)n;
+    HIP_TRY(c, hipMemcpyAsync(c->pa.x, x, fb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pa.y, y, fb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pa.z, z, fb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pa.u, u, fb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pa.v, v, fb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pa.w, w, fb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->pa.type, type, (size_t)n, hipMemcpyHostToDevice, c->stream));
+    if (volume) HIP_TRY(c, hipMemcpyAsync(c->pa.vol, volume, fb, hipMemcpyHostToDevice, c->stream));
+    std::vector<uint32_t> ids((size_t)n);
+    for (int64_t p = 0; p < n; ++p) ids[(size_t)p] = (uint32_t)p;
+    HIP_TRY(c, hipMemcpyAsync(c->pa.id, ids.data(), sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, fill_u32(c->stream, c->u32_dev, 0u, 1));
+    hipLaunchKernelGGL(k_validate, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (long long)n, c->pa, c->geom,
+                       c->u32_dev);
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipMemcpyAsync(c->u32_host, c->u32_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->u32_host[0]) {
+      c->np = 0;
+      return fail(c, PF_ERR_ARG, "particle position outside the grid's cell range");
+    }
+  }
+  double lv = 0.0;
+  for (int64_t p = 0; p < n; ++p)
+    if (type[p] == 0) lv += volume ? (double)volume[p] : 1.0;
+  c->liquid_volume = lv;
+  c->binned = false;
+  return PF_OK;
+}
+
+int pf_particles_download(pf_ctx* c, float* x, float* y, float* z, float* u, float* v, float* w) {
+  if (!c || !x || !y || !z || !u || !v || !w) return PF_ERR_ARG;
+  const long long n = c->np;
+  if (n == 0) return PF_OK;
+  PBuf t = c->pb;
+  hipLaunchKernelGGL(k_unsort, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, c->pa, t.x, t.y, t.z, t.u, t.v,
+                     t.w);
+  LAUNCH_CHECK(c);
+  size_t fb = sizeof(float) * (size_t)n;
+  HIP_TRY(c, hipMemcpyAsync(x, t.x, fb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(y, t.y, fb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(z, t.z, fb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(u, t.u, fb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(v, t.v, fb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(w, t.w, fb, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+int64_t pf_particle_count(const pf_ctx* c) { return c ? c->np : -1; }
+
+int pf_scene_generate(pf_ctx* c, int32_t kind, uint64_t seed) {
+  if (!c || kind < 0 || kind > 2) return PF_ERR_ARG;
+  const Geom& g = c->geom;
+  // filled cells form the box [1, ie) x [1, je) x [1, nz-1): cell-centre tests
+  // y < 0.6 (bubble, dam-break) and x < 0.4 (dam-break)
+  int ie = g.nx - 1, je = g.ny - 1;
+  if (kind == 1 || kind == 2) {
+    je = 1;
+    while (je < g.ny - 1 && (je + 0.5) / g.ny < 0.6) ++je;
+  }
+  if (kind == 2) {
+    ie = 1;
+    while (ie < g.nx - 1 && (ie + 0.5) / g.nx < 0.4) ++ie;
+  }
+  long long ncells = (long long)(ie - 1) * (je - 1) * (g.nz - 2);
+  long long n = 8 * ncells;
+  int st = grow_particles(c, n, false);
+  if (st) return st;
+  c->np = n;
+  c->has_vol = false;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scene, dim3(grid_for(ncells, 256)), dim3(256), 0, c->stream, (int)kind, g, ie, je, seed, c->pa);
+    LAUNCH_CHECK(c);
+  }
+  // liquid volume (type 0 count): read back the types once
+  std::vector<uint8_t> t((size_t)n);
+  if (n > 0) {
+    HIP_TRY(c, hipMemcpyAsync(t.data(), c->pa.type, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  long long liq = 0;
+  for (uint8_t q : t) liq += q == 0;
+  c->liquid_volume = (double)liq;
+  c->binned = false;
+  return PF_OK;
+}
+
+int pf_mark_cells(pf_ctx* c) {
+  if (!c) return PF_ERR_ARG;
+  return run_mark(c);
+}
+
+int pf_p2g(pf_ctx* c, const pf_phase* ph, double Vp) {
+  if (!c || !ph) return PF_ERR_ARG;
+  int st = run_p2g_splat(c, ph, Vp);
+  if (st) return st;
+  return run_face_ops(c, FO_NORMALIZE, 0.f);
+}
+
+int pf_save_velocity(pf_ctx* c) {
+  if (!c) return PF_ERR_ARG;
+  return run_face_ops(c, FO_SAVE, 0.f);
+}
+
+int pf_apply_gravity(pf_ctx* c, double dt, double gravity) {
+  if (!c) return PF_ERR_ARG;
+  return run_face_ops(c, FO_GRAVITY, (float)(dt * gravity));
+}
+
+int pf_apply_walls(pf_ctx* c) {
+  if (!c) return PF_ERR_ARG;
+  return run_face_ops(c, FO_WALLS, 0.f);
+}
+
+int pf_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cfg, double div_correction,
+               pf_solve_stats* stats) {
+  if (!c || !ph || !cfg) return PF_ERR_ARG;
+  return run_project(c, ph, dt, cfg, div_correction, stats, false, 0, 0, 0);
+}
+
+int pf_g2p(pf_ctx* c, double aL, double aG) {
+  if (!c) return PF_ERR_ARG;
+  return run_g2p_advect(c, true, false, aL, aG, 0.0, 2);
+}
+
+int pf_advect(pf_ctx* c, double dt, int32_t order, pf_escape_stats* stats) {
+  if (!c) return PF_ERR_ARG;
+  HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->esc_dev), 0u, 8));
+  int st = run_g2p_advect(c, false, true, 0.0, 0.0, dt, order);
+  if (st) return st;
+  if (stats) {
+    HIP_TRY(c, hipMemcpyAsync(stats, c->esc_dev, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  return PF_OK;
+}
+
+int pf_timestep(pf_ctx* c, double requested_dt, int32_t enabled, double cfl, double min_dt, double* eff,
+                double* vmax_out, double* cfl_dt, int32_t* limited) {
+  if (!c) return PF_ERR_ARG;
+  HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->vmax_dev), 0u, 1));
+  if (c->np > 0) {
+    hipLaunchKernelGGL(k_max_speed, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
+                       reinterpret_cast<uint32_t*>(c->vmax_dev));
+    LAUNCH_CHECK(c);
+  }
+  float vm = 0.f;
+  HIP_TRY(c, hipMemcpyAsync(&vm, c->vmax_dev, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  // computeAdaptiveParticleTimestep3D, timestep3d.h:22-49
+  double v = vm, e = requested_dt, cd = requested_dt;
+  int lim = 0;
+  if (enabled && requested_dt > 0.0 && c->geom.dx > 0.0 && cfl > 0.0 && v > 0.0) {
+    cd = cfl * c->geom.dx / v;
+    double l = std::min(requested_dt, cd);
+    if (min_dt > 0.0) l = std::max(min_dt, l);
+    e = std::min(requested_dt, l);
+    lim = e < requested_dt ? 1 : 0;
+  }
+  if (eff) *eff = e;
+  if (vmax_out) *vmax_out = v;
+  if (cfl_dt) *cfl_dt = cd;
+  if (limited) *limited = lim;
+  return PF_OK;
+}
+
+int pf_step(pf_ctx* c, const pf_step_cfg* cfg, pf_step_stats* stats) {
+  if (!c || !cfg) return PF_ERR_ARG;
+  pf_step_stats S;
+  std::memset(&S, 0, sizeof(S));
+  const bool timing = cfg->timing != 0;
+  record(c, timing, 0);
+  int st = ensure_binned(c);
+  if (st) return st;
+  record(c, timing, 1);
+  // timestep3d.h: the speed scan is always done (stats); it gates dt only if enabled
+  st = pf_timestep(c, cfg->dt, cfg->adaptive_timestep, cfg->adaptive_cfl, cfg->adaptive_min_dt, &S.effective_dt,
+                   &S.max_particle_speed, &S.cfl_limit_dt, &S.limited);
+  if (st) return st;
+  const double dt = S.effective_dt;
+  st = run_mark(c);
+  if (st) return st;
+  record(c, timing, 2);
+  st = run_p2g_splat(c, &cfg->phase, cfg->Vp);
+  if (st) return st;
+  record(c, timing, 3);
+  st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | FO_WALLS, (float)(dt * cfg->gravity));
+  if (st) return st;
+  record(c, timing, 4);
+  st = run_project(c, &cfg->phase, dt, &cfg->solve, cfg->c_div, &S.solve, timing, 5, 6, 7);
+  if (st) return st;
+  HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->esc_dev), 0u, 8));
+  st = run_g2p_advect(c, true, true, cfg->alpha_liquid, cfg->alpha_gas, dt, cfg->advection_order);
+  if (st) return st;
+  record(c, timing, 8);
+  HIP_TRY(c, hipMemcpyAsync(&S.escape, c->esc_dev, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (timing) {
+    float ms;
+    for (int s = 0; s < PF_NSTAGES; ++s) {
+      hipEventElapsedTime(&ms, c->ev[s], c->ev[s + 1]);
+      S.stage_ms[s] = ms;
+    }
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[PF_NSTAGES]);
+    S.total_ms = ms;
+  }
+  if (stats) *stats = S;
+  return PF_OK;
+}
+
+int pf_field_dims(const pf_ctx* c, int32_t field, int32_t dims[3]) {
+  if (!c || !dims || field_slot(field) < 0) return PF_ERR_ARG;
+  const Dims& d = dims_of(c, field);
+  dims[0] = d.nx; dims[1] = d.ny; dims[2] = d.nz;
+  return PF_OK;
+}
+
+int pf_field_download(pf_ctx* c, int32_t field, float* dense) {
+  int slot = field_slot(field);
+  if (!c || !dense || slot < 0) return PF_ERR_ARG;
+  const Dims& d = dims_of(c, field);
+  long long n = (long long)d.nx * d.ny * d.nz;
+  hipLaunchKernelGGL(k_to_dense, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d,
+                     slot == F_MK ? nullptr : c->f[slot], slot == F_MK ? c->marker : nullptr, c->stage_f);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, hipMemcpyAsync(dense, c->stage_f, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+int pf_field_upload(pf_ctx* c, int32_t field, const float* dense, const uint8_t* block_active) {
+  int slot = field_slot(field);
+  if (!c || !dense || slot < 0) return PF_ERR_ARG;
+  const Dims& d = dims_of(c, field);
+  long long n = (long long)d.nx * d.ny * d.nz;
+  if (slot == F_MK) {
+    // markers are AIR/FLUID/SOLID and FLUID cells are interior: the pressure
+    // stencil of a FLUID cell then never leaves the grid
+    for (int k = 0; k < d.nz; ++k)
+      for (int j = 0; j < d.ny; ++j)
+        for (int i = 0; i < d.nx; ++i) {
+          float m = dense[i + (long long)d.nx * (j + (long long)d.ny * k)];
+          if (!(m == 0.f || m == 1.f || m == 2.f)) return fail(c, PF_ERR_ARG, "marker values must be 0, 1 or 2");
+          bool edge = i == 0 || j == 0 || k == 0 || i == d.nx - 1 || j == d.ny - 1 || k == d.nz - 1;
+          if (edge && m == 1.f) return fail(c, PF_ERR_ARG, "FLUID marker on a boundary cell");
+        }
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->stage_f, dense, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  uint8_t* act_in = nullptr;
+  if (block_active) {
+    act_in = reinterpret_cast<uint8_t*>(c->blk_flag32);
+    HIP_TRY(c, hipMemcpyAsync(act_in, block_active, (size_t)d.nblocks, hipMemcpyHostToDevice, c->stream));
+  }
+  hipLaunchKernelGGL(k_from_dense, dim3(grid_for(d.nblocks, 4)), dim3(256), 0, c->stream, d, c->stage_f,
+                     slot == F_MK ? nullptr : c->f[slot], slot == F_MK ? c->marker : nullptr, c->act[slot], act_in);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+int pf_active_blocks(pf_ctx* c, int32_t field, int32_t* bids, int64_t capacity, int64_t* n) {
+  int slot = field_slot(field);
+  if (!c || !n || slot < 0) return PF_ERR_ARG;
+  const Dims& d = dims_of(c, field);
+  uint32_t* list = reinterpret_cast<uint32_t*>(c->stage_f);
+  HIP_TRY(c, compact_flags(*c, c->act[slot], d.nblocks, list, c->u32_dev + 3));
+  HIP_TRY(c, hipMemcpyAsync(c->u32_host, c->u32_dev, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  long long cnt = c->u32_host[3];
+  *n = cnt;
+  if (bids && capacity > 0) {
+    long long m = std::min<long long>(cnt, capacity);
+    HIP_TRY(c, hipMemcpyAsync(bids, list, sizeof(int32_t) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  return PF_OK;
+}
+
+double pf_calibrate_rho_tilde0(const pf_phase* ph, double Vp) { return ph ? calibrate_host(ph, Vp) : 0.0; }
+
+double pf_phi_from_raw(double rt, const pf_phase* pp) {
+  double rmin = std::log(pp->rho_l / pp->rho_g) * pp->rho_g * pp->rho_tilde_0;
+  if (rt < rmin) return 0.0;
+  double v = std::sqrt(std::max(rt - rmin, 0.0) / (pp->alpha_phi * pp->rho_tilde_0 * pp->rho_l));
+  return std::min(v, 1.0);
+}
+
+double pf_beta_from_phi(double phi, const pf_phase* pp) { return 1.0 / (phi * pp->rho_l + (1.0 - phi) * pp->rho_g); }
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// pf_bin.hip -- particle binning by cell (counting sort), per-colour block lists,
+// caller-order download and the seeded scene generator.
+//
+// Binning replaces the reference's per-particle block activation bookkeeping with a
+// sort: the key of a particle is the block-major index of its cell
+// (int)((x - o)/dx) evaluated in FP64 exactly as markCells does
+// (src/driver/sparse_sim3d_tp.cpp:43-46).  A cell's particles end up contiguous,
+// blocks in bid order, so every later kernel reads particles coalesced and finds a
+// cell's range from the scanned counts.
+#include "pf_internal.h"
+
+namespace pf {
+
+__device__ inline int cell_coord(float x, double o, double dx) {
+  return (int)(((double)x - o) / dx);
+}
+
+__global__ void k_bin_count(long long n, PBuf P, Geom g, Dims dc, uint32_t* __restrict__ key,
+                            uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i = cell_coord(P.x[p], g.ox, g.dx);
+  int j = cell_coord(P.y[p], g.oy, g.dx);
+  int k = cell_coord(P.z[p], g.oz, g.dx);
+  // positions are validated into the cell range on entry and clamped by advection;
+  // the clamp only guards memory
+  i = min(max(i, 0), g.nx - 1);
+  j = min(max(j, 0), g.ny - 1);
+  k = min(max(k, 0), g.nz - 1);
+  uint32_t kk = (uint32_t)bm_index(dc, i, j, k);
+  key[p] = kk;
+  rank[p] = atomicAdd(&cnt[kk], 1u);
+}
+
+__global__ void k_bin_scatter(long long n, PBuf S, PBuf D, bool has_vol, const uint32_t* __restrict__ key,
+                              const uint32_t* __restrict__ rank, const uint32_t* __restrict__ start) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t d = start[key[p]] + rank[p];
+  D.x[d] = S.x[p]; D.y[d] = S.y[p]; D.z[d] = S.z[p];
+  D.u[d] = S.u[p]; D.v[d] = S.v[p]; D.w[d] = S.w[p];
+  D.type[d] = S.type[p];
+  D.id[d] = S.id[p];
+  if (has_vol) D.vol[d] = S.vol[p];
+}
+
+// Per-colour lists of cell blocks that hold particles (the P2G source blocks).
+// Append order is free: same-colour blocks write disjoint regions.
+__global__ void k_color_lists(Dims dc, const uint32_t* __restrict__ start, uint32_t* __restrict__ cnt,
+                              uint32_t* __restrict__ lists, long long cap) {
+  long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= dc.nblocks) return;
+  if (start[(b + 1) * kBlockVol] == start[b * kBlockVol]) return;
+  int bx, by, bz;
+  block_coords(dc, b, bx, by, bz);
+  int col = color8(bx, by, bz);
+  uint32_t pos = atomicAdd(&cnt[col], 1u);
+  lists[col * cap + pos] = (uint32_t)b;
+}
+
+// Return particles to the caller's order (out[id[p]] = in[p]).
+__global__ void k_unsort(long long n, PBuf S, float* __restrict__ x, float* __restrict__ y,
+                         float* __restrict__ z, float* __restrict__ u, float* __restrict__ v,
+                         float* __restrict__ w) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t d = S.id[p];
+  x[d] = S.x[p]; y[d] = S.y[p]; z[d] = S.z[p];
+  u[d] = S.u[p]; v[d] = S.v[p]; w[d] = S.w[p];
+}
+
+// Validates that positions lie in the cell range (sets *bad otherwise).
+__global__ void k_validate(long long n, PBuf P, Geom g, uint32_t* bad) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double x = P.x[p], y = P.y[p], z = P.z[p];
+  bool ok = x >= g.ox && y >= g.oy && z >= g.oz && x < g.ox + g.nx * g.dx && y < g.oy + g.ny * g.dx &&
+            z < g.oz + g.nz * g.dx;
+  if (ok) {
+    int i = cell_coord(P.x[p], g.ox, g.dx), j = cell_coord(P.y[p], g.oy, g.dx),
+        k = cell_coord(P.z[p], g.oz, g.dx);
+    ok = i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+  }
+  if (!ok) atomicOr(bad, 1u);
+}
+
+// splitmix64 (Steele, Lea, Flood 2014)
+__device__ inline uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Seeded synthetic scene: 8 jittered particles per filled interior cell of the box
+// [1, ie) x [1, je) x [1, nz-1), in (k, j, i, particle) order.  Positions are
+// (float)((i + U[0,1)) * dx); types as in the construction of the three workloads.
+__global__ void k_scene(int kind, Geom g, int ie, int je, uint64_t seed, PBuf P) {
+#pragma clang fp contract(off)
+  long long ni = ie - 1, nj = je - 1, nk = g.nz - 2;
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ni * nj * nk) return;
+  int i = (int)(c % ni) + 1;
+  int j = (int)((c / ni) % nj) + 1;
+  int k = (int)(c / (ni * nj)) + 1;
+  const double dx = g.dx;
+  const double Lx = g.nx * dx, Ly = g.ny * dx, Lz = g.nz * dx;
+  const double two_pi = 6.283185307179586;
+  uint64_t cid = (uint64_t)i + (uint64_t)g.nx * ((uint64_t)j + (uint64_t)g.ny * (uint64_t)k);
+  uint64_t s = seed * 0x9E3779B97F4A7C15ULL ^ (cid + 1) * 0xD1B54A32D192ED03ULL;
+  long long base = c * 8;
+  for (int q = 0; q < 8; ++q) {
+    double ux = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    double uy = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    double uz = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    float fx = (float)__dmul_rn(__dadd_rn((double)i, ux), dx);
+    float fy = (float)__dmul_rn(__dadd_rn((double)j, uy), dx);
+    float fz = (float)__dmul_rn(__dadd_rn((double)k, uz), dx);
+    uint8_t t = 0;
+    if (kind == 0) {
+      double yi = 0.5 * Ly + 0.05 * Lx * cos(two_pi * fx / Lx) * cos(two_pi * fz / Lz);
+      t = ((double)fy > yi) ? 0 : 1;
+    } else if (kind == 1) {
+      double ddx = fx - 0.5 * Lx, ddy = fy - 0.25 * Ly, ddz = fz - 0.5 * Lz, r = 0.12 * Lx;
+      t = (ddx * ddx + ddy * ddy + ddz * ddz < r * r) ? 1 : 0;
+    }
+    long long p = base + q;
+    P.x[p] = fx; P.y[p] = fy; P.z[p] = fz;
+    P.u[p] = 0.f; P.v[p] = 0.f; P.w[p] = 0.f;
+    P.type[p] = t;
+    P.id[p] = (uint32_t)p;
+  }
+}
+
+}  // namespace pf

@@ -1,0 +1,166 @@
+// pf_internal.h -- device data model of the MI355X sparse two-phase PF-FLIP step.
+//
+// Grid storage: every field is a block-major slab of 4^3 blocks (64 values = 256 B
+// for FP32), indexed directly by the reference's block id
+// bid = bx + nbx*(by + nby*bz) (src/grid/sparse_block_grid3d.h:24) and the in-block
+// offset (i&3) + 4*((j&3) + 4*(k&3)) (:48).  A block's pool slot is its bid, so the
+// reference's block-pointer map becomes a one-byte activity flag per block plus
+// sorted compact lists built by scan; inactive blocks always hold zeros, which is
+// what SparseBlockGrid3D::get returns for them (:50-55).  This spends HBM (cheap at
+// 288 GB) to remove the per-tap indirection of the block map.
+//
+// Particles: FP32 structure-of-arrays, kept sorted by cell (block-major cell index)
+// by a counting sort each step; `id` is the caller's index, used to return
+// particles in the caller's order and to fix the within-cell splat order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "pf_gpu.h"
+
+namespace pf {
+
+constexpr int kBlock = 4;
+constexpr int kBlockVol = 64;
+
+// One field family: node extents and block counts (faces have one more node on
+// their own axis, src/grid/sparse_mac_grid3d.h:14-17).
+struct Dims {
+  int nx, ny, nz;
+  int nbx, nby, nbz;
+  long long nblocks;
+};
+
+inline Dims make_dims(int nx, int ny, int nz) {
+  Dims d;
+  d.nx = nx; d.ny = ny; d.nz = nz;
+  d.nbx = (nx + kBlock - 1) / kBlock;
+  d.nby = (ny + kBlock - 1) / kBlock;
+  d.nbz = (nz + kBlock - 1) / kBlock;
+  d.nblocks = (long long)d.nbx * d.nby * d.nbz;
+  return d;
+}
+
+__host__ __device__ inline long long bid_of(const Dims& d, int bx, int by, int bz) {
+  return bx + (long long)d.nbx * (by + (long long)d.nby * bz);
+}
+
+__host__ __device__ inline long long bm_index(const Dims& d, int i, int j, int k) {
+  return bid_of(d, i >> 2, j >> 2, k >> 2) * kBlockVol + (i & 3) + 4 * (j & 3) + 16 * (k & 3);
+}
+
+__host__ __device__ inline void block_coords(const Dims& d, long long b, int& bx, int& by, int& bz) {
+  bx = (int)(b % d.nbx);
+  long long t = b / d.nbx;
+  by = (int)(t % d.nby);
+  bz = (int)(t / d.nby);
+}
+
+// color8, src/grid/block_color.h:7
+__host__ __device__ inline int color8(int bx, int by, int bz) {
+  return (bx & 1) + 2 * (by & 1) + 4 * (bz & 1);
+}
+
+struct Geom {
+  int nx, ny, nz;
+  double dx, ox, oy, oz;
+};
+
+// Particle SoA (one of two ping-pong buffers).
+struct PBuf {
+  float *x, *y, *z, *u, *v, *w;
+  uint8_t* type;
+  float* vol;     // may be null (all 1.0)
+  uint32_t* id;
+};
+
+enum { F_U = 0, F_V, F_W, F_MU, F_MV, F_MW, F_P, F_MK, F_SU, F_SV, F_SW, F_N };
+enum { CELL_AIR = 0, CELL_FLUID = 1, CELL_SOLID = 2 };
+
+// FP64 scalars of one inner PCG solve, kept on the device so the iteration
+// needs no host round trip.
+struct SolveScalars {
+  double rz, pAp, alpha, beta;
+  double rr, rr0, rr_best, tol2;
+  int iter, done, breakdown, stalled;
+  int iter_best, max_iter, stall_window, pad;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geom geom{};
+  Dims du{}, dv{}, dw{}, dc{};   // u-faces, v-faces, w-faces, cells
+  std::string err;
+
+  // grid fields (block-major)
+  float *f[F_N] = {};            // F_MK slot unused (marker is u8)
+  uint8_t* marker = nullptr;
+  uint8_t* act[F_N] = {};        // activity flag per block
+  uint32_t* cell_count = nullptr;   // per cell (dense block-major)
+  uint32_t* cell_start = nullptr;   // exclusive scan, ncells + 1
+
+  // block lists
+  uint32_t* color_list = nullptr;   // 8 * color_cap
+  uint32_t* color_cnt = nullptr;    // 8
+  long long color_cap = 0;
+  uint32_t* pblk = nullptr;         // sorted list of fluid cell blocks
+  uint32_t* pblk_prev = nullptr;    // previous projection's list (zeroed on reuse)
+  long long npblk = 0, npblk_prev = 0;
+  uint32_t* blk_flag32 = nullptr;   // scratch: per-block u32 flags for compaction
+  uint32_t* blk_scan = nullptr;     // scratch: scan of blk_flag32
+
+  // particles
+  long long np = 0, pcap = 0;
+  PBuf pa{}, pb{};                  // pa is current
+  bool has_vol = false;
+  bool binned = false;
+  uint32_t* key = nullptr;
+  uint32_t* rank = nullptr;
+  double liquid_volume = 0.0;       // sum of volume over liquid particles
+
+  // pressure solve (cells, block-major)
+  float *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *pA = nullptr, *pB = nullptr, *Ap = nullptr;
+  float *cd32 = nullptr, *cx32 = nullptr, *cy32 = nullptr, *cz32 = nullptr, *invd32 = nullptr;
+  double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr;
+  double *cd64 = nullptr, *cx64 = nullptr, *cy64 = nullptr, *cz64 = nullptr;
+  SolveScalars* sc = nullptr;       // device
+  double* partial = nullptr;        // per-workgroup partial sums (kMaxPartials * 4)
+  double* red = nullptr;            // reduced values (device, 16 doubles)
+  double* red_host = nullptr;       // pinned host mirror
+  int* done_host = nullptr;         // pinned host mirror of sc->done
+  uint32_t* u32_dev = nullptr;      // small device scratch (pin detection, counters)
+  uint32_t* u32_host = nullptr;     // pinned mirror
+  int32_t* esc_dev = nullptr;       // 8 clamp counters
+  float* vmax_dev = nullptr;        // max particle speed (as float bits via atomicMax)
+
+  // scan scratch
+  uint32_t* scan_tiles = nullptr;
+  long long scan_tile_cap = 0;
+
+  // staging for field transfers
+  float* stage_f = nullptr;
+  long long stage_cap = 0;
+
+  hipEvent_t ev[PF_NSTAGES + 1] = {};
+};
+
+constexpr int kMaxPartials = 4096;
+
+// ---- launch helpers shared by the kernel files (implemented in pf_util.hip etc.)
+hipError_t fill_u8(hipStream_t s, uint8_t* p, uint8_t v, long long n);
+hipError_t fill_u32(hipStream_t s, uint32_t* p, uint32_t v, long long n);
+hipError_t fill_f32(hipStream_t s, float* p, float v, long long n);
+hipError_t fill_f64(hipStream_t s, double* p, double v, long long n);
+// out[0..n] = exclusive scan of in[0..n), out[n] = total
+hipError_t scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, long long n);
+// list = sorted ids b with flag[b] != 0; count written to *count_dev (device)
+hipError_t compact_flags(Ctx& c, const uint8_t* flags, long long n, uint32_t* list,
+                         uint32_t* count_dev);
+
+}  // namespace pf
+
+struct pf_ctx : public pf::Ctx {};

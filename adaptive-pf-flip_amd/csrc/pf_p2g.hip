@@ -1,0 +1,242 @@
+// pf_p2g.hip -- normalized-cubic P2G splat (Eq. 6) into MAC momentum and raw mass.
+//
+// Reference: spP2G3D_tp / preactivateKernel / splatKernelColor
+// (src/driver/sparse_ops3d_tp.cpp:15-75, 95-124).  Weight w = (1 - d^2/r^2)^3, r = 1.5,
+// normalised per particle over the in-bounds nodes of its stencil; mass += w*m_p,
+// momentum += w*m_p*v.
+//
+// MI355X mapping:
+// * One workgroup = one 64-lane wavefront = one 4^3 source block of cells; lane l
+//   owns cell l of the block and splats that cell's particles (they are contiguous
+//   after binning).  For a particle in cell c the non-zero weights of a face field
+//   lie in c-1..c+2 along the face axis and c-1..c+1 across it (the reference's
+//   6-wide loop box adds only zero weights), so a lane keeps its 36 node sums per
+//   axis in registers and a block's contributions stay inside a 7x6x6 node region.
+// * Region sums are accumulated in LDS: for a fixed stencil offset the 64 lanes hit
+//   64 distinct nodes, and one wave's LDS operations complete in program order, so
+//   the sum order is fixed -- no global atomics, and the result does not change
+//   from run to run.  A cell's particles are visited in ascending caller index.
+// * Regions of two source blocks of equal color8 (src/grid/block_color.h:7) are at
+//   least 8 cells apart and never overlap, so each of the 8 colour launches adds its
+//   regions into HBM with plain read-modify-write.
+// * The activation predicate kern(d^2) > 0 is d^2 < 2.25 (exact: d^2/2.25 < 1 in
+//   correctly rounded FP64 iff d^2 < 2.25).  It is evaluated in FP32 and recomputed
+//   in FP64 inside a 1e-5 band around 2.25, so active-block sets match the FP64
+//   reference bit for bit while weights run in FP32.
+#include "pf_internal.h"
+
+namespace pf {
+
+struct P2GArgs {
+  Geom g;
+  Dims dc;
+  Dims df[3];
+  float* mom[3];
+  float* mass[3];
+  uint8_t* act_vel[3];
+  uint8_t* act_mass[3];
+  const uint32_t* start;
+  const uint32_t* list;   // blocks of this colour
+  const uint32_t* count;  // number of blocks in list (device)
+  PBuf P;
+  bool has_vol;
+  bool pow2;              // dx is a power of two: (x - o)/dx == (x - o)*inv_dx exactly
+  double inv_dx;
+  float mp_l, mp_g;       // rho * Vp per phase
+};
+
+constexpr int kOrdMax = 16;
+constexpr float kGuard = 1e-5f;
+
+template <int AX>
+struct Ax {
+  static constexpr int RX = AX == 0 ? 7 : 6;
+  static constexpr int RY = AX == 1 ? 7 : 6;
+  static constexpr int RZ = AX == 2 ? 7 : 6;
+  static constexpr int NX = AX == 0 ? 4 : 3;
+  static constexpr int NY = AX == 1 ? 4 : 3;
+  static constexpr int NZ = AX == 2 ? 4 : 3;
+  static constexpr int NN = NX * NY * NZ;  // 36
+  static constexpr int RN = RX * RY * RZ;  // 252
+};
+
+__device__ inline double grid_coord(float x, double o, const P2GArgs& a) {
+  double d = (double)x - o;
+  return a.pow2 ? d * a.inv_dx : d / a.g.dx;
+}
+
+// exact FP64 activation test and weight, kern() of sparse_ops3d_tp.cpp:15-19
+__device__ inline float exact_weight(double ddx, double ddy, double ddz, bool& active) {
+  double d2 = __dadd_rn(__dadd_rn(__dmul_rn(ddx, ddx), __dmul_rn(ddy, ddy)), __dmul_rn(ddz, ddz));
+  double q = __ddiv_rn(d2, 2.25);
+  double t = __dsub_rn(1.0, q);
+  active = t > 0.0;
+  return active ? (float)(t * t * t) : 0.f;
+}
+
+template <int AX>
+__device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_t s0, uint32_t cnt,
+                                  const uint8_t* ord, float* s_mom, float* s_mass, uint8_t* s_flag,
+                                  int lx, int ly, int lz) {
+  using A = Ax<AX>;
+  const Dims& fd = a.df[AX];
+  float am[A::NN], aq[A::NN];
+  uint64_t fbits = 0;
+#pragma unroll
+  for (int n = 0; n < A::NN; ++n) { am[n] = 0.f; aq[n] = 0.f; }
+
+  // node validity along each axis (field extents), fixed per lane
+  bool vx[A::NX], vy[A::NY], vz[A::NZ];
+#pragma unroll
+  for (int o = 0; o < A::NX; ++o) { int c = i + o - 1; vx[o] = c >= 0 && c < fd.nx; }
+#pragma unroll
+  for (int o = 0; o < A::NY; ++o) { int c = j + o - 1; vy[o] = c >= 0 && c < fd.ny; }
+#pragma unroll
+  for (int o = 0; o < A::NZ; ++o) { int c = k + o - 1; vz[o] = c >= 0 && c < fd.nz; }
+
+  uint32_t last = 0;
+  bool first = true;
+  for (uint32_t it = 0; it < cnt; ++it) {
+    uint32_t q;
+    if (cnt <= (uint32_t)kOrdMax) {
+      q = s0 + ord[it];
+    } else {  // selection by ascending caller index
+      uint32_t best = 0xffffffffu, bi = 0;
+      for (uint32_t t = 0; t < cnt; ++t) {
+        uint32_t id = a.P.id[s0 + t];
+        if ((first || id > last) && id <= best) { best = id; bi = t; }
+      }
+      last = best;
+      first = false;
+      q = s0 + bi;
+    }
+    double px = grid_coord(a.P.x[q], a.g.ox, a), py = grid_coord(a.P.y[q], a.g.oy, a),
+           pz = grid_coord(a.P.z[q], a.g.oz, a);
+    // stencil coordinates of this axis' face grid (sparse_ops3d_tp.cpp:117-122)
+    double gx = AX == 0 ? px : px - 0.5, gy = AX == 1 ? py : py - 0.5, gz = AX == 2 ? pz : pz - 0.5;
+    float fx = (float)(gx - (double)i), fy = (float)(gy - (double)j), fz = (float)(gz - (double)k);
+    float dx2[A::NX], dy2[A::NY], dz2[A::NZ];
+#pragma unroll
+    for (int o = 0; o < A::NX; ++o) { float d = fx - (float)(o - 1); dx2[o] = d * d; }
+#pragma unroll
+    for (int o = 0; o < A::NY; ++o) { float d = fy - (float)(o - 1); dy2[o] = d * d; }
+#pragma unroll
+    for (int o = 0; o < A::NZ; ++o) { float d = fz - (float)(o - 1); dz2[o] = d * d; }
+    float wt[A::NN];
+    float wsum = 0.f;
+#pragma unroll
+    for (int oz = 0; oz < A::NZ; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < A::NY; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < A::NX; ++ox) {
+          const int n = ox + A::NX * (oy + A::NY * oz);
+          float d2 = dx2[ox] + dy2[oy] + dz2[oz];
+          bool inb = vx[ox] && vy[oy] && vz[oz];
+          float w;
+          bool act;
+          if (fabsf(d2 - 2.25f) < kGuard) {
+            w = exact_weight(gx - (double)(i + ox - 1), gy - (double)(j + oy - 1), gz - (double)(k + oz - 1),
+                             act);
+          } else {
+            act = d2 < 2.25f;
+            float t = 1.f - d2 * (1.f / 2.25f);
+            w = act ? t * t * t : 0.f;
+          }
+          act = act && inb;
+          w = act ? w : 0.f;
+          wt[n] = w;
+          wsum += w;
+          if (act) fbits |= (1ull << n);
+        }
+    uint8_t ty = a.P.type[q];
+    float mp = (ty == 0) ? a.mp_l : a.mp_g;
+    if (a.has_vol) mp *= a.P.vol[q];
+    float vel = AX == 0 ? a.P.u[q] : (AX == 1 ? a.P.v[q] : a.P.w[q]);
+    float sm = wsum > 0.f ? mp / wsum : 0.f;
+    float sq = sm * vel;
+#pragma unroll
+    for (int n = 0; n < A::NN; ++n) {
+      am[n] = fmaf(wt[n], sm, am[n]);
+      aq[n] = fmaf(wt[n], sq, aq[n]);
+    }
+  }
+  // deposit into the LDS region: for fixed n the 64 lanes address 64 distinct nodes
+  const int base = lx + A::RX * (ly + A::RY * lz);
+#pragma unroll
+  for (int oz = 0; oz < A::NZ; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < A::NY; ++oy)
+#pragma unroll
+      for (int ox = 0; ox < A::NX; ++ox) {
+        const int n = ox + A::NX * (oy + A::NY * oz);
+        if ((fbits >> n) & 1ull) {
+          const int r = base + ox + A::RX * (oy + A::RY * oz);
+          atomicAdd(&s_mom[r], aq[n]);
+          atomicAdd(&s_mass[r], am[n]);
+          s_flag[r] = 1;
+        }
+      }
+}
+
+template <int AX>
+__device__ inline void flush_axis(const P2GArgs& a, int bx, int by, int bz, const float* s_mom,
+                                  const float* s_mass, const uint8_t* s_flag, int lane) {
+  using A = Ax<AX>;
+  const Dims& fd = a.df[AX];
+  for (int r = lane; r < A::RN; r += 64) {
+    if (!s_flag[r]) continue;
+    int rx = r % A::RX, ry = (r / A::RX) % A::RY, rz = r / (A::RX * A::RY);
+    int ni = bx * 4 - 1 + rx, nj = by * 4 - 1 + ry, nk = bz * 4 - 1 + rz;
+    if (ni < 0 || nj < 0 || nk < 0 || ni >= fd.nx || nj >= fd.ny || nk >= fd.nz) continue;
+    long long gi = bm_index(fd, ni, nj, nk);
+    a.mom[AX][gi] += s_mom[r];
+    a.mass[AX][gi] += s_mass[r];
+    long long blk = gi >> 6;
+    a.act_vel[AX][blk] = 1;
+    a.act_mass[AX][blk] = 1;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
+  __shared__ float s_mom[3][256];
+  __shared__ float s_mass[3][256];
+  __shared__ uint8_t s_flag[3][256];
+  __shared__ uint8_t s_ord[64 * kOrdMax];
+  if (blockIdx.x >= *a.count) return;
+  const uint32_t b = a.list[blockIdx.x];
+  const int lane = threadIdx.x;
+  for (int t = lane; t < 3 * 256; t += 64) {
+    (&s_mom[0][0])[t] = 0.f;
+    (&s_mass[0][0])[t] = 0.f;
+    (&s_flag[0][0])[t] = 0;
+  }
+  int bx, by, bz;
+  block_coords(a.dc, b, bx, by, bz);
+  const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+  const int i = bx * 4 + lx, j = by * 4 + ly, k = bz * 4 + lz;
+  const long long c = (long long)b * kBlockVol + lane;
+  const uint32_t s0 = a.start[c];
+  const uint32_t cnt = a.start[c + 1] - s0;
+  uint8_t* ord = &s_ord[lane * kOrdMax];
+  if (cnt > 1 && cnt <= (uint32_t)kOrdMax) {
+    // rank of each particle by caller index (ids are unique)
+    for (uint32_t t = 0; t < cnt; ++t) {
+      uint32_t id = a.P.id[s0 + t], r = 0;
+      for (uint32_t u = 0; u < cnt; ++u) r += a.P.id[s0 + u] < id;
+      ord[r] = (uint8_t)t;
+    }
+  } else if (cnt == 1) {
+    ord[0] = 0;
+  }
+  __syncthreads();
+  splat_axis<0>(a, i, j, k, s0, cnt, ord, s_mom[0], s_mass[0], s_flag[0], lx, ly, lz);
+  splat_axis<1>(a, i, j, k, s0, cnt, ord, s_mom[1], s_mass[1], s_flag[1], lx, ly, lz);
+  splat_axis<2>(a, i, j, k, s0, cnt, ord, s_mom[2], s_mass[2], s_flag[2], lx, ly, lz);
+  __syncthreads();
+  flush_axis<0>(a, bx, by, bz, s_mom[0], s_mass[0], s_flag[0], lane);
+  flush_axis<1>(a, bx, by, bz, s_mom[1], s_mass[1], s_flag[1], lane);
+  flush_axis<2>(a, bx, by, bz, s_mom[2], s_mass[2], s_flag[2], lane);
+}
+
+}  // namespace pf

@@ -1,0 +1,525 @@
+// pf_project.hip -- variable-coefficient pressure projection (Eq. 8).
+//
+// Reference: spProjectStepVC3D, src/driver/sparse_ops3d_tp.cpp:234-423.  Rows are the
+// FLUID cells; the off-diagonal of a face is beta = 1/rho(phi(raw face mass))
+// (Eq. 7, src/physics/phasefield.h:10-18) when both cells are FLUID and neither is
+// the pinned cell, the diagonal sums beta over non-SOLID faces (AIR = Dirichlet 0),
+// the RHS is -div(u)/dx + c_div with faces next to SOLID cells taken as 0, and the
+// first FLUID cell (lowest linear id) is pinned when no FLUID cell touches AIR.
+// Preconditioner: Jacobi (diagonal), as the reference's (:328-333); it has no
+// ordering, so any sweep order reproduces it.
+//
+// MI355X form: the operator is matrix-free over block-major cells of the sorted
+// list of blocks that hold FLUID cells; each cell stores its diagonal and the
+// coefficient of its +x/+y/+z faces (the -x coefficient is the -x neighbour's +x
+// one), in FP32 for the inner PCG and FP64 for the true residual.  The inner PCG
+// runs FP32 stencils and vector updates, FP64 dot products, FP64 alpha/beta kept in
+// device memory; an FP64 iterative-refinement loop recomputes b - A x in FP64 and
+// restarts the inner solve on it until the stop rule holds on the FP64 residual.
+#include "pf_internal.h"
+
+namespace pf {
+
+struct PhaseK {
+  double rho_l, rho_g, rmin, den;  // rmin = ln(rho_l/rho_g)*rho_g*rho_tilde_0, den = alpha*rho_tilde_0*rho_l
+};
+
+// betaOfRaw, sparse_ops3d_tp.cpp:89-91 with phiFromRawDensity / betaFromPhi
+__device__ inline double beta_of_raw(double raw, const PhaseK& k) {
+#pragma clang fp contract(off)
+  double phi;
+  if (raw < k.rmin) {
+    phi = 0.0;
+  } else {
+    double d = raw - k.rmin;
+    if (d < 0.0) d = 0.0;
+    double v = sqrt(d / k.den);
+    phi = v < 1.0 ? v : 1.0;
+  }
+  return 1.0 / (phi * k.rho_l + (1.0 - phi) * k.rho_g);
+}
+
+struct CellNbr {
+  long long xp, xm, yp, ym, zp, zm;
+};
+
+// block-major neighbour indices of cell c = b*64 + lane (valid for interior cells)
+__device__ inline CellNbr neighbours(const Dims& dc, long long b, int lane) {
+  const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+  const long long c = b * kBlockVol + lane;
+  const long long sy = (long long)dc.nbx * kBlockVol, sz = (long long)dc.nbx * dc.nby * kBlockVol;
+  CellNbr n;
+  n.xp = lx < 3 ? c + 1 : c + kBlockVol - 3;
+  n.xm = lx > 0 ? c - 1 : c - kBlockVol + 3;
+  n.yp = ly < 3 ? c + 4 : c + sy - 12;
+  n.ym = ly > 0 ? c - 4 : c - sy + 12;
+  n.zp = lz < 3 ? c + 16 : c + sz - 48;
+  n.zm = lz > 0 ? c - 16 : c - sz + 48;
+  return n;
+}
+
+__device__ inline uint8_t marker_at(const uint8_t* mk, const Dims& dc, const Geom& g, int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return CELL_SOLID;
+  return mk[bm_index(dc, i, j, k)];
+}
+
+// ---------------------------------------------------------------- reductions
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// sum (or max) of NV values per thread over a 256-thread workgroup -> part[slot*stride + blockIdx]
+template <int NV>
+__device__ inline void block_reduce_store(double (&v)[NV], const bool (&is_max)[NV], double* part,
+                                          long long stride) {
+  __shared__ double s[4][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = is_max[q] ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[wave][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double r = s[0][q];
+      for (int w = 1; w < 4; ++w) r = is_max[q] ? fmax(r, s[w][q]) : r + s[w][q];
+      part[q * stride + blockIdx.x] = r;
+    }
+  }
+}
+
+// one workgroup sums part[slot*stride + 0..n) in a fixed order
+__device__ inline double block_sum_partials(const double* part, int n, bool is_max) {
+  __shared__ double s[256];
+  double acc = is_max ? 0.0 : 0.0;
+  for (int t = threadIdx.x; t < n; t += 256) acc = is_max ? fmax(acc, part[t]) : acc + part[t];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] = is_max ? fmax(s[threadIdx.x], s[threadIdx.x + w]) : s[threadIdx.x] + s[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = s[0];
+  __syncthreads();
+  return r;
+}
+
+// red[q] = reduction of slot q (NV slots)
+__global__ void __launch_bounds__(256) k_reduce_slots(const double* part, int n, int nslots,
+                                                      unsigned max_mask, double* red) {
+  for (int q = 0; q < nslots; ++q) {
+    double r = block_sum_partials(part + (long long)q * kMaxPartials, n, (max_mask >> q) & 1u);
+    if (threadIdx.x == 0) red[q] = r;
+  }
+}
+
+// ------------------------------------------------------------ set-up kernels
+
+// iterate the list in a fixed block->workgroup assignment (deterministic partials)
+#define FOR_LIST_BLOCKS(A, b, lane)                                                   \
+  for (long long _t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); _t < (A).nlist; \
+       _t += (long long)gridDim.x * 4)                                                \
+    for (long long b = (A).list[_t], _once = 1; _once; _once = 0)
+
+
+struct ProjArgs {
+  Geom g;
+  Dims dc, du, dv, dw;
+  const uint8_t* marker;
+  const uint32_t* list;
+  long long nlist;
+  PhaseK ph;
+  double scale;   // dt / dx^2
+  double cdiv;    // divergence correction
+  uint32_t* pin;  // [0] touches-air flag, [1] min FLUID linear id, [4] FLUID cell count
+};
+
+__device__ inline bool list_block(const ProjArgs& a, long long t, long long& b) {
+  if (t >= a.nlist) return false;
+  b = a.list[t];
+  return true;
+}
+
+// Pin detection, sparse_ops3d_tp.cpp:250-264
+__global__ void __launch_bounds__(256) k_pin_detect(ProjArgs a) {
+  long long b;
+  const int lane = threadIdx.x & 63;
+  long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (!list_block(a, t, b)) return;
+  int bx, by, bz;
+  block_coords(a.dc, b, bx, by, bz);
+  int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
+  bool fl = a.marker[b * kBlockVol + lane] == CELL_FLUID;
+  bool air = false;
+  uint32_t lin = 0xffffffffu;
+  if (fl) {
+    air = marker_at(a.marker, a.dc, a.g, i + 1, j, k) == CELL_AIR ||
+          marker_at(a.marker, a.dc, a.g, i - 1, j, k) == CELL_AIR ||
+          marker_at(a.marker, a.dc, a.g, i, j + 1, k) == CELL_AIR ||
+          marker_at(a.marker, a.dc, a.g, i, j - 1, k) == CELL_AIR ||
+          marker_at(a.marker, a.dc, a.g, i, j, k + 1) == CELL_AIR ||
+          marker_at(a.marker, a.dc, a.g, i, j, k - 1) == CELL_AIR;
+    lin = (uint32_t)(i + (long long)a.g.nx * (j + (long long)a.g.ny * k));
+  }
+  bool anyair = __any(air);
+  int nfl = __popcll(__ballot(fl));
+  uint32_t m = lin;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if (lane == 0) {
+    if (anyair) atomicOr(&a.pin[0], 1u);
+    if (m != 0xffffffffu) atomicMin(&a.pin[1], m);
+    if (nfl) atomicAdd(&a.pin[4], (uint32_t)nfl);
+  }
+}
+
+struct AsmOut {
+  double *b64, *cd64, *cx64, *cy64, *cz64;
+  float *cd32, *cx32, *cy32, *cz32, *invd32;
+  const float *u, *v, *w, *mu, *mv, *mw;
+  double* part;   // slot 0: sum b^2
+};
+
+// RHS and assembly, sparse_ops3d_tp.cpp:266-313
+__global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  double bb = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    const long long c = b * kBlockVol + lane;
+    int bx, by, bz;
+    block_coords(a.dc, b, bx, by, bz);
+    int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
+    bool fl = a.marker[c] == CELL_FLUID;
+    double rhs = 0.0, cd = 0.0, cx = 0.0, cy = 0.0, cz = 0.0, inv = 0.0;
+    if (fl) {
+      const bool touches = a.pin[0] != 0;
+      const uint32_t pc = touches ? 0xffffffffu : a.pin[1];
+      const uint32_t lin = (uint32_t)(i + (long long)a.g.nx * (j + (long long)a.g.ny * k));
+      const bool pinned = lin == pc;
+      const int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+      uint8_t m[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) m[q] = marker_at(a.marker, a.dc, a.g, nb[q][0], nb[q][1], nb[q][2]);
+      const double fb[6] = {beta_of_raw(o.mu[bm_index(a.du, i + 1, j, k)], a.ph),
+                            beta_of_raw(o.mu[bm_index(a.du, i, j, k)], a.ph),
+                            beta_of_raw(o.mv[bm_index(a.dv, i, j + 1, k)], a.ph),
+                            beta_of_raw(o.mv[bm_index(a.dv, i, j, k)], a.ph),
+                            beta_of_raw(o.mw[bm_index(a.dw, i, j, k + 1)], a.ph),
+                            beta_of_raw(o.mw[bm_index(a.dw, i, j, k)], a.ph)};
+      if (!pinned) {
+        double uR = m[0] == CELL_SOLID ? 0.0 : (double)o.u[bm_index(a.du, i + 1, j, k)];
+        double uL = m[1] == CELL_SOLID ? 0.0 : (double)o.u[bm_index(a.du, i, j, k)];
+        double vT = m[2] == CELL_SOLID ? 0.0 : (double)o.v[bm_index(a.dv, i, j + 1, k)];
+        double vB = m[3] == CELL_SOLID ? 0.0 : (double)o.v[bm_index(a.dv, i, j, k)];
+        double wU = m[4] == CELL_SOLID ? 0.0 : (double)o.w[bm_index(a.dw, i, j, k + 1)];
+        double wD = m[5] == CELL_SOLID ? 0.0 : (double)o.w[bm_index(a.dw, i, j, k)];
+        rhs = -((uR - uL) + (vT - vB) + (wU - wD)) / a.g.dx + a.cdiv;
+      }
+      double diag = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        if (m[q] != CELL_SOLID) diag += fb[q];
+      // +x/+y/+z couplings: both cells FLUID and neither pinned
+      auto lin_of = [&](int q) {
+        return (uint32_t)(nb[q][0] + (long long)a.g.nx * (nb[q][1] + (long long)a.g.ny * nb[q][2]));
+      };
+      if (!pinned) {
+        if (m[0] == CELL_FLUID && lin_of(0) != pc) cx = a.scale * fb[0];
+        if (m[2] == CELL_FLUID && lin_of(2) != pc) cy = a.scale * fb[2];
+        if (m[4] == CELL_FLUID && lin_of(4) != pc) cz = a.scale * fb[4];
+      }
+      cd = pinned ? 1.0 : a.scale * diag;
+      inv = pinned ? 1.0 : (cd > 0.0 ? 1.0 / cd : 0.0);
+    }
+    o.b64[c] = rhs;
+    o.cd64[c] = cd; o.cx64[c] = cx; o.cy64[c] = cy; o.cz64[c] = cz;
+    o.cd32[c] = (float)cd; o.cx32[c] = (float)cx; o.cy32[c] = (float)cy; o.cz32[c] = (float)cz;
+    o.invd32[c] = (float)inv;
+    bb += rhs * rhs;
+  }
+  double v[1] = {bb};
+  const bool mx[1] = {false};
+  block_reduce_store<1>(v, mx, o.part, kMaxPartials);
+}
+
+// zero every per-cell array of the listed blocks (blocks that leave the fluid set
+// must read as zero; see the reference's pf.clear(), sparse_ops3d_tp.cpp:240)
+struct ZeroArgs {
+  const uint32_t* list;
+  long long nlist;
+  float* f32[12];
+  int nf32;
+  double* f64[7];
+  int nf64;
+};
+__global__ void __launch_bounds__(256) k_zero_blocks(ZeroArgs z) {
+  long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= z.nlist) return;
+  long long c = (long long)z.list[t] * kBlockVol + (threadIdx.x & 63);
+  for (int q = 0; q < z.nf32; ++q) z.f32[q][c] = 0.f;
+  for (int q = 0; q < z.nf64; ++q) z.f64[q][c] = 0.0;
+}
+
+// ------------------------------------------------------------- solve kernels
+
+struct VecArgs {
+  Dims dc;
+  const uint32_t* list;
+  long long nlist;
+  SolveScalars* sc;
+  double* part;
+};
+
+// FP64 residual r = b - A x (true residual), partial sum r^2, max |r|
+__global__ void __launch_bounds__(256) k_residual64(VecArgs a, const double* __restrict__ b64,
+                                                    const double* __restrict__ x, double* __restrict__ r,
+                                                    const double* __restrict__ cd, const double* __restrict__ cx,
+                                                    const double* __restrict__ cy, const double* __restrict__ cz) {
+  const int lane = threadIdx.x & 63;
+  double rr = 0.0, rm = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    double res = 0.0;
+    if (cd[c] != 0.0) {
+      CellNbr n = neighbours(a.dc, b, lane);
+      double ax = cd[c] * x[c] - (cx[c] * x[n.xp] + cx[n.xm] * x[n.xm] + cy[c] * x[n.yp] + cy[n.ym] * x[n.ym] +
+                                  cz[c] * x[n.zp] + cz[n.zm] * x[n.zm]);
+      res = b64[c] - ax;
+    }
+    r[c] = res;
+    rr += res * res;
+    rm = fmax(rm, fabs(res));
+  }
+  double v[2] = {rr, rm};
+  const bool mx[2] = {false, true};
+  block_reduce_store<2>(v, mx, a.part, kMaxPartials);
+}
+
+// inner-solve start: r32 = (float)(r64*s), x32 = 0, z = D^-1 r, p = 0; partial r.z, r.r
+__global__ void __launch_bounds__(256) k_pcg_init(VecArgs a, const double* __restrict__ r64, double s,
+                                                  float* __restrict__ x32, float* __restrict__ r32,
+                                                  float* __restrict__ z32, float* __restrict__ p,
+                                                  const float* __restrict__ invd) {
+  const int lane = threadIdx.x & 63;
+  double rz = 0.0, rr = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    float r = (float)(r64[c] * s);
+    float z = invd[c] * r;
+    x32[c] = 0.f; r32[c] = r; z32[c] = z; p[c] = 0.f;
+    rz += (double)r * (double)z;
+    rr += (double)r * (double)r;
+  }
+  double v[2] = {rz, rr};
+  const bool mx[2] = {false, false};
+  block_reduce_store<2>(v, mx, a.part, kMaxPartials);
+}
+
+__global__ void __launch_bounds__(256) k_pcg_init_fin(const double* part, int n, SolveScalars* sc,
+                                                      double tol2, int max_iter, int stall_window) {
+  double rz = block_sum_partials(part, n, false);
+  double rr = block_sum_partials(part + kMaxPartials, n, false);
+  if (threadIdx.x == 0) {
+    sc->rz = rz; sc->rr = rr; sc->rr0 = rr; sc->rr_best = rr; sc->tol2 = tol2;
+    sc->beta = 0.0; sc->alpha = 0.0; sc->pAp = 0.0;
+    sc->iter = 0; sc->iter_best = 0; sc->max_iter = max_iter; sc->stall_window = stall_window;
+    sc->breakdown = 0; sc->stalled = 0;
+    sc->done = (rr == 0.0 || max_iter <= 0) ? 1 : 0;
+  }
+}
+
+// p_new = z + beta p_old (own cell and the six neighbours, recomputed on the fly),
+// Ap = A p_new (FP32), partial p.Ap in FP64
+__global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __restrict__ z,
+                                                  const float* __restrict__ pold, float* __restrict__ pnew,
+                                                  float* __restrict__ Ap, const float* __restrict__ cd,
+                                                  const float* __restrict__ cx, const float* __restrict__ cy,
+                                                  const float* __restrict__ cz) {
+  if (a.sc->done) return;
+  const int lane = threadIdx.x & 63;
+  const float beta = (float)a.sc->beta;
+  double pap = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    float d = cd[c];
+    float pc = 0.f, ap = 0.f;
+    if (d != 0.f) {
+      CellNbr n = neighbours(a.dc, b, lane);
+      pc = fmaf(beta, pold[c], z[c]);
+      float pxp = fmaf(beta, pold[n.xp], z[n.xp]), pxm = fmaf(beta, pold[n.xm], z[n.xm]);
+      float pyp = fmaf(beta, pold[n.yp], z[n.yp]), pym = fmaf(beta, pold[n.ym], z[n.ym]);
+      float pzp = fmaf(beta, pold[n.zp], z[n.zp]), pzm = fmaf(beta, pold[n.zm], z[n.zm]);
+      float off = cx[c] * pxp + cx[n.xm] * pxm + cy[c] * pyp + cy[n.ym] * pym + cz[c] * pzp + cz[n.zm] * pzm;
+      ap = d * pc - off;
+    }
+    pnew[c] = pc;
+    Ap[c] = ap;
+    pap += (double)pc * (double)ap;
+  }
+  double v[1] = {pap};
+  const bool mx[1] = {false};
+  block_reduce_store<1>(v, mx, a.part, kMaxPartials);
+}
+
+__global__ void __launch_bounds__(256) k_pcg_alpha(const double* part, int n, SolveScalars* sc) {
+  if (sc->done) return;
+  double pap = block_sum_partials(part, n, false);
+  if (threadIdx.x == 0) {
+    sc->pAp = pap;
+    if (!(pap > 0.0) || !isfinite(pap)) {
+      sc->breakdown = 1;
+      sc->done = 1;
+      sc->alpha = 0.0;
+    } else {
+      sc->alpha = sc->rz / pap;
+    }
+  }
+}
+
+// x += alpha p, r -= alpha Ap, z = D^-1 r; partial r.z and r.r
+__global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict__ x, float* __restrict__ r,
+                                                    float* __restrict__ z, const float* __restrict__ p,
+                                                    const float* __restrict__ Ap, const float* __restrict__ invd) {
+  if (a.sc->done) return;
+  const int lane = threadIdx.x & 63;
+  const float alpha = (float)a.sc->alpha;
+  double rz = 0.0, rr = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    float xn = fmaf(alpha, p[c], x[c]);
+    float rn = fmaf(-alpha, Ap[c], r[c]);
+    float zn = invd[c] * rn;
+    x[c] = xn; r[c] = rn; z[c] = zn;
+    rz += (double)rn * (double)zn;
+    rr += (double)rn * (double)rn;
+  }
+  double v[2] = {rz, rr};
+  const bool mx[2] = {false, false};
+  block_reduce_store<2>(v, mx, a.part, kMaxPartials);
+}
+
+__global__ void __launch_bounds__(256) k_pcg_beta(const double* part, int n, SolveScalars* sc) {
+  if (sc->done) return;
+  double rz = block_sum_partials(part, n, false);
+  double rr = block_sum_partials(part + kMaxPartials, n, false);
+  if (threadIdx.x == 0) {
+    int it = sc->iter + 1;
+    sc->iter = it;
+    sc->rr = rr;
+    if (!isfinite(rr) || !isfinite(rz)) {
+      sc->breakdown = 1;
+      sc->done = 1;
+      return;
+    }
+    if (rr < sc->rr_best) { sc->rr_best = rr; sc->iter_best = it; }
+    if (rr <= sc->tol2 * sc->rr0 || it >= sc->max_iter) { sc->done = 1; return; }
+    if (it - sc->iter_best >= sc->stall_window) { sc->stalled = 1; sc->done = 1; return; }
+    sc->beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
+    sc->rz = rz;
+  }
+}
+
+// x64 += x32 / s  (refinement update)
+__global__ void __launch_bounds__(256) k_ir_accum(VecArgs a, double* __restrict__ x64,
+                                                  const float* __restrict__ x32, double inv_s) {
+  const int lane = threadIdx.x & 63;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    x64[c] += (double)x32[c] * inv_s;
+  }
+}
+
+// x64 start: 0, or the stored pressure (warm start) on FLUID cells
+__global__ void __launch_bounds__(256) k_ir_start(VecArgs a, double* __restrict__ x64, const float* __restrict__ p,
+                                                  const double* __restrict__ cd, int warm) {
+  const int lane = threadIdx.x & 63;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    x64[c] = (warm && cd[c] != 0.0) ? (double)p[c] : 0.0;
+  }
+}
+
+// ------------------------------------------------------- pressure and faces
+
+// p = (float)x on FLUID cells, 0 elsewhere (sparse_ops3d_tp.cpp:366-372)
+__global__ void __launch_bounds__(256) k_store_pressure(VecArgs a, const uint8_t* __restrict__ marker,
+                                                        const double* __restrict__ x64, float* __restrict__ p,
+                                                        uint8_t* __restrict__ act_p) {
+  const int lane = threadIdx.x & 63;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    p[c] = marker[c] == CELL_FLUID ? (float)x64[c] : 0.f;
+    if (lane == 0) act_p[b] = 1;
+  }
+}
+
+struct VelArgs {
+  Geom g;
+  Dims dc, df[3];
+  const uint8_t* marker;
+  const float* p;
+  float* vel[3];
+  const float* mass[3];
+  uint8_t* act_vel[3];
+  PhaseK ph;
+  double s;  // dt / dx
+  const uint32_t* list;
+  long long nlist;
+};
+
+__device__ inline void update_face(const VelArgs& a, int ax, int fi, int fj, int fk, int li, int lj, int lk, int ri,
+                                   int rj, int rk) {
+  const Dims& fd = a.df[ax];
+  uint8_t ml = marker_at(a.marker, a.dc, a.g, li, lj, lk), mr = marker_at(a.marker, a.dc, a.g, ri, rj, rk);
+  bool lf = ml == CELL_FLUID, rf = mr == CELL_FLUID;
+  if (!lf && !rf) return;
+  long long gi = bm_index(fd, fi, fj, fk);
+  a.act_vel[ax][gi >> 6] = 1;
+  if (ml == CELL_SOLID || mr == CELL_SOLID) {
+    a.vel[ax][gi] = 0.f;
+    return;
+  }
+  double pl = lf ? (double)a.p[bm_index(a.dc, li, lj, lk)] : 0.0;
+  double pr = rf ? (double)a.p[bm_index(a.dc, ri, rj, rk)] : 0.0;
+  double dlt = a.s * beta_of_raw((double)a.mass[ax][gi], a.ph) * (pr - pl);
+  a.vel[ax][gi] = a.vel[ax][gi] - (float)dlt;
+}
+
+// u -= (dt/dx) beta_f (p_r - p_l) on faces of FLUID cells; faces touching SOLID := 0
+// (sparse_ops3d_tp.cpp:374-422).  Each face next to a FLUID cell is visited once:
+// as the lower face of a FLUID cell, or as the upper face when the upper
+// neighbour is not FLUID.
+__global__ void __launch_bounds__(256) k_velocity_update(VelArgs a) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= a.nlist) return;
+  long long b = a.list[t];
+  if (a.marker[b * kBlockVol + lane] != CELL_FLUID) return;
+  int bx, by, bz;
+  block_coords(a.dc, b, bx, by, bz);
+  int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
+  // x faces
+  if (i > 0) update_face(a, 0, i, j, k, i - 1, j, k, i, j, k);
+  if (i + 1 < a.g.nx && marker_at(a.marker, a.dc, a.g, i + 1, j, k) != CELL_FLUID)
+    update_face(a, 0, i + 1, j, k, i, j, k, i + 1, j, k);
+  if (j > 0) update_face(a, 1, i, j, k, i, j - 1, k, i, j, k);
+  if (j + 1 < a.g.ny && marker_at(a.marker, a.dc, a.g, i, j + 1, k) != CELL_FLUID)
+    update_face(a, 1, i, j + 1, k, i, j, k, i, j + 1, k);
+  if (k > 0) update_face(a, 2, i, j, k, i, j, k - 1, i, j, k);
+  if (k + 1 < a.g.nz && marker_at(a.marker, a.dc, a.g, i, j, k + 1) != CELL_FLUID)
+    update_face(a, 2, i, j, k + 1, i, j, k, i, j, k + 1);
+}
+
+}  // namespace pf

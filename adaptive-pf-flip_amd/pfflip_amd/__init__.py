@@ -1,0 +1,333 @@
+"""Python binding of the MI355X sparse two-phase PF-FLIP step (include/pf_gpu.h).
+
+Thin ctypes layer over ``libpfflip_hip.so`` (built in-tree by
+``adaptive-pf-flip_amd/build.py``).  There is no CPU fallback: if the HIP library
+is missing or fails to load, importing ``lib()`` raises.
+
+Naming follows the reference (lsfs): ``SparseSim3DTP`` keeps the fields of
+src/driver/sparse_sim3d_tp.h:8-177 that ``step()`` reads, and the field ids are the
+members of SparseMacGrid3D<4> (src/grid/sparse_mac_grid3d.h:9-11).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfflip_hip.so")
+
+FIELD_U, FIELD_V, FIELD_W, FIELD_MU, FIELD_MV, FIELD_MW, FIELD_P, FIELD_MARKER = range(8)
+FIELD_SAVED_U, FIELD_SAVED_V, FIELD_SAVED_W = 8, 9, 10
+STOP_REF_MAXABS, STOP_REL2 = 0, 1
+SCENE_RT, SCENE_BUBBLE, SCENE_DAMBREAK = 0, 1, 2
+STAGES = ("bin", "mark", "p2g", "grid", "assemble", "solve", "velocity", "g2p_advect")
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("reserved", C.c_int32),
+                ("dx", C.c_double), ("ox", C.c_double), ("oy", C.c_double), ("oz", C.c_double)]
+
+
+class Phase(C.Structure):
+    _fields_ = [("rho_l", C.c_double), ("rho_g", C.c_double), ("alpha_phi", C.c_double),
+                ("rho_tilde_0", C.c_double)]
+
+
+class SolveCfg(C.Structure):
+    _fields_ = [("max_iters", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
+                ("inner_rel_tol", C.c_double), ("max_outer", C.c_int32), ("warm_start", C.c_int32),
+                ("check_every", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32),
+                ("converged", C.c_int32), ("breakdown", C.c_int32), ("pinned", C.c_int32),
+                ("fallback_used", C.c_int32), ("reserved0", C.c_int32), ("reserved1", C.c_int32),
+                ("fluid_cells", C.c_int64), ("pinned_cell", C.c_int64),
+                ("initial_max_residual", C.c_double), ("final_max_residual", C.c_double),
+                ("true_rel_residual", C.c_double), ("true_max_residual", C.c_double),
+                ("rhs_norm2", C.c_double), ("ms", C.c_double)]
+
+
+class EscapeStats(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("clamped_liquid", "clamped_gas", "x_lo", "x_hi", "y_lo",
+                                          "y_hi", "z_lo", "z_hi")]
+
+
+class StepCfg(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("Vp", C.c_double),
+                ("alpha_liquid", C.c_double), ("alpha_gas", C.c_double),
+                ("adaptive_cfl", C.c_double), ("adaptive_min_dt", C.c_double),
+                ("c_div", C.c_double), ("adaptive_timestep", C.c_int32),
+                ("advection_order", C.c_int32), ("timing", C.c_int32), ("reserved", C.c_int32),
+                ("phase", Phase), ("solve", SolveCfg)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("effective_dt", C.c_double), ("max_particle_speed", C.c_double),
+                ("cfl_limit_dt", C.c_double), ("limited", C.c_int32), ("reserved", C.c_int32),
+                ("solve", SolveStats), ("escape", EscapeStats), ("stage_ms", C.c_double * 8),
+                ("total_ms", C.c_double)]
+
+
+class PfError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libpfflip_hip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PfError(f"{LIB_PATH} is missing; run adaptive-pf-flip_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        sig = {
+            "pf_create": (C.c_int, [C.POINTER(vp), C.POINTER(GridDesc)]),
+            "pf_destroy": (C.c_int, [vp]),
+            "pf_last_error": (C.c_char_p, [vp]),
+            "pf_abi_version": (C.c_int, []),
+            "pf_synchronize": (C.c_int, [vp]),
+            "pf_particles_upload": (C.c_int, [vp, i64] + [vp] * 8),
+            "pf_particles_download": (C.c_int, [vp] + [vp] * 6),
+            "pf_particle_count": (i64, [vp]),
+            "pf_scene_generate": (C.c_int, [vp, i32, C.c_uint64]),
+            "pf_mark_cells": (C.c_int, [vp]),
+            "pf_p2g": (C.c_int, [vp, C.POINTER(Phase), dbl]),
+            "pf_save_velocity": (C.c_int, [vp]),
+            "pf_apply_gravity": (C.c_int, [vp, dbl, dbl]),
+            "pf_apply_walls": (C.c_int, [vp]),
+            "pf_project": (C.c_int, [vp, C.POINTER(Phase), dbl, C.POINTER(SolveCfg), dbl,
+                                     C.POINTER(SolveStats)]),
+            "pf_g2p": (C.c_int, [vp, dbl, dbl]),
+            "pf_advect": (C.c_int, [vp, dbl, i32, C.POINTER(EscapeStats)]),
+            "pf_timestep": (C.c_int, [vp, dbl, i32, dbl, dbl, C.POINTER(dbl), C.POINTER(dbl),
+                                      C.POINTER(dbl), C.POINTER(i32)]),
+            "pf_step": (C.c_int, [vp, C.POINTER(StepCfg), C.POINTER(StepStats)]),
+            "pf_field_dims": (C.c_int, [vp, i32, vp]),
+            "pf_field_download": (C.c_int, [vp, i32, vp]),
+            "pf_field_upload": (C.c_int, [vp, i32, vp, vp]),
+            "pf_active_blocks": (C.c_int, [vp, i32, vp, i64, C.POINTER(i64)]),
+            "pf_calibrate_rho_tilde0": (dbl, [C.POINTER(Phase), dbl]),
+            "pf_phi_from_raw": (dbl, [dbl, C.POINTER(Phase)]),
+            "pf_beta_from_phi": (dbl, [dbl, C.POINTER(Phase)]),
+            "pf_stream": (vp, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def calibrate_rho_tilde0(phase: Phase, Vp: float = 1.0) -> float:
+    return lib().pf_calibrate_rho_tilde0(C.byref(phase), Vp)
+
+
+def default_solve_cfg(stop_rule=STOP_REF_MAXABS, tol=1e-7, max_iters=600) -> SolveCfg:
+    return SolveCfg(max_iters=max_iters, stop_rule=stop_rule, tol=tol, inner_rel_tol=1e-4,
+                    max_outer=50, warm_start=0, check_every=8)
+
+
+class Context:
+    """One grid + one particle set on the current HIP device (a pf_ctx)."""
+
+    def __init__(self, nx, ny, nz, dx=1.0, origin=(0.0, 0.0, 0.0)):
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.h = C.c_void_p()
+        d = GridDesc(nx, ny, nz, 0, dx, *origin)
+        st = lib().pf_create(C.byref(self.h), C.byref(d))
+        if st != 0:
+            raise PfError(f"pf_create failed with status {st}")
+
+    def close(self):
+        if self.h:
+            lib().pf_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != 0:
+            msg = lib().pf_last_error(self.h)
+            raise PfError(f"{what}: status {st}: {msg.decode() if msg else ''}")
+
+    # particles
+    def upload(self, x, y, z, u=None, v=None, w=None, type=None, volume=None):
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        x, y, z = f(x), f(y), f(z)
+        n = len(x)
+        u = f(u if u is not None else np.zeros(n)); v = f(v if v is not None else np.zeros(n))
+        w = f(w if w is not None else np.zeros(n))
+        t = np.ascontiguousarray(type if type is not None else np.zeros(n), np.uint8)
+        vol = None if volume is None else f(volume)
+        self._keep = (x, y, z, u, v, w, t, vol)
+        self._check(lib().pf_particles_upload(self.h, n, _p(x), _p(y), _p(z), _p(u), _p(v), _p(w),
+                                              _p(t), _p(vol)), "pf_particles_upload")
+
+    def download(self):
+        n = self.count()
+        a = [np.zeros(n, np.float32) for _ in range(6)]
+        self._check(lib().pf_particles_download(self.h, *[_p(q) for q in a]), "pf_particles_download")
+        return a
+
+    def count(self):
+        return lib().pf_particle_count(self.h)
+
+    def scene(self, kind, seed):
+        self._check(lib().pf_scene_generate(self.h, kind, seed), "pf_scene_generate")
+
+    # stages
+    def mark_cells(self):
+        self._check(lib().pf_mark_cells(self.h), "pf_mark_cells")
+
+    def p2g(self, phase, Vp=1.0):
+        self._check(lib().pf_p2g(self.h, C.byref(phase), Vp), "pf_p2g")
+
+    def save_velocity(self):
+        self._check(lib().pf_save_velocity(self.h), "pf_save_velocity")
+
+    def apply_gravity(self, dt, g):
+        self._check(lib().pf_apply_gravity(self.h, dt, g), "pf_apply_gravity")
+
+    def apply_walls(self):
+        self._check(lib().pf_apply_walls(self.h), "pf_apply_walls")
+
+    def project(self, phase, dt, cfg=None, div_correction=0.0):
+        cfg = cfg or default_solve_cfg()
+        st = SolveStats()
+        self._check(lib().pf_project(self.h, C.byref(phase), dt, C.byref(cfg), div_correction,
+                                     C.byref(st)), "pf_project")
+        return st
+
+    def g2p(self, aL, aG):
+        self._check(lib().pf_g2p(self.h, aL, aG), "pf_g2p")
+
+    def advect(self, dt, order=2):
+        es = EscapeStats()
+        self._check(lib().pf_advect(self.h, dt, order, C.byref(es)), "pf_advect")
+        return es
+
+    def timestep(self, dt, enabled, cfl, min_dt):
+        e, v, c = C.c_double(), C.c_double(), C.c_double()
+        lim = C.c_int32()
+        self._check(lib().pf_timestep(self.h, dt, int(enabled), cfl, min_dt, C.byref(e), C.byref(v),
+                                      C.byref(c), C.byref(lim)), "pf_timestep")
+        return e.value, v.value, c.value, lim.value
+
+    def step(self, cfg: StepCfg):
+        st = StepStats()
+        self._check(lib().pf_step(self.h, C.byref(cfg), C.byref(st)), "pf_step")
+        return st
+
+    def synchronize(self):
+        self._check(lib().pf_synchronize(self.h), "pf_synchronize")
+
+    # fields
+    def dims(self, field):
+        d = np.zeros(3, np.int32)
+        self._check(lib().pf_field_dims(self.h, field, _p(d)), "pf_field_dims")
+        return tuple(int(q) for q in d)
+
+    def field(self, field):
+        nx, ny, nz = self.dims(field)
+        out = np.zeros(nx * ny * nz, np.float32)
+        self._check(lib().pf_field_download(self.h, field, _p(out)), "pf_field_download")
+        return out.reshape(nz, ny, nx)
+
+    def set_field(self, field, dense, block_active=None):
+        d = np.ascontiguousarray(dense, np.float32).reshape(-1)
+        ba = None if block_active is None else np.ascontiguousarray(block_active, np.uint8)
+        self._check(lib().pf_field_upload(self.h, field, _p(d), _p(ba)), "pf_field_upload")
+
+    def active_blocks(self, field):
+        n = C.c_int64()
+        self._check(lib().pf_active_blocks(self.h, field, None, 0, C.byref(n)), "pf_active_blocks")
+        out = np.zeros(max(n.value, 1), np.int32)
+        self._check(lib().pf_active_blocks(self.h, field, _p(out), n.value, C.byref(n)),
+                    "pf_active_blocks")
+        return out[: n.value]
+
+
+class SparseSim3DTP:
+    """GPU mirror of the reference's SparseSim3DTP (src/driver/sparse_sim3d_tp.h:8-177).
+
+    Keeps the fields step() reads under the reference's names; particles live on the
+    device between steps.  Stats of the last step land in the *_last fields.
+    """
+
+    def __init__(self, nx, ny, nz, dx):
+        self.ctx = Context(nx, ny, nz, dx)
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.phase = Phase(1.0, 0.01, 1.0, 1.0)
+        self.dt, self.gravity, self.Vp = 0.02, -9.81, 1.0
+        self.alpha_liquid, self.alpha_gas = 0.95, 0.95
+        self.adaptive_timestep, self.adaptive_cfl, self.adaptive_min_dt = False, 0.5, 1e-5
+        self.advection_order = 2
+        self.c_div_volume_correction, self.c_div_strength = False, 1.0
+        self.liquid_volume_target = 0.0
+        self.cg_iters, self.cg_tol = 600, 1e-7
+        self.stop_rule = STOP_REF_MAXABS
+        self.inner_rel_tol, self.warm_start, self.check_every = 1e-4, 0, 8
+        self.timing = False
+        self.effective_dt_last = self.dt
+        self.max_particle_speed_last = 0.0
+        self.c_div_last = 0.0
+        self.liquid_volume_current_last = 0.0
+        self.stats_last = None
+
+    def set_particles(self, x, y, z, u=None, v=None, w=None, type=None, volume=None):
+        self.ctx.upload(x, y, z, u, v, w, type, volume)
+        t = np.asarray(type if type is not None else np.zeros(len(x)), np.uint8)
+        vol = np.ones(len(x)) if volume is None else np.asarray(volume, np.float64)
+        self._liquid_volume = float(vol[t == 0].sum()) * self.Vp
+
+    def calibrate(self):
+        """resetSceneState's rho_tilde_0 = calibrateRhoTilde0(phase, Vp), sparse_sim3d_tp.cpp:396"""
+        self.phase.rho_tilde_0 = calibrate_rho_tilde0(self.phase, self.Vp)
+
+    def step_cfg(self) -> StepCfg:
+        solve = SolveCfg(max_iters=self.cg_iters, stop_rule=self.stop_rule, tol=self.cg_tol,
+                         inner_rel_tol=self.inner_rel_tol, max_outer=50,
+                         warm_start=self.warm_start, check_every=self.check_every)
+        return StepCfg(dt=self.dt, gravity=self.gravity, Vp=self.Vp,
+                       alpha_liquid=self.alpha_liquid, alpha_gas=self.alpha_gas,
+                       adaptive_cfl=self.adaptive_cfl, adaptive_min_dt=self.adaptive_min_dt,
+                       c_div=0.0, adaptive_timestep=int(self.adaptive_timestep),
+                       advection_order=self.advection_order, timing=int(self.timing),
+                       phase=self.phase, solve=solve)
+
+    def step(self):
+        """SparseSim3DTP::step, src/driver/sparse_sim3d_tp.cpp:854-884."""
+        cfg = self.step_cfg()
+        # updateVolumeCorrectionStats, sparse_sim3d_tp.cpp:368-379 (needs the step dt)
+        if self.adaptive_timestep:
+            eff = self.ctx.timestep(self.dt, True, self.adaptive_cfl, self.adaptive_min_dt)[0]
+        else:
+            eff = self.dt
+        self.liquid_volume_current_last = self._liquid_volume
+        err = self.liquid_volume_target - self.liquid_volume_current_last
+        self.c_div_last = 0.0
+        if (self.c_div_volume_correction and self.c_div_strength > 0.0 and
+                self.liquid_volume_target > 0.0 and eff > 0.0):
+            self.c_div_last = self.c_div_strength * err / (eff * self.liquid_volume_target)
+        cfg.c_div = self.c_div_last
+        st = self.ctx.step(cfg)
+        self.effective_dt_last = st.effective_dt
+        self.max_particle_speed_last = st.max_particle_speed
+        self.stats_last = st
+        return st

@@ -1,0 +1,190 @@
+/*
+ * pf_gpu.h -- C ABI of the MI355X (gfx950) sparse two-phase Phase-Field-FLIP step.
+ *
+ * The reference (lsfs, JangHyun-bin/adaptive-pf-flip) has no FFI: its seam is four
+ * free C++ functions called by SparseSim3DTP::step().  Each entry point below names
+ * the reference interface it replaces (paths relative to the reference root).  The
+ * C++ host class in adaptive-pf-flip_amd/host/sparse_sim3d_tp_gpu.h keeps the
+ * reference's SparseSim3DTP field names and step() on top of this ABI.
+ *
+ * Conventions: plain pointers and sizes, int status (PF_OK == 0), no exceptions
+ * cross the boundary.  One context = one grid + one particle set + one HIP stream on
+ * the current device; calls on a context are stream-ordered; downloads synchronise.
+ * A context is not re-entrant (as a SparseMacGrid3D is not).
+ */
+#ifndef PF_GPU_H
+#define PF_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_ABI_VERSION 1
+
+enum pf_status {
+  PF_OK = 0,
+  PF_ERR_ARG = 1,      /* invalid argument (null pointer, bad size, particle outside the grid) */
+  PF_ERR_HIP = 2,      /* a HIP runtime call failed */
+  PF_ERR_STATE = 3,    /* call out of order (e.g. no particles uploaded) */
+  PF_ERR_NOMEM = 4     /* device allocation failed */
+};
+
+/* Field ids: the eight SparseBlockGrid3D<4> members of SparseMacGrid3D<4>
+ * (src/grid/sparse_mac_grid3d.h:9-11), plus the step's pre-gravity copy of u,v,w
+ * (the `saved` grid of sparse_sim3d_tp.cpp:866). */
+enum pf_field {
+  PF_FIELD_U = 0, PF_FIELD_V = 1, PF_FIELD_W = 2,
+  PF_FIELD_MU = 3, PF_FIELD_MV = 4, PF_FIELD_MW = 5,
+  PF_FIELD_P = 6, PF_FIELD_MARKER = 7,
+  PF_FIELD_SAVED_U = 8, PF_FIELD_SAVED_V = 9, PF_FIELD_SAVED_W = 10
+};
+
+enum pf_stop_rule {
+  PF_STOP_REF_MAXABS = 0, /* |b - A x|_inf < tol, the reference rule (sparse_ops3d_tp.cpp:340-357) */
+  PF_STOP_REL2 = 1        /* ||b - A x||_2 / ||b||_2 <= tol, both measured in FP64 */
+};
+
+/* SparseMacGrid3D<4>(nx, ny, nz, dx) with origin (src/grid/sparse_mac_grid3d.h:5-17) */
+typedef struct pf_grid_desc {
+  int32_t nx, ny, nz, reserved;
+  double dx, ox, oy, oz;
+} pf_grid_desc;
+
+/* PhaseParams (src/physics/phasefield.h:4-9) */
+typedef struct pf_phase {
+  double rho_l, rho_g, alpha_phi, rho_tilde_0;
+} pf_phase;
+
+/* Pressure-solve configuration.  The reference passes cg_iters and cg_tol
+ * (sparse_ops3d_tp.h:9-10) and solves in FP64 with Jacobi PCG.  Here the inner
+ * Jacobi PCG runs FP32 stencils with FP64 dot products and scalars, inside an FP64
+ * iterative-refinement loop whose true residual decides the stop. */
+typedef struct pf_solve_cfg {
+  int32_t max_iters;        /* cap on total inner PCG iterations (reference cg_iters) */
+  int32_t stop_rule;        /* pf_stop_rule */
+  double tol;               /* reference cg_tol (absolute) or relative tolerance */
+  double inner_rel_tol;     /* inner FP32 PCG reduction per refinement pass (e.g. 1e-3) */
+  int32_t max_outer;        /* cap on FP64 refinement passes */
+  int32_t warm_start;       /* 1: start from the stored pressure instead of 0 */
+  int32_t check_every;      /* inner iterations between host convergence checks */
+  int32_t reserved;
+} pf_solve_cfg;
+
+/* Mirrors MRPressureSolveStats3D's core fields (src/pressure/multires_pressure3d.h:60-133) */
+typedef struct pf_solve_stats {
+  int32_t iterations, outer_iterations, converged, breakdown;
+  int32_t pinned, fallback_used, reserved0, reserved1;
+  int64_t fluid_cells, pinned_cell;
+  double initial_max_residual, final_max_residual;
+  double true_rel_residual, true_max_residual;
+  double rhs_norm2;
+  double ms;
+} pf_solve_stats;
+
+/* ParticleEscapeStats3D (src/driver/particle_escape3d.h:37-80) */
+typedef struct pf_escape_stats {
+  int32_t clamped_liquid, clamped_gas, x_lo, x_hi, y_lo, y_hi, z_lo, z_hi;
+} pf_escape_stats;
+
+/* The SparseSim3DTP fields that step() reads (src/driver/sparse_sim3d_tp.h:12-28) */
+typedef struct pf_step_cfg {
+  double dt, gravity, Vp, alpha_liquid, alpha_gas;
+  double adaptive_cfl, adaptive_min_dt;
+  double c_div;              /* divergenceCorrection passed to the projection */
+  int32_t adaptive_timestep, advection_order;
+  int32_t timing;            /* 1: record per-stage HIP-event times into pf_step_stats */
+  int32_t reserved;
+  pf_phase phase;
+  pf_solve_cfg solve;
+} pf_step_cfg;
+
+enum pf_stage {
+  PF_STAGE_BIN = 0, PF_STAGE_MARK, PF_STAGE_P2G, PF_STAGE_GRID, PF_STAGE_ASSEMBLE,
+  PF_STAGE_SOLVE, PF_STAGE_VELOCITY, PF_STAGE_G2P_ADVECT, PF_NSTAGES
+};
+
+typedef struct pf_step_stats {
+  double effective_dt, max_particle_speed, cfl_limit_dt;
+  int32_t limited, reserved;
+  pf_solve_stats solve;
+  pf_escape_stats escape;
+  double stage_ms[PF_NSTAGES];
+  double total_ms;
+} pf_step_stats;
+
+typedef struct pf_ctx pf_ctx;
+
+/* ---- lifecycle (replaces constructing SparseSim3DTP / SparseMacGrid3D<4>,
+ *      src/driver/sparse_sim3d_tp.h:162) */
+int pf_create(pf_ctx** out, const pf_grid_desc* desc);
+int pf_destroy(pf_ctx* ctx);
+const char* pf_last_error(const pf_ctx* ctx);
+int pf_abi_version(void);
+int pf_synchronize(pf_ctx* ctx);
+
+/* ---- particles (replaces Particles3DTP, src/particles/particles3d_tp.h:4-35).
+ * Host arrays of n entries; volume may be NULL (all 1.0).  Positions must lie in the
+ * cell range [o, o + n*dx) on each axis; the caller's order is kept for download. */
+int pf_particles_upload(pf_ctx* ctx, int64_t n, const float* x, const float* y, const float* z,
+                        const float* u, const float* v, const float* w, const uint8_t* type,
+                        const float* volume);
+int pf_particles_download(pf_ctx* ctx, float* x, float* y, float* z, float* u, float* v, float* w);
+int64_t pf_particle_count(const pf_ctx* ctx);
+/* Seeded synthetic scene generated on the device (same construction as the oracle's
+ * scene generator): 0 Rayleigh-Taylor, 1 bubble tank, 2 dam-break. */
+int pf_scene_generate(pf_ctx* ctx, int32_t kind, uint64_t seed);
+
+/* ---- step stages, in SparseSim3DTP::step order (sparse_sim3d_tp.cpp:854-884) */
+/* markCells, sparse_sim3d_tp.cpp:22-50 (also bins particles by cell) */
+int pf_mark_cells(pf_ctx* ctx);
+/* spP2G3D_tp, sparse_ops3d_tp.cpp:95-153 / sparse_ops3d_tp.h:8 */
+int pf_p2g(pf_ctx* ctx, const pf_phase* phase, double Vp);
+/* `SparseMacGrid3D<4> saved = grid`, sparse_sim3d_tp.cpp:866 (velocities only) */
+int pf_save_velocity(pf_ctx* ctx);
+/* applyGravity, sparse_sim3d_tp.cpp:52-70 */
+int pf_apply_gravity(pf_ctx* ctx, double dt, double gravity);
+/* applyWallBoundary, sparse_sim3d_tp.cpp:72-97 */
+int pf_apply_walls(pf_ctx* ctx);
+/* spProjectStepVC3D, sparse_ops3d_tp.cpp:234-423 / sparse_ops3d_tp.h:9-10 */
+int pf_project(pf_ctx* ctx, const pf_phase* phase, double dt, const pf_solve_cfg* cfg,
+               double div_correction, pf_solve_stats* stats);
+/* spG2P3D_tp, sparse_ops3d_tp.cpp:155-172 (saved = the pf_save_velocity copy) */
+int pf_g2p(pf_ctx* ctx, double alpha_liquid, double alpha_gas);
+/* spAdvect3D_tp, sparse_ops3d_tp.cpp:201-232 / sparse_ops3d_tp.h:12-15 */
+int pf_advect(pf_ctx* ctx, double dt, int32_t order, pf_escape_stats* stats);
+/* computeAdaptiveParticleTimestep3D, src/driver/timestep3d.h:16-50 */
+int pf_timestep(pf_ctx* ctx, double requested_dt, int32_t enabled, double cfl, double min_dt,
+                double* effective_dt, double* max_speed, double* cfl_limit_dt, int32_t* limited);
+/* SparseSim3DTP::step, sparse_sim3d_tp.cpp:854-884 (default flags: no particle
+ * adaptivity, secondary particles or surface tension), with G2P and advection fused */
+int pf_step(pf_ctx* ctx, const pf_step_cfg* cfg, pf_step_stats* stats);
+
+/* ---- field access (SparseBlockGrid3D::get / ref, sparse_block_grid3d.h:46-55).
+ * Dense arrays in i + fnx*(j + fny*k) order over the field's own extents
+ * (faces: (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1); cells: (nx,ny,nz)). */
+int pf_field_dims(const pf_ctx* ctx, int32_t field, int32_t dims[3]);
+int pf_field_download(pf_ctx* ctx, int32_t field, float* dense);
+/* Writes a field and activates blocks as ref() would: every block listed in
+ * block_active (one byte per block id), or if NULL every block holding a non-zero. */
+int pf_field_upload(pf_ctx* ctx, int32_t field, const float* dense, const uint8_t* block_active);
+/* Active block ids of a field in ascending order (the set of
+ * SparseBlockGrid3D::active_block_ids, sparse_block_grid3d.h:16); count returned in *n. */
+int pf_active_blocks(pf_ctx* ctx, int32_t field, int32_t* bids, int64_t capacity, int64_t* n);
+
+/* ---- host-side physics helpers (no device work) */
+/* calibrateRhoTilde0, src/transfer/transfer3d_tp.cpp:29-36 */
+double pf_calibrate_rho_tilde0(const pf_phase* phase, double Vp);
+/* phiFromRawDensity and betaFromPhi, src/physics/phasefield.h:10-18 */
+double pf_phi_from_raw(double raw, const pf_phase* phase);
+double pf_beta_from_phi(double phi, const pf_phase* phase);
+
+/* ---- measurement hooks: device pointers and the stream, for callers that time
+ * kernels on the library's stream (bench.py) */
+void* pf_stream(pf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
