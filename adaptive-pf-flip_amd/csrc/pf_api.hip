@@ -42,6 +42,30 @@ int fail(pf_ctx* c, int code, const std::string& m) {
   return code;
 }
 
+// per-kernel event bracket (active only while pf_kernel_timing is enabled)
+long long kbegin(pf_ctx* c, int kid) {
+  if (!c->ktime) return -1;
+  while (c->kev.size() < c->kev_used + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    c->kev.push_back(e);
+  }
+  size_t i = c->kev_used;
+  c->kev_used += 2;
+  (void)hipEventRecord(c->kev[i], c->stream);
+  c->krec.push_back({kid, i});
+  return (long long)i;
+}
+void kend(pf_ctx* c, long long i) {
+  if (i >= 0) (void)hipEventRecord(c->kev[(size_t)i + 1], c->stream);
+}
+#define TIMED(c, kid, ...)              \
+  do {                                  \
+    long long _ki = kbegin((c), (kid)); \
+    __VA_ARGS__;                        \
+    kend((c), _ki);                     \
+  } while (0)
+
 template <typename T>
 hipError_t dalloc(T** p, long long n) {
   *p = nullptr;
@@ -136,14 +160,16 @@ int ensure_binned(pf_ctx* c) {
   const long long ncell = c->dc.nblocks * kBlockVol;
   HIP_TRY(c, fill_u32(c->stream, c->cell_count, 0u, ncell));
   if (c->np > 0) {
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa, c->geom,
-                       c->dc, c->key, c->rank, c->cell_count);
+    TIMED(c, PF_K_BIN_COUNT,
+          hipLaunchKernelGGL(k_bin_count, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
+                             c->geom, c->dc, c->key, c->rank, c->cell_count));
     LAUNCH_CHECK(c);
   }
   HIP_TRY(c, scan_u32(*c, c->cell_count, c->cell_start, ncell));
   if (c->np > 0) {
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa, c->pb,
-                       c->has_vol, c->key, c->rank, c->cell_start);
+    TIMED(c, PF_K_BIN_SCATTER,
+          hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
+                             c->pb, c->has_vol, c->key, c->rank, c->cell_start));
     LAUNCH_CHECK(c);
     std::swap(c->pa, c->pb);
   }
@@ -173,12 +199,14 @@ int run_face_ops(pf_ctx* c, int ops, float dtg) {
     float* sv = c->f[F_SU + ax];
     uint8_t* av = c->act[F_U + ax];
     uint8_t* as = c->act[F_SU + ax];
+    long long ki = kbegin(c, PF_K_FACE_OPS);
     if (ax == 0)
       hipLaunchKernelGGL(k_face_ops<0>, grid, dim3(256), 0, c->stream, *d[ax], c->geom, vel, mass, sv, av, as, ops, dtg);
     else if (ax == 1)
       hipLaunchKernelGGL(k_face_ops<1>, grid, dim3(256), 0, c->stream, *d[ax], c->geom, vel, mass, sv, av, as, ops, dtg);
     else
       hipLaunchKernelGGL(k_face_ops<2>, grid, dim3(256), 0, c->stream, *d[ax], c->geom, vel, mass, sv, av, as, ops, dtg);
+    kend(c, ki);
     LAUNCH_CHECK(c);
   }
   return PF_OK;
@@ -215,7 +243,9 @@ int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp) {
   for (int col = 0; col < 8; ++col) {
     a.list = c->color_list + (long long)col * c->color_cap;
     a.count = c->color_cnt + col;
-    hipLaunchKernelGGL(k_p2g_color, dim3((unsigned)std::max<long long>(c->color_cap, 1)), dim3(64), 0, c->stream, a);
+    TIMED(c, PF_K_P2G,
+          hipLaunchKernelGGL(k_p2g_color, dim3((unsigned)std::max<long long>(c->color_cap, 1)), dim3(64), 0,
+                             c->stream, a));
     LAUNCH_CHECK(c);
   }
   return PF_OK;
@@ -293,7 +323,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   ao.mu = c->f[F_MU]; ao.mv = c->f[F_MV]; ao.mw = c->f[F_MW];
   ao.part = c->partial;
   const unsigned G = list_grid(npb);
-  hipLaunchKernelGGL(k_assemble, dim3(G), dim3(256), 0, c->stream, pa, ao);
+  TIMED(c, PF_K_ASSEMBLE, hipLaunchKernelGGL(k_assemble, dim3(G), dim3(256), 0, c->stream, pa, ao));
   LAUNCH_CHECK(c);
   hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 1, 0u, c->red + 2);
   LAUNCH_CHECK(c);
@@ -305,8 +335,9 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   hipLaunchKernelGGL(k_ir_start, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->f[F_P], c->cd64, cfg->warm_start);
   LAUNCH_CHECK(c);
   auto residual = [&]() -> int {
-    hipLaunchKernelGGL(k_residual64, dim3(G), dim3(256), 0, c->stream, va, c->b64, c->x64, c->r64, c->cd64, c->cx64,
-                       c->cy64, c->cz64);
+    TIMED(c, PF_K_RESIDUAL64,
+          hipLaunchKernelGGL(k_residual64, dim3(G), dim3(256), 0, c->stream, va, c->b64, c->x64, c->r64, c->cd64,
+                             c->cx64, c->cy64, c->cz64));
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 2, 2u, c->red);
     LAUNCH_CHECK(c);
@@ -340,19 +371,32 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     LAUNCH_CHECK(c);
     float* pold = c->pA;
     float* pnew = c->pB;
+    int iter_before = 0;
     for (;;) {
+      const size_t rec0 = c->krec.size();
       for (int q = 0; q < check; ++q) {
-        hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(256), 0, c->stream, va, c->z32, pold, pnew, c->Ap, c->cd32,
-                           c->cx32, c->cy32, c->cz32);
+        TIMED(c, PF_K_PCG_SPMV,
+              hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(256), 0, c->stream, va, c->z32, pold, pnew, c->Ap,
+                                 c->cd32, c->cx32, c->cy32, c->cz32));
         hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
-        hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, c->stream, va, c->x32, c->r32, c->z32, pnew, c->Ap,
-                           c->invd32);
+        TIMED(c, PF_K_PCG_UPDATE,
+              hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, c->stream, va, c->x32, c->r32, c->z32, pnew,
+                                 c->Ap, c->invd32));
         hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
         std::swap(pold, pnew);
       }
       LAUNCH_CHECK(c);
       HIP_TRY(c, hipMemcpyAsync(c->done_host, &c->sc->iter, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(c, hipStreamSynchronize(c->stream));
+      // launches after convergence return at once: keep only the iterations that ran
+      const int ran = c->done_host[0] - iter_before;
+      iter_before = c->done_host[0];
+      int kept_s = 0, kept_u = 0;
+      for (size_t r = rec0; r < c->krec.size(); ++r) {
+        int& k = c->krec[r].first;
+        if (k == PF_K_PCG_SPMV && kept_s++ >= ran) k = -1;
+        if (k == PF_K_PCG_UPDATE && kept_u++ >= ran) k = -1;
+      }
       if (c->done_host[1]) break;  // {iter, done, breakdown, stalled}
     }
     const int it = c->done_host[0];
@@ -393,7 +437,8 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   vl.s = dt / g.dx;
   vl.list = c->pblk;
   vl.nlist = npb;
-  hipLaunchKernelGGL(k_velocity_update, dim3(grid_for(npb, 4)), dim3(256), 0, c->stream, vl);
+  TIMED(c, PF_K_VELOCITY,
+        hipLaunchKernelGGL(k_velocity_update, dim3(grid_for(npb, 4)), dim3(256), 0, c->stream, vl));
   LAUNCH_CHECK(c);
   record(c, timing, ev_vel);
   std::swap(c->pblk, c->pblk_prev);
@@ -424,7 +469,8 @@ int run_g2p_advect(pf_ctx* c, bool g2p, bool adv, double aL, double aG, double d
   a.do_g2p = g2p; a.do_advect = adv;
   a.esc = c->esc_dev;
   a.vmax_bits = reinterpret_cast<uint32_t*>(c->vmax_dev);
-  hipLaunchKernelGGL(k_g2p_advect, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, a);
+  TIMED(c, PF_K_G2P_ADVECT,
+        hipLaunchKernelGGL(k_g2p_advect, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, a));
   LAUNCH_CHECK(c);
   if (adv) c->binned = false;
   return PF_OK;
@@ -481,6 +527,51 @@ int pf_abi_version(void) { return PF_ABI_VERSION; }
 const char* pf_last_error(const pf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 void* pf_stream(pf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int pf_kernel_timing(pf_ctx* c, int32_t enable) {
+  if (!c) return PF_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->ktime = enable != 0;
+  c->krec.clear();
+  c->kev_used = 0;
+  return PF_OK;
+}
+
+int pf_kernel_time(pf_ctx* c, int32_t kernel, int64_t* launches, double* total_ms) {
+  if (!c || !launches || !total_ms || kernel < 0 || kernel >= PF_NKERNELS) return PF_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  int64_t n = 0;
+  double t = 0.0;
+  for (const auto& r : c->krec) {
+    if (r.first != kernel) continue;
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->kev[r.second], c->kev[r.second + 1]));
+    t += ms;
+    ++n;
+  }
+  *launches = n;
+  *total_ms = t;
+  return PF_OK;
+}
+
+int pf_copy_probe(pf_ctx* c, int64_t n, double* ms) {
+  if (!c || n <= 0 || n > c->dc.nblocks * kBlockVol) return PF_ERR_ARG;
+  hipEvent_t a, b;
+  HIP_TRY(c, hipEventCreate(&a));
+  HIP_TRY(c, hipEventCreate(&b));
+  long long blocks = std::min<long long>((n + 255) / 256, 1 << 20);
+  HIP_TRY(c, hipEventRecord(a, c->stream));
+  hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->x32, c->r32, (long long)n);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, hipEventRecord(b, c->stream));
+  HIP_TRY(c, hipEventSynchronize(b));
+  float t = 0.f;
+  HIP_TRY(c, hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (ms) *ms = t;
+  return PF_OK;
+}
 
 int pf_synchronize(pf_ctx* c) {
   if (!c) return PF_ERR_ARG;
@@ -567,6 +658,7 @@ int pf_destroy(pf_ctx* c) {
   hipFree(c->scan_tiles); hipFree(c->stage_f);
   for (int e = 0; e <= PF_NSTAGES; ++e)
     if (c->ev[e]) hipEventDestroy(c->ev[e]);
+  for (hipEvent_t e : c->kev) hipEventDestroy(e);
   hipStreamDestroy(c->stream);
   delete c;
   return PF_OK;

@@ -18,6 +18,8 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "pf_gpu.h"
 
@@ -146,6 +148,12 @@ struct Ctx {
   long long stage_cap = 0;
 
   hipEvent_t ev[PF_NSTAGES + 1] = {};
+
+  // per-kernel event timing (measurement hook)
+  bool ktime = false;
+  std::vector<hipEvent_t> kev;
+  size_t kev_used = 0;
+  std::vector<std::pair<int, size_t>> krec;  // (kernel id, index of its start event)
 };
 
 constexpr int kMaxPartials = 4096;

@@ -136,6 +136,12 @@ __global__ void k_compact(const uint8_t* __restrict__ flags, const uint32_t* __r
 
 }  // namespace
 
+__global__ void k_copy_f32(const float* __restrict__ a, float* __restrict__ b, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
 hipError_t fill_u8(hipStream_t s, uint8_t* p, uint8_t v, long long n) { return fill(s, p, v, n); }
 hipError_t fill_u32(hipStream_t s, uint32_t* p, uint32_t v, long long n) { return fill(s, p, v, n); }
 hipError_t fill_f32(hipStream_t s, float* p, float v, long long n) { return fill(s, p, v, n); }

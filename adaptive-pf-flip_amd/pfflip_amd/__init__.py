@@ -23,6 +23,10 @@ FIELD_SAVED_U, FIELD_SAVED_V, FIELD_SAVED_W = 8, 9, 10
 STOP_REF_MAXABS, STOP_REL2 = 0, 1
 SCENE_RT, SCENE_BUBBLE, SCENE_DAMBREAK = 0, 1, 2
 STAGES = ("bin", "mark", "p2g", "grid", "assemble", "solve", "velocity", "g2p_advect")
+KERNELS = ("bin_count", "bin_scatter", "p2g", "face_ops", "assemble", "pcg_spmv", "pcg_update",
+           "residual64", "velocity", "g2p_advect")
+(K_BIN_COUNT, K_BIN_SCATTER, K_P2G, K_FACE_OPS, K_ASSEMBLE, K_PCG_SPMV, K_PCG_UPDATE,
+ K_RESIDUAL64, K_VELOCITY, K_G2P_ADVECT) = range(10)
 
 
 class GridDesc(C.Structure):
@@ -117,6 +121,9 @@ def lib():
             "pf_phi_from_raw": (dbl, [dbl, C.POINTER(Phase)]),
             "pf_beta_from_phi": (dbl, [dbl, C.POINTER(Phase)]),
             "pf_stream": (vp, [vp]),
+            "pf_kernel_timing": (C.c_int, [vp, i32]),
+            "pf_kernel_time": (C.c_int, [vp, i32, C.POINTER(i64), C.POINTER(dbl)]),
+            "pf_copy_probe": (C.c_int, [vp, i64, C.POINTER(dbl)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -236,6 +243,21 @@ class Context:
 
     def synchronize(self):
         self._check(lib().pf_synchronize(self.h), "pf_synchronize")
+
+    # measurement hooks
+    def kernel_timing(self, enable: bool):
+        self._check(lib().pf_kernel_timing(self.h, int(enable)), "pf_kernel_timing")
+
+    def kernel_time(self, kernel: int):
+        """(launches, summed ms) of one PF_K_* kernel since timing was enabled."""
+        n, ms = C.c_int64(), C.c_double()
+        self._check(lib().pf_kernel_time(self.h, kernel, C.byref(n), C.byref(ms)), "pf_kernel_time")
+        return n.value, ms.value
+
+    def copy_probe(self, n: int) -> float:
+        ms = C.c_double()
+        self._check(lib().pf_copy_probe(self.h, n, C.byref(ms)), "pf_copy_probe")
+        return ms.value
 
     # fields
     def dims(self, field):

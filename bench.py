@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the sparse two-phase PF-FLIP step on one MI355X.
+
+Workload (the headline metric "step ms @256^3 two-phase"): Rayleigh-Taylor on a
+256^3 MAC grid of the unit box, 8 jittered particles per interior cell
+(131,096,512 FP32 particles), heavy rho = 1000 above
+y = 0.5 + 0.05 cos(2 pi x) cos(2 pi z), light rho = 1 below, zero initial
+velocity, g = -9.81, dt = 1e-3, FLIP 0.95, pressure solved until
+||b - A x||_2 / ||b||_2 <= 1e-6 measured in FP64.  Synthetic data from the seeded
+scene generator; inputs are resident in HBM before the timed region.
+
+One "step" is SparseSim3DTP::step (bin, mark, P2G, save/gravity/walls, assembly,
+solve, velocity update, G2P + RK2 advection).  W untimed warm-up steps, then K
+timed steps bracketed by device synchronisation.  Prints one JSON line.
+
+The CPU baseline times the reference's own SparseSim3DTP::step (oracle/_ref,
+compiled from the reference sources) on a bounded sample: the same scene
+construction at 64^3 (1,906,624 particles), scaled by the particle count to the
+256^3 workload.  It is a reported baseline, not the target.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "adaptive-pf-flip_amd"))
+
+HBM_PEAK_GBS = 8000.0
+SPMV_BYTES_PER_CELL = 32  # z, p_old, diag, 3 couplings read; p_new, Ap written (FP32)
+
+
+def _ensure_built():
+    import pfflip_amd as G
+    if not os.path.exists(G.LIB_PATH):
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("pf_build", os.path.join(ROOT, "adaptive-pf-flip_amd", "build.py"))
+        m = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(m)
+        m.build_lib()
+    return G
+
+
+def step_cfg(G, n, tol, max_iters, warm):
+    ph = G.Phase(1000.0, 1.0, 1.0, 1.0)
+    ph.rho_tilde_0 = G.calibrate_rho_tilde0(ph, 1.0)
+    solve = G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=1e-4,
+                       max_outer=50, warm_start=warm, check_every=16)
+    return G.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95,
+                     adaptive_cfl=0.5, adaptive_min_dt=1e-5, c_div=0.0, adaptive_timestep=0,
+                     advection_order=2, timing=1, phase=ph, solve=solve)
+
+
+def cpu_baseline(n_sample, n_full, seed):
+    """Reference SparseSim3DTP::step on the n_sample^3 RT scene, scaled to n_full^3."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    dx = 1.0 / n_sample
+    x, y, z, t = O.scene(O.SCENE_RT, n_sample, n_sample, n_sample, dx, seed)
+    ph = O.Phase(1000.0, 1.0, 1.0, 1.0)
+    ph.rho_tilde_0 = O.calibrate_rho_tilde0(ph, 1.0)
+    cfg = O.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95,
+                    adaptive_timestep=0, advection_order=2, cg_iters=600, stop_rule=O.STOP_REF_MAXABS,
+                    adaptive_cfl=0.5, adaptive_min_dt=1e-5, cg_tol=1e-7, c_div_volume_correction=0,
+                    c_div_strength=1.0, liquid_volume_target=0.0, phase=ph)
+    ps = O.Particles(x, y, z, type=t)
+    scale = O.lib().orc_scene_count(O.SCENE_RT, n_full, n_full, n_full) / len(x)
+    if O.ref_available():
+        rs = O.RefSim(n_sample, n_sample, n_sample, dx)
+        rs.config(cfg)
+        rs.set_particles(ps)
+        t0 = time.perf_counter()
+        out = rs.step()
+        wall = (time.perf_counter() - t0) * 1e3
+        kind = "reference"
+        ms = out[12]
+    else:
+        g = O.Grid(n_sample, n_sample, n_sample, dx, faithful=True)
+        t0 = time.perf_counter()
+        g.step(ps, cfg)
+        wall = ms = (time.perf_counter() - t0) * 1e3
+        kind = "port"
+    return {
+        "value": round(ms * scale, 1),
+        "unit": f"ms/step (estimated at {n_full}^3)",
+        "cores": 8,
+        "kind": kind,
+        "sample": (f"one SparseSim3DTP::step of the {n_sample}^3 RT scene ({len(x)} particles, "
+                   f"reference stop rule |r|_inf < 1e-7, cg_iters 600) took {ms:.0f} ms on the host; "
+                   f"scaled x{scale:.2f} by particle count; P2G uses 8 std::async colour tasks, "
+                   f"the rest one thread"),
+        "sample_ms": round(ms, 1),
+        "nproc": os.cpu_count(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256, help="grid cells per axis")
+    ap.add_argument("--seed", type=int, default=20250601)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iters", type=int, default=50000)
+    ap.add_argument("--warm-start", type=int, default=0)
+    ap.add_argument("--cpu-sample-n", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    G = _ensure_built()
+    n = args.n
+    ctx = G.Context(n, n, n, 1.0 / n)
+    ctx.scene(G.SCENE_RT, args.seed)
+    cfg = step_cfg(G, n, args.tol, args.max_iters, args.warm_start)
+    print(f"[bench] {n}^3 RT, {ctx.count()} particles; warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        st = ctx.step(cfg)
+        print(f"[bench] warmup step: {st.total_ms:.1f} ms, solve {st.solve.iterations} it", file=sys.stderr,
+              flush=True)
+    ctx.synchronize()
+    if not args.no_kernel_timing:
+        ctx.kernel_timing(True)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(ctx.step(cfg))
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    print(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step", file=sys.stderr, flush=True)
+
+    stage = {name: round(sum(s.stage_ms[i] for s in stats) / len(stats), 3) for i, name in enumerate(G.STAGES)}
+    iters = [s.solve.iterations for s in stats]
+    fluid = stats[-1].solve.fluid_cells
+    kernels = {}
+    roofline = None
+    if not args.no_kernel_timing:
+        for k, name in enumerate(G.KERNELS):
+            nl, ms = ctx.kernel_time(k)
+            if nl:
+                kernels[name] = {"launches": nl, "avg_ms": round(ms / nl, 5), "total_ms": round(ms, 3)}
+        ctx.kernel_timing(False)
+        sp = kernels.get("pcg_spmv")
+        if sp:
+            bytes_per_launch = SPMV_BYTES_PER_CELL * fluid
+            achieved = bytes_per_launch / (sp["avg_ms"] * 1e-3) / 1e9
+            traffic = None
+            tf = os.path.join(ROOT, "profiles", "pcg_spmv_rt256_traffic.json")
+            if os.path.exists(tf) and n == 256:
+                traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+            roofline = {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": traffic, "bytes_per_launch": bytes_per_launch,
+                        "avg_launch_ms": sp["avg_ms"]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (reference step on the 64^3 sample) ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_sample_n, n, args.seed)
+        print(f"[bench] cpu sample {cpu['sample_ms']} ms", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "step ms @256^3 two-phase",
+        "value": round(ms_per_step, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 dots, scalars and refinement)",
+        "data": "synthetic (seeded RT scene)",
+        "config": {"workload": f"sparse two-phase Rayleigh-Taylor {n}^3, rho 1000:1, dt 1e-3, "
+                               f"{ctx.count()} particles, plain Jacobi PCG + FP64 refinement to "
+                               f"{args.tol:g} rel. residual",
+                   "grid": n, "particles": ctx.count(), "fluid_cells": fluid,
+                   "solver": "jacobi-pcg-fp32+ir-fp64", "warm_start": bool(args.warm_start)},
+        "stage_ms": stage,
+        "pressure_ms_to_1e-6": {"plain_pcg": stage["solve"], "learned_pc": None},
+        "solve_iterations": iters,
+        "true_rel_residual": [s.solve.true_rel_residual for s in stats],
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu_estimate"] = round(cpu["value"] / ms_per_step, 1)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -180,9 +180,22 @@ double pf_calibrate_rho_tilde0(const pf_phase* phase, double Vp);
 double pf_phi_from_raw(double raw, const pf_phase* phase);
 double pf_beta_from_phi(double phi, const pf_phase* phase);
 
-/* ---- measurement hooks: device pointers and the stream, for callers that time
- * kernels on the library's stream (bench.py) */
+/* ---- measurement hooks (no reference counterpart: the reference times whole
+ * steps with steady_clock, apps/bench_large_scale3d_tp.cpp:559-563) */
+enum pf_kernel {
+  PF_K_BIN_COUNT = 0, PF_K_BIN_SCATTER, PF_K_P2G, PF_K_FACE_OPS, PF_K_ASSEMBLE,
+  PF_K_PCG_SPMV, PF_K_PCG_UPDATE, PF_K_RESIDUAL64, PF_K_VELOCITY, PF_K_G2P_ADVECT, PF_NKERNELS
+};
+/* The library's HIP stream (hipStream_t). */
 void* pf_stream(pf_ctx* ctx);
+/* enable != 0: bracket every launch of the kernels above with HIP events on the
+ * library's stream (counters reset); 0: stop. */
+int pf_kernel_timing(pf_ctx* ctx, int32_t enable);
+/* launches and summed event time of one kernel since timing was enabled (synchronises) */
+int pf_kernel_time(pf_ctx* ctx, int32_t kernel, int64_t* launches, double* total_ms);
+/* plain device copy of n floats between two scratch buffers (n <= cell count), timed
+ * with events; a known-size kernel for calibrating HBM traffic counters */
+int pf_copy_probe(pf_ctx* ctx, int64_t n, double* ms);
 
 #ifdef __cplusplus
 }

@@ -313,6 +313,9 @@ def reflib():
                                   C.c_void_p]
         L.ref_g2p.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double]
         L.ref_advect.argtypes = [C.c_void_p, C.c_double, C.c_int, _i32]
+        L.ref_sim_init.restype = C.c_int64
+        L.ref_sim_init.argtypes = [C.c_void_p, C.c_int]
+        L.ref_sim_get_types.argtypes = [C.c_void_p, _u8, _dp]
         L.ref_calibrate_rho_tilde0.restype = C.c_double
         L.ref_calibrate_rho_tilde0.argtypes = [_dp, C.c_double]
         L.ref_phi.restype = C.c_double
@@ -359,6 +362,17 @@ class RefSim:
         reflib().ref_sim_get_particles(self.h, *a)
         return Particles(*a, type=like.type.copy(),
                          volume=None if like.volume is None else like.volume.copy())
+
+    def init_scene(self, kind):
+        """Run the reference's own initialiser; returns (Particles, Phase)."""
+        n = reflib().ref_sim_init(self.h, kind)
+        t = np.zeros(max(n, 1), np.uint8)
+        ph = np.zeros(4)
+        reflib().ref_sim_get_types(self.h, t, ph)
+        a = [np.zeros(n) for _ in range(6)]
+        reflib().ref_sim_get_particles(self.h, *a)
+        self.n = n
+        return Particles(*a, type=t[:n]), Phase(*ph)
 
     def step(self):
         out = np.zeros(13)

@@ -151,6 +151,23 @@ void ref_advect(void* sp, double dt, int order, int32_t* stats) {
   }
 }
 
+// The reference's own scene initialisers (sparse_sim3d_tp.cpp:401-414, 739-776):
+// kind 0 initRayleighTaylor, 1 initBubbleTank, 2 initTwoPhaseDamBreak.
+int64_t ref_sim_init(void* sp, int kind) {
+  auto* s = static_cast<SparseSim3DTP*>(sp);
+  if (kind == 0) s->initRayleighTaylor();
+  else if (kind == 1) s->initBubbleTank();
+  else s->initTwoPhaseDamBreak();
+  return (int64_t)s->particles.size();
+}
+
+void ref_sim_get_types(void* sp, uint8_t* t, double* phase) {
+  auto* s = static_cast<SparseSim3DTP*>(sp);
+  for (size_t p = 0; p < s->particles.size(); ++p) t[p] = s->particles.type[p];
+  phase[0] = s->phase.rho_l; phase[1] = s->phase.rho_g;
+  phase[2] = s->phase.alpha_phi; phase[3] = s->phase.rho_tilde_0;
+}
+
 double ref_calibrate_rho_tilde0(const double* phase, double Vp) {
   return calibrateRhoTilde0(phaseOf(phase), Vp);
 }

@@ -1,0 +1,35 @@
+"""Shared set-up for the parity tests: seeded scenes, oracle runs, error norms."""
+import numpy as np
+
+import pyoracle as O
+
+
+def rel_err(a, b):
+    """Norm-wise relative error ||a - b||_2 / ||b||_2 (pointwise relative error is
+    meaningless where momentum cancels)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def phase(rho_l=1000.0, rho_g=1.0, alpha=1.0):
+    ph = O.Phase(rho_l, rho_g, alpha, 1.0)
+    ph.rho_tilde_0 = O.calibrate_rho_tilde0(ph, 1.0)
+    return ph
+
+
+def scene(kind, n, seed=11, ny=None, nz=None):
+    ny = ny or n
+    nz = nz or n
+    dx = 1.0 / n
+    x, y, z, t = O.scene(kind, n, ny, nz, dx, seed)
+    return dx, x, y, z, t
+
+
+def step_cfg(ph, dt=1e-3, cg_iters=600, cg_tol=1e-7, stop_rule=O.STOP_REF_MAXABS, order=2):
+    return O.StepCfg(dt=dt, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95,
+                     adaptive_timestep=0, advection_order=order, cg_iters=cg_iters,
+                     stop_rule=stop_rule, adaptive_cfl=0.5, adaptive_min_dt=1e-5, cg_tol=cg_tol,
+                     c_div_volume_correction=0, c_div_strength=1.0, liquid_volume_target=0.0,
+                     phase=ph)

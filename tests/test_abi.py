@@ -1,0 +1,67 @@
+"""The C-ABI library loads and exports every symbol include/pf_gpu.h declares, and
+the Python structures have the C layout (no device work: runs without a GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import pfflip_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pf_gpu.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(G.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    src = tmp_path / "sizes.c"
+    src.write_text('#include "pf_gpu.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
+                   "sizeof(pf_grid_desc),sizeof(pf_phase),sizeof(pf_solve_cfg),"
+                   "sizeof(pf_solve_stats),sizeof(pf_escape_stats),sizeof(pf_step_cfg),"
+                   "sizeof(pf_step_stats),offsetof(pf_step_cfg,phase),offsetof(pf_step_stats,solve));"
+                   "return 0;}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    want = [C.sizeof(G.GridDesc), C.sizeof(G.Phase), C.sizeof(G.SolveCfg), C.sizeof(G.SolveStats),
+            C.sizeof(G.EscapeStats), C.sizeof(G.StepCfg), C.sizeof(G.StepStats),
+            G.StepCfg.phase.offset, G.StepStats.solve.offset]
+    assert got == want
+
+
+def test_host_helpers_without_device():
+    # host-only entry points: Eq. 7 / beta and the rho_tilde_0 calibration
+    import pyoracle as O
+    ph = G.Phase(1000.0, 1.0, 1.0, 1.0)
+    oph = O.Phase(1000.0, 1.0, 1.0, 1.0)
+    assert G.calibrate_rho_tilde0(ph) == O.calibrate_rho_tilde0(oph, 1.0)
+    L = G.lib()
+    for raw in (0.0, 5.0, 7.0, 100.0, 900.0, 2000.0):
+        assert L.pf_phi_from_raw(raw, C.byref(ph)) == O.phi(raw, oph)
+    assert L.pf_beta_from_phi(0.25, C.byref(ph)) == O.beta(0.25, oph)
+    assert L.pf_abi_version() == 1
+
+
+def test_null_arguments_are_rejected_without_device():
+    L = G.lib()
+    assert L.pf_create(None, None) == 1  # PF_ERR_ARG
+    d = G.GridDesc(2, 8, 8, 0, 1.0, 0.0, 0.0, 0.0)
+    h = C.c_void_p()
+    assert L.pf_create(C.byref(h), C.byref(d)) == 1  # nx < 3
+    assert L.pf_mark_cells(None) == 1
