@@ -81,9 +81,12 @@ unsigned grid_for(long long work, long long per_block) {
   return (unsigned)g;
 }
 
+// workgroups for the list kernels: a multiple of 8 (one share per XCD), at most
+// kMaxPartials (one partial sum each)
 unsigned list_grid(long long nlist) {
   long long g = (nlist + 3) / 4;
-  if (g < 1) g = 1;
+  g = ((g + 7) / 8) * 8;
+  if (g < 8) g = 8;
   if (g > kMaxPartials) g = kMaxPartials;
   return (unsigned)g;
 }
@@ -352,43 +355,43 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   S.pinned = c->u32_host[0] == 0;
   S.pinned_cell = S.pinned ? (int64_t)c->u32_host[1] : -1;
   const int max_iters = std::max(cfg->max_iters, 0);
-  const int max_outer = cfg->max_outer > 0 ? cfg->max_outer : 50;
+  const int max_updates = cfg->max_outer > 0 ? cfg->max_outer : 200;
   const int check = cfg->check_every > 0 ? cfg->check_every : 8;
-  const double inner = cfg->inner_rel_tol > 0 ? cfg->inner_rel_tol : 1e-4;
+  const double delta = (cfg->inner_rel_tol > 0 && cfg->inner_rel_tol < 1) ? cfg->inner_rel_tol : 0.1;
   auto met = [&]() {
     if (cfg->stop_rule == PF_STOP_REL2) return bnorm2 == 0.0 || std::sqrt(rr) <= cfg->tol * std::sqrt(bnorm2);
     return rmax < cfg->tol;
   };
+  // FP32 residual level that triggers a (checking) replacement: the target itself
+  const double tol2_abs = cfg->stop_rule == PF_STOP_REL2 ? cfg->tol * cfg->tol * bnorm2 : cfg->tol * cfg->tol;
   int total = 0, outer = 0;
   bool converged = met();
-  while (!converged && total < max_iters && outer < max_outer && rmax > 0.0 && std::isfinite(rmax)) {
-    const double s = 1.0 / rmax;
-    hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(256), 0, c->stream, va, c->r64, s, c->x32, c->r32, c->z32, c->pA,
+  if (!converged && max_iters > 0 && std::isfinite(rmax)) {
+    hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32, c->pA,
                        c->invd32);
+    hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
+                       tol2_abs, max_iters);
     LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, inner * inner,
-                       max_iters - total, 64);
-    LAUNCH_CHECK(c);
-    float* pold = c->pA;
-    float* pnew = c->pB;
     int iter_before = 0;
+    double best_rr = rr;
+    int no_gain = 0;
     for (;;) {
       const size_t rec0 = c->krec.size();
       for (int q = 0; q < check; ++q) {
         TIMED(c, PF_K_PCG_SPMV,
-              hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(256), 0, c->stream, va, c->z32, pold, pnew, c->Ap,
+              hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(256), 0, c->stream, va, c->z32, c->pA, c->pB, c->Ap,
                                  c->cd32, c->cx32, c->cy32, c->cz32));
         hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
         TIMED(c, PF_K_PCG_UPDATE,
-              hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, c->stream, va, c->x32, c->r32, c->z32, pnew,
-                                 c->Ap, c->invd32));
+              hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, c->stream, va, c->x32, c->r32, c->z32, c->pA,
+                                 c->pB, c->Ap, c->invd32));
         hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
-        std::swap(pold, pnew);
       }
       LAUNCH_CHECK(c);
-      HIP_TRY(c, hipMemcpyAsync(c->done_host, &c->sc->iter, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(c, hipMemcpyAsync(c->done_host, &c->sc->iter, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(c, hipStreamSynchronize(c->stream));
-      // launches after convergence return at once: keep only the iterations that ran
+      // {iter, done, breakdown, stalled, iter_best, max_iter, stall_window, upd}
+      // launches after a pause return at once: keep the timings of those that ran
       const int ran = c->done_host[0] - iter_before;
       iter_before = c->done_host[0];
       int kept_s = 0, kept_u = 0;
@@ -397,20 +400,34 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
         if (k == PF_K_PCG_SPMV && kept_s++ >= ran) k = -1;
         if (k == PF_K_PCG_UPDATE && kept_u++ >= ran) k = -1;
       }
-      if (c->done_host[1]) break;  // {iter, done, breakdown, stalled}
+      if (!c->done_host[1]) continue;
+      total = c->done_host[0];
+      const bool breakdown = c->done_host[2] != 0;
+      // residual replacement: x64 += x32, r64 = b - A x64 in FP64
+      hipLaunchKernelGGL(k_ir_accum, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->x32);
+      LAUNCH_CHECK(c);
+      st = residual();
+      if (st) return st;
+      rr = c->red_host[0];
+      rmax = c->red_host[1];
+      ++outer;
+      converged = met();
+      if (rr < best_rr * 0.999) { best_rr = rr; no_gain = 0; } else { ++no_gain; }
+      if (converged || total >= max_iters || outer >= max_updates || !std::isfinite(rr) || no_gain >= 5) {
+        if (breakdown) S.breakdown = 1;
+        break;
+      }
+      // resume from the true residual; after a breakdown restart the direction too
+      hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32,
+                         breakdown ? c->pA : nullptr, c->invd32);
+      if (breakdown)
+        hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
+                           tol2_abs, max_iters - total);
+      else
+        hipLaunchKernelGGL(k_pcg_resume, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
+      LAUNCH_CHECK(c);
+      if (breakdown) { iter_before = 0; }
     }
-    const int it = c->done_host[0];
-    if (c->done_host[2]) S.breakdown = 1;
-    total += it;
-    ++outer;
-    hipLaunchKernelGGL(k_ir_accum, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->x32, rmax);
-    LAUNCH_CHECK(c);
-    st = residual();
-    if (st) return st;
-    rr = c->red_host[0];
-    rmax = c->red_host[1];
-    converged = met();
-    if (it == 0) break;
   }
   S.iterations = total;
   S.outer_iterations = outer;
@@ -470,7 +487,7 @@ int run_g2p_advect(pf_ctx* c, bool g2p, bool adv, double aL, double aG, double d
   a.esc = c->esc_dev;
   a.vmax_bits = reinterpret_cast<uint32_t*>(c->vmax_dev);
   TIMED(c, PF_K_G2P_ADVECT,
-        hipLaunchKernelGGL(k_g2p_advect, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, a));
+        hipLaunchKernelGGL(k_g2p_advect, dim3(std::min(grid_for(c->np, 256), 8192u)), dim3(256), 0, c->stream, a));
   LAUNCH_CHECK(c);
   if (adv) c->binned = false;
   return PF_OK;
@@ -818,7 +835,7 @@ int pf_timestep(pf_ctx* c, double requested_dt, int32_t enabled, double cfl, dou
   if (!c) return PF_ERR_ARG;
   HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->vmax_dev), 0u, 1));
   if (c->np > 0) {
-    hipLaunchKernelGGL(k_max_speed, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
+    hipLaunchKernelGGL(k_max_speed, dim3(std::min(grid_for(c->np, 256), 8192u)), dim3(256), 0, c->stream, c->np, c->pa,
                        reinterpret_cast<uint32_t*>(c->vmax_dev));
     LAUNCH_CHECK(c);
   }

@@ -15,6 +15,27 @@ __device__ inline int cell_coord(float x, double o, double dx) {
   return (int)(((double)x - o) / dx);
 }
 
+// Rank of this lane among the lanes of its wave with the same key, plus one
+// atomicAdd per distinct key of the wave (a wave of cell-sorted particles covers
+// about eight cells, so this is ~8x fewer same-address atomics).
+__device__ inline uint32_t wave_aggregated_increment(uint32_t* cnt, uint32_t key) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint64_t todo = __ballot(1);
+  uint32_t my = 0;
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const uint32_t k = __shfl(key, leader, 64);
+    const uint64_t grp = __ballot(key == k) & todo;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[k], (uint32_t)__popcll(grp));
+    base = __shfl(base, leader, 64);
+    if (key == k && ((grp >> lane) & 1ull)) my = base + (uint32_t)__popcll(grp & below);
+    todo &= ~grp;
+  }
+  return my;
+}
+
 __global__ void k_bin_count(long long n, PBuf P, Geom g, Dims dc, uint32_t* __restrict__ key,
                             uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,7 +50,7 @@ __global__ void k_bin_count(long long n, PBuf P, Geom g, Dims dc, uint32_t* __re
   k = min(max(k, 0), g.nz - 1);
   uint32_t kk = (uint32_t)bm_index(dc, i, j, k);
   key[p] = kk;
-  rank[p] = atomicAdd(&cnt[kk], 1u);
+  rank[p] = wave_aggregated_increment(cnt, kk);
 }
 
 __global__ void k_bin_scatter(long long n, PBuf S, PBuf D, bool has_vol, const uint32_t* __restrict__ key,

@@ -57,13 +57,40 @@ __device__ inline void sample_vel(const G2PArgs& a, const float* const* f, float
   out[2] = sample(f[2], a.df[2], px - 0.5f, py - 0.5f, pz);
 }
 
+// per-thread counters -> one atomic per counter per workgroup
+__device__ inline void block_flush_counters(int (&cnt)[8], float speed, int32_t* esc, uint32_t* vmax_bits,
+                                            bool do_esc, bool do_vmax) {
+  __shared__ int s_cnt[4][8];
+  __shared__ float s_v[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt[q] += __shfl_xor(cnt[q], o, 64);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) speed = fmaxf(speed, __shfl_xor(speed, o, 64));
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s_cnt[wave][q] = cnt[q];
+    s_v[wave] = speed;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 && do_esc) {
+    int t = s_cnt[0][threadIdx.x] + s_cnt[1][threadIdx.x] + s_cnt[2][threadIdx.x] + s_cnt[3][threadIdx.x];
+    if (t) atomicAdd(&esc[threadIdx.x], t);
+  }
+  if (threadIdx.x == 8 && do_vmax) {
+    float m = fmaxf(fmaxf(s_v[0], s_v[1]), fmaxf(s_v[2], s_v[3]));
+    atomicMax(vmax_bits, __float_as_uint(m));
+  }
+}
+
+// grid-stride over particles (fixed grid), counters flushed once per workgroup
 __global__ void __launch_bounds__(256) k_g2p_advect(G2PArgs a) {
-  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = p < a.n;
-  bool cl[6] = {false, false, false, false, false, false};
-  bool liquid = false, gas = false;
+  int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   float speed = 0.f;
-  if (live) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += stride) {
     float x = a.P.x[p], y = a.P.y[p], z = a.P.z[p];
     uint8_t ty = a.P.type[p];
     float k1[3];
@@ -77,7 +104,7 @@ __global__ void __launch_bounds__(256) k_g2p_advect(G2PArgs a) {
       v = (v + (k1[1] - sv[1])) * al + k1[1] * (1.f - al);
       w = (w + (k1[2] - sv[2])) * al + k1[2] * (1.f - al);
       a.P.u[p] = u; a.P.v[p] = v; a.P.w[p] = w;
-      if (isfinite(u) && isfinite(v) && isfinite(w)) speed = sqrtf(u * u + v * v + w * w);
+      if (isfinite(u) && isfinite(v) && isfinite(w)) speed = fmaxf(speed, sqrtf(u * u + v * v + w * w));
     }
     if (a.do_advect) {
       float pos[3] = {x, y, z}, nxt[3], k2[3];
@@ -95,47 +122,32 @@ __global__ void __launch_bounds__(256) k_g2p_advect(G2PArgs a) {
         sample_vel(a, a.vel, m[0], m[1], m[2], k2);
         for (int c = 0; c < 3; ++c) nxt[c] = pos[c] + k2[c] * dt;
       }
+      bool any = false;
       for (int c = 0; c < 3; ++c) {
-        cl[2 * c] = nxt[c] < a.lo[c];
-        cl[2 * c + 1] = nxt[c] > a.hi[c];
+        bool lo = nxt[c] < a.lo[c], hi = nxt[c] > a.hi[c];
+        cnt[2 + 2 * c] += lo;
+        cnt[3 + 2 * c] += hi;
+        any = any || lo || hi;
         nxt[c] = fmaxf(a.lo[c], fminf(a.hi[c], nxt[c]));
       }
       a.P.x[p] = nxt[0]; a.P.y[p] = nxt[1]; a.P.z[p] = nxt[2];
-      bool any = cl[0] || cl[1] || cl[2] || cl[3] || cl[4] || cl[5];
-      liquid = any && ty == 0;
-      gas = any && ty == 1;
+      cnt[0] += any && ty == 0;
+      cnt[1] += any && ty == 1;
     }
   }
-  // wave-aggregated counters
-  const int lane = threadIdx.x & 63;
-  int cnt[8];
-  cnt[0] = __popcll(__ballot(liquid));
-  cnt[1] = __popcll(__ballot(gas));
-#pragma unroll
-  for (int q = 0; q < 6; ++q) cnt[2 + q] = __popcll(__ballot(cl[q]));
-  float m = speed;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-  if (lane == 0) {
-    if (a.do_advect)
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (cnt[q]) atomicAdd(&a.esc[q], cnt[q]);
-    if (a.do_g2p) atomicMax(a.vmax_bits, __float_as_uint(m));
-  }
+  block_flush_counters(cnt, speed, a.esc, a.vmax_bits, a.do_advect, a.do_g2p);
 }
 
 // max |v| over particles with finite velocity (timestep3d.h:27-32)
 __global__ void __launch_bounds__(256) k_max_speed(long long n, PBuf P, uint32_t* vmax_bits) {
-  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   float s = 0.f;
-  if (p < n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
     float u = P.u[p], v = P.v[p], w = P.w[p];
-    if (isfinite(u) && isfinite(v) && isfinite(w)) s = sqrtf(u * u + v * v + w * w);
+    if (isfinite(u) && isfinite(v) && isfinite(w)) s = fmaxf(s, sqrtf(u * u + v * v + w * w));
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s = fmaxf(s, __shfl_xor(s, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(s));
+  int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  block_flush_counters(cnt, s, nullptr, vmax_bits, false, true);
 }
 
 }  // namespace pf

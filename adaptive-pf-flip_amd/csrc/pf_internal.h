@@ -86,9 +86,10 @@ enum { CELL_AIR = 0, CELL_FLUID = 1, CELL_SOLID = 2 };
 // needs no host round trip.
 struct SolveScalars {
   double rz, pAp, alpha, beta;
-  double rr, rr0, rr_best, tol2;
+  double rr, rr0, rr_best, tol2;     // tol2 = delta^2 (replacement trigger)
+  double rr_ref, tol2_abs, rz_pending, pad0;
   int iter, done, breakdown, stalled;
-  int iter_best, max_iter, stall_window, pad;
+  int iter_best, max_iter, stall_window, upd;
 };
 
 struct Ctx {

@@ -125,12 +125,17 @@ __global__ void __launch_bounds__(256) k_reduce_slots(const double* part, int n,
 
 // ------------------------------------------------------------ set-up kernels
 
-// iterate the list in a fixed block->workgroup assignment (deterministic partials)
-#define FOR_LIST_BLOCKS(A, b, lane)                                                   \
-  for (long long _t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); _t < (A).nlist; \
-       _t += (long long)gridDim.x * 4)                                                \
+// Iterate the block list with a fixed block -> workgroup assignment (so partial
+// sums are deterministic) that is XCD-aware: workgroups are dispatched round-robin
+// over the 8 XCDs, so workgroup w runs on XCD w % 8; each XCD gets one contiguous
+// eighth of the (sorted) list, i.e. a slab of the domain whose stencil neighbours
+// sit in that XCD's L2.  Requires gridDim.x to be a multiple of 8.
+#define FOR_LIST_BLOCKS(A, b, lane)                                                                   \
+  for (long long _c = ((A).nlist + 7) / 8, _x = blockIdx.x & 7, _g = gridDim.x >> 3,                  \
+                 _e = min((long long)(A).nlist, (_x + 1) * _c),                                       \
+                 _t = _x * _c + (long long)(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);                \
+       _t < _e; _t += _g * 4)                                                                         \
     for (long long b = (A).list[_t], _once = 1; _once; _once = 0)
-
 
 struct ProjArgs {
   Geom g;
@@ -272,6 +277,15 @@ __global__ void __launch_bounds__(256) k_zero_blocks(ZeroArgs z) {
 }
 
 // ------------------------------------------------------------- solve kernels
+//
+// Mixed-precision Jacobi PCG with reliable updates (residual replacement): the CG
+// recursion runs in FP32 with FP64 dot products and FP64 alpha/beta; whenever the
+// FP32 residual has dropped by a factor delta since the last replacement (or
+// reached the tolerance), the host folds x32 into the FP64 solution, recomputes
+// the true residual b - A x in FP64, and the recursion continues from it with the
+// same search direction.  The two p buffers alternate by the device iteration
+// count, so launches that return early (solver paused or done) never desynchronise
+// them.
 
 struct VecArgs {
   Dims dc;
@@ -306,8 +320,9 @@ __global__ void __launch_bounds__(256) k_residual64(VecArgs a, const double* __r
   block_reduce_store<2>(v, mx, a.part, kMaxPartials);
 }
 
-// inner-solve start: r32 = (float)(r64*s), x32 = 0, z = D^-1 r, p = 0; partial r.z, r.r
-__global__ void __launch_bounds__(256) k_pcg_init(VecArgs a, const double* __restrict__ r64, double s,
+// r32 = (float)r64, x32 = 0, z = D^-1 r (and p_A = 0 on a fresh start);
+// partial r.z, r.r
+__global__ void __launch_bounds__(256) k_pcg_load(VecArgs a, const double* __restrict__ r64,
                                                   float* __restrict__ x32, float* __restrict__ r32,
                                                   float* __restrict__ z32, float* __restrict__ p,
                                                   const float* __restrict__ invd) {
@@ -315,9 +330,10 @@ __global__ void __launch_bounds__(256) k_pcg_init(VecArgs a, const double* __res
   double rz = 0.0, rr = 0.0;
   FOR_LIST_BLOCKS(a, b, lane) {
     long long c = b * kBlockVol + lane;
-    float r = (float)(r64[c] * s);
+    float r = (float)r64[c];
     float z = invd[c] * r;
-    x32[c] = 0.f; r32[c] = r; z32[c] = z; p[c] = 0.f;
+    x32[c] = 0.f; r32[c] = r; z32[c] = z;
+    if (p) p[c] = 0.f;
     rz += (double)r * (double)z;
     rr += (double)r * (double)r;
   }
@@ -326,28 +342,44 @@ __global__ void __launch_bounds__(256) k_pcg_init(VecArgs a, const double* __res
   block_reduce_store<2>(v, mx, a.part, kMaxPartials);
 }
 
-__global__ void __launch_bounds__(256) k_pcg_init_fin(const double* part, int n, SolveScalars* sc,
-                                                      double tol2, int max_iter, int stall_window) {
+// fresh start: scalars from the loaded residual
+__global__ void __launch_bounds__(256) k_pcg_start(const double* part, int n, SolveScalars* sc, double delta2,
+                                                   double tol2_abs, int max_iter) {
   double rz = block_sum_partials(part, n, false);
   double rr = block_sum_partials(part + kMaxPartials, n, false);
   if (threadIdx.x == 0) {
-    sc->rz = rz; sc->rr = rr; sc->rr0 = rr; sc->rr_best = rr; sc->tol2 = tol2;
+    sc->rz = rz; sc->rr = rr; sc->rr0 = rr; sc->rr_best = rr; sc->tol2 = delta2;
+    sc->rr_ref = rr; sc->tol2_abs = tol2_abs;
     sc->beta = 0.0; sc->alpha = 0.0; sc->pAp = 0.0;
-    sc->iter = 0; sc->iter_best = 0; sc->max_iter = max_iter; sc->stall_window = stall_window;
-    sc->breakdown = 0; sc->stalled = 0;
+    sc->iter = 0; sc->iter_best = 0; sc->max_iter = max_iter; sc->stall_window = 0;
+    sc->breakdown = 0; sc->stalled = 0; sc->upd = 0;
     sc->done = (rr == 0.0 || max_iter <= 0) ? 1 : 0;
+  }
+}
+
+// after a residual replacement: beta from the replaced residual, same direction
+__global__ void __launch_bounds__(256) k_pcg_resume(const double* part, int n, SolveScalars* sc) {
+  double rz = block_sum_partials(part, n, false);
+  double rr = block_sum_partials(part + kMaxPartials, n, false);
+  if (threadIdx.x == 0) {
+    sc->beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
+    sc->rz = rz; sc->rr = rr; sc->rr_ref = rr;
+    sc->upd = 0;
+    sc->done = (rr == 0.0 || sc->iter >= sc->max_iter) ? 1 : 0;
   }
 }
 
 // p_new = z + beta p_old (own cell and the six neighbours, recomputed on the fly),
 // Ap = A p_new (FP32), partial p.Ap in FP64
-__global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __restrict__ z,
-                                                  const float* __restrict__ pold, float* __restrict__ pnew,
-                                                  float* __restrict__ Ap, const float* __restrict__ cd,
-                                                  const float* __restrict__ cx, const float* __restrict__ cy,
-                                                  const float* __restrict__ cz) {
+__global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __restrict__ z, float* __restrict__ pA,
+                                                  float* __restrict__ pB, float* __restrict__ Ap,
+                                                  const float* __restrict__ cd, const float* __restrict__ cx,
+                                                  const float* __restrict__ cy, const float* __restrict__ cz) {
   if (a.sc->done) return;
   const int lane = threadIdx.x & 63;
+  const bool odd = a.sc->iter & 1;
+  const float* __restrict__ pold = odd ? pB : pA;
+  float* __restrict__ pnew = odd ? pA : pB;
   const float beta = (float)a.sc->beta;
   double pap = 0.0;
   FOR_LIST_BLOCKS(a, b, lane) {
@@ -389,10 +421,12 @@ __global__ void __launch_bounds__(256) k_pcg_alpha(const double* part, int n, So
 
 // x += alpha p, r -= alpha Ap, z = D^-1 r; partial r.z and r.r
 __global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict__ x, float* __restrict__ r,
-                                                    float* __restrict__ z, const float* __restrict__ p,
-                                                    const float* __restrict__ Ap, const float* __restrict__ invd) {
+                                                    float* __restrict__ z, const float* __restrict__ pA,
+                                                    const float* __restrict__ pB, const float* __restrict__ Ap,
+                                                    const float* __restrict__ invd) {
   if (a.sc->done) return;
   const int lane = threadIdx.x & 63;
+  const float* __restrict__ p = (a.sc->iter & 1) ? pA : pB;  // the p_new of this iteration
   const float alpha = (float)a.sc->alpha;
   double rz = 0.0, rr = 0.0;
   FOR_LIST_BLOCKS(a, b, lane) {
@@ -409,6 +443,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict
   block_reduce_store<2>(v, mx, a.part, kMaxPartials);
 }
 
+// iteration end: count, decide on a residual replacement (pause) or continue
 __global__ void __launch_bounds__(256) k_pcg_beta(const double* part, int n, SolveScalars* sc) {
   if (sc->done) return;
   double rz = block_sum_partials(part, n, false);
@@ -422,21 +457,26 @@ __global__ void __launch_bounds__(256) k_pcg_beta(const double* part, int n, Sol
       sc->done = 1;
       return;
     }
-    if (rr < sc->rr_best) { sc->rr_best = rr; sc->iter_best = it; }
-    if (rr <= sc->tol2 * sc->rr0 || it >= sc->max_iter) { sc->done = 1; return; }
-    if (it - sc->iter_best >= sc->stall_window) { sc->stalled = 1; sc->done = 1; return; }
+    // residual replacement when the FP32 residual fell by delta, reached the
+    // target, or the iteration budget is spent; beta is formed after it
+    if (rr <= sc->tol2 * sc->rr_ref || rr <= sc->tol2_abs || it >= sc->max_iter) {
+      sc->upd = 1;
+      sc->done = 1;
+      sc->rz_pending = rz;
+      return;
+    }
     sc->beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
     sc->rz = rz;
   }
 }
 
-// x64 += x32 / s  (refinement update)
+// x64 += x32 (x32 is reset by the next k_pcg_load)
 __global__ void __launch_bounds__(256) k_ir_accum(VecArgs a, double* __restrict__ x64,
-                                                  const float* __restrict__ x32, double inv_s) {
+                                                  const float* __restrict__ x32) {
   const int lane = threadIdx.x & 63;
   FOR_LIST_BLOCKS(a, b, lane) {
     long long c = b * kBlockVol + lane;
-    x64[c] += (double)x32[c] * inv_s;
+    x64[c] += (double)x32[c];
   }
 }
 

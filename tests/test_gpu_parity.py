@@ -199,3 +199,23 @@ def test_reference_scene_steps_match_oracle():
     assert len(x) == len(ps)
     assert np.isfinite(x).all() and np.isfinite(y).all() and np.isfinite(z).all()
     assert y[ps.type == 0].mean() < heavy0
+
+
+@pytest.mark.parametrize("kind,n", [(O.SCENE_RT, 96), (O.SCENE_BUBBLE, 64)])
+def test_solve_reaches_fp64_tolerance(kind, n):
+    # ||b - A x||_2 / ||b||_2 <= 1e-6 measured in FP64 at 1000:1, on grids large
+    # enough that plain FP32 PCG stagnates above it
+    ctx = G.Context(n, n, n, 1.0 / n)
+    ctx.scene(kind, 5)
+    ph = phase()
+    sim = G.SparseSim3DTP(n, n, n, 1.0 / n)
+    sim.ctx = ctx
+    sim._liquid_volume = 0.0
+    sim.phase = _gphase(ph)
+    sim.dt = 1e-3
+    sim.stop_rule, sim.cg_tol, sim.cg_iters = G.STOP_REL2, 1e-6, 50000
+    for _ in range(2):
+        st = sim.step()
+        assert st.solve.converged == 1
+        assert st.solve.true_rel_residual <= 1e-6
+        assert st.solve.breakdown == 0
