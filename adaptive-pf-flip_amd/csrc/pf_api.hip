@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -364,6 +365,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   };
   // FP32 residual level that triggers a (checking) replacement: the target itself
   const double tol2_abs = cfg->stop_rule == PF_STOP_REL2 ? cfg->tol * cfg->tol * bnorm2 : cfg->tol * cfg->tol;
+  static const bool trace = std::getenv("PF_SOLVE_TRACE") != nullptr;
   int total = 0, outer = 0;
   bool converged = met();
   if (!converged && max_iters > 0 && std::isfinite(rmax)) {
@@ -412,6 +414,13 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
       rmax = c->red_host[1];
       ++outer;
       converged = met();
+      if (trace) {
+        double s_rr32 = 0.0;
+        HIP_TRY(c, hipMemcpy(&s_rr32, &c->sc->rr, sizeof(double), hipMemcpyDeviceToHost));
+        std::fprintf(stderr, "[pf solve] update %d iter %d fp32 rel %.3e true rel %.3e max %.3e%s\n", outer, total,
+                     std::sqrt(s_rr32 / std::max(bnorm2, 1e-300)), std::sqrt(rr / std::max(bnorm2, 1e-300)), rmax,
+                     breakdown ? " breakdown" : "");
+      }
       if (rr < best_rr * 0.999) { best_rr = rr; no_gain = 0; } else { ++no_gain; }
       if (converged || total >= max_iters || outer >= max_updates || !std::isfinite(rr) || no_gain >= 5) {
         if (breakdown) S.breakdown = 1;
