@@ -275,15 +275,21 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const long long nb = c->dc.nblocks;
   // 1. blocks of the previous solve return to zero; the pressure field is cleared
   if (c->npblk_prev > 0) {
+    const bool warm = cfg->warm_start != 0;
+    if (warm) {
+      hipLaunchKernelGGL(k_warm_save, dim3(grid_for(c->npblk_prev, 4)), dim3(256), 0, c->stream, c->pblk_prev,
+                         c->npblk_prev, c->f[F_P], c->x64);
+      LAUNCH_CHECK(c);
+    }
     ZeroArgs z{};
     z.list = c->pblk_prev;
     z.nlist = c->npblk_prev;
     float* f32[12] = {c->f[F_P], c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->cd32, c->cx32, c->cy32, c->cz32, c->invd32};
     for (int q = 0; q < 12; ++q) z.f32[q] = f32[q];
     z.nf32 = 12;
-    double* f64[7] = {c->b64, c->x64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64};
+    double* f64[7] = {c->b64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64, c->x64};
     for (int q = 0; q < 7; ++q) z.f64[q] = f64[q];
-    z.nf64 = 7;
+    z.nf64 = warm ? 6 : 7;  // a warm start keeps x64 (the saved pressure)
     hipLaunchKernelGGL(k_zero_blocks, dim3(grid_for(c->npblk_prev, 4)), dim3(256), 0, c->stream, z);
     LAUNCH_CHECK(c);
   }
@@ -336,7 +342,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   // 3. FP64 iterative refinement around the FP32 Jacobi PCG
   VecArgs va;
   va.dc = c->dc; va.list = c->pblk; va.nlist = npb; va.sc = c->sc; va.part = c->partial;
-  hipLaunchKernelGGL(k_ir_start, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->f[F_P], c->cd64, cfg->warm_start);
+  hipLaunchKernelGGL(k_ir_start, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->invd32, cfg->warm_start);
   LAUNCH_CHECK(c);
   auto residual = [&]() -> int {
     TIMED(c, PF_K_RESIDUAL64,

@@ -43,18 +43,22 @@ struct CellNbr {
   long long xp, xm, yp, ym, zp, zm;
 };
 
-// block-major neighbour indices of cell c = b*64 + lane (valid for interior cells)
+// block-major neighbour indices of cell c = b*64 + lane; a neighbour outside the
+// block grid maps to the cell itself (its coefficient is zero: such cells are
+// never FLUID), so every index stays inside the arrays
 __device__ inline CellNbr neighbours(const Dims& dc, long long b, int lane) {
   const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+  int bx, by, bz;
+  block_coords(dc, b, bx, by, bz);  // wave-uniform
   const long long c = b * kBlockVol + lane;
   const long long sy = (long long)dc.nbx * kBlockVol, sz = (long long)dc.nbx * dc.nby * kBlockVol;
   CellNbr n;
-  n.xp = lx < 3 ? c + 1 : c + kBlockVol - 3;
-  n.xm = lx > 0 ? c - 1 : c - kBlockVol + 3;
-  n.yp = ly < 3 ? c + 4 : c + sy - 12;
-  n.ym = ly > 0 ? c - 4 : c - sy + 12;
-  n.zp = lz < 3 ? c + 16 : c + sz - 48;
-  n.zm = lz > 0 ? c - 16 : c - sz + 48;
+  n.xp = lx < 3 ? c + 1 : (bx + 1 < dc.nbx ? c + kBlockVol - 3 : c);
+  n.xm = lx > 0 ? c - 1 : (bx > 0 ? c - kBlockVol + 3 : c);
+  n.yp = ly < 3 ? c + 4 : (by + 1 < dc.nby ? c + sy - 12 : c);
+  n.ym = ly > 0 ? c - 4 : (by > 0 ? c - sy + 12 : c);
+  n.zp = lz < 3 ? c + 16 : (bz + 1 < dc.nbz ? c + sz - 48 : c);
+  n.zm = lz > 0 ? c - 16 : (bz > 0 ? c - sz + 48 : c);
   return n;
 }
 
@@ -239,13 +243,24 @@ __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
       auto lin_of = [&](int q) {
         return (uint32_t)(nb[q][0] + (long long)a.g.nx * (nb[q][1] + (long long)a.g.ny * nb[q][2]));
       };
-      if (!pinned) {
-        if (m[0] == CELL_FLUID && lin_of(0) != pc) cx = a.scale * fb[0];
-        if (m[2] == CELL_FLUID && lin_of(2) != pc) cy = a.scale * fb[2];
-        if (m[4] == CELL_FLUID && lin_of(4) != pc) cz = a.scale * fb[4];
+      // conductance form: a face is coupled when both cells are FLUID and neither
+      // is pinned; the other non-SOLID faces (AIR, or the pinned cell's column the
+      // reference drops) are Dirichlet terms on the diagonal.  Rows are then
+      // cdir*x_c + sum_f c_f (x_c - x_f), so rounding c_f and cdir to FP32 is a
+      // relative perturbation of each conductance and keeps the near-null space.
+      double dir = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        if (m[q] == CELL_SOLID) continue;
+        const bool coupled = !pinned && m[q] == CELL_FLUID && lin_of(q) != pc;
+        if (!coupled) dir += fb[q];
+        if (coupled && q == 0) cx = a.scale * fb[0];
+        if (coupled && q == 2) cy = a.scale * fb[2];
+        if (coupled && q == 4) cz = a.scale * fb[4];
       }
-      cd = pinned ? 1.0 : a.scale * diag;
-      inv = pinned ? 1.0 : (cd > 0.0 ? 1.0 / cd : 0.0);
+      cd = pinned ? 1.0 : a.scale * dir;
+      const double full = a.scale * diag;
+      inv = pinned ? 1.0 : (full > 0.0 ? 1.0 / full : 0.0);
     }
     o.b64[c] = rhs;
     o.cd64[c] = cd; o.cx64[c] = cx; o.cy64[c] = cy; o.cz64[c] = cz;
@@ -304,13 +319,12 @@ __global__ void __launch_bounds__(256) k_residual64(VecArgs a, const double* __r
   double rr = 0.0, rm = 0.0;
   FOR_LIST_BLOCKS(a, b, lane) {
     long long c = b * kBlockVol + lane;
-    double res = 0.0;
-    if (cd[c] != 0.0) {
-      CellNbr n = neighbours(a.dc, b, lane);
-      double ax = cd[c] * x[c] - (cx[c] * x[n.xp] + cx[n.xm] * x[n.xm] + cy[c] * x[n.yp] + cy[n.ym] * x[n.ym] +
-                                  cz[c] * x[n.zp] + cz[n.zm] * x[n.zm]);
-      res = b64[c] - ax;
-    }
+    // non-FLUID cells have zero coefficients and zero right-hand side
+    CellNbr n = neighbours(a.dc, b, lane);
+    const double xc = x[c];
+    double ax = cd[c] * xc + cx[c] * (xc - x[n.xp]) + cx[n.xm] * (xc - x[n.xm]) + cy[c] * (xc - x[n.yp]) +
+                cy[n.ym] * (xc - x[n.ym]) + cz[c] * (xc - x[n.zp]) + cz[n.zm] * (xc - x[n.zm]);
+    double res = b64[c] - ax;
     r[c] = res;
     rr += res * res;
     rm = fmax(rm, fabs(res));
@@ -384,17 +398,19 @@ __global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __rest
   double pap = 0.0;
   FOR_LIST_BLOCKS(a, b, lane) {
     long long c = b * kBlockVol + lane;
-    float d = cd[c];
-    float pc = 0.f, ap = 0.f;
-    if (d != 0.f) {
-      CellNbr n = neighbours(a.dc, b, lane);
-      pc = fmaf(beta, pold[c], z[c]);
-      float pxp = fmaf(beta, pold[n.xp], z[n.xp]), pxm = fmaf(beta, pold[n.xm], z[n.xm]);
-      float pyp = fmaf(beta, pold[n.yp], z[n.yp]), pym = fmaf(beta, pold[n.ym], z[n.ym]);
-      float pzp = fmaf(beta, pold[n.zp], z[n.zp]), pzm = fmaf(beta, pold[n.zm], z[n.zm]);
-      float off = cx[c] * pxp + cx[n.xm] * pxm + cy[c] * pyp + cy[n.ym] * pym + cz[c] * pzp + cz[n.zm] * pzm;
-      ap = d * pc - off;
-    }
+    // non-FLUID cells: z = p = 0 and zero coefficients give p_new = Ap = 0
+    CellNbr n = neighbours(a.dc, b, lane);
+    const float pc = fmaf(beta, pold[c], z[c]);
+    float pxp = fmaf(beta, pold[n.xp], z[n.xp]), pxm = fmaf(beta, pold[n.xm], z[n.xm]);
+    float pyp = fmaf(beta, pold[n.yp], z[n.yp]), pym = fmaf(beta, pold[n.ym], z[n.ym]);
+    float pzp = fmaf(beta, pold[n.zp], z[n.zp]), pzm = fmaf(beta, pold[n.zm], z[n.zm]);
+    float ap = cd[c] * pc;
+    ap = fmaf(cx[c], pc - pxp, ap);
+    ap = fmaf(cx[n.xm], pc - pxm, ap);
+    ap = fmaf(cy[c], pc - pyp, ap);
+    ap = fmaf(cy[n.ym], pc - pym, ap);
+    ap = fmaf(cz[c], pc - pzp, ap);
+    ap = fmaf(cz[n.zm], pc - pzm, ap);
     pnew[c] = pc;
     Ap[c] = ap;
     pap += (double)pc * (double)ap;
@@ -481,13 +497,24 @@ __global__ void __launch_bounds__(256) k_ir_accum(VecArgs a, double* __restrict_
 }
 
 // x64 start: 0, or the stored pressure (warm start) on FLUID cells
-__global__ void __launch_bounds__(256) k_ir_start(VecArgs a, double* __restrict__ x64, const float* __restrict__ p,
-                                                  const double* __restrict__ cd, int warm) {
+// (x64 holds the previous step's pressure for a warm start, saved before the
+// per-solve zeroing; FLUID cells are the ones with a preconditioner entry)
+__global__ void __launch_bounds__(256) k_ir_start(VecArgs a, double* __restrict__ x64,
+                                                  const float* __restrict__ invd, int warm) {
   const int lane = threadIdx.x & 63;
   FOR_LIST_BLOCKS(a, b, lane) {
     long long c = b * kBlockVol + lane;
-    x64[c] = (warm && cd[c] != 0.0) ? (double)p[c] : 0.0;
+    x64[c] = (warm && invd[c] != 0.f) ? x64[c] : 0.0;
   }
+}
+
+// x64 = p over the previous solve's blocks (warm-start save)
+__global__ void __launch_bounds__(256) k_warm_save(const uint32_t* __restrict__ list, long long nlist,
+                                                   const float* __restrict__ p, double* __restrict__ x64) {
+  long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= nlist) return;
+  long long c = (long long)list[t] * kBlockVol + (threadIdx.x & 63);
+  x64[c] = (double)p[c];
 }
 
 // ------------------------------------------------------- pressure and faces
