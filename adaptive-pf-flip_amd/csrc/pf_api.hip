@@ -372,13 +372,16 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   // FP32 residual level that triggers a (checking) replacement: the target itself
   const double tol2_abs = cfg->stop_rule == PF_STOP_REL2 ? cfg->tol * cfg->tol * bnorm2 : cfg->tol * cfg->tol;
   static const bool trace = std::getenv("PF_SOLVE_TRACE") != nullptr;
+  // most FP32 iterations between two residual replacements
+  const char* env_between = std::getenv("PF_SOLVE_REPLACE_EVERY");
+  const int between = env_between ? std::max(8, std::atoi(env_between)) : 200;
   int total = 0, outer = 0;
   bool converged = met();
   if (!converged && max_iters > 0 && std::isfinite(rmax)) {
     hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32, c->pA,
                        c->invd32);
     hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
-                       tol2_abs, max_iters);
+                       tol2_abs, max_iters, between);
     LAUNCH_CHECK(c);
     int iter_before = 0;
     double best_rr = rr;
@@ -428,7 +431,8 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
                      breakdown ? " breakdown" : "");
       }
       if (rr < best_rr * 0.999) { best_rr = rr; no_gain = 0; } else { ++no_gain; }
-      if (converged || total >= max_iters || outer >= max_updates || !std::isfinite(rr) || no_gain >= 5) {
+      // (no gain over 40 replacements: the solve has stagnated)
+      if (converged || total >= max_iters || outer >= max_updates || !std::isfinite(rr) || no_gain >= 40) {
         if (breakdown) S.breakdown = 1;
         break;
       }
@@ -437,7 +441,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
                          breakdown ? c->pA : nullptr, c->invd32);
       if (breakdown)
         hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
-                           tol2_abs, max_iters - total);
+                           tol2_abs, max_iters - total, between);
       else
         hipLaunchKernelGGL(k_pcg_resume, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
       LAUNCH_CHECK(c);

@@ -358,14 +358,14 @@ __global__ void __launch_bounds__(256) k_pcg_load(VecArgs a, const double* __res
 
 // fresh start: scalars from the loaded residual
 __global__ void __launch_bounds__(256) k_pcg_start(const double* part, int n, SolveScalars* sc, double delta2,
-                                                   double tol2_abs, int max_iter) {
+                                                   double tol2_abs, int max_iter, int between) {
   double rz = block_sum_partials(part, n, false);
   double rr = block_sum_partials(part + kMaxPartials, n, false);
   if (threadIdx.x == 0) {
     sc->rz = rz; sc->rr = rr; sc->rr0 = rr; sc->rr_best = rr; sc->tol2 = delta2;
     sc->rr_ref = rr; sc->tol2_abs = tol2_abs;
     sc->beta = 0.0; sc->alpha = 0.0; sc->pAp = 0.0;
-    sc->iter = 0; sc->iter_best = 0; sc->max_iter = max_iter; sc->stall_window = 0;
+    sc->iter = 0; sc->iter_best = 0; sc->max_iter = max_iter; sc->stall_window = between;
     sc->breakdown = 0; sc->stalled = 0; sc->upd = 0;
     sc->done = (rr == 0.0 || max_iter <= 0) ? 1 : 0;
   }
@@ -378,6 +378,7 @@ __global__ void __launch_bounds__(256) k_pcg_resume(const double* part, int n, S
   if (threadIdx.x == 0) {
     sc->beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
     sc->rz = rz; sc->rr = rr; sc->rr_ref = rr;
+    sc->iter_best = sc->iter;  // iteration of the last replacement
     sc->upd = 0;
     sc->done = (rr == 0.0 || sc->iter >= sc->max_iter) ? 1 : 0;
   }
@@ -473,9 +474,13 @@ __global__ void __launch_bounds__(256) k_pcg_beta(const double* part, int n, Sol
       sc->done = 1;
       return;
     }
-    // residual replacement when the FP32 residual fell by delta, reached the
-    // target, or the iteration budget is spent; beta is formed after it
-    if (rr <= sc->tol2 * sc->rr_ref || rr <= sc->tol2_abs || it >= sc->max_iter) {
+    // residual replacement when the FP32 residual fell by delta below the largest
+    // residual since the last replacement, reached the target, ran `stall_window`
+    // iterations (the FP32 recursion drifts from b - A x with iteration count), or
+    // the iteration budget is spent; beta is formed after it
+    if (rr > sc->rr_ref) sc->rr_ref = rr;
+    if (rr <= sc->tol2 * sc->rr_ref || rr <= sc->tol2_abs || it >= sc->max_iter ||
+        it - sc->iter_best >= sc->stall_window) {
       sc->upd = 1;
       sc->done = 1;
       sc->rz_pending = rz;

@@ -142,8 +142,8 @@ def calibrate_rho_tilde0(phase: Phase, Vp: float = 1.0) -> float:
 
 
 def default_solve_cfg(stop_rule=STOP_REF_MAXABS, tol=1e-7, max_iters=600) -> SolveCfg:
-    return SolveCfg(max_iters=max_iters, stop_rule=stop_rule, tol=tol, inner_rel_tol=1e-4,
-                    max_outer=50, warm_start=0, check_every=8)
+    return SolveCfg(max_iters=max_iters, stop_rule=stop_rule, tol=tol, inner_rel_tol=0.1,
+                    max_outer=10000, warm_start=0, check_every=8)
 
 
 class Context:
@@ -304,7 +304,7 @@ class SparseSim3DTP:
         self.liquid_volume_target = 0.0
         self.cg_iters, self.cg_tol = 600, 1e-7
         self.stop_rule = STOP_REF_MAXABS
-        self.inner_rel_tol, self.warm_start, self.check_every = 1e-4, 0, 8
+        self.inner_rel_tol, self.warm_start, self.check_every = 0.1, 0, 8
         self.timing = False
         self.effective_dt_last = self.dt
         self.max_particle_speed_last = 0.0
@@ -324,7 +324,7 @@ class SparseSim3DTP:
 
     def step_cfg(self) -> StepCfg:
         solve = SolveCfg(max_iters=self.cg_iters, stop_rule=self.stop_rule, tol=self.cg_tol,
-                         inner_rel_tol=self.inner_rel_tol, max_outer=50,
+                         inner_rel_tol=self.inner_rel_tol, max_outer=10000,
                          warm_start=self.warm_start, check_every=self.check_every)
         return StepCfg(dt=self.dt, gravity=self.gravity, Vp=self.Vp,
                        alpha_liquid=self.alpha_liquid, alpha_gas=self.alpha_gas,

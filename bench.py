@@ -45,8 +45,8 @@ def _ensure_built():
 def step_cfg(G, n, tol, max_iters, warm):
     ph = G.Phase(1000.0, 1.0, 1.0, 1.0)
     ph.rho_tilde_0 = G.calibrate_rho_tilde0(ph, 1.0)
-    solve = G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=1e-4,
-                       max_outer=50, warm_start=warm, check_every=16)
+    solve = G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=0.1,
+                       max_outer=10000, warm_start=warm, check_every=16)
     return G.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95,
                      adaptive_cfl=0.5, adaptive_min_dt=1e-5, c_div=0.0, adaptive_timestep=0,
                      advection_order=2, timing=1, phase=ph, solve=solve)
