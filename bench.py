@@ -159,6 +159,11 @@ def main():
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": traffic, "bytes_per_launch": bytes_per_launch,
                         "avg_launch_ms": sp["avg_ms"]}
+    # achievable HBM bandwidth of a plain copy kernel on this box (reported beside
+    # the 8 TB/s peak; also the known-size kernel for calibrating HBM counters)
+    ncopy = min(64 << 20, (n ** 3) // 2)
+    copy_ms = min(ctx.copy_probe(ncopy) for _ in range(5))
+    copy_gbs = round(2 * 4 * ncopy / (copy_ms * 1e-3) / 1e9, 1)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (reference step on the 64^3 sample) ...", file=sys.stderr, flush=True)
@@ -189,6 +194,7 @@ def main():
         "true_rel_residual": [s.solve.true_rel_residual for s in stats],
         "kernels": kernels,
         "roofline": roofline,
+        "copy_kernel_gbs": copy_gbs,
         "cpu_baseline": cpu,
     }
     if cpu:
