@@ -298,7 +298,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   uint8_t* flag8 = reinterpret_cast<uint8_t*>(c->blk_flag32);
   hipLaunchKernelGGL(k_fluid_flags, dim3(grid_for(nb, 4)), dim3(256), 0, c->stream, c->dc, c->marker, flag8);
   LAUNCH_CHECK(c);
-  HIP_TRY(c, compact_flags(*c, flag8, nb, c->pblk, c->u32_dev + 2));
+  HIP_TRY(c, compact_flags_morton(*c, c->dc, flag8, c->pblk, c->u32_dev + 2));
   HIP_TRY(c, fill_u32(c->stream, c->u32_dev, 0u, 1));
   HIP_TRY(c, fill_u32(c->stream, c->u32_dev + 1, 0xffffffffu, 1));
   HIP_TRY(c, fill_u32(c->stream, c->u32_dev + 4, 0u, 1));
@@ -392,11 +392,9 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
         TIMED(c, PF_K_PCG_SPMV,
               hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(256), 0, c->stream, va, c->z32, c->pA, c->pB, c->Ap,
                                  c->cd32, c->cx32, c->cy32, c->cz32));
-        hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
         TIMED(c, PF_K_PCG_UPDATE,
               hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, c->stream, va, c->x32, c->r32, c->z32, c->pA,
                                  c->pB, c->Ap, c->invd32));
-        hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
       }
       LAUNCH_CHECK(c);
       HIP_TRY(c, hipMemcpyAsync(c->done_host, &c->sc->iter, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -591,13 +589,19 @@ int pf_kernel_time(pf_ctx* c, int32_t kernel, int64_t* launches, double* total_m
 }
 
 int pf_copy_probe(pf_ctx* c, int64_t n, double* ms) {
-  if (!c || n <= 0 || n > c->dc.nblocks * kBlockVol) return PF_ERR_ARG;
+  if (!c || n <= 0 || (n & 3) != 0 || n > (1LL << 31)) return PF_ERR_ARG;
+  // its own buffers (allocated here: a measurement hook, not a step stage)
+  float *src = nullptr, *dst = nullptr;
+  HIP_TRY(c, dalloc(&src, n));
+  HIP_TRY(c, dalloc(&dst, n));
   hipEvent_t a, b;
   HIP_TRY(c, hipEventCreate(&a));
   HIP_TRY(c, hipEventCreate(&b));
-  long long blocks = std::min<long long>((n + 255) / 256, 1 << 20);
+  const long long n4 = n / 4;
+  long long blocks = std::min<long long>((n4 + 255) / 256, 1 << 16);
   HIP_TRY(c, hipEventRecord(a, c->stream));
-  hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->x32, c->r32, (long long)n);
+  hipLaunchKernelGGL(k_copy_f32x4, dim3((unsigned)blocks), dim3(256), 0, c->stream,
+                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
   LAUNCH_CHECK(c);
   HIP_TRY(c, hipEventRecord(b, c->stream));
   HIP_TRY(c, hipEventSynchronize(b));
@@ -605,6 +609,8 @@ int pf_copy_probe(pf_ctx* c, int64_t n, double* ms) {
   HIP_TRY(c, hipEventElapsedTime(&t, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  HIP_TRY(c, hipFree(src));
+  HIP_TRY(c, hipFree(dst));
   if (ms) *ms = t;
   return PF_OK;
 }
@@ -645,7 +651,9 @@ int pf_create(pf_ctx** out, const pf_grid_desc* d) {
   HIP_TRY(c, dalloc(&c->pblk, maxblk));
   HIP_TRY(c, dalloc(&c->pblk_prev, maxblk));
   HIP_TRY(c, dalloc(&c->blk_flag32, maxblk));
-  HIP_TRY(c, dalloc(&c->blk_scan, maxblk + 1));
+  c->morton_cap = morton_codes(c->dc) + 1;
+  HIP_TRY(c, dalloc(&c->blk_scan, std::max(maxblk + 1, c->morton_cap)));
+  HIP_TRY(c, dalloc(&c->morton_flags, c->morton_cap));
   float** f32[] = {&c->x32, &c->r32, &c->z32, &c->pA, &c->pB, &c->Ap, &c->cd32, &c->cx32, &c->cy32, &c->cz32, &c->invd32};
   for (float** p : f32) HIP_TRY(c, dalloc(p, ncell));
   double** f64[] = {&c->b64, &c->x64, &c->r64, &c->cd64, &c->cx64, &c->cy64, &c->cz64};
@@ -677,7 +685,7 @@ int pf_destroy(pf_ctx* c) {
   for (int f = 0; f < F_N; ++f) { hipFree(c->f[f]); hipFree(c->act[f]); }
   hipFree(c->marker); hipFree(c->cell_count); hipFree(c->cell_start);
   hipFree(c->color_list); hipFree(c->color_cnt); hipFree(c->pblk); hipFree(c->pblk_prev);
-  hipFree(c->blk_flag32); hipFree(c->blk_scan);
+  hipFree(c->blk_flag32); hipFree(c->blk_scan); hipFree(c->morton_flags);
   PBuf* bufs[2] = {&c->pa, &c->pb};
   for (PBuf* b : bufs) {
     hipFree(b->x); hipFree(b->y); hipFree(b->z); hipFree(b->u); hipFree(b->v); hipFree(b->w);

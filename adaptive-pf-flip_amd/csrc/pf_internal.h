@@ -90,6 +90,8 @@ struct SolveScalars {
   double rr_ref, tol2_abs, rz_pending, pad0;
   int iter, done, breakdown, stalled;
   int iter_best, max_iter, stall_window, upd;
+  unsigned ticket;  // last-workgroup counter (reset by the last workgroup)
+  int pad1, pad2, pad3;
 };
 
 struct Ctx {
@@ -114,7 +116,9 @@ struct Ctx {
   uint32_t* pblk_prev = nullptr;    // previous projection's list (zeroed on reuse)
   long long npblk = 0, npblk_prev = 0;
   uint32_t* blk_flag32 = nullptr;   // scratch: per-block u32 flags for compaction
-  uint32_t* blk_scan = nullptr;     // scratch: scan of blk_flag32
+  uint32_t* blk_scan = nullptr;     // scratch: scan of block flags (>= max(blocks, Morton codes) + 1)
+  uint8_t* morton_flags = nullptr;  // scratch: block flags in Morton order
+  long long morton_cap = 0;
 
   // particles
   long long np = 0, pcap = 0;
@@ -169,6 +173,11 @@ hipError_t scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, long long n);
 // list = sorted ids b with flag[b] != 0; count written to *count_dev (device)
 hipError_t compact_flags(Ctx& c, const uint8_t* flags, long long n, uint32_t* list,
                          uint32_t* count_dev);
+// the same set of flagged blocks (flags indexed by bid), listed in Morton order
+hipError_t compact_flags_morton(Ctx& c, const Dims& d, const uint8_t* flags, uint32_t* list,
+                                uint32_t* count_dev);
+// size of the Morton code space covering d's block grid
+long long morton_codes(const Dims& d);
 
 }  // namespace pf
 

@@ -102,6 +102,41 @@ __device__ inline void block_reduce_store(double (&v)[NV], const bool (&is_max)[
   }
 }
 
+// True in exactly one workgroup: the last to finish storing its partial sums.
+// Release: every workgroup fences its partial before taking a ticket; acquire: the
+// last one fences again and reads the partials with device-coherent loads.  The
+// ticket is reset for the next kernel of the stream.
+__device__ inline bool last_block(unsigned* ticket) {
+  __shared__ bool s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned t = atomicAdd(ticket, 1u);
+    s_last = (t == gridDim.x - 1);
+    if (s_last) atomicExch(ticket, 0u);
+  }
+  __syncthreads();
+  if (s_last) __threadfence();
+  return s_last;
+}
+
+// fixed-order sum of part[0..n) read coherently (partials of other workgroups)
+__device__ inline double block_sum_coherent(const double* part, int n) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < n; t += 256)
+    acc += __hip_atomic_load(&part[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = s[0];
+  __syncthreads();
+  return r;
+}
+
 // one workgroup sums part[slot*stride + 0..n) in a fixed order
 __device__ inline double block_sum_partials(const double* part, int n, bool is_max) {
   __shared__ double s[256];
@@ -419,24 +454,51 @@ __global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __rest
   double v[1] = {pap};
   const bool mx[1] = {false};
   block_reduce_store<1>(v, mx, a.part, kMaxPartials);
-}
-
-__global__ void __launch_bounds__(256) k_pcg_alpha(const double* part, int n, SolveScalars* sc) {
-  if (sc->done) return;
-  double pap = block_sum_partials(part, n, false);
-  if (threadIdx.x == 0) {
-    sc->pAp = pap;
-    if (!(pap > 0.0) || !isfinite(pap)) {
-      sc->breakdown = 1;
-      sc->done = 1;
-      sc->alpha = 0.0;
-    } else {
-      sc->alpha = sc->rz / pap;
+  // the last workgroup forms alpha = r.z / p.Ap (no separate launch)
+  if (last_block(&a.sc->ticket)) {
+    const double s_pap = block_sum_coherent(a.part, gridDim.x);
+    if (threadIdx.x == 0) {
+      SolveScalars* sc = a.sc;
+      sc->pAp = s_pap;
+      if (!(s_pap > 0.0) || !isfinite(s_pap)) {
+        sc->breakdown = 1;
+        sc->done = 1;
+        sc->alpha = 0.0;
+      } else {
+        sc->alpha = sc->rz / s_pap;
+      }
     }
   }
 }
 
-// x += alpha p, r -= alpha Ap, z = D^-1 r; partial r.z and r.r
+// iteration end (last workgroup of the update): count, decide on a residual
+// replacement (pause) or continue with beta = r.z_new / r.z
+__device__ inline void finish_iteration(SolveScalars* sc, double rz, double rr) {
+  int it = sc->iter + 1;
+  sc->iter = it;
+  sc->rr = rr;
+  if (!isfinite(rr) || !isfinite(rz)) {
+    sc->breakdown = 1;
+    sc->done = 1;
+    return;
+  }
+  // residual replacement when the FP32 residual fell by delta below the largest
+  // residual since the last replacement, reached the target, ran `stall_window`
+  // iterations (the FP32 recursion drifts from b - A x with iteration count), or
+  // the iteration budget is spent; beta is formed after it
+  if (rr > sc->rr_ref) sc->rr_ref = rr;
+  if (rr <= sc->tol2 * sc->rr_ref || rr <= sc->tol2_abs || it >= sc->max_iter ||
+      it - sc->iter_best >= sc->stall_window) {
+    sc->upd = 1;
+    sc->done = 1;
+    sc->rz_pending = rz;
+    return;
+  }
+  sc->beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
+  sc->rz = rz;
+}
+
+// x += alpha p, r -= alpha Ap, z = D^-1 r; r.z and r.r, finished by the last workgroup
 __global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict__ x, float* __restrict__ r,
                                                     float* __restrict__ z, const float* __restrict__ pA,
                                                     const float* __restrict__ pB, const float* __restrict__ Ap,
@@ -458,36 +520,10 @@ __global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict
   double v[2] = {rz, rr};
   const bool mx[2] = {false, false};
   block_reduce_store<2>(v, mx, a.part, kMaxPartials);
-}
-
-// iteration end: count, decide on a residual replacement (pause) or continue
-__global__ void __launch_bounds__(256) k_pcg_beta(const double* part, int n, SolveScalars* sc) {
-  if (sc->done) return;
-  double rz = block_sum_partials(part, n, false);
-  double rr = block_sum_partials(part + kMaxPartials, n, false);
-  if (threadIdx.x == 0) {
-    int it = sc->iter + 1;
-    sc->iter = it;
-    sc->rr = rr;
-    if (!isfinite(rr) || !isfinite(rz)) {
-      sc->breakdown = 1;
-      sc->done = 1;
-      return;
-    }
-    // residual replacement when the FP32 residual fell by delta below the largest
-    // residual since the last replacement, reached the target, ran `stall_window`
-    // iterations (the FP32 recursion drifts from b - A x with iteration count), or
-    // the iteration budget is spent; beta is formed after it
-    if (rr > sc->rr_ref) sc->rr_ref = rr;
-    if (rr <= sc->tol2 * sc->rr_ref || rr <= sc->tol2_abs || it >= sc->max_iter ||
-        it - sc->iter_best >= sc->stall_window) {
-      sc->upd = 1;
-      sc->done = 1;
-      sc->rz_pending = rz;
-      return;
-    }
-    sc->beta = sc->rz != 0.0 ? rz / sc->rz : 0.0;
-    sc->rz = rz;
+  if (last_block(&a.sc->ticket)) {
+    const double s_rz = block_sum_coherent(a.part, gridDim.x);
+    const double s_rr = block_sum_coherent(a.part + kMaxPartials, gridDim.x);
+    if (threadIdx.x == 0) finish_iteration(a.sc, s_rz, s_rr);
   }
 }
 

@@ -161,8 +161,8 @@ def main():
                         "avg_launch_ms": sp["avg_ms"]}
     # achievable HBM bandwidth of a plain copy kernel on this box (reported beside
     # the 8 TB/s peak; also the known-size kernel for calibrating HBM counters)
-    ncopy = min(64 << 20, (n ** 3) // 2)
-    copy_ms = min(ctx.copy_probe(ncopy) for _ in range(5))
+    ncopy = 256 << 20  # 1 GiB read + 1 GiB written: past every cache
+    copy_ms = min(ctx.copy_probe(ncopy) for _ in range(3))
     copy_gbs = round(2 * 4 * ncopy / (copy_ms * 1e-3) / 1e9, 1)
     cpu = None
     if not args.no_cpu_baseline:
