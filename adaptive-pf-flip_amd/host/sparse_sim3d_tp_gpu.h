@@ -1,0 +1,124 @@
+// sparse_sim3d_tp_gpu.h -- C++17 drop-in for the reference's SparseSim3DTP whose step
+// runs on one MI355X through the C ABI in include/pf_gpu.h.
+//
+// Mirrors src/driver/sparse_sim3d_tp.h:8-177 (the fields step() reads and the
+// per-step stats it writes, under the same names), Particles3DTP
+// (src/particles/particles3d_tp.h:4-35), PhaseParams (src/physics/phasefield.h:4-9)
+// and the scene initialisers initRayleighTaylor / initBubbleTank /
+// initTwoPhaseDamBreak (src/driver/sparse_sim3d_tp.cpp:401-414, 739-776).
+//
+// Particles live on the device between steps.  The host `particles` member is the
+// caller's view: edits to it are uploaded by the next step() after
+// markParticlesDirty(); syncParticlesToHost() brings the device state back (caller
+// order preserved).  Errors from the ABI are thrown as std::runtime_error, as the
+// reference throws from its multires solver (multires_pressure3d.cpp:307-333).
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pf_gpu.h"
+
+namespace pfgpu {
+
+struct Vec3 {
+  double x = 0.0, y = 0.0, z = 0.0;
+};
+
+struct Particles3DTP {
+  std::vector<Vec3> pos, vel;
+  std::vector<unsigned char> type;  // 0 = liquid, 1 = gas
+  std::vector<double> volume;       // volume multiplier (1.0 = legacy)
+  size_t size() const { return pos.size(); }
+  void add(const Vec3& p, const Vec3& v, unsigned char t, double vol = 1.0) {
+    pos.push_back(p);
+    vel.push_back(v);
+    type.push_back(t);
+    volume.push_back(vol);
+  }
+};
+
+struct PhaseParams {
+  double rho_l = 1.0;
+  double rho_g = 0.01;
+  double alpha_phi = 1.0;
+  double rho_tilde_0 = 1.0;
+};
+
+class SparseSim3DTP {
+ public:
+  Particles3DTP particles;
+  PhaseParams phase;
+  double dt = 0.02, gravity = -9.81, Vp = 1.0, alpha_liquid = 0.95, alpha_gas = 0.95;
+  bool adaptive_timestep = false;
+  double adaptive_cfl = 0.5;
+  double adaptive_min_dt = 1e-5;
+  double effective_dt_last = 0.02;
+  double max_particle_speed_last = 0.0;
+  double cfl_limit_dt_last = 0.02;
+  int adaptive_timestep_limited_last = 0;
+  int advection_order = 2;
+  bool c_div_volume_correction = false;
+  double c_div_strength = 1.0;
+  double liquid_volume_target = 0.0;
+  double liquid_volume_current_last = 0.0;
+  double liquid_volume_error_last = 0.0;
+  double c_div_last = 0.0;
+  int cg_iters = 600;
+  double cg_tol = 1e-7;
+  int particle_boundary_clamped_liquid_last = 0;
+  int particle_boundary_clamped_gas_last = 0;
+  int particle_boundary_clamped_liquid_total = 0;
+  int particle_boundary_clamped_gas_total = 0;
+  int particle_boundary_clamped_x_lo_last = 0;
+  int particle_boundary_clamped_x_hi_last = 0;
+  int particle_boundary_clamped_y_lo_last = 0;
+  int particle_boundary_clamped_y_hi_last = 0;
+  int particle_boundary_clamped_z_lo_last = 0;
+  int particle_boundary_clamped_z_hi_last = 0;
+
+  // GPU solver controls (no reference counterpart): the reference's rule is
+  // |r|_inf < cg_tol; PF_STOP_REL2 stops on ||b - Ax||_2/||b||_2 <= cg_tol in FP64
+  int32_t stop_rule = PF_STOP_REF_MAXABS;
+  int32_t warm_start = 0;
+  bool stage_timing = false;
+  pf_step_stats stats_last{};
+
+  SparseSim3DTP(int nx, int ny, int nz, double dx);
+  ~SparseSim3DTP();
+  SparseSim3DTP(const SparseSim3DTP&) = delete;
+  SparseSim3DTP& operator=(const SparseSim3DTP&) = delete;
+
+  void initTwoPhaseDamBreak();
+  void initRayleighTaylor();
+  void initBubbleTank();
+  void step();
+
+  // device <-> host particle state
+  void markParticlesDirty() { dirty_ = true; }
+  void syncParticlesToHost();
+  // SparseMacGrid3D::activeCellBlocks / totalCellBlocks (sparse_mac_grid3d.h:45-46)
+  size_t activeCellBlocks();
+  size_t totalCellBlocks() const;
+  int nx() const { return nx_; }
+  int ny() const { return ny_; }
+  int nz() const { return nz_; }
+  double dx() const { return dx_; }
+  pf_ctx* context() { return ctx_; }
+
+ private:
+  void check(int status, const char* what);
+  void upload();
+  void resetSceneState();
+  void seedCell(int i, int j, int k, unsigned char type);
+  double liquidVolume() const;
+
+  pf_ctx* ctx_ = nullptr;
+  int nx_, ny_, nz_;
+  double dx_;
+  bool dirty_ = true;
+};
+
+}  // namespace pfgpu

@@ -153,3 +153,17 @@ def test_rejects_particles_outside_the_grid():
         c.upload([8.5], [1.0], [1.0])
     with pytest.raises(G.PfError):
         c.set_field(G.FIELD_MARKER, np.ones((8, 8, 8), np.float32))  # FLUID on the boundary
+
+
+def test_cpp_drop_in_class_runs_the_reference_integration_tests():
+    # adaptive-pf-flip_amd/host/sim_demo.cpp: the reference's SparseSim3DTP tests
+    # (tests/test_sparse_sim3d_tp.cpp:110-169, 749-786) against the C++ drop-in class
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "adaptive-pf-flip_amd", "host", "sim_demo")
+    assert os.path.exists(exe), "build the demo with adaptive-pf-flip_amd/build.py"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    assert "all checks passed" in out.stdout
