@@ -90,8 +90,6 @@ struct SolveScalars {
   double rr_ref, tol2_abs, rz_pending, pad0;
   int iter, done, breakdown, stalled;
   int iter_best, max_iter, stall_window, upd;
-  unsigned ticket;  // last-workgroup counter (reset by the last workgroup)
-  int pad1, pad2, pad3;
 };
 
 struct Ctx {

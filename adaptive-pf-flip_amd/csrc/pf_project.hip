@@ -102,37 +102,16 @@ __device__ inline void block_reduce_store(double (&v)[NV], const bool (&is_max)[
   }
 }
 
-// True in exactly one workgroup: the last to finish storing its partial sums.
-// Release: every workgroup fences its partial before taking a ticket; acquire: the
-// last one fences again and reads the partials with device-coherent loads.  The
-// ticket is reset for the next kernel of the stream.
-__device__ inline bool last_block(unsigned* ticket) {
-  __shared__ bool s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(ticket, 1u);
-    s_last = (t == gridDim.x - 1);
-    if (s_last) atomicExch(ticket, 0u);
-  }
-  __syncthreads();
-  if (s_last) __threadfence();
-  return s_last;
-}
-
-// fixed-order sum of part[0..n) read coherently (partials of other workgroups)
-__device__ inline double block_sum_coherent(const double* part, int n) {
-  __shared__ double s[256];
+// fixed-order sum of part[0..n) by one 256-thread workgroup: strided per-thread
+// sums, then a wave butterfly and a 4-way combine (n <= 1024 in the solver loop)
+__device__ inline double block_sum_fast(const double* part, int n) {
+  __shared__ double s_w[4];
   double acc = 0.0;
-  for (int t = threadIdx.x; t < n; t += 256)
-    acc += __hip_atomic_load(&part[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  s[threadIdx.x] = acc;
+  for (int t = threadIdx.x; t < n; t += 256) acc += part[t];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  double r = s[0];
+  const double r = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
   __syncthreads();
   return r;
 }
@@ -454,25 +433,26 @@ __global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __rest
   double v[1] = {pap};
   const bool mx[1] = {false};
   block_reduce_store<1>(v, mx, a.part, kMaxPartials);
-  // the last workgroup forms alpha = r.z / p.Ap (no separate launch)
-  if (last_block(&a.sc->ticket)) {
-    const double s_pap = block_sum_coherent(a.part, gridDim.x);
-    if (threadIdx.x == 0) {
-      SolveScalars* sc = a.sc;
-      sc->pAp = s_pap;
-      if (!(s_pap > 0.0) || !isfinite(s_pap)) {
-        sc->breakdown = 1;
-        sc->done = 1;
-        sc->alpha = 0.0;
-      } else {
-        sc->alpha = sc->rz / s_pap;
-      }
+}
+
+// alpha = r.z / p.Ap from the SpMV partials (one workgroup)
+__global__ void __launch_bounds__(256) k_pcg_alpha(const double* part, int n, SolveScalars* sc) {
+  if (sc->done) return;
+  const double pap = block_sum_fast(part, n);
+  if (threadIdx.x == 0) {
+    sc->pAp = pap;
+    if (!(pap > 0.0) || !isfinite(pap)) {
+      sc->breakdown = 1;
+      sc->done = 1;
+      sc->alpha = 0.0;
+    } else {
+      sc->alpha = sc->rz / pap;
     }
   }
 }
 
-// iteration end (last workgroup of the update): count, decide on a residual
-// replacement (pause) or continue with beta = r.z_new / r.z
+// iteration end: count, decide on a residual replacement (pause) or continue with
+// beta = r.z_new / r.z
 __device__ inline void finish_iteration(SolveScalars* sc, double rz, double rr) {
   int it = sc->iter + 1;
   sc->iter = it;
@@ -498,7 +478,7 @@ __device__ inline void finish_iteration(SolveScalars* sc, double rz, double rr) 
   sc->rz = rz;
 }
 
-// x += alpha p, r -= alpha Ap, z = D^-1 r; r.z and r.r, finished by the last workgroup
+// x += alpha p, r -= alpha Ap, z = D^-1 r; partial r.z and r.r
 __global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict__ x, float* __restrict__ r,
                                                     float* __restrict__ z, const float* __restrict__ pA,
                                                     const float* __restrict__ pB, const float* __restrict__ Ap,
@@ -520,11 +500,13 @@ __global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict
   double v[2] = {rz, rr};
   const bool mx[2] = {false, false};
   block_reduce_store<2>(v, mx, a.part, kMaxPartials);
-  if (last_block(&a.sc->ticket)) {
-    const double s_rz = block_sum_coherent(a.part, gridDim.x);
-    const double s_rr = block_sum_coherent(a.part + kMaxPartials, gridDim.x);
-    if (threadIdx.x == 0) finish_iteration(a.sc, s_rz, s_rr);
-  }
+}
+
+__global__ void __launch_bounds__(256) k_pcg_beta(const double* part, int n, SolveScalars* sc) {
+  if (sc->done) return;
+  const double rz = block_sum_fast(part, n);
+  const double rr = block_sum_fast(part + kMaxPartials, n);
+  if (threadIdx.x == 0) finish_iteration(sc, rz, rr);
 }
 
 // x64 += x32 (x32 is reset by the next k_pcg_load)
