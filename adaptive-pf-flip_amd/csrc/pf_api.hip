@@ -82,6 +82,16 @@ unsigned grid_for(long long work, long long per_block) {
   return (unsigned)g;
 }
 
+// workgroups for a particle loop split into 8 contiguous XCD shares: a multiple
+// of 8, at most cap
+unsigned xcd_grid(long long work, long long per_block, unsigned cap) {
+  long long g = (work + per_block - 1) / per_block;
+  g = ((g + 7) / 8) * 8;
+  if (g < 8) g = 8;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
 // workgroups for the list kernels: a multiple of 8 (one share per XCD), at most
 // kMaxPartials (one partial sum each)
 unsigned list_grid(long long nlist) {
@@ -324,7 +334,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   pa.scale = dt / (g.dx * g.dx);
   pa.cdiv = cdiv;
   pa.pin = c->u32_dev;
-  hipLaunchKernelGGL(k_pin_detect, dim3(grid_for(npb, 4)), dim3(256), 0, c->stream, pa);
+  hipLaunchKernelGGL(k_pin_detect, dim3(list_grid(npb)), dim3(256), 0, c->stream, pa);
   LAUNCH_CHECK(c);
   AsmOut ao;
   ao.b64 = c->b64; ao.cd64 = c->cd64; ao.cx64 = c->cx64; ao.cy64 = c->cy64; ao.cz64 = c->cz64;
@@ -510,7 +520,7 @@ int run_g2p_advect(pf_ctx* c, bool g2p, bool adv, double aL, double aG, double d
   a.esc = c->esc_dev;
   a.vmax_bits = reinterpret_cast<uint32_t*>(c->vmax_dev);
   TIMED(c, PF_K_G2P_ADVECT,
-        hipLaunchKernelGGL(k_g2p_advect, dim3(std::min(grid_for(c->np, 256), 8192u)), dim3(256), 0, c->stream, a));
+        hipLaunchKernelGGL(k_g2p_advect, dim3(xcd_grid(c->np, 256, 8192)), dim3(256), 0, c->stream, a));
   LAUNCH_CHECK(c);
   if (adv) c->binned = false;
   return PF_OK;

@@ -89,8 +89,14 @@ __device__ inline void block_flush_counters(int (&cnt)[8], float speed, int32_t*
 __global__ void __launch_bounds__(256) k_g2p_advect(G2PArgs a) {
   int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   float speed = 0.f;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += stride) {
+  // Workgroups are dealt round-robin to the 8 XCDs; workgroup w (XCD w % 8) walks
+  // a contiguous eighth of the cell-sorted particles, so each XCD's L2 holds the
+  // grid around one region instead of every XCD reading the whole grid.
+  const long long chunk = (a.n + 7) / 8;
+  const long long p_end = min(a.n, (long long)((blockIdx.x & 7) + 1) * chunk);
+  const long long stride = (long long)(gridDim.x >> 3) * blockDim.x;
+  for (long long p = (long long)(blockIdx.x & 7) * chunk + (long long)(blockIdx.x >> 3) * blockDim.x + threadIdx.x;
+       p < p_end; p += stride) {
     float x = a.P.x[p], y = a.P.y[p], z = a.P.z[p];
     uint8_t ty = a.P.type[p];
     float k1[3];

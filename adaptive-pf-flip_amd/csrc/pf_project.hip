@@ -173,36 +173,51 @@ __device__ inline bool list_block(const ProjArgs& a, long long t, long long& b) 
   return true;
 }
 
-// Pin detection, sparse_ops3d_tp.cpp:250-264
+// Pin detection, sparse_ops3d_tp.cpp:250-264: does any FLUID cell have an AIR
+// 6-neighbour, the smallest FLUID linear id, and the FLUID cell count.  Grid-stride
+// over the list (XCD-aware), combined per workgroup so each workgroup issues one
+// atomic per quantity.
 __global__ void __launch_bounds__(256) k_pin_detect(ProjArgs a) {
-  long long b;
+  __shared__ uint32_t s_air[4], s_min[4], s_cnt[4];
   const int lane = threadIdx.x & 63;
-  long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (!list_block(a, t, b)) return;
-  int bx, by, bz;
-  block_coords(a.dc, b, bx, by, bz);
-  int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
-  bool fl = a.marker[b * kBlockVol + lane] == CELL_FLUID;
-  bool air = false;
-  uint32_t lin = 0xffffffffu;
-  if (fl) {
-    air = marker_at(a.marker, a.dc, a.g, i + 1, j, k) == CELL_AIR ||
-          marker_at(a.marker, a.dc, a.g, i - 1, j, k) == CELL_AIR ||
-          marker_at(a.marker, a.dc, a.g, i, j + 1, k) == CELL_AIR ||
-          marker_at(a.marker, a.dc, a.g, i, j - 1, k) == CELL_AIR ||
-          marker_at(a.marker, a.dc, a.g, i, j, k + 1) == CELL_AIR ||
-          marker_at(a.marker, a.dc, a.g, i, j, k - 1) == CELL_AIR;
-    lin = (uint32_t)(i + (long long)a.g.nx * (j + (long long)a.g.ny * k));
+  bool anyair = false;
+  uint32_t m = 0xffffffffu, nfl = 0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    int bx, by, bz;
+    block_coords(a.dc, b, bx, by, bz);
+    int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
+    bool fl = a.marker[b * kBlockVol + lane] == CELL_FLUID;
+    bool air = false;
+    uint32_t lin = 0xffffffffu;
+    if (fl) {
+      air = marker_at(a.marker, a.dc, a.g, i + 1, j, k) == CELL_AIR ||
+            marker_at(a.marker, a.dc, a.g, i - 1, j, k) == CELL_AIR ||
+            marker_at(a.marker, a.dc, a.g, i, j + 1, k) == CELL_AIR ||
+            marker_at(a.marker, a.dc, a.g, i, j - 1, k) == CELL_AIR ||
+            marker_at(a.marker, a.dc, a.g, i, j, k + 1) == CELL_AIR ||
+            marker_at(a.marker, a.dc, a.g, i, j, k - 1) == CELL_AIR;
+      lin = (uint32_t)(i + (long long)a.g.nx * (j + (long long)a.g.ny * k));
+    }
+    anyair = anyair || __any(air);
+    nfl += (uint32_t)__popcll(__ballot(fl));
+    m = min(m, lin);
   }
-  bool anyair = __any(air);
-  int nfl = __popcll(__ballot(fl));
-  uint32_t m = lin;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  const int w = threadIdx.x >> 6;
   if (lane == 0) {
-    if (anyair) atomicOr(&a.pin[0], 1u);
-    if (m != 0xffffffffu) atomicMin(&a.pin[1], m);
-    if (nfl) atomicAdd(&a.pin[4], (uint32_t)nfl);
+    s_air[w] = anyair ? 1u : 0u;
+    s_min[w] = m;
+    s_cnt[w] = nfl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t air = s_air[0] | s_air[1] | s_air[2] | s_air[3];
+    const uint32_t mn = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+    const uint32_t cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    if (air) atomicOr(&a.pin[0], 1u);
+    if (mn != 0xffffffffu) atomicMin(&a.pin[1], mn);
+    if (cnt) atomicAdd(&a.pin[4], cnt);
   }
 }
 

@@ -31,6 +31,14 @@ HBM_PEAK_GBS = 8000.0
 SPMV_BYTES_PER_CELL = 32  # z, p_old, diag, 3 couplings read; p_new, Ap written (FP32)
 
 
+def read_traffic(path):
+    """HBM bytes per launch from a counter summary line 'hbm_bytes_per_launch: N'."""
+    for line in open(path):
+        if line.strip().startswith("hbm_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1].strip().strip("`")))
+    return None
+
+
 def _ensure_built():
     import pfflip_amd as G
     if not os.path.exists(G.LIB_PATH):
@@ -129,8 +137,9 @@ def main():
         ctx.kernel_timing(True)
     stats = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         stats.append(ctx.step(cfg))
+        print(f"[bench] step {i + 1}/{args.steps} issued", file=sys.stderr, flush=True)
     ctx.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
@@ -152,15 +161,16 @@ def main():
             bytes_per_launch = SPMV_BYTES_PER_CELL * fluid
             achieved = bytes_per_launch / (sp["avg_ms"] * 1e-3) / 1e9
             traffic = None
-            tf = os.path.join(ROOT, "profiles", "pcg_spmv_rt256_traffic.json")
+            tf = os.path.join(ROOT, "profiles", "pcg_spmv_rt256_traffic.md")
             if os.path.exists(tf) and n == 256:
-                traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+                traffic = read_traffic(tf)
             roofline = {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": traffic, "bytes_per_launch": bytes_per_launch,
                         "avg_launch_ms": sp["avg_ms"]}
     # achievable HBM bandwidth of a plain copy kernel on this box (reported beside
     # the 8 TB/s peak; also the known-size kernel for calibrating HBM counters)
+    print("[bench] copy probe", file=sys.stderr, flush=True)
     ncopy = 256 << 20  # 1 GiB read + 1 GiB written: past every cache
     copy_ms = min(ctx.copy_probe(ncopy) for _ in range(3))
     copy_gbs = round(2 * 4 * ncopy / (copy_ms * 1e-3) / 1e9, 1)
