@@ -193,8 +193,9 @@ void* pf_stream(pf_ctx* ctx);
 int pf_kernel_timing(pf_ctx* ctx, int32_t enable);
 /* launches and summed event time of one kernel since timing was enabled (synchronises) */
 int pf_kernel_time(pf_ctx* ctx, int32_t kernel, int64_t* launches, double* total_ms);
-/* plain device copy of n floats between two scratch buffers (n <= cell count), timed
- * with events; a known-size kernel for calibrating HBM traffic counters */
+/* plain device copy of n floats (n a multiple of 4) between two buffers allocated
+ * for the call, best of four event-timed passes; a known-size kernel for
+ * calibrating HBM traffic counters */
 int pf_copy_probe(pf_ctx* ctx, int64_t n, double* ms);
 
 #ifdef __cplusplus
