@@ -226,7 +226,8 @@ int run_face_ops(pf_ctx* c, int ops, float dtg) {
   return PF_OK;
 }
 
-int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp) {
+// single: the single-phase trilinear unit-mass splat (ph and Vp unused)
+int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false) {
   int st = ensure_binned(c);
   if (st) return st;
   const Dims* d[3] = {&c->du, &c->dv, &c->dw};
@@ -251,15 +252,17 @@ int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp) {
   a.has_vol = c->has_vol;
   a.pow2 = is_pow2(c->geom.dx);
   a.inv_dx = 1.0 / c->geom.dx;
-  a.mp_l = (float)(ph->rho_l * Vp);
-  a.mp_g = (float)(ph->rho_g * Vp);
+  a.mp_l = single ? 1.f : (float)(ph->rho_l * Vp);
+  a.mp_g = single ? 1.f : (float)(ph->rho_g * Vp);
   if (c->np == 0) return PF_OK;
   for (int col = 0; col < 8; ++col) {
     a.list = c->color_list + (long long)col * c->color_cap;
     a.count = c->color_cnt + col;
-    TIMED(c, PF_K_P2G,
-          hipLaunchKernelGGL(k_p2g_color, dim3((unsigned)std::max<long long>(c->color_cap, 1)), dim3(64), 0,
-                             c->stream, a));
+    const dim3 grid((unsigned)std::max<long long>(c->color_cap, 1));
+    if (single)
+      TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<1>, grid, dim3(64), 0, c->stream, a));
+    else
+      TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<0>, grid, dim3(64), 0, c->stream, a));
     LAUNCH_CHECK(c);
   }
   return PF_OK;
@@ -276,8 +279,10 @@ void record(pf_ctx* c, bool timing, int slot) {
   if (timing) hipEventRecord(c->ev[slot], c->stream);
 }
 
+// single: the single-phase unit-density projection (ph unused, cdiv ignored)
 int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cfg, double cdiv,
-                pf_solve_stats* out, bool timing, int ev_assemble, int ev_solve, int ev_vel) {
+                pf_solve_stats* out, bool timing, int ev_assemble, int ev_solve, int ev_vel,
+                bool single = false) {
   pf_solve_stats S;
   std::memset(&S, 0, sizeof(S));
   S.pinned_cell = -1;
@@ -330,10 +335,12 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   ProjArgs pa;
   pa.g = g; pa.dc = c->dc; pa.du = c->du; pa.dv = c->dv; pa.dw = c->dw;
   pa.marker = c->marker; pa.list = c->pblk; pa.nlist = npb;
-  pa.ph = phase_k(ph);
+  const pf_phase unit_phase = {1.0, 1.0, 1.0, 1.0};
+  pa.ph = phase_k(single ? &unit_phase : ph);
   pa.scale = dt / (g.dx * g.dx);
-  pa.cdiv = cdiv;
+  pa.cdiv = single ? 0.0 : cdiv;
   pa.pin = c->u32_dev;
+  pa.single = single ? 1 : 0;
   hipLaunchKernelGGL(k_pin_detect, dim3(list_grid(npb)), dim3(256), 0, c->stream, pa);
   LAUNCH_CHECK(c);
   AsmOut ao;
@@ -487,6 +494,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   vl.s = dt / g.dx;
   vl.list = c->pblk;
   vl.nlist = npb;
+  vl.unit_beta = single ? 1 : 0;
   TIMED(c, PF_K_VELOCITY,
         hipLaunchKernelGGL(k_velocity_update, dim3(grid_for(npb, 4)), dim3(256), 0, c->stream, vl));
   LAUNCH_CHECK(c);
@@ -840,6 +848,13 @@ int pf_p2g(pf_ctx* c, const pf_phase* ph, double Vp) {
   return run_face_ops(c, FO_NORMALIZE, 0.f);
 }
 
+int pf_p2g_single(pf_ctx* c) {
+  if (!c) return PF_ERR_ARG;
+  int st = run_p2g_splat(c, nullptr, 1.0, true);
+  if (st) return st;
+  return run_face_ops(c, FO_NORMALIZE, 0.f);
+}
+
 int pf_save_velocity(pf_ctx* c) {
   if (!c) return PF_ERR_ARG;
   return run_face_ops(c, FO_SAVE, 0.f);
@@ -859,6 +874,11 @@ int pf_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cfg
                pf_solve_stats* stats) {
   if (!c || !ph || !cfg) return PF_ERR_ARG;
   return run_project(c, ph, dt, cfg, div_correction, stats, false, 0, 0, 0);
+}
+
+int pf_project_single(pf_ctx* c, double dt, const pf_solve_cfg* cfg, pf_solve_stats* stats) {
+  if (!c || !cfg) return PF_ERR_ARG;
+  return run_project(c, nullptr, dt, cfg, 0.0, stats, false, 0, 0, 0, true);
 }
 
 int pf_g2p(pf_ctx* c, double aL, double aG) {
@@ -909,6 +929,9 @@ int pf_timestep(pf_ctx* c, double requested_dt, int32_t enabled, double cfl, dou
 
 int pf_step(pf_ctx* c, const pf_step_cfg* cfg, pf_step_stats* stats) {
   if (!c || !cfg) return PF_ERR_ARG;
+  if (cfg->model != PF_MODEL_TWO_PHASE && cfg->model != PF_MODEL_SINGLE_PHASE)
+    return fail(c, PF_ERR_ARG, "pf_step_cfg.model must be PF_MODEL_TWO_PHASE or PF_MODEL_SINGLE_PHASE");
+  const bool single = cfg->model == PF_MODEL_SINGLE_PHASE;
   pf_step_stats S;
   std::memset(&S, 0, sizeof(S));
   const bool timing = cfg->timing != 0;
@@ -924,16 +947,18 @@ int pf_step(pf_ctx* c, const pf_step_cfg* cfg, pf_step_stats* stats) {
   st = run_mark(c);
   if (st) return st;
   record(c, timing, 2);
-  st = run_p2g_splat(c, &cfg->phase, cfg->Vp);
+  st = run_p2g_splat(c, &cfg->phase, cfg->Vp, single);
   if (st) return st;
   record(c, timing, 3);
   st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | FO_WALLS, (float)(dt * cfg->gravity));
   if (st) return st;
   record(c, timing, 4);
-  st = run_project(c, &cfg->phase, dt, &cfg->solve, cfg->c_div, &S.solve, timing, 5, 6, 7);
+  st = run_project(c, &cfg->phase, dt, &cfg->solve, cfg->c_div, &S.solve, timing, 5, 6, 7, single);
   if (st) return st;
   HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->esc_dev), 0u, 8));
-  st = run_g2p_advect(c, true, true, cfg->alpha_liquid, cfg->alpha_gas, dt, cfg->advection_order);
+  // the single-phase blend uses one alpha (SparseSim3D::alpha, sparse_sim3d.h:8)
+  st = run_g2p_advect(c, true, true, cfg->alpha_liquid, single ? cfg->alpha_liquid : cfg->alpha_gas, dt,
+                      cfg->advection_order);
   if (st) return st;
   record(c, timing, 8);
   HIP_TRY(c, hipMemcpyAsync(&S.escape, c->esc_dev, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, c->stream));

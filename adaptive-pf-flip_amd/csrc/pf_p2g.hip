@@ -1,4 +1,5 @@
-// pf_p2g.hip -- normalized-cubic P2G splat (Eq. 6) into MAC momentum and raw mass.
+// pf_p2g.hip -- normalized-cubic P2G splat (Eq. 6) into MAC momentum and raw mass,
+// and the single-phase trilinear splat (same launch structure).
 //
 // Reference: spP2G3D_tp / preactivateKernel / splatKernelColor
 // (src/driver/sparse_ops3d_tp.cpp:15-75, 95-124).  Weight w = (1 - d^2/r^2)^3, r = 1.5,
@@ -74,7 +75,39 @@ __device__ inline float exact_weight(double ddx, double ddy, double ddz, bool& a
   return active ? (float)(t * t * t) : 0.f;
 }
 
+// Single-phase trilinear splat, spP2G3D (src/driver/sparse_ops3d.cpp:22-68, 72-100):
+// nodes floor(g)..floor(g)+1 per axis with weights (1-f, f), w = (float)(wx*wy*wz)
+// in FP64, unit particle mass, no normalisation; every in-bounds stencil node
+// activates its block, zero weights included.  For a particle in cell c the nodes
+// lie in c..c+1 along the face axis and c-1..c+1 across it, inside the cubic
+// kernel's register window, so both kernels share the region and flush logic.
 template <int AX>
+__device__ inline void trilinear_weights(double gx, double gy, double gz, int i, int j, int k,
+                                         const bool* vx, const bool* vy, const bool* vz, float* wt,
+                                         uint64_t& fbits) {
+  using A = Ax<AX>;
+  const int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
+  const double fx = gx - i0, fy = gy - j0, fz = gz - k0;
+  const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy}, wz[2] = {1.0 - fz, fz};
+#pragma unroll
+  for (int n = 0; n < A::NN; ++n) wt[n] = 0.f;
+#pragma unroll
+  for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+      for (int di = 0; di < 2; ++di) {
+        // window offsets (node - cell + 1); node indices of the window are in range
+        const int ox = i0 + di - i + 1, oy = j0 + dj - j + 1, oz = k0 + dk - k + 1;
+        if (ox < 0 || ox >= A::NX || oy < 0 || oy >= A::NY || oz < 0 || oz >= A::NZ) continue;
+        if (!(vx[ox] && vy[oy] && vz[oz])) continue;
+        const int n = ox + A::NX * (oy + A::NY * oz);
+        wt[n] = (float)(__dmul_rn(__dmul_rn(wx[di], wy[dj]), wz[dk]));
+        fbits |= (1ull << n);
+      }
+}
+
+template <int AX, int KT>
 __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_t s0, uint32_t cnt,
                                   const uint8_t* ord, float* s_mom, float* s_mass, uint8_t* s_flag,
                                   int lx, int ly, int lz) {
@@ -114,6 +147,18 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
            pz = grid_coord(a.P.z[q], a.g.oz, a);
     // stencil coordinates of this axis' face grid (sparse_ops3d_tp.cpp:117-122)
     double gx = AX == 0 ? px : px - 0.5, gy = AX == 1 ? py : py - 0.5, gz = AX == 2 ? pz : pz - 0.5;
+    const float vel = AX == 0 ? a.P.u[q] : (AX == 1 ? a.P.v[q] : a.P.w[q]);
+    if (KT == 1) {
+      float wt[A::NN];
+      trilinear_weights<AX>(gx, gy, gz, i, j, k, vx, vy, vz, wt, fbits);
+      // mass += (float)(w * 1.0) = w; momentum += (float)(w * v) with v exact in FP64
+#pragma unroll
+      for (int n = 0; n < A::NN; ++n) {
+        am[n] += wt[n];
+        aq[n] += (float)((double)wt[n] * (double)vel);
+      }
+      continue;
+    }
     float fx = (float)(gx - (double)i), fy = (float)(gy - (double)j), fz = (float)(gz - (double)k);
     float dx2[A::NX], dy2[A::NY], dz2[A::NZ];
 #pragma unroll
@@ -152,7 +197,6 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
     uint8_t ty = a.P.type[q];
     float mp = (ty == 0) ? a.mp_l : a.mp_g;
     if (a.has_vol) mp *= a.P.vol[q];
-    float vel = AX == 0 ? a.P.u[q] : (AX == 1 ? a.P.v[q] : a.P.w[q]);
     float sm = wsum > 0.f ? mp / wsum : 0.f;
     float sq = sm * vel;
 #pragma unroll
@@ -198,6 +242,8 @@ __device__ inline void flush_axis(const P2GArgs& a, int bx, int by, int bz, cons
   }
 }
 
+// KT = 0: normalized cubic (two-phase); KT = 1: trilinear, unit mass (single-phase)
+template <int KT>
 __global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
   __shared__ float s_mom[3][256];
   __shared__ float s_mass[3][256];
@@ -230,9 +276,9 @@ __global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
     ord[0] = 0;
   }
   __syncthreads();
-  splat_axis<0>(a, i, j, k, s0, cnt, ord, s_mom[0], s_mass[0], s_flag[0], lx, ly, lz);
-  splat_axis<1>(a, i, j, k, s0, cnt, ord, s_mom[1], s_mass[1], s_flag[1], lx, ly, lz);
-  splat_axis<2>(a, i, j, k, s0, cnt, ord, s_mom[2], s_mass[2], s_flag[2], lx, ly, lz);
+  splat_axis<0, KT>(a, i, j, k, s0, cnt, ord, s_mom[0], s_mass[0], s_flag[0], lx, ly, lz);
+  splat_axis<1, KT>(a, i, j, k, s0, cnt, ord, s_mom[1], s_mass[1], s_flag[1], lx, ly, lz);
+  splat_axis<2, KT>(a, i, j, k, s0, cnt, ord, s_mom[2], s_mass[2], s_flag[2], lx, ly, lz);
   __syncthreads();
   flush_axis<0>(a, bx, by, bz, s_mom[0], s_mass[0], s_flag[0], lane);
   flush_axis<1>(a, bx, by, bz, s_mom[1], s_mass[1], s_flag[1], lane);

@@ -165,6 +165,8 @@ struct ProjArgs {
   double scale;   // dt / dx^2
   double cdiv;    // divergence correction
   uint32_t* pin;  // [0] touches-air flag, [1] min FLUID linear id, [4] FLUID cell count
+  int single;     // single-phase spProjectStep3D (sparse_ops3d.cpp:170-232): unit
+                  // coefficients, no pin, divergence without solid-face masking
 };
 
 __device__ inline bool list_block(const ProjArgs& a, long long t, long long& b) {
@@ -198,7 +200,7 @@ __global__ void __launch_bounds__(256) k_pin_detect(ProjArgs a) {
             marker_at(a.marker, a.dc, a.g, i, j, k - 1) == CELL_AIR;
       lin = (uint32_t)(i + (long long)a.g.nx * (j + (long long)a.g.ny * k));
     }
-    anyair = anyair || __any(air);
+    anyair = anyair || a.single || __any(air);  // the single-phase solve never pins
     nfl += (uint32_t)__popcll(__ballot(fl));
     m = min(m, lin);
   }
@@ -249,13 +251,22 @@ __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
       uint8_t m[6];
 #pragma unroll
       for (int q = 0; q < 6; ++q) m[q] = marker_at(a.marker, a.dc, a.g, nb[q][0], nb[q][1], nb[q][2]);
-      const double fb[6] = {beta_of_raw(o.mu[bm_index(a.du, i + 1, j, k)], a.ph),
-                            beta_of_raw(o.mu[bm_index(a.du, i, j, k)], a.ph),
-                            beta_of_raw(o.mv[bm_index(a.dv, i, j + 1, k)], a.ph),
-                            beta_of_raw(o.mv[bm_index(a.dv, i, j, k)], a.ph),
-                            beta_of_raw(o.mw[bm_index(a.dw, i, j, k + 1)], a.ph),
-                            beta_of_raw(o.mw[bm_index(a.dw, i, j, k)], a.ph)};
-      if (!pinned) {
+      double fb[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+      if (!a.single) {
+        fb[0] = beta_of_raw(o.mu[bm_index(a.du, i + 1, j, k)], a.ph);
+        fb[1] = beta_of_raw(o.mu[bm_index(a.du, i, j, k)], a.ph);
+        fb[2] = beta_of_raw(o.mv[bm_index(a.dv, i, j + 1, k)], a.ph);
+        fb[3] = beta_of_raw(o.mv[bm_index(a.dv, i, j, k)], a.ph);
+        fb[4] = beta_of_raw(o.mw[bm_index(a.dw, i, j, k + 1)], a.ph);
+        fb[5] = beta_of_raw(o.mw[bm_index(a.dw, i, j, k)], a.ph);
+      }
+      if (a.single) {
+        // sparse_ops3d.cpp:180-187: the six face values summed left to right
+        const double d = (double)o.u[bm_index(a.du, i + 1, j, k)] - (double)o.u[bm_index(a.du, i, j, k)] +
+                         (double)o.v[bm_index(a.dv, i, j + 1, k)] - (double)o.v[bm_index(a.dv, i, j, k)] +
+                         (double)o.w[bm_index(a.dw, i, j, k + 1)] - (double)o.w[bm_index(a.dw, i, j, k)];
+        rhs = -(d / a.g.dx);
+      } else if (!pinned) {
         double uR = m[0] == CELL_SOLID ? 0.0 : (double)o.u[bm_index(a.du, i + 1, j, k)];
         double uL = m[1] == CELL_SOLID ? 0.0 : (double)o.u[bm_index(a.du, i, j, k)];
         double vT = m[2] == CELL_SOLID ? 0.0 : (double)o.v[bm_index(a.dv, i, j + 1, k)];
@@ -581,6 +592,7 @@ struct VelArgs {
   double s;  // dt / dx
   const uint32_t* list;
   long long nlist;
+  int unit_beta;  // single-phase path: beta = 1 on every face (sparse_ops3d.cpp:270-316)
 };
 
 __device__ inline void update_face(const VelArgs& a, int ax, int fi, int fj, int fk, int li, int lj, int lk, int ri,
@@ -597,7 +609,7 @@ __device__ inline void update_face(const VelArgs& a, int ax, int fi, int fj, int
   }
   double pl = lf ? (double)a.p[bm_index(a.dc, li, lj, lk)] : 0.0;
   double pr = rf ? (double)a.p[bm_index(a.dc, ri, rj, rk)] : 0.0;
-  double dlt = a.s * beta_of_raw((double)a.mass[ax][gi], a.ph) * (pr - pl);
+  double dlt = a.unit_beta ? a.s * (pr - pl) : a.s * beta_of_raw((double)a.mass[ax][gi], a.ph) * (pr - pl);
   a.vel[ax][gi] = a.vel[ax][gi] - (float)dlt;
 }
 

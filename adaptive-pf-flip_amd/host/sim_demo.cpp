@@ -3,12 +3,15 @@
 //   tests/test_sparse_sim3d_tp.cpp:110-148  RT step stays stable (12 steps)
 //   tests/test_sparse_sim3d_tp.cpp:749-786  bubble rises, headspace stays sparse (24 steps)
 //   tests/test_sparse_sim3d_tp.cpp:150-169  adaptive timestep reports the effective dt
+//   tests/test_sparse_sim3d.cpp:6-36         single-phase dam-break falls, stays sparse (20 steps)
 // Exit status 0 when every check passes.
 #include <cmath>
 #include <cstdio>
 
+#include "sparse_sim3d_gpu.h"
 #include "sparse_sim3d_tp_gpu.h"
 
+using pfgpu::SparseSim3D;
 using pfgpu::SparseSim3DTP;
 
 static int failures = 0;
@@ -92,6 +95,34 @@ int main() {
     CHECK(std::fabs(sim.effective_dt_last - 0.005) < 1e-9);
     CHECK(sim.adaptive_timestep_limited_last == 1);
     std::printf("adaptive dt: %.6f\n", sim.effective_dt_last);
+  }
+  {
+    SparseSim3D sim(16, 16, 16, 1.0);
+    sim.initDamBreak();
+    const size_t n0 = sim.particles.size();
+    CHECK(n0 > 0);
+    double my0 = 0.0;
+    for (const auto& p : sim.particles.pos) my0 += p.y;
+    my0 /= (double)n0;
+    size_t maxActive = 0;
+    for (int s = 0; s < 20; ++s) {
+      sim.step();
+      maxActive = std::max(maxActive, sim.activeCellBlocks());
+    }
+    sim.syncParticlesToHost();
+    CHECK(sim.particles.size() == n0);
+    bool fin = true;
+    double my1 = 0.0;
+    for (const auto& p : sim.particles.pos) {
+      fin = fin && std::isfinite(p.x) && std::isfinite(p.y) && std::isfinite(p.z);
+      my1 += p.y;
+    }
+    my1 /= (double)n0;
+    CHECK(fin);
+    CHECK(my1 < my0);
+    CHECK(maxActive > 0 && maxActive < sim.totalCellBlocks());
+    std::printf("single-phase dam-break: mean y %.4f -> %.4f, max active %zu of %zu\n", my0, my1, maxActive,
+                sim.totalCellBlocks());
   }
   std::printf(failures ? "FAILED %d checks\n" : "all checks passed\n", failures);
   return failures ? 1 : 0;

@@ -60,12 +60,15 @@ class EscapeStats(C.Structure):
                                           "y_hi", "z_lo", "z_hi")]
 
 
+MODEL_TWO_PHASE, MODEL_SINGLE_PHASE = 0, 1
+
+
 class StepCfg(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("Vp", C.c_double),
                 ("alpha_liquid", C.c_double), ("alpha_gas", C.c_double),
                 ("adaptive_cfl", C.c_double), ("adaptive_min_dt", C.c_double),
                 ("c_div", C.c_double), ("adaptive_timestep", C.c_int32),
-                ("advection_order", C.c_int32), ("timing", C.c_int32), ("reserved", C.c_int32),
+                ("advection_order", C.c_int32), ("timing", C.c_int32), ("model", C.c_int32),
                 ("phase", Phase), ("solve", SolveCfg)]
 
 
@@ -120,6 +123,8 @@ def lib():
             "pf_calibrate_rho_tilde0": (dbl, [C.POINTER(Phase), dbl]),
             "pf_phi_from_raw": (dbl, [dbl, C.POINTER(Phase)]),
             "pf_beta_from_phi": (dbl, [dbl, C.POINTER(Phase)]),
+            "pf_p2g_single": (C.c_int, [vp]),
+            "pf_project_single": (C.c_int, [vp, dbl, C.POINTER(SolveCfg), C.POINTER(SolveStats)]),
             "pf_stream": (vp, [vp]),
             "pf_kernel_timing": (C.c_int, [vp, i32]),
             "pf_kernel_time": (C.c_int, [vp, i32, C.POINTER(i64), C.POINTER(dbl)]),
@@ -204,6 +209,17 @@ class Context:
 
     def p2g(self, phase, Vp=1.0):
         self._check(lib().pf_p2g(self.h, C.byref(phase), Vp), "pf_p2g")
+
+    def p2g_single(self):
+        """spP2G3D (single-phase trilinear splat, unit mass)."""
+        self._check(lib().pf_p2g_single(self.h), "pf_p2g_single")
+
+    def project_single(self, dt, cfg=None):
+        """spProjectStep3D (single-phase, rho = 1, no pinned cell)."""
+        cfg = cfg or default_solve_cfg(STOP_REF_MAXABS, 1e-6, 300)
+        st = SolveStats()
+        self._check(lib().pf_project_single(self.h, dt, C.byref(cfg), C.byref(st)), "pf_project_single")
+        return st
 
     def save_velocity(self):
         self._check(lib().pf_save_velocity(self.h), "pf_save_velocity")
@@ -351,5 +367,43 @@ class SparseSim3DTP:
         st = self.ctx.step(cfg)
         self.effective_dt_last = st.effective_dt
         self.max_particle_speed_last = st.max_particle_speed
+        self.stats_last = st
+        return st
+
+
+class SparseSim3D:
+    """GPU mirror of the reference's single-phase SparseSim3D (src/driver/sparse_sim3d.h:5-15):
+    trilinear unit-mass P2G, rho = 1 Jacobi PCG projection, FLIP blend with one alpha,
+    RK2 advection."""
+
+    def __init__(self, nx, ny, nz, dx):
+        self.ctx = Context(nx, ny, nz, dx)
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.dt, self.alpha, self.gravity = 0.05, 0.95, -9.81
+        self.cg_iters, self.cg_tol = 300, 1e-6
+        self.stop_rule = STOP_REF_MAXABS
+        self.inner_rel_tol, self.warm_start, self.check_every = 0.1, 0, 8
+        # CFL-limited dt (timestep3d.h:16-50); off in the reference class
+        self.adaptive_timestep, self.adaptive_cfl, self.adaptive_min_dt = False, 1.0, 1e-5
+        self.timing = False
+        self.stats_last = None
+
+    def set_particles(self, x, y, z, u=None, v=None, w=None):
+        self.ctx.upload(x, y, z, u, v, w)
+
+    def step_cfg(self) -> StepCfg:
+        solve = SolveCfg(max_iters=self.cg_iters, stop_rule=self.stop_rule, tol=self.cg_tol,
+                         inner_rel_tol=self.inner_rel_tol, max_outer=10000,
+                         warm_start=self.warm_start, check_every=self.check_every)
+        return StepCfg(dt=self.dt, gravity=self.gravity, Vp=1.0, alpha_liquid=self.alpha,
+                       alpha_gas=self.alpha, adaptive_cfl=self.adaptive_cfl,
+                       adaptive_min_dt=self.adaptive_min_dt, c_div=0.0,
+                       adaptive_timestep=int(self.adaptive_timestep), advection_order=2,
+                       timing=int(self.timing), model=MODEL_SINGLE_PHASE,
+                       phase=Phase(1.0, 1.0, 1.0, 1.0), solve=solve)
+
+    def step(self):
+        """SparseSim3D::step, src/driver/sparse_sim3d.cpp:106-115."""
+        st = self.ctx.step(self.step_cfg())
         self.stats_last = st
         return st

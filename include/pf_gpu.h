@@ -88,14 +88,21 @@ typedef struct pf_escape_stats {
   int32_t clamped_liquid, clamped_gas, x_lo, x_hi, y_lo, y_hi, z_lo, z_hi;
 } pf_escape_stats;
 
-/* The SparseSim3DTP fields that step() reads (src/driver/sparse_sim3d_tp.h:12-28) */
+enum pf_model {
+  PF_MODEL_TWO_PHASE = 0,    /* SparseSim3DTP::step, sparse_sim3d_tp.cpp:854-884 */
+  PF_MODEL_SINGLE_PHASE = 1  /* SparseSim3D::step, sparse_sim3d.cpp:106-115 */
+};
+
+/* The SparseSim3DTP fields that step() reads (src/driver/sparse_sim3d_tp.h:12-28).
+ * For PF_MODEL_SINGLE_PHASE (SparseSim3D, sparse_sim3d.h:5-15): alpha_liquid is the
+ * one FLIP ratio, phase, Vp, alpha_gas and c_div are unused. */
 typedef struct pf_step_cfg {
   double dt, gravity, Vp, alpha_liquid, alpha_gas;
   double adaptive_cfl, adaptive_min_dt;
   double c_div;              /* divergenceCorrection passed to the projection */
   int32_t adaptive_timestep, advection_order;
   int32_t timing;            /* 1: record per-stage HIP-event times into pf_step_stats */
-  int32_t reserved;
+  int32_t model;             /* pf_model */
   pf_phase phase;
   pf_solve_cfg solve;
 } pf_step_cfg;
@@ -141,6 +148,9 @@ int pf_scene_generate(pf_ctx* ctx, int32_t kind, uint64_t seed);
 int pf_mark_cells(pf_ctx* ctx);
 /* spP2G3D_tp, sparse_ops3d_tp.cpp:95-153 / sparse_ops3d_tp.h:8 */
 int pf_p2g(pf_ctx* ctx, const pf_phase* phase, double Vp);
+/* spP2G3D, sparse_ops3d.cpp:72-125 / sparse_ops3d.h:6: the single-phase trilinear
+ * splat with unit particle mass */
+int pf_p2g_single(pf_ctx* ctx);
 /* `SparseMacGrid3D<4> saved = grid`, sparse_sim3d_tp.cpp:866 (velocities only) */
 int pf_save_velocity(pf_ctx* ctx);
 /* applyGravity, sparse_sim3d_tp.cpp:52-70 */
@@ -150,15 +160,21 @@ int pf_apply_walls(pf_ctx* ctx);
 /* spProjectStepVC3D, sparse_ops3d_tp.cpp:234-423 / sparse_ops3d_tp.h:9-10 */
 int pf_project(pf_ctx* ctx, const pf_phase* phase, double dt, const pf_solve_cfg* cfg,
                double div_correction, pf_solve_stats* stats);
-/* spG2P3D_tp, sparse_ops3d_tp.cpp:155-172 (saved = the pf_save_velocity copy) */
+/* spProjectStep3D, sparse_ops3d.cpp:170-319 / sparse_ops3d.h:9: the single-phase
+ * (rho = 1) projection: unit face coefficients, no pinned cell */
+int pf_project_single(pf_ctx* ctx, double dt, const pf_solve_cfg* cfg, pf_solve_stats* stats);
+/* spG2P3D_tp, sparse_ops3d_tp.cpp:155-172 (saved = the pf_save_velocity copy); the
+ * single-phase spG2P3D (sparse_ops3d.cpp:127-142) is alpha_liquid == alpha_gas */
 int pf_g2p(pf_ctx* ctx, double alpha_liquid, double alpha_gas);
-/* spAdvect3D_tp, sparse_ops3d_tp.cpp:201-232 / sparse_ops3d_tp.h:12-15 */
+/* spAdvect3D_tp, sparse_ops3d_tp.cpp:201-232 / sparse_ops3d_tp.h:12-15; order 2 is
+ * also the single-phase spAdvect3D (sparse_ops3d.cpp:144-168) */
 int pf_advect(pf_ctx* ctx, double dt, int32_t order, pf_escape_stats* stats);
 /* computeAdaptiveParticleTimestep3D, src/driver/timestep3d.h:16-50 */
 int pf_timestep(pf_ctx* ctx, double requested_dt, int32_t enabled, double cfl, double min_dt,
                 double* effective_dt, double* max_speed, double* cfl_limit_dt, int32_t* limited);
 /* SparseSim3DTP::step, sparse_sim3d_tp.cpp:854-884 (default flags: no particle
- * adaptivity, secondary particles or surface tension), with G2P and advection fused */
+ * adaptivity, secondary particles or surface tension), with G2P and advection fused;
+ * cfg->model = PF_MODEL_SINGLE_PHASE runs SparseSim3D::step, sparse_sim3d.cpp:106-115 */
 int pf_step(pf_ctx* ctx, const pf_step_cfg* cfg, pf_step_stats* stats);
 
 /* ---- field access (SparseBlockGrid3D::get / ref, sparse_block_grid3d.h:46-55).

@@ -421,6 +421,67 @@ void orc_p2g(orc_grid* g, int64_t n, const double* x, const double* y, const dou
   normalize_axis(g, ORC_W, ORC_MW, nx - 1, ny - 1, nz);   /* :144-152 */
 }
 
+/* ------------------------------------------- single-phase path (sparse_ops3d.cpp) */
+
+/* preactivateField, src/driver/sparse_ops3d.cpp:22-40: every in-bounds node of the
+ * 2x2x2 trilinear stencil activates its block (zero weights included) */
+static void preactivate_tri(orc_field* f, double gx, double gy, double gz, int nx, int ny, int nz) {
+  int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
+  for (int dk = 0; dk < 2; ++dk)
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) {
+        int i = i0 + di, j = j0 + dj, k = k0 + dk;
+        if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) continue;
+        field_activate(f, i / BLK, j / BLK, k / BLK);
+      }
+}
+
+/* splatFieldColor, src/driver/sparse_ops3d.cpp:42-68, colours merged (particle order
+ * per node is unchanged): w = (float)(wx*wy*wz); field += (float)(w*mom) */
+static void splat_tri(const orc_grid* g, orc_field* fld, orc_field* mass, double gx, double gy,
+                      double gz, double mom, double m, int nx, int ny, int nz) {
+  int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
+  double fx = gx - i0, fy = gy - j0, fz = gz - k0;
+  double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy}, wz[2] = {1.0 - fz, fz};
+  for (int dk = 0; dk < 2; ++dk)
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) {
+        int i = i0 + di, j = j0 + dj, k = k0 + dk;
+        if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) continue;
+        double w = st(g, wx[di] * wy[dj] * wz[dk]);
+        double* a = field_ref(fld, i, j, k);
+        *a = acc(g, *a, w * mom);
+        double* b = field_ref(mass, i, j, k);
+        *b = acc(g, *b, w * m);
+      }
+}
+
+/* spP2G3D, src/driver/sparse_ops3d.cpp:72-125: trilinear splat, unit particle mass */
+void orc_p2g_single(orc_grid* g, int64_t n, const double* x, const double* y, const double* z,
+                    const double* u, const double* v, const double* w) {
+  for (int f = ORC_U; f <= ORC_MW; ++f) field_clear(&g->f[f]);
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const double mp = 1.0;
+  for (int64_t p = 0; p < n; ++p) {
+    double px = (x[p] - g->ox) / g->dx, py = (y[p] - g->oy) / g->dx, pz = (z[p] - g->oz) / g->dx;
+    preactivate_tri(&g->f[ORC_U], px, py - 0.5, pz - 0.5, nx + 1, ny, nz);
+    preactivate_tri(&g->f[ORC_MU], px, py - 0.5, pz - 0.5, nx + 1, ny, nz);
+    preactivate_tri(&g->f[ORC_V], px - 0.5, py, pz - 0.5, nx, ny + 1, nz);
+    preactivate_tri(&g->f[ORC_MV], px - 0.5, py, pz - 0.5, nx, ny + 1, nz);
+    preactivate_tri(&g->f[ORC_W], px - 0.5, py - 0.5, pz, nx, ny, nz + 1);
+    preactivate_tri(&g->f[ORC_MW], px - 0.5, py - 0.5, pz, nx, ny, nz + 1);
+  }
+  for (int64_t p = 0; p < n; ++p) {
+    double px = (x[p] - g->ox) / g->dx, py = (y[p] - g->oy) / g->dx, pz = (z[p] - g->oz) / g->dx;
+    splat_tri(g, &g->f[ORC_U], &g->f[ORC_MU], px, py - 0.5, pz - 0.5, mp * u[p], mp, nx + 1, ny, nz);
+    splat_tri(g, &g->f[ORC_V], &g->f[ORC_MV], px - 0.5, py, pz - 0.5, mp * v[p], mp, nx, ny + 1, nz);
+    splat_tri(g, &g->f[ORC_W], &g->f[ORC_MW], px - 0.5, py - 0.5, pz, mp * w[p], mp, nx, ny, nz + 1);
+  }
+  normalize_axis(g, ORC_U, ORC_MU, nx, ny - 1, nz - 1);   /* :101-110 */
+  normalize_axis(g, ORC_V, ORC_MV, nx - 1, ny, nz - 1);   /* :111-119 */
+  normalize_axis(g, ORC_W, ORC_MW, nx - 1, ny - 1, nz);   /* :120-124 */
+}
+
 /* applyGravity, src/driver/sparse_sim3d_tp.cpp:52-70 */
 void orc_apply_gravity(orc_grid* g, double dt, double gravity) {
   orc_field* MV = &g->f[ORC_MV];
@@ -463,9 +524,11 @@ void orc_apply_walls(orc_grid* g) {
     }
 }
 
-/* spProjectStepVC3D, sparse_ops3d_tp.cpp:234-423 */
-void orc_project(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, double cg_tol,
-                 double div_correction, int stop_rule, orc_solve_stats* stt) {
+/* spProjectStepVC3D, sparse_ops3d_tp.cpp:234-423 (single = 0), and the single-phase
+ * spProjectStep3D, sparse_ops3d.cpp:170-319 (single = 1: unit coefficients, no pin,
+ * no c_div, the divergence summed left to right, :180-187) */
+static void project_impl(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, double cg_tol,
+                         double div_correction, int stop_rule, int single, orc_solve_stats* stt) {
   orc_solve_stats dummy;
   orc_solve_stats* S = stt ? stt : &dummy;
   memset(S, 0, sizeof(*S));
@@ -516,7 +579,7 @@ void orc_project(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, doub
     for (int nn = 0; nn < 6; ++nn)
       if (is_air(g, i + di[nn], j + dj[nn], k + dk[nn])) { touches_air = 1; break; }
   }
-  if (!touches_air) pc = cells[0];
+  if (!touches_air && !single) pc = cells[0];
   S->pinned = pc >= 0;
   S->pinned_cell = pc;
 
@@ -538,6 +601,21 @@ void orc_project(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, doub
     int i = (int)(cells[t] % nx);
     int64_t q = cells[t] / nx;
     int j = (int)(q % ny), k = (int)(q / ny);
+    if (single) {
+      /* the reference sums the float face values in float arithmetic, then divides in double */
+      double sum;
+      if (g->faithful) {
+        float fs = (float)field_get(U, i + 1, j, k) - (float)field_get(U, i, j, k) +
+                   (float)field_get(V, i, j + 1, k) - (float)field_get(V, i, j, k) +
+                   (float)field_get(W, i, j, k + 1) - (float)field_get(W, i, j, k);
+        sum = fs;
+      } else {
+        sum = field_get(U, i + 1, j, k) - field_get(U, i, j, k) + field_get(V, i, j + 1, k) -
+              field_get(V, i, j, k) + field_get(W, i, j, k + 1) - field_get(W, i, j, k);
+      }
+      r[t] = -(sum / g->dx);
+      continue;
+    }
     double uR = is_solid(g, i + 1, j, k) ? 0.0 : field_get(U, i + 1, j, k);
     double uL = is_solid(g, i - 1, j, k) ? 0.0 : field_get(U, i, j, k);
     double vT = is_solid(g, i, j + 1, k) ? 0.0 : field_get(V, i, j + 1, k);
@@ -551,9 +629,12 @@ void orc_project(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, doub
     int i = (int)(cells[t] % nx);
     int64_t q = cells[t] / nx;
     int j = (int)(q % ny), k = (int)(q / ny);
-    double fb[6] = {beta_of_raw(field_get(MU, i + 1, j, k), pp), beta_of_raw(field_get(MU, i, j, k), pp),
-                    beta_of_raw(field_get(MV, i, j + 1, k), pp), beta_of_raw(field_get(MV, i, j, k), pp),
-                    beta_of_raw(field_get(MW, i, j, k + 1), pp), beta_of_raw(field_get(MW, i, j, k), pp)};
+    double fb[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+    if (!single) {
+      fb[0] = beta_of_raw(field_get(MU, i + 1, j, k), pp); fb[1] = beta_of_raw(field_get(MU, i, j, k), pp);
+      fb[2] = beta_of_raw(field_get(MV, i, j + 1, k), pp); fb[3] = beta_of_raw(field_get(MV, i, j, k), pp);
+      fb[4] = beta_of_raw(field_get(MW, i, j, k + 1), pp); fb[5] = beta_of_raw(field_get(MW, i, j, k), pp);
+    }
     for (int nn = 0; nn < 6; ++nn) {
       nbr[6 * t + nn] = -1;
       int ni = i + di[nn], nj = j + dj[nn], nk = k + dk[nn];
@@ -681,13 +762,24 @@ void orc_project(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, doub
       double pl = lf ? field_get(P, li, lj, lk) : 0.0;
       double pr = rf ? field_get(P, i, j, k) : 0.0;
       double cur = field_get(F, i, j, k);
-      double dlt = s * beta_of_raw(field_get(M, i, j, k), pp) * (pr - pl);
+      double dlt = single ? s * (pr - pl) : s * beta_of_raw(field_get(M, i, j, k), pp) * (pr - pl);
       *field_ref(F, i, j, k) = g->faithful ? (double)((float)cur - (float)dlt) : cur - dlt;
     }
     free(mark);
   }
   free(isf); free(cells); free(index); free(x); free(r); free(z); free(pd); free(Ap); free(b);
   free(diag); free(nbr); free(coeff);
+}
+
+void orc_project(orc_grid* g, const orc_phase* pp, double dt, int cg_iters, double cg_tol,
+                 double div_correction, int stop_rule, orc_solve_stats* st) {
+  project_impl(g, pp, dt, cg_iters, cg_tol, div_correction, stop_rule, 0, st);
+}
+
+/* spProjectStep3D, src/driver/sparse_ops3d.cpp:170-319 */
+void orc_project_single(orc_grid* g, double dt, int cg_iters, double cg_tol, int stop_rule,
+                        orc_solve_stats* st) {
+  project_impl(g, NULL, dt, cg_iters, cg_tol, 0.0, stop_rule, 1, st);
 }
 
 /* sampleField, src/driver/sparse_ops3d_common.h:30-51 (returns float in the reference) */
@@ -855,6 +947,38 @@ void orc_step(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u
   orc_grid_destroy(saved);
   S->ms_mark = t2 - t1; S->ms_p2g = t3 - t2; S->ms_save_gravity_walls = t4 - t3;
   S->ms_project = t5 - t4; S->ms_g2p = t6 - t5; S->ms_advect = t7 - t6; S->ms_total = t7 - t0;
+}
+
+/* SparseSim3D::step, src/driver/sparse_sim3d.cpp:106-115: mark, trilinear P2G, saved
+ * copy, gravity, walls, unit-density projection, FLIP blend with one alpha, RK2.
+ * markCells / applyGravity / applyWallBoundary (:26-104) are the two-phase ones. */
+void orc_step_single(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u,
+                     double* v, double* w, double dt, double alpha, double gravity, int cg_iters,
+                     double cg_tol, int stop_rule, orc_step_stats* S) {
+  orc_step_stats local;
+  if (!S) S = &local;
+  memset(S, 0, sizeof(*S));
+  uint8_t* type = (uint8_t*)calloc(n > 0 ? (size_t)n : 1, 1);
+  double t0 = now_ms();
+  orc_mark_cells(g, n, x, y, z);
+  double t1 = now_ms();
+  orc_p2g_single(g, n, x, y, z, u, v, w);
+  double t2 = now_ms();
+  orc_grid* saved = orc_grid_clone(g);
+  orc_apply_gravity(g, dt, gravity);
+  orc_apply_walls(g);
+  double t3 = now_ms();
+  orc_project_single(g, dt, cg_iters, cg_tol, stop_rule, &S->solve);
+  double t4 = now_ms();
+  orc_g2p(g, saved, n, x, y, z, u, v, w, type, alpha, alpha);
+  double t5 = now_ms();
+  orc_advect(g, n, x, y, z, type, dt, 2, &S->escape, 0, 0);
+  double t6 = now_ms();
+  orc_grid_destroy(saved);
+  free(type);
+  S->ts.requested_dt = S->ts.effective_dt = dt;
+  S->ms_mark = t1 - t0; S->ms_p2g = t2 - t1; S->ms_save_gravity_walls = t3 - t2;
+  S->ms_project = t4 - t3; S->ms_g2p = t5 - t4; S->ms_advect = t6 - t5; S->ms_total = t6 - t0;
 }
 
 /* ------------------------------------------------------------------ scenes */

@@ -30,6 +30,12 @@
  *   orc_advect .................... clamps 7.5 / 0.5 (:123-161), RK3 closed form (:163-180).
  *   orc_timestep .................. 0.05 / 0.2 (test_sparse_sim3d_tp.cpp:85-107).
  *   orc_step ...................... vs oracle/_ref SparseSim3DTP::step on identical state.
+ *   orc_p2g_single ................ partition of unity (unit masses), brute-force 2x2x2 active
+ *                                   sets, bit-exact vs oracle/_ref spP2G3D.
+ *   orc_project_single ............ dense direct solve of the unit-coefficient system; bit-exact
+ *                                   vs oracle/_ref spProjectStep3D.
+ *   orc_step_single ............... the reference's dam-break test (test_sparse_sim3d.cpp:6-36)
+ *                                   and bit-exact vs oracle/_ref SparseSim3D::step.
  */
 #ifndef PF_ORACLE_H
 #define PF_ORACLE_H
@@ -124,6 +130,15 @@ typedef struct orc_step_stats {
 void orc_step(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u, double* v,
               double* w, const uint8_t* type, const double* vol, const orc_step_cfg* cfg,
               orc_step_stats* st);
+
+/* single-phase sparse path (src/driver/sparse_ops3d.cpp, sparse_sim3d.cpp) */
+void orc_p2g_single(orc_grid* g, int64_t n, const double* x, const double* y, const double* z,
+                    const double* u, const double* v, const double* w);
+void orc_project_single(orc_grid* g, double dt, int cg_iters, double cg_tol, int stop_rule,
+                        orc_solve_stats* st);
+void orc_step_single(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u,
+                     double* v, double* w, double dt, double alpha, double gravity, int cg_iters,
+                     double cg_tol, int stop_rule, orc_step_stats* st);
 
 /* seeded synthetic scenes (shared construction with the HIP generator) */
 enum { ORC_SCENE_RT = 0, ORC_SCENE_BUBBLE = 1, ORC_SCENE_DAMBREAK = 2 };

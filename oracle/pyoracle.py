@@ -125,6 +125,12 @@ def lib():
                                    C.c_double, C.c_double, C.POINTER(TimestepStats)]
         L.orc_step.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _u8,
                                C.c_void_p, C.POINTER(StepCfg), C.POINTER(StepStats)]
+        L.orc_p2g_single.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_project_single.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int,
+                                         C.POINTER(SolveStats)]
+        L.orc_step_single.argtypes = ([C.c_void_p, C.c_int64] + [_dp] * 6 +
+                                      [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                                       C.c_int, C.POINTER(StepStats)])
         L.orc_scene_count.restype = C.c_int64
         L.orc_scene_count.argtypes = [C.c_int] * 4
         L.orc_scene_fill.restype = C.c_int64
@@ -234,6 +240,22 @@ class Grid:
                          _vptr(idx), C.byref(ne))
         return es, idx[: ne.value]
 
+    # single-phase path (src/driver/sparse_ops3d.cpp, sparse_sim3d.cpp)
+    def p2g_single(self, ps: Particles):
+        lib().orc_p2g_single(self.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w)
+
+    def project_single(self, dt, iters=300, tol=1e-6, stop_rule=STOP_REF_MAXABS):
+        st = SolveStats()
+        lib().orc_project_single(self.h, dt, iters, tol, stop_rule, C.byref(st))
+        return st
+
+    def step_single(self, ps: Particles, dt, alpha=0.95, gravity=-9.81, iters=300, tol=1e-6,
+                    stop_rule=STOP_REF_MAXABS):
+        st = StepStats()
+        lib().orc_step_single(self.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, dt, alpha,
+                              gravity, iters, tol, stop_rule, C.byref(st))
+        return st
+
     def step(self, ps: Particles, cfg: StepCfg):
         st = StepStats()
         lib().orc_step(self.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, ps.type,
@@ -324,8 +346,85 @@ def reflib():
         L.ref_beta.argtypes = [C.c_double, _dp]
         L.ref_timestep.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
                                    C.c_double, C.c_double, _dp]
+        L.ref1_create.restype = C.c_void_p
+        L.ref1_create.argtypes = [C.c_int] * 3 + [C.c_double]
+        L.ref1_destroy.argtypes = [C.c_void_p]
+        L.ref1_config.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.ref1_init_dambreak.restype = C.c_int64
+        L.ref1_init_dambreak.argtypes = [C.c_void_p]
+        L.ref1_set_particles.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 6
+        L.ref1_get_particles.argtypes = [C.c_void_p] + [_dp] * 6
+        L.ref1_step.restype = C.c_double
+        L.ref1_step.argtypes = [C.c_void_p]
+        L.ref1_field_dense.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ref1_active_blocks.restype = C.c_int64
+        L.ref1_active_blocks.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.ref1_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.ref1_p2g.argtypes = [C.c_void_p]
+        L.ref1_project.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double]
         _ref = L
     return _ref
+
+
+class RefSim1:
+    """The reference's single-phase SparseSim3D (src/driver/sparse_sim3d.h:5-15)."""
+
+    def __init__(self, nx, ny, nz, dx=1.0):
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.h = reflib().ref1_create(nx, ny, nz, dx)
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            reflib().ref1_destroy(self.h)
+            self.h = None
+
+    def config(self, dt, alpha=0.95, gravity=-9.81, iters=300, tol=1e-6):
+        reflib().ref1_config(self.h, np.array([dt, alpha, gravity, tol], np.float64), iters)
+
+    def init_dambreak(self) -> Particles:
+        self.n = reflib().ref1_init_dambreak(self.h)
+        a = [np.zeros(self.n) for _ in range(6)]
+        reflib().ref1_get_particles(self.h, *a)
+        return Particles(*a)
+
+    def set_particles(self, ps: Particles):
+        self.n = len(ps)
+        reflib().ref1_set_particles(self.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w)
+
+    def get_particles(self) -> Particles:
+        a = [np.zeros(self.n) for _ in range(6)]
+        reflib().ref1_get_particles(self.h, *a)
+        return Particles(*a)
+
+    def step(self):
+        return reflib().ref1_step(self.h)
+
+    def dims(self, f):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        return {0: (nx + 1, ny, nz), 1: (nx, ny + 1, nz), 2: (nx, ny, nz + 1),
+                3: (nx + 1, ny, nz), 4: (nx, ny + 1, nz), 5: (nx, ny, nz + 1)}.get(f, (nx, ny, nz))
+
+    def dense(self, f):
+        nx, ny, nz = self.dims(f)
+        out = np.zeros(nx * ny * nz)
+        reflib().ref1_field_dense(self.h, f, out)
+        return out.reshape(nz, ny, nx)
+
+    def active_blocks(self, f):
+        n = reflib().ref1_active_blocks(self.h, f, None)
+        out = np.zeros(max(n, 1), np.int32)
+        reflib().ref1_active_blocks(self.h, f, _vptr(out))
+        return out[:n]
+
+    def set(self, f, i, j, k, v):
+        reflib().ref1_set(self.h, f, i, j, k, float(v))
+
+    def p2g(self):
+        reflib().ref1_p2g(self.h)
+
+    def project(self, dt, iters=300, tol=1e-6):
+        reflib().ref1_project(self.h, dt, iters, tol)
 
 
 class RefSim:

@@ -7,8 +7,12 @@
 // spProjectStepVC3D, spG2P3D_tp, spAdvect3D_tp (src/driver/sparse_ops3d_tp.h:8-15),
 // calibrateRhoTilde0 (src/transfer/transfer3d_tp.h:5), phiFromRawDensity /
 // betaFromPhi (src/physics/phasefield.h:10-18), computeAdaptiveParticleTimestep3D
-// (src/driver/timestep3d.h:16).
+// (src/driver/timestep3d.h:16); and the single-phase SparseSim3D
+// (src/driver/sparse_sim3d.h:5-15) with spP2G3D / spProjectStep3D
+// (src/driver/sparse_ops3d.h:5-9).
+#include "driver/sparse_ops3d.h"
 #include "driver/sparse_ops3d_tp.h"
+#include "driver/sparse_sim3d.h"
 #include "driver/sparse_sim3d_tp.h"
 #include "driver/timestep3d.h"
 #include "grid/sparse_mac_grid3d.h"
@@ -181,6 +185,65 @@ void ref_timestep(int64_t n, const double* u, const double* v, const double* w, 
   TimestepStats3D st = computeAdaptiveParticleTimestep3D(ps, dx, dt, enabled != 0, cfl, mindt);
   out[0] = st.effective_dt; out[1] = st.max_particle_speed; out[2] = st.cfl_limit_dt;
   out[3] = st.limited;
+}
+
+// ---- single-phase SparseSim3D
+void* ref1_create(int nx, int ny, int nz, double dx) { return new SparseSim3D(nx, ny, nz, dx); }
+void ref1_destroy(void* s) { delete static_cast<SparseSim3D*>(s); }
+// cfg: dt, alpha, gravity, cg_tol ; cg_iters
+void ref1_config(void* sp, const double* cfg, int cg_iters) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  s->dt = cfg[0]; s->alpha = cfg[1]; s->gravity = cfg[2]; s->cg_tol = cfg[3]; s->cg_iters = cg_iters;
+}
+int64_t ref1_init_dambreak(void* sp) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  s->initDamBreak();
+  return (int64_t)s->particles.size();
+}
+void ref1_set_particles(void* sp, int64_t n, const double* x, const double* y, const double* z,
+                        const double* u, const double* v, const double* w) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  s->particles = Particles3D();
+  for (int64_t p = 0; p < n; ++p) s->particles.add({x[p], y[p], z[p]}, {u[p], v[p], w[p]}, 0);
+}
+void ref1_get_particles(void* sp, double* x, double* y, double* z, double* u, double* v, double* w) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  for (size_t p = 0; p < s->particles.size(); ++p) {
+    x[p] = s->particles.pos[p].x; y[p] = s->particles.pos[p].y; z[p] = s->particles.pos[p].z;
+    u[p] = s->particles.vel[p].x; v[p] = s->particles.vel[p].y; w[p] = s->particles.vel[p].z;
+  }
+}
+double ref1_step(void* sp) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  auto t0 = std::chrono::steady_clock::now();
+  s->step();
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+void ref1_field_dense(void* sp, int f, double* out) {
+  auto& F = fieldOf(static_cast<SparseSim3D*>(sp)->grid, f);
+  for (int k = 0; k < F.nz; ++k)
+    for (int j = 0; j < F.ny; ++j)
+      for (int i = 0; i < F.nx; ++i)
+        out[i + (int64_t)F.nx * (j + (int64_t)F.ny * k)] = F.get(i, j, k);
+}
+int64_t ref1_active_blocks(void* sp, int f, int32_t* out) {
+  auto& F = fieldOf(static_cast<SparseSim3D*>(sp)->grid, f);
+  if (out) for (size_t a = 0; a < F.active_block_ids.size(); ++a) out[a] = F.active_block_ids[a];
+  return (int64_t)F.active_block_ids.size();
+}
+void ref1_set(void* sp, int f, int i, int j, int k, double v) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  if (f == 7) s->grid.setCell(i, j, k, (int)v);
+  else fieldOf(s->grid, f).ref(i, j, k) = (float)v;
+}
+void ref1_p2g(void* sp) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  spP2G3D(s->grid, s->particles);
+}
+void ref1_project(void* sp, double dt, int iters, double tol) {
+  auto* s = static_cast<SparseSim3D*>(sp);
+  spProjectStep3D(s->grid, dt, iters, tol);
 }
 
 }  // extern "C"
