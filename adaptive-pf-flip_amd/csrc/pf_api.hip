@@ -22,8 +22,30 @@
 #include "pf_p2g.hip"
 #include "pf_project.hip"
 #include "pf_g2p.hip"
+#include "pf_lpc.hip"
 
 using namespace pf;
+
+// Learned-preconditioner handle: FP32 master weights with Adam moments, their BF16
+// layer matrices, and activation buffers grown to the largest tile count seen.
+struct pf_lpc {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  LpcW bf{};
+  bool prepped = false;
+  int adam_t = 0;
+  long long cap = 0, cap_train = 0;
+  __bf16 *X = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr;
+  float *o = nullptr, *go = nullptr, *scale = nullptr;
+  float* part = nullptr;
+  long long part_cap = 0;
+  float *zbuf = nullptr, *qbuf = nullptr, *gz = nullptr;
+  long long cell_cap = 0;
+  float* stage = nullptr;
+  long long stage_cap = 0;
+};
 
 #define HIP_TRY(c, expr)                                                         \
   do {                                                                           \
@@ -268,6 +290,156 @@ int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false)
   return PF_OK;
 }
 
+// ---- learned preconditioner helpers
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+constexpr int kLpcWgradWGs = 512;
+
+int lpc_grow(pf_lpc* L, long long nt, bool train) {
+  if (nt > L->cap) {
+    dfree(L->X); dfree(L->h1); dfree(L->h2); dfree(L->o); dfree(L->scale);
+    HIP_TRY(L, dalloc(&L->X, nt * kHaloVol * 4));
+    HIP_TRY(L, dalloc(&L->h1, nt * kTileVol * 16));
+    HIP_TRY(L, dalloc(&L->h2, nt * kTileVol * 32));
+    HIP_TRY(L, dalloc(&L->o, nt * kTileVol));
+    HIP_TRY(L, dalloc(&L->scale, nt));
+    L->cap = nt;
+  }
+  if (train && nt > L->cap_train) {
+    dfree(L->h3); dfree(L->d3); dfree(L->d2); dfree(L->d1); dfree(L->go);
+    HIP_TRY(L, dalloc(&L->h3, nt * kTileVol * 16));
+    HIP_TRY(L, dalloc(&L->d3, nt * kTileVol * 16));
+    HIP_TRY(L, dalloc(&L->d2, nt * kTileVol * 32));
+    HIP_TRY(L, dalloc(&L->d1, nt * kTileVol * 16));
+    HIP_TRY(L, dalloc(&L->go, nt * kTileVol));
+    L->cap_train = nt;
+  }
+  const long long need = std::max<long long>((long long)kLpcWgradWGs * kNW2, 1024LL * 16);
+  if (need > L->part_cap) {
+    dfree(L->part);
+    HIP_TRY(L, dalloc(&L->part, need));
+    L->part_cap = need;
+  }
+  return PF_OK;
+}
+
+int lpc_prep(pf_lpc* L, hipStream_t s) {
+  if (L->prepped) return PF_OK;
+  hipLaunchKernelGGL(k_lpc_prep, dim3((32 * kpad(16) + 255) / 256 + 1), dim3(256), 0, s, L->w, L->bf);
+  HIP_TRY(L, hipGetLastError());
+  L->prepped = true;
+  return PF_OK;
+}
+
+// o (and h1, h2, optionally h3) from L->X for nt tiles
+int lpc_forward(pf_lpc* L, hipStream_t s, long long nt, bool store_h3, const int* done) {
+  int st = lpc_prep(L, s);
+  if (st) return st;
+  const dim3 grid((unsigned)(nt * 4));
+  ConvArgs a{};
+  a.ntiles = nt;
+  a.done = done;
+  a.in = L->X; a.wt = L->bf.w1f; a.out = L->h1;
+  hipLaunchKernelGGL((k_lpc_conv<4, 16, true, EPI_RELU>), grid, dim3(256), 0, s, a);
+  a.in = L->h1; a.wt = L->bf.w2f; a.out = L->h2;
+  hipLaunchKernelGGL((k_lpc_conv<16, 32, false, EPI_RELU>), grid, dim3(256), 0, s, a);
+  a.in = L->h2; a.wt = L->bf.w3f; a.out = store_h3 ? L->h3 : nullptr; a.wh = L->bf.wh; a.o = L->o;
+  hipLaunchKernelGGL((k_lpc_conv<32, 16, false, EPI_HEAD>), grid, dim3(256), 0, s, a);
+  HIP_TRY(L, hipGetLastError());
+  return PF_OK;
+}
+
+template <int CIN, int COUT, bool PADDED>
+int lpc_wgrad(pf_lpc* L, hipStream_t s, const __bf16* in, const __bf16* g, long long nt, float* dst) {
+  const int nwg = (int)std::min<long long>(kLpcWgradWGs, nt * 16);
+  hipLaunchKernelGGL((k_lpc_wgrad<CIN, COUT, PADDED>), dim3(nwg), dim3(256), 0, s, in, g, nt, L->part);
+  const int n = 27 * CIN * COUT;
+  hipLaunchKernelGGL(k_lpc_sum_parts, dim3((n + 255) / 256), dim3(256), 0, s, L->part, nwg, n, dst);
+  HIP_TRY(L, hipGetLastError());
+  return PF_OK;
+}
+
+// L->grad = d(sum o * go)/dw, from the activations of the last lpc_forward(store_h3)
+int lpc_backward(pf_lpc* L, hipStream_t s, long long nt) {
+  const long long ncells = nt * kTileVol;
+  const int hg = (int)std::min<long long>(1024, (ncells + 15) / 16);
+  hipLaunchKernelGGL(k_lpc_head_bwd, dim3(hg), dim3(256), 0, s, L->h3, L->go, L->bf.wh, ncells, L->d3, L->part);
+  hipLaunchKernelGGL(k_lpc_sum_parts, dim3(1), dim3(256), 0, s, L->part, hg, kNWH, L->grad + kOffWH);
+  const dim3 grid((unsigned)(nt * 4));
+  ConvArgs a{};
+  a.ntiles = nt;
+  a.in = L->d3; a.wt = L->bf.w3b; a.out = L->d2; a.mask = L->h2;
+  hipLaunchKernelGGL((k_lpc_conv<16, 32, false, EPI_MASK>), grid, dim3(256), 0, s, a);
+  a.in = L->d2; a.wt = L->bf.w2b; a.out = L->d1; a.mask = L->h1;
+  hipLaunchKernelGGL((k_lpc_conv<32, 16, false, EPI_MASK>), grid, dim3(256), 0, s, a);
+  HIP_TRY(L, hipGetLastError());
+  int st = lpc_wgrad<32, 16, false>(L, s, L->h2, L->d3, nt, L->grad + kOffW3);
+  if (!st) st = lpc_wgrad<16, 32, false>(L, s, L->h1, L->d2, nt, L->grad + kOffW2);
+  if (!st) st = lpc_wgrad<4, 16, true>(L, s, L->X, L->d1, nt, L->grad + kOffW1);
+  return st;
+}
+
+LpcTiles lpc_tiles_args(pf_ctx* c, const float* r) {
+  LpcTiles T{};
+  T.dc = c->dc; T.g = c->geom; T.ntx = c->ntx; T.nty = c->nty; T.ntz = c->ntz;
+  T.tiles = c->tiles; T.ntiles = c->ntiles;
+  T.marker = c->marker; T.r = r; T.invd = c->invd32; T.fbeta = c->fbeta; T.fphi = c->fphi;
+  T.scale = c->lpc->scale; T.X = c->lpc->X;
+  return T;
+}
+
+unsigned lpc_scatter_grid(long long nt) { return (unsigned)std::max<long long>(1, std::min<long long>(nt, 1024)); }
+
+// z = M(r) on the context's stream for the last assembled system; partial r.z in
+// part[slot] (gridDim = lpc_scatter_grid); skipped while *done
+int lpc_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part, int slot) {
+  pf_lpc* L = c->lpc;
+  int st = lpc_grow(L, c->ntiles, false);
+  if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  LpcTiles T = lpc_tiles_args(c, r);
+  T.done = done;
+  hipLaunchKernelGGL(k_lpc_gather, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, T);
+  st = lpc_forward(L, c->stream, c->ntiles, false, done);
+  if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  hipLaunchKernelGGL(k_lpc_scatter, dim3(lpc_scatter_grid(c->ntiles)), dim3(256), 0, c->stream, T, L->o, z, part, slot);
+  LAUNCH_CHECK(c);
+  return PF_OK;
+}
+
+// 16^3 tiles holding the current fluid-block list (c->pblk, npblk blocks)
+int lpc_build_tiles(pf_ctx* c) {
+  c->ntx = (c->dc.nbx + 3) / 4; c->nty = (c->dc.nby + 3) / 4; c->ntz = (c->dc.nbz + 3) / 4;
+  const long long nall = (long long)c->ntx * c->nty * c->ntz;
+  if (nall > c->ntiles_all) {
+    dfree(c->tiles); dfree(c->tile_flags);
+    HIP_TRY(c, dalloc(&c->tiles, nall));
+    HIP_TRY(c, dalloc(&c->tile_flags, nall));
+    c->ntiles_all = nall;
+  }
+  HIP_TRY(c, fill_u8(c->stream, c->tile_flags, 0, nall));
+  hipLaunchKernelGGL(k_lpc_tile_flags, dim3(grid_for(c->npblk, 256)), dim3(256), 0, c->stream, c->pblk, c->npblk, c->dc,
+                     c->ntx, c->nty, c->tile_flags);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, compact_flags(*c, c->tile_flags, nall, c->tiles, c->u32_dev + 8));
+  HIP_TRY(c, hipMemcpyAsync(c->u32_host + 8, c->u32_dev + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->ntiles = c->u32_host[8];
+  return PF_OK;
+}
+
+int lpc_features_alloc(pf_ctx* c) {
+  if (c->fbeta) return PF_OK;
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  HIP_TRY(c, dalloc(&c->fbeta, ncell));
+  HIP_TRY(c, dalloc(&c->fphi, ncell));
+  return PF_OK;
+}
+
 int read_small(pf_ctx* c) {
   HIP_TRY(c, hipMemcpyAsync(c->red_host, c->red, sizeof(double) * 16, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipMemcpyAsync(c->u32_host, c->u32_dev, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, c->stream));
@@ -332,6 +504,19 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     if (out) *out = S;
     return PF_OK;
   }
+  // learned preconditioner: tiles and features of this system (for the solve or for
+  // recording training snapshots)
+  const bool want_learned = cfg->precond == PF_PRECOND_LEARNED;
+  if (want_learned && !c->lpc) return fail(c, PF_ERR_STATE, "precond = learned but no network attached (pf_lpc_attach)");
+  const bool lpc_system = c->lpc && (want_learned || c->snap_cap > 0);
+  c->tiles_valid = false;
+  c->nsnap = 0;
+  c->snap_last_iter = -1;
+  if (lpc_system) {
+    int s2 = lpc_features_alloc(c);
+    if (!s2) s2 = lpc_build_tiles(c);
+    if (s2) return s2;
+  }
   ProjArgs pa;
   pa.g = g; pa.dc = c->dc; pa.du = c->du; pa.dv = c->dv; pa.dw = c->dw;
   pa.marker = c->marker; pa.list = c->pblk; pa.nlist = npb;
@@ -349,6 +534,8 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   ao.u = c->f[F_U]; ao.v = c->f[F_V]; ao.w = c->f[F_W];
   ao.mu = c->f[F_MU]; ao.mv = c->f[F_MV]; ao.mw = c->f[F_MW];
   ao.part = c->partial;
+  ao.fbeta = lpc_system ? c->fbeta : nullptr;
+  ao.fphi = lpc_system ? c->fphi : nullptr;
   const unsigned G = list_grid(npb);
   TIMED(c, PF_K_ASSEMBLE, hipLaunchKernelGGL(k_assemble, dim3(G), dim3(256), 0, c->stream, pa, ao));
   LAUNCH_CHECK(c);
@@ -394,19 +581,50 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const int between = env_between ? std::max(8, std::atoi(env_between)) : 200;
   int total = 0, outer = 0;
   bool converged = met();
+  if (lpc_system) c->tiles_valid = true;
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  // residual snapshot (training data), at most one per `snap_every` iterations
+  auto snapshot = [&](int iter) -> int {
+    if (c->snap_cap <= 0 || c->nsnap >= c->snap_cap) return PF_OK;
+    if (c->snap_last_iter >= 0 && iter - c->snap_last_iter < c->snap_every) return PF_OK;
+    HIP_TRY(c, hipMemcpyAsync(c->snaps + (long long)c->nsnap * ncell, c->r32, sizeof(float) * (size_t)ncell,
+                              hipMemcpyDeviceToDevice, c->stream));
+    c->nsnap++;
+    c->snap_last_iter = iter;
+    return PF_OK;
+  };
+  bool learned = want_learned;
+  const unsigned GT = learned ? lpc_scatter_grid(c->ntiles) : 1u;
+  int* done_dev = &c->sc->done;
   if (!converged && max_iters > 0 && std::isfinite(rmax)) {
-    hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32, c->pA,
-                       c->invd32);
-    hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
-                       tol2_abs, max_iters, between);
-    LAUNCH_CHECK(c);
+    // (re)start: r32 from r64, x32 = 0, p = 0, z = M(r)
+    auto start = [&](int budget) -> int {
+      hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32, c->pA,
+                         c->invd32, learned ? 1 : 0);
+      LAUNCH_CHECK(c);
+      if (learned) {
+        int s2 = lpc_apply(c, c->r32, c->z32, nullptr, c->partial, 2);
+        if (s2) return s2;
+        hipLaunchKernelGGL(k_pcg_start_flex, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, (int)GT, c->sc,
+                           delta * delta, tol2_abs, budget, between);
+      } else {
+        hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
+                           tol2_abs, budget, between);
+      }
+      LAUNCH_CHECK(c);
+      return PF_OK;
+    };
+    st = start(max_iters);
+    if (st) return st;
+    st = snapshot(0);
+    if (st) return st;
     // the two iteration kernels use at most 1024 workgroups, so the per-iteration
     // scalar kernels sum few partials
     const unsigned GS = std::min(G, 1024u);
     VecArgs vs = va;
-    int iter_before = 0;
-    double best_rr = rr;
-    int no_gain = 0;
+    int iter_before = 0, iter_base = 0;
+    double best_rr = rr, prev_rr = rr;
+    int no_gain = 0, learned_slow = 0;
     for (;;) {
       const size_t rec0 = c->krec.size();
       for (int q = 0; q < check; ++q) {
@@ -414,10 +632,21 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
               hipLaunchKernelGGL(k_pcg_spmv, dim3(GS), dim3(256), 0, c->stream, vs, c->z32, c->pA, c->pB, c->Ap,
                                  c->cd32, c->cx32, c->cy32, c->cz32));
         hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, c->sc);
-        TIMED(c, PF_K_PCG_UPDATE,
-              hipLaunchKernelGGL(k_pcg_update, dim3(GS), dim3(256), 0, c->stream, vs, c->x32, c->r32, c->z32, c->pA,
-                                 c->pB, c->Ap, c->invd32));
-        hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, c->sc);
+        if (learned) {
+          TIMED(c, PF_K_PCG_UPDATE,
+                hipLaunchKernelGGL(k_pcg_update_flex, dim3(GS), dim3(256), 0, c->stream, vs, c->x32, c->r32, c->z32,
+                                   c->pA, c->pB, c->Ap));
+          long long ki = kbegin(c, PF_K_LPC_APPLY);
+          st = lpc_apply(c, c->r32, c->z32, done_dev, c->partial, 2);
+          kend(c, ki);
+          if (st) return st;
+          hipLaunchKernelGGL(k_pcg_beta_flex, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, (int)GT, c->sc);
+        } else {
+          TIMED(c, PF_K_PCG_UPDATE,
+                hipLaunchKernelGGL(k_pcg_update, dim3(GS), dim3(256), 0, c->stream, vs, c->x32, c->r32, c->z32, c->pA,
+                                   c->pB, c->Ap, c->invd32));
+          hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, c->sc);
+        }
       }
       LAUNCH_CHECK(c);
       HIP_TRY(c, hipMemcpyAsync(c->done_host, &c->sc->iter, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -426,14 +655,19 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
       // launches after a pause return at once: keep the timings of those that ran
       const int ran = c->done_host[0] - iter_before;
       iter_before = c->done_host[0];
-      int kept_s = 0, kept_u = 0;
+      int kept_s = 0, kept_u = 0, kept_l = 0;
       for (size_t r = rec0; r < c->krec.size(); ++r) {
         int& k = c->krec[r].first;
         if (k == PF_K_PCG_SPMV && kept_s++ >= ran) k = -1;
         if (k == PF_K_PCG_UPDATE && kept_u++ >= ran) k = -1;
+        if (k == PF_K_LPC_APPLY && kept_l++ >= ran) k = -1;
+      }
+      if (ran > 0) {
+        st = snapshot(iter_base + c->done_host[0]);
+        if (st) return st;
       }
       if (!c->done_host[1]) continue;
-      total = c->done_host[0];
+      total = iter_base + c->done_host[0];
       const bool breakdown = c->done_host[2] != 0;
       // residual replacement: x64 += x32, r64 = b - A x64 in FP64
       hipLaunchKernelGGL(k_ir_accum, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->x32);
@@ -447,9 +681,9 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
       if (trace) {
         double s_rr32 = 0.0;
         HIP_TRY(c, hipMemcpy(&s_rr32, &c->sc->rr, sizeof(double), hipMemcpyDeviceToHost));
-        std::fprintf(stderr, "[pf solve] update %d iter %d fp32 rel %.3e true rel %.3e max %.3e%s\n", outer, total,
+        std::fprintf(stderr, "[pf solve] update %d iter %d fp32 rel %.3e true rel %.3e max %.3e%s%s\n", outer, total,
                      std::sqrt(s_rr32 / std::max(bnorm2, 1e-300)), std::sqrt(rr / std::max(bnorm2, 1e-300)), rmax,
-                     breakdown ? " breakdown" : "");
+                     breakdown ? " breakdown" : "", learned ? " (learned)" : "");
       }
       if (rr < best_rr * 0.999) { best_rr = rr; no_gain = 0; } else { ++no_gain; }
       // (no gain over 40 replacements: the solve has stagnated)
@@ -457,16 +691,45 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
         if (breakdown) S.breakdown = 1;
         break;
       }
+      // the learned preconditioner falls back to Jacobi when two replacements in a
+      // row gained less than 10% (or the recursion broke down)
+      if (learned) {
+        learned_slow = (rr > 0.9 * prev_rr || breakdown) ? learned_slow + 1 : 0;
+        if (learned_slow >= 2 || breakdown) {
+          learned = false;
+          S.fallback_used = 1;
+          iter_base = total;
+          iter_before = 0;
+          prev_rr = rr;
+          st = start(max_iters - total);
+          if (st) return st;
+          continue;
+        }
+      }
+      prev_rr = rr;
       // resume from the true residual; after a breakdown restart the direction too
       hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32,
-                         breakdown ? c->pA : nullptr, c->invd32);
-      if (breakdown)
-        hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
-                           tol2_abs, max_iters - total, between);
-      else
-        hipLaunchKernelGGL(k_pcg_resume, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
+                         breakdown ? c->pA : nullptr, c->invd32, learned ? 1 : 0);
       LAUNCH_CHECK(c);
-      if (breakdown) { iter_before = 0; }
+      if (learned) {
+        st = lpc_apply(c, c->r32, c->z32, nullptr, c->partial, 2);
+        if (st) return st;
+      }
+      if (breakdown) {
+        if (learned)
+          hipLaunchKernelGGL(k_pcg_start_flex, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, (int)GT, c->sc,
+                             delta * delta, tol2_abs, max_iters - total, between);
+        else
+          hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc, delta * delta,
+                             tol2_abs, max_iters - total, between);
+        iter_base = total;
+        iter_before = 0;
+      } else if (learned) {
+        hipLaunchKernelGGL(k_pcg_resume_flex, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, (int)GT, c->sc);
+      } else {
+        hipLaunchKernelGGL(k_pcg_resume, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, c->sc);
+      }
+      LAUNCH_CHECK(c);
     }
   }
   S.iterations = total;
@@ -725,6 +988,7 @@ int pf_destroy(pf_ctx* c) {
   for (float* p : f32) hipFree(p);
   double* f64[] = {c->b64, c->x64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64};
   for (double* p : f64) hipFree(p);
+  dfree(c->fbeta); dfree(c->fphi); dfree(c->tiles); dfree(c->tile_flags); dfree(c->snaps);
   hipFree(c->sc); hipFree(c->partial); hipFree(c->red);
   hipHostFree(c->red_host); hipHostFree(c->done_host); hipHostFree(c->u32_host);
   hipFree(c->u32_dev); hipFree(c->esc_dev); hipFree(c->vmax_dev);
@@ -1054,5 +1318,249 @@ double pf_phi_from_raw(double rt, const pf_phase* pp) {
 }
 
 double pf_beta_from_phi(double phi, const pf_phase* pp) { return 1.0 / (phi * pp->rho_l + (1.0 - phi) * pp->rho_g); }
+
+// ---------------------------------------------------------------- learned preconditioner
+
+// splitmix64 stream and Box-Muller normals for the He initialisation
+static double lpc_uniform(uint64_t& st) {
+  uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+int pf_lpc_create(pf_lpc** out, uint64_t seed) {
+  if (!out) return PF_ERR_ARG;
+  *out = nullptr;
+  pf_lpc* L = new pf_lpc();
+  HIP_TRY(L, hipGetDevice(&L->device));
+  HIP_TRY(L, hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+  HIP_TRY(L, dalloc(&L->w, kNW));
+  HIP_TRY(L, dalloc(&L->m, kNW));
+  HIP_TRY(L, dalloc(&L->v, kNW));
+  HIP_TRY(L, dalloc(&L->grad, kNW));
+  HIP_TRY(L, dalloc(&L->bf.w1f, 16 * kpad(4)));
+  HIP_TRY(L, dalloc(&L->bf.w1b, 4 * kpad(16)));
+  HIP_TRY(L, dalloc(&L->bf.w2f, 32 * kpad(16)));
+  HIP_TRY(L, dalloc(&L->bf.w2b, 16 * kpad(32)));
+  HIP_TRY(L, dalloc(&L->bf.w3f, 16 * kpad(32)));
+  HIP_TRY(L, dalloc(&L->bf.w3b, 32 * kpad(16)));
+  HIP_TRY(L, dalloc(&L->bf.wh, 16));
+  // He initialisation: N(0, 2 / fan_in) for the ReLU layers, N(0, 1/16) for the head
+  std::vector<float> w(kNW);
+  uint64_t st = seed;
+  const int off[4] = {kOffW1, kOffW2, kOffW3, kOffWH};
+  const int cnt[4] = {kNW1, kNW2, kNW3, kNWH};
+  const double sd[4] = {std::sqrt(2.0 / (27 * 4)), std::sqrt(2.0 / (27 * 16)), std::sqrt(2.0 / (27 * 32)),
+                        std::sqrt(1.0 / 16)};
+  for (int l = 0; l < 4; ++l)
+    for (int i = 0; i < cnt[l]; ++i) {
+      double u1 = lpc_uniform(st), u2 = lpc_uniform(st);
+      w[off[l] + i] = (float)(sd[l] * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+    }
+  HIP_TRY(L, hipMemcpy(L->w, w.data(), sizeof(float) * kNW, hipMemcpyHostToDevice));
+  *out = L;
+  return PF_OK;
+}
+
+int pf_lpc_destroy(pf_lpc* L) {
+  if (!L) return PF_OK;
+  hipStreamSynchronize(L->stream);
+  dfree(L->w); dfree(L->m); dfree(L->v); dfree(L->grad);
+  dfree(L->bf.w1f); dfree(L->bf.w1b); dfree(L->bf.w2f); dfree(L->bf.w2b); dfree(L->bf.w3f); dfree(L->bf.w3b);
+  dfree(L->bf.wh);
+  dfree(L->X); dfree(L->h1); dfree(L->h2); dfree(L->h3); dfree(L->d3); dfree(L->d2); dfree(L->d1);
+  dfree(L->o); dfree(L->go); dfree(L->scale); dfree(L->part);
+  dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz); dfree(L->stage);
+  hipStreamDestroy(L->stream);
+  delete L;
+  return PF_OK;
+}
+
+const char* pf_lpc_last_error(const pf_lpc* L) { return L ? L->err.c_str() : "null network"; }
+
+int64_t pf_lpc_num_weights(void) { return kNW; }
+
+int pf_lpc_get_weights(pf_lpc* L, float* w, int64_t n) {
+  if (!L || !w || n != kNW) return PF_ERR_ARG;
+  HIP_TRY(L, hipStreamSynchronize(L->stream));
+  HIP_TRY(L, hipMemcpy(w, L->w, sizeof(float) * kNW, hipMemcpyDeviceToHost));
+  return PF_OK;
+}
+
+int pf_lpc_set_weights(pf_lpc* L, const float* w, int64_t n) {
+  if (!L || !w || n != kNW) return PF_ERR_ARG;
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(w[i])) { L->err = "non-finite weight"; return PF_ERR_ARG; }
+  HIP_TRY(L, hipStreamSynchronize(L->stream));
+  HIP_TRY(L, hipMemcpy(L->w, w, sizeof(float) * kNW, hipMemcpyHostToDevice));
+  HIP_TRY(L, hipMemset(L->m, 0, sizeof(float) * kNW));
+  HIP_TRY(L, hipMemset(L->v, 0, sizeof(float) * kNW));
+  L->adam_t = 0;
+  L->prepped = false;
+  return PF_OK;
+}
+
+static int lpc_stage_input(pf_lpc* L, int64_t nt, const float* X) {
+  const long long n = nt * (long long)kHaloVol * 4;
+  if (n > L->stage_cap) {
+    dfree(L->stage);
+    HIP_TRY(L, dalloc(&L->stage, n));
+    L->stage_cap = n;
+  }
+  HIP_TRY(L, hipMemcpyAsync(L->stage, X, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, L->stream));
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)std::min<long long>((n + 255) / 256, 65536)), dim3(256), 0,
+                     L->stream, L->stage, L->X, n);
+  HIP_TRY(L, hipGetLastError());
+  return PF_OK;
+}
+
+int pf_lpc_forward_tiles(pf_lpc* L, int64_t nt, const float* X, float* O) {
+  if (!L || !X || !O || nt <= 0) return PF_ERR_ARG;
+  int st = lpc_grow(L, nt, false);
+  if (!st) st = lpc_stage_input(L, nt, X);
+  if (!st) st = lpc_forward(L, L->stream, nt, false, nullptr);
+  if (st) return st;
+  HIP_TRY(L, hipMemcpyAsync(O, L->o, sizeof(float) * (size_t)(nt * kTileVol), hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(L, hipStreamSynchronize(L->stream));
+  return PF_OK;
+}
+
+int pf_lpc_grad_tiles(pf_lpc* L, int64_t nt, const float* X, const float* gO, float* gW) {
+  if (!L || !X || !gO || !gW || nt <= 0) return PF_ERR_ARG;
+  int st = lpc_grow(L, nt, true);
+  if (!st) st = lpc_stage_input(L, nt, X);
+  if (!st) st = lpc_forward(L, L->stream, nt, true, nullptr);
+  if (st) return st;
+  HIP_TRY(L, hipMemcpyAsync(L->go, gO, sizeof(float) * (size_t)(nt * kTileVol), hipMemcpyHostToDevice, L->stream));
+  st = lpc_backward(L, L->stream, nt);
+  if (st) return st;
+  HIP_TRY(L, hipMemcpyAsync(gW, L->grad, sizeof(float) * kNW, hipMemcpyDeviceToHost, L->stream));
+  HIP_TRY(L, hipStreamSynchronize(L->stream));
+  return PF_OK;
+}
+
+int pf_lpc_attach(pf_ctx* c, pf_lpc* L) {
+  if (!c) return PF_ERR_ARG;
+  if (L && L->device != c->device) return fail(c, PF_ERR_ARG, "network and context are on different devices");
+  if (L) HIP_TRY(c, hipStreamSynchronize(L->stream));
+  c->lpc = L;
+  c->tiles_valid = false;
+  return PF_OK;
+}
+
+int pf_lpc_record(pf_ctx* c, int32_t max_snapshots, int32_t every) {
+  if (!c || max_snapshots < 0 || every < 0) return PF_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  dfree(c->snaps);
+  c->snap_cap = 0;
+  c->nsnap = 0;
+  if (max_snapshots == 0) return PF_OK;
+  HIP_TRY(c, dalloc(&c->snaps, (long long)max_snapshots * c->dc.nblocks * kBlockVol));
+  c->snap_cap = max_snapshots;
+  c->snap_every = std::max(every, 1);
+  return PF_OK;
+}
+
+int32_t pf_lpc_snapshot_count(const pf_ctx* c) { return c ? c->nsnap : 0; }
+
+// loss ||r - A z||^2 / ||r||^2 of z = M(r) (learned) or z = D^-1 r for snapshot k;
+// leaves z in L->zbuf, q = r - A z in L->qbuf, ||r||^2 in *rr
+static int lpc_loss_eval(pf_lpc* L, pf_ctx* c, int k, bool learned, bool store, double* loss, double* rr) {
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  if (ncell > L->cell_cap) {
+    dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz);
+    HIP_TRY(L, dalloc(&L->zbuf, ncell));
+    HIP_TRY(L, dalloc(&L->qbuf, ncell));
+    HIP_TRY(L, dalloc(&L->gz, ncell));
+    L->cell_cap = ncell;
+  }
+  int st = lpc_grow(L, c->ntiles, store);
+  if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  const float* r = c->snaps + (long long)k * ncell;
+  LpcTiles T = lpc_tiles_args(c, r);
+  hipLaunchKernelGGL(k_lpc_gather, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, T);
+  if (learned) {
+    st = lpc_forward(L, c->stream, c->ntiles, store, nullptr);
+    if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  }
+  hipLaunchKernelGGL(k_lpc_scatter, dim3(lpc_scatter_grid(c->ntiles)), dim3(256), 0, c->stream, T,
+                     learned ? L->o : nullptr, L->zbuf, nullptr, 0);
+  VecArgs va{};
+  va.dc = c->dc; va.list = c->pblk_prev; va.nlist = c->npblk_prev; va.sc = c->sc; va.part = c->partial;
+  const unsigned G = list_grid(c->npblk_prev);
+  hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->zbuf, r, L->qbuf, c->cd32, c->cx32,
+                     c->cy32, c->cz32, 1.f);
+  hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 2, 0u, c->red);
+  LAUNCH_CHECK(c);
+  st = read_small(c);
+  if (st) return st;
+  *rr = c->red_host[1];
+  *loss = *rr > 0.0 ? c->red_host[0] / *rr : 0.0;
+  return PF_OK;
+}
+
+static int lpc_check_system(pf_lpc* L, pf_ctx* c) {
+  if (!L || !c) return PF_ERR_ARG;
+  if (c->lpc != L) return fail(c, PF_ERR_STATE, "network is not attached to this context");
+  if (!c->tiles_valid || c->nsnap <= 0)
+    return fail(c, PF_ERR_STATE, "no recorded system: pf_lpc_record, then solve with the network attached");
+  HIP_TRY(c, hipStreamSynchronize(L->stream));
+  return PF_OK;
+}
+
+int pf_lpc_loss(pf_lpc* L, pf_ctx* c, int32_t k, double* loss_learned, double* loss_jacobi) {
+  int st = lpc_check_system(L, c);
+  if (st) return st;
+  if (k < 0 || k >= c->nsnap) return fail(c, PF_ERR_ARG, "snapshot index out of range");
+  double rr = 0.0, a = 0.0, b = 0.0;
+  st = lpc_loss_eval(L, c, k, true, false, &a, &rr);
+  if (!st) st = lpc_loss_eval(L, c, k, false, false, &b, &rr);
+  if (st) return st;
+  if (loss_learned) *loss_learned = a;
+  if (loss_jacobi) *loss_jacobi = b;
+  return PF_OK;
+}
+
+int pf_lpc_train(pf_lpc* L, pf_ctx* c, int32_t steps, double lr, uint64_t seed, double* mean_loss) {
+  int st = lpc_check_system(L, c);
+  if (st) return st;
+  if (steps < 0 || !(lr > 0.0)) return fail(c, PF_ERR_ARG, "steps >= 0 and lr > 0 required");
+  uint64_t rs = seed;
+  double lsum = 0.0;
+  for (int it = 0; it < steps; ++it) {
+    const int k = (int)(lpc_uniform(rs) * c->nsnap) % c->nsnap;
+    double loss = 0.0, rr = 0.0;
+    st = lpc_loss_eval(L, c, k, true, true, &loss, &rr);
+    if (st) return st;
+    lsum += loss;
+    if (!(rr > 0.0)) continue;
+    // dL/dz = -2 A q / ||r||^2 (A symmetric), then through z = r/diag + s * o
+    VecArgs va{};
+    va.dc = c->dc; va.list = c->pblk_prev; va.nlist = c->npblk_prev; va.sc = c->sc; va.part = c->partial;
+    const unsigned G = list_grid(c->npblk_prev);
+    hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->qbuf, nullptr, L->gz, c->cd32,
+                       c->cx32, c->cy32, c->cz32, (float)(-2.0 / rr));
+    LpcTiles T = lpc_tiles_args(c, c->snaps + (long long)k * c->dc.nblocks * kBlockVol);
+    hipLaunchKernelGGL(k_lpc_gz_to_tiles, dim3(lpc_scatter_grid(c->ntiles)), dim3(256), 0, c->stream, T, L->gz,
+                       L->go);
+    LAUNCH_CHECK(c);
+    st = lpc_backward(L, c->stream, c->ntiles);
+    if (st) return fail(c, st, "learned preconditioner: " + L->err);
+    L->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float bc1 = 1.f - std::pow(b1, (float)L->adam_t), bc2 = 1.f - std::pow(b2, (float)L->adam_t);
+    hipLaunchKernelGGL(k_lpc_adam, dim3((kNW + 255) / 256), dim3(256), 0, c->stream, L->w, L->m, L->v, L->grad, kNW,
+                       (float)lr, b1, b2, 1e-8f, bc1, bc2);
+    LAUNCH_CHECK(c);
+    L->prepped = false;
+    st = lpc_prep(L, c->stream);
+    if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (mean_loss) *mean_loss = steps > 0 ? lsum / steps : 0.0;
+  return PF_OK;
+}
 
 }  // extern "C"

@@ -23,6 +23,8 @@
 
 #include "pf_gpu.h"
 
+struct pf_lpc;
+
 namespace pf {
 
 constexpr int kBlock = 4;
@@ -87,7 +89,7 @@ enum { CELL_AIR = 0, CELL_FLUID = 1, CELL_SOLID = 2 };
 struct SolveScalars {
   double rz, pAp, alpha, beta;
   double rr, rr0, rr_best, tol2;     // tol2 = delta^2 (replacement trigger)
-  double rr_ref, tol2_abs, rz_pending, pad0;
+  double rr_ref, tol2_abs, rz_pending, rz_cross;  // rz_cross: r_new . z_old at a pause (flexible beta)
   int iter, done, breakdown, stalled;
   int iter_best, max_iter, stall_window, upd;
 };
@@ -151,6 +153,19 @@ struct Ctx {
   long long stage_cap = 0;
 
   hipEvent_t ev[PF_NSTAGES + 1] = {};
+
+  // learned preconditioner (attached, not owned) and its per-system data
+  ::pf_lpc* lpc = nullptr;
+  float* fbeta = nullptr;           // cell beta feature (mean face beta) on FLUID cells
+  float* fphi = nullptr;            // cell phi feature (mean face phi) on FLUID cells
+  uint32_t* tiles = nullptr;        // 16^3 tiles holding FLUID blocks (sorted ids)
+  uint8_t* tile_flags = nullptr;
+  long long ntiles = 0, ntiles_all = 0;
+  int ntx = 0, nty = 0, ntz = 0;
+  bool tiles_valid = false;         // tiles/features describe the last assembled system
+  // residual snapshots recorded during solves (training data)
+  float* snaps = nullptr;
+  int snap_cap = 0, nsnap = 0, snap_every = 0, snap_last_iter = -1;
 
   // per-kernel event timing (measurement hook)
   bool ktime = false;

@@ -24,18 +24,20 @@ struct PhaseK {
   double rho_l, rho_g, rmin, den;  // rmin = ln(rho_l/rho_g)*rho_g*rho_tilde_0, den = alpha*rho_tilde_0*rho_l
 };
 
+// phiFromRawDensity, src/physics/phasefield.h:10-14
+__device__ inline double phi_of_raw(double raw, const PhaseK& k) {
+#pragma clang fp contract(off)
+  if (raw < k.rmin) return 0.0;
+  double d = raw - k.rmin;
+  if (d < 0.0) d = 0.0;
+  double v = sqrt(d / k.den);
+  return v < 1.0 ? v : 1.0;
+}
+
 // betaOfRaw, sparse_ops3d_tp.cpp:89-91 with phiFromRawDensity / betaFromPhi
 __device__ inline double beta_of_raw(double raw, const PhaseK& k) {
 #pragma clang fp contract(off)
-  double phi;
-  if (raw < k.rmin) {
-    phi = 0.0;
-  } else {
-    double d = raw - k.rmin;
-    if (d < 0.0) d = 0.0;
-    double v = sqrt(d / k.den);
-    phi = v < 1.0 ? v : 1.0;
-  }
+  const double phi = phi_of_raw(raw, k);
   return 1.0 / (phi * k.rho_l + (1.0 - phi) * k.rho_g);
 }
 
@@ -228,6 +230,7 @@ struct AsmOut {
   float *cd32, *cx32, *cy32, *cz32, *invd32;
   const float *u, *v, *w, *mu, *mv, *mw;
   double* part;   // slot 0: sum b^2
+  float *fbeta, *fphi;  // optional learned-preconditioner features (mean face beta / phi)
 };
 
 // RHS and assembly, sparse_ops3d_tp.cpp:266-313
@@ -306,6 +309,30 @@ __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
     o.cd64[c] = cd; o.cx64[c] = cx; o.cy64[c] = cy; o.cz64[c] = cz;
     o.cd32[c] = (float)cd; o.cx32[c] = (float)cx; o.cy32[c] = (float)cy; o.cz32[c] = (float)cz;
     o.invd32[c] = (float)inv;
+    if (o.fbeta) {
+      // cell features of the learned preconditioner: mean beta and mean phi over the
+      // six faces (the cell phi of interface_diagnostics3d.cpp:137-148)
+      float fb_mean = 0.f, fp_mean = 0.f;
+      if (fl) {
+        if (a.single) {
+          fb_mean = 1.f;
+          fp_mean = 1.f;
+        } else {
+          const double raw[6] = {o.mu[bm_index(a.du, i + 1, j, k)], o.mu[bm_index(a.du, i, j, k)],
+                                 o.mv[bm_index(a.dv, i, j + 1, k)], o.mv[bm_index(a.dv, i, j, k)],
+                                 o.mw[bm_index(a.dw, i, j, k + 1)], o.mw[bm_index(a.dw, i, j, k)]};
+          double sb = 0.0, sp = 0.0;
+          for (int q = 0; q < 6; ++q) {
+            sp += phi_of_raw(raw[q], a.ph);
+            sb += beta_of_raw(raw[q], a.ph);
+          }
+          fb_mean = (float)(sb / 6.0);
+          fp_mean = (float)(sp / 6.0);
+        }
+      }
+      o.fbeta[c] = fb_mean;
+      o.fphi[c] = fp_mean;
+    }
     bb += rhs * rhs;
   }
   double v[1] = {bb};
@@ -375,18 +402,20 @@ __global__ void __launch_bounds__(256) k_residual64(VecArgs a, const double* __r
 }
 
 // r32 = (float)r64, x32 = 0, z = D^-1 r (and p_A = 0 on a fresh start);
-// partial r.z, r.r
+// partial r.z, r.r.  keep_z (learned preconditioner): z is left for the network and
+// slot 0 gets r_new . z_old (the flexible beta's correction term)
 __global__ void __launch_bounds__(256) k_pcg_load(VecArgs a, const double* __restrict__ r64,
                                                   float* __restrict__ x32, float* __restrict__ r32,
                                                   float* __restrict__ z32, float* __restrict__ p,
-                                                  const float* __restrict__ invd) {
+                                                  const float* __restrict__ invd, int keep_z = 0) {
   const int lane = threadIdx.x & 63;
   double rz = 0.0, rr = 0.0;
   FOR_LIST_BLOCKS(a, b, lane) {
     long long c = b * kBlockVol + lane;
     float r = (float)r64[c];
-    float z = invd[c] * r;
-    x32[c] = 0.f; r32[c] = r; z32[c] = z;
+    float z = keep_z ? z32[c] : invd[c] * r;
+    x32[c] = 0.f; r32[c] = r;
+    if (!keep_z) z32[c] = z;
     if (p) p[c] = 0.f;
     rz += (double)r * (double)z;
     rr += (double)r * (double)r;
@@ -533,6 +562,82 @@ __global__ void __launch_bounds__(256) k_pcg_beta(const double* part, int n, Sol
   const double rz = block_sum_fast(part, n);
   const double rr = block_sum_fast(part + kMaxPartials, n);
   if (threadIdx.x == 0) finish_iteration(sc, rz, rr);
+}
+
+// Flexible PCG (Notay 2000): with a preconditioner that changes between iterations
+// (the learned network), beta = r_new . (z_new - z_old) / (r_old . z_old); a
+// negative or non-finite beta restarts the direction (beta = 0), as the reference's
+// multires solver does (multires_pressure3d.cpp:1424-1434).
+__device__ inline double flexible_beta(double rz_new, double rz_cross, double rz_old) {
+  const double b = rz_old != 0.0 ? (rz_new - rz_cross) / rz_old : 0.0;
+  return (isfinite(b) && b > 0.0) ? b : 0.0;
+}
+
+// update without z: x += alpha p, r -= alpha Ap; partial r_new . z_old (slot 0), r.r (slot 1)
+__global__ void __launch_bounds__(256) k_pcg_update_flex(VecArgs a, float* __restrict__ x, float* __restrict__ r,
+                                                         const float* __restrict__ z, const float* __restrict__ pA,
+                                                         const float* __restrict__ pB, const float* __restrict__ Ap) {
+  if (a.sc->done) return;
+  const int lane = threadIdx.x & 63;
+  const float* __restrict__ p = (a.sc->iter & 1) ? pA : pB;
+  const float alpha = (float)a.sc->alpha;
+  double rzo = 0.0, rr = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    long long c = b * kBlockVol + lane;
+    float xn = fmaf(alpha, p[c], x[c]);
+    float rn = fmaf(-alpha, Ap[c], r[c]);
+    x[c] = xn; r[c] = rn;
+    rzo += (double)rn * (double)z[c];
+    rr += (double)rn * (double)rn;
+  }
+  double v[2] = {rzo, rr};
+  const bool mx[2] = {false, false};
+  block_reduce_store<2>(v, mx, a.part, kMaxPartials);
+}
+
+// iteration end with the flexible beta: slot 2 (nz partials) r.z_new, slots 0/1 (n)
+__global__ void __launch_bounds__(256) k_pcg_beta_flex(const double* part, int n, int nz, SolveScalars* sc) {
+  if (sc->done) return;
+  const double rzo = block_sum_fast(part, n);
+  const double rr = block_sum_fast(part + kMaxPartials, n);
+  const double rz = block_sum_fast(part + 2 * kMaxPartials, nz);
+  if (threadIdx.x == 0) {
+    const double beta = flexible_beta(rz, rzo, sc->rz);
+    sc->rz_cross = rzo;  // kept for the resume after a residual replacement
+    finish_iteration(sc, rz, rr);
+    if (!sc->done) sc->beta = beta;
+  }
+}
+
+// fresh start / resume for the flexible solve (z from the network, slot 2)
+__global__ void __launch_bounds__(256) k_pcg_start_flex(const double* part, int n, int nz, SolveScalars* sc,
+                                                        double delta2, double tol2_abs, int max_iter, int between) {
+  const double rr = block_sum_partials(part + kMaxPartials, n, false);
+  const double rz = block_sum_partials(part + 2 * kMaxPartials, nz, false);
+  if (threadIdx.x == 0) {
+    sc->rz = rz; sc->rr = rr; sc->rr0 = rr; sc->rr_best = rr; sc->tol2 = delta2;
+    sc->rr_ref = rr; sc->tol2_abs = tol2_abs;
+    sc->beta = 0.0; sc->alpha = 0.0; sc->pAp = 0.0;
+    sc->iter = 0; sc->iter_best = 0; sc->max_iter = max_iter; sc->stall_window = between;
+    sc->breakdown = 0; sc->stalled = 0; sc->upd = 0;
+    sc->done = (rr == 0.0 || max_iter <= 0) ? 1 : 0;
+  }
+}
+
+// The network has already replaced z_old by M(r_recursive) in the paused iteration,
+// so the cross term r_new . z_old is the one formed then with the recursive residual
+// (the replaced residual differs from it by the FP32 drift only).
+__global__ void __launch_bounds__(256) k_pcg_resume_flex(const double* part, int n, int nz, SolveScalars* sc) {
+  const double rzo = sc->rz_cross;
+  const double rr = block_sum_partials(part + kMaxPartials, n, false);
+  const double rz = block_sum_partials(part + 2 * kMaxPartials, nz, false);
+  if (threadIdx.x == 0) {
+    sc->beta = flexible_beta(rz, rzo, sc->rz);
+    sc->rz = rz; sc->rr = rr; sc->rr_ref = rr;
+    sc->iter_best = sc->iter;
+    sc->upd = 0;
+    sc->done = (rr == 0.0 || sc->iter >= sc->max_iter) ? 1 : 0;
+  }
 }
 
 // x64 += x32 (x32 is reset by the next k_pcg_load)

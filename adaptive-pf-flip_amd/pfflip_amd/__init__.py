@@ -24,9 +24,9 @@ STOP_REF_MAXABS, STOP_REL2 = 0, 1
 SCENE_RT, SCENE_BUBBLE, SCENE_DAMBREAK = 0, 1, 2
 STAGES = ("bin", "mark", "p2g", "grid", "assemble", "solve", "velocity", "g2p_advect")
 KERNELS = ("bin_count", "bin_scatter", "p2g", "face_ops", "assemble", "pcg_spmv", "pcg_update",
-           "residual64", "velocity", "g2p_advect")
+           "residual64", "velocity", "g2p_advect", "lpc_apply")
 (K_BIN_COUNT, K_BIN_SCATTER, K_P2G, K_FACE_OPS, K_ASSEMBLE, K_PCG_SPMV, K_PCG_UPDATE,
- K_RESIDUAL64, K_VELOCITY, K_G2P_ADVECT) = range(10)
+ K_RESIDUAL64, K_VELOCITY, K_G2P_ADVECT, K_LPC_APPLY) = range(11)
 
 
 class GridDesc(C.Structure):
@@ -42,7 +42,7 @@ class Phase(C.Structure):
 class SolveCfg(C.Structure):
     _fields_ = [("max_iters", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
                 ("inner_rel_tol", C.c_double), ("max_outer", C.c_int32), ("warm_start", C.c_int32),
-                ("check_every", C.c_int32), ("reserved", C.c_int32)]
+                ("check_every", C.c_int32), ("precond", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -126,6 +126,19 @@ def lib():
             "pf_p2g_single": (C.c_int, [vp]),
             "pf_project_single": (C.c_int, [vp, dbl, C.POINTER(SolveCfg), C.POINTER(SolveStats)]),
             "pf_stream": (vp, [vp]),
+            "pf_lpc_create": (C.c_int, [C.POINTER(vp), C.c_uint64]),
+            "pf_lpc_destroy": (C.c_int, [vp]),
+            "pf_lpc_last_error": (C.c_char_p, [vp]),
+            "pf_lpc_num_weights": (i64, []),
+            "pf_lpc_get_weights": (C.c_int, [vp, vp, i64]),
+            "pf_lpc_set_weights": (C.c_int, [vp, vp, i64]),
+            "pf_lpc_forward_tiles": (C.c_int, [vp, i64, vp, vp]),
+            "pf_lpc_grad_tiles": (C.c_int, [vp, i64, vp, vp, vp]),
+            "pf_lpc_attach": (C.c_int, [vp, vp]),
+            "pf_lpc_record": (C.c_int, [vp, i32, i32]),
+            "pf_lpc_snapshot_count": (i32, [vp]),
+            "pf_lpc_train": (C.c_int, [vp, vp, i32, dbl, C.c_uint64, C.POINTER(dbl)]),
+            "pf_lpc_loss": (C.c_int, [vp, vp, i32, C.POINTER(dbl), C.POINTER(dbl)]),
             "pf_kernel_timing": (C.c_int, [vp, i32]),
             "pf_kernel_time": (C.c_int, [vp, i32, C.POINTER(i64), C.POINTER(dbl)]),
             "pf_copy_probe": (C.c_int, [vp, i64, C.POINTER(dbl)]),
@@ -146,9 +159,12 @@ def calibrate_rho_tilde0(phase: Phase, Vp: float = 1.0) -> float:
     return lib().pf_calibrate_rho_tilde0(C.byref(phase), Vp)
 
 
-def default_solve_cfg(stop_rule=STOP_REF_MAXABS, tol=1e-7, max_iters=600) -> SolveCfg:
+PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+
+
+def default_solve_cfg(stop_rule=STOP_REF_MAXABS, tol=1e-7, max_iters=600, precond=PRECOND_JACOBI) -> SolveCfg:
     return SolveCfg(max_iters=max_iters, stop_rule=stop_rule, tol=tol, inner_rel_tol=0.1,
-                    max_outer=10000, warm_start=0, check_every=8)
+                    max_outer=10000, warm_start=0, check_every=8, precond=precond)
 
 
 class Context:
@@ -407,3 +423,78 @@ class SparseSim3D:
         st = self.ctx.step(self.step_cfg())
         self.stats_last = st
         return st
+
+
+class LearnedPreconditioner:
+    """The learned block preconditioner (a pf_lpc): conv3^3 network 4-16-32-16 plus a
+    1x1x1 head over 16^3 tiles, BF16 on MFMA; trained on the GPU."""
+
+    def __init__(self, seed=0):
+        self.h = C.c_void_p()
+        st = lib().pf_lpc_create(C.byref(self.h), seed)
+        if st != 0:
+            raise PfError(f"pf_lpc_create failed with status {st}")
+
+    def close(self):
+        if self.h:
+            lib().pf_lpc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what, ctx=None):
+        if st != 0:
+            msg = lib().pf_last_error(ctx.h) if ctx is not None else lib().pf_lpc_last_error(self.h)
+            raise PfError(f"{what}: status {st}: {msg.decode() if msg else ''}")
+
+    @staticmethod
+    def num_weights():
+        return int(lib().pf_lpc_num_weights())
+
+    def weights(self):
+        w = np.zeros(self.num_weights(), np.float32)
+        self._check(lib().pf_lpc_get_weights(self.h, _p(w), len(w)), "pf_lpc_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(lib().pf_lpc_set_weights(self.h, _p(w), len(w)), "pf_lpc_set_weights")
+
+    def forward_tiles(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        nt = X.shape[0]
+        O = np.zeros((nt, 16, 16, 16), np.float32)
+        self._check(lib().pf_lpc_forward_tiles(self.h, nt, _p(X), _p(O)), "pf_lpc_forward_tiles")
+        return O
+
+    def grad_tiles(self, X, gO):
+        X = np.ascontiguousarray(X, np.float32)
+        gO = np.ascontiguousarray(gO, np.float32)
+        gW = np.zeros(self.num_weights(), np.float32)
+        self._check(lib().pf_lpc_grad_tiles(self.h, X.shape[0], _p(X), _p(gO), _p(gW)), "pf_lpc_grad_tiles")
+        return gW
+
+    def attach(self, ctx):
+        ctx._check(lib().pf_lpc_attach(ctx.h, self.h), "pf_lpc_attach")
+
+    def train(self, ctx, steps, lr=1e-3, seed=0):
+        loss = C.c_double()
+        self._check(lib().pf_lpc_train(self.h, ctx.h, steps, lr, seed, C.byref(loss)), "pf_lpc_train", ctx)
+        return loss.value
+
+    def loss(self, ctx, k):
+        a, b = C.c_double(), C.c_double()
+        self._check(lib().pf_lpc_loss(self.h, ctx.h, k, C.byref(a), C.byref(b)), "pf_lpc_loss", ctx)
+        return a.value, b.value
+
+
+def lpc_record(ctx, max_snapshots, every=1):
+    ctx._check(lib().pf_lpc_record(ctx.h, max_snapshots, every), "pf_lpc_record")
+
+
+def lpc_snapshot_count(ctx):
+    return int(lib().pf_lpc_snapshot_count(ctx.h))

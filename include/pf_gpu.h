@@ -69,8 +69,14 @@ typedef struct pf_solve_cfg {
   int32_t max_outer;        /* cap on FP64 refinement passes */
   int32_t warm_start;       /* 1: start from the stored pressure instead of 0 */
   int32_t check_every;      /* inner iterations between host convergence checks */
-  int32_t reserved;
+  int32_t precond;          /* pf_precond */
 } pf_solve_cfg;
+
+enum pf_precond {
+  PF_PRECOND_JACOBI = 0,    /* the reference's diagonal preconditioner (sparse_ops3d_tp.cpp:322-327) */
+  PF_PRECOND_LEARNED = 1    /* flexible PCG with the attached learned network (pf_lpc_attach);
+                               falls back to Jacobi when it stagnates (stats.fallback_used) */
+};
 
 /* Mirrors MRPressureSolveStats3D's core fields (src/pressure/multires_pressure3d.h:60-133) */
 typedef struct pf_solve_stats {
@@ -200,7 +206,9 @@ double pf_beta_from_phi(double phi, const pf_phase* phase);
  * steps with steady_clock, apps/bench_large_scale3d_tp.cpp:559-563) */
 enum pf_kernel {
   PF_K_BIN_COUNT = 0, PF_K_BIN_SCATTER, PF_K_P2G, PF_K_FACE_OPS, PF_K_ASSEMBLE,
-  PF_K_PCG_SPMV, PF_K_PCG_UPDATE, PF_K_RESIDUAL64, PF_K_VELOCITY, PF_K_G2P_ADVECT, PF_NKERNELS
+  PF_K_PCG_SPMV, PF_K_PCG_UPDATE, PF_K_RESIDUAL64, PF_K_VELOCITY, PF_K_G2P_ADVECT,
+  PF_K_LPC_APPLY,      /* one application of the learned preconditioner (5 kernels) */
+  PF_NKERNELS
 };
 /* The library's HIP stream (hipStream_t). */
 void* pf_stream(pf_ctx* ctx);
@@ -213,6 +221,39 @@ int pf_kernel_time(pf_ctx* ctx, int32_t kernel, int64_t* launches, double* total
  * for the call, best of four event-timed passes; a known-size kernel for
  * calibrating HBM traffic counters */
 int pf_copy_probe(pf_ctx* ctx, int64_t n, double* ms);
+
+/* ---- learned block preconditioner (no reference counterpart: the reference
+ * preconditions with Jacobi, sparse_ops3d_tp.cpp:322-327).  A 3-layer 3x3x3
+ * convolutional network over 16^3 cell tiles with a 1-cell halo (widths 16-32-16
+ * and a 1x1x1 head), inputs (r / diag, beta, phi) per cell, BF16 on MFMA with FP32
+ * accumulation; z = r / diag + s_tile * network.  Trained on the GPU with
+ * hand-written backward kernels on residuals recorded from the solver's own runs,
+ * loss ||r - A M(r)||^2 / ||r||^2, Adam. */
+typedef struct pf_lpc pf_lpc;
+int pf_lpc_create(pf_lpc** out, uint64_t seed);       /* He-initialised weights */
+int pf_lpc_destroy(pf_lpc* lpc);
+const char* pf_lpc_last_error(const pf_lpc* lpc);
+int64_t pf_lpc_num_weights(void);                     /* 29,392 */
+/* flat FP32 weights: W1[27][4][16], W2[27][16][32], W3[27][32][16], Wh[16];
+ * tap = dx + 3 dy + 9 dz */
+int pf_lpc_get_weights(pf_lpc* lpc, float* w, int64_t n);
+int pf_lpc_set_weights(pf_lpc* lpc, const float* w, int64_t n);
+/* network on host tiles: X[ntiles][18][18][18][4] (z, y, x, channel) -> O[ntiles][16^3] */
+int pf_lpc_forward_tiles(pf_lpc* lpc, int64_t ntiles, const float* X, float* O);
+/* gradient of sum(O * gO) w.r.t. the flat weights, gW[pf_lpc_num_weights()] */
+int pf_lpc_grad_tiles(pf_lpc* lpc, int64_t ntiles, const float* X, const float* gO, float* gW);
+/* attach a network to a context (NULL detaches); solves with precond =
+ * PF_PRECOND_LEARNED use it; the network must outlive its use */
+int pf_lpc_attach(pf_ctx* ctx, pf_lpc* lpc);
+/* record up to max_snapshots FP32 residuals, one per `every` inner iterations, during
+ * each following solve (the set is replaced by every solve); 0 stops recording */
+int pf_lpc_record(pf_ctx* ctx, int32_t max_snapshots, int32_t every);
+int32_t pf_lpc_snapshot_count(const pf_ctx* ctx);
+/* `steps` Adam steps on the attached context's last system and its snapshots
+ * (uniformly drawn from the seed); *mean_loss = mean of ||r - A M r||^2/||r||^2 */
+int pf_lpc_train(pf_lpc* lpc, pf_ctx* ctx, int32_t steps, double lr, uint64_t seed, double* mean_loss);
+/* the loss of the network and of Jacobi (M = D^-1) on one recorded snapshot */
+int pf_lpc_loss(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, double* loss_learned, double* loss_jacobi);
 
 #ifdef __cplusplus
 }
