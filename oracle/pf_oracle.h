@@ -140,6 +140,78 @@ void orc_step_single(orc_grid* g, int64_t n, double* x, double* y, double* z, do
                      double* v, double* w, double dt, double alpha, double gravity, int cg_iters,
                      double cg_tol, int stop_rule, orc_step_stats* st);
 
+/* ---- multires two-phase path (pf_oracle_mr.c): 2-level leaf layout of 4^3 blocks,
+ * fine face patches, finite-volume projection (src/grid/multires_*.h,
+ * src/driver/multires_ops3d_tp.cpp, src/pressure/multires_pressure3d.cpp,
+ * src/driver/multires_sim3d_tp.cpp).  Leaves are (level, bx, by, bz) int32 quadruples;
+ * positions are relative to the grid origin 0, as in the reference. */
+typedef struct orc_mr orc_mr;
+
+typedef struct orc_mr_solve_cfg {          /* MRPressureSolveConfig3D, multires_pressure3d.h:137-166 */
+  int32_t max_iterations, use_jacobi, use_flexible_beta, adaptive_restart;
+  double absolute_tolerance, relative_tolerance, restart_growth_threshold;
+  int32_t relaxation_sweeps, pad0;
+  double relaxation_omega, relaxation_min_omega, divergence_correction;
+} orc_mr_solve_cfg;
+
+typedef struct orc_mr_solve_stats {        /* MRPressureSolveStats3D core fields */
+  int64_t active_cells, pinned_cell;
+  int32_t iterations, restarts, beta_resets, relaxation_accepted;
+  int32_t relaxation_rejected, converged, breakdown, pad0;
+  double initial_residual, final_residual, effective_tolerance;
+  double true_max_residual, true_rel_residual;   /* recomputed in FP64 at exit */
+} orc_mr_solve_stats;
+
+typedef struct orc_mr_sim {                /* MRSim3DTP fields step() reads + refinement state */
+  int32_t nx, ny, nz, pad0;
+  double dx;
+  orc_phase phase;
+  double dt, gravity, Vp, alpha_liquid, alpha_gas;
+  int32_t adaptive_timestep, advection_order;
+  double adaptive_cfl, adaptive_min_dt;
+  int32_t c_div_volume_correction, pad1;
+  double c_div_strength, liquid_volume_target;
+  orc_mr_solve_cfg solve;
+  int32_t dynamic_refinement, particle_padding, gas_padding, hysteresis, max_fine_leaves;
+  int32_t retained_valid, retained[6], budget_limited, dynamic_last_fine_leaves;
+} orc_mr_sim;
+
+typedef struct orc_mr_step_stats {
+  orc_timestep_stats ts;
+  orc_mr_solve_stats solve;
+  orc_escape_stats escape;
+  double c_div;
+} orc_mr_step_stats;
+
+orc_mr* orc_mr_create(int nx, int ny, int nz, double dx, int faithful);
+void orc_mr_destroy(orc_mr* g);
+orc_mr* orc_mr_clone(const orc_mr* g);
+void orc_mr_set_leaves(orc_mr* g, int n, const int32_t* leaves);
+int64_t orc_mr_leaf_count(const orc_mr* g);
+void orc_mr_get_leaves(const orc_mr* g, int32_t* out);
+int64_t orc_mr_layout_fine_box(int nx, int ny, int nz, int level, int box_valid, const int32_t box[6],
+                               int32_t* out, int64_t cap);
+int64_t orc_mr_layout_bubble_band(int nx, int ny, int nz, int32_t* out, int64_t cap);
+void orc_mr_face_valid(const orc_mr* g, int axis, uint8_t* out);
+void orc_mr_field_dense(const orc_mr* g, int field, double* out);
+void orc_mr_set(orc_mr* g, int field, int i, int j, int k, double v);
+void orc_mr_mark_cells(orc_mr* g, int64_t n, const double* x, const double* y, const double* z);
+void orc_mr_p2g(orc_mr* g, int64_t n, const double* x, const double* y, const double* z, const double* u,
+                const double* v, const double* w, const uint8_t* type, const double* vol,
+                const orc_phase* pp, double Vp);
+void orc_mr_apply_gravity(orc_mr* g, double dt, double gravity);
+void orc_mr_apply_walls(orc_mr* g);
+void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solve_cfg* cfg,
+                    orc_mr_solve_stats* st);
+void orc_mr_g2p(const orc_mr* g, const orc_mr* saved, int64_t n, const double* x, const double* y,
+                const double* z, double* u, double* v, double* w, const uint8_t* type, double aL, double aG);
+void orc_mr_advect(const orc_mr* g, int64_t n, double* x, double* y, double* z, const uint8_t* type,
+                   double dt, int order, orc_escape_stats* es);
+void orc_mr_dynamic_refinement(orc_mr* g, orc_mr_sim* sim, int64_t n, const double* x, const double* y,
+                               const double* z, const uint8_t* type);
+void orc_mr_step(orc_mr* g, orc_mr_sim* sim, int64_t n, double* x, double* y, double* z, double* u,
+                 double* v, double* w, const uint8_t* type, const double* vol, orc_mr_step_stats* st);
+
 /* seeded synthetic scenes (shared construction with the HIP generator) */
 enum { ORC_SCENE_RT = 0, ORC_SCENE_BUBBLE = 1, ORC_SCENE_DAMBREAK = 2 };
 int64_t orc_scene_count(int kind, int nx, int ny, int nz);

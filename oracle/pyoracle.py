@@ -67,6 +67,50 @@ class StepCfg(C.Structure):
                 ("liquid_volume_target", C.c_double), ("phase", Phase)]
 
 
+class MRSolveCfg(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("use_jacobi", C.c_int32), ("use_flexible_beta", C.c_int32),
+                ("adaptive_restart", C.c_int32), ("absolute_tolerance", C.c_double),
+                ("relative_tolerance", C.c_double), ("restart_growth_threshold", C.c_double),
+                ("relaxation_sweeps", C.c_int32), ("pad0", C.c_int32), ("relaxation_omega", C.c_double),
+                ("relaxation_min_omega", C.c_double), ("divergence_correction", C.c_double)]
+
+
+def mr_solve_cfg(max_iterations=160, abs_tol=1e-7, rel_tol=1e-6, jacobi=1, flexible=0, restart=1, growth=10.0,
+                 sweeps=0, omega=0.67, min_omega=0.05, cdiv=0.0):
+    """MRPressureSolveConfig3D with MRSim3DTP's defaults (multires_sim3d_tp.h:36-47)."""
+    return MRSolveCfg(max_iterations, jacobi, flexible, restart, abs_tol, rel_tol, growth, sweeps, 0, omega,
+                      min_omega, cdiv)
+
+
+class MRSolveStats(C.Structure):
+    _fields_ = [("active_cells", C.c_int64), ("pinned_cell", C.c_int64), ("iterations", C.c_int32),
+                ("restarts", C.c_int32), ("beta_resets", C.c_int32), ("relaxation_accepted", C.c_int32),
+                ("relaxation_rejected", C.c_int32), ("converged", C.c_int32), ("breakdown", C.c_int32),
+                ("pad0", C.c_int32), ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("effective_tolerance", C.c_double), ("true_max_residual", C.c_double),
+                ("true_rel_residual", C.c_double)]
+
+
+class MRSim(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pad0", C.c_int32), ("dx", C.c_double),
+                ("phase", Phase), ("dt", C.c_double), ("gravity", C.c_double), ("Vp", C.c_double),
+                ("alpha_liquid", C.c_double), ("alpha_gas", C.c_double), ("adaptive_timestep", C.c_int32),
+                ("advection_order", C.c_int32), ("adaptive_cfl", C.c_double), ("adaptive_min_dt", C.c_double),
+                ("c_div_volume_correction", C.c_int32), ("pad1", C.c_int32), ("c_div_strength", C.c_double),
+                ("liquid_volume_target", C.c_double), ("solve", MRSolveCfg), ("dynamic_refinement", C.c_int32),
+                ("particle_padding", C.c_int32), ("gas_padding", C.c_int32), ("hysteresis", C.c_int32),
+                ("max_fine_leaves", C.c_int32), ("retained_valid", C.c_int32), ("retained", C.c_int32 * 6),
+                ("budget_limited", C.c_int32), ("dynamic_last_fine_leaves", C.c_int32)]
+
+
+def mr_sim(nx, ny, nz, dx=1.0, phase=None, dt=0.02, order=2, dynamic=1, ppad=1, gpad=2, hyst=1, max_fine=0,
+           adaptive=0, cfl=0.5, mindt=1e-5, solve=None, aL=0.95, aG=0.95, gravity=-9.81):
+    """MRSim3DTP's defaults (src/driver/multires_sim3d_tp.h:11-70)."""
+    return MRSim(nx, ny, nz, 0, dx, phase or Phase(1.0, 0.01, 1.0, 1.0), dt, gravity, 1.0, aL, aG, adaptive, order,
+                 cfl, mindt, 0, 0, 1.0, 0.0, solve or mr_solve_cfg(), dynamic, ppad, gpad, hyst, max_fine, 0,
+                 (C.c_int32 * 6)(), 0, 0)
+
+
 class StepStats(C.Structure):
     _fields_ = [("ts", TimestepStats), ("solve", SolveStats), ("escape", EscapeStats),
                 ("liquid_volume", C.c_double), ("c_div", C.c_double),
@@ -131,6 +175,34 @@ def lib():
         L.orc_step_single.argtypes = ([C.c_void_p, C.c_int64] + [_dp] * 6 +
                                       [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
                                        C.c_int, C.POINTER(StepStats)])
+        L.orc_mr_create.restype = C.c_void_p
+        L.orc_mr_create.argtypes = [C.c_int] * 3 + [C.c_double, C.c_int]
+        L.orc_mr_destroy.argtypes = [C.c_void_p]
+        L.orc_mr_clone.restype = C.c_void_p
+        L.orc_mr_clone.argtypes = [C.c_void_p]
+        L.orc_mr_set_leaves.argtypes = [C.c_void_p, C.c_int, _i32]
+        L.orc_mr_leaf_count.restype = C.c_int64
+        L.orc_mr_leaf_count.argtypes = [C.c_void_p]
+        L.orc_mr_get_leaves.argtypes = [C.c_void_p, _i32]
+        L.orc_mr_layout_fine_box.restype = C.c_int64
+        L.orc_mr_layout_fine_box.argtypes = [C.c_int] * 5 + [_i32, C.c_void_p, C.c_int64]
+        L.orc_mr_layout_bubble_band.restype = C.c_int64
+        L.orc_mr_layout_bubble_band.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_int64]
+        L.orc_mr_face_valid.argtypes = [C.c_void_p, C.c_int, _u8]
+        L.orc_mr_field_dense.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_mr_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.orc_mr_mark_cells.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp]
+        L.orc_mr_p2g.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 6 + [_u8, C.c_void_p, C.POINTER(Phase),
+                                                                          C.c_double]
+        L.orc_mr_apply_gravity.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_mr_apply_walls.argtypes = [C.c_void_p]
+        L.orc_mr_project.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, C.POINTER(MRSolveCfg),
+                                     C.POINTER(MRSolveStats)]
+        L.orc_mr_g2p.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + [_dp] * 6 + [_u8, C.c_double, C.c_double]
+        L.orc_mr_advect.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _u8, C.c_double, C.c_int,
+                                    C.POINTER(EscapeStats)]
+        L.orc_mr_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + [_dp] * 6 + [_u8, C.c_void_p, C.c_void_p]
+        L.orc_mr_dynamic_refinement.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, _dp, _dp, _dp, _u8]
         L.orc_scene_count.restype = C.c_int64
         L.orc_scene_count.argtypes = [C.c_int] * 4
         L.orc_scene_fill.restype = C.c_int64
@@ -346,6 +418,31 @@ def reflib():
         L.ref_beta.argtypes = [C.c_double, _dp]
         L.ref_timestep.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
                                    C.c_double, C.c_double, _dp]
+        L.ref_mr_create.restype = C.c_void_p
+        L.ref_mr_create.argtypes = [C.c_int] * 3 + [C.c_double]
+        L.ref_mr_destroy.argtypes = [C.c_void_p]
+        L.ref_mr_clone.restype = C.c_void_p
+        L.ref_mr_clone.argtypes = [C.c_void_p]
+        L.ref_mr_set_leaves.argtypes = [C.c_void_p, C.c_int, _i32]
+        L.ref_mr_get_leaves.restype = C.c_int64
+        L.ref_mr_get_leaves.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_mr_layout_fine_box.restype = C.c_int64
+        L.ref_mr_layout_fine_box.argtypes = [C.c_int] * 3 + [C.c_double, C.c_int, _i32, C.c_void_p]
+        L.ref_mr_config.argtypes = [C.c_void_p, _dp, _i32, _dp]
+        L.ref_mr_set_particles.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 6 + [_u8, C.c_void_p]
+        L.ref_mr_get_particles.restype = C.c_int64
+        L.ref_mr_get_particles.argtypes = [C.c_void_p] + [C.c_void_p] * 7
+        L.ref_mr_init_bubble.restype = C.c_int64
+        L.ref_mr_init_bubble.argtypes = [C.c_void_p]
+        L.ref_mr_step.argtypes = [C.c_void_p, _dp]
+        L.ref_mr_dynamic.argtypes = [C.c_void_p]
+        L.ref_mr_p2g.argtypes = [C.c_void_p]
+        L.ref_mr_g2p.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        L.ref_mr_advect.argtypes = [C.c_void_p, C.c_double, C.c_int, _i32]
+        L.ref_mr_project.argtypes = [C.c_void_p, C.c_double, _i32, _dp, _dp]
+        L.ref_mr_field_dense.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ref_mr_face_valid.argtypes = [C.c_void_p, C.c_int, _u8]
+        L.ref_mr_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.ref1_create.restype = C.c_void_p
         L.ref1_create.argtypes = [C.c_int] * 3 + [C.c_double]
         L.ref1_destroy.argtypes = [C.c_void_p]
@@ -364,6 +461,111 @@ def reflib():
         L.ref1_project.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double]
         _ref = L
     return _ref
+
+
+class MRGrid:
+    """Oracle multires grid (pf_oracle_mr.c): 2-level layout, fine face patches."""
+
+    def __init__(self, nx, ny, nz, dx=1.0, faithful=True, _h=None):
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.h = _h if _h is not None else lib().orc_mr_create(nx, ny, nz, dx, int(faithful))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mr_destroy(self.h)
+            self.h = None
+
+    def clone(self):
+        return MRGrid(self.nx, self.ny, self.nz, self.dx, _h=lib().orc_mr_clone(self.h))
+
+    def set_leaves(self, leaves):
+        lv = np.ascontiguousarray(np.asarray(leaves, np.int32).reshape(-1, 4))
+        lib().orc_mr_set_leaves(self.h, len(lv), lv)
+
+    def leaves(self):
+        n = lib().orc_mr_leaf_count(self.h)
+        out = np.zeros((max(n, 1), 4), np.int32)
+        lib().orc_mr_get_leaves(self.h, out)
+        return out[:n]
+
+    def dims(self, f):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        if f <= MW:
+            a = f % 3
+            return (nx + (a == 0), ny + (a == 1), nz + (a == 2))
+        return (nx, ny, nz)
+
+    def dense(self, f):
+        fx, fy, fz = self.dims(f)
+        out = np.zeros(fx * fy * fz)
+        lib().orc_mr_field_dense(self.h, f, out)
+        return out.reshape(fz, fy, fx)
+
+    def face_valid(self, axis):
+        fx, fy, fz = self.dims(axis)
+        out = np.zeros(fx * fy * fz, np.uint8)
+        lib().orc_mr_face_valid(self.h, axis, out)
+        return out.reshape(fz, fy, fx)
+
+    def set(self, f, i, j, k, v):
+        lib().orc_mr_set(self.h, f, i, j, k, float(v))
+
+    def mark_cells(self, ps):
+        lib().orc_mr_mark_cells(self.h, len(ps), ps.x, ps.y, ps.z)
+
+    def p2g(self, ps, phase, Vp=1.0):
+        lib().orc_mr_p2g(self.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, ps.type, _vptr(ps.volume),
+                         C.byref(phase), Vp)
+
+    def apply_gravity(self, dt, g):
+        lib().orc_mr_apply_gravity(self.h, dt, g)
+
+    def apply_walls(self):
+        lib().orc_mr_apply_walls(self.h)
+
+    def project(self, phase, dt, cfg=None):
+        cfg = cfg or mr_solve_cfg()
+        st = MRSolveStats()
+        lib().orc_mr_project(self.h, C.byref(phase), dt, C.byref(cfg), C.byref(st))
+        return st
+
+    def g2p(self, saved, ps, aL, aG):
+        lib().orc_mr_g2p(self.h, saved.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, ps.type, aL, aG)
+
+    def advect(self, ps, dt, order=2):
+        es = EscapeStats()
+        lib().orc_mr_advect(self.h, len(ps), ps.x, ps.y, ps.z, ps.type, dt, order, C.byref(es))
+        return es
+
+
+class MRStepStats(C.Structure):
+    _fields_ = [("ts", TimestepStats), ("solve", MRSolveStats), ("escape", EscapeStats), ("c_div", C.c_double)]
+
+
+def mr_step(g, sim, ps):
+    st = MRStepStats()
+    lib().orc_mr_step(g.h, C.byref(sim), len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, ps.type, _vptr(ps.volume),
+                      C.byref(st))
+    return st
+
+
+def mr_dynamic_refinement(g, sim, ps):
+    lib().orc_mr_dynamic_refinement(g.h, C.byref(sim), len(ps), ps.x, ps.y, ps.z, ps.type)
+
+
+def mr_layout_fine_box(nx, ny, nz, box=None, level=1):
+    b = np.zeros(6, np.int32) if box is None else np.asarray(box, np.int32)
+    n = lib().orc_mr_layout_fine_box(nx, ny, nz, level, int(box is not None), b, None, 0)
+    out = np.zeros((max(n, 1), 4), np.int32)
+    lib().orc_mr_layout_fine_box(nx, ny, nz, level, int(box is not None), b, _vptr(out), n)
+    return out[:n]
+
+
+def mr_layout_bubble_band(nx, ny, nz):
+    n = lib().orc_mr_layout_bubble_band(nx, ny, nz, None, 0)
+    out = np.zeros((max(n, 1), 4), np.int32)
+    lib().orc_mr_layout_bubble_band(nx, ny, nz, _vptr(out), n)
+    return out[:n]
 
 
 class RefSim1:
@@ -513,3 +715,112 @@ class RefSim:
         st = np.zeros(9, np.int32)
         reflib().ref_advect(self.h, dt, order, st)
         return st
+
+
+class RefMR:
+    """The reference's MRSim3DTP (src/driver/multires_sim3d_tp.h) and its stage functions."""
+
+    def __init__(self, nx, ny, nz, dx=1.0, _h=None):
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.h = _h if _h is not None else reflib().ref_mr_create(nx, ny, nz, dx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            reflib().ref_mr_destroy(self.h)
+            self.h = None
+
+    def clone(self):
+        return RefMR(self.nx, self.ny, self.nz, self.dx, _h=reflib().ref_mr_clone(self.h))
+
+    def set_leaves(self, leaves):
+        lv = np.ascontiguousarray(np.asarray(leaves, np.int32).reshape(-1, 4))
+        reflib().ref_mr_set_leaves(self.h, len(lv), lv)
+
+    def leaves(self):
+        n = reflib().ref_mr_get_leaves(self.h, None)
+        out = np.zeros((max(n, 1), 4), np.int32)
+        reflib().ref_mr_get_leaves(self.h, _vptr(out))
+        return out[:n]
+
+    def config(self, dt=0.02, phase=None, gravity=-9.81, Vp=1.0, aL=0.95, aG=0.95, adaptive=0, cfl=0.5,
+               mindt=1e-5, order=2, cg_iters=160, cg_tol=1e-7, cg_rel_tol=1e-6, jacobi=1, flexible=0, restart=1,
+               growth=10.0, sweeps=0, omega=0.67, min_omega=0.05, dynamic=1, ppad=1, gpad=2, hyst=1,
+               max_fine=0, cdiv_corr=0, cdiv_strength=1.0, lv_target=0.0):
+        phase = phase or Phase(1.0, 0.01, 1.0, 1.0)
+        c = np.array([dt, gravity, Vp, aL, aG, cfl, mindt, cg_tol, cg_rel_tol, growth, omega, min_omega,
+                      cdiv_strength, lv_target], np.float64)
+        ic = np.array([adaptive, order, cg_iters, jacobi, flexible, restart, sweeps, dynamic, ppad, gpad, hyst,
+                       max_fine, cdiv_corr], np.int32)
+        reflib().ref_mr_config(self.h, c, ic, phase.as_array())
+
+    def set_particles(self, ps):
+        reflib().ref_mr_set_particles(self.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, ps.type,
+                                      _vptr(ps.volume))
+
+    def particles(self):
+        n = reflib().ref_mr_get_particles(self.h, *([None] * 7))
+        a = [np.zeros(n) for _ in range(6)]
+        t = np.zeros(max(n, 1), np.uint8)
+        reflib().ref_mr_get_particles(self.h, *[_vptr(q) for q in a], _vptr(t))
+        return Particles(*a, type=t[:n])
+
+    def init_bubble(self):
+        return reflib().ref_mr_init_bubble(self.h)
+
+    def step(self):
+        out = np.zeros(13)
+        reflib().ref_mr_step(self.h, out)
+        return out
+
+    def dynamic(self):
+        reflib().ref_mr_dynamic(self.h)
+
+    def p2g(self):
+        reflib().ref_mr_p2g(self.h)
+
+    def g2p(self, saved, aL, aG):
+        reflib().ref_mr_g2p(self.h, saved.h, aL, aG)
+
+    def advect(self, dt, order=2):
+        st = np.zeros(8, np.int32)
+        reflib().ref_mr_advect(self.h, dt, order, st)
+        return st
+
+    def project(self, dt, cfg: MRSolveCfg):
+        ic = np.array([cfg.max_iterations, cfg.use_jacobi, cfg.use_flexible_beta, cfg.adaptive_restart,
+                       cfg.relaxation_sweeps], np.int32)
+        c = np.array([cfg.absolute_tolerance, cfg.relative_tolerance, cfg.restart_growth_threshold,
+                      cfg.relaxation_omega, cfg.relaxation_min_omega, cfg.divergence_correction], np.float64)
+        out = np.zeros(12)
+        reflib().ref_mr_project(self.h, dt, ic, c, out)
+        return out
+
+    def dims(self, f):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        if f <= MW:
+            a = f % 3
+            return (nx + (a == 0), ny + (a == 1), nz + (a == 2))
+        return (nx, ny, nz)
+
+    def dense(self, f):
+        fx, fy, fz = self.dims(f)
+        out = np.zeros(fx * fy * fz)
+        reflib().ref_mr_field_dense(self.h, f, out)
+        return out.reshape(fz, fy, fx)
+
+    def face_valid(self, axis):
+        fx, fy, fz = self.dims(axis)
+        out = np.zeros(fx * fy * fz, np.uint8)
+        reflib().ref_mr_face_valid(self.h, axis, out)
+        return out.reshape(fz, fy, fx)
+
+    def set(self, f, i, j, k, v):
+        reflib().ref_mr_set(self.h, f, i, j, k, float(v))
+
+
+def ref_mr_layout_fine_box(nx, ny, nz, box=None, dx=1.0):
+    b = np.zeros(6, np.int32) if box is None else np.asarray(box, np.int32)
+    n = reflib().ref_mr_layout_fine_box(nx, ny, nz, dx, int(box is not None), b, None)
+    out = np.zeros((max(n, 1), 4), np.int32)
+    reflib().ref_mr_layout_fine_box(nx, ny, nz, dx, int(box is not None), b, _vptr(out))
+    return out[:n]

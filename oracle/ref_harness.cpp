@@ -9,7 +9,11 @@
 // betaFromPhi (src/physics/phasefield.h:10-18), computeAdaptiveParticleTimestep3D
 // (src/driver/timestep3d.h:16); and the single-phase SparseSim3D
 // (src/driver/sparse_sim3d.h:5-15) with spP2G3D / spProjectStep3D
-// (src/driver/sparse_ops3d.h:5-9).
+// (src/driver/sparse_ops3d.h:5-9); and the multires MRSim3DTP
+// (src/driver/multires_sim3d_tp.h) with mrP2G3D_tp / mrG2P3D_tp / mrAdvect3D_tp
+// (src/driver/multires_ops3d_tp.h) and projectMR3D (src/pressure/multires_pressure3d.h).
+#include "driver/multires_ops3d_tp.h"
+#include "driver/multires_sim3d_tp.h"
 #include "driver/sparse_ops3d.h"
 #include "driver/sparse_ops3d_tp.h"
 #include "driver/sparse_sim3d.h"
@@ -17,6 +21,7 @@
 #include "driver/timestep3d.h"
 #include "grid/sparse_mac_grid3d.h"
 #include "physics/phasefield.h"
+#include "pressure/multires_pressure3d.h"
 #include "transfer/transfer3d_tp.h"
 
 #include <chrono>
@@ -244,6 +249,180 @@ void ref1_p2g(void* sp) {
 void ref1_project(void* sp, double dt, int iters, double tol) {
   auto* s = static_cast<SparseSim3D*>(sp);
   spProjectStep3D(s->grid, dt, iters, tol);
+}
+
+// ---- multires MRSim3DTP
+namespace {
+MRSim3DTP* mrs(void* p) { return static_cast<MRSim3DTP*>(p); }
+std::map<MRFaceKey3D, float>& mrFace(MRMacGrid3D<4>& g, int f) {
+  switch (f) {
+    case 0: return g.ufield;
+    case 1: return g.vfield;
+    case 2: return g.wfield;
+    case 3: return g.mu;
+    case 4: return g.mv;
+    default: return g.mw;
+  }
+}
+}  // namespace
+
+void* ref_mr_create(int nx, int ny, int nz, double dx) { return new MRSim3DTP(nx, ny, nz, dx); }
+void ref_mr_destroy(void* s) { delete mrs(s); }
+void* ref_mr_clone(void* s) { return new MRSim3DTP(*mrs(s)); }
+
+void ref_mr_set_leaves(void* sp, int n, const int32_t* lv) {
+  auto* s = mrs(sp);
+  s->layout.leaf_blocks.clear();
+  for (int a = 0; a < n; ++a) s->layout.leaf_blocks.push_back({lv[4 * a], lv[4 * a + 1], lv[4 * a + 2], lv[4 * a + 3]});
+  s->layout.enforceTwoToOneBalance();
+  s->grid = MRMacGrid3D<4>(s->layout);
+}
+int64_t ref_mr_get_leaves(void* sp, int32_t* out) {
+  const auto& L = mrs(sp)->grid.layout.leaves();
+  if (out)
+    for (size_t a = 0; a < L.size(); ++a) {
+      out[4 * a] = L[a].level; out[4 * a + 1] = L[a].bx; out[4 * a + 2] = L[a].by; out[4 * a + 3] = L[a].bz;
+    }
+  return (int64_t)L.size();
+}
+// layoutWithFineBox through the public layout API (setCoarseEverywhere + refineFineCellBox)
+int64_t ref_mr_layout_fine_box(int nx, int ny, int nz, double dx, int valid, const int32_t* b, int32_t* out) {
+  MRLayout3D<4> L(nx, ny, nz, dx);
+  L.setCoarseEverywhere(1);
+  if (valid) {
+    int x0 = std::max(0, std::min(nx, b[0])), y0 = std::max(0, std::min(ny, b[1])), z0 = std::max(0, std::min(nz, b[2]));
+    int x1 = std::max(0, std::min(nx, b[3])), y1 = std::max(0, std::min(ny, b[4])), z1 = std::max(0, std::min(nz, b[5]));
+    if (x0 < x1 && y0 < y1 && z0 < z1) L.refineFineCellBox(x0, y0, z0, x1, y1, z1);
+  }
+  L.enforceTwoToOneBalance();
+  const auto& lv = L.leaves();
+  if (out)
+    for (size_t a = 0; a < lv.size(); ++a) {
+      out[4 * a] = lv[a].level; out[4 * a + 1] = lv[a].bx; out[4 * a + 2] = lv[a].by; out[4 * a + 3] = lv[a].bz;
+    }
+  return (int64_t)lv.size();
+}
+// cfg: dt, gravity, Vp, aL, aG, cfl, mindt, cg_tol, cg_rel_tol, restart_growth, relax_omega,
+//      relax_min_omega, c_div_strength, liquid_volume_target
+// icfg: adaptive_timestep, advection_order, cg_iters, jacobi, flexible, adaptive_restart, relax_sweeps,
+//       dynamic_refinement, particle_padding, gas_padding, hysteresis, max_fine_leaves, c_div_volume_correction
+void ref_mr_config(void* sp, const double* c, const int32_t* ic, const double* phase) {
+  auto* s = mrs(sp);
+  s->dt = c[0]; s->gravity = c[1]; s->Vp = c[2]; s->alpha_liquid = c[3]; s->alpha_gas = c[4];
+  s->adaptive_cfl = c[5]; s->adaptive_min_dt = c[6]; s->cg_tol = c[7]; s->cg_rel_tol = c[8];
+  s->cg_restart_growth = c[9]; s->cg_relaxation_omega = c[10]; s->cg_relaxation_min_omega = c[11];
+  s->c_div_strength = c[12]; s->liquid_volume_target = c[13];
+  s->adaptive_timestep = ic[0] != 0; s->advection_order = ic[1]; s->cg_iters = ic[2];
+  s->cg_jacobi_preconditioner = ic[3] != 0; s->cg_flexible_beta = ic[4] != 0; s->cg_adaptive_restart = ic[5] != 0;
+  s->cg_relaxation_sweeps = ic[6]; s->dynamic_refinement = ic[7] != 0; s->dynamic_particle_padding = ic[8];
+  s->dynamic_gas_padding = ic[9]; s->dynamic_hysteresis_cells = ic[10]; s->dynamic_max_fine_leaves = ic[11];
+  s->c_div_volume_correction = ic[12] != 0;
+  s->phase = phaseOf(phase);
+}
+void ref_mr_set_particles(void* sp, int64_t n, const double* x, const double* y, const double* z, const double* u,
+                          const double* v, const double* w, const uint8_t* t, const double* vol) {
+  auto* s = mrs(sp);
+  s->particles = Particles3DTP();
+  for (int64_t p = 0; p < n; ++p) s->particles.add({x[p], y[p], z[p]}, {u[p], v[p], w[p]}, t[p], vol ? vol[p] : 1.0);
+}
+int64_t ref_mr_get_particles(void* sp, double* x, double* y, double* z, double* u, double* v, double* w, uint8_t* t) {
+  auto* s = mrs(sp);
+  if (x)
+    for (size_t p = 0; p < s->particles.size(); ++p) {
+      x[p] = s->particles.pos[p].x; y[p] = s->particles.pos[p].y; z[p] = s->particles.pos[p].z;
+      u[p] = s->particles.vel[p].x; v[p] = s->particles.vel[p].y; w[p] = s->particles.vel[p].z;
+      if (t) t[p] = s->particles.type[p];
+    }
+  return (int64_t)s->particles.size();
+}
+int64_t ref_mr_init_bubble(void* sp) {
+  auto* s = mrs(sp);
+  s->initBubbleTankInterfaceBand();
+  return (int64_t)s->particles.size();
+}
+// out: effective_dt, c_div_last, iterations, converged, final_residual, restarts, active cells,
+//      pinned cell, clamped liquid, clamped gas, ms, dynamic_last_fine_leaves, budget_limited
+void ref_mr_step(void* sp, double* out) {
+  auto* s = mrs(sp);
+  auto t0 = std::chrono::steady_clock::now();
+  s->step();
+  auto t1 = std::chrono::steady_clock::now();
+  if (out) {
+    const auto& st = s->last_pressure_stats;
+    out[0] = s->effective_dt_last; out[1] = s->c_div_last; out[2] = st.iterations; out[3] = st.converged;
+    out[4] = st.final_residual; out[5] = st.restarts; out[6] = st.active_cells; out[7] = st.pinned_cell;
+    out[8] = s->particle_boundary_clamped_liquid_last; out[9] = s->particle_boundary_clamped_gas_last;
+    out[10] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    out[11] = s->dynamic_last_fine_leaves; out[12] = s->dynamic_budget_limited;
+  }
+}
+void ref_mr_dynamic(void* sp) { mrs(sp)->updateDynamicRefinement(); }
+void ref_mr_p2g(void* sp) {
+  auto* s = mrs(sp);
+  mrP2G3D_tp(s->grid, s->particles, s->phase, s->Vp);
+}
+void ref_mr_g2p(void* sp, void* savedp, double aL, double aG) {
+  mrG2P3D_tp(mrs(sp)->grid, mrs(sp)->particles, mrs(savedp)->grid, aL, aG);
+}
+void ref_mr_advect(void* sp, double dt, int order, int32_t* stats) {
+  ParticleEscapeStats3D es;
+  mrAdvect3D_tp(mrs(sp)->particles, mrs(sp)->grid, dt, &es, order, nullptr);
+  if (stats) {
+    stats[0] = es.clamped_liquid; stats[1] = es.clamped_gas; stats[2] = es.clamped_x_lo; stats[3] = es.clamped_x_hi;
+    stats[4] = es.clamped_y_lo; stats[5] = es.clamped_y_hi; stats[6] = es.clamped_z_lo; stats[7] = es.clamped_z_hi;
+  }
+}
+// cfg as orc_mr_solve_cfg; out: iterations, restarts, beta_resets, relax accepted, relax rejected,
+// converged, breakdown, initial residual, final residual, effective tolerance, active cells, pinned cell
+void ref_mr_project(void* sp, double dt, const int32_t* ic, const double* c, double* out) {
+  auto* s = mrs(sp);
+  MRPressureSolveConfig3D cf;
+  cf.max_iterations = ic[0]; cf.use_jacobi_preconditioner = ic[1] != 0; cf.use_flexible_cg_beta = ic[2] != 0;
+  cf.adaptive_restart = ic[3] != 0; cf.relaxation_sweeps = ic[4];
+  cf.absolute_tolerance = c[0]; cf.relative_tolerance = c[1]; cf.restart_growth_threshold = c[2];
+  cf.relaxation_omega = c[3]; cf.relaxation_min_omega = c[4]; cf.divergence_correction = c[5];
+  MRPressureSolveStats3D st;
+  projectMR3D(s->grid, s->phase, dt, cf, &st);
+  if (out) {
+    out[0] = st.iterations; out[1] = st.restarts; out[2] = st.beta_resets; out[3] = st.relaxation_accepted;
+    out[4] = st.relaxation_rejected; out[5] = st.converged; out[6] = st.breakdown; out[7] = st.initial_residual;
+    out[8] = st.final_residual; out[9] = st.effective_tolerance; out[10] = st.active_cells; out[11] = st.pinned_cell;
+  }
+}
+// dense views: faces over the fine face grids; P / MK per fine cell (its leaf cell's value)
+void ref_mr_field_dense(void* sp, int f, double* out) {
+  auto& g = mrs(sp)->grid;
+  const int nx = g.layout.nx, ny = g.layout.ny, nz = g.layout.nz;
+  if (f <= 5) {
+    const int ax = f % 3;
+    const int fx = nx + (ax == 0), fy = ny + (ax == 1), fz = nz + (ax == 2);
+    std::fill(out, out + (int64_t)fx * fy * fz, 0.0);
+    for (const auto& kv : mrFace(g, f))
+      out[kv.first.fineX + (int64_t)fx * (kv.first.fineY + (int64_t)fy * kv.first.fineZ)] = kv.second;
+    return;
+  }
+  const MRScalarGrid3D<4>& sg = f == 6 ? g.p : g.marker;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) out[i + (int64_t)nx * (j + (int64_t)ny * k)] = sg.get(sg.cellAtFineCell(i, j, k));
+}
+void ref_mr_face_valid(void* sp, int axis, uint8_t* out) {
+  auto& g = mrs(sp)->grid;
+  const int nx = g.layout.nx, ny = g.layout.ny, nz = g.layout.nz;
+  const int fx = nx + (axis == 0), fy = ny + (axis == 1), fz = nz + (axis == 2);
+  std::fill(out, out + (int64_t)fx * fy * fz, (uint8_t)0);
+  const auto& faces = axis == 0 ? g.uFaceRefs() : (axis == 1 ? g.vFaceRefs() : g.wFaceRefs());
+  for (const auto& k : faces) out[k.fineX + (int64_t)fx * (k.fineY + (int64_t)fy * k.fineZ)] = 1;
+}
+void ref_mr_set(void* sp, int f, int i, int j, int k, double v) {
+  auto& g = mrs(sp)->grid;
+  if (f <= 5) {
+    mrFace(g, f)[MRFaceKey3D{f % 3, i, j, k, 1, 1}] = (float)v;
+    return;
+  }
+  MRScalarGrid3D<4>& sg = f == 6 ? g.p : g.marker;
+  MRCellKey3D c = sg.cellAtFineCell(i, j, k);
+  if (c.block.level >= 0) sg.ref(c) = (float)v;
 }
 
 }  // extern "C"
