@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +24,7 @@
 #include "pf_project.hip"
 #include "pf_g2p.hip"
 #include "pf_lpc.hip"
+#include "pf_mr.hip"
 
 using namespace pf;
 
@@ -249,7 +251,8 @@ int run_face_ops(pf_ctx* c, int ops, float dtg) {
 }
 
 // single: the single-phase trilinear unit-mass splat (ph and Vp unused)
-int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false) {
+// mr: the cubic splat over the enumerated multires faces
+int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false, bool mr = false) {
   int st = ensure_binned(c);
   if (st) return st;
   const Dims* d[3] = {&c->du, &c->dv, &c->dw};
@@ -274,6 +277,7 @@ int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false)
   a.has_vol = c->has_vol;
   a.pow2 = is_pow2(c->geom.dx);
   a.inv_dx = 1.0 / c->geom.dx;
+  for (int ax = 0; ax < 3; ++ax) a.valid[ax] = mr ? c->mr_valid[ax] : nullptr;
   a.mp_l = single ? 1.f : (float)(ph->rho_l * Vp);
   a.mp_g = single ? 1.f : (float)(ph->rho_g * Vp);
   if (c->np == 0) return PF_OK;
@@ -281,7 +285,9 @@ int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false)
     a.list = c->color_list + (long long)col * c->color_cap;
     a.count = c->color_cnt + col;
     const dim3 grid((unsigned)std::max<long long>(c->color_cap, 1));
-    if (single)
+    if (mr)
+      TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<2>, grid, dim3(64), 0, c->stream, a));
+    else if (single)
       TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<1>, grid, dim3(64), 0, c->stream, a));
     else
       TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<0>, grid, dim3(64), 0, c->stream, a));
@@ -440,6 +446,220 @@ int lpc_features_alloc(pf_ctx* c) {
   return PF_OK;
 }
 
+// ---- multires layout helpers (host)
+
+MrGrid mr_grid(const pf_ctx* c) {
+  MrGrid m;
+  m.dc = c->dc; m.du = c->du; m.dv = c->dv; m.dw = c->dw; m.g = c->geom;
+  m.lvl = c->mr_lvl; m.marker = c->marker;
+  for (int a = 0; a < 3; ++a) m.valid[a] = c->mr_valid[a];
+  return m;
+}
+
+
+using Leaf = std::array<int32_t, 4>;
+
+// MRLayout3D::intersectsDomain (multires_layout3d.h:137-153)
+bool mr_block_in_domain(int nx, int ny, int nz, const Leaf& k) {
+  const int s = 4 << k[0];
+  const int x0 = k[1] * s, y0 = k[2] * s, z0 = k[3] * s;
+  return std::max(0, x0) < std::min(nx, x0 + s) && std::max(0, y0) < std::min(ny, y0 + s) &&
+         std::max(0, z0) < std::min(nz, z0 + s);
+}
+
+// refineFineCellBox (multires_layout3d.h:62-97): leaves overlapping the box split one level
+void mr_refine(std::vector<Leaf>& leaves, int nx, int ny, int nz, int x0, int y0, int z0, int x1, int y1, int z1) {
+  std::vector<Leaf> next;
+  next.reserve(leaves.size() * 2);
+  for (const Leaf& k : leaves) {
+    const int s = 4 << k[0];
+    const int bx0 = k[1] * s, by0 = k[2] * s, bz0 = k[3] * s;
+    const int bx1 = std::min(nx, bx0 + s), by1 = std::min(ny, by0 + s), bz1 = std::min(nz, bz0 + s);
+    const bool overlaps = bx0 < x1 && bx1 > x0 && by0 < y1 && by1 > y0 && bz0 < z1 && bz1 > z0;
+    if (!overlaps || k[0] == 0) {
+      next.push_back(k);
+      continue;
+    }
+    for (int cz = 0; cz < 2; ++cz)
+      for (int cy = 0; cy < 2; ++cy)
+        for (int cx = 0; cx < 2; ++cx) {
+          const Leaf ch = {k[0] - 1, k[1] * 2 + cx, k[2] * 2 + cy, k[3] * 2 + cz};
+          if (mr_block_in_domain(nx, ny, nz, ch)) next.push_back(ch);
+        }
+  }
+  std::sort(next.begin(), next.end());
+  next.erase(std::unique(next.begin(), next.end()), next.end());
+  leaves.swap(next);
+}
+
+// layoutWithFineBox (multires_sim3d_tp.cpp:172-180) on a `level` base
+std::vector<Leaf> mr_layout_box(int nx, int ny, int nz, int level, bool valid, const int32_t* box) {
+  std::vector<Leaf> lv;
+  const int s = 4 << level;
+  for (int bz = 0; bz < (nz + s - 1) / s; ++bz)
+    for (int by = 0; by < (ny + s - 1) / s; ++by)
+      for (int bx = 0; bx < (nx + s - 1) / s; ++bx) lv.push_back({level, bx, by, bz});
+  std::sort(lv.begin(), lv.end());
+  if (valid) {
+    const int x0 = std::clamp(box[0], 0, nx), y0 = std::clamp(box[1], 0, ny), z0 = std::clamp(box[2], 0, nz);
+    const int x1 = std::clamp(box[3], 0, nx), y1 = std::clamp(box[4], 0, ny), z1 = std::clamp(box[5], 0, nz);
+    if (x0 < x1 && y0 < y1 && z0 < z1) mr_refine(lv, nx, ny, nz, x0, y0, z0, x1, y1, z1);
+  }
+  return lv;
+}
+
+long long mr_fine_count(const std::vector<Leaf>& lv) {
+  long long n = 0;
+  for (const Leaf& k : lv) n += k[0] == 0;
+  return n;
+}
+
+int mr_apply_layout(pf_ctx* c, std::vector<Leaf> lv) {
+  std::sort(lv.begin(), lv.end());
+  lv.erase(std::unique(lv.begin(), lv.end()), lv.end());
+  const Geom& g = c->geom;
+  // the leaves must tile the domain: every fine cell in exactly one leaf
+  long long covered = 0;
+  for (const Leaf& k : lv) {
+    if (k[0] < 0 || k[0] > 6 || k[1] < 0 || k[2] < 0 || k[3] < 0 || !mr_block_in_domain(g.nx, g.ny, g.nz, k))
+      return fail(c, PF_ERR_ARG, "multires leaf outside the domain or with a level outside 0..6");
+    const int s = 4 << k[0];
+    covered += (long long)(std::min(g.nx, (k[1] + 1) * s) - k[1] * s) * (std::min(g.ny, (k[2] + 1) * s) - k[2] * s) *
+               (std::min(g.nz, (k[3] + 1) * s) - k[3] * s);
+  }
+  if (covered != (long long)g.nx * g.ny * g.nz) return fail(c, PF_ERR_ARG, "multires leaves must tile the domain once");
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  const Dims* fd[3] = {&c->du, &c->dv, &c->dw};
+  if (!c->mr_lvl) {
+    HIP_TRY(c, dalloc(&c->mr_lvl, ncell));
+    for (int a = 0; a < 3; ++a) {
+      HIP_TRY(c, dalloc(&c->mr_valid[a], fd[a]->nblocks * kBlockVol));
+      HIP_TRY(c, dalloc(&c->mr_cf32[a], fd[a]->nblocks * kBlockVol));
+      HIP_TRY(c, dalloc(&c->mr_cf64[a], fd[a]->nblocks * kBlockVol));
+    }
+    HIP_TRY(c, dalloc(&c->mr_vol64, ncell));
+    HIP_TRY(c, dalloc(&c->mr_pfull, ncell));
+    HIP_TRY(c, dalloc(&c->mr_bbox, 16));
+    HIP_TRY(c, hipHostMalloc((void**)&c->mr_bbox_host, sizeof(int) * 16));
+  }
+  if ((long long)lv.size() > c->mr_leaves_cap) {
+    dfree(c->mr_leaves_dev);
+    HIP_TRY(c, dalloc(&c->mr_leaves_dev, 4 * (long long)lv.size()));
+    c->mr_leaves_cap = (long long)lv.size();
+  }
+  c->mr_leaves.assign(4 * lv.size(), 0);
+  for (size_t a = 0; a < lv.size(); ++a)
+    for (int q = 0; q < 4; ++q) c->mr_leaves[4 * a + q] = lv[a][q];
+  HIP_TRY(c, hipMemcpyAsync(c->mr_leaves_dev, c->mr_leaves.data(), sizeof(int32_t) * c->mr_leaves.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, fill_u8(c->stream, c->mr_lvl, 255, ncell));
+  hipLaunchKernelGGL(k_mr_levels, dim3((unsigned)lv.size()), dim3(256), 0, c->stream, c->mr_leaves_dev, (int)lv.size(),
+                     c->dc, c->geom, c->mr_lvl);
+  LAUNCH_CHECK(c);
+  c->mr = true;
+  for (int a = 0; a < 3; ++a) {
+    hipLaunchKernelGGL(k_mr_valid, dim3(grid_for(fd[a]->nblocks * kBlockVol, 256)), dim3(256), 0, c->stream, mr_grid(c),
+                       a, c->mr_valid[a]);
+    LAUNCH_CHECK(c);
+  }
+  // MRMacGrid3D<4>(layout): every field starts empty
+  for (int f = 0; f < F_N; ++f) {
+    if (f == F_MK) continue;
+    const Dims& d = f == F_P ? c->dc : *fd[(f == F_SU || f == F_SV || f == F_SW) ? f - F_SU : f % 3];
+    HIP_TRY(c, fill_f32(c->stream, c->f[f], 0.f, d.nblocks * kBlockVol));
+    HIP_TRY(c, fill_u8(c->stream, c->act[f], 0, d.nblocks));
+  }
+  HIP_TRY(c, fill_u8(c->stream, c->marker, 0, ncell));
+  HIP_TRY(c, fill_f64(c->stream, c->mr_pfull, 0.0, ncell));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+// updateDynamicRefinement (multires_sim3d_tp.cpp:577-669): particle and gas cell
+// boxes (padded), hysteresis, fine-leaf budget; the layout changes only when its
+// leaves differ
+int mr_dynamic(pf_ctx* c, const pf_mr_refine_cfg* rc) {
+  if (!rc->dynamic_refinement) return PF_OK;
+  const Geom& g = c->geom;
+  const int lim[3] = {g.nx, g.ny, g.nz};
+  HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->mr_bbox), 0x7fffffffu, 12));
+  if (c->np > 0)
+    hipLaunchKernelGGL(k_mr_bbox, dim3(std::min(grid_for(c->np, 256), 1024u)), dim3(256), 0, c->stream, c->np, c->pa,
+                       c->geom, c->mr_bbox);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, hipMemcpyAsync(c->mr_bbox_host, c->mr_bbox, sizeof(int) * 12, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int* bb = c->mr_bbox_host;
+  bool dv = false;
+  int d[6] = {0, 0, 0, 0, 0, 0};
+  auto include = [&](int x0, int y0, int z0, int x1, int y1, int z1) {
+    if (x0 >= x1 || y0 >= y1 || z0 >= z1) return;
+    const int b[6] = {x0, y0, z0, x1, y1, z1};
+    if (!dv) {
+      std::copy(b, b + 6, d);
+      dv = true;
+      return;
+    }
+    for (int a = 0; a < 3; ++a) { d[a] = std::min(d[a], b[a]); d[3 + a] = std::max(d[3 + a], b[3 + a]); }
+  };
+  if (bb[0] != 0x7fffffff) {
+    const int p = std::max(0, rc->particle_padding);
+    include(bb[0] - p, bb[1] - p, bb[2] - p, -bb[3] + p + 1, -bb[4] + p + 1, -bb[5] + p + 1);
+  }
+  if (bb[6] != 0x7fffffff) {
+    const int p = std::max(0, rc->gas_padding);
+    include(bb[6] - p, bb[7] - p, bb[8] - p, -bb[9] + p + 1, -bb[10] + p + 1, -bb[11] + p + 1);
+  }
+  auto clampbox = [&](int* b, bool& valid) {
+    if (!valid) return;
+    for (int a = 0; a < 3; ++a) { b[a] = std::clamp(b[a], 0, lim[a]); b[3 + a] = std::clamp(b[3 + a], 0, lim[a]); }
+    valid = b[0] < b[3] && b[1] < b[4] && b[2] < b[5];
+  };
+  clampbox(d, dv);
+  int t[6] = {0, 0, 0, 0, 0, 0};
+  bool tv = false;
+  c->mr_budget_limited = 0;
+  if (dv) {
+    const int* r = c->mr_retained;
+    const bool contains = c->mr_retained_valid && r[0] <= d[0] && r[1] <= d[1] && r[2] <= d[2] && r[3] >= d[3] &&
+                          r[4] >= d[4] && r[5] >= d[5];
+    if (contains) {
+      std::copy(r, r + 6, t);
+      tv = true;
+    } else {
+      const int pad = std::max(0, rc->hysteresis_cells);
+      for (int a = 0; a < 3; ++a) { t[a] = d[a] - pad; t[3 + a] = d[3 + a] + pad; }
+      tv = true;
+      clampbox(t, tv);
+    }
+    // fitBoxToFineLeafBudget (:194-210): shrink the longest axis away from the centre
+    if (tv && rc->max_fine_leaves > 0 &&
+        mr_fine_count(mr_layout_box(g.nx, g.ny, g.nz, 1, true, t)) > rc->max_fine_leaves) {
+      c->mr_budget_limited = 1;
+      const double cc[3] = {0.5 * (t[0] + t[3]), 0.5 * (t[1] + t[4]), 0.5 * (t[2] + t[5])};
+      while (mr_fine_count(mr_layout_box(g.nx, g.ny, g.nz, 1, true, t)) > rc->max_fine_leaves) {
+        int ax = 0;
+        if (t[4] - t[1] > t[3] - t[0]) ax = 1;
+        if (t[5] - t[2] > t[3 + ax] - t[ax]) ax = 2;
+        if (t[3 + ax] - t[ax] <= 1) break;
+        if ((t[3 + ax] - cc[ax]) >= (cc[ax] - t[ax])) --t[3 + ax];
+        else ++t[ax];
+        tv = t[0] < t[3] && t[1] < t[4] && t[2] < t[5];
+        if (!tv) break;
+      }
+    }
+  }
+  c->mr_retained_valid = tv;
+  if (tv) std::copy(t, t + 6, c->mr_retained);
+  std::vector<Leaf> next = mr_layout_box(g.nx, g.ny, g.nz, 1, tv, t);
+  c->mr_last_fine_leaves = (int)mr_fine_count(next);
+  bool same = c->mr && next.size() * 4 == c->mr_leaves.size();
+  for (size_t a = 0; same && a < next.size(); ++a)
+    for (int q = 0; q < 4; ++q) same = same && next[a][q] == c->mr_leaves[4 * a + q];
+  if (!same) return mr_apply_layout(c, next);
+  return PF_OK;
+}
+
 int read_small(pf_ctx* c) {
   HIP_TRY(c, hipMemcpyAsync(c->red_host, c->red, sizeof(double) * 16, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipMemcpyAsync(c->u32_host, c->u32_dev, sizeof(uint32_t) * 16, hipMemcpyDeviceToHost, c->stream));
@@ -452,9 +672,13 @@ void record(pf_ctx* c, bool timing, int slot) {
 }
 
 // single: the single-phase unit-density projection (ph unused, cdiv ignored)
+// single: the single-phase unit-density projection (ph unused, cdiv ignored);
+// c->mr: the multires finite-volume projection, stopping when
+// |rhs - A x|_inf < max(cfg->tol, mr_rel_tol * |rhs|_inf)
 int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cfg, double cdiv,
                 pf_solve_stats* out, bool timing, int ev_assemble, int ev_solve, int ev_vel,
-                bool single = false) {
+                bool single = false, double mr_rel_tol = 0.0) {
+  const bool mr = c->mr;
   pf_solve_stats S;
   std::memset(&S, 0, sizeof(S));
   S.pinned_cell = -1;
@@ -508,6 +732,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   // recording training snapshots)
   const bool want_learned = cfg->precond == PF_PRECOND_LEARNED;
   if (want_learned && !c->lpc) return fail(c, PF_ERR_STATE, "precond = learned but no network attached (pf_lpc_attach)");
+  if (want_learned && mr) return fail(c, PF_ERR_ARG, "the learned preconditioner runs on the sparse path only");
   const bool lpc_system = c->lpc && (want_learned || c->snap_cap > 0);
   c->tiles_valid = false;
   c->nsnap = 0;
@@ -526,6 +751,30 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   pa.cdiv = single ? 0.0 : cdiv;
   pa.pin = c->u32_dev;
   pa.single = single ? 1 : 0;
+  const unsigned G = list_grid(npb);
+  if (mr) {
+    // multires: couplings are written per fine face, so the arrays start at zero
+    const Dims* fd[3] = {&c->du, &c->dv, &c->dw};
+    for (int a = 0; a < 3; ++a) {
+      HIP_TRY(c, fill_f32(c->stream, c->mr_cf32[a], 0.f, fd[a]->nblocks * kBlockVol));
+      HIP_TRY(c, fill_f64(c->stream, c->mr_cf64[a], 0.0, fd[a]->nblocks * kBlockVol));
+    }
+    MrAsm ma{};
+    ma.m = mr_grid(c);
+    ma.list = c->pblk; ma.nlist = npb;
+    ma.ph = phase_k(ph);
+    ma.dt = dt; ma.cdiv = cdiv; ma.pin = c->u32_dev;
+    ma.u = c->f[F_U]; ma.v = c->f[F_V]; ma.w = c->f[F_W];
+    ma.mu = c->f[F_MU]; ma.mv = c->f[F_MV]; ma.mw = c->f[F_MW];
+    ma.b64 = c->b64; ma.cd64 = c->cd64; ma.vol64 = c->mr_vol64;
+    ma.cd32 = c->cd32; ma.invd32 = c->invd32;
+    for (int a = 0; a < 3; ++a) { ma.cf64[a] = c->mr_cf64[a]; ma.cf32[a] = c->mr_cf32[a]; }
+    ma.part = c->partial;
+    hipLaunchKernelGGL(k_mr_pin, dim3(G), dim3(256), 0, c->stream, ma);
+    LAUNCH_CHECK(c);
+    TIMED(c, PF_K_ASSEMBLE, hipLaunchKernelGGL(k_mr_assemble, dim3(G), dim3(256), 0, c->stream, ma));
+    LAUNCH_CHECK(c);
+  } else {
   hipLaunchKernelGGL(k_pin_detect, dim3(list_grid(npb)), dim3(256), 0, c->stream, pa);
   LAUNCH_CHECK(c);
   AsmOut ao;
@@ -536,9 +785,9 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   ao.part = c->partial;
   ao.fbeta = lpc_system ? c->fbeta : nullptr;
   ao.fphi = lpc_system ? c->fphi : nullptr;
-  const unsigned G = list_grid(npb);
   TIMED(c, PF_K_ASSEMBLE, hipLaunchKernelGGL(k_assemble, dim3(G), dim3(256), 0, c->stream, pa, ao));
   LAUNCH_CHECK(c);
+  }
   hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 1, 0u, c->red + 2);
   LAUNCH_CHECK(c);
   record(c, timing, ev_assemble);
@@ -548,7 +797,18 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   va.dc = c->dc; va.list = c->pblk; va.nlist = npb; va.sc = c->sc; va.part = c->partial;
   hipLaunchKernelGGL(k_ir_start, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->invd32, cfg->warm_start);
   LAUNCH_CHECK(c);
+  MrOp mop{};
+  MrOp64 mop64{};
+  if (mr) {
+    mop.m = mop64.m = mr_grid(c);
+    mop.cd = c->cd32; mop64.cd = c->cd64; mop64.vol = c->mr_vol64;
+    for (int a = 0; a < 3; ++a) { mop.cf[a] = c->mr_cf32[a]; mop64.cf[a] = c->mr_cf64[a]; }
+  }
   auto residual = [&]() -> int {
+    if (mr)
+      TIMED(c, PF_K_RESIDUAL64,
+            hipLaunchKernelGGL(k_mr_residual64, dim3(G), dim3(256), 0, c->stream, va, mop64, c->b64, c->x64, c->r64));
+    else
     TIMED(c, PF_K_RESIDUAL64,
           hipLaunchKernelGGL(k_residual64, dim3(G), dim3(256), 0, c->stream, va, c->b64, c->x64, c->r64, c->cd64,
                              c->cx64, c->cy64, c->cz64));
@@ -569,12 +829,18 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const int max_updates = cfg->max_outer > 0 ? cfg->max_outer : 200;
   const int check = cfg->check_every > 0 ? cfg->check_every : 8;
   const double delta = (cfg->inner_rel_tol > 0 && cfg->inner_rel_tol < 1) ? cfg->inner_rel_tol : 0.1;
+  // the multires rule: max(absolute, relative * initial max residual), with x0 = 0
+  // (multires_pressure3d.cpp:858-863)
+  const double mr_tol = std::max(std::max(0.0, cfg->tol), std::max(0.0, mr_rel_tol) * rmax);
   auto met = [&]() {
+    if (mr) return rmax < mr_tol;
     if (cfg->stop_rule == PF_STOP_REL2) return bnorm2 == 0.0 || std::sqrt(rr) <= cfg->tol * std::sqrt(bnorm2);
     return rmax < cfg->tol;
   };
   // FP32 residual level that triggers a (checking) replacement: the target itself
-  const double tol2_abs = cfg->stop_rule == PF_STOP_REL2 ? cfg->tol * cfg->tol * bnorm2 : cfg->tol * cfg->tol;
+  // (none for the multires rows, whose FP32 residual is scaled by the cell volume)
+  const double tol2_abs = mr ? 0.0
+                             : (cfg->stop_rule == PF_STOP_REL2 ? cfg->tol * cfg->tol * bnorm2 : cfg->tol * cfg->tol);
   static const bool trace = std::getenv("PF_SOLVE_TRACE") != nullptr;
   // most FP32 iterations between two residual replacements
   const char* env_between = std::getenv("PF_SOLVE_REPLACE_EVERY");
@@ -628,6 +894,10 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     for (;;) {
       const size_t rec0 = c->krec.size();
       for (int q = 0; q < check; ++q) {
+        if (mr)
+          TIMED(c, PF_K_PCG_SPMV,
+                hipLaunchKernelGGL(k_mr_spmv, dim3(GS), dim3(256), 0, c->stream, vs, mop, c->z32, c->pA, c->pB, c->Ap));
+        else
         TIMED(c, PF_K_PCG_SPMV,
               hipLaunchKernelGGL(k_pcg_spmv, dim3(GS), dim3(256), 0, c->stream, vs, c->z32, c->pA, c->pB, c->Ap,
                                  c->cd32, c->cx32, c->cy32, c->cz32));
@@ -739,9 +1009,27 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   S.true_max_residual = rmax;
   S.true_rel_residual = bnorm2 > 0 ? std::sqrt(rr / bnorm2) : 0.0;
   S.fluid_cells = c->u32_host[4];
+  if (mr) S.pinned_cell = S.pinned ? 0 : -1;  // index in the reference's FLUID cell order
   record(c, timing, ev_solve);
 
   // 4. pressure store and face velocity update
+  if (mr) {
+    const long long ncell = c->dc.nblocks * kBlockVol;
+    hipLaunchKernelGGL(k_mr_store_pressure, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, mr_grid(c), c->x64,
+                       c->f[F_P], c->mr_pfull, c->act[F_P]);
+    const Dims* fd[3] = {&c->du, &c->dv, &c->dw};
+    const PhaseK pk = phase_k(ph);
+    for (int a = 0; a < 3; ++a)
+      TIMED(c, PF_K_VELOCITY,
+            hipLaunchKernelGGL(k_mr_velocity, dim3(grid_for(fd[a]->nblocks * kBlockVol, 256)), dim3(256), 0, c->stream,
+                               mr_grid(c), a, c->mr_pfull, c->f[F_U + a], c->f[F_MU + a], pk, dt));
+    LAUNCH_CHECK(c);
+    record(c, timing, ev_vel);
+    std::swap(c->pblk, c->pblk_prev);
+    c->npblk_prev = npb;
+    if (out) *out = S;
+    return PF_OK;
+  }
   hipLaunchKernelGGL(k_store_pressure, dim3(G), dim3(256), 0, c->stream, va, c->marker, c->x64, c->f[F_P],
                      c->act[F_P]);
   LAUNCH_CHECK(c);
@@ -988,6 +1276,9 @@ int pf_destroy(pf_ctx* c) {
   for (float* p : f32) hipFree(p);
   double* f64[] = {c->b64, c->x64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64};
   for (double* p : f64) hipFree(p);
+  dfree(c->mr_lvl); dfree(c->mr_vol64); dfree(c->mr_pfull); dfree(c->mr_bbox); dfree(c->mr_leaves_dev);
+  for (int a = 0; a < 3; ++a) { dfree(c->mr_valid[a]); dfree(c->mr_cf32[a]); dfree(c->mr_cf64[a]); }
+  if (c->mr_bbox_host) hipHostFree(c->mr_bbox_host);
   dfree(c->fbeta); dfree(c->fphi); dfree(c->tiles); dfree(c->tile_flags); dfree(c->snaps);
   hipFree(c->sc); hipFree(c->partial); hipFree(c->red);
   hipHostFree(c->red_host); hipHostFree(c->done_host); hipHostFree(c->u32_host);
@@ -1560,6 +1851,149 @@ int pf_lpc_train(pf_lpc* L, pf_ctx* c, int32_t steps, double lr, uint64_t seed, 
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (mean_loss) *mean_loss = steps > 0 ? lsum / steps : 0.0;
+  return PF_OK;
+}
+
+// ---------------------------------------------------------------- multires
+
+int pf_mr_set_layout(pf_ctx* c, int64_t n, const int32_t* leaves) {
+  if (!c || n <= 0 || !leaves) return PF_ERR_ARG;
+  if (c->geom.ox != 0.0 || c->geom.oy != 0.0 || c->geom.oz != 0.0)
+    return fail(c, PF_ERR_ARG, "the multires path takes positions relative to the origin 0");
+  std::vector<Leaf> lv((size_t)n);
+  for (int64_t a = 0; a < n; ++a)
+    for (int q = 0; q < 4; ++q) lv[(size_t)a][q] = leaves[4 * a + q];
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return mr_apply_layout(c, lv);
+}
+
+int pf_mr_get_layout(pf_ctx* c, int32_t* leaves, int64_t cap, int64_t* n) {
+  if (!c || !n) return PF_ERR_ARG;
+  *n = (int64_t)c->mr_leaves.size() / 4;
+  if (leaves) std::copy(c->mr_leaves.begin(), c->mr_leaves.begin() + 4 * std::min<int64_t>(cap, *n), leaves);
+  return PF_OK;
+}
+
+int64_t pf_mr_layout_fine_box(int32_t nx, int32_t ny, int32_t nz, int32_t level, int32_t box_valid, const int32_t* box,
+                              int32_t* leaves, int64_t cap) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || level < 0 || level > 6 || (box_valid && !box)) return -1;
+  std::vector<Leaf> lv = mr_layout_box(nx, ny, nz, level, box_valid != 0, box);
+  if (leaves)
+    for (int64_t a = 0; a < std::min<int64_t>(cap, (int64_t)lv.size()); ++a)
+      for (int q = 0; q < 4; ++q) leaves[4 * a + q] = lv[(size_t)a][q];
+  return (int64_t)lv.size();
+}
+
+int pf_mr_face_valid(pf_ctx* c, int32_t axis, uint8_t* dense) {
+  if (!c || !dense || axis < 0 || axis > 2) return PF_ERR_ARG;
+  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  const Dims& d = axis == 0 ? c->du : (axis == 1 ? c->dv : c->dw);
+  std::vector<uint8_t> bm((size_t)(d.nblocks * kBlockVol));
+  HIP_TRY(c, hipMemcpyAsync(bm.data(), c->mr_valid[axis], bm.size(), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) dense[i + (long long)d.nx * (j + (long long)d.ny * k)] = bm[(size_t)bm_index(d, i, j, k)];
+  return PF_OK;
+}
+
+int pf_mr_mark_cells(pf_ctx* c) {
+  if (!c) return PF_ERR_ARG;
+  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  int st = ensure_binned(c);
+  if (st) return st;
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  HIP_TRY(c, fill_u8(c->stream, c->act[F_MK], 0, c->dc.nblocks));
+  hipLaunchKernelGGL(k_mr_mark, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, mr_grid(c), c->cell_start,
+                     c->marker, c->act[F_MK]);
+  LAUNCH_CHECK(c);
+  return PF_OK;
+}
+
+int pf_mr_p2g(pf_ctx* c, const pf_phase* ph, double Vp) {
+  if (!c || !ph) return PF_ERR_ARG;
+  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  int st = run_p2g_splat(c, ph, Vp, false, true);
+  if (st) return st;
+  return run_face_ops(c, FO_NORMALIZE, 0.f);
+}
+
+static pf_solve_cfg mr_to_solve(const pf_mr_solve_cfg* m) {
+  pf_solve_cfg s{};
+  s.max_iters = m->max_iterations;
+  s.stop_rule = PF_STOP_REF_MAXABS;
+  s.tol = m->absolute_tolerance;
+  s.inner_rel_tol = m->inner_rel_tol > 0 ? m->inner_rel_tol : 0.1;
+  s.max_outer = m->max_outer > 0 ? m->max_outer : 10000;
+  s.check_every = m->check_every > 0 ? m->check_every : 8;
+  s.precond = PF_PRECOND_JACOBI;
+  return s;
+}
+
+int pf_mr_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_mr_solve_cfg* cfg, double c_div,
+                  pf_solve_stats* stats) {
+  if (!c || !ph || !cfg) return PF_ERR_ARG;
+  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  if (cfg->max_iterations < 0 || cfg->absolute_tolerance < 0 || cfg->relative_tolerance < 0)
+    return fail(c, PF_ERR_ARG, "invalid multires solve configuration");
+  const pf_solve_cfg s = mr_to_solve(cfg);
+  return run_project(c, ph, dt, &s, c_div, stats, false, 0, 0, 0, false, cfg->relative_tolerance);
+}
+
+int pf_mr_update_refinement(pf_ctx* c, const pf_mr_refine_cfg* rc, int32_t* fine_leaves, int32_t* budget_limited) {
+  if (!c || !rc) return PF_ERR_ARG;
+  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  int st = mr_dynamic(c, rc);
+  if (st) return st;
+  if (fine_leaves) *fine_leaves = c->mr_last_fine_leaves;
+  if (budget_limited) *budget_limited = c->mr_budget_limited;
+  return PF_OK;
+}
+
+int pf_mr_step(pf_ctx* c, const pf_mr_step_cfg* cfg, pf_step_stats* stats) {
+  if (!c || !cfg) return PF_ERR_ARG;
+  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  pf_step_stats S;
+  std::memset(&S, 0, sizeof(S));
+  const bool timing = cfg->timing != 0;
+  record(c, timing, 0);
+  int st = mr_dynamic(c, &cfg->refine);
+  if (st) return st;
+  st = ensure_binned(c);
+  if (st) return st;
+  record(c, timing, 1);
+  st = pf_timestep(c, cfg->dt, cfg->adaptive_timestep, cfg->adaptive_cfl, cfg->adaptive_min_dt, &S.effective_dt,
+                   &S.max_particle_speed, &S.cfl_limit_dt, &S.limited);
+  if (st) return st;
+  const double dt = S.effective_dt;
+  st = pf_mr_mark_cells(c);
+  if (st) return st;
+  record(c, timing, 2);
+  st = run_p2g_splat(c, &cfg->phase, cfg->Vp, false, true);
+  if (st) return st;
+  record(c, timing, 3);
+  st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | FO_WALLS, (float)(dt * cfg->gravity));
+  if (st) return st;
+  record(c, timing, 4);
+  const pf_solve_cfg s = mr_to_solve(&cfg->solve);
+  st = run_project(c, &cfg->phase, dt, &s, cfg->c_div, &S.solve, timing, 5, 6, 7, false, cfg->solve.relative_tolerance);
+  if (st) return st;
+  HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->esc_dev), 0u, 8));
+  st = run_g2p_advect(c, true, true, cfg->alpha_liquid, cfg->alpha_gas, dt, cfg->advection_order);
+  if (st) return st;
+  record(c, timing, 8);
+  HIP_TRY(c, hipMemcpyAsync(&S.escape, c->esc_dev, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (timing) {
+    float ms;
+    for (int q = 0; q < PF_NSTAGES; ++q) {
+      hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]);
+      S.stage_ms[q] = ms;
+    }
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[PF_NSTAGES]);
+    S.total_ms = ms;
+  }
+  if (stats) *stats = S;
   return PF_OK;
 }
 

@@ -167,6 +167,23 @@ struct Ctx {
   float* snaps = nullptr;
   int snap_cap = 0, nsnap = 0, snap_every = 0, snap_last_iter = -1;
 
+  // multires layout (2-level leaves of 4^3 blocks over the fine grid)
+  bool mr = false;
+  std::vector<int32_t> mr_leaves;   // sorted (level, bx, by, bz)
+  int32_t* mr_leaves_dev = nullptr;
+  long long mr_leaves_cap = 0;
+  uint8_t* mr_lvl = nullptr;        // level byte per fine cell
+  uint8_t* mr_valid[3] = {};        // enumerated fine faces
+  float* mr_cf32[3] = {};           // coupling conductance per fine face
+  double* mr_cf64[3] = {};
+  double* mr_vol64 = nullptr;       // leaf-cell volume at FLUID anchors
+  double* mr_pfull = nullptr;       // FP64 pressure per fine cell
+  int* mr_bbox = nullptr;           // particle cell boxes (device)
+  int* mr_bbox_host = nullptr;
+  bool mr_retained_valid = false;   // dynamic refinement state (MRSim3DTP::dynamic_retained_*)
+  int mr_retained[6] = {0, 0, 0, 0, 0, 0};
+  int mr_budget_limited = 0, mr_last_fine_leaves = 0;
+
   // per-kernel event timing (measurement hook)
   bool ktime = false;
   std::vector<hipEvent_t> kev;

@@ -44,6 +44,7 @@ struct P2GArgs {
   bool pow2;              // dx is a power of two: (x - o)/dx == (x - o)*inv_dx exactly
   double inv_dx;
   float mp_l, mp_g;       // rho * Vp per phase
+  const uint8_t* valid[3];  // KT = 2 (multires): the enumerated fine faces per axis
 };
 
 constexpr int kOrdMax = 16;
@@ -126,6 +127,21 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
   for (int o = 0; o < A::NY; ++o) { int c = j + o - 1; vy[o] = c >= 0 && c < fd.ny; }
 #pragma unroll
   for (int o = 0; o < A::NZ; ++o) { int c = k + o - 1; vz[o] = c >= 0 && c < fd.nz; }
+  // multires: only the enumerated faces take (and normalise) weights
+  // (collectWeightedFaces, multires_ops3d_tp.cpp:62-75)
+  uint64_t vbits = ~0ull;
+  if (KT == 2) {
+    vbits = 0;
+#pragma unroll
+    for (int oz = 0; oz < A::NZ; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < A::NY; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < A::NX; ++ox)
+          if (vx[ox] && vy[oy] && vz[oz] &&
+              a.valid[AX][bm_index(fd, i + ox - 1, j + oy - 1, k + oz - 1)])
+            vbits |= 1ull << (ox + A::NX * (oy + A::NY * oz));
+  }
 
   uint32_t last = 0;
   bool first = true;
@@ -188,7 +204,7 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
             float t = 1.f - d2 * (1.f / 2.25f);
             w = act ? t * t * t : 0.f;
           }
-          act = act && inb;
+          act = act && inb && ((vbits >> n) & 1ull);
           w = act ? w : 0.f;
           wt[n] = w;
           wsum += w;
@@ -242,7 +258,8 @@ __device__ inline void flush_axis(const P2GArgs& a, int bx, int by, int bz, cons
   }
 }
 
-// KT = 0: normalized cubic (two-phase); KT = 1: trilinear, unit mass (single-phase)
+// KT = 0: normalized cubic (two-phase); KT = 1: trilinear, unit mass (single-phase);
+// KT = 2: normalized cubic over the enumerated multires faces
 template <int KT>
 __global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
   __shared__ float s_mom[3][256];

@@ -63,6 +63,22 @@ class EscapeStats(C.Structure):
 MODEL_TWO_PHASE, MODEL_SINGLE_PHASE = 0, 1
 
 
+class MRSolveCfg(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("check_every", C.c_int32), ("absolute_tolerance", C.c_double),
+                ("relative_tolerance", C.c_double), ("inner_rel_tol", C.c_double), ("max_outer", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+def mr_solve_cfg(max_iterations=160, abs_tol=1e-7, rel_tol=1e-6):
+    """MRSim3DTP's solver defaults (cg_iters 160, cg_tol 1e-7, cg_rel_tol 1e-6)."""
+    return MRSolveCfg(max_iterations, 8, abs_tol, rel_tol, 0.1, 10000, 0)
+
+
+class MRRefineCfg(C.Structure):
+    _fields_ = [("dynamic_refinement", C.c_int32), ("particle_padding", C.c_int32), ("gas_padding", C.c_int32),
+                ("hysteresis_cells", C.c_int32), ("max_fine_leaves", C.c_int32), ("reserved", C.c_int32)]
+
+
 class StepCfg(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("Vp", C.c_double),
                 ("alpha_liquid", C.c_double), ("alpha_gas", C.c_double),
@@ -70,6 +86,15 @@ class StepCfg(C.Structure):
                 ("c_div", C.c_double), ("adaptive_timestep", C.c_int32),
                 ("advection_order", C.c_int32), ("timing", C.c_int32), ("model", C.c_int32),
                 ("phase", Phase), ("solve", SolveCfg)]
+
+
+class MRStepCfg(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("Vp", C.c_double),
+                ("alpha_liquid", C.c_double), ("alpha_gas", C.c_double),
+                ("adaptive_cfl", C.c_double), ("adaptive_min_dt", C.c_double),
+                ("c_div", C.c_double), ("adaptive_timestep", C.c_int32),
+                ("advection_order", C.c_int32), ("timing", C.c_int32), ("reserved", C.c_int32),
+                ("phase", Phase), ("solve", MRSolveCfg), ("refine", MRRefineCfg)]
 
 
 class StepStats(C.Structure):
@@ -126,6 +151,16 @@ def lib():
             "pf_p2g_single": (C.c_int, [vp]),
             "pf_project_single": (C.c_int, [vp, dbl, C.POINTER(SolveCfg), C.POINTER(SolveStats)]),
             "pf_stream": (vp, [vp]),
+            "pf_mr_set_layout": (C.c_int, [vp, i64, vp]),
+            "pf_mr_get_layout": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
+            "pf_mr_layout_fine_box": (i64, [i32, i32, i32, i32, i32, vp, vp, i64]),
+            "pf_mr_face_valid": (C.c_int, [vp, i32, vp]),
+            "pf_mr_mark_cells": (C.c_int, [vp]),
+            "pf_mr_p2g": (C.c_int, [vp, C.POINTER(Phase), dbl]),
+            "pf_mr_project": (C.c_int, [vp, C.POINTER(Phase), dbl, C.POINTER(MRSolveCfg), dbl,
+                                         C.POINTER(SolveStats)]),
+            "pf_mr_update_refinement": (C.c_int, [vp, C.POINTER(MRRefineCfg), C.POINTER(i32), C.POINTER(i32)]),
+            "pf_mr_step": (C.c_int, [vp, C.POINTER(MRStepCfg), C.POINTER(StepStats)]),
             "pf_lpc_create": (C.c_int, [C.POINTER(vp), C.c_uint64]),
             "pf_lpc_destroy": (C.c_int, [vp]),
             "pf_lpc_last_error": (C.c_char_p, [vp]),
@@ -498,3 +533,100 @@ def lpc_record(ctx, max_snapshots, every=1):
 
 def lpc_snapshot_count(ctx):
     return int(lib().pf_lpc_snapshot_count(ctx.h))
+
+
+def mr_layout_fine_box(nx, ny, nz, box=None, level=1):
+    """layoutWithFineBox: setCoarseEverywhere(level) + refineFineCellBox(box)."""
+    b = np.zeros(6, np.int32) if box is None else np.ascontiguousarray(box, np.int32)
+    n = lib().pf_mr_layout_fine_box(nx, ny, nz, level, int(box is not None), _p(b), None, 0)
+    out = np.zeros((max(n, 1), 4), np.int32)
+    lib().pf_mr_layout_fine_box(nx, ny, nz, level, int(box is not None), _p(b), _p(out), n)
+    return out[:n]
+
+
+class MRContext(Context):
+    """A context with a multires layout (MRSim3DTP's grid on the GPU)."""
+
+    def set_layout(self, leaves):
+        lv = np.ascontiguousarray(np.asarray(leaves, np.int32).reshape(-1, 4))
+        self._check(lib().pf_mr_set_layout(self.h, len(lv), _p(lv)), "pf_mr_set_layout")
+
+    def layout(self):
+        n = C.c_int64()
+        self._check(lib().pf_mr_get_layout(self.h, None, 0, C.byref(n)), "pf_mr_get_layout")
+        out = np.zeros((max(n.value, 1), 4), np.int32)
+        self._check(lib().pf_mr_get_layout(self.h, _p(out), n.value, C.byref(n)), "pf_mr_get_layout")
+        return out[:n.value]
+
+    def face_valid(self, axis):
+        d = np.zeros(3, np.int32)
+        self._check(lib().pf_field_dims(self.h, axis, _p(d)), "pf_field_dims")
+        out = np.zeros(int(d.prod()), np.uint8)
+        self._check(lib().pf_mr_face_valid(self.h, axis, _p(out)), "pf_mr_face_valid")
+        return out.reshape(d[2], d[1], d[0])
+
+    def mr_mark_cells(self):
+        self._check(lib().pf_mr_mark_cells(self.h), "pf_mr_mark_cells")
+
+    def mr_p2g(self, phase, Vp=1.0):
+        self._check(lib().pf_mr_p2g(self.h, C.byref(phase), Vp), "pf_mr_p2g")
+
+    def mr_project(self, phase, dt, cfg=None, c_div=0.0):
+        cfg = cfg or mr_solve_cfg()
+        st = SolveStats()
+        self._check(lib().pf_mr_project(self.h, C.byref(phase), dt, C.byref(cfg), c_div, C.byref(st)),
+                    "pf_mr_project")
+        return st
+
+    def mr_update_refinement(self, refine):
+        fl, bl = C.c_int32(), C.c_int32()
+        self._check(lib().pf_mr_update_refinement(self.h, C.byref(refine), C.byref(fl), C.byref(bl)),
+                    "pf_mr_update_refinement")
+        return fl.value, bl.value
+
+    def mr_step(self, cfg):
+        st = StepStats()
+        self._check(lib().pf_mr_step(self.h, C.byref(cfg), C.byref(st)), "pf_mr_step")
+        return st
+
+
+class MRSim3DTP:
+    """GPU mirror of the reference's MRSim3DTP (src/driver/multires_sim3d_tp.h:10-191):
+    the fields step() reads, dynamic refinement, step()."""
+
+    def __init__(self, nx, ny, nz, dx):
+        self.ctx = MRContext(nx, ny, nz, dx)
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.phase = Phase(1.0, 0.01, 1.0, 1.0)
+        self.dt, self.gravity, self.Vp = 0.02, -9.81, 1.0
+        self.alpha_liquid, self.alpha_gas = 0.95, 0.95
+        self.adaptive_timestep, self.adaptive_cfl, self.adaptive_min_dt = False, 0.5, 1e-5
+        self.advection_order = 2
+        self.cg_iters, self.cg_tol, self.cg_rel_tol = 160, 1e-7, 1e-6
+        self.dynamic_refinement, self.dynamic_particle_padding, self.dynamic_gas_padding = True, 1, 2
+        self.dynamic_hysteresis_cells, self.dynamic_max_fine_leaves = 1, 0
+        self.c_div_last = 0.0
+        self.timing = False
+        self.stats_last = None
+
+    def set_layout(self, leaves):
+        self.ctx.set_layout(leaves)
+
+    def set_particles(self, x, y, z, u=None, v=None, w=None, type=None, volume=None):
+        self.ctx.upload(x, y, z, u, v, w, type, volume)
+
+    def step_cfg(self):
+        solve = MRSolveCfg(self.cg_iters, 8, self.cg_tol, self.cg_rel_tol, 0.1, 10000, 0)
+        refine = MRRefineCfg(int(self.dynamic_refinement), self.dynamic_particle_padding, self.dynamic_gas_padding,
+                             self.dynamic_hysteresis_cells, self.dynamic_max_fine_leaves, 0)
+        return MRStepCfg(dt=self.dt, gravity=self.gravity, Vp=self.Vp, alpha_liquid=self.alpha_liquid,
+                         alpha_gas=self.alpha_gas, adaptive_cfl=self.adaptive_cfl,
+                         adaptive_min_dt=self.adaptive_min_dt, c_div=self.c_div_last,
+                         adaptive_timestep=int(self.adaptive_timestep), advection_order=self.advection_order,
+                         timing=int(self.timing), reserved=0, phase=self.phase, solve=solve, refine=refine)
+
+    def step(self):
+        """MRSim3DTP::step, src/driver/multires_sim3d_tp.cpp:671-734."""
+        st = self.ctx.mr_step(self.step_cfg())
+        self.stats_last = st
+        return st

@@ -222,6 +222,65 @@ int pf_kernel_time(pf_ctx* ctx, int32_t kernel, int64_t* launches, double* total
  * calibrating HBM traffic counters */
 int pf_copy_probe(pf_ctx* ctx, int64_t n, double* ms);
 
+/* ---- multires two-phase path (MRSim3DTP, src/driver/multires_sim3d_tp.h:8-191).
+ * The context's grid is the fine grid; a layout of 2-level leaves (4^3 blocks at
+ * level 0 = fine or 1 = coarse, MRLayout3D, src/grid/multires_layout3d.h:27-162)
+ * turns it into the reference's multires grid: fields stay dense fine arrays (faces
+ * the reference does not enumerate hold zero), markers and pressure are per leaf
+ * cell (broadcast over its fine cells in downloads).  Positions are relative to the
+ * origin 0, as the reference's multires code assumes. */
+typedef struct pf_mr_solve_cfg {    /* MRPressureSolveConfig3D (multires_pressure3d.h:137-166) */
+  int32_t max_iterations;           /* cap on inner iterations (reference max_iterations) */
+  int32_t check_every;              /* inner iterations between host convergence checks */
+  double absolute_tolerance;        /* stop when |rhs - A x|_inf < max(abs, rel * |rhs|_inf) */
+  double relative_tolerance;
+  double inner_rel_tol;             /* inner FP32 PCG reduction per refinement pass */
+  int32_t max_outer, reserved;
+} pf_mr_solve_cfg;
+
+typedef struct pf_mr_refine_cfg {   /* MRSim3DTP dynamic refinement fields (multires_sim3d_tp.h:60-68) */
+  int32_t dynamic_refinement, particle_padding, gas_padding, hysteresis_cells, max_fine_leaves, reserved;
+} pf_mr_refine_cfg;
+
+typedef struct pf_mr_step_cfg {     /* the MRSim3DTP fields step() reads */
+  double dt, gravity, Vp, alpha_liquid, alpha_gas;
+  double adaptive_cfl, adaptive_min_dt;
+  double c_div;
+  int32_t adaptive_timestep, advection_order;
+  int32_t timing, reserved;
+  pf_phase phase;
+  pf_mr_solve_cfg solve;
+  pf_mr_refine_cfg refine;
+} pf_mr_step_cfg;
+
+/* MRLayout3D::leaf_blocks (multires_layout3d.h:30): n (level, bx, by, bz) leaves that
+ * tile the domain; they are sorted, the grid fields cleared (MRMacGrid3D(layout)) */
+int pf_mr_set_layout(pf_ctx* ctx, int64_t n, const int32_t* leaves);
+int pf_mr_get_layout(pf_ctx* ctx, int32_t* leaves, int64_t capacity, int64_t* n);
+/* layoutWithFineBox (multires_sim3d_tp.cpp:172-180): setCoarseEverywhere(level) then
+ * refineFineCellBox(box = x0, y0, z0, x1, y1, z1) when box_valid; returns the leaf
+ * count, writes at most capacity leaves (host only) */
+int64_t pf_mr_layout_fine_box(int32_t nx, int32_t ny, int32_t nz, int32_t level, int32_t box_valid,
+                              const int32_t* box, int32_t* leaves, int64_t capacity);
+/* the enumerated faces of an axis (MRMacGrid3D::uFaces etc., multires_mac_grid3d.h:43-45)
+ * as one byte per fine face, dense in the face field's extents */
+int pf_mr_face_valid(pf_ctx* ctx, int32_t axis, uint8_t* dense);
+/* markCells, multires_sim3d_tp.cpp:39-70 (also bins particles) */
+int pf_mr_mark_cells(pf_ctx* ctx);
+/* mrP2G3D_tp, multires_ops3d_tp.cpp:213-278 / multires_ops3d_tp.h:8 */
+int pf_mr_p2g(pf_ctx* ctx, const pf_phase* phase, double Vp);
+/* projectMR3D(grid, phase, dt, config, stats), multires_pressure3d.cpp:747-1524 /
+ * multires_pressure3d.h:194-196; gravity and walls are pf_apply_gravity /
+ * pf_apply_walls, G2P and advection pf_g2p / pf_advect (mrG2P3D_tp, mrAdvect3D_tp) */
+int pf_mr_project(pf_ctx* ctx, const pf_phase* phase, double dt, const pf_mr_solve_cfg* cfg, double c_div,
+                  pf_solve_stats* stats);
+/* MRSim3DTP::updateDynamicRefinement, multires_sim3d_tp.cpp:577-669 (state kept in
+ * the context); reports the fine leaf count and whether the budget cut the box */
+int pf_mr_update_refinement(pf_ctx* ctx, const pf_mr_refine_cfg* cfg, int32_t* fine_leaves, int32_t* budget_limited);
+/* MRSim3DTP::step, multires_sim3d_tp.cpp:671-734 (default flags: no particle
+ * adaptivity, secondary particles or surface tension) */
+int pf_mr_step(pf_ctx* ctx, const pf_mr_step_cfg* cfg, pf_step_stats* stats);
+
 /* ---- learned block preconditioner (no reference counterpart: the reference
  * preconditions with Jacobi, sparse_ops3d_tp.cpp:322-327).  A 3-layer 3x3x3
  * convolutional network over 16^3 cell tiles with a 1-cell halo (widths 16-32-16

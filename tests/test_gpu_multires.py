@@ -1,0 +1,170 @@
+"""Multires two-phase path (MRSim3DTP) on the HIP path against the multires oracle.
+
+Bars: layouts, enumerated faces, markers, refinement boxes and fine-leaf counts
+bit-exact; face mass and momentum within 1e-5 norm-wise relative error of the FP64
+oracle; pressure solved to the reference's max-norm rule, compared with a tight
+tolerance; particle state after whole steps within the stated tolerances.
+"""
+import numpy as np
+import pytest
+
+import pfflip_amd as G
+import pyoracle as O
+from _helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-5
+
+
+def _phase(rho_l=1.0, rho_g=0.01):
+    ph = O.Phase(rho_l, rho_g, 1.0, 1.0)
+    ph.rho_tilde_0 = O.calibrate_rho_tilde0(ph, 1.0)
+    return ph, G.Phase(ph.rho_l, ph.rho_g, ph.alpha_phi, ph.rho_tilde_0)
+
+
+def _bubble_particles(n, seed=3, vel=True):
+    """the reference's bubble-tank lattice (multires_sim3d_tp.cpp:483-496), FP32 values"""
+    water = n // 2
+    cx, cy, cz = n * 0.5, water * 0.375, n * 0.5
+    r = max(1.25, n * 0.1875)
+    pts = []
+    for k in range(1, n - 1):
+        for j in range(1, water):
+            for i in range(1, n - 1):
+                gas = (i + 0.5 - cx) ** 2 + (j + 0.5 - cy) ** 2 + (k + 0.5 - cz) ** 2 < r * r
+                for s in range(8):
+                    pts.append((i + 0.25 + 0.5 * (s & 1), j + 0.25 + 0.5 * ((s >> 1) & 1),
+                                k + 0.25 + 0.5 * ((s >> 2) & 1), 1 if gas else 0))
+    p = np.array(pts)
+    rng = np.random.default_rng(seed)
+    x, y, z = (p[:, q].astype(np.float32) for q in range(3))
+    u, v, w = ((rng.normal(scale=0.3, size=len(p)) if vel else np.zeros(len(p))).astype(np.float32) for _ in range(3))
+    t = p[:, 3].astype(np.uint8)
+    return x, y, z, u, v, w, t
+
+
+def _ctx(n, leaves, parts):
+    ctx = G.MRContext(n, n, n, 1.0)
+    ctx.set_layout(leaves)
+    ctx.upload(*parts)
+    g = O.MRGrid(n, n, n, 1.0, faithful=False)
+    g.set_leaves(leaves)
+    x, y, z, u, v, w, t = parts
+    return ctx, g, O.Particles(x, y, z, u, v, w, type=t)
+
+
+@pytest.mark.parametrize("dims,box", [((16, 16, 16), [4, 4, 4, 8, 12, 12]), ((17, 10, 9), [4, 3, 2, 13, 9, 8])])
+def test_layout_and_enumerated_faces(dims, box):
+    nx, ny, nz = dims
+    leaves = O.mr_layout_fine_box(nx, ny, nz, box)
+    ctx = G.MRContext(nx, ny, nz, 1.0)
+    ctx.set_layout(leaves)
+    g = O.MRGrid(nx, ny, nz, 1.0)
+    g.set_leaves(leaves)
+    assert np.array_equal(ctx.layout(), g.leaves())
+    for a in range(3):
+        assert np.array_equal(ctx.face_valid(a), g.face_valid(a)), a
+
+
+def test_markers_and_p2g_match_fp64_oracle():
+    n = 16
+    ph, gph = _phase()
+    leaves = O.mr_layout_bubble_band(n, n, n)
+    ctx, g, ps = _ctx(n, leaves, _bubble_particles(n))
+    ctx.mr_mark_cells()
+    g.mark_cells(ps)
+    assert np.array_equal(ctx.field(G.FIELD_MARKER), g.dense(O.MK))
+    ctx.mr_p2g(gph, 1.0)
+    g.p2g(ps, ph, 1.0)
+    for f in range(6):
+        assert rel_err(ctx.field(f), g.dense(f)) < FIELD_TOL, f
+        valid = g.face_valid(f % 3)
+        assert not ((ctx.field(f) != 0) & (valid == 0)).any(), f
+    for fv in range(3):
+        mom = ctx.field(fv).astype(np.float64) * ctx.field(fv + 3)
+        assert rel_err(mom, g.dense(fv) * g.dense(fv + 3)) < FIELD_TOL
+
+
+def test_projection_matches_fp64_oracle():
+    n, dt = 16, 0.02
+    ph, gph = _phase()
+    leaves = O.mr_layout_bubble_band(n, n, n)
+    ctx, g, ps = _ctx(n, leaves, _bubble_particles(n))
+    ctx.mr_mark_cells(); ctx.mr_p2g(gph, 1.0); ctx.save_velocity(); ctx.apply_gravity(dt, -9.81); ctx.apply_walls()
+    g.mark_cells(ps); g.p2g(ps, ph, 1.0); g.apply_gravity(dt, -9.81); g.apply_walls()
+    for f in range(3):
+        assert rel_err(ctx.field(f), g.dense(f)) < FIELD_TOL
+    st = ctx.mr_project(gph, dt, G.mr_solve_cfg(20000, 1e-11, 0.0))
+    ost = g.project(ph, dt, O.mr_solve_cfg(max_iterations=20000, abs_tol=1e-12, rel_tol=0.0))
+    assert st.converged == 1 and st.true_max_residual < 1e-11
+    assert st.fluid_cells == ost.active_cells and st.pinned_cell == ost.pinned_cell
+    assert rel_err(ctx.field(G.FIELD_P), g.dense(O.P)) < 1e-6
+    for f in range(3):
+        assert rel_err(ctx.field(f), g.dense(f)) < 1e-5, f
+
+
+def test_reference_stop_rule_and_defaults():
+    # MRSim3DTP's defaults (cg_iters 160, cg_tol 1e-7, cg_rel_tol 1e-6): the solve
+    # stops on |rhs - A x|_inf < max(1e-7, 1e-6 |rhs|_inf)
+    n, dt = 16, 0.02
+    ph, gph = _phase()
+    ctx, g, ps = _ctx(n, O.mr_layout_bubble_band(n, n, n), _bubble_particles(n))
+    ctx.mr_mark_cells(); ctx.mr_p2g(gph, 1.0); ctx.apply_gravity(dt, -9.81); ctx.apply_walls()
+    st = ctx.mr_project(gph, dt, G.mr_solve_cfg())
+    tol = max(1e-7, 1e-6 * st.initial_max_residual)
+    assert st.converged == 1 and st.true_max_residual < tol
+
+
+def test_steps_match_fp64_oracle_with_dynamic_refinement():
+    n, dt = 16, 0.02
+    ph, gph = _phase()
+    parts = _bubble_particles(n, vel=False)
+    leaves = O.mr_layout_fine_box(n, n, n)          # coarse everywhere: the first step refines
+    ctx, g, ps = _ctx(n, leaves, parts)
+    sim = G.MRSim3DTP(n, n, n, 1.0)
+    sim.ctx = ctx
+    sim.phase = gph
+    sim.dt = dt
+    sim.cg_iters, sim.cg_tol, sim.cg_rel_tol = 20000, 1e-11, 0.0
+    osim = O.mr_sim(n, n, n, 1.0, phase=ph, dt=dt,
+                    solve=O.mr_solve_cfg(max_iterations=20000, abs_tol=1e-12, rel_tol=0.0))
+    pa = ps.copy()
+    for s in range(2):
+        st = sim.step()
+        ost = O.mr_step(g, osim, pa)
+        assert st.solve.converged == 1
+        if s == 0:
+            assert np.array_equal(ctx.layout(), g.leaves())
+    x, y, z, u, v, w = ctx.download()
+    assert max(np.abs(x - pa.x).max(), np.abs(y - pa.y).max(), np.abs(z - pa.z).max()) < 1e-5
+    assert rel_err(np.concatenate([u, v, w]), np.concatenate([pa.u, pa.v, pa.w])) < 1e-4
+
+
+def test_refinement_box_hysteresis_and_budget():
+    # updateDynamicRefinement on a corner column with a gas pocket: the same box,
+    # fine-leaf count and budget flag as the oracle
+    n = 24
+    pts = []
+    for k in range(1, 10):
+        for j in range(1, 12):
+            for i in range(1, 9):
+                for s in range(8):
+                    pts.append((i + 0.25 + 0.5 * (s & 1), j + 0.25 + 0.5 * ((s >> 1) & 1),
+                                k + 0.25 + 0.5 * ((s >> 2) & 1), 1 if (i - 4) ** 2 + (j - 4) ** 2 < 4 else 0))
+    p = np.array(pts)
+    x, y, z = (p[:, q].astype(np.float32) for q in range(3))
+    t = p[:, 3].astype(np.uint8)
+    ps = O.Particles(x, y, z, type=t)
+    for max_fine in (0, 20):
+        ctx = G.MRContext(n, n, n, 1.0)
+        ctx.set_layout(O.mr_layout_fine_box(n, n, n))
+        ctx.upload(x, y, z, type=t)
+        fl, bl = ctx.mr_update_refinement(G.MRRefineCfg(1, 1, 2, 1, max_fine, 0))
+        g = O.MRGrid(n, n, n, 1.0)
+        g.set_leaves(O.mr_layout_fine_box(n, n, n))
+        osim = O.mr_sim(n, n, n, 1.0, max_fine=max_fine)
+        O.mr_dynamic_refinement(g, osim, ps)
+        assert np.array_equal(ctx.layout(), g.leaves())
+        assert (fl, bl) == (osim.dynamic_last_fine_leaves, osim.budget_limited)
+        assert bl == (1 if max_fine else 0)
