@@ -1884,6 +1884,23 @@ int64_t pf_mr_layout_fine_box(int32_t nx, int32_t ny, int32_t nz, int32_t level,
   return (int64_t)lv.size();
 }
 
+int64_t pf_mr_layout_refine(int32_t nx, int32_t ny, int32_t nz, const int32_t* in, int64_t n, const int32_t* box,
+                            int32_t* leaves, int64_t cap) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || n < 0 || (n > 0 && !in) || !box) return -1;
+  std::vector<Leaf> lv((size_t)n);
+  for (int64_t a = 0; a < n; ++a)
+    for (int q = 0; q < 4; ++q) lv[(size_t)a][q] = in[4 * a + q];
+  std::sort(lv.begin(), lv.end());
+  lv.erase(std::unique(lv.begin(), lv.end()), lv.end());
+  const int x0 = std::clamp(box[0], 0, nx), y0 = std::clamp(box[1], 0, ny), z0 = std::clamp(box[2], 0, nz);
+  const int x1 = std::clamp(box[3], 0, nx), y1 = std::clamp(box[4], 0, ny), z1 = std::clamp(box[5], 0, nz);
+  if (x0 < x1 && y0 < y1 && z0 < z1) mr_refine(lv, nx, ny, nz, x0, y0, z0, x1, y1, z1);
+  if (leaves)
+    for (int64_t a = 0; a < std::min<int64_t>(cap, (int64_t)lv.size()); ++a)
+      for (int q = 0; q < 4; ++q) leaves[4 * a + q] = lv[(size_t)a][q];
+  return (int64_t)lv.size();
+}
+
 int pf_mr_face_valid(pf_ctx* c, int32_t axis, uint8_t* dense) {
   if (!c || !dense || axis < 0 || axis > 2) return PF_ERR_ARG;
   if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
@@ -1942,7 +1959,8 @@ int pf_mr_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_mr_solve_cf
 
 int pf_mr_update_refinement(pf_ctx* c, const pf_mr_refine_cfg* rc, int32_t* fine_leaves, int32_t* budget_limited) {
   if (!c || !rc) return PF_ERR_ARG;
-  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  if (c->geom.ox != 0.0 || c->geom.oy != 0.0 || c->geom.oz != 0.0)
+    return fail(c, PF_ERR_ARG, "the multires path takes positions relative to the origin 0");
   int st = mr_dynamic(c, rc);
   if (st) return st;
   if (fine_leaves) *fine_leaves = c->mr_last_fine_leaves;
@@ -1950,15 +1968,24 @@ int pf_mr_update_refinement(pf_ctx* c, const pf_mr_refine_cfg* rc, int32_t* fine
   return PF_OK;
 }
 
+int pf_mr_retained_box(pf_ctx* c, int32_t* box, int32_t* valid) {
+  if (!c || !box || !valid) return PF_ERR_ARG;
+  *valid = c->mr_retained_valid ? 1 : 0;
+  for (int q = 0; q < 6; ++q) box[q] = c->mr_retained_valid ? c->mr_retained[q] : 0;
+  return PF_OK;
+}
+
 int pf_mr_step(pf_ctx* c, const pf_mr_step_cfg* cfg, pf_step_stats* stats) {
   if (!c || !cfg) return PF_ERR_ARG;
-  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  if (c->geom.ox != 0.0 || c->geom.oy != 0.0 || c->geom.oz != 0.0)
+    return fail(c, PF_ERR_ARG, "the multires path takes positions relative to the origin 0");
   pf_step_stats S;
   std::memset(&S, 0, sizeof(S));
   const bool timing = cfg->timing != 0;
   record(c, timing, 0);
   int st = mr_dynamic(c, &cfg->refine);
   if (st) return st;
+  if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout or dynamic refinement)");
   st = ensure_binned(c);
   if (st) return st;
   record(c, timing, 1);

@@ -1,0 +1,97 @@
+// multires_sim3d_tp_gpu.h -- C++17 drop-in for the reference's MRSim3DTP
+// (src/driver/multires_sim3d_tp.h:10-191) whose step runs on one MI355X through the C
+// ABI in include/pf_gpu.h (pf_mr_*).
+//
+// Keeps the fields step() reads and the per-step stats it writes under the
+// reference's names, the layout (as MRLayout3D leaf blocks), initBubbleTankInterfaceBand,
+// updateDynamicRefinement and step().  Particles live on the device between steps
+// (see sparse_sim3d_tp_gpu.h); the default particle adaptivity, secondary-particle
+// and surface-tension flags are off, as in the reference.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pf_gpu.h"
+#include "sparse_sim3d_tp_gpu.h"
+
+namespace pfgpu {
+
+struct MRBlockKey3D {
+  int level = 0, bx = 0, by = 0, bz = 0;
+};
+
+// the MRPressureSolveStats3D fields this build reports
+struct MRPressureSolveStats3D {
+  int active_cells = 0, pinned_cell = -1, iterations = 0;
+  double initial_residual = 0.0, final_residual = 0.0, effective_tolerance = 0.0;
+  bool converged = false, breakdown = false;
+};
+
+class MRSim3DTP {
+ public:
+  Particles3DTP particles;
+  PhaseParams phase;
+  double dt = 0.02, gravity = -9.81, Vp = 1.0, alpha_liquid = 0.95, alpha_gas = 0.95;
+  bool adaptive_timestep = false;
+  double adaptive_cfl = 0.5, adaptive_min_dt = 1e-5;
+  double effective_dt_last = 0.02, max_particle_speed_last = 0.0, cfl_limit_dt_last = 0.02;
+  int adaptive_timestep_limited_last = 0;
+  int advection_order = 2;
+  bool c_div_volume_correction = false;
+  double c_div_strength = 1.0, liquid_volume_target = 0.0, liquid_volume_current_last = 0.0;
+  double liquid_volume_error_last = 0.0, c_div_last = 0.0;
+  int cg_iters = 160;
+  double cg_tol = 1e-7, cg_rel_tol = 1e-6;
+  bool dynamic_refinement = true;
+  int dynamic_particle_padding = 1, dynamic_gas_padding = 2, dynamic_hysteresis_cells = 1;
+  int dynamic_max_fine_leaves = 0;
+  bool dynamic_budget_limited = false;
+  int dynamic_last_fine_leaves = 0;
+  bool dynamic_retained_box_valid = false;
+  int dynamic_retained_x0 = 0, dynamic_retained_y0 = 0, dynamic_retained_z0 = 0;
+  int dynamic_retained_x1 = 0, dynamic_retained_y1 = 0, dynamic_retained_z1 = 0;
+  int particle_boundary_clamped_liquid_last = 0, particle_boundary_clamped_gas_last = 0;
+  MRPressureSolveStats3D last_pressure_stats;
+  pf_step_stats stats_last{};
+
+  MRSim3DTP(int nx, int ny, int nz, double dx);
+  ~MRSim3DTP();
+  MRSim3DTP(const MRSim3DTP&) = delete;
+  MRSim3DTP& operator=(const MRSim3DTP&) = delete;
+
+  void initBubbleTankInterfaceBand();
+  void updateDynamicRefinement();
+  void step();
+
+  // the layout (MRLayout3D::leaves / countLevel / leafAtFineCell)
+  const std::vector<MRBlockKey3D>& leaves() const { return leaves_; }
+  size_t countLevel(int level) const;
+  int levelAtFineCell(int x, int y, int z) const;
+  void setLayout(const std::vector<MRBlockKey3D>& leaves);
+  int activePressureCellCount() const;
+  int uFaceCount();
+  int vFaceCount();
+  int wFaceCount();
+
+  void markParticlesDirty() { dirty_ = true; }
+  void syncParticlesToHost();
+  pf_ctx* context() { return ctx_; }
+
+ private:
+  void check(int status, const char* what);
+  void upload();
+  void fetchLayout();
+  void fetchRetained();
+  int faceCount(int axis);
+  pf_ctx* ctx_ = nullptr;
+  int nx_, ny_, nz_;
+  double dx_;
+  bool dirty_ = true;
+  std::vector<MRBlockKey3D> leaves_;
+};
+
+}  // namespace pfgpu

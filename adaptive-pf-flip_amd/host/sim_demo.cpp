@@ -4,13 +4,24 @@
 //   tests/test_sparse_sim3d_tp.cpp:749-786  bubble rises, headspace stays sparse (24 steps)
 //   tests/test_sparse_sim3d_tp.cpp:150-169  adaptive timestep reports the effective dt
 //   tests/test_sparse_sim3d.cpp:6-36         single-phase dam-break falls, stays sparse (20 steps)
+// and the reference's MRSim3DTP tests against the multires drop-in class:
+//   tests/test_multires_sim3d_tp.cpp:136-148  bubble tank: refined lower band, coarse headspace
+//   tests/test_multires_sim3d_tp.cpp:345-401  dynamic refinement: occupancy, hysteresis, budget
+//   tests/test_multires_sim3d_tp.cpp:403-420  bubble tank step conserves particles (4 steps)
+//   tests/test_multires_sim3d_tp.cpp:422-439  adaptive timestep reports the effective dt
+//   tests/test_multires_sim3d_tp.cpp:558-573  c_div uses the liquid volume error
+//   tests/test_multires_sim3d_tp.cpp:575-594  high density ratio pressure converges
+// Host particle edits are followed by markParticlesDirty() (particles live on the
+// device) and device state is read back with syncParticlesToHost().
 // Exit status 0 when every check passes.
 #include <cmath>
 #include <cstdio>
 
+#include "multires_sim3d_tp_gpu.h"
 #include "sparse_sim3d_gpu.h"
 #include "sparse_sim3d_tp_gpu.h"
 
+using pfgpu::MRSim3DTP;
 using pfgpu::SparseSim3D;
 using pfgpu::SparseSim3DTP;
 
@@ -23,7 +34,10 @@ static int failures = 0;
     }                                                                 \
   } while (0)
 
-static double meanY(const SparseSim3DTP& sim, unsigned char type) {
+static bool close(double a, double b) { return std::fabs(a - b) <= 1e-5 * std::max(1.0, std::fabs(b)); }
+
+template <class Sim>
+static double meanY(const Sim& sim, unsigned char type) {
   double s = 0.0;
   int n = 0;
   for (size_t i = 0; i < sim.particles.size(); ++i)
@@ -34,7 +48,8 @@ static double meanY(const SparseSim3DTP& sim, unsigned char type) {
   return n ? s / n : 0.0;
 }
 
-static bool finite(const SparseSim3DTP& sim) {
+template <class Sim>
+static bool finite(const Sim& sim) {
   for (const auto& p : sim.particles.pos)
     if (!std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.z)) return false;
   return true;
@@ -123,6 +138,135 @@ int main() {
     CHECK(maxActive > 0 && maxActive < sim.totalCellBlocks());
     std::printf("single-phase dam-break: mean y %.4f -> %.4f, max active %zu of %zu\n", my0, my1, maxActive,
                 sim.totalCellBlocks());
+  }
+  {
+    MRSim3DTP sim(8, 12, 8, 1.0);
+    sim.initBubbleTankInterfaceBand();
+    CHECK(sim.countLevel(0) > 0);
+    CHECK(sim.countLevel(1) > 0);
+    CHECK(sim.particles.size() > 0);
+    CHECK(meanY(sim, 1) > 0.0);
+    CHECK(sim.activePressureCellCount() < 8 * 12 * 8);
+    CHECK(sim.uFaceCount() > 0 && sim.vFaceCount() > 0 && sim.wFaceCount() > 0);
+    std::printf("mr bubble band: %zu fine / %zu coarse leaves, %d cells, faces %d %d %d\n", sim.countLevel(0),
+                sim.countLevel(1), sim.activePressureCellCount(), sim.uFaceCount(), sim.vFaceCount(),
+                sim.wFaceCount());
+  }
+  {
+    MRSim3DTP sim(16, 16, 16, 1.0);
+    sim.dynamic_particle_padding = 0;
+    sim.dynamic_gas_padding = 1;
+    sim.particles.add({2.5, 2.5, 2.5}, {0.0, 0.0, 0.0}, 0);
+    sim.particles.add({3.5, 2.5, 2.5}, {0.0, 0.0, 0.0}, 1);
+    sim.updateDynamicRefinement();
+    CHECK(sim.levelAtFineCell(2, 2, 2) == 0);
+    CHECK(sim.levelAtFineCell(14, 14, 14) == 1);
+    sim.particles.pos[0] = {13.5, 13.5, 13.5};
+    sim.particles.pos[1] = {12.5, 13.5, 13.5};
+    sim.markParticlesDirty();
+    sim.updateDynamicRefinement();
+    CHECK(sim.levelAtFineCell(13, 13, 13) == 0);
+    CHECK(sim.levelAtFineCell(2, 2, 2) == 1);
+  }
+  {
+    MRSim3DTP sim(16, 16, 16, 1.0);
+    sim.dynamic_particle_padding = 0;
+    sim.dynamic_gas_padding = 0;
+    sim.dynamic_hysteresis_cells = 2;
+    sim.particles.add({2.5, 2.5, 2.5}, {0.0, 0.0, 0.0}, 0);
+    sim.updateDynamicRefinement();
+    const size_t initialLeaves = sim.countLevel(0);
+    sim.particles.pos[0] = {4.5, 2.5, 2.5};
+    sim.markParticlesDirty();
+    sim.updateDynamicRefinement();
+    CHECK(sim.dynamic_retained_box_valid);
+    CHECK(sim.countLevel(0) == initialLeaves);
+    CHECK(sim.levelAtFineCell(2, 2, 2) == 0);
+    CHECK(sim.levelAtFineCell(4, 2, 2) == 0);
+  }
+  {
+    MRSim3DTP sim(32, 32, 32, 1.0);
+    sim.dynamic_particle_padding = 8;
+    sim.dynamic_gas_padding = 0;
+    sim.dynamic_hysteresis_cells = 0;
+    sim.dynamic_max_fine_leaves = 8;
+    sim.particles.add({2.5, 2.5, 2.5}, {0.0, 0.0, 0.0}, 0);
+    sim.particles.add({29.5, 29.5, 29.5}, {0.0, 0.0, 0.0}, 0);
+    sim.updateDynamicRefinement();
+    CHECK(sim.dynamic_budget_limited);
+    CHECK(sim.dynamic_last_fine_leaves == (int)sim.countLevel(0));
+    CHECK(sim.countLevel(0) <= (size_t)sim.dynamic_max_fine_leaves);
+    std::printf("mr budget: %zu fine leaves (limit 8)\n", sim.countLevel(0));
+  }
+  {
+    MRSim3DTP sim(8, 12, 8, 1.0);
+    sim.dt = 0.02;
+    sim.cg_iters = 120;
+    sim.initBubbleTankInterfaceBand();
+    const size_t n0 = sim.particles.size();
+    const double gas0 = meanY(sim, 1);
+    for (int s = 0; s < 4; ++s) sim.step();
+    sim.syncParticlesToHost();
+    CHECK(sim.particles.size() == n0);
+    CHECK(finite(sim));
+    CHECK(std::isfinite(meanY(sim, 1)));
+    CHECK(meanY(sim, 1) >= gas0 - 0.05);
+    CHECK(sim.activePressureCellCount() < 8 * 12 * 8);
+    std::printf("mr bubble step: gas mean y %.4f -> %.4f, %d pressure cells\n", gas0, meanY(sim, 1),
+                sim.activePressureCellCount());
+  }
+  {
+    MRSim3DTP sim(8, 12, 8, 1.0);
+    sim.dt = 0.2;
+    sim.cg_iters = 20;
+    sim.adaptive_timestep = true;
+    sim.adaptive_cfl = 0.5;
+    sim.adaptive_min_dt = 1e-5;
+    sim.initBubbleTankInterfaceBand();
+    CHECK(!sim.particles.vel.empty());
+    sim.particles.vel[0] = {100.0, 0.0, 0.0};
+    sim.markParticlesDirty();
+    sim.step();
+    CHECK(close(sim.dt, 0.2));
+    CHECK(close(sim.max_particle_speed_last, 100.0));
+    CHECK(close(sim.effective_dt_last, 0.005));
+    CHECK(sim.adaptive_timestep_limited_last == 1);
+    std::printf("mr adaptive dt: %.6f\n", sim.effective_dt_last);
+  }
+  {
+    MRSim3DTP sim(8, 12, 8, 1.0);
+    sim.dt = 0.02;
+    sim.cg_iters = 120;
+    sim.c_div_volume_correction = true;
+    sim.c_div_strength = 1.0;
+    sim.initBubbleTankInterfaceBand();
+    double liquidVolume = 0.0;
+    for (size_t i = 0; i < sim.particles.size(); ++i)
+      if (sim.particles.type[i] == 0) liquidVolume += sim.particles.volume[i];
+    sim.liquid_volume_target = liquidVolume + 1.0;
+    sim.step();
+    CHECK(close(sim.liquid_volume_current_last, liquidVolume));
+    CHECK(close(sim.liquid_volume_error_last, 1.0));
+    CHECK(close(sim.c_div_last, 1.0 / (sim.dt * sim.liquid_volume_target)));
+  }
+  {
+    MRSim3DTP sim(8, 12, 8, 1.0);
+    sim.phase.rho_l = 1000.0;
+    sim.phase.rho_g = 1.0;
+    sim.dt = 0.02;
+    sim.cg_iters = 160;
+    sim.cg_rel_tol = 1e-5;
+    sim.initBubbleTankInterfaceBand();
+    const size_t n0 = sim.particles.size();
+    sim.step();
+    sim.syncParticlesToHost();
+    CHECK(sim.particles.size() == n0);
+    CHECK(finite(sim));
+    CHECK(sim.last_pressure_stats.converged);
+    CHECK(sim.last_pressure_stats.final_residual <= sim.last_pressure_stats.effective_tolerance);
+    CHECK(!sim.last_pressure_stats.breakdown);
+    std::printf("mr rho ratio 1000: %d iterations, residual %.3e <= %.3e\n", sim.last_pressure_stats.iterations,
+                sim.last_pressure_stats.final_residual, sim.last_pressure_stats.effective_tolerance);
   }
   std::printf(failures ? "FAILED %d checks\n" : "all checks passed\n", failures);
   return failures ? 1 : 0;

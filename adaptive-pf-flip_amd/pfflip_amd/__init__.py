@@ -160,6 +160,8 @@ def lib():
             "pf_mr_project": (C.c_int, [vp, C.POINTER(Phase), dbl, C.POINTER(MRSolveCfg), dbl,
                                          C.POINTER(SolveStats)]),
             "pf_mr_update_refinement": (C.c_int, [vp, C.POINTER(MRRefineCfg), C.POINTER(i32), C.POINTER(i32)]),
+            "pf_mr_layout_refine": (i64, [i32, i32, i32, vp, i64, vp, vp, i64]),
+            "pf_mr_retained_box": (C.c_int, [vp, vp, C.POINTER(i32)]),
             "pf_mr_step": (C.c_int, [vp, C.POINTER(MRStepCfg), C.POINTER(StepStats)]),
             "pf_lpc_create": (C.c_int, [C.POINTER(vp), C.c_uint64]),
             "pf_lpc_destroy": (C.c_int, [vp]),
@@ -544,6 +546,19 @@ def mr_layout_fine_box(nx, ny, nz, box=None, level=1):
     return out[:n]
 
 
+def mr_layout_refine(nx, ny, nz, leaves, box):
+    """refineFineCellBox (multires_layout3d.h:62-97) of `box` = (x0, y0, z0, x1, y1, z1)
+    applied to the (n, 4) leaves; the sorted result"""
+    lv = np.ascontiguousarray(leaves, np.int32).reshape(-1, 4)
+    b = np.ascontiguousarray(box, np.int32)
+    n = lib().pf_mr_layout_refine(nx, ny, nz, _p(lv), len(lv), _p(b), None, 0)
+    if n < 0:
+        raise PfError("pf_mr_layout_refine: bad arguments")
+    out = np.zeros((max(n, 1), 4), np.int32)
+    lib().pf_mr_layout_refine(nx, ny, nz, _p(lv), len(lv), _p(b), _p(out), n)
+    return out[:n]
+
+
 class MRContext(Context):
     """A context with a multires layout (MRSim3DTP's grid on the GPU)."""
 
@@ -583,6 +598,12 @@ class MRContext(Context):
         self._check(lib().pf_mr_update_refinement(self.h, C.byref(refine), C.byref(fl), C.byref(bl)),
                     "pf_mr_update_refinement")
         return fl.value, bl.value
+
+    def mr_retained_box(self):
+        """(valid, [x0, y0, z0, x1, y1, z1]) of the last refinement update's kept box"""
+        box, valid = np.zeros(6, np.int32), C.c_int32()
+        self._check(lib().pf_mr_retained_box(self.h, _p(box), C.byref(valid)), "pf_mr_retained_box")
+        return bool(valid.value), box
 
     def mr_step(self, cfg):
         st = StepStats()

@@ -60,6 +60,47 @@ def step_cfg(G, n, tol, max_iters, warm):
                      advection_order=2, timing=1, phase=ph, solve=solve)
 
 
+def pressure_comparison(G, ctx, cfg, tol):
+    """Pressure solve to ||b - Ax||_2/||b||_2 <= tol (FP64) on the current system,
+    plain Jacobi PCG vs the learned preconditioner inside flexible PCG, same inputs.
+    The pre-projection state (mark, P2G, save, gravity, walls) is rebuilt for each."""
+    from pfflip_amd import lpc_train
+    if not os.path.exists(lpc_train.DEFAULT_WEIGHTS):
+        return {"learned_pc": None, "note": "no trained weights in the package"}
+    net = G.LearnedPreconditioner(0)
+    net.set_weights(lpc_train.load_weights())
+    net.attach(ctx)
+    dt, ph = cfg.dt, cfg.phase
+
+    def prepare():
+        ctx.mark_cells()
+        ctx.p2g(ph, cfg.Vp)
+        ctx.save_velocity()
+        ctx.apply_gravity(dt, cfg.gravity)
+        ctx.apply_walls()
+
+    out = {}
+    for name, pre in (("learned_pc", G.PRECOND_LEARNED), ("plain_pcg", G.PRECOND_JACOBI),
+                      ("learned_pc", G.PRECOND_LEARNED)):
+        prepare()
+        solve = G.default_solve_cfg(G.STOP_REL2, tol, 50000, pre)
+        solve.check_every = 16
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        st = ctx.project(ph, dt, solve)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        # the first learned solve also allocates the network's buffers; keep the second
+        out[name] = {"ms": round(ms, 2), "iterations": st.iterations, "true_rel_residual": st.true_rel_residual,
+                     "converged": st.converged, "fallback_to_jacobi": st.fallback_used}
+        print(f"[bench] pressure solve {name}: {ms:.1f} ms, {st.iterations} it, fallback {st.fallback_used}",
+              file=sys.stderr, flush=True)
+    out["speedup_learned_vs_plain"] = round(out["plain_pcg"]["ms"] / out["learned_pc"]["ms"], 3)
+    G.lib().pf_lpc_attach(ctx.h, None)
+    net.close()
+    return out
+
+
 def cpu_baseline(n_sample, n_full, seed):
     """Reference SparseSim3DTP::step on the n_sample^3 RT scene, scaled to n_full^3."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -116,6 +157,7 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -174,6 +216,9 @@ def main():
     ncopy = 256 << 20  # 1 GiB read + 1 GiB written: past every cache
     copy_ms = min(ctx.copy_probe(ncopy) for _ in range(3))
     copy_gbs = round(2 * 4 * ncopy / (copy_ms * 1e-3) / 1e9, 1)
+    pressure = {"plain_pcg": {"ms": stage["solve"]}, "learned_pc": None}
+    if not args.no_learned:
+        pressure = pressure_comparison(G, ctx, cfg, args.tol)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (reference step on the 64^3 sample) ...", file=sys.stderr, flush=True)
@@ -181,7 +226,7 @@ def main():
         print(f"[bench] cpu sample {cpu['sample_ms']} ms", file=sys.stderr, flush=True)
 
     line = {
-        "metric": "step ms @256^3 two-phase",
+        "metric": "step ms @256\u00b3 two-phase; pressure ms to 1e-6 resid, learned-PC vs plain PCG",
         "value": round(ms_per_step, 3),
         "unit": "ms/step",
         "n_gpus": 1,
@@ -199,7 +244,7 @@ def main():
                    "grid": n, "particles": ctx.count(), "fluid_cells": fluid,
                    "solver": "jacobi-pcg-fp32+ir-fp64", "warm_start": bool(args.warm_start)},
         "stage_ms": stage,
-        "pressure_ms_to_1e-6": {"plain_pcg": stage["solve"], "learned_pc": None},
+        "pressure_ms_to_1e-6": pressure,
         "solve_iterations": iters,
         "true_rel_residual": [s.solve.true_rel_residual for s in stats],
         "kernels": kernels,

@@ -262,6 +262,11 @@ int pf_mr_get_layout(pf_ctx* ctx, int32_t* leaves, int64_t capacity, int64_t* n)
  * count, writes at most capacity leaves (host only) */
 int64_t pf_mr_layout_fine_box(int32_t nx, int32_t ny, int32_t nz, int32_t level, int32_t box_valid,
                               const int32_t* box, int32_t* leaves, int64_t capacity);
+/* refineFineCellBox (multires_layout3d.h:62-97) applied to n leaves: leaves of level
+ * > 0 overlapping the box (clamped to the domain) split into their in-domain children;
+ * returns the new count, writes at most capacity leaves (host only) */
+int64_t pf_mr_layout_refine(int32_t nx, int32_t ny, int32_t nz, const int32_t* in, int64_t n, const int32_t* box,
+                            int32_t* leaves, int64_t capacity);
 /* the enumerated faces of an axis (MRMacGrid3D::uFaces etc., multires_mac_grid3d.h:43-45)
  * as one byte per fine face, dense in the face field's extents */
 int pf_mr_face_valid(pf_ctx* ctx, int32_t axis, uint8_t* dense);
@@ -275,8 +280,13 @@ int pf_mr_p2g(pf_ctx* ctx, const pf_phase* phase, double Vp);
 int pf_mr_project(pf_ctx* ctx, const pf_phase* phase, double dt, const pf_mr_solve_cfg* cfg, double c_div,
                   pf_solve_stats* stats);
 /* MRSim3DTP::updateDynamicRefinement, multires_sim3d_tp.cpp:577-669 (state kept in
- * the context); reports the fine leaf count and whether the budget cut the box */
+ * the context); reports the fine leaf count and whether the budget cut the box.  With
+ * cfg->dynamic_refinement 0 nothing changes and the last update's figures are reported
+ * (pf_mr_step's own update included). */
 int pf_mr_update_refinement(pf_ctx* ctx, const pf_mr_refine_cfg* cfg, int32_t* fine_leaves, int32_t* budget_limited);
+/* the refinement box the last update kept (MRSim3DTP::dynamic_retained_box_valid and
+ * dynamic_retained_x0 .. z1, multires_sim3d_tp.h:68-74) as x0, y0, z0, x1, y1, z1 */
+int pf_mr_retained_box(pf_ctx* ctx, int32_t* box, int32_t* valid);
 /* MRSim3DTP::step, multires_sim3d_tp.cpp:671-734 (default flags: no particle
  * adaptivity, secondary particles or surface tension) */
 int pf_mr_step(pf_ctx* ctx, const pf_mr_step_cfg* cfg, pf_step_stats* stats);

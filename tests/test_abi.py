@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import pfflip_amd as G
@@ -65,3 +66,36 @@ def test_null_arguments_are_rejected_without_device():
     h = C.c_void_p()
     assert L.pf_create(C.byref(h), C.byref(d)) == 1  # nx < 3
     assert L.pf_mark_cells(None) == 1
+
+
+def test_multires_struct_layouts_match_the_header(tmp_path):
+    src = tmp_path / "mr_sizes.c"
+    src.write_text('#include "pf_gpu.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+                   "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\","
+                   "sizeof(pf_mr_solve_cfg),sizeof(pf_mr_refine_cfg),sizeof(pf_mr_step_cfg),"
+                   "offsetof(pf_mr_step_cfg,solve),offsetof(pf_mr_step_cfg,refine));return 0;}\n")
+    exe = tmp_path / "mr_sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    want = [C.sizeof(G.MRSolveCfg), C.sizeof(G.MRRefineCfg), C.sizeof(G.MRStepCfg),
+            G.MRStepCfg.solve.offset, G.MRStepCfg.refine.offset]
+    assert got == want
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (8, 12, 8), (17, 10, 9), (24, 40, 20)])
+def test_multires_layout_helpers_match_the_oracle(dims):
+    # host-only layout builders: layoutWithFineBox for several boxes, and the bubble
+    # band (two refineFineCellBox calls) composed from pf_mr_layout_fine_box and
+    # pf_mr_layout_refine, leaf for leaf against the oracle's layouts
+    import pyoracle as O
+    nx, ny, nz = dims
+    for box in (None, [0, 0, 0, nx, ny, nz], [1, 2, 3, nx // 2 + 1, ny - 2, nz // 3 + 2], [5, 5, 5, 6, 6, 6]):
+        assert np.array_equal(G.mr_layout_fine_box(nx, ny, nz, box), O.mr_layout_fine_box(nx, ny, nz, box))
+    water = ny // 2
+    band = G.mr_layout_fine_box(nx, ny, nz, [0, 0, 0, nx, min(water + 2, ny), nz])
+    bubble = [nx // 2 - nx // 4, water // 4, nz // 2 - nz // 4, nx // 2 + nx // 4, water, nz // 2 + nz // 4]
+    got = G.mr_layout_refine(nx, ny, nz, band, bubble)
+    assert np.array_equal(got, O.mr_layout_bubble_band(nx, ny, nz))
+    # an empty or out-of-domain box leaves the layout as it was
+    assert np.array_equal(G.mr_layout_refine(nx, ny, nz, band, [nx, 0, 0, nx + 4, 4, 4]), band)

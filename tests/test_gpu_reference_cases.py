@@ -157,7 +157,10 @@ def test_rejects_particles_outside_the_grid():
 
 def test_cpp_drop_in_class_runs_the_reference_integration_tests():
     # adaptive-pf-flip_amd/host/sim_demo.cpp: the reference's SparseSim3DTP tests
-    # (tests/test_sparse_sim3d_tp.cpp:110-169, 749-786) against the C++ drop-in class
+    # (tests/test_sparse_sim3d_tp.cpp:110-169, 749-786), SparseSim3D dam-break
+    # (tests/test_sparse_sim3d.cpp:6-36) and MRSim3DTP tests
+    # (tests/test_multires_sim3d_tp.cpp:136-148, 345-439, 558-594) against the C++
+    # drop-in classes
     import os
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
