@@ -539,8 +539,6 @@ int mr_apply_layout(pf_ctx* c, std::vector<Leaf> lv) {
     }
     HIP_TRY(c, dalloc(&c->mr_vol64, ncell));
     HIP_TRY(c, dalloc(&c->mr_pfull, ncell));
-    HIP_TRY(c, dalloc(&c->mr_bbox, 16));
-    HIP_TRY(c, hipHostMalloc((void**)&c->mr_bbox_host, sizeof(int) * 16));
   }
   if ((long long)lv.size() > c->mr_leaves_cap) {
     dfree(c->mr_leaves_dev);
@@ -582,6 +580,10 @@ int mr_dynamic(pf_ctx* c, const pf_mr_refine_cfg* rc) {
   if (!rc->dynamic_refinement) return PF_OK;
   const Geom& g = c->geom;
   const int lim[3] = {g.nx, g.ny, g.nz};
+  // the box buffers belong to refinement, which may run before any layout exists
+  if (!c->mr_bbox) HIP_TRY(c, dalloc(&c->mr_bbox, 16));
+  if (!c->mr_bbox_host) HIP_TRY(c, hipHostMalloc((void**)&c->mr_bbox_host, sizeof(int) * 16));
+  if (c->np > 0 && !c->pa.x) return fail(c, PF_ERR_STATE, "particle buffers missing");
   HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->mr_bbox), 0x7fffffffu, 12));
   if (c->np > 0)
     hipLaunchKernelGGL(k_mr_bbox, dim3(std::min(grid_for(c->np, 256), 1024u)), dim3(256), 0, c->stream, c->np, c->pa,

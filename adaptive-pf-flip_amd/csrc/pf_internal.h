@@ -36,7 +36,20 @@ struct Dims {
   int nx, ny, nz;
   int nbx, nby, nbz;
   long long nblocks;
+  uint64_t inv_nbx, inv_nby;  // division constants of nbx and nby (fastdiv_u32)
 };
+
+// ceil(2^64 / d) for d >= 2, 0 for d = 1 (Lemire, Kaser, Kurz, "Faster remainder by
+// direct computation", 2019: floor(n / d) = floor(c n / 2^64) for every 32-bit n)
+inline uint64_t fastdiv_constant(uint32_t d) { return d <= 1 ? 0 : ~0ull / d + 1; }
+
+// floor(n / d) from c = fastdiv_constant(d): the high 64 bits of the 96-bit product
+// c n, formed from 32-bit halves (no 64-bit division on the device)
+__host__ __device__ inline uint32_t fastdiv_u32(uint32_t n, uint64_t c) {
+  if (c == 0) return n;
+  const uint64_t lo = (c & 0xffffffffull) * n, hi = (c >> 32) * n;
+  return (uint32_t)((hi + (lo >> 32)) >> 32);
+}
 
 inline Dims make_dims(int nx, int ny, int nz) {
   Dims d;
@@ -45,6 +58,8 @@ inline Dims make_dims(int nx, int ny, int nz) {
   d.nby = (ny + kBlock - 1) / kBlock;
   d.nbz = (nz + kBlock - 1) / kBlock;
   d.nblocks = (long long)d.nbx * d.nby * d.nbz;
+  d.inv_nbx = fastdiv_constant((uint32_t)d.nbx);
+  d.inv_nby = fastdiv_constant((uint32_t)d.nby);
   return d;
 }
 
@@ -56,11 +71,14 @@ __host__ __device__ inline long long bm_index(const Dims& d, int i, int j, int k
   return bid_of(d, i >> 2, j >> 2, k >> 2) * kBlockVol + (i & 3) + 4 * (j & 3) + 16 * (k & 3);
 }
 
+// block ids are below 2^26 (cells below 2^32), so 32-bit arithmetic is exact
 __host__ __device__ inline void block_coords(const Dims& d, long long b, int& bx, int& by, int& bz) {
-  bx = (int)(b % d.nbx);
-  long long t = b / d.nbx;
-  by = (int)(t % d.nby);
-  bz = (int)(t / d.nby);
+  const uint32_t n = (uint32_t)b;
+  const uint32_t t = fastdiv_u32(n, d.inv_nbx);
+  const uint32_t z = fastdiv_u32(t, d.inv_nby);
+  bx = (int)(n - t * (uint32_t)d.nbx);
+  by = (int)(t - z * (uint32_t)d.nby);
+  bz = (int)z;
 }
 
 // color8, src/grid/block_color.h:7
