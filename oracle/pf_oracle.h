@@ -152,6 +152,13 @@ typedef struct orc_mr_solve_cfg {          /* MRPressureSolveConfig3D, multires_
   double absolute_tolerance, relative_tolerance, restart_growth_threshold;
   int32_t relaxation_sweeps, pad0;
   double relaxation_omega, relaxation_min_omega, divergence_correction;
+  /* coarse correction and coarse preconditioner (:148-165) */
+  int32_t use_coarse_correction, coarse_correction_iterations, coarse_correction_sweeps;
+  int32_t use_coarse_preconditioner, coarse_preconditioner_iterations;
+  int32_t coarse_preconditioner_auto_disable, coarse_preconditioner_auto_disable_after, pad1;
+  double coarse_correction_absolute_tolerance, coarse_correction_relative_tolerance, coarse_correction_min_scale;
+  double coarse_preconditioner_absolute_tolerance, coarse_preconditioner_relative_tolerance;
+  double coarse_preconditioner_scale, coarse_preconditioner_min_rz_gain, coarse_preconditioner_max_work_ratio;
 } orc_mr_solve_cfg;
 
 typedef struct orc_mr_solve_stats {        /* MRPressureSolveStats3D core fields */
@@ -160,6 +167,13 @@ typedef struct orc_mr_solve_stats {        /* MRPressureSolveStats3D core fields
   int32_t relaxation_rejected, converged, breakdown, pad0;
   double initial_residual, final_residual, effective_tolerance;
   double true_max_residual, true_rel_residual;   /* recomputed in FP64 at exit */
+  /* coarse machinery (multires_pressure3d.h:70-86, 99-107) */
+  int32_t cc_cells, cc_iterations, cc_sweeps, cc_accepted_sweeps, cc_rejected_sweeps, cc_accepted;
+  int32_t cc_converged, cc_breakdown;
+  int32_t cp_cells, cp_applications, cp_accepted, cp_rejected, cp_skipped, cp_budget_limited, cp_iterations;
+  int32_t cp_auto_disabled, cp_auto_disable_iteration, cp_wasted_streak, cp_breakdown, cp_budget_exhausted;
+  double cc_initial_residual, cc_final_residual, cc_last_scale, cc_effective_tolerance;
+  double cp_effective_tolerance, cp_last_rz_gain;
 } orc_mr_solve_stats;
 
 typedef struct orc_mr_sim {                /* MRSim3DTP fields step() reads + refinement state */
@@ -201,6 +215,7 @@ void orc_mr_p2g(orc_mr* g, int64_t n, const double* x, const double* y, const do
                 const orc_phase* pp, double Vp);
 void orc_mr_apply_gravity(orc_mr* g, double dt, double gravity);
 void orc_mr_apply_walls(orc_mr* g);
+int orc_mr_solve_cfg_valid(const orc_mr_solve_cfg* cfg);
 void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solve_cfg* cfg,
                     orc_mr_solve_stats* st);
 void orc_mr_g2p(const orc_mr* g, const orc_mr* saved, int64_t n, const double* x, const double* y,

@@ -555,14 +555,281 @@ static double nbr_distance(const orc_mr* g, double h, int nx, int ny, int nz) {
 
 typedef struct { int64_t idx; double coeff; } mr_term;
 
+/* ---- level-1 coarse machinery (multires_pressure3d.cpp:39-70, 470-700) */
+
+typedef struct {           /* the assembled rows: A x = diag x - sum coeff x_nbr, pin row = identity */
+  int64_t N, pin;
+  const double* diag;
+  const int* noff;
+  const mr_term* off;
+} mr_op;
+
+/* applyA (rows of :783-812); the pinned row is the identity */
+static void mr_apply(const mr_op* A, const double* x, double* out) {
+  for (int64_t t = 0; t < A->N; ++t) {
+    if (t == A->pin) { out[t] = x[t]; continue; }
+    double o = 0.0;
+    for (int q = 0; q < A->noff[t]; ++q) o += A->off[6 * 64 * t + q].coeff * x[A->off[6 * 64 * t + q].idx];
+    out[t] = A->diag[t] * x[t] - o;
+  }
+}
+
+typedef struct {           /* MRPressureAggregation3D, multires_pressure3d.h:29-36 */
+  int64_t nf, nc;
+  int64_t* f2c;
+  double* fv;
+  double* cv;
+} mr_agg;
+
+static void mr_agg_free(mr_agg* a) { free(a->f2c); free(a->fv); free(a->cv); }
+
+/* buildLevel1AggregationFromCells (:52-70) with level1ParentTuple (:39-50): a cell's
+ * coarse id is its 2x-parent's order of first appearance in the cell list; coarse
+ * volumes sum the fine ones (buildMRPressureAggregation3D, :470-505).  Returns 0 when
+ * a leaf has a level above 1 (the reference throws there). */
+static int mr_level1_aggregation(const orc_mr* g, const int64_t* cells, int64_t N, mr_agg* a) {
+  const int64_t hx = (g->nx + 1) / 2, hy = (g->ny + 1) / 2, hz = (g->nz + 1) / 2;
+  int64_t* first = (int64_t*)malloc(sizeof(int64_t) * (size_t)(hx * hy * hz));
+  for (int64_t q = 0; q < hx * hy * hz; ++q) first[q] = -1;
+  a->nf = N;
+  a->nc = 0;
+  a->f2c = (int64_t*)malloc(sizeof(int64_t) * (size_t)(N > 0 ? N : 1));
+  a->fv = (double*)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+  for (int64_t t = 0; t < N; ++t) {
+    if (cell_level(g, cells[t]) > 1) { free(first); return 0; }
+    int x0, y0, z0, st;
+    cell_box(g, cells[t], &x0, &y0, &z0, &st);
+    const int64_t key = x0 / 2 + hx * (y0 / 2 + hy * (int64_t)(z0 / 2));
+    if (first[key] < 0) first[key] = a->nc++;
+    a->f2c[t] = first[key];
+    const double h = cell_h(g, cells[t]);
+    a->fv[t] = h * h * h;
+  }
+  free(first);
+  a->cv = (double*)calloc((size_t)(a->nc > 0 ? a->nc : 1), sizeof(double));
+  for (int64_t t = 0; t < N; ++t) a->cv[a->f2c[t]] += a->fv[t];
+  return 1;
+}
+
+/* restrictMRPressureVolumeWeighted3D (:518-539) */
+static void mr_restrict(const mr_agg* a, const double* fine, double* coarse) {
+  for (int64_t c = 0; c < a->nc; ++c) coarse[c] = 0.0;
+  for (int64_t i = 0; i < a->nf; ++i) coarse[a->f2c[i]] += a->fv[i] * fine[i];
+  for (int64_t c = 0; c < a->nc; ++c) coarse[c] /= a->cv[c];
+}
+
+/* prolongMRPressurePiecewiseConstant3D (:541-552) */
+static void mr_prolong(const mr_agg* a, const double* coarse, double* fine) {
+  for (int64_t i = 0; i < a->nf; ++i) fine[i] = coarse[a->f2c[i]];
+}
+
+/* weightedDotVolumes (:369-380) */
+static double mr_wdot(const mr_agg* a, const double* x, const double* y) {
+  double s = 0.0;
+  for (int64_t c = 0; c < a->nc; ++c) s += a->cv[c] * x[c] * y[c];
+  return s;
+}
+
+static double mr_wnorm(const mr_agg* a, const double* v) {
+  const double n2 = mr_wdot(a, v, v);
+  return n2 > 0.0 ? sqrt(n2) : 0.0;
+}
+
+typedef struct {           /* scratch of the coarse solves */
+  double *rhs, *x, *r, *p, *ap;    /* coarse */
+  double *fp, *fap, *corr;         /* fine */
+} mr_coarse_work;
+
+/* the coarse operator R A P, applied matrix-free through the fine rows, with the
+ * coarse pin as an identity row (applyCoarse, :922-929 and :1181-1190) */
+static void mr_apply_coarse(const mr_op* A, const mr_agg* a, int64_t cpin, mr_coarse_work* w, const double* in,
+                            double* out) {
+  mr_prolong(a, in, w->fp);
+  mr_apply(A, w->fp, w->fap);
+  mr_restrict(a, w->fap, out);
+  if (cpin >= 0) out[cpin] = in[cpin];
+}
+
+/* the coarse CG shared by the correction (:946-1015) and the preconditioner
+ * (:1214-1281): x = 0, r = p = rhs (pin zeroed); stops when the volume-weighted norm
+ * of r is at most `eff`; returns the iterations; *breakdown on a non-finite or
+ * vanishing scalar */
+static int mr_coarse_cg(const mr_op* A, const mr_agg* a, int64_t cpin, mr_coarse_work* w, int max_it, double eff,
+                        int* converged, int* breakdown) {
+  const int64_t nc = a->nc;
+  for (int64_t c = 0; c < nc; ++c) { w->x[c] = 0.0; w->r[c] = w->rhs[c]; w->p[c] = w->r[c]; w->ap[c] = 0.0; }
+  if (cpin >= 0) { w->r[cpin] = 0.0; w->p[cpin] = 0.0; }
+  const double res0 = mr_wnorm(a, w->r);
+  *converged = res0 <= eff;
+  *breakdown = 0;
+  double rr = mr_wdot(a, w->r, w->r);
+  if (!isfinite(rr)) { *breakdown = 1; return 0; }
+  int iters = 0;
+  for (int it = 0; !*converged && !*breakdown && it < max_it; ++it) {
+    mr_apply_coarse(A, a, cpin, w, w->p, w->ap);
+    const double pAp = mr_wdot(a, w->p, w->ap);
+    if (!isfinite(pAp) || fabs(pAp) < 1e-30) { *breakdown = 1; break; }
+    const double alpha = rr / pAp;
+    if (!isfinite(alpha)) { *breakdown = 1; break; }
+    for (int64_t c = 0; c < nc; ++c) { w->x[c] += alpha * w->p[c]; w->r[c] -= alpha * w->ap[c]; }
+    if (cpin >= 0) { w->x[cpin] = 0.0; w->r[cpin] = 0.0; }
+    iters = it + 1;
+    const double res = mr_wnorm(a, w->r);
+    if (!isfinite(res)) { *breakdown = 1; break; }
+    if (res <= eff) { *converged = 1; break; }
+    const double rrn = mr_wdot(a, w->r, w->r);
+    if (!isfinite(rrn) || fabs(rr) < 1e-30) { *breakdown = 1; break; }
+    const double beta = rrn / rr;
+    if (!isfinite(beta)) { *breakdown = 1; break; }
+    rr = rrn;
+    for (int64_t c = 0; c < nc; ++c) w->p[c] = w->r[c] + beta * w->p[c];
+    if (cpin >= 0) w->p[cpin] = 0.0;
+  }
+  return iters;
+}
+
+/* residualFromPressure (:870-884): out = rhs - A x; max |out|, +inf if any entry is
+ * not finite */
+static double mr_residual_from(const mr_op* A, const double* rhs, const double* x, double* Ax, double* out) {
+  mr_apply(A, x, Ax);
+  double res = 0.0;
+  for (int64_t i = 0; i < A->N; ++i) {
+    out[i] = rhs[i] - Ax[i];
+    if (!isfinite(out[i])) return INFINITY;
+    res = fmax(res, fabs(out[i]));
+  }
+  return res;
+}
+
+typedef struct {           /* the coarse preconditioner's state across one solve (:1119-1160) */
+  int ready, disabled, streak;
+  int64_t cpin;
+  mr_agg agg;
+  mr_coarse_work w;
+} mr_cprec;
+
+/* applyPreconditioner with the additive coarse term (:1283-1340): Jacobi z, then
+ * z + scale P x_c when a coarse solve ran and r.z grew by at least min_rz_gain */
+static void mr_precondition(const mr_op* A, const orc_mr_solve_cfg* cf, orc_mr_solve_stats* st, mr_cprec* cp,
+                            const double* r, double* z, double* cand) {
+  const int64_t N = A->N;
+  for (int64_t t = 0; t < N; ++t) z[t] = (cf->use_jacobi && A->diag[t] > 0.0) ? r[t] / A->diag[t] : r[t];
+  if (!cp) return;
+  double rzb = 0.0;
+  for (int64_t t = 0; t < N; ++t) rzb += r[t] * z[t];
+  const int base_ok = isfinite(rzb) && rzb > 0.0;
+  if (cp->disabled) return;
+  const int skipped_before = st->cp_skipped;
+  int accepted = 0, wasted = 0, solved = 0;
+  /* solveCoarsePreconditioner (:1192-1281) */
+  if (cp->ready) {
+    int max_c = cf->coarse_preconditioner_iterations;
+    int budget_ok = 1;
+    if (cf->coarse_preconditioner_max_work_ratio > 0.0) {
+      const double allowed = cf->coarse_preconditioner_max_work_ratio * (double)(st->iterations + 1 > 1 ? st->iterations + 1 : 1);
+      const int remaining = (int)floor(allowed + 1e-12) - st->cp_iterations;
+      if (remaining < max_c) {
+        st->cp_budget_limited++;
+        st->cp_skipped++;
+        st->cp_budget_exhausted = 1;
+        budget_ok = 0;
+      }
+    }
+    if (budget_ok) {
+      st->cp_applications++;
+      mr_restrict(&cp->agg, r, cp->w.rhs);
+      double r0 = 0.0;
+      {
+        for (int64_t c = 0; c < cp->agg.nc; ++c) cp->w.r[c] = cp->w.rhs[c];
+        if (cp->cpin >= 0) cp->w.r[cp->cpin] = 0.0;
+        r0 = mr_wnorm(&cp->agg, cp->w.r);
+      }
+      const double eff = fmax(cf->coarse_preconditioner_absolute_tolerance,
+                              cf->coarse_preconditioner_relative_tolerance * r0);
+      st->cp_effective_tolerance = eff;
+      int conv = 0, bd = 0;
+      const int it = mr_coarse_cg(A, &cp->agg, cp->cpin, &cp->w, max_c, eff, &conv, &bd);
+      if (bd) {
+        st->cp_breakdown = 1;
+      } else {
+        st->cp_iterations += it;
+        mr_prolong(&cp->agg, cp->w.x, cp->w.corr);
+        solved = 1;
+      }
+    }
+  }
+  if (solved) {
+    for (int64_t t = 0; t < N; ++t) cand[t] = z[t] + cf->coarse_preconditioner_scale * cp->w.corr[t];
+    double rzc = 0.0;
+    for (int64_t t = 0; t < N; ++t) rzc += r[t] * cand[t];
+    if (isfinite(rzc) && base_ok) st->cp_last_rz_gain = rzc / rzb - 1.0;
+    const double min_acc = base_ok ? rzb * (1.0 + cf->coarse_preconditioner_min_rz_gain) : 0.0;
+    if (isfinite(rzc) && rzc > 0.0 && (!base_ok || rzc >= min_acc)) {
+      memcpy(z, cand, sizeof(double) * (size_t)N);
+      st->cp_accepted++;
+      accepted = 1;
+    } else {
+      st->cp_rejected++;
+      wasted = 1;
+    }
+  } else if (st->cp_skipped > skipped_before) {
+    wasted = 1;
+  }
+  /* recordCoarsePreconditionerOutcome (:1157-1178) */
+  if (accepted) {
+    cp->streak = 0;
+    st->cp_wasted_streak = 0;
+    return;
+  }
+  if (!wasted) return;
+  cp->streak++;
+  st->cp_wasted_streak = cp->streak;
+  if (cf->coarse_preconditioner_auto_disable && cf->coarse_preconditioner_auto_disable_after > 0 &&
+      cp->streak >= cf->coarse_preconditioner_auto_disable_after) {
+    cp->disabled = 1;
+    st->cp_auto_disabled = 1;
+    if (st->cp_auto_disable_iteration < 0) st->cp_auto_disable_iteration = st->iterations;
+  }
+}
+
+static void mr_coarse_work_alloc(mr_coarse_work* w, int64_t nc, int64_t nf) {
+  const size_t c = (size_t)(nc > 0 ? nc : 1), f = (size_t)(nf > 0 ? nf : 1);
+  w->rhs = (double*)calloc(c, 8); w->x = (double*)calloc(c, 8); w->r = (double*)calloc(c, 8);
+  w->p = (double*)calloc(c, 8); w->ap = (double*)calloc(c, 8);
+  w->fp = (double*)calloc(f, 8); w->fap = (double*)calloc(f, 8); w->corr = (double*)calloc(f, 8);
+}
+
+static void mr_coarse_work_free(mr_coarse_work* w) {
+  free(w->rhs); free(w->x); free(w->r); free(w->p); free(w->ap); free(w->fp); free(w->fap); free(w->corr);
+}
+
 /* projectMR3D (phase-aware), multires_pressure3d.cpp:747-1449 and the face update
- * :1451-1524.  The coarse correction / coarse preconditioner options are not restated
- * (their defaults are off, multires_pressure3d.h:150-166). */
+ * :1451-1524, with the coarse correction (:886-1060) and the additive coarse
+ * preconditioner (:1119-1341) when enabled. */
+/* validateSolveConfig, multires_pressure3d.cpp:307-333 (the reference throws) */
+int orc_mr_solve_cfg_valid(const orc_mr_solve_cfg* c) {
+  return !(c->max_iterations < 0 || c->absolute_tolerance < 0.0 || c->relative_tolerance < 0.0 ||
+           c->restart_growth_threshold < 0.0 || c->relaxation_sweeps < 0 || c->relaxation_omega < 0.0 ||
+           c->relaxation_min_omega < 0.0 || c->coarse_correction_iterations < 0 ||
+           c->coarse_correction_sweeps < 0 || c->coarse_correction_absolute_tolerance < 0.0 ||
+           c->coarse_correction_relative_tolerance < 0.0 || c->coarse_correction_min_scale <= 0.0 ||
+           c->coarse_correction_min_scale > 1.0 || c->coarse_preconditioner_iterations < 0 ||
+           c->coarse_preconditioner_absolute_tolerance < 0.0 || c->coarse_preconditioner_relative_tolerance < 0.0 ||
+           c->coarse_preconditioner_scale < 0.0 || c->coarse_preconditioner_min_rz_gain < 0.0 ||
+           c->coarse_preconditioner_max_work_ratio < 0.0 || c->coarse_preconditioner_auto_disable_after < 0 ||
+           !isfinite(c->divergence_correction));
+}
+
 void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solve_cfg* cf, orc_mr_solve_stats* st) {
   orc_mr_solve_stats local;
   if (!st) st = &local;
   memset(st, 0, sizeof(*st));
   st->pinned_cell = -1;
+  st->cp_auto_disable_iteration = -1;
+  if (!orc_mr_solve_cfg_valid(cf)) {
+    st->breakdown = 1;
+    return;
+  }
   /* fluidProjectionCells: leafCells order (leaves sorted, then lz, ly, lx) */
   int64_t nl = (int64_t)g->nleaves * 64;
   int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nl > 0 ? nl : 1));
@@ -652,9 +919,71 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
   st->converged = res0 < tol;
 #define PRECOND()                                                                   \
   for (int64_t t = 0; t < N; ++t) z[t] = (cf->use_jacobi && diag[t] > 0.0) ? r[t] / diag[t] : r[t];
+  const mr_op A = {N, pin, diag, noff, off};
   if (res0 >= tol) {
-    /* damped-Jacobi pre-relaxation with omega backoff, :1062-1110 */
     double cur = res0;
+    /* Galerkin coarse correction sweeps with a halving scale search, :886-1060 */
+    if (cf->use_coarse_correction && cur >= tol) {
+      mr_agg agg;
+      if (mr_level1_aggregation(g, cells, N, &agg)) {
+        st->cc_cells = (int32_t)agg.nc;
+        st->cc_initial_residual = cur;
+        st->cc_final_residual = cur;
+        if (agg.nc > 0 && agg.nc < N) {
+          const int64_t cpin = pin >= 0 ? agg.f2c[pin] : -1;
+          mr_coarse_work w;
+          mr_coarse_work_alloc(&w, agg.nc, N);
+          double* bx = (double*)malloc(8 * (size_t)N);
+          double* br = (double*)malloc(8 * (size_t)N);
+          for (int sw = 0; sw < cf->coarse_correction_sweeps && cur >= tol && !st->cc_breakdown; ++sw) {
+            st->cc_sweeps++;
+            mr_restrict(&agg, r, w.rhs);
+            for (int64_t c = 0; c < agg.nc; ++c) w.r[c] = w.rhs[c];
+            if (cpin >= 0) w.r[cpin] = 0.0;
+            const double eff = fmax(cf->coarse_correction_absolute_tolerance,
+                                    cf->coarse_correction_relative_tolerance * mr_wnorm(&agg, w.r));
+            st->cc_effective_tolerance = eff;
+            int conv = 0, bd = 0;
+            const int its = mr_coarse_cg(&A, &agg, cpin, &w, cf->coarse_correction_iterations, eff, &conv, &bd);
+            st->cc_converged = conv;
+            st->cc_iterations += its;
+            if (bd) { st->cc_breakdown = 1; break; }
+            mr_prolong(&agg, w.x, w.corr);
+            double best = cur, bscale = 0.0;
+            int have = 0;
+            for (double scale = 1.0; scale >= cf->coarse_correction_min_scale; scale *= 0.5) {
+              for (int64_t t = 0; t < N; ++t) cx[t] = x[t] + scale * w.corr[t];
+              const double cres = mr_residual_from(&A, rhs, cx, Ap, cr);
+              if (isfinite(cres) && cres <= best) {
+                best = cres;
+                bscale = scale;
+                memcpy(bx, cx, 8 * (size_t)N);
+                memcpy(br, cr, 8 * (size_t)N);
+                have = 1;
+              }
+            }
+            st->cc_final_residual = best;
+            if (best < cur && have) {
+              memcpy(x, bx, 8 * (size_t)N);
+              memcpy(r, br, 8 * (size_t)N);
+              cur = best;
+              st->final_residual = cur;
+              st->cc_accepted = 1;
+              st->cc_last_scale = bscale;
+              st->cc_accepted_sweeps++;
+            } else {
+              st->cc_rejected_sweeps++;
+              break;
+            }
+          }
+          free(bx);
+          free(br);
+          mr_coarse_work_free(&w);
+        }
+        mr_agg_free(&agg);
+      }
+    }
+    /* damped-Jacobi pre-relaxation with omega backoff, :1062-1110 */
     double om = fmax(0.0, cf->relaxation_omega), omin = fmax(0.0, cf->relaxation_min_omega);
     if (omin > om) omin = om;
     for (int sw = 0; sw < cf->relaxation_sweeps && cur >= tol; ++sw) {
@@ -692,7 +1021,20 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
     if (st->converged) st->final_residual = cur;
   }
   if (!st->converged && res0 >= tol) {
-    PRECOND();
+    /* the additive coarse preconditioner's aggregation and pin, :1119-1143 */
+    mr_cprec cpre;
+    memset(&cpre, 0, sizeof(cpre));
+    mr_cprec* cpp = NULL;
+    if (cf->use_coarse_preconditioner && cf->coarse_preconditioner_iterations > 0 &&
+        cf->coarse_preconditioner_scale > 0.0 && mr_level1_aggregation(g, cells, N, &cpre.agg)) {
+      st->cp_cells = (int32_t)cpre.agg.nc;
+      cpre.ready = cpre.agg.nc > 0 && cpre.agg.nc < N;
+      cpre.cpin = (cpre.ready && pin >= 0) ? cpre.agg.f2c[pin] : -1;
+      mr_coarse_work_alloc(&cpre.w, cpre.agg.nc, N);
+      cpp = &cpre;
+    }
+#define PRECOND_PCG() mr_precondition(&A, cf, st, cpp, r, z, cx)
+    PRECOND_PCG();
     memcpy(p, z, 8 * (size_t)N);
     double rz = 0.0;
     for (int64_t t = 0; t < N; ++t) rz += r[t] * z[t];
@@ -719,7 +1061,7 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
       if (res < tol) { st->converged = 1; break; }
       if (cf->adaptive_restart && cf->restart_growth_threshold > 0.0 && res > prev * cf->restart_growth_threshold &&
           res > res0 * cf->restart_growth_threshold && it + 1 < cf->max_iterations) {
-        PRECOND();
+        PRECOND_PCG();
         rz = 0.0;
         for (int64_t t = 0; t < N; ++t) rz += r[t] * z[t];
         if (!isfinite(rz)) { st->breakdown = 1; break; }
@@ -730,7 +1072,7 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
       }
       prev = res;
       memcpy(zp, z, 8 * (size_t)N);
-      PRECOND();
+      PRECOND_PCG();
       double rzn = 0.0;
       for (int64_t t = 0; t < N; ++t) rzn += r[t] * z[t];
       if (!isfinite(rzn)) { st->breakdown = 1; break; }
@@ -748,6 +1090,11 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
       if (beta < 0.0) { beta = 0.0; st->beta_resets++; }
       rz = rzn;
       for (int64_t t = 0; t < N; ++t) p[t] = z[t] + beta * p[t];
+    }
+#undef PRECOND_PCG
+    if (cpp) {
+      mr_agg_free(&cpre.agg);
+      mr_coarse_work_free(&cpre.w);
     }
   }
   /* true residual (FP64) for comparisons */

@@ -72,14 +72,34 @@ class MRSolveCfg(C.Structure):
                 ("adaptive_restart", C.c_int32), ("absolute_tolerance", C.c_double),
                 ("relative_tolerance", C.c_double), ("restart_growth_threshold", C.c_double),
                 ("relaxation_sweeps", C.c_int32), ("pad0", C.c_int32), ("relaxation_omega", C.c_double),
-                ("relaxation_min_omega", C.c_double), ("divergence_correction", C.c_double)]
+                ("relaxation_min_omega", C.c_double), ("divergence_correction", C.c_double),
+                ("use_coarse_correction", C.c_int32), ("coarse_correction_iterations", C.c_int32),
+                ("coarse_correction_sweeps", C.c_int32), ("use_coarse_preconditioner", C.c_int32),
+                ("coarse_preconditioner_iterations", C.c_int32), ("coarse_preconditioner_auto_disable", C.c_int32),
+                ("coarse_preconditioner_auto_disable_after", C.c_int32), ("pad1", C.c_int32),
+                ("coarse_correction_absolute_tolerance", C.c_double),
+                ("coarse_correction_relative_tolerance", C.c_double), ("coarse_correction_min_scale", C.c_double),
+                ("coarse_preconditioner_absolute_tolerance", C.c_double),
+                ("coarse_preconditioner_relative_tolerance", C.c_double),
+                ("coarse_preconditioner_scale", C.c_double), ("coarse_preconditioner_min_rz_gain", C.c_double),
+                ("coarse_preconditioner_max_work_ratio", C.c_double)]
 
 
 def mr_solve_cfg(max_iterations=160, abs_tol=1e-7, rel_tol=1e-6, jacobi=1, flexible=0, restart=1, growth=10.0,
-                 sweeps=0, omega=0.67, min_omega=0.05, cdiv=0.0):
-    """MRPressureSolveConfig3D with MRSim3DTP's defaults (multires_sim3d_tp.h:36-47)."""
-    return MRSolveCfg(max_iterations, jacobi, flexible, restart, abs_tol, rel_tol, growth, sweeps, 0, omega,
-                      min_omega, cdiv)
+                 sweeps=0, omega=0.67, min_omega=0.05, cdiv=0.0, **coarse):
+    """MRPressureSolveConfig3D with MRSim3DTP's defaults (multires_sim3d_tp.h:36-60); the
+    coarse options are keyword arguments named as the structure's fields."""
+    c = MRSolveCfg(max_iterations, jacobi, flexible, restart, abs_tol, rel_tol, growth, sweeps, 0, omega,
+                   min_omega, cdiv)
+    c.coarse_correction_iterations, c.coarse_correction_sweeps = 32, 1
+    c.coarse_correction_relative_tolerance, c.coarse_correction_min_scale = 1e-3, 1.0 / 64.0
+    c.coarse_preconditioner_iterations, c.coarse_preconditioner_relative_tolerance = 8, 1e-2
+    c.coarse_preconditioner_scale, c.coarse_preconditioner_auto_disable_after = 1.0, 4
+    for k, v in coarse.items():
+        if not hasattr(c, k):
+            raise TypeError(f"unknown solve option {k}")
+        setattr(c, k, v)
+    return c
 
 
 class MRSolveStats(C.Structure):
@@ -88,7 +108,15 @@ class MRSolveStats(C.Structure):
                 ("relaxation_rejected", C.c_int32), ("converged", C.c_int32), ("breakdown", C.c_int32),
                 ("pad0", C.c_int32), ("initial_residual", C.c_double), ("final_residual", C.c_double),
                 ("effective_tolerance", C.c_double), ("true_max_residual", C.c_double),
-                ("true_rel_residual", C.c_double)]
+                ("true_rel_residual", C.c_double)] + [
+        (n, C.c_int32) for n in ("cc_cells", "cc_iterations", "cc_sweeps", "cc_accepted_sweeps",
+                                 "cc_rejected_sweeps", "cc_accepted", "cc_converged", "cc_breakdown", "cp_cells",
+                                 "cp_applications", "cp_accepted", "cp_rejected", "cp_skipped",
+                                 "cp_budget_limited", "cp_iterations", "cp_auto_disabled",
+                                 "cp_auto_disable_iteration", "cp_wasted_streak", "cp_breakdown",
+                                 "cp_budget_exhausted")] + [
+        (n, C.c_double) for n in ("cc_initial_residual", "cc_final_residual", "cc_last_scale",
+                                  "cc_effective_tolerance", "cp_effective_tolerance", "cp_last_rz_gain")]
 
 
 class MRSim(C.Structure):
@@ -788,10 +816,17 @@ class RefMR:
 
     def project(self, dt, cfg: MRSolveCfg):
         ic = np.array([cfg.max_iterations, cfg.use_jacobi, cfg.use_flexible_beta, cfg.adaptive_restart,
-                       cfg.relaxation_sweeps], np.int32)
+                       cfg.relaxation_sweeps, cfg.use_coarse_correction, cfg.coarse_correction_iterations,
+                       cfg.coarse_correction_sweeps, cfg.use_coarse_preconditioner,
+                       cfg.coarse_preconditioner_iterations, cfg.coarse_preconditioner_auto_disable,
+                       cfg.coarse_preconditioner_auto_disable_after], np.int32)
         c = np.array([cfg.absolute_tolerance, cfg.relative_tolerance, cfg.restart_growth_threshold,
-                      cfg.relaxation_omega, cfg.relaxation_min_omega, cfg.divergence_correction], np.float64)
-        out = np.zeros(12)
+                      cfg.relaxation_omega, cfg.relaxation_min_omega, cfg.divergence_correction,
+                      cfg.coarse_correction_absolute_tolerance, cfg.coarse_correction_relative_tolerance,
+                      cfg.coarse_correction_min_scale, cfg.coarse_preconditioner_absolute_tolerance,
+                      cfg.coarse_preconditioner_relative_tolerance, cfg.coarse_preconditioner_scale,
+                      cfg.coarse_preconditioner_min_rz_gain, cfg.coarse_preconditioner_max_work_ratio], np.float64)
+        out = np.zeros(30)
         reflib().ref_mr_project(self.h, dt, ic, c, out)
         return out
 

@@ -372,21 +372,44 @@ void ref_mr_advect(void* sp, double dt, int order, int32_t* stats) {
     stats[4] = es.clamped_y_lo; stats[5] = es.clamped_y_hi; stats[6] = es.clamped_z_lo; stats[7] = es.clamped_z_hi;
   }
 }
-// cfg as orc_mr_solve_cfg; out: iterations, restarts, beta_resets, relax accepted, relax rejected,
-// converged, breakdown, initial residual, final residual, effective tolerance, active cells, pinned cell
+// cfg as orc_mr_solve_cfg: ic = max_iterations, jacobi, flexible, restart, relaxation sweeps,
+// coarse correction (on, iterations, sweeps), coarse preconditioner (on, iterations, auto disable,
+// auto disable after); c = abs tol, rel tol, growth, omega, min omega, divergence correction,
+// coarse correction (abs tol, rel tol, min scale), coarse preconditioner (abs tol, rel tol, scale,
+// min rz gain, max work ratio).
+// out: iterations, restarts, beta_resets, relax accepted, relax rejected, converged, breakdown,
+// initial residual, final residual, effective tolerance, active cells, pinned cell, then the coarse
+// statistics (see the assignments below)
 void ref_mr_project(void* sp, double dt, const int32_t* ic, const double* c, double* out) {
   auto* s = mrs(sp);
   MRPressureSolveConfig3D cf;
   cf.max_iterations = ic[0]; cf.use_jacobi_preconditioner = ic[1] != 0; cf.use_flexible_cg_beta = ic[2] != 0;
   cf.adaptive_restart = ic[3] != 0; cf.relaxation_sweeps = ic[4];
+  cf.use_coarse_correction = ic[5] != 0; cf.coarse_correction_iterations = ic[6]; cf.coarse_correction_sweeps = ic[7];
+  cf.use_coarse_preconditioner = ic[8] != 0; cf.coarse_preconditioner_iterations = ic[9];
+  cf.coarse_preconditioner_auto_disable = ic[10] != 0; cf.coarse_preconditioner_auto_disable_after = ic[11];
   cf.absolute_tolerance = c[0]; cf.relative_tolerance = c[1]; cf.restart_growth_threshold = c[2];
   cf.relaxation_omega = c[3]; cf.relaxation_min_omega = c[4]; cf.divergence_correction = c[5];
+  cf.coarse_correction_absolute_tolerance = c[6]; cf.coarse_correction_relative_tolerance = c[7];
+  cf.coarse_correction_min_scale = c[8]; cf.coarse_preconditioner_absolute_tolerance = c[9];
+  cf.coarse_preconditioner_relative_tolerance = c[10]; cf.coarse_preconditioner_scale = c[11];
+  cf.coarse_preconditioner_min_rz_gain = c[12]; cf.coarse_preconditioner_max_work_ratio = c[13];
   MRPressureSolveStats3D st;
   projectMR3D(s->grid, s->phase, dt, cf, &st);
   if (out) {
     out[0] = st.iterations; out[1] = st.restarts; out[2] = st.beta_resets; out[3] = st.relaxation_accepted;
     out[4] = st.relaxation_rejected; out[5] = st.converged; out[6] = st.breakdown; out[7] = st.initial_residual;
     out[8] = st.final_residual; out[9] = st.effective_tolerance; out[10] = st.active_cells; out[11] = st.pinned_cell;
+    out[12] = st.coarse_correction_cells; out[13] = st.coarse_correction_iterations;
+    out[14] = st.coarse_correction_sweeps; out[15] = st.coarse_correction_accepted_sweeps;
+    out[16] = st.coarse_correction_rejected_sweeps; out[17] = st.coarse_correction_accepted;
+    out[18] = st.coarse_correction_last_scale; out[19] = st.coarse_correction_final_residual;
+    out[20] = st.coarse_preconditioner_cells; out[21] = st.coarse_preconditioner_applications;
+    out[22] = st.coarse_preconditioner_accepted_applications; out[23] = st.coarse_preconditioner_rejected_applications;
+    out[24] = st.coarse_preconditioner_skipped_applications;
+    out[25] = st.coarse_preconditioner_budget_limited_applications;
+    out[26] = st.coarse_preconditioner_iterations; out[27] = st.coarse_preconditioner_auto_disabled;
+    out[28] = st.coarse_preconditioner_auto_disable_iteration; out[29] = st.coarse_preconditioner_last_rz_gain;
   }
 }
 // dense views: faces over the fine face grids; P / MK per fine cell (its leaf cell's value)

@@ -292,3 +292,104 @@ def test_step_bit_exact_with_moving_refinement_and_budget():
             _compare_step(g, r, pa)
         if max_fine:
             assert sim.budget_limited == 1
+
+
+def _coarse_case(n, filled):
+    """a bubble-band system on the reference build and the oracle, markers shared;
+    `filled`: liquid in every interior cell (no AIR, so the solve pins a cell)"""
+    r = _bubble_ref(n)
+    ph = _phase()
+    r.config(dt=0.02, phase=ph)
+    ps = r.particles()
+    if filled:
+        pts = [(i + 0.5, j + 0.5, k + 0.5) for k in range(1, n - 1) for j in range(1, n - 1) for i in range(1, n - 1)]
+        p = np.array(pts)
+        ps = O.Particles(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), type=np.zeros(len(p), np.uint8))
+    rng = np.random.default_rng(7)
+    for a in (ps.u, ps.v, ps.w):
+        a[:] = rng.normal(size=len(ps))
+    r.set_particles(ps)
+    g = O.MRGrid(n, n, n, 1.0, faithful=True)
+    g.set_leaves(r.leaves())
+    r.p2g()
+    g.p2g(ps, ph, 1.0)
+    g.mark_cells(ps)
+    mk = g.dense(O.MK)
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                r.set(O.MK, i, j, k, mk[k, j, i])
+    return r, g, ph
+
+
+COARSE_CASES = [
+    dict(use_coarse_correction=1),
+    dict(use_coarse_correction=1, coarse_correction_sweeps=3, coarse_correction_iterations=6),
+    dict(use_coarse_correction=1, coarse_correction_sweeps=2, coarse_correction_min_scale=0.25,
+         coarse_correction_relative_tolerance=0.0, coarse_correction_iterations=200),
+    dict(use_coarse_preconditioner=1),
+    dict(use_coarse_preconditioner=1, coarse_preconditioner_min_rz_gain=0.5, coarse_preconditioner_auto_disable=1,
+         coarse_preconditioner_auto_disable_after=2),
+    dict(use_coarse_preconditioner=1, coarse_preconditioner_max_work_ratio=1.5, coarse_preconditioner_iterations=4),
+    dict(use_coarse_preconditioner=1, coarse_preconditioner_scale=0.5, use_coarse_correction=1, flexible=1),
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("filled", [False, True])
+@pytest.mark.parametrize("opts", COARSE_CASES)
+def test_coarse_machinery_bit_exact(filled, opts):
+    # level-1 aggregation, volume-weighted restriction, piecewise-constant
+    # prolongation and the pinned R A P coarse CG (multires_pressure3d.cpp:39-70,
+    # 470-700), used as the pre-solve correction with its scale search (:886-1060) and
+    # as the additive preconditioner with rz-gain guard, work budget and auto-disable
+    # (:1119-1341): statistics and fields equal the reference build's
+    n = 16
+    r, g, ph = _coarse_case(n, filled)
+    opts = dict(opts)
+    flexible = opts.pop("flexible", 0)
+    cfg = O.mr_solve_cfg(max_iterations=400, flexible=flexible, **opts)
+    st = g.project(ph, 0.02, cfg)
+    out = r.project(0.02, cfg)
+    assert st.pinned_cell == out[11] and (st.pinned_cell >= 0) == filled
+    assert (st.iterations, st.restarts, st.beta_resets, st.converged, st.breakdown) == \
+        tuple(int(v) for v in (out[0], out[1], out[2], out[5], out[6]))
+    assert st.final_residual == out[8]
+    cc = (st.cc_cells, st.cc_iterations, st.cc_sweeps, st.cc_accepted_sweeps, st.cc_rejected_sweeps, st.cc_accepted)
+    assert cc == tuple(int(v) for v in out[12:18])
+    assert st.cc_last_scale == out[18] and st.cc_final_residual == out[19]
+    cp = (st.cp_cells, st.cp_applications, st.cp_accepted, st.cp_rejected, st.cp_skipped, st.cp_budget_limited,
+          st.cp_iterations, st.cp_auto_disabled, st.cp_auto_disable_iteration)
+    assert cp == tuple(int(v) for v in out[20:29])
+    assert st.cp_last_rz_gain == out[29]
+    if opts.get("use_coarse_correction"):
+        assert st.cc_cells > 0 and st.cc_sweeps > 0
+    if opts.get("use_coarse_preconditioner"):
+        assert st.cp_applications + st.cp_skipped > 0
+    for f in (O.U, O.V, O.W, O.P):
+        assert np.array_equal(g.dense(f), r.dense(f)), f
+
+
+def test_coarse_correction_lowers_the_residual_and_invalid_configs_are_rejected():
+    # without the reference build: an accepted coarse sweep never raises the max-norm
+    # residual (the scale search keeps the best of 1, 1/2, ... 1/64 against the start);
+    # validateSolveConfig's cases (multires_pressure3d.cpp:307-333) stop the solve
+    n = 16
+    ph = _phase()
+    g = O.MRGrid(n, n, n, 1.0)
+    g.set_leaves(O.mr_layout_bubble_band(n, n, n))
+    pts = [(i + 0.5, j + 0.5, k + 0.5) for k in range(1, n - 1) for j in range(1, n // 2) for i in range(1, n - 1)]
+    p = np.array(pts)
+    rng = np.random.default_rng(3)
+    ps = O.Particles(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), rng.normal(size=len(p)),
+                     rng.normal(size=len(p)), rng.normal(size=len(p)), type=np.zeros(len(p), np.uint8))
+    g.mark_cells(ps)
+    g.p2g(ps, ph, 1.0)
+    base = g.clone()
+    st = g.project(ph, 0.02, O.mr_solve_cfg(max_iterations=0, use_coarse_correction=1, coarse_correction_sweeps=4))
+    assert st.cc_sweeps >= 1 and st.cc_accepted == 1
+    assert st.final_residual < st.initial_residual and st.cc_final_residual == st.final_residual
+    for bad in (dict(coarse_correction_min_scale=0.0), dict(coarse_correction_min_scale=1.5),
+                dict(coarse_preconditioner_scale=-1.0), dict(coarse_preconditioner_auto_disable_after=-1)):
+        st = base.clone().project(ph, 0.02, O.mr_solve_cfg(**bad))
+        assert st.breakdown == 1 and st.iterations == 0, bad
