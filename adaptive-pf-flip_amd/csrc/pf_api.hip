@@ -25,6 +25,7 @@
 #include "pf_g2p.hip"
 #include "pf_lpc.hip"
 #include "pf_mr.hip"
+#include "pf_mr_ref.hip"
 
 using namespace pf;
 
@@ -673,7 +674,453 @@ void record(pf_ctx* c, bool timing, int slot) {
   if (timing) hipEventRecord(c->ev[slot], c->stream);
 }
 
-// single: the single-phase unit-density projection (ph unused, cdiv ignored)
+// ---------------------------------------------------------------- reference-order multires solve
+
+// validateSolveConfig, multires_pressure3d.cpp:307-333
+bool mr_cfg_valid(const pf_mr_solve_cfg& c) {
+  return !(c.max_iterations < 0 || c.absolute_tolerance < 0.0 || c.relative_tolerance < 0.0 ||
+           c.restart_growth_threshold < 0.0 || c.relaxation_sweeps < 0 || c.relaxation_omega < 0.0 ||
+           c.relaxation_min_omega < 0.0 || c.coarse_correction_iterations < 0 || c.coarse_correction_sweeps < 0 ||
+           c.coarse_correction_absolute_tolerance < 0.0 || c.coarse_correction_relative_tolerance < 0.0 ||
+           c.coarse_correction_min_scale <= 0.0 || c.coarse_correction_min_scale > 1.0 ||
+           c.coarse_preconditioner_iterations < 0 || c.coarse_preconditioner_absolute_tolerance < 0.0 ||
+           c.coarse_preconditioner_relative_tolerance < 0.0 || c.coarse_preconditioner_scale < 0.0 ||
+           c.coarse_preconditioner_min_rz_gain < 0.0 || c.coarse_preconditioner_max_work_ratio < 0.0 ||
+           c.coarse_preconditioner_auto_disable_after < 0);
+}
+
+constexpr int kMrfVectors = 15;
+
+// projectMR3D's iteration (multires_pressure3d.cpp:846-1449) on FP64 device vectors:
+// the coarse correction, the damped-Jacobi relaxation, then PCG with adaptive
+// restart, (flexible) beta with resets and the additive coarse preconditioner.  x is
+// c->x64 (zero on entry); the rows, the pin and the reference rhs come from the
+// assembly.  Fills c->mr_detail and the iteration outcome.
+int mr_reference_solve(pf_ctx* c, const pf_mr_solve_cfg& cf, unsigned G, const VecArgs& va, const MrOp64& op,
+                       long long nfluid, int& iters, bool& conv, bool& brk) {
+  pf_mr_solve_detail& D = c->mr_detail;
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  double* F[kMrfVectors];
+  for (int q = 0; q < kMrfVectors; ++q) F[q] = c->mrf + (long long)q * ncell;
+  double *r = F[0], *z = F[1], *zp = F[2], *p = F[3], *Ap = F[4], *cand_x = F[5], *cand_r = F[6];
+  double *best_x = F[7], *best_r = F[8], *fp = F[9], *fap = F[10], *corr = F[11];
+  const double *rhs = F[12], *dref = F[13], *vtrue = F[14];
+  double* x = c->x64;
+  MrfArgs A;
+  A.va = va;
+  A.op = op;
+  A.dref = dref;
+  A.vtrue = vtrue;
+  A.pin = *c->mr_pin_host;
+  const long long N = nfluid;
+
+  auto reduce = [&](unsigned g, int nslots, unsigned mask) -> int {
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)g, nslots, mask, c->red);
+    LAUNCH_CHECK(c);
+    return read_small(c);
+  };
+  // one vector operation; its reductions land in red_host[0..1]
+  auto vec = [&](int opk, double s0, const double* a, const double* b, const double* cc, double* out, double* out2,
+                 bool want) -> int {
+    MrfVec v{opk, cf.use_jacobi_preconditioner ? 1 : 0, s0, a, b, cc, out, out2};
+    hipLaunchKernelGGL(k_mrf_vec, dim3(G), dim3(256), 0, c->stream, A, v);
+    LAUNCH_CHECK(c);
+    const bool maxslot = opk == MRF_MAXABS || opk == MRF_XR;
+    return want ? reduce(G, 2, maxslot ? 1u : 0u) : PF_OK;
+  };
+  auto apply = [&](const double* in, double* out) -> int {
+    hipLaunchKernelGGL(k_mrf_apply<false>, dim3(G), dim3(256), 0, c->stream, A, in, out, nullptr, nullptr);
+    LAUNCH_CHECK(c);
+    return PF_OK;
+  };
+  // residualFromPressure (:870-884): out = rhs - A xin, max |out| (+inf if not finite)
+  auto residual_from = [&](const double* xin, double* out, double& res) -> int {
+    hipLaunchKernelGGL(k_mrf_apply<true>, dim3(G), dim3(256), 0, c->stream, A, xin, Ap, rhs, out);
+    LAUNCH_CHECK(c);
+    int st = reduce(G, 1, 1u);
+    res = c->red_host[0];
+    return st;
+  };
+  auto copy = [&](double* dst, const double* src) -> int {
+    HIP_TRY(c, hipMemcpyAsync(dst, src, sizeof(double) * (size_t)ncell, hipMemcpyDeviceToDevice, c->stream));
+    return PF_OK;
+  };
+  auto max_abs_r = [&](double& out) -> int {
+    int s2 = vec(MRF_MAXABS, 0.0, r, nullptr, nullptr, nullptr, nullptr, true);
+    out = c->red_host[0];
+    return s2;
+  };
+
+  // ---- the level-1 coarse space (:39-70, 470-552) on a dense (n/2)^3 grid
+  const Geom& g = c->geom;
+  CoarseGrid cg;
+  cg.hx = (g.nx + 1) / 2;
+  cg.hy = (g.ny + 1) / 2;
+  cg.hz = (g.nz + 1) / 2;
+  cg.nc = (long long)cg.hx * cg.hy * cg.hz;
+  cg.dc = c->dc;
+  cg.g = g;
+  const unsigned Gc = (unsigned)std::min<long long>(kMaxPartials, std::max<long long>(1, (cg.nc + 255) / 256));
+  bool coarse_built = false;
+  long long n_agg = 0, cpin = -1;
+  double *Vc = nullptr, *crhs = nullptr, *cx = nullptr, *cr = nullptr, *cp = nullptr, *cap = nullptr;
+  auto build_coarse = [&]() -> int {
+    if (coarse_built) return PF_OK;
+    if (cg.nc * 6 > c->mrc_cap) {
+      dfree(c->mrc);
+      c->mrc = nullptr;
+      c->mrc_cap = 0;
+      HIP_TRY(c, dalloc(&c->mrc, cg.nc * 6));
+      c->mrc_cap = cg.nc * 6;
+    }
+    Vc = c->mrc;
+    crhs = Vc + cg.nc;
+    cx = crhs + cg.nc;
+    cr = cx + cg.nc;
+    cp = cr + cg.nc;
+    cap = cp + cg.nc;
+    hipLaunchKernelGGL(k_mrf_coarse_volume, dim3(Gc), dim3(256), 0, c->stream, cg, op.vol, vtrue, Vc, c->partial);
+    LAUNCH_CHECK(c);
+    int s2 = reduce(Gc, 1, 0u);
+    if (s2) return s2;
+    n_agg = (long long)c->red_host[0];
+    cpin = -1;
+    if (A.pin >= 0) {
+      const long long b = A.pin >> 6;
+      const int lane = (int)(A.pin & 63);
+      int bx, by, bz;
+      block_coords(c->dc, b, bx, by, bz);
+      const int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
+      cpin = (long long)(i >> 1) + (long long)cg.hx * ((long long)(j >> 1) + (long long)cg.hy * (long long)(k >> 1));
+    }
+    coarse_built = true;
+    return PF_OK;
+  };
+  auto coarse_op = [&](int opk, double s0, const double* a, const double* b, bool want) -> int {
+    MrcVec v{opk, cpin, s0, Vc, a, b, cx, cr, cp};
+    if (opk == MRC_PINSET) v.x = cap;
+    hipLaunchKernelGGL(k_mrf_coarse, dim3(Gc), dim3(256), 0, c->stream, cg, v, c->partial);
+    LAUNCH_CHECK(c);
+    return want ? reduce(Gc, 1, 0u) : PF_OK;
+  };
+  auto restrict_to = [&](const double* fine, double* coarse) -> int {
+    hipLaunchKernelGGL(k_mrf_restrict, dim3(Gc), dim3(256), 0, c->stream, cg, op.vol, vtrue, Vc, fine, coarse);
+    LAUNCH_CHECK(c);
+    return PF_OK;
+  };
+  auto prolong_to = [&](const double* coarse, double* fine) -> int {
+    hipLaunchKernelGGL(k_mrf_prolong, dim3(G), dim3(256), 0, c->stream, A, cg, coarse, fine);
+    LAUNCH_CHECK(c);
+    return PF_OK;
+  };
+  // the pinned coarse CG on R A P (:946-1015 and :1214-1281), right-hand side in crhs
+  auto coarse_cg = [&](int max_it, double abs_t, double rel_t, double& eff, int& its, bool& cconv,
+                       bool& cbd) -> int {
+    int s2 = coarse_op(MRC_INIT, 0.0, crhs, nullptr, true);
+    if (s2) return s2;
+    double rr = c->red_host[0];
+    const double res0 = rr > 0.0 ? std::sqrt(rr) : 0.0;
+    eff = std::max(abs_t, rel_t * res0);
+    cconv = res0 <= eff;
+    cbd = false;
+    its = 0;
+    if (!std::isfinite(rr)) {
+      cbd = true;
+      return PF_OK;
+    }
+    for (int it = 0; !cconv && !cbd && it < max_it; ++it) {
+      if ((s2 = prolong_to(cp, fp)) || (s2 = apply(fp, fap)) || (s2 = restrict_to(fap, cap)) ||
+          (s2 = coarse_op(MRC_PINSET, 0.0, cp, nullptr, false)))
+        return s2;
+      if ((s2 = coarse_op(MRC_DOT, 0.0, cp, cap, true))) return s2;
+      const double pAp = c->red_host[0];
+      if (!std::isfinite(pAp) || std::abs(pAp) < 1e-30) { cbd = true; break; }
+      const double alpha = rr / pAp;
+      if (!std::isfinite(alpha)) { cbd = true; break; }
+      if ((s2 = coarse_op(MRC_XR, alpha, cap, nullptr, true))) return s2;
+      const double rrn = c->red_host[0];
+      its = it + 1;
+      const double res = rrn > 0.0 ? std::sqrt(rrn) : 0.0;
+      if (!std::isfinite(res)) { cbd = true; break; }
+      if (res <= eff) { cconv = true; break; }
+      if (!std::isfinite(rrn) || std::abs(rr) < 1e-30) { cbd = true; break; }
+      const double beta = rrn / rr;
+      if (!std::isfinite(beta)) { cbd = true; break; }
+      rr = rrn;
+      if ((s2 = coarse_op(MRC_P, beta, nullptr, nullptr, false))) return s2;
+    }
+    return PF_OK;
+  };
+
+  int st = PF_OK;
+  iters = 0;
+  conv = false;
+  brk = false;
+  // rhs and the initial residual (:846-863); x = 0
+  if ((st = copy(r, rhs))) return st;
+  double res0 = 0.0;
+  if ((st = max_abs_r(res0))) return st;
+  D.initial_residual = res0;
+  D.final_residual = res0;
+  const double tol = std::max(std::max(0.0, cf.absolute_tolerance), std::max(0.0, cf.relative_tolerance) * res0);
+  D.effective_tolerance = tol;
+  conv = res0 < tol;
+  double cur = res0;
+  if (res0 >= tol) {
+    // the coarse correction sweeps with a halving scale search (:886-1060)
+    if (cf.use_coarse_correction && cur >= tol) {
+      if ((st = build_coarse())) return st;
+      D.coarse_correction_cells = (int32_t)n_agg;
+      D.coarse_correction_initial_residual = cur;
+      D.coarse_correction_final_residual = cur;
+      if (n_agg > 0 && n_agg < N) {
+        for (int sw = 0; sw < cf.coarse_correction_sweeps && cur >= tol && !D.coarse_correction_breakdown; ++sw) {
+          D.coarse_correction_sweeps++;
+          if ((st = restrict_to(r, crhs))) return st;
+          double eff = 0.0;
+          int its = 0;
+          bool cc = false, cb = false;
+          if ((st = coarse_cg(cf.coarse_correction_iterations, cf.coarse_correction_absolute_tolerance,
+                              cf.coarse_correction_relative_tolerance, eff, its, cc, cb)))
+            return st;
+          D.coarse_correction_effective_tolerance = eff;
+          D.coarse_correction_converged = cc ? 1 : 0;
+          D.coarse_correction_iterations += its;
+          if (cb) {
+            D.coarse_correction_breakdown = 1;
+            break;
+          }
+          if ((st = prolong_to(cx, corr))) return st;
+          double best = cur, bscale = 0.0;
+          bool have = false;
+          for (double scale = 1.0; scale >= cf.coarse_correction_min_scale; scale *= 0.5) {
+            if ((st = vec(MRF_AXPY, scale, x, corr, nullptr, cand_x, nullptr, false))) return st;
+            double cres = 0.0;
+            if ((st = residual_from(cand_x, cand_r, cres))) return st;
+            if (std::isfinite(cres) && cres <= best) {
+              best = cres;
+              bscale = scale;
+              std::swap(cand_x, best_x);
+              std::swap(cand_r, best_r);
+              have = true;
+            }
+          }
+          D.coarse_correction_final_residual = best;
+          if (best < cur && have) {
+            if ((st = copy(x, best_x)) || (st = copy(r, best_r))) return st;
+            cur = best;
+            D.final_residual = cur;
+            D.coarse_correction_accepted = 1;
+            D.coarse_correction_last_scale = bscale;
+            D.coarse_correction_accepted_sweeps++;
+          } else {
+            D.coarse_correction_rejected_sweeps++;
+            break;
+          }
+        }
+      }
+    }
+    // damped-Jacobi relaxation with omega backoff (:1062-1110)
+    double om = std::max(0.0, cf.relaxation_omega), omin = std::max(0.0, cf.relaxation_min_omega);
+    if (omin > om) omin = om;
+    D.relaxation_final_omega = om;
+    for (int sw = 0; sw < cf.relaxation_sweeps && cur >= tol; ++sw) {
+      if ((st = vec(MRF_JACOBI, 0.0, r, nullptr, nullptr, z, nullptr, false))) return st;
+      double trial = om;
+      bool accepted = false;
+      while (trial >= omin && trial > 0.0) {
+        if ((st = vec(MRF_AXPY, trial, x, z, nullptr, cand_x, nullptr, false))) return st;
+        double cres = 0.0;
+        if ((st = residual_from(cand_x, cand_r, cres))) return st;
+        if (std::isfinite(cres) && cres <= cur) {
+          if ((st = copy(x, cand_x)) || (st = copy(r, cand_r))) return st;
+          cur = cres;
+          D.final_residual = cur;
+          om = trial;
+          D.relaxation_final_omega = om;
+          D.relaxation_accepted++;
+          accepted = true;
+          break;
+        }
+        D.relaxation_rejected++;
+        if (trial <= omin) break;
+        trial = std::max(omin, trial * 0.5);
+      }
+      if (!accepted) break;
+    }
+    conv = cur < tol;
+    if (conv) D.final_residual = cur;
+  }
+  if (conv || res0 < tol) return PF_OK;
+
+  // PCG with the additive coarse preconditioner (:1112-1449)
+  const bool use_cp = cf.use_coarse_preconditioner && cf.coarse_preconditioner_iterations > 0 &&
+                      cf.coarse_preconditioner_scale > 0.0;
+  bool cp_ready = false, cp_disabled = false;
+  int cp_streak = 0;
+  if (use_cp) {
+    if ((st = build_coarse())) return st;
+    D.coarse_preconditioner_cells = (int32_t)n_agg;
+    cp_ready = n_agg > 0 && n_agg < N;
+  }
+  // applyPreconditioner (:1283-1340): Jacobi, then z + scale P x_c when a coarse
+  // solve ran and r.z grew by at least min_rz_gain
+  auto precondition = [&]() -> int {
+    int s2 = vec(MRF_JACOBI, 0.0, r, nullptr, nullptr, z, nullptr, use_cp);
+    if (s2 || !use_cp) return s2;
+    const double rzb = c->red_host[0];
+    const bool base_ok = std::isfinite(rzb) && rzb > 0.0;
+    if (cp_disabled) return PF_OK;
+    const int skipped_before = D.coarse_preconditioner_skipped_applications;
+    bool accepted = false, wasted = false, solved = false;
+    if (cp_ready) {
+      const int max_c = cf.coarse_preconditioner_iterations;
+      bool budget_ok = true;
+      if (cf.coarse_preconditioner_max_work_ratio > 0.0) {
+        const double allowed = cf.coarse_preconditioner_max_work_ratio * (double)std::max(1, iters + 1);
+        const int remaining = (int)std::floor(allowed + 1e-12) - D.coarse_preconditioner_iterations;
+        if (remaining < max_c) {
+          D.coarse_preconditioner_budget_limited_applications++;
+          D.coarse_preconditioner_skipped_applications++;
+          D.coarse_preconditioner_budget_exhausted = 1;
+          budget_ok = false;
+        }
+      }
+      if (budget_ok) {
+        D.coarse_preconditioner_applications++;
+        if ((s2 = restrict_to(r, crhs))) return s2;
+        double eff = 0.0;
+        int its = 0;
+        bool cc = false, cb = false;
+        if ((s2 = coarse_cg(max_c, cf.coarse_preconditioner_absolute_tolerance,
+                            cf.coarse_preconditioner_relative_tolerance, eff, its, cc, cb)))
+          return s2;
+        D.coarse_preconditioner_effective_tolerance = eff;
+        if (cb) {
+          D.coarse_preconditioner_breakdown = 1;
+        } else {
+          D.coarse_preconditioner_iterations += its;
+          if ((s2 = prolong_to(cx, corr))) return s2;
+          solved = true;
+        }
+      }
+    }
+    if (solved) {
+      if ((s2 = vec(MRF_AXPY, cf.coarse_preconditioner_scale, z, corr, nullptr, cand_x, nullptr, false))) return s2;
+      if ((s2 = vec(MRF_DOT, 0.0, r, cand_x, nullptr, nullptr, nullptr, true))) return s2;
+      const double rzc = c->red_host[0];
+      if (std::isfinite(rzc) && base_ok) D.coarse_preconditioner_last_rz_gain = rzc / rzb - 1.0;
+      const double min_acc = base_ok ? rzb * (1.0 + cf.coarse_preconditioner_min_rz_gain) : 0.0;
+      if (std::isfinite(rzc) && rzc > 0.0 && (!base_ok || rzc >= min_acc)) {
+        if ((s2 = copy(z, cand_x))) return s2;
+        D.coarse_preconditioner_accepted_applications++;
+        accepted = true;
+      } else {
+        D.coarse_preconditioner_rejected_applications++;
+        wasted = true;
+      }
+    } else if (D.coarse_preconditioner_skipped_applications > skipped_before) {
+      wasted = true;
+    }
+    // recordCoarsePreconditionerOutcome (:1157-1178)
+    if (accepted) {
+      cp_streak = 0;
+      return PF_OK;
+    }
+    if (!wasted) return PF_OK;
+    ++cp_streak;
+    if (cf.coarse_preconditioner_auto_disable && cf.coarse_preconditioner_auto_disable_after > 0 &&
+        cp_streak >= cf.coarse_preconditioner_auto_disable_after) {
+      cp_disabled = true;
+      D.coarse_preconditioner_auto_disabled = 1;
+      if (D.coarse_preconditioner_auto_disable_iteration < 0) D.coarse_preconditioner_auto_disable_iteration = iters;
+    }
+    return PF_OK;
+  };
+  auto dot = [&](const double* a, const double* b, double& out) -> int {
+    int s2 = vec(MRF_DOT, 0.0, a, b, nullptr, nullptr, nullptr, true);
+    out = c->red_host[0];
+    return s2;
+  };
+  auto stop_on_breakdown = [&]() -> int {
+    brk = true;
+    return max_abs_r(D.final_residual);
+  };
+
+  if ((st = precondition()) || (st = copy(p, z))) return st;
+  double rz = 0.0;
+  if ((st = dot(r, z, rz))) return st;
+  if (!std::isfinite(rz) && (st = stop_on_breakdown())) return st;
+  double prev = D.final_residual;
+  for (int it = 0; !brk && it < cf.max_iterations; ++it) {
+    if ((st = apply(p, Ap))) return st;
+    double pAp = 0.0;
+    if ((st = dot(p, Ap, pAp))) return st;
+    if (!std::isfinite(pAp) || std::abs(pAp) < 1e-30) {
+      if ((st = stop_on_breakdown())) return st;
+      break;
+    }
+    const double alpha = rz / pAp;
+    if (!std::isfinite(alpha)) {
+      if ((st = stop_on_breakdown())) return st;
+      break;
+    }
+    if ((st = vec(MRF_XR, alpha, p, Ap, nullptr, x, r, true))) return st;
+    const double res = c->red_host[0];
+    const bool finite_state = c->red_host[1] == 0.0;
+    iters = it + 1;
+    D.final_residual = res;
+    if (!finite_state || !std::isfinite(res)) {
+      if ((st = stop_on_breakdown())) return st;
+      break;
+    }
+    if (res < tol) {
+      conv = true;
+      break;
+    }
+    if (cf.adaptive_restart && cf.restart_growth_threshold > 0.0 && res > prev * cf.restart_growth_threshold &&
+        res > res0 * cf.restart_growth_threshold && it + 1 < cf.max_iterations) {
+      if ((st = precondition()) || (st = dot(r, z, rz))) return st;
+      if (!std::isfinite(rz)) {
+        if ((st = stop_on_breakdown())) return st;
+        break;
+      }
+      if ((st = copy(p, z))) return st;
+      D.restarts++;
+      prev = res;
+      continue;
+    }
+    prev = res;
+    if ((st = copy(zp, z)) || (st = precondition())) return st;
+    double rzn = 0.0;
+    if ((st = dot(r, z, rzn))) return st;
+    if (!std::isfinite(rzn)) {
+      if ((st = stop_on_breakdown())) return st;
+      break;
+    }
+    double beta = 0.0;
+    if (rz != 0.0) {
+      if (cf.use_flexible_cg_beta) {
+        if ((st = vec(MRF_FLEX, 0.0, r, z, zp, nullptr, nullptr, true))) return st;
+        beta = c->red_host[0] / rz;
+      } else {
+        beta = rzn / rz;
+      }
+    }
+    if (!std::isfinite(beta)) {
+      beta = 0.0;
+      D.beta_resets++;
+    }
+    if (beta < 0.0) {
+      beta = 0.0;
+      D.beta_resets++;
+    }
+    rz = rzn;
+    if ((st = vec(MRF_P, beta, z, nullptr, nullptr, p, nullptr, false))) return st;
+  }
+  return PF_OK;
+}
+
 // single: the single-phase unit-density projection (ph unused, cdiv ignored);
 // c->mr: the multires finite-volume projection, stopping when
 // |rhs - A x|_inf < max(cfg->tol, mr_rel_tol * |rhs|_inf)
@@ -754,6 +1201,17 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   pa.pin = c->u32_dev;
   pa.single = single ? 1 : 0;
   const unsigned G = list_grid(npb);
+  const bool refmode = mr && c->mr_cfg && c->mr_cfg->mode == PF_MR_SOLVE_REFERENCE;
+  if (mr) {
+    c->mr_detail = pf_mr_solve_detail{};
+    c->mr_detail.coarse_preconditioner_auto_disable_iteration = -1;
+  }
+  if (refmode) {
+    if (!c->mrf) HIP_TRY(c, dalloc(&c->mrf, (long long)kMrfVectors * c->dc.nblocks * kBlockVol));
+    if (!c->mr_pin_dev) HIP_TRY(c, dalloc(&c->mr_pin_dev, 1));
+    if (!c->mr_pin_host) HIP_TRY(c, hipHostMalloc((void**)&c->mr_pin_host, sizeof(long long)));
+    HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->mr_pin_dev), 0xffffffffu, 2));
+  }
   if (mr) {
     // multires: couplings are written per fine face, so the arrays start at zero
     const Dims* fd[3] = {&c->du, &c->dv, &c->dw};
@@ -772,6 +1230,13 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     ma.cd32 = c->cd32; ma.invd32 = c->invd32;
     for (int a = 0; a < 3; ++a) { ma.cf64[a] = c->mr_cf64[a]; ma.cf32[a] = c->mr_cf32[a]; }
     ma.part = c->partial;
+    if (refmode) {
+      const long long ncell = c->dc.nblocks * kBlockVol;
+      ma.rref = c->mrf + 12 * ncell;
+      ma.dref = c->mrf + 13 * ncell;
+      ma.vtrue = c->mrf + 14 * ncell;
+      ma.pin_cell = c->mr_pin_dev;
+    }
     hipLaunchKernelGGL(k_mr_pin, dim3(G), dim3(256), 0, c->stream, ma);
     LAUNCH_CHECK(c);
     TIMED(c, PF_K_ASSEMBLE, hipLaunchKernelGGL(k_mr_assemble, dim3(G), dim3(256), 0, c->stream, ma));
@@ -864,7 +1329,24 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   bool learned = want_learned;
   const unsigned GT = learned ? lpc_scatter_grid(c->ntiles) : 1u;
   int* done_dev = &c->sc->done;
-  if (!converged && max_iters > 0 && std::isfinite(rmax)) {
+  if (refmode) {
+    // the reference's FP64 iteration with its options; x in x64
+    HIP_TRY(c, hipMemcpyAsync(c->mr_pin_host, c->mr_pin_dev, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->mr_detail.reference_order = 1;
+    int ri = 0;
+    bool rc = false, rb = false;
+    st = mr_reference_solve(c, *c->mr_cfg, G, va, mop64, (long long)c->u32_host[4], ri, rc, rb);
+    if (st) return st;
+    total = ri;
+    converged = rc;
+    if (rb) S.breakdown = 1;
+    st = residual();
+    if (st) return st;
+    rr = c->red_host[0];
+    rmax = c->red_host[1];
+  }
+  if (!refmode && !converged && max_iters > 0 && std::isfinite(rmax)) {
     // (re)start: r32 from r64, x32 = 0, p = 0, z = M(r)
     auto start = [&](int budget) -> int {
       hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32, c->pA,
@@ -1279,6 +1761,8 @@ int pf_destroy(pf_ctx* c) {
   double* f64[] = {c->b64, c->x64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64};
   for (double* p : f64) hipFree(p);
   dfree(c->mr_lvl); dfree(c->mr_vol64); dfree(c->mr_pfull); dfree(c->mr_bbox); dfree(c->mr_leaves_dev);
+  dfree(c->mrf); dfree(c->mrc); dfree(c->mr_pin_dev);
+  if (c->mr_pin_host) hipHostFree(c->mr_pin_host);
   for (int a = 0; a < 3; ++a) { dfree(c->mr_valid[a]); dfree(c->mr_cf32[a]); dfree(c->mr_cf64[a]); }
   if (c->mr_bbox_host) hipHostFree(c->mr_bbox_host);
   dfree(c->fbeta); dfree(c->fphi); dfree(c->tiles); dfree(c->tile_flags); dfree(c->snaps);
@@ -1949,14 +2433,65 @@ static pf_solve_cfg mr_to_solve(const pf_mr_solve_cfg* m) {
   return s;
 }
 
+// the multires projection with `cfg` in force for this solve only
+static int mr_project_with(pf_ctx* c, const pf_phase* ph, double dt, const pf_mr_solve_cfg* cfg, double c_div,
+                           pf_solve_stats* stats, bool timing, int e0, int e1, int e2) {
+  if (cfg->max_iterations < 0 || cfg->absolute_tolerance < 0 || cfg->relative_tolerance < 0)
+    return fail(c, PF_ERR_ARG, "invalid multires solve configuration");
+  if (cfg->mode != PF_MR_SOLVE_FAST && cfg->mode != PF_MR_SOLVE_REFERENCE)
+    return fail(c, PF_ERR_ARG, "unknown multires solve mode");
+  if (cfg->mode == PF_MR_SOLVE_REFERENCE && (!mr_cfg_valid(*cfg) || !std::isfinite(c_div)))
+    return fail(c, PF_ERR_ARG, "projectMR3D invalid solve config");
+  const pf_solve_cfg s = mr_to_solve(cfg);
+  c->mr_cfg = cfg;
+  const int st = run_project(c, ph, dt, &s, c_div, stats, timing, e0, e1, e2, false, cfg->relative_tolerance);
+  c->mr_cfg = nullptr;
+  return st;
+}
+
 int pf_mr_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_mr_solve_cfg* cfg, double c_div,
                   pf_solve_stats* stats) {
   if (!c || !ph || !cfg) return PF_ERR_ARG;
   if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
-  if (cfg->max_iterations < 0 || cfg->absolute_tolerance < 0 || cfg->relative_tolerance < 0)
-    return fail(c, PF_ERR_ARG, "invalid multires solve configuration");
-  const pf_solve_cfg s = mr_to_solve(cfg);
-  return run_project(c, ph, dt, &s, c_div, stats, false, 0, 0, 0, false, cfg->relative_tolerance);
+  return mr_project_with(c, ph, dt, cfg, c_div, stats, false, 0, 0, 0);
+}
+
+void pf_mr_solve_defaults(pf_mr_solve_cfg* m) {
+  if (!m) return;
+  *m = pf_mr_solve_cfg{};
+  m->max_iterations = 160;
+  m->check_every = 8;
+  m->absolute_tolerance = 1e-7;
+  m->relative_tolerance = 1e-6;
+  m->inner_rel_tol = 0.1;
+  m->max_outer = 10000;
+  m->mode = PF_MR_SOLVE_FAST;
+  m->use_jacobi_preconditioner = 1;
+  m->use_flexible_cg_beta = 0;
+  m->adaptive_restart = 1;
+  m->relaxation_sweeps = 0;
+  m->restart_growth_threshold = 10.0;
+  m->relaxation_omega = 0.67;
+  m->relaxation_min_omega = 0.05;
+  m->coarse_correction_iterations = 32;
+  m->coarse_correction_sweeps = 1;
+  m->coarse_correction_absolute_tolerance = 0.0;
+  m->coarse_correction_relative_tolerance = 1e-3;
+  m->coarse_correction_min_scale = 1.0 / 64.0;
+  m->coarse_preconditioner_iterations = 8;
+  m->coarse_preconditioner_absolute_tolerance = 0.0;
+  m->coarse_preconditioner_relative_tolerance = 1e-2;
+  m->coarse_preconditioner_scale = 1.0;
+  m->coarse_preconditioner_min_rz_gain = 0.0;
+  m->coarse_preconditioner_max_work_ratio = 0.0;
+  m->coarse_preconditioner_auto_disable = 0;
+  m->coarse_preconditioner_auto_disable_after = 4;
+}
+
+int pf_mr_last_solve_detail(pf_ctx* c, pf_mr_solve_detail* d) {
+  if (!c || !d) return PF_ERR_ARG;
+  *d = c->mr_detail;
+  return PF_OK;
 }
 
 int pf_mr_update_refinement(pf_ctx* c, const pf_mr_refine_cfg* rc, int32_t* fine_leaves, int32_t* budget_limited) {
@@ -2004,8 +2539,7 @@ int pf_mr_step(pf_ctx* c, const pf_mr_step_cfg* cfg, pf_step_stats* stats) {
   st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | FO_WALLS, (float)(dt * cfg->gravity));
   if (st) return st;
   record(c, timing, 4);
-  const pf_solve_cfg s = mr_to_solve(&cfg->solve);
-  st = run_project(c, &cfg->phase, dt, &s, cfg->c_div, &S.solve, timing, 5, 6, 7, false, cfg->solve.relative_tolerance);
+  st = mr_project_with(c, &cfg->phase, dt, &cfg->solve, cfg->c_div, &S.solve, timing, 5, 6, 7);
   if (st) return st;
   HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->esc_dev), 0u, 8));
   st = run_g2p_advect(c, true, true, cfg->alpha_liquid, cfg->alpha_gas, dt, cfg->advection_order);

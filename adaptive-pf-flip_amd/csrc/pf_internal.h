@@ -201,6 +201,14 @@ struct Ctx {
   bool mr_retained_valid = false;   // dynamic refinement state (MRSim3DTP::dynamic_retained_*)
   int mr_retained[6] = {0, 0, 0, 0, 0, 0};
   int mr_budget_limited = 0, mr_last_fine_leaves = 0;
+  // the reference-order FP64 multires solve (options of MRPressureSolveConfig3D)
+  const pf_mr_solve_cfg* mr_cfg = nullptr;  // set by pf_mr_project / pf_mr_step for one solve
+  pf_mr_solve_detail mr_detail{};
+  double* mrf = nullptr;            // 15 fine FP64 vectors (ncell each)
+  double* mrc = nullptr;            // 6 coarse FP64 vectors
+  long long mrc_cap = 0;
+  long long* mr_pin_dev = nullptr;  // pinned anchor (block-major index) or -1
+  long long* mr_pin_host = nullptr;
 
   // per-kernel event timing (measurement hook)
   bool ktime = false;

@@ -132,6 +132,11 @@ struct MrAsm {
   float *cd32, *cf32[3], *invd32;
   double* part;
   uint32_t* order_key;    // FLUID anchors: the reference's leafCells order (for the pin)
+  // optional, the reference's own row quantities (for the reference-order solve):
+  double* dref;           // diag = sum of cond / V over non-solid patches (1 on the pinned row)
+  double* rref;           // rhs = -divergence + cdiv (0 on the pinned row)
+  double* vtrue;          // h^3 of the leaf cell (the pinned one included)
+  long long* pin_cell;    // the pinned anchor's block-major index
 };
 
 __device__ inline bool mr_is_anchor(const MrGrid& m, int i, int j, int k, int& L) {
@@ -242,7 +247,7 @@ __global__ void __launch_bounds__(256) k_mr_assemble(MrAsm a) {
     block_coords(a.m.dc, b, bx, by, bz);
     const int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
     int L = 0;
-    double rhs = 0.0, cd = 0.0, diag = 0.0, V = 0.0;
+    double rhs = 0.0, cd = 0.0, diag = 0.0, V = 0.0, dref = 0.0, vtrue = 0.0;
     const bool anchor = i < a.m.g.nx && j < a.m.g.ny && k < a.m.g.nz && mr_is_anchor(a.m, i, j, k, L) &&
                         a.m.marker[c] == CELL_FLUID;
     bool pinned = false;
@@ -264,6 +269,7 @@ __global__ void __launch_bounds__(256) k_mr_assemble(MrAsm a) {
         const double dist = 0.5 * h + 0.5 * hn;
         const double cond = (dist > 0.0) ? a.dt * beta_of_raw((double)mr_face_mass(a, axis, fi), a.ph) * area / dist : 0.0;
         diag += cond;
+        dref += (V > 0.0) ? cond / V : 0.0;
         bool coupled = false;
         if (mn == CELL_FLUID) {
           const int sn = 1 << Ln;
@@ -276,8 +282,11 @@ __global__ void __launch_bounds__(256) k_mr_assemble(MrAsm a) {
         div += sign * (double)mr_face_vel(a, axis, fi) * area / V;
       });
       rhs = pinned ? 0.0 : -div + a.cdiv;
-      if (pinned) { cd = 1.0; diag = 1.0; V = 1.0; }
+      vtrue = V;
+      if (pinned) { cd = 1.0; diag = 1.0; V = 1.0; dref = 1.0; }
+      if (pinned && a.pin_cell) a.pin_cell[0] = c;
     }
+    if (a.dref) { a.dref[c] = dref; a.rref[c] = anchor ? rhs : 0.0; a.vtrue[c] = vtrue; }
     // S = V A: rows of the symmetric conductance form
     a.b64[c] = V * rhs;
     a.vol64[c] = anchor ? V : 0.0;

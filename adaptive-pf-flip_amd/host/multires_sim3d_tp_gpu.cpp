@@ -216,12 +216,36 @@ void MRSim3DTP::step() {
   cfg.adaptive_timestep = adaptive_timestep ? 1 : 0;
   cfg.advection_order = advection_order;
   cfg.phase = pf_phase{phase.rho_l, phase.rho_g, phase.alpha_phi, phase.rho_tilde_0};
-  cfg.solve.max_iterations = cg_iters;
-  cfg.solve.check_every = 8;
-  cfg.solve.absolute_tolerance = cg_tol;
-  cfg.solve.relative_tolerance = cg_rel_tol;
-  cfg.solve.inner_rel_tol = 0.1;
-  cfg.solve.max_outer = 10000;
+  pf_mr_solve_cfg& sc = cfg.solve;
+  pf_mr_solve_defaults(&sc);
+  sc.max_iterations = cg_iters;
+  sc.absolute_tolerance = cg_tol;
+  sc.relative_tolerance = cg_rel_tol;
+  sc.use_jacobi_preconditioner = cg_jacobi_preconditioner;
+  sc.use_flexible_cg_beta = cg_flexible_beta;
+  sc.adaptive_restart = cg_adaptive_restart;
+  sc.restart_growth_threshold = cg_restart_growth;
+  sc.relaxation_sweeps = cg_relaxation_sweeps;
+  sc.relaxation_omega = cg_relaxation_omega;
+  sc.relaxation_min_omega = cg_relaxation_min_omega;
+  sc.use_coarse_correction = cg_coarse_correction;
+  sc.coarse_correction_iterations = cg_coarse_correction_iters;
+  sc.coarse_correction_sweeps = cg_coarse_correction_sweeps;
+  sc.coarse_correction_absolute_tolerance = cg_coarse_correction_abs_tol;
+  sc.coarse_correction_relative_tolerance = cg_coarse_correction_rel_tol;
+  sc.coarse_correction_min_scale = cg_coarse_correction_min_scale;
+  sc.use_coarse_preconditioner = cg_coarse_preconditioner;
+  sc.coarse_preconditioner_iterations = cg_coarse_preconditioner_iters;
+  sc.coarse_preconditioner_absolute_tolerance = cg_coarse_preconditioner_abs_tol;
+  sc.coarse_preconditioner_relative_tolerance = cg_coarse_preconditioner_rel_tol;
+  sc.coarse_preconditioner_scale = cg_coarse_preconditioner_scale;
+  sc.coarse_preconditioner_min_rz_gain = cg_coarse_preconditioner_min_rz_gain;
+  sc.coarse_preconditioner_max_work_ratio = cg_coarse_preconditioner_max_work_ratio;
+  sc.coarse_preconditioner_auto_disable = cg_coarse_preconditioner_auto_disable;
+  sc.coarse_preconditioner_auto_disable_after = cg_coarse_preconditioner_auto_disable_after;
+  const bool options = !cg_jacobi_preconditioner || cg_flexible_beta || cg_relaxation_sweeps > 0 ||
+                       cg_coarse_correction || cg_coarse_preconditioner;
+  sc.mode = (options || cg_reference_order) ? PF_MR_SOLVE_REFERENCE : PF_MR_SOLVE_FAST;
   cfg.refine.dynamic_refinement = dynamic_refinement ? 1 : 0;
   cfg.refine.particle_padding = dynamic_particle_padding;
   cfg.refine.gas_padding = dynamic_gas_padding;
@@ -265,6 +289,33 @@ void MRSim3DTP::step() {
   last_pressure_stats.effective_tolerance = std::max(cg_tol, cg_rel_tol * s.initial_max_residual);
   last_pressure_stats.converged = s.converged != 0;
   last_pressure_stats.breakdown = s.breakdown != 0;
+  pf_mr_solve_detail d{};
+  check(pf_mr_last_solve_detail(ctx_, &d), "pf_mr_last_solve_detail");
+  if (d.reference_order) {
+    MRPressureSolveStats3D& L = last_pressure_stats;
+    L.final_residual = d.final_residual;
+    L.effective_tolerance = d.effective_tolerance;
+    L.restarts = d.restarts;
+    L.beta_resets = d.beta_resets;
+    L.relaxation_accepted = d.relaxation_accepted;
+    L.relaxation_rejected = d.relaxation_rejected;
+    L.coarse_correction_cells = d.coarse_correction_cells;
+    L.coarse_correction_iterations = d.coarse_correction_iterations;
+    L.coarse_correction_sweeps = d.coarse_correction_sweeps;
+    L.coarse_correction_accepted_sweeps = d.coarse_correction_accepted_sweeps;
+    L.coarse_correction_rejected_sweeps = d.coarse_correction_rejected_sweeps;
+    L.coarse_correction_accepted = d.coarse_correction_accepted != 0;
+    L.coarse_correction_last_scale = d.coarse_correction_last_scale;
+    L.coarse_preconditioner_cells = d.coarse_preconditioner_cells;
+    L.coarse_preconditioner_applications = d.coarse_preconditioner_applications;
+    L.coarse_preconditioner_accepted_applications = d.coarse_preconditioner_accepted_applications;
+    L.coarse_preconditioner_rejected_applications = d.coarse_preconditioner_rejected_applications;
+    L.coarse_preconditioner_skipped_applications = d.coarse_preconditioner_skipped_applications;
+    L.coarse_preconditioner_iterations = d.coarse_preconditioner_iterations;
+    L.coarse_preconditioner_auto_disabled = d.coarse_preconditioner_auto_disabled != 0;
+    L.coarse_preconditioner_auto_disable_iteration = d.coarse_preconditioner_auto_disable_iteration;
+    L.coarse_preconditioner_last_rz_gain = d.coarse_preconditioner_last_rz_gain;
+  }
   particle_boundary_clamped_liquid_last = stats_last.escape.clamped_liquid;
   particle_boundary_clamped_gas_last = stats_last.escape.clamped_gas;
 }

@@ -24,11 +24,21 @@ struct MRBlockKey3D {
   int level = 0, bx = 0, by = 0, bz = 0;
 };
 
-// the MRPressureSolveStats3D fields this build reports
+// the MRPressureSolveStats3D fields this build reports (multires_pressure3d.h:60-136);
+// the option counters are filled when the solve ran in the reference's order
 struct MRPressureSolveStats3D {
   int active_cells = 0, pinned_cell = -1, iterations = 0;
+  int restarts = 0, beta_resets = 0, relaxation_accepted = 0, relaxation_rejected = 0;
+  int coarse_correction_cells = 0, coarse_correction_iterations = 0, coarse_correction_sweeps = 0;
+  int coarse_correction_accepted_sweeps = 0, coarse_correction_rejected_sweeps = 0;
+  int coarse_preconditioner_cells = 0, coarse_preconditioner_applications = 0;
+  int coarse_preconditioner_accepted_applications = 0, coarse_preconditioner_rejected_applications = 0;
+  int coarse_preconditioner_skipped_applications = 0, coarse_preconditioner_iterations = 0;
+  int coarse_preconditioner_auto_disable_iteration = -1;
   double initial_residual = 0.0, final_residual = 0.0, effective_tolerance = 0.0;
-  bool converged = false, breakdown = false;
+  double coarse_correction_last_scale = 0.0, coarse_preconditioner_last_rz_gain = 0.0;
+  bool converged = false, breakdown = false, coarse_correction_accepted = false;
+  bool coarse_preconditioner_auto_disabled = false;
 };
 
 class MRSim3DTP {
@@ -46,6 +56,25 @@ class MRSim3DTP {
   double liquid_volume_error_last = 0.0, c_div_last = 0.0;
   int cg_iters = 160;
   double cg_tol = 1e-7, cg_rel_tol = 1e-6;
+  // the reference's solver options (multires_sim3d_tp.h:37-60); any non-default one
+  // runs the solve in the reference's FP64 order (PF_MR_SOLVE_REFERENCE)
+  bool cg_jacobi_preconditioner = true, cg_flexible_beta = false, cg_adaptive_restart = true;
+  double cg_restart_growth = 10.0;
+  int cg_relaxation_sweeps = 0;
+  double cg_relaxation_omega = 0.67, cg_relaxation_min_omega = 0.05;
+  bool cg_coarse_correction = false;
+  int cg_coarse_correction_iters = 32, cg_coarse_correction_sweeps = 1;
+  double cg_coarse_correction_abs_tol = 0.0, cg_coarse_correction_rel_tol = 1e-3;
+  double cg_coarse_correction_min_scale = 1.0 / 64.0;
+  bool cg_coarse_preconditioner = false;
+  int cg_coarse_preconditioner_iters = 8;
+  double cg_coarse_preconditioner_abs_tol = 0.0, cg_coarse_preconditioner_rel_tol = 1e-2;
+  double cg_coarse_preconditioner_scale = 1.0, cg_coarse_preconditioner_min_rz_gain = 0.0;
+  double cg_coarse_preconditioner_max_work_ratio = 0.0;
+  bool cg_coarse_preconditioner_auto_disable = false;
+  int cg_coarse_preconditioner_auto_disable_after = 4;
+  // no reference counterpart: run the reference's order even with default options
+  bool cg_reference_order = false;
   bool dynamic_refinement = true;
   int dynamic_particle_padding = 1, dynamic_gas_padding = 2, dynamic_hysteresis_cells = 1;
   int dynamic_max_fine_leaves = 0;

@@ -11,6 +11,7 @@
 //   tests/test_multires_sim3d_tp.cpp:422-439  adaptive timestep reports the effective dt
 //   tests/test_multires_sim3d_tp.cpp:558-573  c_div uses the liquid volume error
 //   tests/test_multires_sim3d_tp.cpp:575-594  high density ratio pressure converges
+// plus a step with the reference's coarse correction and coarse preconditioner on
 // Host particle edits are followed by markParticlesDirty() (particles live on the
 // device) and device state is read back with syncParticlesToHost().
 // Exit status 0 when every check passes.
@@ -267,6 +268,30 @@ int main() {
     CHECK(!sim.last_pressure_stats.breakdown);
     std::printf("mr rho ratio 1000: %d iterations, residual %.3e <= %.3e\n", sim.last_pressure_stats.iterations,
                 sim.last_pressure_stats.final_residual, sim.last_pressure_stats.effective_tolerance);
+  }
+  {
+    // the reference's coarse options through the class fields: the solve runs in the
+    // reference's FP64 order and reports the option counters
+    MRSim3DTP sim(16, 16, 16, 1.0);
+    sim.dt = 0.02;
+    sim.cg_iters = 400;
+    sim.cg_coarse_correction = true;
+    sim.cg_coarse_preconditioner = true;
+    sim.initBubbleTankInterfaceBand();
+    const size_t n0 = sim.particles.size();
+    sim.step();
+    sim.syncParticlesToHost();
+    const auto& st = sim.last_pressure_stats;
+    CHECK(sim.particles.size() == n0);
+    CHECK(finite(sim));
+    CHECK(st.converged);
+    CHECK(st.final_residual < st.effective_tolerance);
+    CHECK(st.coarse_correction_cells > 0 && st.coarse_correction_sweeps > 0);
+    CHECK(st.coarse_preconditioner_applications > 0);
+    std::printf("mr coarse options: %d iterations, %d coarse sweeps (%d accepted), %d coarse preconditioner "
+                "applications (%d accepted)\n", st.iterations, st.coarse_correction_sweeps,
+                st.coarse_correction_accepted_sweeps, st.coarse_preconditioner_applications,
+                st.coarse_preconditioner_accepted_applications);
   }
   std::printf(failures ? "FAILED %d checks\n" : "all checks passed\n", failures);
   return failures ? 1 : 0;

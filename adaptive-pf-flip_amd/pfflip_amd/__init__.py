@@ -63,15 +63,57 @@ class EscapeStats(C.Structure):
 MODEL_TWO_PHASE, MODEL_SINGLE_PHASE = 0, 1
 
 
+MR_SOLVE_FAST, MR_SOLVE_REFERENCE = 0, 1
+
+
 class MRSolveCfg(C.Structure):
     _fields_ = [("max_iterations", C.c_int32), ("check_every", C.c_int32), ("absolute_tolerance", C.c_double),
                 ("relative_tolerance", C.c_double), ("inner_rel_tol", C.c_double), ("max_outer", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("mode", C.c_int32), ("use_jacobi_preconditioner", C.c_int32), ("use_flexible_cg_beta", C.c_int32),
+                ("adaptive_restart", C.c_int32), ("relaxation_sweeps", C.c_int32),
+                ("restart_growth_threshold", C.c_double), ("relaxation_omega", C.c_double),
+                ("relaxation_min_omega", C.c_double), ("use_coarse_correction", C.c_int32),
+                ("coarse_correction_iterations", C.c_int32), ("coarse_correction_sweeps", C.c_int32),
+                ("use_coarse_preconditioner", C.c_int32), ("coarse_correction_absolute_tolerance", C.c_double),
+                ("coarse_correction_relative_tolerance", C.c_double), ("coarse_correction_min_scale", C.c_double),
+                ("coarse_preconditioner_iterations", C.c_int32), ("coarse_preconditioner_auto_disable", C.c_int32),
+                ("coarse_preconditioner_auto_disable_after", C.c_int32), ("reserved", C.c_int32),
+                ("coarse_preconditioner_absolute_tolerance", C.c_double),
+                ("coarse_preconditioner_relative_tolerance", C.c_double),
+                ("coarse_preconditioner_scale", C.c_double), ("coarse_preconditioner_min_rz_gain", C.c_double),
+                ("coarse_preconditioner_max_work_ratio", C.c_double)]
 
 
-def mr_solve_cfg(max_iterations=160, abs_tol=1e-7, rel_tol=1e-6):
-    """MRSim3DTP's solver defaults (cg_iters 160, cg_tol 1e-7, cg_rel_tol 1e-6)."""
-    return MRSolveCfg(max_iterations, 8, abs_tol, rel_tol, 0.1, 10000, 0)
+class MRSolveDetail(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "reference_order", "restarts", "beta_resets", "relaxation_accepted", "relaxation_rejected",
+        "coarse_correction_cells", "coarse_correction_iterations", "coarse_correction_sweeps",
+        "coarse_correction_accepted_sweeps", "coarse_correction_rejected_sweeps", "coarse_correction_accepted",
+        "coarse_correction_converged", "coarse_correction_breakdown", "coarse_preconditioner_cells",
+        "coarse_preconditioner_applications", "coarse_preconditioner_accepted_applications",
+        "coarse_preconditioner_rejected_applications", "coarse_preconditioner_skipped_applications",
+        "coarse_preconditioner_budget_limited_applications", "coarse_preconditioner_iterations",
+        "coarse_preconditioner_auto_disabled", "coarse_preconditioner_auto_disable_iteration",
+        "coarse_preconditioner_breakdown", "coarse_preconditioner_budget_exhausted")] + [
+        (n, C.c_double) for n in (
+            "initial_residual", "final_residual", "effective_tolerance", "relaxation_final_omega",
+            "coarse_correction_effective_tolerance", "coarse_correction_initial_residual",
+            "coarse_correction_final_residual", "coarse_correction_last_scale",
+            "coarse_preconditioner_effective_tolerance", "coarse_preconditioner_last_rz_gain")]
+
+
+def mr_solve_cfg(max_iterations=160, abs_tol=1e-7, rel_tol=1e-6, **options):
+    """MRSim3DTP's solver defaults (multires_sim3d_tp.h:34-60: cg_iters 160, cg_tol 1e-7,
+    cg_rel_tol 1e-6, ...) from pf_mr_solve_defaults; `options` set fields by name (for
+    example mode=MR_SOLVE_REFERENCE, use_coarse_preconditioner=1)."""
+    c = MRSolveCfg()
+    lib().pf_mr_solve_defaults(C.byref(c))
+    c.max_iterations, c.absolute_tolerance, c.relative_tolerance = max_iterations, abs_tol, rel_tol
+    for k, v in options.items():
+        if k not in dict(MRSolveCfg._fields_):
+            raise TypeError(f"unknown multires solve option {k}")
+        setattr(c, k, v)
+    return c
 
 
 class MRRefineCfg(C.Structure):
@@ -163,6 +205,8 @@ def lib():
             "pf_mr_layout_refine": (i64, [i32, i32, i32, vp, i64, vp, vp, i64]),
             "pf_mr_retained_box": (C.c_int, [vp, vp, C.POINTER(i32)]),
             "pf_mr_step": (C.c_int, [vp, C.POINTER(MRStepCfg), C.POINTER(StepStats)]),
+            "pf_mr_solve_defaults": (None, [C.POINTER(MRSolveCfg)]),
+            "pf_mr_last_solve_detail": (C.c_int, [vp, C.POINTER(MRSolveDetail)]),
             "pf_lpc_create": (C.c_int, [C.POINTER(vp), C.c_uint64]),
             "pf_lpc_destroy": (C.c_int, [vp]),
             "pf_lpc_last_error": (C.c_char_p, [vp]),
@@ -605,6 +649,12 @@ class MRContext(Context):
         self._check(lib().pf_mr_retained_box(self.h, _p(box), C.byref(valid)), "pf_mr_retained_box")
         return bool(valid.value), box
 
+    def mr_solve_detail(self):
+        """MRPressureSolveStats3D counters of the last multires projection"""
+        d = MRSolveDetail()
+        self._check(lib().pf_mr_last_solve_detail(self.h, C.byref(d)), "pf_mr_last_solve_detail")
+        return d
+
     def mr_step(self, cfg):
         st = StepStats()
         self._check(lib().pf_mr_step(self.h, C.byref(cfg), C.byref(st)), "pf_mr_step")
@@ -626,6 +676,9 @@ class MRSim3DTP:
         self.cg_iters, self.cg_tol, self.cg_rel_tol = 160, 1e-7, 1e-6
         self.dynamic_refinement, self.dynamic_particle_padding, self.dynamic_gas_padding = True, 1, 2
         self.dynamic_hysteresis_cells, self.dynamic_max_fine_leaves = 1, 0
+        # the reference's other MRPressureSolveConfig3D options by field name, e.g.
+        # {"mode": MR_SOLVE_REFERENCE, "use_coarse_correction": 1}
+        self.solve_options = {}
         self.c_div_last = 0.0
         self.timing = False
         self.stats_last = None
@@ -637,7 +690,7 @@ class MRSim3DTP:
         self.ctx.upload(x, y, z, u, v, w, type, volume)
 
     def step_cfg(self):
-        solve = MRSolveCfg(self.cg_iters, 8, self.cg_tol, self.cg_rel_tol, 0.1, 10000, 0)
+        solve = mr_solve_cfg(self.cg_iters, self.cg_tol, self.cg_rel_tol, **self.solve_options)
         refine = MRRefineCfg(int(self.dynamic_refinement), self.dynamic_particle_padding, self.dynamic_gas_padding,
                              self.dynamic_hysteresis_cells, self.dynamic_max_fine_leaves, 0)
         return MRStepCfg(dt=self.dt, gravity=self.gravity, Vp=self.Vp, alpha_liquid=self.alpha_liquid,

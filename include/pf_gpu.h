@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 1
+#define PF_ABI_VERSION 2
 
 enum pf_status {
   PF_OK = 0,
@@ -229,14 +229,53 @@ int pf_copy_probe(pf_ctx* ctx, int64_t n, double* ms);
  * the reference does not enumerate hold zero), markers and pressure are per leaf
  * cell (broadcast over its fine cells in downloads).  Positions are relative to the
  * origin 0, as the reference's multires code assumes. */
+enum pf_mr_solve_mode {
+  PF_MR_SOLVE_FAST = 0,       /* FP32 Jacobi PCG with FP64 refinement; the option fields below are ignored */
+  PF_MR_SOLVE_REFERENCE = 1   /* the reference's FP64 iteration (multires_pressure3d.cpp:846-1449) with the
+                                 options below, validated as validateSolveConfig (:307-333) */
+};
+
 typedef struct pf_mr_solve_cfg {    /* MRPressureSolveConfig3D (multires_pressure3d.h:137-166) */
   int32_t max_iterations;           /* cap on inner iterations (reference max_iterations) */
   int32_t check_every;              /* inner iterations between host convergence checks */
   double absolute_tolerance;        /* stop when |rhs - A x|_inf < max(abs, rel * |rhs|_inf) */
   double relative_tolerance;
   double inner_rel_tol;             /* inner FP32 PCG reduction per refinement pass */
-  int32_t max_outer, reserved;
+  int32_t max_outer;
+  int32_t mode;                     /* pf_mr_solve_mode */
+  /* PF_MR_SOLVE_REFERENCE: the reference's options under their names */
+  int32_t use_jacobi_preconditioner, use_flexible_cg_beta, adaptive_restart, relaxation_sweeps;
+  double restart_growth_threshold, relaxation_omega, relaxation_min_omega;
+  int32_t use_coarse_correction, coarse_correction_iterations, coarse_correction_sweeps;
+  int32_t use_coarse_preconditioner;
+  double coarse_correction_absolute_tolerance, coarse_correction_relative_tolerance;
+  double coarse_correction_min_scale;
+  int32_t coarse_preconditioner_iterations, coarse_preconditioner_auto_disable;
+  int32_t coarse_preconditioner_auto_disable_after, reserved;
+  double coarse_preconditioner_absolute_tolerance, coarse_preconditioner_relative_tolerance;
+  double coarse_preconditioner_scale, coarse_preconditioner_min_rz_gain;
+  double coarse_preconditioner_max_work_ratio;
 } pf_mr_solve_cfg;
+
+/* MRPressureSolveStats3D fields beyond pf_solve_stats (multires_pressure3d.h:60-136),
+ * filled by the last multires projection in PF_MR_SOLVE_REFERENCE mode (zero, and
+ * reference_order 0, after a PF_MR_SOLVE_FAST solve) */
+typedef struct pf_mr_solve_detail {
+  int32_t reference_order, restarts, beta_resets, relaxation_accepted, relaxation_rejected;
+  int32_t coarse_correction_cells, coarse_correction_iterations, coarse_correction_sweeps;
+  int32_t coarse_correction_accepted_sweeps, coarse_correction_rejected_sweeps;
+  int32_t coarse_correction_accepted, coarse_correction_converged, coarse_correction_breakdown;
+  int32_t coarse_preconditioner_cells, coarse_preconditioner_applications;
+  int32_t coarse_preconditioner_accepted_applications, coarse_preconditioner_rejected_applications;
+  int32_t coarse_preconditioner_skipped_applications, coarse_preconditioner_budget_limited_applications;
+  int32_t coarse_preconditioner_iterations, coarse_preconditioner_auto_disabled;
+  int32_t coarse_preconditioner_auto_disable_iteration, coarse_preconditioner_breakdown;
+  int32_t coarse_preconditioner_budget_exhausted;
+  double initial_residual, final_residual, effective_tolerance, relaxation_final_omega;
+  double coarse_correction_effective_tolerance, coarse_correction_initial_residual;
+  double coarse_correction_final_residual, coarse_correction_last_scale;
+  double coarse_preconditioner_effective_tolerance, coarse_preconditioner_last_rz_gain;
+} pf_mr_solve_detail;
 
 typedef struct pf_mr_refine_cfg {   /* MRSim3DTP dynamic refinement fields (multires_sim3d_tp.h:60-68) */
   int32_t dynamic_refinement, particle_padding, gas_padding, hysteresis_cells, max_fine_leaves, reserved;
@@ -252,6 +291,14 @@ typedef struct pf_mr_step_cfg {     /* the MRSim3DTP fields step() reads */
   pf_mr_solve_cfg solve;
   pf_mr_refine_cfg refine;
 } pf_mr_step_cfg;
+
+/* MRSim3DTP's solver defaults (multires_sim3d_tp.h:34-60) in PF_MR_SOLVE_FAST mode:
+ * max 160 iterations, |r|_inf < max(1e-7, 1e-6 |r0|_inf), Jacobi, adaptive restart
+ * (growth 10), no relaxation, coarse options off (32 iterations, 1 sweep, 1e-3, 1/64;
+ * 8 iterations, 1e-2, scale 1, auto-disable after 4) */
+void pf_mr_solve_defaults(pf_mr_solve_cfg* cfg);
+/* the MRPressureSolveStats3D counters of the last multires projection */
+int pf_mr_last_solve_detail(pf_ctx* ctx, pf_mr_solve_detail* detail);
 
 /* MRLayout3D::leaf_blocks (multires_layout3d.h:30): n (level, bx, by, bz) leaves that
  * tile the domain; they are sorted, the grid fields cleared (MRMacGrid3D(layout)) */

@@ -33,3 +33,18 @@ def step_cfg(ph, dt=1e-3, cg_iters=600, cg_tol=1e-7, stop_rule=O.STOP_REF_MAXABS
                      stop_rule=stop_rule, adaptive_cfl=0.5, adaptive_min_dt=1e-5, cg_tol=cg_tol,
                      c_div_volume_correction=0, c_div_strength=1.0, liquid_volume_target=0.0,
                      phase=ph)
+
+
+# option sets of MRPressureSolveConfig3D's coarse machinery (multires_pressure3d.h:148-165)
+# exercised by the multires oracle and GPU tests; "flexible" selects use_flexible_cg_beta
+COARSE_CASES = [
+    dict(use_coarse_correction=1),
+    dict(use_coarse_correction=1, coarse_correction_sweeps=3, coarse_correction_iterations=6),
+    dict(use_coarse_correction=1, coarse_correction_sweeps=2, coarse_correction_min_scale=0.25,
+         coarse_correction_relative_tolerance=0.0, coarse_correction_iterations=200),
+    dict(use_coarse_preconditioner=1),
+    dict(use_coarse_preconditioner=1, coarse_preconditioner_min_rz_gain=0.5, coarse_preconditioner_auto_disable=1,
+         coarse_preconditioner_auto_disable_after=2),
+    dict(use_coarse_preconditioner=1, coarse_preconditioner_max_work_ratio=1.5, coarse_preconditioner_iterations=4),
+    dict(use_coarse_preconditioner=1, coarse_preconditioner_scale=0.5, use_coarse_correction=1, flexible=1),
+]

@@ -56,7 +56,7 @@ def test_host_helpers_without_device():
     for raw in (0.0, 5.0, 7.0, 100.0, 900.0, 2000.0):
         assert L.pf_phi_from_raw(raw, C.byref(ph)) == O.phi(raw, oph)
     assert L.pf_beta_from_phi(0.25, C.byref(ph)) == O.beta(0.25, oph)
-    assert L.pf_abi_version() == 1
+    assert L.pf_abi_version() == 2
 
 
 def test_null_arguments_are_rejected_without_device():
@@ -73,13 +73,16 @@ def test_multires_struct_layouts_match_the_header(tmp_path):
     src.write_text('#include "pf_gpu.h"\n#include <stdio.h>\n#include <stddef.h>\n'
                    "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\","
                    "sizeof(pf_mr_solve_cfg),sizeof(pf_mr_refine_cfg),sizeof(pf_mr_step_cfg),"
-                   "offsetof(pf_mr_step_cfg,solve),offsetof(pf_mr_step_cfg,refine));return 0;}\n")
+                   "offsetof(pf_mr_step_cfg,solve),offsetof(pf_mr_step_cfg,refine));"
+                   "printf(\"%zu %zu %zu\\n\",sizeof(pf_mr_solve_detail),offsetof(pf_mr_solve_cfg,mode),"
+                   "offsetof(pf_mr_solve_cfg,coarse_preconditioner_max_work_ratio));return 0;}\n")
     exe = tmp_path / "mr_sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
                                           check=True).stdout.split()]
     want = [C.sizeof(G.MRSolveCfg), C.sizeof(G.MRRefineCfg), C.sizeof(G.MRStepCfg),
-            G.MRStepCfg.solve.offset, G.MRStepCfg.refine.offset]
+            G.MRStepCfg.solve.offset, G.MRStepCfg.refine.offset, C.sizeof(G.MRSolveDetail),
+            G.MRSolveCfg.mode.offset, G.MRSolveCfg.coarse_preconditioner_max_work_ratio.offset]
     assert got == want
 
 
@@ -99,3 +102,20 @@ def test_multires_layout_helpers_match_the_oracle(dims):
     assert np.array_equal(got, O.mr_layout_bubble_band(nx, ny, nz))
     # an empty or out-of-domain box leaves the layout as it was
     assert np.array_equal(G.mr_layout_refine(nx, ny, nz, band, [nx, 0, 0, nx + 4, 4, 4]), band)
+
+
+def test_multires_solve_defaults_are_the_reference_defaults():
+    # MRSim3DTP (multires_sim3d_tp.h:34-60) and MRPressureSolveConfig3D
+    # (multires_pressure3d.h:137-166) defaults, host-only
+    c = G.mr_solve_cfg()
+    assert (c.max_iterations, c.absolute_tolerance, c.relative_tolerance) == (160, 1e-7, 1e-6)
+    assert (c.mode, c.use_jacobi_preconditioner, c.use_flexible_cg_beta, c.adaptive_restart) == (0, 1, 0, 1)
+    assert (c.restart_growth_threshold, c.relaxation_sweeps, c.relaxation_omega, c.relaxation_min_omega) == \
+        (10.0, 0, 0.67, 0.05)
+    assert (c.use_coarse_correction, c.coarse_correction_iterations, c.coarse_correction_sweeps) == (0, 32, 1)
+    assert (c.coarse_correction_relative_tolerance, c.coarse_correction_min_scale) == (1e-3, 1.0 / 64.0)
+    assert (c.use_coarse_preconditioner, c.coarse_preconditioner_iterations,
+            c.coarse_preconditioner_relative_tolerance, c.coarse_preconditioner_scale) == (0, 8, 1e-2, 1.0)
+    assert (c.coarse_preconditioner_auto_disable, c.coarse_preconditioner_auto_disable_after) == (0, 4)
+    with pytest.raises(TypeError):
+        G.mr_solve_cfg(no_such_option=1)

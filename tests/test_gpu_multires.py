@@ -10,7 +10,7 @@ import pytest
 
 import pfflip_amd as G
 import pyoracle as O
-from _helpers import rel_err
+from _helpers import COARSE_CASES, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -168,3 +168,71 @@ def test_refinement_box_hysteresis_and_budget():
         assert np.array_equal(ctx.layout(), g.leaves())
         assert (fl, bl) == (osim.dynamic_last_fine_leaves, osim.budget_limited)
         assert bl == (1 if max_fine else 0)
+
+
+def _shared_system(n, filled):
+    """the same multires system on the GPU and in the FP64 oracle: the GPU's FP32
+    face fields after P2G, gravity and walls are copied into the oracle grid"""
+    ph, gph = _phase()
+    leaves = O.mr_layout_bubble_band(n, n, n)
+    parts = list(_bubble_particles(n))
+    if filled:
+        pts = np.array([(i + 0.5, j + 0.5, k + 0.5) for k in range(1, n - 1) for j in range(1, n - 1)
+                        for i in range(1, n - 1)])
+        rng = np.random.default_rng(5)
+        parts = [pts[:, 0].astype(np.float32), pts[:, 1].astype(np.float32), pts[:, 2].astype(np.float32)] + \
+            [rng.normal(scale=0.3, size=len(pts)).astype(np.float32) for _ in range(3)] + \
+            [np.zeros(len(pts), np.uint8)]
+    ctx, g, ps = _ctx(n, leaves, parts)
+    ctx.mr_mark_cells(); ctx.mr_p2g(gph, 1.0); ctx.apply_gravity(0.02, -9.81); ctx.apply_walls()
+    g.mark_cells(ps)
+    for f in range(6):
+        a = ctx.field(f)
+        for k, j, i in zip(*np.nonzero(a)):
+            g.set(f, int(i), int(j), int(k), float(a[k, j, i]))
+    return ctx, g, ph, gph
+
+
+@pytest.mark.parametrize("filled", [False, True])
+@pytest.mark.parametrize("case", range(7))
+def test_reference_order_solve_with_options_matches_oracle(filled, case):
+    # PF_MR_SOLVE_REFERENCE: the reference's FP64 iteration with its coarse
+    # correction, coarse preconditioner, flexible beta and restarts on the GPU
+    # against the oracle (itself bit-exact with the reference build) on the same
+    # system; reductions run in another order, so counts may differ by a step
+    opts = dict(COARSE_CASES[case])
+    flexible = opts.pop("flexible", 0)
+    n = 16
+    ctx, g, ph, gph = _shared_system(n, filled)
+    ocfg = O.mr_solve_cfg(max_iterations=400, flexible=flexible, **opts)
+    ost = g.project(ph, 0.02, ocfg)
+    gcfg = G.mr_solve_cfg(400, 1e-7, 1e-6, mode=G.MR_SOLVE_REFERENCE, use_flexible_cg_beta=flexible, **opts)
+    st = ctx.mr_project(gph, 0.02, gcfg)
+    d = ctx.mr_solve_detail()
+    print(case, filled, st.iterations, ost.iterations, d.coarse_correction_sweeps, ost.cc_sweeps,
+          d.coarse_preconditioner_applications, ost.cp_applications, flush=True)
+    assert d.reference_order == 1
+    assert st.converged == ost.converged == 1
+    assert abs(st.iterations - ost.iterations) <= 2
+    assert st.true_max_residual < ost.effective_tolerance
+    assert d.coarse_correction_cells == ost.cc_cells and d.coarse_preconditioner_cells == ost.cp_cells
+    assert abs(d.coarse_correction_sweeps - ost.cc_sweeps) <= 1
+    assert abs(d.coarse_preconditioner_applications - ost.cp_applications) <= 2
+    assert abs(d.coarse_preconditioner_skipped_applications - ost.cp_skipped) <= 2
+    assert d.coarse_preconditioner_auto_disabled == ost.cp_auto_disabled
+    assert rel_err(ctx.field(G.FIELD_P), g.dense(O.P)) < 1e-6
+    for f in range(3):
+        assert rel_err(ctx.field(f), g.dense(f)) < 1e-5, f
+
+
+def test_reference_order_rejects_invalid_options():
+    # validateSolveConfig (multires_pressure3d.cpp:307-333)
+    n = 16
+    ctx, g, ph, gph = _shared_system(n, False)
+    for bad in (dict(coarse_correction_min_scale=0.0), dict(coarse_preconditioner_scale=-1.0),
+                dict(relaxation_sweeps=-1)):
+        with pytest.raises(G.PfError):
+            ctx.mr_project(gph, 0.02, G.mr_solve_cfg(mode=G.MR_SOLVE_REFERENCE, **bad))
+    # the fast mode ignores the option fields
+    st = ctx.mr_project(gph, 0.02, G.mr_solve_cfg(coarse_correction_min_scale=0.0))
+    assert st.converged == 1 and ctx.mr_solve_detail().reference_order == 0

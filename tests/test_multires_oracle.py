@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import pyoracle as O
+from _helpers import COARSE_CASES
 
 needs_ref = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
 
@@ -322,17 +323,6 @@ def _coarse_case(n, filled):
     return r, g, ph
 
 
-COARSE_CASES = [
-    dict(use_coarse_correction=1),
-    dict(use_coarse_correction=1, coarse_correction_sweeps=3, coarse_correction_iterations=6),
-    dict(use_coarse_correction=1, coarse_correction_sweeps=2, coarse_correction_min_scale=0.25,
-         coarse_correction_relative_tolerance=0.0, coarse_correction_iterations=200),
-    dict(use_coarse_preconditioner=1),
-    dict(use_coarse_preconditioner=1, coarse_preconditioner_min_rz_gain=0.5, coarse_preconditioner_auto_disable=1,
-         coarse_preconditioner_auto_disable_after=2),
-    dict(use_coarse_preconditioner=1, coarse_preconditioner_max_work_ratio=1.5, coarse_preconditioner_iterations=4),
-    dict(use_coarse_preconditioner=1, coarse_preconditioner_scale=0.5, use_coarse_correction=1, flexible=1),
-]
 
 
 @needs_ref
