@@ -70,7 +70,7 @@ int fail(pf_ctx* c, int code, const std::string& m) {
 
 // per-kernel event bracket (active only while pf_kernel_timing is enabled)
 long long kbegin(pf_ctx* c, int kid) {
-  if (!c->ktime) return -1;
+  if (!((c->ktime >> kid) & 1u)) return -1;
   while (c->kev.size() < c->kev_used + 2) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return -1;
@@ -1371,6 +1371,8 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     // the two iteration kernels use at most 1024 workgroups, so the per-iteration
     // scalar kernels sum few partials
     const unsigned GS = std::min(G, 1024u);
+    // the buffer-addressed SpMV takes 32-bit byte offsets: arrays below 2 GiB
+    const bool spmv_buffers = c->dc.nblocks * kBlockVol * 4 < (1LL << 31);
     VecArgs vs = va;
     int iter_before = 0, iter_base = 0;
     double best_rr = rr, prev_rr = rr;
@@ -1381,10 +1383,14 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
         if (mr)
           TIMED(c, PF_K_PCG_SPMV,
                 hipLaunchKernelGGL(k_mr_spmv, dim3(GS), dim3(256), 0, c->stream, vs, mop, c->z32, c->pA, c->pB, c->Ap));
-        else
+        else if (spmv_buffers)
         TIMED(c, PF_K_PCG_SPMV,
               hipLaunchKernelGGL(k_pcg_spmv, dim3(GS), dim3(256), 0, c->stream, vs, c->z32, c->pA, c->pB, c->Ap,
                                  c->cd32, c->cx32, c->cy32, c->cz32));
+        else
+        TIMED(c, PF_K_PCG_SPMV,
+              hipLaunchKernelGGL(k_pcg_spmv_gather, dim3(GS), dim3(256), 0, c->stream, vs, c->z32, c->pA, c->pB,
+                                 c->Ap, c->cd32, c->cx32, c->cy32, c->cz32));
         hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, c->sc);
         if (learned) {
           TIMED(c, PF_K_PCG_UPDATE,
@@ -1621,13 +1627,17 @@ const char* pf_last_error(const pf_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 
 void* pf_stream(pf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-int pf_kernel_timing(pf_ctx* c, int32_t enable) {
+int pf_kernel_timing_mask(pf_ctx* c, uint32_t mask) {
   if (!c) return PF_ERR_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  c->ktime = enable != 0;
+  c->ktime = mask & ((1u << PF_NKERNELS) - 1u);
   c->krec.clear();
   c->kev_used = 0;
   return PF_OK;
+}
+
+int pf_kernel_timing(pf_ctx* c, int32_t enable) {
+  return pf_kernel_timing_mask(c, enable ? 0xffffffffu : 0u);
 }
 
 int pf_kernel_time(pf_ctx* c, int32_t kernel, int64_t* launches, double* total_ms) {

@@ -211,7 +211,7 @@ struct Ctx {
   long long* mr_pin_host = nullptr;
 
   // per-kernel event timing (measurement hook)
-  bool ktime = false;
+  uint32_t ktime = 0;                // kernels (bit PF_K_*) bracketed with events
   std::vector<hipEvent_t> kev;
   size_t kev_used = 0;
   std::vector<std::pair<int, size_t>> krec;  // (kernel id, index of its start event)

@@ -145,6 +145,18 @@ __global__ void __launch_bounds__(256) k_reduce_slots(const double* part, int n,
 
 // ------------------------------------------------------------ set-up kernels
 
+// a value every lane of the wave holds, moved to a scalar register (the list
+// position and block id below depend only on the wave index)
+__device__ inline uint32_t wave_uniform_u32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// list[t] for a wave-uniform t through the constant address space (the lists are not
+// written while a kernel walks them): a scalar load, which leaves the vector-memory
+// counter -- and the previous block's stores -- alone
+__device__ inline uint32_t list_at(const uint32_t* list, uint32_t t) {
+  typedef const __attribute__((address_space(4))) uint32_t* cptr;
+  return ((cptr)(uintptr_t)list)[t];
+}
+
 // Iterate the block list with a fixed block -> workgroup assignment (so partial
 // sums are deterministic) that is XCD-aware: workgroups are dispatched round-robin
 // over the 8 XCDs, so workgroup w runs on XCD w % 8; each XCD gets one contiguous
@@ -155,7 +167,7 @@ __global__ void __launch_bounds__(256) k_reduce_slots(const double* part, int n,
                  _e = min((long long)(A).nlist, (_x + 1) * _c),                                       \
                  _t = _x * _c + (long long)(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);                \
        _t < _e; _t += _g * 4)                                                                         \
-    for (long long b = (A).list[_t], _once = 1; _once; _once = 0)
+    for (long long b = list_at((A).list, wave_uniform_u32((uint32_t)_t)), _once = 1; _once; _once = 0)
 
 struct ProjArgs {
   Geom g;
@@ -454,11 +466,12 @@ __global__ void __launch_bounds__(256) k_pcg_resume(const double* part, int n, S
 }
 
 // p_new = z + beta p_old (own cell and the six neighbours, recomputed on the fly),
-// Ap = A p_new (FP32), partial p.Ap in FP64
-__global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __restrict__ z, float* __restrict__ pA,
-                                                  float* __restrict__ pB, float* __restrict__ Ap,
-                                                  const float* __restrict__ cd, const float* __restrict__ cx,
-                                                  const float* __restrict__ cy, const float* __restrict__ cz) {
+// Ap = A p_new (FP32), partial p.Ap in FP64 -- for arrays of 2 GiB and more
+__global__ void __launch_bounds__(256) k_pcg_spmv_gather(VecArgs a, const float* __restrict__ z,
+                                                         float* __restrict__ pA, float* __restrict__ pB,
+                                                         float* __restrict__ Ap, const float* __restrict__ cd,
+                                                         const float* __restrict__ cx, const float* __restrict__ cy,
+                                                         const float* __restrict__ cz) {
   if (a.sc->done) return;
   const int lane = threadIdx.x & 63;
   const bool odd = a.sc->iter & 1;
@@ -484,6 +497,131 @@ __global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __rest
     pnew[c] = pc;
     Ap[c] = ap;
     pap += (double)pc * (double)ap;
+  }
+  double v[1] = {pap};
+  const bool mx[1] = {false};
+  block_reduce_store<1>(v, mx, a.part, kMaxPartials);
+}
+
+// raw buffer access over a whole array of fewer than 2^31 bytes: 32-bit byte offsets
+// from a scalar descriptor; an offset at or past the array's size reads 0 and touches
+// no memory (the masked loads below use kNoLoad)
+constexpr uint32_t kNoLoad = 0x80000000u;
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ inline void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32((int)__float_as_uint(v), r, off, 0, 0);
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+struct F4 {
+  float v[4];
+};
+__device__ inline F4 buf_ld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  F4 f;
+  f.v[0] = __uint_as_float(q.x);
+  f.v[1] = __uint_as_float(q.y);
+  f.v[2] = __uint_as_float(q.z);
+  f.v[3] = __uint_as_float(q.w);
+  return f;
+}
+__device__ inline void buf_st4(__amdgpu_buffer_rsrc_t r, uint32_t off, const float* v) {
+  u32x4 q;
+  q.x = __float_as_uint(v[0]);
+  q.y = __float_as_uint(v[1]);
+  q.z = __float_as_uint(v[2]);
+  q.w = __float_as_uint(v[3]);
+  __builtin_amdgcn_raw_buffer_store_b128(q, r, off, 0, 0);
+}
+
+// the same product with arrays below 2 GiB, by cell rows: a lane holds one x-row of
+// 4 cells (row = ly + 4 lz, 16 lanes per 4^3 block) and a wave four consecutive list
+// blocks.  x-neighbours are in the lane's registers; y- and z-neighbour rows come
+// from lanes +-1 and +-4 of the same block; only rows on a block face read the
+// neighbouring block (other lanes issue no memory access there).  At the domain
+// boundary the neighbour is the cell itself, whose term vanishes, as in
+// neighbours().  Per cell the arithmetic is k_pcg_spmv_gather's, in the same order.
+__global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __restrict__ z, float* __restrict__ pA,
+                                                  float* __restrict__ pB, float* __restrict__ Ap,
+                                                  const float* __restrict__ cd, const float* __restrict__ cx,
+                                                  const float* __restrict__ cy, const float* __restrict__ cz) {
+  if (a.sc->done) return;
+  const int lane = threadIdx.x & 63;
+  const int qb = lane >> 4, row = lane & 15, ly = row & 3, lz = row >> 2;
+  const bool odd = a.sc->iter & 1;
+  const uint32_t bytes = (uint32_t)(a.dc.nblocks * kBlockVol * 4);
+  const __amdgpu_buffer_rsrc_t rz = buf_rsrc(z, bytes), rpo = buf_rsrc(odd ? pB : pA, bytes);
+  const __amdgpu_buffer_rsrc_t rpn = buf_rsrc(odd ? pA : pB, bytes), rap = buf_rsrc(Ap, bytes);
+  const __amdgpu_buffer_rsrc_t rcd = buf_rsrc(cd, bytes), rcx = buf_rsrc(cx, bytes);
+  const __amdgpu_buffer_rsrc_t rcy = buf_rsrc(cy, bytes), rcz = buf_rsrc(cz, bytes);
+  const float beta = (float)a.sc->beta;
+  const uint32_t sy = (uint32_t)a.dc.nbx * kBlockVol * 4, sz = (uint32_t)a.dc.nbx * a.dc.nby * kBlockVol * 4;
+  double pap = 0.0;
+  // the XCD segments of FOR_LIST_BLOCKS, walked in slots of four list entries
+  const long long seg = (a.nlist + 7) / 8, xcd = blockIdx.x & 7;
+  const long long s0 = xcd * seg, e = min((long long)a.nlist, (xcd + 1) * seg);
+  const long long nslot = e > s0 ? (e - s0 + 3) / 4 : 0, step = (long long)(gridDim.x >> 3) * 4;
+  for (long long sl = (long long)(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6); sl < nslot; sl += step) {
+    const uint32_t t0 = wave_uniform_u32((uint32_t)(s0 + sl * 4));
+    const uint32_t last = (uint32_t)(e - 1);
+    const uint32_t b0 = list_at(a.list, t0), b1 = list_at(a.list, min(t0 + 1, last));
+    const uint32_t b2 = list_at(a.list, min(t0 + 2, last)), b3 = list_at(a.list, min(t0 + 3, last));
+    const bool valid = (long long)t0 + qb < e;
+    const uint32_t b = qb == 0 ? b0 : (qb == 1 ? b1 : (qb == 2 ? b2 : b3));
+    int bx, by, bz;
+    block_coords(a.dc, b, bx, by, bz);
+    const uint32_t o = valid ? (b * kBlockVol + 4 * row) * 4 : kNoLoad;
+    const bool fxp = valid && bx + 1 < a.dc.nbx, fxm = valid && bx > 0;
+    const bool fyp = valid && ly == 3 && by + 1 < a.dc.nby, fym = valid && ly == 0 && by > 0;
+    const bool fzp = valid && lz == 3 && bz + 1 < a.dc.nbz, fzm = valid && lz == 0 && bz > 0;
+    const uint32_t oxp = fxp ? o + kBlockVol * 4 : kNoLoad, oxm = fxm ? o - kBlockVol * 4 + 12 : kNoLoad;
+    const uint32_t oyp = fyp ? o + sy - 48 : kNoLoad, oym = fym ? o - sy + 48 : kNoLoad;
+    const uint32_t ozp = fzp ? o + sz - 192 : kNoLoad, ozm = fzm ? o - sz + 192 : kNoLoad;
+    const F4 po = buf_ld4(rpo, o), zo = buf_ld4(rz, o), cdv = buf_ld4(rcd, o);
+    const F4 cxv = buf_ld4(rcx, o), cyv = buf_ld4(rcy, o), czv = buf_ld4(rcz, o);
+    const float poxp = buf_ld(rpo, oxp), zxp = buf_ld(rz, oxp);
+    const float poxm = buf_ld(rpo, oxm), zxm = buf_ld(rz, oxm), cxxm = buf_ld(rcx, oxm);
+    const F4 poyp = buf_ld4(rpo, oyp), zyp = buf_ld4(rz, oyp);
+    const F4 poym = buf_ld4(rpo, oym), zym = buf_ld4(rz, oym), cyym = buf_ld4(rcy, oym);
+    const F4 pozp = buf_ld4(rpo, ozp), zzp = buf_ld4(rz, ozp);
+    const F4 pozm = buf_ld4(rpo, ozm), zzm = buf_ld4(rz, ozm), czzm = buf_ld4(rcz, ozm);
+    // non-FLUID cells: z = p = 0 and zero coefficients give p_new = Ap = 0
+    float pc[4], ap[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pc[i] = fmaf(beta, po.v[i], zo.v[i]);
+    const float xpn = fmaf(beta, poxp, zxp), xmn = fmaf(beta, poxm, zxm);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float syp = __shfl(pc[i], lane + 1, 64), sym = __shfl(pc[i], lane - 1, 64);
+      const float szp = __shfl(pc[i], lane + 4, 64), szm = __shfl(pc[i], lane - 4, 64);
+      const float scy = __shfl(cyv.v[i], lane - 1, 64), scz = __shfl(czv.v[i], lane - 4, 64);
+      const float pxp = i < 3 ? pc[i + 1] : (fxp ? xpn : pc[3]);
+      const float pxm = i > 0 ? pc[i - 1] : (fxm ? xmn : pc[0]);
+      const float cxm = i > 0 ? cxv.v[i - 1] : (fxm ? cxxm : 0.f);
+      const float pyp = ly < 3 ? syp : (fyp ? fmaf(beta, poyp.v[i], zyp.v[i]) : pc[i]);
+      const float pym = ly > 0 ? sym : (fym ? fmaf(beta, poym.v[i], zym.v[i]) : pc[i]);
+      const float cym = ly > 0 ? scy : (fym ? cyym.v[i] : 0.f);
+      const float pzp = lz < 3 ? szp : (fzp ? fmaf(beta, pozp.v[i], zzp.v[i]) : pc[i]);
+      const float pzm = lz > 0 ? szm : (fzm ? fmaf(beta, pozm.v[i], zzm.v[i]) : pc[i]);
+      const float czm = lz > 0 ? scz : (fzm ? czzm.v[i] : 0.f);
+      float s = cdv.v[i] * pc[i];
+      s = fmaf(cxv.v[i], pc[i] - pxp, s);
+      s = fmaf(cxm, pc[i] - pxm, s);
+      s = fmaf(cyv.v[i], pc[i] - pyp, s);
+      s = fmaf(cym, pc[i] - pym, s);
+      s = fmaf(czv.v[i], pc[i] - pzp, s);
+      s = fmaf(czm, pc[i] - pzm, s);
+      ap[i] = s;
+    }
+    buf_st4(rpn, o, pc);
+    buf_st4(rap, o, ap);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pap += (double)pc[i] * (double)ap[i];
   }
   double v[1] = {pap};
   const bool mx[1] = {false};
@@ -543,14 +681,30 @@ __global__ void __launch_bounds__(256) k_pcg_update(VecArgs a, float* __restrict
   const float* __restrict__ p = (a.sc->iter & 1) ? pA : pB;  // the p_new of this iteration
   const float alpha = (float)a.sc->alpha;
   double rz = 0.0, rr = 0.0;
-  FOR_LIST_BLOCKS(a, b, lane) {
-    long long c = b * kBlockVol + lane;
-    float xn = fmaf(alpha, p[c], x[c]);
-    float rn = fmaf(-alpha, Ap[c], r[c]);
-    float zn = invd[c] * rn;
-    x[c] = xn; r[c] = rn; z[c] = zn;
-    rz += (double)rn * (double)zn;
-    rr += (double)rn * (double)rn;
+  // the blocks of FOR_LIST_BLOCKS in the same order, software-pipelined: the next
+  // block's loads are issued before this block's stores, so waiting for them does
+  // not wait for the stores (the vector-memory counter completes in issue order)
+  const long long seg = (a.nlist + 7) / 8, xcd = blockIdx.x & 7, step = (long long)(gridDim.x >> 3) * 4;
+  const long long end = min((long long)a.nlist, (xcd + 1) * seg);
+  long long t = xcd * seg + (long long)(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);
+  if (t < end) {
+    long long c = (long long)list_at(a.list, wave_uniform_u32((uint32_t)t)) * kBlockVol + lane;
+    float pv = p[c], xv = x[c], av = Ap[c], rv = r[c], dv = invd[c];
+    for (;;) {
+      const long long tn = t + step;
+      const bool more = tn < end;
+      const long long cn = more ? (long long)list_at(a.list, wave_uniform_u32((uint32_t)tn)) * kBlockVol + lane : c;
+      const float pn = p[cn], xn0 = x[cn], an = Ap[cn], rn0 = r[cn], dn = invd[cn];
+      const float xn = fmaf(alpha, pv, xv);
+      const float rn = fmaf(-alpha, av, rv);
+      const float zn = dv * rn;
+      x[c] = xn; r[c] = rn; z[c] = zn;
+      rz += (double)rn * (double)zn;
+      rr += (double)rn * (double)rn;
+      if (!more) break;
+      t = tn; c = cn;
+      pv = pn; xv = xn0; av = an; rv = rn0; dv = dn;
+    }
   }
   double v[2] = {rz, rr};
   const bool mx[2] = {false, false};

@@ -221,6 +221,7 @@ def lib():
             "pf_lpc_train": (C.c_int, [vp, vp, i32, dbl, C.c_uint64, C.POINTER(dbl)]),
             "pf_lpc_loss": (C.c_int, [vp, vp, i32, C.POINTER(dbl), C.POINTER(dbl)]),
             "pf_kernel_timing": (C.c_int, [vp, i32]),
+            "pf_kernel_timing_mask": (C.c_int, [vp, C.c_uint32]),
             "pf_kernel_time": (C.c_int, [vp, i32, C.POINTER(i64), C.POINTER(dbl)]),
             "pf_copy_probe": (C.c_int, [vp, i64, C.POINTER(dbl)]),
         }
@@ -358,8 +359,15 @@ class Context:
         self._check(lib().pf_synchronize(self.h), "pf_synchronize")
 
     # measurement hooks
-    def kernel_timing(self, enable: bool):
-        self._check(lib().pf_kernel_timing(self.h, int(enable)), "pf_kernel_timing")
+    def kernel_timing(self, enable: bool, kernels=None):
+        """event-time every kernel (enable) or only the named ones (KERNELS names)"""
+        if kernels is None:
+            self._check(lib().pf_kernel_timing(self.h, int(enable)), "pf_kernel_timing")
+            return
+        mask = 0
+        for k in kernels:
+            mask |= 1 << KERNELS.index(k)
+        self._check(lib().pf_kernel_timing_mask(self.h, mask if enable else 0), "pf_kernel_timing_mask")
 
     def kernel_time(self, kernel: int):
         """(launches, summed ms) of one PF_K_* kernel since timing was enabled."""

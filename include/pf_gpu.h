@@ -215,6 +215,8 @@ void* pf_stream(pf_ctx* ctx);
 /* enable != 0: bracket every launch of the kernels above with HIP events on the
  * library's stream (counters reset); 0: stop. */
 int pf_kernel_timing(pf_ctx* ctx, int32_t enable);
+/* the same for the kernels whose bit (1 << PF_K_*) is set in mask (0: stop) */
+int pf_kernel_timing_mask(pf_ctx* ctx, uint32_t mask);
 /* launches and summed event time of one kernel since timing was enabled (synchronises) */
 int pf_kernel_time(pf_ctx* ctx, int32_t kernel, int64_t* launches, double* total_ms);
 /* plain device copy of n floats (n a multiple of 4) between two buffers allocated
