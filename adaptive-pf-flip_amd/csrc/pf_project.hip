@@ -539,6 +539,58 @@ __device__ inline void buf_st4(__amdgpu_buffer_rsrc_t r, uint32_t off, const flo
   __builtin_amdgcn_raw_buffer_store_b128(q, r, off, 0, 0);
 }
 
+// one lane's inputs for a cell row (below): own row, x-face cells, y/z-face rows
+struct RowLoads {
+  uint32_t o;
+  bool fxp, fxm, fyp, fym, fzp, fzm;
+  F4 po, zo, cd, cx, cy, cz;
+  float poxp, zxp, poxm, zxm, cxxm;
+  F4 poyp, zyp, poym, zym, cyym, pozp, zzp, pozm, zzm, czzm;
+};
+
+struct RowBufs {
+  __amdgpu_buffer_rsrc_t z, po, pn, ap, cd, cx, cy, cz;
+  uint32_t sy, sz;
+};
+
+__device__ inline void row_load(const RowBufs& B, const Dims& dc, uint32_t b, bool valid, int row, RowLoads& L) {
+  const int ly = row & 3, lz = row >> 2;
+  int bx, by, bz;
+  block_coords(dc, b, bx, by, bz);
+  const uint32_t o = valid ? (b * kBlockVol + 4 * row) * 4 : kNoLoad;
+  L.o = o;
+  L.fxp = valid && bx + 1 < dc.nbx;
+  L.fxm = valid && bx > 0;
+  L.fyp = valid && ly == 3 && by + 1 < dc.nby;
+  L.fym = valid && ly == 0 && by > 0;
+  L.fzp = valid && lz == 3 && bz + 1 < dc.nbz;
+  L.fzm = valid && lz == 0 && bz > 0;
+  const uint32_t oxp = L.fxp ? o + kBlockVol * 4 : kNoLoad, oxm = L.fxm ? o - kBlockVol * 4 + 12 : kNoLoad;
+  const uint32_t oyp = L.fyp ? o + B.sy - 48 : kNoLoad, oym = L.fym ? o - B.sy + 48 : kNoLoad;
+  const uint32_t ozp = L.fzp ? o + B.sz - 192 : kNoLoad, ozm = L.fzm ? o - B.sz + 192 : kNoLoad;
+  L.po = buf_ld4(B.po, o);
+  L.zo = buf_ld4(B.z, o);
+  L.cd = buf_ld4(B.cd, o);
+  L.cx = buf_ld4(B.cx, o);
+  L.cy = buf_ld4(B.cy, o);
+  L.cz = buf_ld4(B.cz, o);
+  L.poxp = buf_ld(B.po, oxp);
+  L.zxp = buf_ld(B.z, oxp);
+  L.poxm = buf_ld(B.po, oxm);
+  L.zxm = buf_ld(B.z, oxm);
+  L.cxxm = buf_ld(B.cx, oxm);
+  L.poyp = buf_ld4(B.po, oyp);
+  L.zyp = buf_ld4(B.z, oyp);
+  L.poym = buf_ld4(B.po, oym);
+  L.zym = buf_ld4(B.z, oym);
+  L.cyym = buf_ld4(B.cy, oym);
+  L.pozp = buf_ld4(B.po, ozp);
+  L.zzp = buf_ld4(B.z, ozp);
+  L.pozm = buf_ld4(B.po, ozm);
+  L.zzm = buf_ld4(B.z, ozm);
+  L.czzm = buf_ld4(B.cz, ozm);
+}
+
 // the same product with arrays below 2 GiB, by cell rows: a lane holds one x-row of
 // 4 cells (row = ly + 4 lz, 16 lanes per 4^3 block) and a wave four consecutive list
 // blocks.  x-neighbours are in the lane's registers; y- and z-neighbour rows come
@@ -546,6 +598,8 @@ __device__ inline void buf_st4(__amdgpu_buffer_rsrc_t r, uint32_t off, const flo
 // neighbouring block (other lanes issue no memory access there).  At the domain
 // boundary the neighbour is the cell itself, whose term vanishes, as in
 // neighbours().  Per cell the arithmetic is k_pcg_spmv_gather's, in the same order.
+// Software-pipelined: the next four blocks' loads are issued before this four's
+// stores, so waiting for them leaves the stores in flight.
 __global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __restrict__ z, float* __restrict__ pA,
                                                   float* __restrict__ pB, float* __restrict__ Ap,
                                                   const float* __restrict__ cd, const float* __restrict__ cx,
@@ -555,73 +609,80 @@ __global__ void __launch_bounds__(256) k_pcg_spmv(VecArgs a, const float* __rest
   const int qb = lane >> 4, row = lane & 15, ly = row & 3, lz = row >> 2;
   const bool odd = a.sc->iter & 1;
   const uint32_t bytes = (uint32_t)(a.dc.nblocks * kBlockVol * 4);
-  const __amdgpu_buffer_rsrc_t rz = buf_rsrc(z, bytes), rpo = buf_rsrc(odd ? pB : pA, bytes);
-  const __amdgpu_buffer_rsrc_t rpn = buf_rsrc(odd ? pA : pB, bytes), rap = buf_rsrc(Ap, bytes);
-  const __amdgpu_buffer_rsrc_t rcd = buf_rsrc(cd, bytes), rcx = buf_rsrc(cx, bytes);
-  const __amdgpu_buffer_rsrc_t rcy = buf_rsrc(cy, bytes), rcz = buf_rsrc(cz, bytes);
+  RowBufs B;
+  B.z = buf_rsrc(z, bytes);
+  B.po = buf_rsrc(odd ? pB : pA, bytes);
+  B.pn = buf_rsrc(odd ? pA : pB, bytes);
+  B.ap = buf_rsrc(Ap, bytes);
+  B.cd = buf_rsrc(cd, bytes);
+  B.cx = buf_rsrc(cx, bytes);
+  B.cy = buf_rsrc(cy, bytes);
+  B.cz = buf_rsrc(cz, bytes);
+  B.sy = (uint32_t)a.dc.nbx * kBlockVol * 4;
+  B.sz = (uint32_t)a.dc.nbx * a.dc.nby * kBlockVol * 4;
   const float beta = (float)a.sc->beta;
-  const uint32_t sy = (uint32_t)a.dc.nbx * kBlockVol * 4, sz = (uint32_t)a.dc.nbx * a.dc.nby * kBlockVol * 4;
   double pap = 0.0;
   // the XCD segments of FOR_LIST_BLOCKS, walked in slots of four list entries
   const long long seg = (a.nlist + 7) / 8, xcd = blockIdx.x & 7;
   const long long s0 = xcd * seg, e = min((long long)a.nlist, (xcd + 1) * seg);
   const long long nslot = e > s0 ? (e - s0 + 3) / 4 : 0, step = (long long)(gridDim.x >> 3) * 4;
-  for (long long sl = (long long)(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6); sl < nslot; sl += step) {
-    const uint32_t t0 = wave_uniform_u32((uint32_t)(s0 + sl * 4));
-    const uint32_t last = (uint32_t)(e - 1);
+  const uint32_t last = e > 0 ? (uint32_t)(e - 1) : 0;
+  // this lane's block of the slot starting at list position t0 (clamped in range)
+  auto slot_block = [&](uint32_t t0, bool& valid) -> uint32_t {
     const uint32_t b0 = list_at(a.list, t0), b1 = list_at(a.list, min(t0 + 1, last));
     const uint32_t b2 = list_at(a.list, min(t0 + 2, last)), b3 = list_at(a.list, min(t0 + 3, last));
-    const bool valid = (long long)t0 + qb < e;
-    const uint32_t b = qb == 0 ? b0 : (qb == 1 ? b1 : (qb == 2 ? b2 : b3));
-    int bx, by, bz;
-    block_coords(a.dc, b, bx, by, bz);
-    const uint32_t o = valid ? (b * kBlockVol + 4 * row) * 4 : kNoLoad;
-    const bool fxp = valid && bx + 1 < a.dc.nbx, fxm = valid && bx > 0;
-    const bool fyp = valid && ly == 3 && by + 1 < a.dc.nby, fym = valid && ly == 0 && by > 0;
-    const bool fzp = valid && lz == 3 && bz + 1 < a.dc.nbz, fzm = valid && lz == 0 && bz > 0;
-    const uint32_t oxp = fxp ? o + kBlockVol * 4 : kNoLoad, oxm = fxm ? o - kBlockVol * 4 + 12 : kNoLoad;
-    const uint32_t oyp = fyp ? o + sy - 48 : kNoLoad, oym = fym ? o - sy + 48 : kNoLoad;
-    const uint32_t ozp = fzp ? o + sz - 192 : kNoLoad, ozm = fzm ? o - sz + 192 : kNoLoad;
-    const F4 po = buf_ld4(rpo, o), zo = buf_ld4(rz, o), cdv = buf_ld4(rcd, o);
-    const F4 cxv = buf_ld4(rcx, o), cyv = buf_ld4(rcy, o), czv = buf_ld4(rcz, o);
-    const float poxp = buf_ld(rpo, oxp), zxp = buf_ld(rz, oxp);
-    const float poxm = buf_ld(rpo, oxm), zxm = buf_ld(rz, oxm), cxxm = buf_ld(rcx, oxm);
-    const F4 poyp = buf_ld4(rpo, oyp), zyp = buf_ld4(rz, oyp);
-    const F4 poym = buf_ld4(rpo, oym), zym = buf_ld4(rz, oym), cyym = buf_ld4(rcy, oym);
-    const F4 pozp = buf_ld4(rpo, ozp), zzp = buf_ld4(rz, ozp);
-    const F4 pozm = buf_ld4(rpo, ozm), zzm = buf_ld4(rz, ozm), czzm = buf_ld4(rcz, ozm);
-    // non-FLUID cells: z = p = 0 and zero coefficients give p_new = Ap = 0
-    float pc[4], ap[4];
+    valid = (long long)t0 + qb < e;
+    return qb == 0 ? b0 : (qb == 1 ? b1 : (qb == 2 ? b2 : b3));
+  };
+  long long sl = (long long)(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);
+  if (sl < nslot) {
+    bool valid;
+    uint32_t b = slot_block(wave_uniform_u32((uint32_t)(s0 + sl * 4)), valid);
+    RowLoads cur, nxt;
+    row_load(B, a.dc, b, valid, row, cur);
+    for (;;) {
+      const long long sn = sl + step;
+      const bool more = sn < nslot;
+      bool vn = false;
+      const uint32_t bn = more ? slot_block(wave_uniform_u32((uint32_t)(s0 + sn * 4)), vn) : b;
+      row_load(B, a.dc, bn, more && vn, row, nxt);
+      // non-FLUID cells: z = p = 0 and zero coefficients give p_new = Ap = 0
+      float pc[4], ap[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) pc[i] = fmaf(beta, po.v[i], zo.v[i]);
-    const float xpn = fmaf(beta, poxp, zxp), xmn = fmaf(beta, poxm, zxm);
+      for (int i = 0; i < 4; ++i) pc[i] = fmaf(beta, cur.po.v[i], cur.zo.v[i]);
+      const float xpn = fmaf(beta, cur.poxp, cur.zxp), xmn = fmaf(beta, cur.poxm, cur.zxm);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float syp = __shfl(pc[i], lane + 1, 64), sym = __shfl(pc[i], lane - 1, 64);
-      const float szp = __shfl(pc[i], lane + 4, 64), szm = __shfl(pc[i], lane - 4, 64);
-      const float scy = __shfl(cyv.v[i], lane - 1, 64), scz = __shfl(czv.v[i], lane - 4, 64);
-      const float pxp = i < 3 ? pc[i + 1] : (fxp ? xpn : pc[3]);
-      const float pxm = i > 0 ? pc[i - 1] : (fxm ? xmn : pc[0]);
-      const float cxm = i > 0 ? cxv.v[i - 1] : (fxm ? cxxm : 0.f);
-      const float pyp = ly < 3 ? syp : (fyp ? fmaf(beta, poyp.v[i], zyp.v[i]) : pc[i]);
-      const float pym = ly > 0 ? sym : (fym ? fmaf(beta, poym.v[i], zym.v[i]) : pc[i]);
-      const float cym = ly > 0 ? scy : (fym ? cyym.v[i] : 0.f);
-      const float pzp = lz < 3 ? szp : (fzp ? fmaf(beta, pozp.v[i], zzp.v[i]) : pc[i]);
-      const float pzm = lz > 0 ? szm : (fzm ? fmaf(beta, pozm.v[i], zzm.v[i]) : pc[i]);
-      const float czm = lz > 0 ? scz : (fzm ? czzm.v[i] : 0.f);
-      float s = cdv.v[i] * pc[i];
-      s = fmaf(cxv.v[i], pc[i] - pxp, s);
-      s = fmaf(cxm, pc[i] - pxm, s);
-      s = fmaf(cyv.v[i], pc[i] - pyp, s);
-      s = fmaf(cym, pc[i] - pym, s);
-      s = fmaf(czv.v[i], pc[i] - pzp, s);
-      s = fmaf(czm, pc[i] - pzm, s);
-      ap[i] = s;
+      for (int i = 0; i < 4; ++i) {
+        const float syp = __shfl(pc[i], lane + 1, 64), sym = __shfl(pc[i], lane - 1, 64);
+        const float szp = __shfl(pc[i], lane + 4, 64), szm = __shfl(pc[i], lane - 4, 64);
+        const float scy = __shfl(cur.cy.v[i], lane - 1, 64), scz = __shfl(cur.cz.v[i], lane - 4, 64);
+        const float pxp = i < 3 ? pc[i + 1] : (cur.fxp ? xpn : pc[3]);
+        const float pxm = i > 0 ? pc[i - 1] : (cur.fxm ? xmn : pc[0]);
+        const float cxm = i > 0 ? cur.cx.v[i - 1] : (cur.fxm ? cur.cxxm : 0.f);
+        const float pyp = ly < 3 ? syp : (cur.fyp ? fmaf(beta, cur.poyp.v[i], cur.zyp.v[i]) : pc[i]);
+        const float pym = ly > 0 ? sym : (cur.fym ? fmaf(beta, cur.poym.v[i], cur.zym.v[i]) : pc[i]);
+        const float cym = ly > 0 ? scy : (cur.fym ? cur.cyym.v[i] : 0.f);
+        const float pzp = lz < 3 ? szp : (cur.fzp ? fmaf(beta, cur.pozp.v[i], cur.zzp.v[i]) : pc[i]);
+        const float pzm = lz > 0 ? szm : (cur.fzm ? fmaf(beta, cur.pozm.v[i], cur.zzm.v[i]) : pc[i]);
+        const float czm = lz > 0 ? scz : (cur.fzm ? cur.czzm.v[i] : 0.f);
+        float sacc = cur.cd.v[i] * pc[i];
+        sacc = fmaf(cur.cx.v[i], pc[i] - pxp, sacc);
+        sacc = fmaf(cxm, pc[i] - pxm, sacc);
+        sacc = fmaf(cur.cy.v[i], pc[i] - pyp, sacc);
+        sacc = fmaf(cym, pc[i] - pym, sacc);
+        sacc = fmaf(cur.cz.v[i], pc[i] - pzp, sacc);
+        sacc = fmaf(czm, pc[i] - pzm, sacc);
+        ap[i] = sacc;
+      }
+      buf_st4(B.pn, cur.o, pc);
+      buf_st4(B.ap, cur.o, ap);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pap += (double)pc[i] * (double)ap[i];
+      if (!more) break;
+      sl = sn;
+      b = bn;
+      cur = nxt;
     }
-    buf_st4(rpn, o, pc);
-    buf_st4(rap, o, ap);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) pap += (double)pc[i] * (double)ap[i];
   }
   double v[1] = {pap};
   const bool mx[1] = {false};
