@@ -933,9 +933,13 @@ void orc_step(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u
   orc_mark_cells(g, n, x, y, z);
   double t2 = now_ms();
   orc_p2g(g, n, x, y, z, u, v, w, type, vol, &c->phase, c->Vp);
+  orc_interface_diagnostics(g, &c->phase, &S->iface);
   double t3 = now_ms();
   orc_grid* saved = orc_grid_clone(g);
   orc_apply_gravity(g, stepDt, c->gravity);
+  if (c->surface_tension)
+    orc_surface_tension(g, &c->phase, stepDt, c->surface_tension_strength, c->surface_tension_max_delta_speed,
+                        c->surface_tension_smoothing_radius, &S->surface);
   orc_apply_walls(g);
   double t4 = now_ms();
   orc_project(g, &c->phase, stepDt, c->cg_iters, c->cg_tol, S->c_div, c->stop_rule, &S->solve);
@@ -1040,4 +1044,208 @@ int64_t orc_scene_fill(int kind, int nx, int ny, int nz, double dx, uint64_t see
         }
       }
   return p;
+}
+
+/* ------------------------------------------------ interface diagnostics, CSF */
+/* src/driver/interface_diagnostics3d.cpp: cell phi as the mean of the six face phi
+ * (:137-148), central-difference gradient with clamped indices (:158-175), normal
+ * grad / |grad| (zero below 1e-12, :177-181), curvature as the divergence of the
+ * normals (:188-208), smoothing over a clamped (2r+1)^3 box of interface cells
+ * (:210-235), CSF velocity increment grad * (-strength * kappa * dt) limited to
+ * max_delta_speed and added half to each face of the cell (:404-440). */
+
+#define IFACE_PHI_LO 0.05
+#define IFACE_PHI_HI 0.95
+#define IFACE_GRAD_EPS 1e-5
+#define IFACE_NORMAL_EPS 1e-12
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static double cell_phi(const orc_grid* g, const orc_phase* pp, int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= g->nx || j >= g->ny || k >= g->nz) return 0.0;
+  double s = 0.0;
+  s += orc_phi(field_get(&g->f[ORC_MU], i, j, k), pp);
+  s += orc_phi(field_get(&g->f[ORC_MU], i + 1, j, k), pp);
+  s += orc_phi(field_get(&g->f[ORC_MV], i, j, k), pp);
+  s += orc_phi(field_get(&g->f[ORC_MV], i, j + 1, k), pp);
+  s += orc_phi(field_get(&g->f[ORC_MW], i, j, k), pp);
+  s += orc_phi(field_get(&g->f[ORC_MW], i, j, k + 1), pp);
+  return s / 6.0;
+}
+
+static double phi_clamped(const orc_grid* g, const orc_phase* pp, int i, int j, int k) {
+  return cell_phi(g, pp, clampi(i, 0, g->nx - 1), clampi(j, 0, g->ny - 1), clampi(k, 0, g->nz - 1));
+}
+
+static void iface_gradient(const orc_grid* g, const orc_phase* pp, int i, int j, int k, double out[3]) {
+  const int im = clampi(i - 1, 0, g->nx - 1), ip = clampi(i + 1, 0, g->nx - 1);
+  const int jm = clampi(j - 1, 0, g->ny - 1), jp = clampi(j + 1, 0, g->ny - 1);
+  const int km = clampi(k - 1, 0, g->nz - 1), kp = clampi(k + 1, 0, g->nz - 1);
+  const double dx = g->dx;
+  out[0] = (phi_clamped(g, pp, ip, j, k) - phi_clamped(g, pp, im, j, k)) / ((ip - im > 1 ? ip - im : 1) * dx);
+  out[1] = (phi_clamped(g, pp, i, jp, k) - phi_clamped(g, pp, i, jm, k)) / ((jp - jm > 1 ? jp - jm : 1) * dx);
+  out[2] = (phi_clamped(g, pp, i, j, kp) - phi_clamped(g, pp, i, j, km)) / ((kp - km > 1 ? kp - km : 1) * dx);
+}
+
+static double vlen(const double v[3]) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+static void iface_normal(const orc_grid* g, const orc_phase* pp, int i, int j, int k, double n[3]) {
+  double gr[3];
+  iface_gradient(g, pp, i, j, k, gr);
+  const double mag = vlen(gr);
+  if (mag <= IFACE_NORMAL_EPS || !isfinite(mag)) { n[0] = n[1] = n[2] = 0.0; return; }
+  const double inv = 1.0 / mag;
+  n[0] = gr[0] * inv; n[1] = gr[1] * inv; n[2] = gr[2] * inv;
+}
+
+static double iface_curvature(const orc_grid* g, const orc_phase* pp, int i, int j, int k) {
+  const int im = clampi(i - 1, 0, g->nx - 1), ip = clampi(i + 1, 0, g->nx - 1);
+  const int jm = clampi(j - 1, 0, g->ny - 1), jp = clampi(j + 1, 0, g->ny - 1);
+  const int km = clampi(k - 1, 0, g->nz - 1), kp = clampi(k + 1, 0, g->nz - 1);
+  const double dx = g->dx;
+  double nxm[3], nxp[3], nym[3], nyp[3], nzm[3], nzp[3];
+  iface_normal(g, pp, im, j, k, nxm); iface_normal(g, pp, ip, j, k, nxp);
+  iface_normal(g, pp, i, jm, k, nym); iface_normal(g, pp, i, jp, k, nyp);
+  iface_normal(g, pp, i, j, km, nzm); iface_normal(g, pp, i, j, kp, nzp);
+  return (nxp[0] - nxm[0]) / ((ip - im > 1 ? ip - im : 1) * dx) +
+         (nyp[1] - nym[1]) / ((jp - jm > 1 ? jp - jm : 1) * dx) +
+         (nzp[2] - nzm[2]) / ((kp - km > 1 ? kp - km : 1) * dx);
+}
+
+static int iface_phi(double phi) { return phi > IFACE_PHI_LO && phi < IFACE_PHI_HI; }
+
+void orc_interface_fields(const orc_grid* g, const orc_phase* pp, double* phi, double* grad, double* curvature) {
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i) {
+        const int64_t c = i + (int64_t)g->nx * (j + (int64_t)g->ny * k);
+        double gr[3];
+        iface_gradient(g, pp, i, j, k, gr);
+        phi[c] = cell_phi(g, pp, i, j, k);
+        grad[c] = vlen(gr);
+        curvature[c] = iface_curvature(g, pp, i, j, k);
+      }
+}
+
+/* diagnoseSparseInterface3D, :362-379, over the FLUID cells in linear order */
+void orc_interface_diagnostics(const orc_grid* g, const orc_phase* pp, orc_iface_stats* S) {
+  memset(S, 0, sizeof(*S));
+  S->finite = 1;
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i) {
+        if ((int)(field_get(&g->f[ORC_MK], i, j, k) + 0.5) != ORC_FLUID) continue;
+        const double phi = cell_phi(g, pp, i, j, k);
+        double gr[3];
+        iface_gradient(g, pp, i, j, k, gr);
+        const double gm = vlen(gr), kap = iface_curvature(g, pp, i, j, k);
+        /* accumulate, :26-53 */
+        if (!isfinite(phi) || !isfinite(gm) || !isfinite(kap)) { S->finite = 0; continue; }
+        if (S->sample_cells == 0) { S->phi_min = phi; S->phi_max = phi; }
+        else { S->phi_min = fmin(S->phi_min, phi); S->phi_max = fmax(S->phi_max, phi); }
+        S->sample_cells++;
+        S->phi_mean += phi;
+        if (!iface_phi(phi)) continue;
+        S->interface_cells++;
+        S->grad_mean += gm;
+        S->grad_max = fmax(S->grad_max, gm);
+        S->curvature_abs_mean += fabs(kap);
+        S->curvature_abs_max = fmax(S->curvature_abs_max, fabs(kap));
+      }
+  /* finishStats, :56-66 */
+  if (S->sample_cells > 0) S->phi_mean /= (double)S->sample_cells;
+  if (S->interface_cells > 0) {
+    S->grad_mean /= (double)S->interface_cells;
+    S->curvature_abs_mean /= (double)S->interface_cells;
+  }
+  S->surface_tension_candidate = S->finite && S->interface_cells > 0 && S->grad_max > IFACE_GRAD_EPS;
+}
+
+/* `float += (float)x` (faithful) or a double add on face field f */
+static void face_add(orc_grid* g, int f, int i, int j, int k, double add) {
+  double* r = field_ref(&g->f[f], i, j, k);
+  *r = acc(g, *r, add);
+}
+
+/* applySparseSurfaceTension3D, :404-440 */
+void orc_surface_tension(orc_grid* g, const orc_phase* pp, double dt, double strength, double maxd, int radius,
+                         orc_st_stats* S) {
+  memset(S, 0, sizeof(*S));
+  S->finite = 1;
+  S->capillary_stable = 1;
+  /* initSurfaceStats, :87-102 */
+  S->enabled = strength > 0.0 && dt > 0.0;
+  S->strength = strength;
+  S->max_delta_speed_limit = maxd;
+  S->curvature_smoothing_radius = clampi(radius, 0, 3);
+  {
+    const double rmin = pp->rho_l < pp->rho_g ? pp->rho_l : pp->rho_g;
+    const double pi = 3.14159265358979323846;
+    S->capillary_dt_limit = (g->dx <= 0.0 || strength <= 0.0 || rmin <= 0.0 || !isfinite(rmin))
+                                ? 0.0 : sqrt((rmin * g->dx * g->dx * g->dx) / (2.0 * pi * strength));
+  }
+  S->capillary_stable = !S->enabled || S->capillary_dt_limit <= 0.0 || dt <= S->capillary_dt_limit + 1e-12;
+  if (!S->enabled) return;
+  const int r = S->curvature_smoothing_radius;
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i) {
+        if ((int)(field_get(&g->f[ORC_MK], i, j, k) + 0.5) != ORC_FLUID) continue;
+        const double phi = cell_phi(g, pp, i, j, k);
+        if (!iface_phi(phi)) continue;
+        double gr[3];
+        iface_gradient(g, pp, i, j, k, gr);
+        if (vlen(gr) <= IFACE_GRAD_EPS) continue;
+        const double raw = iface_curvature(g, pp, i, j, k);
+        double sm = raw;
+        if (r > 0 && isfinite(raw)) {  /* sparseSmoothedCurvature, :210-235 */
+          double sum = 0.0;
+          int cnt = 0;
+          for (int dz = -r; dz <= r; ++dz)
+            for (int dy = -r; dy <= r; ++dy)
+              for (int dx = -r; dx <= r; ++dx) {
+                const int ni = clampi(i + dx, 0, g->nx - 1), nj = clampi(j + dy, 0, g->ny - 1);
+                const int nk = clampi(k + dz, 0, g->nz - 1);
+                if (!iface_phi(cell_phi(g, pp, ni, nj, nk))) continue;
+                const double c = iface_curvature(g, pp, ni, nj, nk);
+                if (!isfinite(c)) continue;
+                sum += c;
+                ++cnt;
+              }
+          sm = cnt > 0 ? sum / (double)cnt : raw;
+        }
+        /* limitedDelta, :72-79 */
+        const double f = -strength * sm * dt;
+        double d[3] = {gr[0] * f, gr[1] * f, gr[2] * f};
+        const double mag0 = vlen(d);
+        if (!isfinite(mag0)) { d[0] = d[1] = d[2] = 0.0; }
+        else if (maxd > 0.0 && mag0 > maxd) {
+          const double s = maxd / mag0;
+          d[0] *= s; d[1] *= s; d[2] *= s;
+        }
+        /* accumulateSurfaceStats, :104-122 */
+        const double mag = vlen(d);
+        if (!isfinite(mag) || !isfinite(raw) || !isfinite(sm)) { S->finite = 0; continue; }
+        S->applied_cells++;
+        S->mean_delta_speed += mag;
+        S->max_delta_speed = fmax(S->max_delta_speed, mag);
+        S->raw_curvature_abs_mean += fabs(raw);
+        S->raw_curvature_abs_max = fmax(S->raw_curvature_abs_max, fabs(raw));
+        S->smoothed_curvature_abs_mean += fabs(sm);
+        S->smoothed_curvature_abs_max = fmax(S->smoothed_curvature_abs_max, fabs(sm));
+        if (!S->finite) continue;
+        if (field_get(&g->f[ORC_MU], i, j, k) > 0.0) face_add(g, ORC_U, i, j, k, 0.5 * d[0]);
+        if (field_get(&g->f[ORC_MU], i + 1, j, k) > 0.0) face_add(g, ORC_U, i + 1, j, k, 0.5 * d[0]);
+        if (field_get(&g->f[ORC_MV], i, j, k) > 0.0) face_add(g, ORC_V, i, j, k, 0.5 * d[1]);
+        if (field_get(&g->f[ORC_MV], i, j + 1, k) > 0.0) face_add(g, ORC_V, i, j + 1, k, 0.5 * d[1]);
+        if (field_get(&g->f[ORC_MW], i, j, k) > 0.0) face_add(g, ORC_W, i, j, k, 0.5 * d[2]);
+        if (field_get(&g->f[ORC_MW], i, j, k + 1) > 0.0) face_add(g, ORC_W, i, j, k + 1, 0.5 * d[2]);
+      }
+  /* finishSurfaceStats, :124-131 */
+  if (S->applied_cells > 0) {
+    const double n = (double)S->applied_cells;
+    S->mean_delta_speed /= n;
+    S->raw_curvature_abs_mean /= n;
+    S->smoothed_curvature_abs_mean /= n;
+  }
 }

@@ -117,7 +117,24 @@ typedef struct orc_step_cfg {
   int32_t c_div_volume_correction, pad0;
   double c_div_strength, liquid_volume_target;
   orc_phase phase;
+  /* CSF surface tension (sparse_sim3d_tp.h:156-159) */
+  int32_t surface_tension, surface_tension_smoothing_radius;
+  double surface_tension_strength, surface_tension_max_delta_speed;
 } orc_step_cfg;
+
+/* InterfaceDiagnostics3D, src/driver/interface_diagnostics3d.h:7-19 */
+typedef struct orc_iface_stats {
+  int32_t sample_cells, interface_cells, finite, surface_tension_candidate;
+  double phi_min, phi_max, phi_mean, grad_mean, grad_max, curvature_abs_mean, curvature_abs_max;
+} orc_iface_stats;
+
+/* SurfaceTensionStats3D, src/driver/interface_diagnostics3d.h:21-36 */
+typedef struct orc_st_stats {
+  int32_t enabled, applied_cells, finite, curvature_smoothing_radius, capillary_stable, pad0;
+  double strength, max_delta_speed_limit, mean_delta_speed, max_delta_speed;
+  double raw_curvature_abs_mean, raw_curvature_abs_max, smoothed_curvature_abs_mean, smoothed_curvature_abs_max;
+  double capillary_dt_limit;
+} orc_st_stats;
 
 typedef struct orc_step_stats {
   orc_timestep_stats ts;
@@ -125,7 +142,18 @@ typedef struct orc_step_stats {
   orc_escape_stats escape;
   double liquid_volume, c_div;
   double ms_mark, ms_p2g, ms_save_gravity_walls, ms_project, ms_g2p, ms_advect, ms_total;
+  orc_iface_stats iface;
+  orc_st_stats surface;
 } orc_step_stats;
+
+/* diagnoseSparseInterface3D and applySparseSurfaceTension3D,
+ * src/driver/interface_diagnostics3d.cpp:362-440 */
+void orc_interface_diagnostics(const orc_grid* g, const orc_phase* pp, orc_iface_stats* out);
+void orc_surface_tension(orc_grid* g, const orc_phase* pp, double dt, double strength, double max_delta_speed,
+                         int smoothing_radius, orc_st_stats* out);
+/* per-cell values of the interface stencils (tests): phi, |grad phi|, curvature, at
+ * every cell of the grid, x fastest */
+void orc_interface_fields(const orc_grid* g, const orc_phase* pp, double* phi, double* grad, double* curvature);
 
 void orc_step(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u, double* v,
               double* w, const uint8_t* type, const double* vol, const orc_step_cfg* cfg,

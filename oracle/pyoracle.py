@@ -64,7 +64,32 @@ class StepCfg(C.Structure):
                 ("adaptive_cfl", C.c_double), ("adaptive_min_dt", C.c_double),
                 ("cg_tol", C.c_double), ("c_div_volume_correction", C.c_int32),
                 ("pad0", C.c_int32), ("c_div_strength", C.c_double),
-                ("liquid_volume_target", C.c_double), ("phase", Phase)]
+                ("liquid_volume_target", C.c_double), ("phase", Phase),
+                ("surface_tension", C.c_int32), ("surface_tension_smoothing_radius", C.c_int32),
+                ("surface_tension_strength", C.c_double), ("surface_tension_max_delta_speed", C.c_double)]
+
+
+class IfaceStats(C.Structure):
+    """InterfaceDiagnostics3D (src/driver/interface_diagnostics3d.h:7-19)"""
+    _fields_ = [("sample_cells", C.c_int32), ("interface_cells", C.c_int32), ("finite", C.c_int32),
+                ("surface_tension_candidate", C.c_int32)] + [
+        (n, C.c_double) for n in ("phi_min", "phi_max", "phi_mean", "grad_mean", "grad_max", "curvature_abs_mean",
+                                  "curvature_abs_max")]
+
+    def values(self):
+        return [getattr(self, n) for n, _ in self._fields_]
+
+
+class StStats(C.Structure):
+    """SurfaceTensionStats3D (src/driver/interface_diagnostics3d.h:21-36)"""
+    _fields_ = [(n, C.c_int32) for n in ("enabled", "applied_cells", "finite", "curvature_smoothing_radius",
+                                          "capillary_stable", "pad0")] + [
+        (n, C.c_double) for n in ("strength", "max_delta_speed_limit", "mean_delta_speed", "max_delta_speed",
+                                  "raw_curvature_abs_mean", "raw_curvature_abs_max", "smoothed_curvature_abs_mean",
+                                  "smoothed_curvature_abs_max", "capillary_dt_limit")]
+
+    def values(self):
+        return [getattr(self, n) for n, _ in self._fields_ if n != "pad0"]
 
 
 class MRSolveCfg(C.Structure):
@@ -144,7 +169,8 @@ class StepStats(C.Structure):
                 ("liquid_volume", C.c_double), ("c_div", C.c_double),
                 ("ms_mark", C.c_double), ("ms_p2g", C.c_double),
                 ("ms_save_gravity_walls", C.c_double), ("ms_project", C.c_double),
-                ("ms_g2p", C.c_double), ("ms_advect", C.c_double), ("ms_total", C.c_double)]
+                ("ms_g2p", C.c_double), ("ms_advect", C.c_double), ("ms_total", C.c_double),
+                ("iface", IfaceStats), ("surface", StStats)]
 
 
 def build(ref: bool = True) -> None:
@@ -197,6 +223,10 @@ def lib():
                                    C.c_double, C.c_double, C.POINTER(TimestepStats)]
         L.orc_step.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _u8,
                                C.c_void_p, C.POINTER(StepCfg), C.POINTER(StepStats)]
+        L.orc_interface_diagnostics.argtypes = [C.c_void_p, C.POINTER(Phase), C.POINTER(IfaceStats)]
+        L.orc_surface_tension.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, C.c_double, C.c_double,
+                                          C.c_int, C.POINTER(StStats)]
+        L.orc_interface_fields.argtypes = [C.c_void_p, C.POINTER(Phase), _dp, _dp, _dp]
         L.orc_p2g_single.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_project_single.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int,
                                          C.POINTER(SolveStats)]
@@ -362,6 +392,23 @@ class Grid:
                        _vptr(ps.volume), C.byref(cfg), C.byref(st))
         return st
 
+    def interface_diagnostics(self, phase):
+        st = IfaceStats()
+        lib().orc_interface_diagnostics(self.h, C.byref(phase), C.byref(st))
+        return st
+
+    def surface_tension(self, phase, dt, strength=0.02, max_delta_speed=0.25, radius=0):
+        st = StStats()
+        lib().orc_surface_tension(self.h, C.byref(phase), dt, strength, max_delta_speed, radius, C.byref(st))
+        return st
+
+    def interface_fields(self, phase):
+        """(phi, |grad phi|, curvature) per cell, arrays [nz, ny, nx]"""
+        n = self.nx * self.ny * self.nz
+        out = [np.zeros(n) for _ in range(3)]
+        lib().orc_interface_fields(self.h, C.byref(phase), *out)
+        return [a.reshape(self.nz, self.ny, self.nx) for a in out]
+
 
 def phi(raw, phase):
     return lib().orc_phi(raw, C.byref(phase))
@@ -425,6 +472,10 @@ def reflib():
                                             _u8, C.c_void_p]
         L.ref_sim_get_particles.argtypes = [C.c_void_p] + [_dp] * 6
         L.ref_sim_step.argtypes = [C.c_void_p, _dp]
+        L.ref_interface_diagnostics.argtypes = [C.c_void_p, _dp]
+        L.ref_surface_tension.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.ref_sim_surface_tension.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.ref_sim_last_interface.argtypes = [C.c_void_p, _dp]
         L.ref_field_dims.argtypes = [C.c_void_p, C.c_int, _i32]
         L.ref_field_dense.argtypes = [C.c_void_p, C.c_int, _dp]
         L.ref_active_blocks.restype = C.c_int64
@@ -679,6 +730,24 @@ class RefSim:
         ic = np.array([cfg.adaptive_timestep, cfg.advection_order, cfg.cg_iters,
                        cfg.c_div_volume_correction], np.int32)
         reflib().ref_sim_config(self.h, c, ic, cfg.phase.as_array())
+        reflib().ref_sim_surface_tension(self.h, cfg.surface_tension, cfg.surface_tension_strength,
+                                         cfg.surface_tension_max_delta_speed, cfg.surface_tension_smoothing_radius)
+
+    def interface_diagnostics(self):
+        out = np.zeros(11)
+        reflib().ref_interface_diagnostics(self.h, out)
+        return out
+
+    def surface_tension(self, dt, strength=0.02, max_delta_speed=0.25, radius=0):
+        out = np.zeros(14)
+        reflib().ref_surface_tension(self.h, dt, strength, max_delta_speed, radius, out)
+        return out
+
+    def last_interface(self):
+        """(interface_diagnostics_last, surface_tension_stats_last) after a step"""
+        out = np.zeros(25)
+        reflib().ref_sim_last_interface(self.h, out)
+        return out[:11], out[11:]
 
     def set_particles(self, ps: Particles):
         self.n = len(ps)

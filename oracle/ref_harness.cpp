@@ -103,6 +103,42 @@ void ref_sim_step(void* sp, double* out) {
   }
 }
 
+static void iface_out(const InterfaceDiagnostics3D& d, double* out) {
+  out[0] = d.sample_cells; out[1] = d.interface_cells; out[2] = d.finite; out[3] = d.surface_tension_candidate;
+  out[4] = d.phi_min; out[5] = d.phi_max; out[6] = d.phi_mean; out[7] = d.grad_mean; out[8] = d.grad_max;
+  out[9] = d.curvature_abs_mean; out[10] = d.curvature_abs_max;
+}
+static void st_out(const SurfaceTensionStats3D& t, double* out) {
+  out[0] = t.enabled; out[1] = t.applied_cells; out[2] = t.finite; out[3] = t.curvature_smoothing_radius;
+  out[4] = t.capillary_stable; out[5] = t.strength; out[6] = t.max_delta_speed_limit; out[7] = t.mean_delta_speed;
+  out[8] = t.max_delta_speed; out[9] = t.raw_curvature_abs_mean; out[10] = t.raw_curvature_abs_max;
+  out[11] = t.smoothed_curvature_abs_mean; out[12] = t.smoothed_curvature_abs_max; out[13] = t.capillary_dt_limit;
+}
+// diagnoseSparseInterface3D on the sim's grid (11 values, InterfaceDiagnostics3D order)
+void ref_interface_diagnostics(void* sp, double* out) {
+  auto* s = static_cast<SparseSim3DTP*>(sp);
+  iface_out(diagnoseSparseInterface3D(s->grid, s->phase), out);
+}
+// applySparseSurfaceTension3D on the sim's grid (14 values, SurfaceTensionStats3D order)
+void ref_surface_tension(void* sp, double dt, double strength, double maxd, int radius, double* out) {
+  auto* s = static_cast<SparseSim3DTP*>(sp);
+  st_out(applySparseSurfaceTension3D(s->grid, s->phase, dt, strength, maxd, radius), out);
+}
+// the sim's surface tension flags (sparse_sim3d_tp.h:156-159)
+void ref_sim_surface_tension(void* sp, int on, double strength, double maxd, int radius) {
+  auto* s = static_cast<SparseSim3DTP*>(sp);
+  s->surface_tension = on != 0;
+  s->surface_tension_strength = strength;
+  s->surface_tension_max_delta_speed = maxd;
+  s->surface_tension_curvature_smoothing_radius = radius;
+}
+// interface_diagnostics_last (11) then surface_tension_stats_last (14)
+void ref_sim_last_interface(void* sp, double* out) {
+  auto* s = static_cast<SparseSim3DTP*>(sp);
+  iface_out(s->interface_diagnostics_last, out);
+  st_out(s->surface_tension_stats_last, out + 11);
+}
+
 void ref_field_dims(void* sp, int f, int32_t* dims) {
   auto& F = fieldOf(static_cast<SparseSim3DTP*>(sp)->grid, f);
   dims[0] = F.nx; dims[1] = F.ny; dims[2] = F.nz;
