@@ -26,6 +26,7 @@
 #include "pf_lpc.hip"
 #include "pf_mr.hip"
 #include "pf_mr_ref.hip"
+#include "pf_iface.hip"
 
 using namespace pf;
 
@@ -1772,6 +1773,7 @@ int pf_destroy(pf_ctx* c) {
   for (double* p : f64) hipFree(p);
   dfree(c->mr_lvl); dfree(c->mr_vol64); dfree(c->mr_pfull); dfree(c->mr_bbox); dfree(c->mr_leaves_dev);
   dfree(c->mrf); dfree(c->mrc); dfree(c->mr_pin_dev);
+  dfree(c->iface); dfree(c->iface_bad);
   if (c->mr_pin_host) hipHostFree(c->mr_pin_host);
   for (int a = 0; a < 3; ++a) { dfree(c->mr_valid[a]); dfree(c->mr_cf32[a]); dfree(c->mr_cf64[a]); }
   if (c->mr_bbox_host) hipHostFree(c->mr_bbox_host);
@@ -1949,6 +1951,152 @@ int pf_advect(pf_ctx* c, double dt, int32_t order, pf_escape_stats* stats) {
   return PF_OK;
 }
 
+// ---------------------------------------------------------------- interface diagnostics, CSF
+namespace {
+
+constexpr int kIfaceFields = 9;
+
+// the cell phase field, normals and curvature from the current face masses and
+// markers (interface_diagnostics3d.cpp:137-208)
+int iface_fields(pf_ctx* c, const pf_phase* ph, IfaceArgs& a) {
+  const long long n = c->dc.nblocks * kBlockVol;
+  if (!c->iface) {
+    HIP_TRY(c, dalloc(&c->iface, n * kIfaceFields));
+    HIP_TRY(c, dalloc(&c->iface_bad, 1));
+  }
+  a.dc = c->dc; a.du = c->du; a.dv = c->dv; a.dw = c->dw;
+  a.g = c->geom;
+  a.mu = c->f[F_MU]; a.mv = c->f[F_MV]; a.mw = c->f[F_MW];
+  a.marker = c->marker;
+  a.ph = phase_k(ph);
+  a.phi = c->iface;
+  a.gmag = c->iface + n;
+  for (int d = 0; d < 3; ++d) a.nrm[d] = c->iface + (2 + d) * n;
+  a.kap = c->iface + 5 * n;
+  const dim3 g(grid_for(n, 256));
+  const long long ki = kbegin(c, PF_K_INTERFACE);
+  hipLaunchKernelGGL(k_iface_phi, g, dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_iface_normal, g, dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_iface_curvature, g, dim3(256), 0, c->stream, a);
+  kend(c, ki);
+  LAUNCH_CHECK(c);
+  return PF_OK;
+}
+
+unsigned iface_groups(const pf_ctx* c) {
+  return std::min<unsigned>(grid_for(c->dc.nblocks * kBlockVol, 256), 1024u);
+}
+
+// accumulate and finishStats (interface_diagnostics3d.cpp:26-66) over the FLUID cells
+int iface_reduce(pf_ctx* c, const IfaceArgs& a, pf_iface_stats* out) {
+  const unsigned nwg = iface_groups(c);
+  const CombineOps ops{kIfaceSlots, {0, 1, 2, 0, 0, 0, 2, 0, 2, 0}};
+  const long long ki = kbegin(c, PF_K_INTERFACE);
+  hipLaunchKernelGGL(k_iface_diag, dim3(nwg), dim3(256), 0, c->stream, a, c->partial);
+  hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(64), 0, c->stream, c->partial, (int)nwg, ops, c->red);
+  kend(c, ki);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, hipMemcpyAsync(c->red_host, c->red, sizeof(double) * kIfaceSlots, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const double* v = c->red_host;
+  pf_iface_stats S;
+  std::memset(&S, 0, sizeof(S));
+  S.finite = v[9] == 0.0 ? 1 : 0;
+  S.sample_cells = (int32_t)v[0];
+  S.interface_cells = (int32_t)v[4];
+  if (S.sample_cells > 0) {
+    S.phi_min = v[1];
+    S.phi_max = v[2];
+    S.phi_mean = v[3] / v[0];
+  }
+  S.grad_max = v[6];
+  S.curvature_abs_max = v[8];
+  if (S.interface_cells > 0) {
+    S.grad_mean = v[5] / v[4];
+    S.curvature_abs_mean = v[7] / v[4];
+  }
+  S.surface_tension_candidate = S.finite && S.interface_cells > 0 && S.grad_max > kIfaceGradEps;
+  *out = S;
+  return PF_OK;
+}
+
+// initSurfaceStats (:87-102), the CSF increments (:404-440), finishSurfaceStats (:124-131)
+int surface_apply(pf_ctx* c, const pf_phase* ph, const IfaceArgs& a, double dt, double strength, double maxd,
+                  int radius, pf_surface_stats* out) {
+  pf_surface_stats S;
+  std::memset(&S, 0, sizeof(S));
+  S.finite = 1;
+  S.enabled = strength > 0.0 && dt > 0.0;
+  S.strength = strength;
+  S.max_delta_speed_limit = maxd;
+  S.curvature_smoothing_radius = std::min(std::max(radius, 0), 3);
+  const double rmin = std::min(ph->rho_l, ph->rho_g), dx = c->geom.dx;
+  const double pi = 3.14159265358979323846;
+  S.capillary_dt_limit = (dx <= 0.0 || strength <= 0.0 || rmin <= 0.0 || !std::isfinite(rmin))
+                             ? 0.0 : std::sqrt((rmin * dx * dx * dx) / (2.0 * pi * strength));
+  S.capillary_stable = !S.enabled || S.capillary_dt_limit <= 0.0 || dt <= S.capillary_dt_limit + 1e-12;
+  if (S.enabled) {
+    const long long n = c->dc.nblocks * kBlockVol;
+    CsfArgs cs;
+    cs.f = a;
+    cs.dt = dt;
+    cs.strength = strength;
+    cs.maxd = maxd;
+    cs.radius = S.curvature_smoothing_radius;
+    for (int d = 0; d < 3; ++d) cs.delta[d] = c->iface + (6 + d) * n;
+    cs.first_bad = c->iface_bad;
+    const unsigned nwg = iface_groups(c);
+    const CombineOps ops{kCsfSlots, {0, 0, 2, 0, 2, 0, 2, 0}};
+    HIP_TRY(c, fill_u32(c->stream, c->iface_bad, 0xffffffffu, 1));
+    const long long ki = kbegin(c, PF_K_SURFACE);
+    hipLaunchKernelGGL(k_csf_delta, dim3(nwg), dim3(256), 0, c->stream, cs, c->partial);
+    hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(64), 0, c->stream, c->partial, (int)nwg, ops, c->red);
+    const Dims* fd[3] = {&c->du, &c->dv, &c->dw};
+    for (int ax = 0; ax < 3; ++ax)
+      hipLaunchKernelGGL(k_csf_faces, dim3(grid_for(fd[ax]->nblocks * kBlockVol, 256)), dim3(256), 0, c->stream,
+                         cs, ax, c->f[F_U + ax], static_cast<const float*>(c->f[F_MU + ax]));
+    kend(c, ki);
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipMemcpyAsync(c->red_host, c->red, sizeof(double) * kCsfSlots, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const double* v = c->red_host;
+    S.applied_cells = (int32_t)v[0];
+    S.finite = v[7] == 0.0 ? 1 : 0;
+    S.max_delta_speed = v[2];
+    S.raw_curvature_abs_max = v[4];
+    S.smoothed_curvature_abs_max = v[6];
+    S.mean_delta_speed = v[1];
+    S.raw_curvature_abs_mean = v[3];
+    S.smoothed_curvature_abs_mean = v[5];
+    if (S.applied_cells > 0) {
+      S.mean_delta_speed /= v[0];
+      S.raw_curvature_abs_mean /= v[0];
+      S.smoothed_curvature_abs_mean /= v[0];
+    }
+  }
+  if (out) *out = S;
+  return PF_OK;
+}
+
+}  // namespace
+
+int pf_interface_diagnostics(pf_ctx* c, const pf_phase* ph, pf_iface_stats* out) {
+  if (!c || !ph || !out) return PF_ERR_ARG;
+  IfaceArgs a;
+  int st = iface_fields(c, ph, a);
+  if (st) return st;
+  return iface_reduce(c, a, out);
+}
+
+int pf_apply_surface_tension(pf_ctx* c, const pf_phase* ph, double dt, double strength, double max_delta_speed,
+                             int32_t smoothing_radius, pf_surface_stats* out) {
+  if (!c || !ph) return PF_ERR_ARG;
+  IfaceArgs a;
+  int st = iface_fields(c, ph, a);
+  if (st) return st;
+  return surface_apply(c, ph, a, dt, strength, max_delta_speed, smoothing_radius, out);
+}
+
 int pf_timestep(pf_ctx* c, double requested_dt, int32_t enabled, double cfl, double min_dt, double* eff,
                 double* vmax_out, double* cfl_dt, int32_t* limited) {
   if (!c) return PF_ERR_ARG;
@@ -1985,6 +2133,11 @@ int pf_step(pf_ctx* c, const pf_step_cfg* cfg, pf_step_stats* stats) {
   const bool single = cfg->model == PF_MODEL_SINGLE_PHASE;
   pf_step_stats S;
   std::memset(&S, 0, sizeof(S));
+  // the defaults of InterfaceDiagnostics3D and SurfaceTensionStats3D
+  // (interface_diagnostics3d.h:7-36), kept where a part is skipped
+  S.iface.finite = 1;
+  S.surface.finite = 1;
+  S.surface.capillary_stable = 1;
   const bool timing = cfg->timing != 0;
   record(c, timing, 0);
   int st = ensure_binned(c);
@@ -2001,8 +2154,30 @@ int pf_step(pf_ctx* c, const pf_step_cfg* cfg, pf_step_stats* stats) {
   st = run_p2g_splat(c, &cfg->phase, cfg->Vp, single);
   if (st) return st;
   record(c, timing, 3);
-  st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | FO_WALLS, (float)(dt * cfg->gravity));
+  // interface_diagnostics_last, then the saved copy, gravity, surface tension and
+  // walls (sparse_sim3d_tp.cpp:865-874); the face operations leave the masses the
+  // diagnostics read unchanged, so they fuse into one pass unless the CSF
+  // increments have to land between gravity and the walls
+  const bool csf = !single && cfg->surface_tension;
+  const bool diag = !single && !cfg->skip_interface_diagnostics;
+  st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | (csf ? 0 : FO_WALLS), (float)(dt * cfg->gravity));
   if (st) return st;
+  if (diag || csf) {
+    IfaceArgs ia;
+    st = iface_fields(c, &cfg->phase, ia);
+    if (st) return st;
+    if (diag) {
+      st = iface_reduce(c, ia, &S.iface);
+      if (st) return st;
+    }
+    if (csf) {
+      st = surface_apply(c, &cfg->phase, ia, dt, cfg->surface_tension_strength,
+                         cfg->surface_tension_max_delta_speed, cfg->surface_tension_smoothing_radius, &S.surface);
+      if (st) return st;
+      st = run_face_ops(c, FO_WALLS, 0.f);
+      if (st) return st;
+    }
+  }
   record(c, timing, 4);
   st = run_project(c, &cfg->phase, dt, &cfg->solve, cfg->c_div, &S.solve, timing, 5, 6, 7, single);
   if (st) return st;

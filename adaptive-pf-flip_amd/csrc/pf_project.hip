@@ -31,7 +31,7 @@ __device__ inline double phi_of_raw(double raw, const PhaseK& k) {
   double d = raw - k.rmin;
   if (d < 0.0) d = 0.0;
   double v = sqrt(d / k.den);
-  return v < 1.0 ? v : 1.0;
+  return 1.0 < v ? 1.0 : v;  // std::min(v, 1.0): a NaN mass gives a NaN phase
 }
 
 // betaOfRaw, sparse_ops3d_tp.cpp:89-91 with phiFromRawDensity / betaFromPhi

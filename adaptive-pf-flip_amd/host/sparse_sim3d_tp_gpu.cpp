@@ -156,6 +156,11 @@ void SparseSim3DTP::step() {
   cfg.solve.max_outer = 10000;
   cfg.solve.warm_start = warm_start;
   cfg.solve.check_every = 8;
+  cfg.surface_tension = surface_tension ? 1 : 0;
+  cfg.surface_tension_strength = surface_tension_strength;
+  cfg.surface_tension_max_delta_speed = surface_tension_max_delta_speed;
+  cfg.surface_tension_smoothing_radius = surface_tension_curvature_smoothing_radius;
+  cfg.skip_interface_diagnostics = interface_diagnostics ? 0 : 1;
   // updateVolumeCorrectionStats (sparse_sim3d_tp.cpp:368-379) needs this step's dt
   double stepDt = dt;
   if (adaptive_timestep) {
@@ -174,6 +179,8 @@ void SparseSim3DTP::step() {
   max_particle_speed_last = stats_last.max_particle_speed;
   cfl_limit_dt_last = stats_last.cfl_limit_dt;
   adaptive_timestep_limited_last = stats_last.limited;
+  interface_diagnostics_last = stats_last.iface;
+  surface_tension_stats_last = stats_last.surface;
   // storeParticleBoundaryStats, sparse_sim3d_tp.cpp:116-131
   const pf_escape_stats& e = stats_last.escape;
   particle_boundary_clamped_liquid_last = e.clamped_liquid;

@@ -40,6 +40,23 @@ struct Particles3DTP {
   }
 };
 
+// InterfaceDiagnostics3D and SurfaceTensionStats3D
+// (src/driver/interface_diagnostics3d.h:7-36): the ABI structs carry the same field
+// names; these give them the reference's default values
+using InterfaceDiagnostics3D = pf_iface_stats;
+using SurfaceTensionStats3D = pf_surface_stats;
+inline InterfaceDiagnostics3D defaultInterfaceDiagnostics() {
+  InterfaceDiagnostics3D d{};
+  d.finite = 1;
+  return d;
+}
+inline SurfaceTensionStats3D defaultSurfaceTensionStats() {
+  SurfaceTensionStats3D s{};
+  s.finite = 1;
+  s.capillary_stable = 1;
+  return s;
+}
+
 struct PhaseParams {
   double rho_l = 1.0;
   double rho_g = 0.01;
@@ -78,6 +95,14 @@ class SparseSim3DTP {
   int particle_boundary_clamped_y_hi_last = 0;
   int particle_boundary_clamped_z_lo_last = 0;
   int particle_boundary_clamped_z_hi_last = 0;
+  InterfaceDiagnostics3D interface_diagnostics_last = defaultInterfaceDiagnostics();
+  bool surface_tension = false;
+  double surface_tension_strength = 0.02;
+  double surface_tension_max_delta_speed = 0.25;
+  int surface_tension_curvature_smoothing_radius = 0;
+  SurfaceTensionStats3D surface_tension_stats_last = defaultSurfaceTensionStats();
+  // false: skip the per-step interface statistics (the reference always computes them)
+  bool interface_diagnostics = true;
 
   // GPU solver controls (no reference counterpart): the reference's rule is
   // |r|_inf < cg_tol; PF_STOP_REL2 stops on ||b - Ax||_2/||b||_2 <= cg_tol in FP64

@@ -24,7 +24,7 @@ STOP_REF_MAXABS, STOP_REL2 = 0, 1
 SCENE_RT, SCENE_BUBBLE, SCENE_DAMBREAK = 0, 1, 2
 STAGES = ("bin", "mark", "p2g", "grid", "assemble", "solve", "velocity", "g2p_advect")
 KERNELS = ("bin_count", "bin_scatter", "p2g", "face_ops", "assemble", "pcg_spmv", "pcg_update",
-           "residual64", "velocity", "g2p_advect", "lpc_apply")
+           "residual64", "velocity", "g2p_advect", "lpc_apply", "interface", "surface")
 (K_BIN_COUNT, K_BIN_SCATTER, K_P2G, K_FACE_OPS, K_ASSEMBLE, K_PCG_SPMV, K_PCG_UPDATE,
  K_RESIDUAL64, K_VELOCITY, K_G2P_ADVECT, K_LPC_APPLY) = range(11)
 
@@ -127,7 +127,10 @@ class StepCfg(C.Structure):
                 ("adaptive_cfl", C.c_double), ("adaptive_min_dt", C.c_double),
                 ("c_div", C.c_double), ("adaptive_timestep", C.c_int32),
                 ("advection_order", C.c_int32), ("timing", C.c_int32), ("model", C.c_int32),
-                ("phase", Phase), ("solve", SolveCfg)]
+                ("phase", Phase), ("solve", SolveCfg),
+                ("surface_tension", C.c_int32), ("surface_tension_smoothing_radius", C.c_int32),
+                ("surface_tension_strength", C.c_double), ("surface_tension_max_delta_speed", C.c_double),
+                ("skip_interface_diagnostics", C.c_int32), ("reserved", C.c_int32)]
 
 
 class MRStepCfg(C.Structure):
@@ -139,11 +142,29 @@ class MRStepCfg(C.Structure):
                 ("phase", Phase), ("solve", MRSolveCfg), ("refine", MRRefineCfg)]
 
 
+class IfaceStats(C.Structure):
+    """InterfaceDiagnostics3D, src/driver/interface_diagnostics3d.h:7-19"""
+    _fields_ = [(n, C.c_int32) for n in ("sample_cells", "interface_cells", "finite",
+                                         "surface_tension_candidate")] + \
+               [(n, C.c_double) for n in ("phi_min", "phi_max", "phi_mean", "grad_mean", "grad_max",
+                                          "curvature_abs_mean", "curvature_abs_max")]
+
+
+class SurfaceStats(C.Structure):
+    """SurfaceTensionStats3D, src/driver/interface_diagnostics3d.h:21-36"""
+    _fields_ = [(n, C.c_int32) for n in ("enabled", "applied_cells", "finite", "curvature_smoothing_radius",
+                                         "capillary_stable", "reserved")] + \
+               [(n, C.c_double) for n in ("strength", "max_delta_speed_limit", "mean_delta_speed",
+                                          "max_delta_speed", "raw_curvature_abs_mean", "raw_curvature_abs_max",
+                                          "smoothed_curvature_abs_mean", "smoothed_curvature_abs_max",
+                                          "capillary_dt_limit")]
+
+
 class StepStats(C.Structure):
     _fields_ = [("effective_dt", C.c_double), ("max_particle_speed", C.c_double),
                 ("cfl_limit_dt", C.c_double), ("limited", C.c_int32), ("reserved", C.c_int32),
                 ("solve", SolveStats), ("escape", EscapeStats), ("stage_ms", C.c_double * 8),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("iface", IfaceStats), ("surface", SurfaceStats)]
 
 
 class PfError(RuntimeError):
@@ -183,6 +204,9 @@ def lib():
             "pf_timestep": (C.c_int, [vp, dbl, i32, dbl, dbl, C.POINTER(dbl), C.POINTER(dbl),
                                       C.POINTER(dbl), C.POINTER(i32)]),
             "pf_step": (C.c_int, [vp, C.POINTER(StepCfg), C.POINTER(StepStats)]),
+            "pf_interface_diagnostics": (C.c_int, [vp, C.POINTER(Phase), C.POINTER(IfaceStats)]),
+            "pf_apply_surface_tension": (C.c_int, [vp, C.POINTER(Phase), dbl, dbl, dbl, i32,
+                                                   C.POINTER(SurfaceStats)]),
             "pf_field_dims": (C.c_int, [vp, i32, vp]),
             "pf_field_download": (C.c_int, [vp, i32, vp]),
             "pf_field_upload": (C.c_int, [vp, i32, vp, vp]),
@@ -355,6 +379,22 @@ class Context:
         self._check(lib().pf_step(self.h, C.byref(cfg), C.byref(st)), "pf_step")
         return st
 
+    def interface_diagnostics(self, phase: Phase) -> IfaceStats:
+        """diagnoseSparseInterface3D over the current masses and markers"""
+        out = IfaceStats()
+        self._check(lib().pf_interface_diagnostics(self.h, C.byref(phase), C.byref(out)),
+                    "pf_interface_diagnostics")
+        return out
+
+    def apply_surface_tension(self, phase: Phase, dt, strength=0.02, max_delta_speed=0.25,
+                              smoothing_radius=0) -> SurfaceStats:
+        """applySparseSurfaceTension3D on the current face velocities"""
+        out = SurfaceStats()
+        self._check(lib().pf_apply_surface_tension(self.h, C.byref(phase), dt, strength, max_delta_speed,
+                                                   int(smoothing_radius), C.byref(out)),
+                    "pf_apply_surface_tension")
+        return out
+
     def synchronize(self):
         self._check(lib().pf_synchronize(self.h), "pf_synchronize")
 
@@ -427,10 +467,19 @@ class SparseSim3DTP:
         self.stop_rule = STOP_REF_MAXABS
         self.inner_rel_tol, self.warm_start, self.check_every = 0.1, 0, 8
         self.timing = False
+        # CSF surface tension (sparse_sim3d_tp.h:156-159)
+        self.surface_tension = False
+        self.surface_tension_strength = 0.02
+        self.surface_tension_max_delta_speed = 0.25
+        self.surface_tension_curvature_smoothing_radius = 0
+        # False skips the per-step interface statistics (the reference always computes them)
+        self.interface_diagnostics = True
         self.effective_dt_last = self.dt
         self.max_particle_speed_last = 0.0
         self.c_div_last = 0.0
         self.liquid_volume_current_last = 0.0
+        self.interface_diagnostics_last = IfaceStats(finite=1)
+        self.surface_tension_stats_last = SurfaceStats(finite=1, capillary_stable=1)
         self.stats_last = None
 
     def set_particles(self, x, y, z, u=None, v=None, w=None, type=None, volume=None):
@@ -452,7 +501,12 @@ class SparseSim3DTP:
                        adaptive_cfl=self.adaptive_cfl, adaptive_min_dt=self.adaptive_min_dt,
                        c_div=0.0, adaptive_timestep=int(self.adaptive_timestep),
                        advection_order=self.advection_order, timing=int(self.timing),
-                       phase=self.phase, solve=solve)
+                       phase=self.phase, solve=solve,
+                       surface_tension=int(self.surface_tension),
+                       surface_tension_smoothing_radius=self.surface_tension_curvature_smoothing_radius,
+                       surface_tension_strength=self.surface_tension_strength,
+                       surface_tension_max_delta_speed=self.surface_tension_max_delta_speed,
+                       skip_interface_diagnostics=int(not self.interface_diagnostics))
 
     def step(self):
         """SparseSim3DTP::step, src/driver/sparse_sim3d_tp.cpp:854-884."""
@@ -472,6 +526,8 @@ class SparseSim3DTP:
         st = self.ctx.step(cfg)
         self.effective_dt_last = st.effective_dt
         self.max_particle_speed_last = st.max_particle_speed
+        self.interface_diagnostics_last = st.iface
+        self.surface_tension_stats_last = st.surface
         self.stats_last = st
         return st
 

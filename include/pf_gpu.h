@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 2
+#define PF_ABI_VERSION 3
 
 enum pf_status {
   PF_OK = 0,
@@ -111,6 +111,13 @@ typedef struct pf_step_cfg {
   int32_t model;             /* pf_model */
   pf_phase phase;
   pf_solve_cfg solve;
+  /* CSF surface tension (SparseSim3DTP fields, sparse_sim3d_tp.h:156-159; the
+   * reference's defaults are off, 0.02, 0.25 and 0) */
+  int32_t surface_tension, surface_tension_smoothing_radius;
+  double surface_tension_strength, surface_tension_max_delta_speed;
+  /* 1: leave pf_step_stats.iface zero instead of computing the interface
+   * diagnostics the reference computes every step (sparse_sim3d_tp.cpp:865) */
+  int32_t skip_interface_diagnostics, reserved;
 } pf_step_cfg;
 
 enum pf_stage {
@@ -118,13 +125,30 @@ enum pf_stage {
   PF_STAGE_SOLVE, PF_STAGE_VELOCITY, PF_STAGE_G2P_ADVECT, PF_NSTAGES
 };
 
+/* InterfaceDiagnostics3D, src/driver/interface_diagnostics3d.h:7-19 */
+typedef struct pf_iface_stats {
+  int32_t sample_cells, interface_cells, finite, surface_tension_candidate;
+  double phi_min, phi_max, phi_mean, grad_mean, grad_max, curvature_abs_mean, curvature_abs_max;
+} pf_iface_stats;
+
+/* SurfaceTensionStats3D, src/driver/interface_diagnostics3d.h:21-36 */
+typedef struct pf_surface_stats {
+  int32_t enabled, applied_cells, finite, curvature_smoothing_radius, capillary_stable, reserved;
+  double strength, max_delta_speed_limit, mean_delta_speed, max_delta_speed;
+  double raw_curvature_abs_mean, raw_curvature_abs_max;
+  double smoothed_curvature_abs_mean, smoothed_curvature_abs_max;
+  double capillary_dt_limit;
+} pf_surface_stats;
+
 typedef struct pf_step_stats {
   double effective_dt, max_particle_speed, cfl_limit_dt;
   int32_t limited, reserved;
   pf_solve_stats solve;
   pf_escape_stats escape;
-  double stage_ms[PF_NSTAGES];
+  double stage_ms[PF_NSTAGES];   /* PF_STAGE_GRID includes the interface work */
   double total_ms;
+  pf_iface_stats iface;          /* SparseSim3DTP::interface_diagnostics_last */
+  pf_surface_stats surface;      /* SparseSim3DTP::surface_tension_stats_last */
 } pf_step_stats;
 
 typedef struct pf_ctx pf_ctx;
@@ -175,11 +199,20 @@ int pf_g2p(pf_ctx* ctx, double alpha_liquid, double alpha_gas);
 /* spAdvect3D_tp, sparse_ops3d_tp.cpp:201-232 / sparse_ops3d_tp.h:12-15; order 2 is
  * also the single-phase spAdvect3D (sparse_ops3d.cpp:144-168) */
 int pf_advect(pf_ctx* ctx, double dt, int32_t order, pf_escape_stats* stats);
+/* diagnoseSparseInterface3D, src/driver/interface_diagnostics3d.cpp:362-379 /
+ * interface_diagnostics3d.h:38: statistics of the face-averaged phase field, its
+ * gradient and curvature over the FLUID cells (masses from the last P2G) */
+int pf_interface_diagnostics(pf_ctx* ctx, const pf_phase* phase, pf_iface_stats* out);
+/* applySparseSurfaceTension3D, interface_diagnostics3d.cpp:404-440 / .h:44: CSF
+ * velocity increments added to the faces with positive mass; out may be NULL */
+int pf_apply_surface_tension(pf_ctx* ctx, const pf_phase* phase, double dt, double strength,
+                             double max_delta_speed, int32_t smoothing_radius, pf_surface_stats* out);
 /* computeAdaptiveParticleTimestep3D, src/driver/timestep3d.h:16-50 */
 int pf_timestep(pf_ctx* ctx, double requested_dt, int32_t enabled, double cfl, double min_dt,
                 double* effective_dt, double* max_speed, double* cfl_limit_dt, int32_t* limited);
-/* SparseSim3DTP::step, sparse_sim3d_tp.cpp:854-884 (default flags: no particle
- * adaptivity, secondary particles or surface tension), with G2P and advection fused;
+/* SparseSim3DTP::step, sparse_sim3d_tp.cpp:854-884 (no particle adaptivity or
+ * secondary particles; interface diagnostics and optional surface tension as the
+ * reference), with G2P and advection fused;
  * cfg->model = PF_MODEL_SINGLE_PHASE runs SparseSim3D::step, sparse_sim3d.cpp:106-115 */
 int pf_step(pf_ctx* ctx, const pf_step_cfg* cfg, pf_step_stats* stats);
 
@@ -208,6 +241,8 @@ enum pf_kernel {
   PF_K_BIN_COUNT = 0, PF_K_BIN_SCATTER, PF_K_P2G, PF_K_FACE_OPS, PF_K_ASSEMBLE,
   PF_K_PCG_SPMV, PF_K_PCG_UPDATE, PF_K_RESIDUAL64, PF_K_VELOCITY, PF_K_G2P_ADVECT,
   PF_K_LPC_APPLY,      /* one application of the learned preconditioner (5 kernels) */
+  PF_K_INTERFACE,      /* interface fields and diagnostics (5 kernels) */
+  PF_K_SURFACE,        /* CSF increments and face updates (5 kernels) */
   PF_NKERNELS
 };
 /* The library's HIP stream (hipStream_t). */

@@ -202,7 +202,7 @@ double orc_phi(double rt, const orc_phase* pp) {
   double d = rt - rmin;
   if (d < 0.0) d = 0.0;
   double v = sqrt(d / (pp->alpha_phi * pp->rho_tilde_0 * pp->rho_l));
-  return v < 1.0 ? v : 1.0;
+  return 1.0 < v ? 1.0 : v;  /* std::min(v, 1.0): NaN stays NaN */
 }
 
 /* betaFromPhi, src/physics/phasefield.h:17-18 */
@@ -940,6 +940,8 @@ void orc_step(orc_grid* g, int64_t n, double* x, double* y, double* z, double* u
   if (c->surface_tension)
     orc_surface_tension(g, &c->phase, stepDt, c->surface_tension_strength, c->surface_tension_max_delta_speed,
                         c->surface_tension_smoothing_radius, &S->surface);
+  else  /* SurfaceTensionStats3D() */
+    orc_st_init(&S->surface, &c->phase, 0.0, 0.0, 0.0, 0.0, 0);
   orc_apply_walls(g);
   double t4 = now_ms();
   orc_project(g, &c->phase, stepDt, c->cg_iters, c->cg_tol, S->c_div, c->stop_rule, &S->solve);
@@ -1052,7 +1054,9 @@ int64_t orc_scene_fill(int kind, int nx, int ny, int nz, double dx, uint64_t see
  * grad / |grad| (zero below 1e-12, :177-181), curvature as the divergence of the
  * normals (:188-208), smoothing over a clamped (2r+1)^3 box of interface cells
  * (:210-235), CSF velocity increment grad * (-strength * kappa * dt) limited to
- * max_delta_speed and added half to each face of the cell (:404-440). */
+ * max_delta_speed and added half to each face of the cell (:404-440).  The stencils
+ * read face masses through an orc_iface_src, so the multires variants (mrFacePhi
+ * and the rest, :237-340, in pf_oracle_mr.c) evaluate the same fine-grid formulas. */
 
 #define IFACE_PHI_LO 0.05
 #define IFACE_PHI_HI 0.95
@@ -1061,52 +1065,52 @@ int64_t orc_scene_fill(int kind, int nx, int ny, int nz, double dx, uint64_t see
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-static double cell_phi(const orc_grid* g, const orc_phase* pp, int i, int j, int k) {
-  if (i < 0 || j < 0 || k < 0 || i >= g->nx || j >= g->ny || k >= g->nz) return 0.0;
-  double s = 0.0;
-  s += orc_phi(field_get(&g->f[ORC_MU], i, j, k), pp);
-  s += orc_phi(field_get(&g->f[ORC_MU], i + 1, j, k), pp);
-  s += orc_phi(field_get(&g->f[ORC_MV], i, j, k), pp);
-  s += orc_phi(field_get(&g->f[ORC_MV], i, j + 1, k), pp);
-  s += orc_phi(field_get(&g->f[ORC_MW], i, j, k), pp);
-  s += orc_phi(field_get(&g->f[ORC_MW], i, j, k + 1), pp);
-  return s / 6.0;
+double orc_iface_cell_phi(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= s->nx || j >= s->ny || k >= s->nz) return 0.0;
+  double sum = 0.0;
+  sum += orc_phi(s->mass(s->g, 0, i, j, k), pp);
+  sum += orc_phi(s->mass(s->g, 0, i + 1, j, k), pp);
+  sum += orc_phi(s->mass(s->g, 1, i, j, k), pp);
+  sum += orc_phi(s->mass(s->g, 1, i, j + 1, k), pp);
+  sum += orc_phi(s->mass(s->g, 2, i, j, k), pp);
+  sum += orc_phi(s->mass(s->g, 2, i, j, k + 1), pp);
+  return sum / 6.0;
 }
 
-static double phi_clamped(const orc_grid* g, const orc_phase* pp, int i, int j, int k) {
-  return cell_phi(g, pp, clampi(i, 0, g->nx - 1), clampi(j, 0, g->ny - 1), clampi(k, 0, g->nz - 1));
+static double phi_clamped(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k) {
+  return orc_iface_cell_phi(s, pp, clampi(i, 0, s->nx - 1), clampi(j, 0, s->ny - 1), clampi(k, 0, s->nz - 1));
 }
 
-static void iface_gradient(const orc_grid* g, const orc_phase* pp, int i, int j, int k, double out[3]) {
-  const int im = clampi(i - 1, 0, g->nx - 1), ip = clampi(i + 1, 0, g->nx - 1);
-  const int jm = clampi(j - 1, 0, g->ny - 1), jp = clampi(j + 1, 0, g->ny - 1);
-  const int km = clampi(k - 1, 0, g->nz - 1), kp = clampi(k + 1, 0, g->nz - 1);
-  const double dx = g->dx;
-  out[0] = (phi_clamped(g, pp, ip, j, k) - phi_clamped(g, pp, im, j, k)) / ((ip - im > 1 ? ip - im : 1) * dx);
-  out[1] = (phi_clamped(g, pp, i, jp, k) - phi_clamped(g, pp, i, jm, k)) / ((jp - jm > 1 ? jp - jm : 1) * dx);
-  out[2] = (phi_clamped(g, pp, i, j, kp) - phi_clamped(g, pp, i, j, km)) / ((kp - km > 1 ? kp - km : 1) * dx);
+void orc_iface_gradient(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, double out[3]) {
+  const int im = clampi(i - 1, 0, s->nx - 1), ip = clampi(i + 1, 0, s->nx - 1);
+  const int jm = clampi(j - 1, 0, s->ny - 1), jp = clampi(j + 1, 0, s->ny - 1);
+  const int km = clampi(k - 1, 0, s->nz - 1), kp = clampi(k + 1, 0, s->nz - 1);
+  const double dx = s->dx;
+  out[0] = (phi_clamped(s, pp, ip, j, k) - phi_clamped(s, pp, im, j, k)) / ((ip - im > 1 ? ip - im : 1) * dx);
+  out[1] = (phi_clamped(s, pp, i, jp, k) - phi_clamped(s, pp, i, jm, k)) / ((jp - jm > 1 ? jp - jm : 1) * dx);
+  out[2] = (phi_clamped(s, pp, i, j, kp) - phi_clamped(s, pp, i, j, km)) / ((kp - km > 1 ? kp - km : 1) * dx);
 }
 
 static double vlen(const double v[3]) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
 
-static void iface_normal(const orc_grid* g, const orc_phase* pp, int i, int j, int k, double n[3]) {
+static void iface_normal(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, double n[3]) {
   double gr[3];
-  iface_gradient(g, pp, i, j, k, gr);
+  orc_iface_gradient(s, pp, i, j, k, gr);
   const double mag = vlen(gr);
   if (mag <= IFACE_NORMAL_EPS || !isfinite(mag)) { n[0] = n[1] = n[2] = 0.0; return; }
   const double inv = 1.0 / mag;
   n[0] = gr[0] * inv; n[1] = gr[1] * inv; n[2] = gr[2] * inv;
 }
 
-static double iface_curvature(const orc_grid* g, const orc_phase* pp, int i, int j, int k) {
-  const int im = clampi(i - 1, 0, g->nx - 1), ip = clampi(i + 1, 0, g->nx - 1);
-  const int jm = clampi(j - 1, 0, g->ny - 1), jp = clampi(j + 1, 0, g->ny - 1);
-  const int km = clampi(k - 1, 0, g->nz - 1), kp = clampi(k + 1, 0, g->nz - 1);
-  const double dx = g->dx;
+double orc_iface_curvature(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k) {
+  const int im = clampi(i - 1, 0, s->nx - 1), ip = clampi(i + 1, 0, s->nx - 1);
+  const int jm = clampi(j - 1, 0, s->ny - 1), jp = clampi(j + 1, 0, s->ny - 1);
+  const int km = clampi(k - 1, 0, s->nz - 1), kp = clampi(k + 1, 0, s->nz - 1);
+  const double dx = s->dx;
   double nxm[3], nxp[3], nym[3], nyp[3], nzm[3], nzp[3];
-  iface_normal(g, pp, im, j, k, nxm); iface_normal(g, pp, ip, j, k, nxp);
-  iface_normal(g, pp, i, jm, k, nym); iface_normal(g, pp, i, jp, k, nyp);
-  iface_normal(g, pp, i, j, km, nzm); iface_normal(g, pp, i, j, kp, nzp);
+  iface_normal(s, pp, im, j, k, nxm); iface_normal(s, pp, ip, j, k, nxp);
+  iface_normal(s, pp, i, jm, k, nym); iface_normal(s, pp, i, jp, k, nyp);
+  iface_normal(s, pp, i, j, km, nzm); iface_normal(s, pp, i, j, kp, nzp);
   return (nxp[0] - nxm[0]) / ((ip - im > 1 ? ip - im : 1) * dx) +
          (nyp[1] - nym[1]) / ((jp - jm > 1 ? jp - jm : 1) * dx) +
          (nzp[2] - nzm[2]) / ((kp - km > 1 ? kp - km : 1) * dx);
@@ -1114,51 +1118,149 @@ static double iface_curvature(const orc_grid* g, const orc_phase* pp, int i, int
 
 static int iface_phi(double phi) { return phi > IFACE_PHI_LO && phi < IFACE_PHI_HI; }
 
-void orc_interface_fields(const orc_grid* g, const orc_phase* pp, double* phi, double* grad, double* curvature) {
-  for (int k = 0; k < g->nz; ++k)
-    for (int j = 0; j < g->ny; ++j)
-      for (int i = 0; i < g->nx; ++i) {
-        const int64_t c = i + (int64_t)g->nx * (j + (int64_t)g->ny * k);
-        double gr[3];
-        iface_gradient(g, pp, i, j, k, gr);
-        phi[c] = cell_phi(g, pp, i, j, k);
-        grad[c] = vlen(gr);
-        curvature[c] = iface_curvature(g, pp, i, j, k);
+/* sparseSmoothedCurvature / mrSmoothedCurvature, :210-235 and :322-348 */
+static double smoothed_curvature(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, double raw,
+                                 int r) {
+  if (r <= 0 || !isfinite(raw)) return raw;
+  double sum = 0.0;
+  int cnt = 0;
+  for (int dz = -r; dz <= r; ++dz)
+    for (int dy = -r; dy <= r; ++dy)
+      for (int dx = -r; dx <= r; ++dx) {
+        const int ni = clampi(i + dx, 0, s->nx - 1), nj = clampi(j + dy, 0, s->ny - 1);
+        const int nk = clampi(k + dz, 0, s->nz - 1);
+        if (!iface_phi(orc_iface_cell_phi(s, pp, ni, nj, nk))) continue;
+        const double c = orc_iface_curvature(s, pp, ni, nj, nk);
+        if (!isfinite(c)) continue;
+        sum += c;
+        ++cnt;
       }
+  return cnt > 0 ? sum / (double)cnt : raw;
 }
 
-/* diagnoseSparseInterface3D, :362-379, over the FLUID cells in linear order */
-void orc_interface_diagnostics(const orc_grid* g, const orc_phase* pp, orc_iface_stats* S) {
-  memset(S, 0, sizeof(*S));
-  S->finite = 1;
-  for (int k = 0; k < g->nz; ++k)
-    for (int j = 0; j < g->ny; ++j)
-      for (int i = 0; i < g->nx; ++i) {
-        if ((int)(field_get(&g->f[ORC_MK], i, j, k) + 0.5) != ORC_FLUID) continue;
-        const double phi = cell_phi(g, pp, i, j, k);
-        double gr[3];
-        iface_gradient(g, pp, i, j, k, gr);
-        const double gm = vlen(gr), kap = iface_curvature(g, pp, i, j, k);
-        /* accumulate, :26-53 */
-        if (!isfinite(phi) || !isfinite(gm) || !isfinite(kap)) { S->finite = 0; continue; }
-        if (S->sample_cells == 0) { S->phi_min = phi; S->phi_max = phi; }
-        else { S->phi_min = fmin(S->phi_min, phi); S->phi_max = fmax(S->phi_max, phi); }
-        S->sample_cells++;
-        S->phi_mean += phi;
-        if (!iface_phi(phi)) continue;
-        S->interface_cells++;
-        S->grad_mean += gm;
-        S->grad_max = fmax(S->grad_max, gm);
-        S->curvature_abs_mean += fabs(kap);
-        S->curvature_abs_max = fmax(S->curvature_abs_max, fabs(kap));
-      }
-  /* finishStats, :56-66 */
+/* accumulate, :26-53 */
+void orc_iface_accumulate(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, orc_iface_stats* S) {
+  const double phi = orc_iface_cell_phi(s, pp, i, j, k);
+  double gr[3];
+  orc_iface_gradient(s, pp, i, j, k, gr);
+  const double gm = vlen(gr), kap = orc_iface_curvature(s, pp, i, j, k);
+  if (!isfinite(phi) || !isfinite(gm) || !isfinite(kap)) { S->finite = 0; return; }
+  if (S->sample_cells == 0) { S->phi_min = phi; S->phi_max = phi; }
+  else { S->phi_min = fmin(S->phi_min, phi); S->phi_max = fmax(S->phi_max, phi); }
+  S->sample_cells++;
+  S->phi_mean += phi;
+  if (!iface_phi(phi)) return;
+  S->interface_cells++;
+  S->grad_mean += gm;
+  S->grad_max = fmax(S->grad_max, gm);
+  S->curvature_abs_mean += fabs(kap);
+  S->curvature_abs_max = fmax(S->curvature_abs_max, fabs(kap));
+}
+
+/* finishStats, :56-66 */
+void orc_iface_finish(orc_iface_stats* S) {
   if (S->sample_cells > 0) S->phi_mean /= (double)S->sample_cells;
   if (S->interface_cells > 0) {
     S->grad_mean /= (double)S->interface_cells;
     S->curvature_abs_mean /= (double)S->interface_cells;
   }
   S->surface_tension_candidate = S->finite && S->interface_cells > 0 && S->grad_max > IFACE_GRAD_EPS;
+}
+
+/* initSurfaceStats and capillaryDtLimit, :81-102 */
+void orc_st_init(orc_st_stats* S, const orc_phase* pp, double dx, double dt, double strength, double maxd,
+                 int radius) {
+  memset(S, 0, sizeof(*S));
+  S->finite = 1;
+  S->enabled = strength > 0.0 && dt > 0.0;
+  S->strength = strength;
+  S->max_delta_speed_limit = maxd;
+  S->curvature_smoothing_radius = clampi(radius, 0, 3);
+  const double rmin = pp->rho_l < pp->rho_g ? pp->rho_l : pp->rho_g;
+  const double pi = 3.14159265358979323846;
+  S->capillary_dt_limit = (dx <= 0.0 || strength <= 0.0 || rmin <= 0.0 || !isfinite(rmin))
+                              ? 0.0 : sqrt((rmin * dx * dx * dx) / (2.0 * pi * strength));
+  S->capillary_stable = !S->enabled || S->capillary_dt_limit <= 0.0 || dt <= S->capillary_dt_limit + 1e-12;
+}
+
+/* one FLUID cell of applySparseSurfaceTension3D / applyMRSurfaceTension3D (:415-431,
+ * :462-477): -1 when it is no interface cell or its gradient is below the threshold,
+ * else accumulateSurfaceStats (:104-122) and 1 when the increment d applies (the
+ * stats are still finite), 0 when it does not */
+int orc_st_cell(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, double dt, double strength,
+                double maxd, orc_st_stats* S, double d[3]) {
+  const double phi = orc_iface_cell_phi(s, pp, i, j, k);
+  if (!iface_phi(phi)) return -1;
+  double gr[3];
+  orc_iface_gradient(s, pp, i, j, k, gr);
+  if (vlen(gr) <= IFACE_GRAD_EPS) return -1;
+  const double raw = orc_iface_curvature(s, pp, i, j, k);
+  const double sm = smoothed_curvature(s, pp, i, j, k, raw, S->curvature_smoothing_radius);
+  /* limitedDelta, :72-79 */
+  const double f = -strength * sm * dt;
+  d[0] = gr[0] * f; d[1] = gr[1] * f; d[2] = gr[2] * f;
+  const double mag0 = vlen(d);
+  if (!isfinite(mag0)) { d[0] = d[1] = d[2] = 0.0; }
+  else if (maxd > 0.0 && mag0 > maxd) {
+    const double sc = maxd / mag0;
+    d[0] *= sc; d[1] *= sc; d[2] *= sc;
+  }
+  const double mag = vlen(d);
+  if (!isfinite(mag) || !isfinite(raw) || !isfinite(sm)) { S->finite = 0; return 0; }
+  S->applied_cells++;
+  S->mean_delta_speed += mag;
+  S->max_delta_speed = fmax(S->max_delta_speed, mag);
+  S->raw_curvature_abs_mean += fabs(raw);
+  S->raw_curvature_abs_max = fmax(S->raw_curvature_abs_max, fabs(raw));
+  S->smoothed_curvature_abs_mean += fabs(sm);
+  S->smoothed_curvature_abs_max = fmax(S->smoothed_curvature_abs_max, fabs(sm));
+  return S->finite ? 1 : 0;
+}
+
+/* finishSurfaceStats, :124-131 */
+void orc_st_finish(orc_st_stats* S) {
+  if (S->applied_cells > 0) {
+    const double n = (double)S->applied_cells;
+    S->mean_delta_speed /= n;
+    S->raw_curvature_abs_mean /= n;
+    S->smoothed_curvature_abs_mean /= n;
+  }
+}
+
+static double sparse_mass(const void* gp, int axis, int i, int j, int k) {
+  const orc_grid* g = (const orc_grid*)gp;
+  return field_get(&g->f[ORC_MU + axis], i, j, k);
+}
+
+static orc_iface_src sparse_src(const orc_grid* g) {
+  orc_iface_src s = {g->nx, g->ny, g->nz, g->dx, g, sparse_mass};
+  return s;
+}
+
+void orc_interface_fields(const orc_grid* g, const orc_phase* pp, double* phi, double* grad, double* curvature) {
+  const orc_iface_src s = sparse_src(g);
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i) {
+        const int64_t c = i + (int64_t)g->nx * (j + (int64_t)g->ny * k);
+        double gr[3];
+        orc_iface_gradient(&s, pp, i, j, k, gr);
+        phi[c] = orc_iface_cell_phi(&s, pp, i, j, k);
+        grad[c] = vlen(gr);
+        curvature[c] = orc_iface_curvature(&s, pp, i, j, k);
+      }
+}
+
+/* diagnoseSparseInterface3D, :362-379, over the FLUID cells in linear order */
+void orc_interface_diagnostics(const orc_grid* g, const orc_phase* pp, orc_iface_stats* S) {
+  const orc_iface_src s = sparse_src(g);
+  memset(S, 0, sizeof(*S));
+  S->finite = 1;
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i)
+        if ((int)(field_get(&g->f[ORC_MK], i, j, k) + 0.5) == ORC_FLUID) orc_iface_accumulate(&s, pp, i, j, k, S);
+  orc_iface_finish(S);
 }
 
 /* `float += (float)x` (faithful) or a double add on face field f */
@@ -1170,70 +1272,15 @@ static void face_add(orc_grid* g, int f, int i, int j, int k, double add) {
 /* applySparseSurfaceTension3D, :404-440 */
 void orc_surface_tension(orc_grid* g, const orc_phase* pp, double dt, double strength, double maxd, int radius,
                          orc_st_stats* S) {
-  memset(S, 0, sizeof(*S));
-  S->finite = 1;
-  S->capillary_stable = 1;
-  /* initSurfaceStats, :87-102 */
-  S->enabled = strength > 0.0 && dt > 0.0;
-  S->strength = strength;
-  S->max_delta_speed_limit = maxd;
-  S->curvature_smoothing_radius = clampi(radius, 0, 3);
-  {
-    const double rmin = pp->rho_l < pp->rho_g ? pp->rho_l : pp->rho_g;
-    const double pi = 3.14159265358979323846;
-    S->capillary_dt_limit = (g->dx <= 0.0 || strength <= 0.0 || rmin <= 0.0 || !isfinite(rmin))
-                                ? 0.0 : sqrt((rmin * g->dx * g->dx * g->dx) / (2.0 * pi * strength));
-  }
-  S->capillary_stable = !S->enabled || S->capillary_dt_limit <= 0.0 || dt <= S->capillary_dt_limit + 1e-12;
+  const orc_iface_src s = sparse_src(g);
+  orc_st_init(S, pp, g->dx, dt, strength, maxd, radius);
   if (!S->enabled) return;
-  const int r = S->curvature_smoothing_radius;
   for (int k = 0; k < g->nz; ++k)
     for (int j = 0; j < g->ny; ++j)
       for (int i = 0; i < g->nx; ++i) {
         if ((int)(field_get(&g->f[ORC_MK], i, j, k) + 0.5) != ORC_FLUID) continue;
-        const double phi = cell_phi(g, pp, i, j, k);
-        if (!iface_phi(phi)) continue;
-        double gr[3];
-        iface_gradient(g, pp, i, j, k, gr);
-        if (vlen(gr) <= IFACE_GRAD_EPS) continue;
-        const double raw = iface_curvature(g, pp, i, j, k);
-        double sm = raw;
-        if (r > 0 && isfinite(raw)) {  /* sparseSmoothedCurvature, :210-235 */
-          double sum = 0.0;
-          int cnt = 0;
-          for (int dz = -r; dz <= r; ++dz)
-            for (int dy = -r; dy <= r; ++dy)
-              for (int dx = -r; dx <= r; ++dx) {
-                const int ni = clampi(i + dx, 0, g->nx - 1), nj = clampi(j + dy, 0, g->ny - 1);
-                const int nk = clampi(k + dz, 0, g->nz - 1);
-                if (!iface_phi(cell_phi(g, pp, ni, nj, nk))) continue;
-                const double c = iface_curvature(g, pp, ni, nj, nk);
-                if (!isfinite(c)) continue;
-                sum += c;
-                ++cnt;
-              }
-          sm = cnt > 0 ? sum / (double)cnt : raw;
-        }
-        /* limitedDelta, :72-79 */
-        const double f = -strength * sm * dt;
-        double d[3] = {gr[0] * f, gr[1] * f, gr[2] * f};
-        const double mag0 = vlen(d);
-        if (!isfinite(mag0)) { d[0] = d[1] = d[2] = 0.0; }
-        else if (maxd > 0.0 && mag0 > maxd) {
-          const double s = maxd / mag0;
-          d[0] *= s; d[1] *= s; d[2] *= s;
-        }
-        /* accumulateSurfaceStats, :104-122 */
-        const double mag = vlen(d);
-        if (!isfinite(mag) || !isfinite(raw) || !isfinite(sm)) { S->finite = 0; continue; }
-        S->applied_cells++;
-        S->mean_delta_speed += mag;
-        S->max_delta_speed = fmax(S->max_delta_speed, mag);
-        S->raw_curvature_abs_mean += fabs(raw);
-        S->raw_curvature_abs_max = fmax(S->raw_curvature_abs_max, fabs(raw));
-        S->smoothed_curvature_abs_mean += fabs(sm);
-        S->smoothed_curvature_abs_max = fmax(S->smoothed_curvature_abs_max, fabs(sm));
-        if (!S->finite) continue;
+        double d[3];
+        if (orc_st_cell(&s, pp, i, j, k, dt, strength, maxd, S, d) != 1) continue;
         if (field_get(&g->f[ORC_MU], i, j, k) > 0.0) face_add(g, ORC_U, i, j, k, 0.5 * d[0]);
         if (field_get(&g->f[ORC_MU], i + 1, j, k) > 0.0) face_add(g, ORC_U, i + 1, j, k, 0.5 * d[0]);
         if (field_get(&g->f[ORC_MV], i, j, k) > 0.0) face_add(g, ORC_V, i, j, k, 0.5 * d[1]);
@@ -1241,11 +1288,5 @@ void orc_surface_tension(orc_grid* g, const orc_phase* pp, double dt, double str
         if (field_get(&g->f[ORC_MW], i, j, k) > 0.0) face_add(g, ORC_W, i, j, k, 0.5 * d[2]);
         if (field_get(&g->f[ORC_MW], i, j, k + 1) > 0.0) face_add(g, ORC_W, i, j, k + 1, 0.5 * d[2]);
       }
-  /* finishSurfaceStats, :124-131 */
-  if (S->applied_cells > 0) {
-    const double n = (double)S->applied_cells;
-    S->mean_delta_speed /= n;
-    S->raw_curvature_abs_mean /= n;
-    S->smoothed_curvature_abs_mean /= n;
-  }
+  orc_st_finish(S);
 }

@@ -146,6 +146,26 @@ typedef struct orc_step_stats {
   orc_st_stats surface;
 } orc_step_stats;
 
+/* the fine-grid interface stencils (interface_diagnostics3d.cpp:137-235, :237-348)
+ * over any face-mass source: mass(g, axis, i, j, k) for a face inside the face
+ * extents of an nx x ny x nz grid */
+typedef struct orc_iface_src {
+  int nx, ny, nz;
+  double dx;
+  const void* g;
+  double (*mass)(const void* g, int axis, int i, int j, int k);
+} orc_iface_src;
+double orc_iface_cell_phi(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k);
+void orc_iface_gradient(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, double out[3]);
+double orc_iface_curvature(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k);
+void orc_iface_accumulate(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, orc_iface_stats* S);
+void orc_iface_finish(orc_iface_stats* S);
+void orc_st_init(orc_st_stats* S, const orc_phase* pp, double dx, double dt, double strength, double maxd,
+                 int radius);
+int orc_st_cell(const orc_iface_src* s, const orc_phase* pp, int i, int j, int k, double dt, double strength,
+                double maxd, orc_st_stats* S, double d[3]);
+void orc_st_finish(orc_st_stats* S);
+
 /* diagnoseSparseInterface3D and applySparseSurfaceTension3D,
  * src/driver/interface_diagnostics3d.cpp:362-440 */
 void orc_interface_diagnostics(const orc_grid* g, const orc_phase* pp, orc_iface_stats* out);
@@ -216,6 +236,9 @@ typedef struct orc_mr_sim {                /* MRSim3DTP fields step() reads + re
   orc_mr_solve_cfg solve;
   int32_t dynamic_refinement, particle_padding, gas_padding, hysteresis, max_fine_leaves;
   int32_t retained_valid, retained[6], budget_limited, dynamic_last_fine_leaves;
+  /* CSF surface tension (multires_sim3d_tp.h fields as in sparse_sim3d_tp.h:156-159) */
+  int32_t surface_tension, surface_tension_smoothing_radius;
+  double surface_tension_strength, surface_tension_max_delta_speed;
 } orc_mr_sim;
 
 typedef struct orc_mr_step_stats {
@@ -223,6 +246,8 @@ typedef struct orc_mr_step_stats {
   orc_mr_solve_stats solve;
   orc_escape_stats escape;
   double c_div;
+  orc_iface_stats iface;
+  orc_st_stats surface;
 } orc_mr_step_stats;
 
 orc_mr* orc_mr_create(int nx, int ny, int nz, double dx, int faithful);
@@ -252,6 +277,11 @@ void orc_mr_advect(const orc_mr* g, int64_t n, double* x, double* y, double* z, 
                    double dt, int order, orc_escape_stats* es);
 void orc_mr_dynamic_refinement(orc_mr* g, orc_mr_sim* sim, int64_t n, const double* x, const double* y,
                                const double* z, const uint8_t* type);
+/* diagnoseMRInterface3D and applyMRSurfaceTension3D, interface_diagnostics3d.cpp:381-401,
+ * :447-496 */
+void orc_mr_interface_diagnostics(const orc_mr* g, const orc_phase* pp, orc_iface_stats* out);
+void orc_mr_surface_tension(orc_mr* g, const orc_phase* pp, double dt, double strength, double max_delta_speed,
+                            int smoothing_radius, orc_st_stats* out);
 void orc_mr_step(orc_mr* g, orc_mr_sim* sim, int64_t n, double* x, double* y, double* z, double* u,
                  double* v, double* w, const uint8_t* type, const double* vol, orc_mr_step_stats* st);
 

@@ -1273,8 +1273,78 @@ static void dynamic_box(const orc_mr_sim* s, int64_t n, const double* x, const d
   *valid_out = tv;
 }
 
-/* MRSim3DTP::step, multires_sim3d_tp.cpp:671-734, default flags (no particle adaptivity,
- * secondary particles or surface tension) */
+/* ---------------------------------------------------------------- interface */
+/* diagnoseMRInterface3D and applyMRSurfaceTension3D (interface_diagnostics3d.cpp:381-401,
+ * :447-496): the sparse stencils over the fine 1x1 face patches (mrFacePhi, :238-251,
+ * reads 0 for a patch the grid does not hold, as the array does), evaluated once per
+ * FLUID leaf cell at its fine centre clamped into the grid (mrCellFineCenter,
+ * :350-358), in leafCells order (leaves sorted, then lz, ly, lx; only cells that
+ * intersect the domain, multires_scalar_grid3d.h:75-88, 126-137). */
+
+static double mr_mass(const void* gp, int axis, int i, int j, int k) {
+  const orc_mr* g = (const orc_mr*)gp;
+  return g->f[ORC_MU + axis][fidx(g, axis, i, j, k)];
+}
+
+static orc_iface_src mr_src(const orc_mr* g) {
+  orc_iface_src s = {g->nx, g->ny, g->nz, g->dx, g, mr_mass};
+  return s;
+}
+
+static inline int clampi_mr(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* the clamped fine centre of FLUID leaf cell id, or 0 when it is skipped */
+static int mr_fluid_centre(const orc_mr* g, int64_t id, int* i, int* j, int* k) {
+  int x0, y0, z0, st;
+  cell_box(g, id, &x0, &y0, &z0, &st);
+  if (!(x0 < g->nx && x0 + st > 0 && y0 < g->ny && y0 + st > 0 && z0 < g->nz && z0 + st > 0)) return 0;
+  if ((int)(g->mk[id] + 0.5) != 1) return 0;
+  *i = clampi_mr(x0 + st / 2, 0, g->nx - 1);
+  *j = clampi_mr(y0 + st / 2, 0, g->ny - 1);
+  *k = clampi_mr(z0 + st / 2, 0, g->nz - 1);
+  return 1;
+}
+
+void orc_mr_interface_diagnostics(const orc_mr* g, const orc_phase* pp, orc_iface_stats* S) {
+  const orc_iface_src s = mr_src(g);
+  memset(S, 0, sizeof(*S));
+  S->finite = 1;
+  for (int64_t id = 0; id < (int64_t)g->nleaves * 64; ++id) {
+    int i, j, k;
+    if (mr_fluid_centre(g, id, &i, &j, &k)) orc_iface_accumulate(&s, pp, i, j, k, S);
+  }
+  orc_iface_finish(S);
+}
+
+static void mr_face_add(orc_mr* g, int axis, int i, int j, int k, double add) {
+  const int64_t fi = fidx(g, axis, i, j, k);
+  if (!(g->f[ORC_MU + axis][fi] > 0.0)) return;
+  double* r = &g->f[axis][fi];
+  *r = g->faithful ? (double)((float)*r + (float)add) : *r + add;
+}
+
+void orc_mr_surface_tension(orc_mr* g, const orc_phase* pp, double dt, double strength, double maxd, int radius,
+                            orc_st_stats* S) {
+  const orc_iface_src s = mr_src(g);
+  orc_st_init(S, pp, g->dx, dt, strength, maxd, radius);
+  if (!S->enabled) return;
+  for (int64_t id = 0; id < (int64_t)g->nleaves * 64; ++id) {
+    int i, j, k;
+    double d[3];
+    if (!mr_fluid_centre(g, id, &i, &j, &k)) continue;
+    if (orc_st_cell(&s, pp, i, j, k, dt, strength, maxd, S, d) != 1) continue;
+    mr_face_add(g, 0, i, j, k, 0.5 * d[0]);
+    mr_face_add(g, 0, i + 1, j, k, 0.5 * d[0]);
+    mr_face_add(g, 1, i, j, k, 0.5 * d[1]);
+    mr_face_add(g, 1, i, j + 1, k, 0.5 * d[1]);
+    mr_face_add(g, 2, i, j, k, 0.5 * d[2]);
+    mr_face_add(g, 2, i, j, k + 1, 0.5 * d[2]);
+  }
+  orc_st_finish(S);
+}
+
+/* MRSim3DTP::step, multires_sim3d_tp.cpp:671-734 (no particle adaptivity or
+ * secondary particles; interface diagnostics and optional surface tension) */
 void orc_mr_step(orc_mr* g, orc_mr_sim* sim, int64_t n, double* x, double* y, double* z, double* u, double* v,
                  double* w, const uint8_t* type, const double* vol, orc_mr_step_stats* S) {
   orc_mr_step_stats local;
@@ -1291,8 +1361,14 @@ void orc_mr_step(orc_mr* g, orc_mr_sim* sim, int64_t n, double* x, double* y, do
     S->c_div = sim->c_div_strength * (sim->liquid_volume_target - lv) / (dt * sim->liquid_volume_target);
   orc_mr_mark_cells(g, n, x, y, z);
   orc_mr_p2g(g, n, x, y, z, u, v, w, type, vol, &sim->phase, sim->Vp);
+  orc_mr_interface_diagnostics(g, &sim->phase, &S->iface);
   orc_mr* saved = orc_mr_clone(g);
   orc_mr_apply_gravity(g, dt, sim->gravity);
+  if (sim->surface_tension)
+    orc_mr_surface_tension(g, &sim->phase, dt, sim->surface_tension_strength, sim->surface_tension_max_delta_speed,
+                           sim->surface_tension_smoothing_radius, &S->surface);
+  else
+    orc_st_init(&S->surface, &sim->phase, 0.0, 0.0, 0.0, 0.0, 0);
   orc_mr_apply_walls(g);
   orc_mr_solve_cfg cf = sim->solve;
   cf.divergence_correction = S->c_div;

@@ -153,7 +153,9 @@ class MRSim(C.Structure):
                 ("liquid_volume_target", C.c_double), ("solve", MRSolveCfg), ("dynamic_refinement", C.c_int32),
                 ("particle_padding", C.c_int32), ("gas_padding", C.c_int32), ("hysteresis", C.c_int32),
                 ("max_fine_leaves", C.c_int32), ("retained_valid", C.c_int32), ("retained", C.c_int32 * 6),
-                ("budget_limited", C.c_int32), ("dynamic_last_fine_leaves", C.c_int32)]
+                ("budget_limited", C.c_int32), ("dynamic_last_fine_leaves", C.c_int32),
+                ("surface_tension", C.c_int32), ("surface_tension_smoothing_radius", C.c_int32),
+                ("surface_tension_strength", C.c_double), ("surface_tension_max_delta_speed", C.c_double)]
 
 
 def mr_sim(nx, ny, nz, dx=1.0, phase=None, dt=0.02, order=2, dynamic=1, ppad=1, gpad=2, hyst=1, max_fine=0,
@@ -260,6 +262,9 @@ def lib():
         L.orc_mr_advect.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _u8, C.c_double, C.c_int,
                                     C.POINTER(EscapeStats)]
         L.orc_mr_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + [_dp] * 6 + [_u8, C.c_void_p, C.c_void_p]
+        L.orc_mr_interface_diagnostics.argtypes = [C.c_void_p, C.POINTER(Phase), C.POINTER(IfaceStats)]
+        L.orc_mr_surface_tension.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, C.c_double, C.c_double,
+                                             C.c_int, C.POINTER(StStats)]
         L.orc_mr_dynamic_refinement.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, _dp, _dp, _dp, _u8]
         L.orc_scene_count.restype = C.c_int64
         L.orc_scene_count.argtypes = [C.c_int] * 4
@@ -338,6 +343,13 @@ class Grid:
 
     def set(self, f, i, j, k, v):
         lib().orc_set(self.h, f, i, j, k, float(v))
+
+    def load_dense(self, f, a):
+        """write the non-zero entries of a dense [nz, ny, nx] array (activating their
+        blocks, as ref() does); entries that are zero are left alone"""
+        a = np.asarray(a)
+        for k, j, i in zip(*np.nonzero(a)):
+            self.set(f, int(i), int(j), int(k), a[k, j, i])
 
     def mark_cells(self, ps: Particles):
         lib().orc_mark_cells(self.h, len(ps), ps.x, ps.y, ps.z)
@@ -522,6 +534,10 @@ def reflib():
         L.ref_mr_field_dense.argtypes = [C.c_void_p, C.c_int, _dp]
         L.ref_mr_face_valid.argtypes = [C.c_void_p, C.c_int, _u8]
         L.ref_mr_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.ref_mr_interface_diagnostics.argtypes = [C.c_void_p, _dp]
+        L.ref_mr_surface_tension.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.ref_mr_surface_flags.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.ref_mr_last_interface.argtypes = [C.c_void_p, _dp]
         L.ref1_create.restype = C.c_void_p
         L.ref1_create.argtypes = [C.c_int] * 3 + [C.c_double]
         L.ref1_destroy.argtypes = [C.c_void_p]
@@ -616,9 +632,20 @@ class MRGrid:
         lib().orc_mr_advect(self.h, len(ps), ps.x, ps.y, ps.z, ps.type, dt, order, C.byref(es))
         return es
 
+    def interface_diagnostics(self, phase):
+        st = IfaceStats()
+        lib().orc_mr_interface_diagnostics(self.h, C.byref(phase), C.byref(st))
+        return st
+
+    def surface_tension(self, phase, dt, strength=0.02, max_delta_speed=0.25, radius=0):
+        st = StStats()
+        lib().orc_mr_surface_tension(self.h, C.byref(phase), dt, strength, max_delta_speed, radius, C.byref(st))
+        return st
+
 
 class MRStepStats(C.Structure):
-    _fields_ = [("ts", TimestepStats), ("solve", MRSolveStats), ("escape", EscapeStats), ("c_div", C.c_double)]
+    _fields_ = [("ts", TimestepStats), ("solve", MRSolveStats), ("escape", EscapeStats), ("c_div", C.c_double),
+                ("iface", IfaceStats), ("surface", StStats)]
 
 
 def mr_step(g, sim, ps):
@@ -871,6 +898,25 @@ class RefMR:
 
     def dynamic(self):
         reflib().ref_mr_dynamic(self.h)
+
+    def interface_diagnostics(self):
+        out = np.zeros(11)
+        reflib().ref_mr_interface_diagnostics(self.h, out)
+        return out
+
+    def surface_tension(self, dt, strength=0.02, max_delta_speed=0.25, radius=0):
+        out = np.zeros(14)
+        reflib().ref_mr_surface_tension(self.h, dt, strength, max_delta_speed, radius, out)
+        return out
+
+    def surface_flags(self, on, strength=0.02, max_delta_speed=0.25, radius=0):
+        reflib().ref_mr_surface_flags(self.h, int(on), strength, max_delta_speed, radius)
+
+    def last_interface(self):
+        """(interface_diagnostics_last, surface_tension_stats_last) of the last step"""
+        out = np.zeros(25)
+        reflib().ref_mr_last_interface(self.h, out)
+        return out[:11], out[11:]
 
     def p2g(self):
         reflib().ref_mr_p2g(self.h)

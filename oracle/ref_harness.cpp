@@ -473,6 +473,28 @@ void ref_mr_face_valid(void* sp, int axis, uint8_t* out) {
   const auto& faces = axis == 0 ? g.uFaceRefs() : (axis == 1 ? g.vFaceRefs() : g.wFaceRefs());
   for (const auto& k : faces) out[k.fineX + (int64_t)fx * (k.fineY + (int64_t)fy * k.fineZ)] = 1;
 }
+// diagnoseMRInterface3D / applyMRSurfaceTension3D on the sim's grid, the surface
+// tension flags, and interface_diagnostics_last (11) then surface_tension_stats_last (14)
+void ref_mr_interface_diagnostics(void* sp, double* out) {
+  auto* s = mrs(sp);
+  iface_out(diagnoseMRInterface3D(s->grid, s->phase), out);
+}
+void ref_mr_surface_tension(void* sp, double dt, double strength, double maxd, int radius, double* out) {
+  auto* s = mrs(sp);
+  st_out(applyMRSurfaceTension3D(s->grid, s->phase, dt, strength, maxd, radius), out);
+}
+void ref_mr_surface_flags(void* sp, int on, double strength, double maxd, int radius) {
+  auto* s = mrs(sp);
+  s->surface_tension = on != 0;
+  s->surface_tension_strength = strength;
+  s->surface_tension_max_delta_speed = maxd;
+  s->surface_tension_curvature_smoothing_radius = radius;
+}
+void ref_mr_last_interface(void* sp, double* out) {
+  auto* s = mrs(sp);
+  iface_out(s->interface_diagnostics_last, out);
+  st_out(s->surface_tension_stats_last, out + 11);
+}
 void ref_mr_set(void* sp, int f, int i, int j, int k, double v) {
   auto& g = mrs(sp)->grid;
   if (f <= 5) {

@@ -35,6 +35,9 @@ def test_struct_layouts_match_the_header(tmp_path):
                    "sizeof(pf_grid_desc),sizeof(pf_phase),sizeof(pf_solve_cfg),"
                    "sizeof(pf_solve_stats),sizeof(pf_escape_stats),sizeof(pf_step_cfg),"
                    "sizeof(pf_step_stats),offsetof(pf_step_cfg,phase),offsetof(pf_step_stats,solve));"
+                   "printf(\"%zu %zu %zu %zu %zu %zu\\n\",sizeof(pf_iface_stats),sizeof(pf_surface_stats),"
+                   "offsetof(pf_step_cfg,surface_tension_strength),offsetof(pf_step_cfg,skip_interface_diagnostics),"
+                   "offsetof(pf_step_stats,iface),offsetof(pf_step_stats,surface));"
                    "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -42,7 +45,9 @@ def test_struct_layouts_match_the_header(tmp_path):
                                           check=True).stdout.split()]
     want = [C.sizeof(G.GridDesc), C.sizeof(G.Phase), C.sizeof(G.SolveCfg), C.sizeof(G.SolveStats),
             C.sizeof(G.EscapeStats), C.sizeof(G.StepCfg), C.sizeof(G.StepStats),
-            G.StepCfg.phase.offset, G.StepStats.solve.offset]
+            G.StepCfg.phase.offset, G.StepStats.solve.offset,
+            C.sizeof(G.IfaceStats), C.sizeof(G.SurfaceStats), G.StepCfg.surface_tension_strength.offset,
+            G.StepCfg.skip_interface_diagnostics.offset, G.StepStats.iface.offset, G.StepStats.surface.offset]
     assert got == want
 
 
@@ -56,7 +61,7 @@ def test_host_helpers_without_device():
     for raw in (0.0, 5.0, 7.0, 100.0, 900.0, 2000.0):
         assert L.pf_phi_from_raw(raw, C.byref(ph)) == O.phi(raw, oph)
     assert L.pf_beta_from_phi(0.25, C.byref(ph)) == O.beta(0.25, oph)
-    assert L.pf_abi_version() == 2
+    assert L.pf_abi_version() == 3
 
 
 def test_null_arguments_are_rejected_without_device():
