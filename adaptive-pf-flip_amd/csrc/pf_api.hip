@@ -1957,8 +1957,8 @@ namespace {
 constexpr int kIfaceFields = 9;
 
 // the cell phase field, normals and curvature from the current face masses and
-// markers (interface_diagnostics3d.cpp:137-208)
-int iface_fields(pf_ctx* c, const pf_phase* ph, IfaceArgs& a) {
+// markers (interface_diagnostics3d.cpp:137-208); mr: sample the multires leaf cells
+int iface_fields(pf_ctx* c, const pf_phase* ph, IfaceArgs& a, bool mr) {
   const long long n = c->dc.nblocks * kBlockVol;
   if (!c->iface) {
     HIP_TRY(c, dalloc(&c->iface, n * kIfaceFields));
@@ -1973,6 +1973,7 @@ int iface_fields(pf_ctx* c, const pf_phase* ph, IfaceArgs& a) {
   a.gmag = c->iface + n;
   for (int d = 0; d < 3; ++d) a.nrm[d] = c->iface + (2 + d) * n;
   a.kap = c->iface + 5 * n;
+  a.lvl = mr ? c->mr_lvl : nullptr;
   const dim3 g(grid_for(n, 256));
   const long long ki = kbegin(c, PF_K_INTERFACE);
   hipLaunchKernelGGL(k_iface_phi, g, dim3(256), 0, c->stream, a);
@@ -2047,7 +2048,7 @@ int surface_apply(pf_ctx* c, const pf_phase* ph, const IfaceArgs& a, double dt, 
     cs.first_bad = c->iface_bad;
     const unsigned nwg = iface_groups(c);
     const CombineOps ops{kCsfSlots, {0, 0, 2, 0, 2, 0, 2, 0}};
-    HIP_TRY(c, fill_u32(c->stream, c->iface_bad, 0xffffffffu, 1));
+    HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->iface_bad), 0xffffffffu, 2));
     const long long ki = kbegin(c, PF_K_SURFACE);
     hipLaunchKernelGGL(k_csf_delta, dim3(nwg), dim3(256), 0, c->stream, cs, c->partial);
     hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(64), 0, c->stream, c->partial, (int)nwg, ops, c->red);
@@ -2078,12 +2079,48 @@ int surface_apply(pf_ctx* c, const pf_phase* ph, const IfaceArgs& a, double dt, 
   return PF_OK;
 }
 
+// normalisation, the saved copy and gravity, then interface_diagnostics_last,
+// surface tension and the walls (sparse_sim3d_tp.cpp:865-874,
+// multires_sim3d_tp.cpp:683-694).  The face operations leave the masses the
+// diagnostics read unchanged, so they fuse into one pass unless the CSF increments
+// have to land between gravity and the walls.
+int step_face_stage(pf_ctx* c, const pf_phase* ph, double dt, double gravity, bool mr, bool diag, bool csf,
+                    double strength, double maxd, int radius, pf_step_stats* S) {
+  int st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | (csf ? 0 : FO_WALLS), (float)(dt * gravity));
+  if (st || !(diag || csf)) return st;
+  IfaceArgs ia;
+  st = iface_fields(c, ph, ia, mr);
+  if (st) return st;
+  if (diag) {
+    st = iface_reduce(c, ia, &S->iface);
+    if (st) return st;
+  }
+  if (!csf) return PF_OK;
+  st = surface_apply(c, ph, ia, dt, strength, maxd, radius, &S->surface);
+  if (st) return st;
+  return run_face_ops(c, FO_WALLS, 0.f);
+}
+
+// the defaults of InterfaceDiagnostics3D and SurfaceTensionStats3D
+// (interface_diagnostics3d.h:7-36), kept where a part is skipped
+void default_interface_stats(pf_step_stats* S) {
+  S->iface.finite = 1;
+  S->surface.finite = 1;
+  S->surface.capillary_stable = 1;
+}
+
+int interface_entry(pf_ctx* c, const pf_phase* ph, bool mr, IfaceArgs& a) {
+  if (mr && !c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");
+  if (!mr && c->mr) return fail(c, PF_ERR_STATE, "a multires context: use the pf_mr_ interface entry points");
+  return iface_fields(c, ph, a, mr);
+}
+
 }  // namespace
 
 int pf_interface_diagnostics(pf_ctx* c, const pf_phase* ph, pf_iface_stats* out) {
   if (!c || !ph || !out) return PF_ERR_ARG;
   IfaceArgs a;
-  int st = iface_fields(c, ph, a);
+  int st = interface_entry(c, ph, false, a);
   if (st) return st;
   return iface_reduce(c, a, out);
 }
@@ -2092,7 +2129,24 @@ int pf_apply_surface_tension(pf_ctx* c, const pf_phase* ph, double dt, double st
                              int32_t smoothing_radius, pf_surface_stats* out) {
   if (!c || !ph) return PF_ERR_ARG;
   IfaceArgs a;
-  int st = iface_fields(c, ph, a);
+  int st = interface_entry(c, ph, false, a);
+  if (st) return st;
+  return surface_apply(c, ph, a, dt, strength, max_delta_speed, smoothing_radius, out);
+}
+
+int pf_mr_interface_diagnostics(pf_ctx* c, const pf_phase* ph, pf_iface_stats* out) {
+  if (!c || !ph || !out) return PF_ERR_ARG;
+  IfaceArgs a;
+  int st = interface_entry(c, ph, true, a);
+  if (st) return st;
+  return iface_reduce(c, a, out);
+}
+
+int pf_mr_apply_surface_tension(pf_ctx* c, const pf_phase* ph, double dt, double strength, double max_delta_speed,
+                                int32_t smoothing_radius, pf_surface_stats* out) {
+  if (!c || !ph) return PF_ERR_ARG;
+  IfaceArgs a;
+  int st = interface_entry(c, ph, true, a);
   if (st) return st;
   return surface_apply(c, ph, a, dt, strength, max_delta_speed, smoothing_radius, out);
 }
@@ -2133,11 +2187,7 @@ int pf_step(pf_ctx* c, const pf_step_cfg* cfg, pf_step_stats* stats) {
   const bool single = cfg->model == PF_MODEL_SINGLE_PHASE;
   pf_step_stats S;
   std::memset(&S, 0, sizeof(S));
-  // the defaults of InterfaceDiagnostics3D and SurfaceTensionStats3D
-  // (interface_diagnostics3d.h:7-36), kept where a part is skipped
-  S.iface.finite = 1;
-  S.surface.finite = 1;
-  S.surface.capillary_stable = 1;
+  default_interface_stats(&S);
   const bool timing = cfg->timing != 0;
   record(c, timing, 0);
   int st = ensure_binned(c);
@@ -2154,30 +2204,11 @@ int pf_step(pf_ctx* c, const pf_step_cfg* cfg, pf_step_stats* stats) {
   st = run_p2g_splat(c, &cfg->phase, cfg->Vp, single);
   if (st) return st;
   record(c, timing, 3);
-  // interface_diagnostics_last, then the saved copy, gravity, surface tension and
-  // walls (sparse_sim3d_tp.cpp:865-874); the face operations leave the masses the
-  // diagnostics read unchanged, so they fuse into one pass unless the CSF
-  // increments have to land between gravity and the walls
-  const bool csf = !single && cfg->surface_tension;
-  const bool diag = !single && !cfg->skip_interface_diagnostics;
-  st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | (csf ? 0 : FO_WALLS), (float)(dt * cfg->gravity));
+  // the single-phase step has no interface work (sparse_sim3d.cpp:106-115)
+  st = step_face_stage(c, &cfg->phase, dt, cfg->gravity, false, !single && !cfg->skip_interface_diagnostics,
+                       !single && cfg->surface_tension, cfg->surface_tension_strength,
+                       cfg->surface_tension_max_delta_speed, cfg->surface_tension_smoothing_radius, &S);
   if (st) return st;
-  if (diag || csf) {
-    IfaceArgs ia;
-    st = iface_fields(c, &cfg->phase, ia);
-    if (st) return st;
-    if (diag) {
-      st = iface_reduce(c, ia, &S.iface);
-      if (st) return st;
-    }
-    if (csf) {
-      st = surface_apply(c, &cfg->phase, ia, dt, cfg->surface_tension_strength,
-                         cfg->surface_tension_max_delta_speed, cfg->surface_tension_smoothing_radius, &S.surface);
-      if (st) return st;
-      st = run_face_ops(c, FO_WALLS, 0.f);
-      if (st) return st;
-    }
-  }
   record(c, timing, 4);
   st = run_project(c, &cfg->phase, dt, &cfg->solve, cfg->c_div, &S.solve, timing, 5, 6, 7, single);
   if (st) return st;
@@ -2703,6 +2734,7 @@ int pf_mr_step(pf_ctx* c, const pf_mr_step_cfg* cfg, pf_step_stats* stats) {
     return fail(c, PF_ERR_ARG, "the multires path takes positions relative to the origin 0");
   pf_step_stats S;
   std::memset(&S, 0, sizeof(S));
+  default_interface_stats(&S);
   const bool timing = cfg->timing != 0;
   record(c, timing, 0);
   int st = mr_dynamic(c, &cfg->refine);
@@ -2721,7 +2753,9 @@ int pf_mr_step(pf_ctx* c, const pf_mr_step_cfg* cfg, pf_step_stats* stats) {
   st = run_p2g_splat(c, &cfg->phase, cfg->Vp, false, true);
   if (st) return st;
   record(c, timing, 3);
-  st = run_face_ops(c, FO_NORMALIZE | FO_SAVE | FO_GRAVITY | FO_WALLS, (float)(dt * cfg->gravity));
+  st = step_face_stage(c, &cfg->phase, dt, cfg->gravity, true, !cfg->skip_interface_diagnostics,
+                       cfg->surface_tension != 0, cfg->surface_tension_strength, cfg->surface_tension_max_delta_speed,
+                       cfg->surface_tension_smoothing_radius, &S);
   if (st) return st;
   record(c, timing, 4);
   st = mr_project_with(c, &cfg->phase, dt, &cfg->solve, cfg->c_div, &S.solve, timing, 5, 6, 7);

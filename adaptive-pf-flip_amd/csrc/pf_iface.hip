@@ -12,6 +12,12 @@
 // to each face of the cell (:404-440) -- a face adds its lower cell's share first,
 // as the reference's loop over sorted cells does, and no cell at or after the first
 // non-finite increment adds anything.
+//
+// Multires grids (diagnoseMRInterface3D / applyMRSurfaceTension3D, :381-401, :447-496)
+// use the same fine-grid stencils over the fine face patches (faces the layout does
+// not enumerate hold zero mass, as the reference's map lookups return), but sample
+// each FLUID leaf cell once, at its fine centre clamped into the grid, and visit
+// leaf cells in leafCells order: level, then block (bx, by, bz), then lz, ly, lx.
 #include "pf_internal.h"
 
 namespace pf {
@@ -30,6 +36,7 @@ struct IfaceArgs {
   double* gmag;     // |grad phi| per cell
   double* nrm[3];   // unit normal per cell
   double* kap;      // curvature (FLUID or interface cells)
+  const uint8_t* lvl;  // multires: leaf level per fine cell (255 = none); null on the sparse grid
 };
 
 __device__ inline void cell_of(const Dims& dc, long long q, int& i, int& j, int& k) {
@@ -44,6 +51,30 @@ __device__ inline void cell_of(const Dims& dc, long long q, int& i, int& j, int&
 __device__ inline int clampd(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ inline bool iface_phi(double p) { return p > kIfacePhiLo && p < kIfacePhiHi; }
+
+// whether fine cell (i, j, k) is where its leaf cell is sampled: every cell on the
+// sparse grid and in a level-0 leaf; in a level-1 leaf (2^3 fine cells from an even
+// corner) the centre corner + 1, clamped into the grid (mrCellFineCenter, :350-358)
+__device__ inline bool iface_sample_point(const IfaceArgs& a, long long q, int i, int j, int k) {
+  if (!a.lvl) return true;
+  const int L = a.lvl[q];
+  if (L == 0) return true;
+  if (L != 1) return false;
+  return i == min((i & ~1) + 1, a.g.nx - 1) && j == min((j & ~1) + 1, a.g.ny - 1) &&
+         k == min((k & ~1) + 1, a.g.nz - 1);
+}
+
+// the position of a sample point in the reference's visiting order: the linear cell
+// id on the sparse grid (collectCellsWithMarker sorts them), the leafCells rank on a
+// multires grid as (level, bx, by, bz, lz, ly, lx) packed into 64 bits
+__device__ inline unsigned long long iface_order(const IfaceArgs& a, long long q, int i, int j, int k) {
+  if (!a.lvl) return (unsigned long long)i + (unsigned long long)a.g.nx * (j + (unsigned long long)a.g.ny * k);
+  const int L = a.lvl[q];
+  const unsigned ci = (unsigned)i >> L, cj = (unsigned)j >> L, ck = (unsigned)k >> L;
+  return ((unsigned long long)L << 60) | ((unsigned long long)(ci >> 2) << 42) |
+         ((unsigned long long)(cj >> 2) << 24) | ((unsigned long long)(ck >> 2) << 6) |
+         (unsigned long long)((ci & 3) + 4 * ((cj & 3) + 4 * (ck & 3)));
+}
 
 // sparseCellPhi (:137-148)
 __global__ void __launch_bounds__(256) k_iface_phi(IfaceArgs a) {
@@ -143,6 +174,11 @@ __global__ void __launch_bounds__(256) k_iface_diag(IfaceArgs a, double* __restr
   const long long n = a.dc.nblocks * kBlockVol;
   for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
     if (a.marker[q] != CELL_FLUID) continue;
+    if (a.lvl) {
+      int i, j, k;
+      cell_of(a.dc, q, i, j, k);
+      if (!iface_sample_point(a, q, i, j, k)) continue;
+    }
     const double phi = a.phi[q], gm = a.gmag[q], kp = a.kap[q];
     if (!isfinite(phi) || !isfinite(gm) || !isfinite(kp)) {
       v[9] += 1.0;
@@ -204,7 +240,7 @@ struct CsfArgs {
   double dt, strength, maxd;
   int radius;
   double* delta[3];            // CSF increment per cell (NaN where none applies)
-  uint32_t* first_bad;         // least linear id of a non-finite increment
+  unsigned long long* first_bad;  // least visiting-order key of a non-finite increment
 };
 
 constexpr int kCsfSlots = 8;   // applied, |d| sum, |d| max, |raw| sum, |raw| max, |sm| sum, |sm| max, non-finite
@@ -219,10 +255,12 @@ __global__ void __launch_bounds__(256) k_csf_delta(CsfArgs c, double* __restrict
   const long long n = a.dc.nblocks * kBlockVol;
   for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
     double d[3] = {NAN, NAN, NAN};
+    int i = 0, j = 0, k = 0;
+    if (a.marker[q] == CELL_FLUID && a.lvl) cell_of(a.dc, q, i, j, k);
     // a NaN gradient passes the threshold test (:425-426) and ends as a zero increment
-    if (a.marker[q] == CELL_FLUID && iface_phi(a.phi[q]) && !(a.gmag[q] <= kIfaceGradEps)) {
-      int i, j, k;
-      cell_of(a.dc, q, i, j, k);
+    if (a.marker[q] == CELL_FLUID && iface_phi(a.phi[q]) && !(a.gmag[q] <= kIfaceGradEps) &&
+        (!a.lvl || iface_sample_point(a, q, i, j, k))) {
+      if (!a.lvl) cell_of(a.dc, q, i, j, k);
       double gr[3];
       gradient_at(a, i, j, k, gr);
       const double raw = a.kap[q];
@@ -261,7 +299,7 @@ __global__ void __launch_bounds__(256) k_csf_delta(CsfArgs c, double* __restrict
       const double mag = vlen3(d);
       if (!isfinite(mag) || !isfinite(raw) || !isfinite(sm)) {
         v[7] += 1.0;
-        atomicMin(c.first_bad, (uint32_t)(i + a.g.nx * (j + a.g.ny * k)));
+        atomicMin(c.first_bad, iface_order(a, q, i, j, k));
         d[0] = d[1] = d[2] = NAN;
       } else {
         v[0] += 1.0;
@@ -299,9 +337,9 @@ __global__ void __launch_bounds__(256) k_csf_delta(CsfArgs c, double* __restrict
   }
 }
 
-// the faces of one axis: u += (float)(0.5 d) from the lower cell, then from the
-// upper cell, where the face mass is positive, the cell's increment applies (not
-// NaN) and the cell precedes the first non-finite increment
+// the faces of one axis: u += (float)(0.5 d) from the two cells beside the face, in
+// the reference's visiting order, where the face mass is positive, the cell's
+// increment applies (not NaN) and the cell precedes the first non-finite increment
 __global__ void __launch_bounds__(256) k_csf_faces(CsfArgs c, int axis, float* __restrict__ vel,
                                                    const float* __restrict__ mass) {
   const IfaceArgs& a = c.f;
@@ -312,23 +350,28 @@ __global__ void __launch_bounds__(256) k_csf_faces(CsfArgs c, int axis, float* _
   cell_of(fd, q, i, j, k);
   if (i >= fd.nx || j >= fd.ny || k >= fd.nz) return;
   if (!(mass[q] > 0.0f)) return;
-  const uint32_t bad = *c.first_bad;
+  const unsigned long long bad = *c.first_bad;
   const int li = i - (axis == 0), lj = j - (axis == 1), lk = k - (axis == 2);
-  float u = vel[q];
+  double d0 = NAN, d1 = NAN;
+  unsigned long long k0 = ~0ull, k1 = ~0ull;
   if (li >= 0 && lj >= 0 && lk >= 0) {
-    const uint32_t id = (uint32_t)(li + a.g.nx * (lj + a.g.ny * lk));
-    if (id < bad) {
-      const double d = c.delta[axis][bm_index(a.dc, li, lj, lk)];
-      if (!isnan(d)) u = u + (float)(0.5 * d);
-    }
+    const long long cq = bm_index(a.dc, li, lj, lk);
+    k0 = iface_order(a, cq, li, lj, lk);
+    if (k0 < bad) d0 = c.delta[axis][cq];
   }
   if (i < a.g.nx && j < a.g.ny && k < a.g.nz) {
-    const uint32_t id = (uint32_t)(i + a.g.nx * (j + a.g.ny * k));
-    if (id < bad) {
-      const double d = c.delta[axis][bm_index(a.dc, i, j, k)];
-      if (!isnan(d)) u = u + (float)(0.5 * d);
-    }
+    const long long cq = bm_index(a.dc, i, j, k);
+    k1 = iface_order(a, cq, i, j, k);
+    if (k1 < bad) d1 = c.delta[axis][cq];
   }
+  if (k1 < k0) {
+    const double t = d0;
+    d0 = d1;
+    d1 = t;
+  }
+  float u = vel[q];
+  if (!isnan(d0)) u = u + (float)(0.5 * d0);
+  if (!isnan(d1)) u = u + (float)(0.5 * d1);
   vel[q] = u;
 }
 

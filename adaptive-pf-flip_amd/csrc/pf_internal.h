@@ -213,7 +213,7 @@ struct Ctx {
   // interface fields and CSF increments (FP64 per cell of the slab: phi, |grad phi|,
   // three normal components, curvature, three increments), allocated on first use
   double* iface = nullptr;
-  uint32_t* iface_bad = nullptr;    // least linear cell id with a non-finite increment
+  unsigned long long* iface_bad = nullptr;  // least visiting-order key of a non-finite increment
 
   // per-kernel event timing (measurement hook)
   uint32_t ktime = 0;                // kernels (bit PF_K_*) bracketed with events

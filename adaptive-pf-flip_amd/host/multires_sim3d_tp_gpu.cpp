@@ -251,6 +251,11 @@ void MRSim3DTP::step() {
   cfg.refine.gas_padding = dynamic_gas_padding;
   cfg.refine.hysteresis_cells = dynamic_hysteresis_cells;
   cfg.refine.max_fine_leaves = dynamic_max_fine_leaves;
+  cfg.surface_tension = surface_tension ? 1 : 0;
+  cfg.surface_tension_strength = surface_tension_strength;
+  cfg.surface_tension_max_delta_speed = surface_tension_max_delta_speed;
+  cfg.surface_tension_smoothing_radius = surface_tension_curvature_smoothing_radius;
+  cfg.skip_interface_diagnostics = interface_diagnostics ? 0 : 1;
   // updateVolumeCorrectionStats (multires_sim3d_tp.cpp:439-450) needs this step's dt
   double stepDt = dt;
   if (adaptive_timestep) {
@@ -280,6 +285,8 @@ void MRSim3DTP::step() {
   max_particle_speed_last = stats_last.max_particle_speed;
   cfl_limit_dt_last = stats_last.cfl_limit_dt;
   adaptive_timestep_limited_last = stats_last.limited;
+  interface_diagnostics_last = stats_last.iface;
+  surface_tension_stats_last = stats_last.surface;
   const pf_solve_stats& s = stats_last.solve;
   last_pressure_stats.active_cells = (int)s.fluid_cells;
   last_pressure_stats.pinned_cell = (int)s.pinned_cell;

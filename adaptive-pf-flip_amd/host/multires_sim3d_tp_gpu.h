@@ -5,8 +5,8 @@
 // Keeps the fields step() reads and the per-step stats it writes under the
 // reference's names, the layout (as MRLayout3D leaf blocks), initBubbleTankInterfaceBand,
 // updateDynamicRefinement and step().  Particles live on the device between steps
-// (see sparse_sim3d_tp_gpu.h); the default particle adaptivity, secondary-particle
-// and surface-tension flags are off, as in the reference.
+// (see sparse_sim3d_tp_gpu.h); particle adaptivity and secondary particles are not
+// built, and surface tension is off by default, as in the reference.
 #pragma once
 
 #include <array>
@@ -84,6 +84,14 @@ class MRSim3DTP {
   int dynamic_retained_x0 = 0, dynamic_retained_y0 = 0, dynamic_retained_z0 = 0;
   int dynamic_retained_x1 = 0, dynamic_retained_y1 = 0, dynamic_retained_z1 = 0;
   int particle_boundary_clamped_liquid_last = 0, particle_boundary_clamped_gas_last = 0;
+  InterfaceDiagnostics3D interface_diagnostics_last = defaultInterfaceDiagnostics();
+  bool surface_tension = false;
+  double surface_tension_strength = 0.02;
+  double surface_tension_max_delta_speed = 0.25;
+  int surface_tension_curvature_smoothing_radius = 0;
+  SurfaceTensionStats3D surface_tension_stats_last = defaultSurfaceTensionStats();
+  // false: skip the per-step interface statistics (the reference always computes them)
+  bool interface_diagnostics = true;
   MRPressureSolveStats3D last_pressure_stats;
   pf_step_stats stats_last{};
 

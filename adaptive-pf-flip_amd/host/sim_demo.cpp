@@ -293,6 +293,33 @@ int main() {
                 st.coarse_correction_accepted_sweeps, st.coarse_preconditioner_applications,
                 st.coarse_preconditioner_accepted_applications);
   }
+  {
+    // surface tension through the class fields (sparse and multires): the step
+    // reports interface diagnostics every step and CSF statistics when enabled
+    SparseSim3DTP sim(16, 16, 16, 1.0);
+    sim.dt = 0.01;
+    sim.initBubbleTank();
+    sim.step();
+    CHECK(sim.interface_diagnostics_last.interface_cells > 0 && sim.interface_diagnostics_last.finite == 1);
+    CHECK(sim.surface_tension_stats_last.enabled == 0 && sim.surface_tension_stats_last.finite == 1);
+    sim.surface_tension = true;
+    sim.surface_tension_strength = 0.5;
+    sim.surface_tension_curvature_smoothing_radius = 1;
+    sim.step();
+    const auto& st = sim.surface_tension_stats_last;
+    CHECK(st.enabled == 1 && st.applied_cells > 0 && st.finite == 1 && st.curvature_smoothing_radius == 1);
+    CHECK(st.max_delta_speed <= sim.surface_tension_max_delta_speed);
+    MRSim3DTP mr(16, 16, 16, 1.0);
+    mr.dt = 0.01;
+    mr.surface_tension = true;
+    mr.initBubbleTankInterfaceBand();
+    mr.step();
+    CHECK(mr.interface_diagnostics_last.interface_cells > 0);
+    CHECK(mr.surface_tension_stats_last.enabled == 1 && mr.surface_tension_stats_last.applied_cells > 0);
+    std::printf("surface tension: sparse %d interface cells, %d CSF cells (max |dv| %.3e); multires %d / %d\n",
+                sim.interface_diagnostics_last.interface_cells, st.applied_cells, st.max_delta_speed,
+                mr.interface_diagnostics_last.interface_cells, mr.surface_tension_stats_last.applied_cells);
+  }
   std::printf(failures ? "FAILED %d checks\n" : "all checks passed\n", failures);
   return failures ? 1 : 0;
 }

@@ -139,7 +139,10 @@ class MRStepCfg(C.Structure):
                 ("adaptive_cfl", C.c_double), ("adaptive_min_dt", C.c_double),
                 ("c_div", C.c_double), ("adaptive_timestep", C.c_int32),
                 ("advection_order", C.c_int32), ("timing", C.c_int32), ("reserved", C.c_int32),
-                ("phase", Phase), ("solve", MRSolveCfg), ("refine", MRRefineCfg)]
+                ("phase", Phase), ("solve", MRSolveCfg), ("refine", MRRefineCfg),
+                ("surface_tension", C.c_int32), ("surface_tension_smoothing_radius", C.c_int32),
+                ("surface_tension_strength", C.c_double), ("surface_tension_max_delta_speed", C.c_double),
+                ("skip_interface_diagnostics", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class IfaceStats(C.Structure):
@@ -207,6 +210,9 @@ def lib():
             "pf_interface_diagnostics": (C.c_int, [vp, C.POINTER(Phase), C.POINTER(IfaceStats)]),
             "pf_apply_surface_tension": (C.c_int, [vp, C.POINTER(Phase), dbl, dbl, dbl, i32,
                                                    C.POINTER(SurfaceStats)]),
+            "pf_mr_interface_diagnostics": (C.c_int, [vp, C.POINTER(Phase), C.POINTER(IfaceStats)]),
+            "pf_mr_apply_surface_tension": (C.c_int, [vp, C.POINTER(Phase), dbl, dbl, dbl, i32,
+                                                      C.POINTER(SurfaceStats)]),
             "pf_field_dims": (C.c_int, [vp, i32, vp]),
             "pf_field_download": (C.c_int, [vp, i32, vp]),
             "pf_field_upload": (C.c_int, [vp, i32, vp, vp]),
@@ -719,6 +725,22 @@ class MRContext(Context):
         self._check(lib().pf_mr_last_solve_detail(self.h, C.byref(d)), "pf_mr_last_solve_detail")
         return d
 
+    def mr_interface_diagnostics(self, phase: Phase) -> IfaceStats:
+        """diagnoseMRInterface3D over the current masses and leaf markers"""
+        out = IfaceStats()
+        self._check(lib().pf_mr_interface_diagnostics(self.h, C.byref(phase), C.byref(out)),
+                    "pf_mr_interface_diagnostics")
+        return out
+
+    def mr_apply_surface_tension(self, phase: Phase, dt, strength=0.02, max_delta_speed=0.25,
+                                 smoothing_radius=0) -> SurfaceStats:
+        """applyMRSurfaceTension3D on the current face velocities"""
+        out = SurfaceStats()
+        self._check(lib().pf_mr_apply_surface_tension(self.h, C.byref(phase), dt, strength, max_delta_speed,
+                                                      int(smoothing_radius), C.byref(out)),
+                    "pf_mr_apply_surface_tension")
+        return out
+
     def mr_step(self, cfg):
         st = StepStats()
         self._check(lib().pf_mr_step(self.h, C.byref(cfg), C.byref(st)), "pf_mr_step")
@@ -743,6 +765,15 @@ class MRSim3DTP:
         # the reference's other MRPressureSolveConfig3D options by field name, e.g.
         # {"mode": MR_SOLVE_REFERENCE, "use_coarse_correction": 1}
         self.solve_options = {}
+        # CSF surface tension (multires_sim3d_tp.h:170-175)
+        self.surface_tension = False
+        self.surface_tension_strength = 0.02
+        self.surface_tension_max_delta_speed = 0.25
+        self.surface_tension_curvature_smoothing_radius = 0
+        # False skips the per-step interface statistics (the reference always computes them)
+        self.interface_diagnostics = True
+        self.interface_diagnostics_last = IfaceStats(finite=1)
+        self.surface_tension_stats_last = SurfaceStats(finite=1, capillary_stable=1)
         self.c_div_last = 0.0
         self.timing = False
         self.stats_last = None
@@ -761,10 +792,17 @@ class MRSim3DTP:
                          alpha_gas=self.alpha_gas, adaptive_cfl=self.adaptive_cfl,
                          adaptive_min_dt=self.adaptive_min_dt, c_div=self.c_div_last,
                          adaptive_timestep=int(self.adaptive_timestep), advection_order=self.advection_order,
-                         timing=int(self.timing), reserved=0, phase=self.phase, solve=solve, refine=refine)
+                         timing=int(self.timing), reserved=0, phase=self.phase, solve=solve, refine=refine,
+                         surface_tension=int(self.surface_tension),
+                         surface_tension_smoothing_radius=self.surface_tension_curvature_smoothing_radius,
+                         surface_tension_strength=self.surface_tension_strength,
+                         surface_tension_max_delta_speed=self.surface_tension_max_delta_speed,
+                         skip_interface_diagnostics=int(not self.interface_diagnostics))
 
     def step(self):
         """MRSim3DTP::step, src/driver/multires_sim3d_tp.cpp:671-734."""
         st = self.ctx.mr_step(self.step_cfg())
+        self.interface_diagnostics_last = st.iface
+        self.surface_tension_stats_last = st.surface
         self.stats_last = st
         return st

@@ -327,6 +327,11 @@ typedef struct pf_mr_step_cfg {     /* the MRSim3DTP fields step() reads */
   pf_phase phase;
   pf_mr_solve_cfg solve;
   pf_mr_refine_cfg refine;
+  /* CSF surface tension and the diagnostics switch, as in pf_step_cfg (MRSim3DTP
+   * fields, multires_sim3d_tp.h:170-175) */
+  int32_t surface_tension, surface_tension_smoothing_radius;
+  double surface_tension_strength, surface_tension_max_delta_speed;
+  int32_t skip_interface_diagnostics, reserved2;
 } pf_mr_step_cfg;
 
 /* MRSim3DTP's solver defaults (multires_sim3d_tp.h:34-60) in PF_MR_SOLVE_FAST mode:
@@ -371,8 +376,15 @@ int pf_mr_update_refinement(pf_ctx* ctx, const pf_mr_refine_cfg* cfg, int32_t* f
 /* the refinement box the last update kept (MRSim3DTP::dynamic_retained_box_valid and
  * dynamic_retained_x0 .. z1, multires_sim3d_tp.h:68-74) as x0, y0, z0, x1, y1, z1 */
 int pf_mr_retained_box(pf_ctx* ctx, int32_t* box, int32_t* valid);
-/* MRSim3DTP::step, multires_sim3d_tp.cpp:671-734 (default flags: no particle
- * adaptivity, secondary particles or surface tension) */
+/* diagnoseMRInterface3D and applyMRSurfaceTension3D, interface_diagnostics3d.cpp:381-401,
+ * :447-496 / interface_diagnostics3d.h:41, 51: the pf_interface_diagnostics /
+ * pf_apply_surface_tension stencils on the fine face patches, one sample per FLUID
+ * leaf cell at its fine centre, in the reference's leaf order */
+int pf_mr_interface_diagnostics(pf_ctx* ctx, const pf_phase* phase, pf_iface_stats* out);
+int pf_mr_apply_surface_tension(pf_ctx* ctx, const pf_phase* phase, double dt, double strength,
+                                double max_delta_speed, int32_t smoothing_radius, pf_surface_stats* out);
+/* MRSim3DTP::step, multires_sim3d_tp.cpp:671-734 (no particle adaptivity or secondary
+ * particles; interface diagnostics and optional surface tension as the reference) */
 int pf_mr_step(pf_ctx* ctx, const pf_mr_step_cfg* cfg, pf_step_stats* stats);
 
 /* ---- learned block preconditioner (no reference counterpart: the reference

@@ -632,6 +632,13 @@ class MRGrid:
         lib().orc_mr_advect(self.h, len(ps), ps.x, ps.y, ps.z, ps.type, dt, order, C.byref(es))
         return es
 
+    def load_dense(self, f, a):
+        """write the non-zero entries of a dense [z, y, x] array: fine faces, or for
+        ORC_P / ORC_MK the leaf cell holding each fine cell"""
+        a = np.asarray(a)
+        for k, j, i in zip(*np.nonzero(a)):
+            self.set(f, int(i), int(j), int(k), a[k, j, i])
+
     def interface_diagnostics(self, phase):
         st = IfaceStats()
         lib().orc_mr_interface_diagnostics(self.h, C.byref(phase), C.byref(st))

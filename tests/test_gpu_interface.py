@@ -5,17 +5,20 @@ Both sides start from the same FP32 face masses, velocities and markers (the GPU
 own P2G output copied into the oracle grid), so every per-cell value is computed
 from identical inputs in the same FP64 operation order.  Bars: counts, minima and
 maxima exact; sums and means within 1e-12 relative (the GPU adds in another order);
-face velocities after the CSF increments bit-exact.  Through a whole step the GPU's
-FP32 P2G differs from the FP64 oracle's, so particle state is held to the same
-1e-4 norm-wise bar as the step without surface tension, and the step's statistics
-are recomputed by the oracle from the GPU's own post-step masses.
+face velocities after the CSF increments bit-exact.  The multires variants sample
+each FLUID leaf cell once and follow the reference's leaf order; they are held to the
+same bars on two-level layouts with the interface in fine and in coarse cells.
+Through a whole step the GPU's FP32 P2G differs from the FP64 oracle's, so particle
+state is held to the same 1e-4 norm-wise bar as the step without surface tension,
+and the step's statistics are recomputed by the oracle from the GPU's own post-step
+masses.
 """
 import numpy as np
 import pytest
 
 import pfflip_amd as G
 import pyoracle as O
-from _helpers import phase, rel_err, scene, step_cfg
+from _helpers import bubble_lattice, phase, rel_err, scene, step_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -209,3 +212,108 @@ def test_step_diagnostics_flag_changes_only_the_statistics():
     assert s1.surface.enabled == 0 and s1.surface.finite == 1 and s1.surface.capillary_stable == 1
     for a, b in zip(p1, p2):
         assert np.array_equal(a, b)
+
+
+# ---- multires variants (diagnoseMRInterface3D, applyMRSurfaceTension3D)
+
+# fine boxes on a level-1 base for a 32^3 bubble tank (water below y = 16): the
+# interface band fine, liquid below y = 8 coarse, the water surface in coarse cells
+MR_BOXES = {"band": None, "mid": [0, 8, 0, 32, 20, 32], "low": [0, 0, 0, 32, 8, 32]}
+
+
+def _mr_leaves(n, box):
+    return O.mr_layout_bubble_band(n, n, n) if MR_BOXES[box] is None else O.mr_layout_fine_box(n, n, n, MR_BOXES[box])
+
+
+def _mr_after_p2g(box, n=32, seed=3):
+    print(f"[mr-iface] box={box} n={n} seed={seed}", flush=True)
+    leaves = _mr_leaves(n, box)
+    ctx = G.MRContext(n, n, n, 1.0)
+    ctx.set_layout(leaves)
+    ctx.upload(*bubble_lattice(n, seed))
+    oph = phase(1.0, 0.01)
+    ctx.mr_mark_cells()
+    ctx.mr_p2g(_gphase(oph), 1.0)
+    return ctx, oph, leaves
+
+
+def _mr_oracle_copy(ctx, n, leaves, fields=(0, 1, 2, 3, 4, 5)):
+    g = O.MRGrid(n, n, n, 1.0, faithful=True)
+    g.set_leaves(leaves)
+    for f in fields:
+        g.load_dense(f, ctx.field(f).astype(np.float64))
+    g.load_dense(O.MK, ctx.field(G.FIELD_MARKER).astype(np.float64))
+    return g
+
+
+@pytest.mark.parametrize("box", sorted(MR_BOXES))
+def test_mr_interface_diagnostics_match_oracle(box):
+    n = 32
+    ctx, oph, leaves = _mr_after_p2g(box, n)
+    g = _mr_oracle_copy(ctx, n, leaves, fields=(3, 4, 5))
+    gs = ctx.mr_interface_diagnostics(_gphase(oph))
+    assert gs.interface_cells > 0
+    _check_iface(gs, g.interface_diagnostics(oph))
+
+
+@pytest.mark.parametrize("box,strength,maxd,radius", [("band", 0.02, 0.25, 0), ("low", 0.5, 0.01, 1),
+                                                      ("mid", 2.0, 0.0, 2), ("low", 0.1, 0.2, 3)])
+def test_mr_surface_tension_matches_oracle(box, strength, maxd, radius):
+    n = 32
+    ctx, oph, leaves = _mr_after_p2g(box, n, seed=5)
+    g = _mr_oracle_copy(ctx, n, leaves)
+    gs = ctx.mr_apply_surface_tension(_gphase(oph), 0.01, strength, maxd, radius)
+    assert gs.applied_cells > 0
+    _check_surface(gs, g.surface_tension(oph, 0.01, strength, maxd, radius))
+    for f in range(3):
+        assert np.array_equal(ctx.field(f).astype(np.float64), g.dense(f)), f
+
+
+def test_interface_entry_points_check_the_grid_kind():
+    n = 16
+    ctx, oph, leaves = _mr_after_p2g("band", n)
+    with pytest.raises(G.PfError):
+        ctx.interface_diagnostics(_gphase(oph))
+    sp = G.MRContext(n, n, n, 1.0)      # no layout set: a sparse context
+    with pytest.raises(G.PfError):
+        sp.mr_interface_diagnostics(_gphase(oph))
+
+
+def test_mr_step_with_surface_tension_matches_oracle():
+    n, dt = 32, 0.02
+    print(f"[mr-iface-step] n={n}", flush=True)
+    leaves = _mr_leaves(n, "low")
+    parts = bubble_lattice(n, vel=False)
+    oph = phase(1.0, 0.01)
+    ctx = G.MRContext(n, n, n, 1.0)
+    ctx.set_layout(leaves)
+    ctx.upload(*parts)
+    sim = G.MRSim3DTP(n, n, n, 1.0)
+    sim.ctx = ctx
+    sim.phase = _gphase(oph)
+    sim.dt = dt
+    sim.dynamic_refinement = False
+    sim.cg_iters, sim.cg_tol, sim.cg_rel_tol = 20000, 1e-11, 0.0
+    sim.surface_tension, sim.surface_tension_strength = True, 0.3
+    sim.surface_tension_max_delta_speed, sim.surface_tension_curvature_smoothing_radius = 0.1, 1
+    osim = O.mr_sim(n, n, n, 1.0, phase=oph, dt=dt, dynamic=0,
+                    solve=O.mr_solve_cfg(max_iterations=20000, abs_tol=1e-12, rel_tol=0.0))
+    osim.surface_tension, osim.surface_tension_strength = 1, 0.3
+    osim.surface_tension_max_delta_speed, osim.surface_tension_smoothing_radius = 0.1, 1
+    go = O.MRGrid(n, n, n, 1.0, faithful=False)
+    go.set_leaves(leaves)
+    x, y, z, u, v, w, t = parts
+    pa = O.Particles(x, y, z, u, v, w, type=t)
+    for _ in range(2):
+        st = sim.step()
+        ost = O.mr_step(go, osim, pa)
+        assert st.solve.converged == 1
+        assert st.surface.enabled == 1 and st.surface.applied_cells > 0 and st.iface.interface_cells > 0
+        g = _mr_oracle_copy(ctx, n, leaves, fields=(3, 4, 5))
+        _check_iface(st.iface, g.interface_diagnostics(oph))
+        _check_surface(st.surface, g.surface_tension(oph, st.effective_dt, 0.3, 0.1, 1))
+        assert abs(st.surface.applied_cells - ost.surface.applied_cells) <= 0.02 * ost.surface.applied_cells
+        assert sim.surface_tension_stats_last.applied_cells == st.surface.applied_cells
+    gx, gy, gz, gu, gv, gw = ctx.download()
+    assert max(np.abs(gx - pa.x).max(), np.abs(gy - pa.y).max(), np.abs(gz - pa.z).max()) < 1e-5
+    assert rel_err(np.concatenate([gu, gv, gw]), np.concatenate([pa.u, pa.v, pa.w])) < 1e-4

@@ -10,7 +10,7 @@ import pytest
 
 import pfflip_amd as G
 import pyoracle as O
-from _helpers import COARSE_CASES, rel_err
+from _helpers import COARSE_CASES, bubble_lattice as _bubble_particles, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -21,27 +21,6 @@ def _phase(rho_l=1.0, rho_g=0.01):
     ph = O.Phase(rho_l, rho_g, 1.0, 1.0)
     ph.rho_tilde_0 = O.calibrate_rho_tilde0(ph, 1.0)
     return ph, G.Phase(ph.rho_l, ph.rho_g, ph.alpha_phi, ph.rho_tilde_0)
-
-
-def _bubble_particles(n, seed=3, vel=True):
-    """the reference's bubble-tank lattice (multires_sim3d_tp.cpp:483-496), FP32 values"""
-    water = n // 2
-    cx, cy, cz = n * 0.5, water * 0.375, n * 0.5
-    r = max(1.25, n * 0.1875)
-    pts = []
-    for k in range(1, n - 1):
-        for j in range(1, water):
-            for i in range(1, n - 1):
-                gas = (i + 0.5 - cx) ** 2 + (j + 0.5 - cy) ** 2 + (k + 0.5 - cz) ** 2 < r * r
-                for s in range(8):
-                    pts.append((i + 0.25 + 0.5 * (s & 1), j + 0.25 + 0.5 * ((s >> 1) & 1),
-                                k + 0.25 + 0.5 * ((s >> 2) & 1), 1 if gas else 0))
-    p = np.array(pts)
-    rng = np.random.default_rng(seed)
-    x, y, z = (p[:, q].astype(np.float32) for q in range(3))
-    u, v, w = ((rng.normal(scale=0.3, size=len(p)) if vel else np.zeros(len(p))).astype(np.float32) for _ in range(3))
-    t = p[:, 3].astype(np.uint8)
-    return x, y, z, u, v, w, t
 
 
 def _ctx(n, leaves, parts):

@@ -149,3 +149,146 @@ def test_step_with_surface_tension_bit_exact():
         pr = rs.get_particles(pa)
         for a in ("x", "y", "z", "u", "v", "w"):
             assert np.array_equal(getattr(pa, a), getattr(pr, a)), a
+
+
+# ---- multires variants (diagnoseMRInterface3D, applyMRSurfaceTension3D)
+
+# fine boxes on a level-1 base for the reference's bubble tank at 32^3: its own
+# interface band (FLUID leaves all fine), liquid below y = 8 coarse, and the water
+# surface and bubble in coarse cells
+MR_BOXES = {"band": None, "mid": [0, 8, 0, 32, 20, 32], "low": [0, 0, 0, 32, 8, 32]}
+
+
+def _mr_case(n=32, seed=3, box="band"):
+    """a bubble-tank scene on a two-level layout after P2G on the reference build
+    and the oracle, markers shared"""
+    r = O.RefMR(n, n, n, 1.0)
+    r.init_bubble()
+    if MR_BOXES[box] is not None:
+        r.set_leaves(O.mr_layout_fine_box(n, n, n, MR_BOXES[box]))
+    ph = phase(1.0, 0.01)
+    r.config(dt=0.02, phase=ph)
+    ps = r.particles()
+    rng = np.random.default_rng(seed)
+    for a in (ps.u, ps.v, ps.w):
+        a[:] = rng.normal(scale=0.5, size=len(ps))
+    r.set_particles(ps)
+    g = O.MRGrid(n, n, n, 1.0, faithful=True)
+    g.set_leaves(r.leaves())
+    r.p2g()
+    g.p2g(ps, ph, 1.0)
+    g.mark_cells(ps)
+    mk = g.dense(O.MK)
+    for k, j, i in zip(*np.nonzero(mk)):
+        r.set(O.MK, int(i), int(j), int(k), float(mk[k, j, i]))
+    return g, r, ph, ps
+
+
+def test_mr_on_a_uniform_fine_layout_is_the_sparse_diagnostic():
+    # with every leaf at level 0 the multires grid is the sparse grid: the same cells
+    # are sampled at the same points, only the summation order differs
+    n = 16
+    dx, x, y, z, t = scene(O.SCENE_BUBBLE, n, seed=6, ny=n)
+    ph = phase()
+    ps = O.Particles(np.asarray(x) / dx, np.asarray(y) / dx, np.asarray(z) / dx, type=t)
+    g = O.Grid(n, n, n, 1.0, faithful=True)
+    g.mark_cells(ps)
+    g.p2g(ps, ph, 1.0)
+    m = O.MRGrid(n, n, n, 1.0, faithful=True)
+    m.set_leaves(O.mr_layout_fine_box(n, n, n, [0, 0, 0, n, n, n]))
+    assert set(m.leaves()[:, 0]) == {0}
+    for f in (O.MU, O.MV, O.MW, O.U, O.V, O.W):
+        a = g.dense(f)
+        for k, j, i in zip(*np.nonzero(a)):
+            m.set(f, int(i), int(j), int(k), a[k, j, i])
+    mk = g.dense(O.MK)
+    for k, j, i in zip(*np.nonzero(mk)):
+        m.set(O.MK, int(i), int(j), int(k), mk[k, j, i])
+    s, q = g.interface_diagnostics(ph), m.interface_diagnostics(ph)
+    assert s.interface_cells > 0
+    for f in ("sample_cells", "interface_cells", "finite", "phi_min", "phi_max", "grad_max", "curvature_abs_max"):
+        assert getattr(s, f) == getattr(q, f), f
+    for f in ("phi_mean", "grad_mean", "curvature_abs_mean"):
+        assert getattr(q, f) == pytest.approx(getattr(s, f), rel=1e-12)
+    a, b = g.surface_tension(ph, 0.01, 0.5, 0.1, 1), m.surface_tension(ph, 0.01, 0.5, 0.1, 1)
+    assert a.applied_cells == b.applied_cells > 0 and a.max_delta_speed == b.max_delta_speed
+    for f in (O.U, O.V, O.W):
+        assert np.array_equal(g.dense(f), m.dense(f)), f
+
+
+@needs_ref
+@pytest.mark.parametrize("box", ["mid", "low"])
+def test_mr_samples_each_fluid_leaf_cell_once(box):
+    g, r, ph, ps = _mr_case(box=box)
+    lv = g.leaves()
+    assert set(lv[:, 0]) == {0, 1}
+    # FLUID leaf cells: markers are stored per leaf cell, so count distinct leaf cells
+    mk = g.dense(O.MK)
+    n = g.nx
+    level = np.full((n, n, n), -1)
+    for L, bx, by, bz in lv:
+        s = 4 << L
+        level[bz * s:(bz + 1) * s, by * s:(by + 1) * s, bx * s:(bx + 1) * s] = L
+    kk, jj, ii = np.nonzero(mk == 1)
+    leaf_cells = {(int(i) >> level[k, j, i], int(j) >> level[k, j, i], int(k) >> level[k, j, i], level[k, j, i])
+                  for k, j, i in zip(kk, jj, ii)}
+    st = g.interface_diagnostics(ph)
+    assert st.finite == 1 and st.sample_cells == len(leaf_cells)
+    assert any(c[3] == 1 for c in leaf_cells) and any(c[3] == 0 for c in leaf_cells)
+
+
+@needs_ref
+@pytest.mark.parametrize("box", sorted(MR_BOXES))
+def test_mr_interface_diagnostics_bit_exact(box):
+    g, r, ph, ps = _mr_case(box=box)
+    for f in range(6):
+        assert np.array_equal(g.dense(f), r.dense(f))
+    st = g.interface_diagnostics(ph)
+    assert st.interface_cells > 0
+    assert st.values() == list(r.interface_diagnostics())
+
+
+@needs_ref
+@pytest.mark.parametrize("box,strength,maxd,radius", [("band", 0.02, 0.25, 0), ("low", 0.5, 0.01, 1),
+                                                      ("mid", 2.0, 0.0, 2), ("low", 0.1, 0.2, 3)])
+def test_mr_surface_tension_bit_exact(box, strength, maxd, radius):
+    g, r, ph, ps = _mr_case(box=box)
+    st = g.surface_tension(ph, 0.01, strength, maxd, radius)
+    out = r.surface_tension(0.01, strength, maxd, radius)
+    assert st.applied_cells > 0
+    assert st.values() == list(out)
+    for f in range(3):
+        assert np.array_equal(g.dense(f), r.dense(f)), f
+
+
+@needs_ref
+def test_mr_step_with_surface_tension_bit_exact():
+    n = 32
+    r = O.RefMR(n, n, n, 1.0)
+    r.init_bubble()
+    # a fixed layout with the water surface in coarse cells (no dynamic refinement)
+    r.set_leaves(O.mr_layout_fine_box(n, n, n, MR_BOXES["low"]))
+    ph = phase(1.0, 0.01)
+    r.config(dt=0.02, phase=ph, dynamic=0)
+    r.surface_flags(1, 0.3, 0.1, 1)
+    ps = r.particles()
+    g = O.MRGrid(n, n, n, 1.0, faithful=True)
+    g.set_leaves(r.leaves())
+    sim = O.mr_sim(n, n, n, 1.0, phase=ph, dynamic=0)
+    sim.surface_tension, sim.surface_tension_strength = 1, 0.3
+    sim.surface_tension_max_delta_speed, sim.surface_tension_smoothing_radius = 0.1, 1
+    for _ in range(3):
+        r.step()
+        st = O.mr_step(g, sim, ps)
+        di, sr = r.last_interface()
+        assert st.iface.values() == list(di)
+        assert st.surface.values() == list(sr) and st.surface.applied_cells > 0
+        rp = r.particles()
+        for a in ("x", "y", "z", "u", "v", "w"):
+            assert np.array_equal(getattr(ps, a), getattr(rp, a)), a
+    # surface tension off: SurfaceTensionStats3D() defaults in the step's stats
+    sim.surface_tension = 0
+    r.surface_flags(0)
+    r.step()
+    st = O.mr_step(g, sim, ps)
+    assert st.surface.values() == list(r.last_interface()[1])
