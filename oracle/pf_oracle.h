@@ -285,6 +285,46 @@ void orc_mr_surface_tension(orc_mr* g, const orc_phase* pp, double dt, double st
 void orc_mr_step(orc_mr* g, orc_mr_sim* sim, int64_t n, double* x, double* y, double* z, double* u,
                  double* v, double* w, const uint8_t* type, const double* vol, orc_mr_step_stats* st);
 
+/* ---- particle adaptivity (pf_oracle_adapt.c): src/driver/particle_adaptivity3d.h */
+typedef struct orc_pa_domain {            /* ParticleCellDomain3D, :13-48 */
+  int32_t nx, ny, nz, pad0;
+  double dx, ox, oy, oz;
+} orc_pa_domain;
+
+typedef struct orc_adapt_cfg {            /* SparseSim3DTP fields, sparse_sim3d_tp.h:29-60 */
+  int32_t narrow_band_air, narrow_band_air_radius;
+  int32_t gas_particle_coarsening, gas_particles_per_cell_target;
+  uint32_t gas_particle_coarsening_seed;
+  int32_t liquid_particle_coarsening, liquid_particles_per_cell_target;
+  uint32_t liquid_particle_coarsening_seed;
+  int32_t liquid_particle_refill, liquid_particle_refill_interface_only;
+  int32_t liquid_particle_refill_interface_radius, liquid_particle_refill_max_added_per_step;
+  int32_t liquid_refill_particles_per_cell_target;
+  uint32_t liquid_particle_refill_seed;
+} orc_adapt_cfg;
+
+typedef struct orc_adapt_stats {          /* the *Result3D structs, :50-73 */
+  int32_t narrow_band_air_removed, narrow_band_air_liquid_cells, narrow_band_air_gas_before,
+      narrow_band_air_gas_after;
+  int32_t gas_removed, gas_cells, gas_overfull_cells, gas_before, gas_after;
+  int32_t liquid_removed, liquid_cells, liquid_overfull_cells, liquid_before, liquid_after;
+  int32_t refill_added, refill_cells, refill_interface_cells, refill_underfull_cells, refill_budget_limited,
+      refill_before, refill_after;
+  int32_t pad0;
+} orc_adapt_stats;
+
+uint32_t orc_mix32(uint32_t x);
+uint32_t orc_particle_score(const orc_pa_domain* d, double x, double y, double z, int i, int j, int k,
+                            uint32_t seed);
+uint32_t orc_slot_score(uint64_t cell, int slot, uint32_t seed);
+/* applyParticleAdaptivity (sparse_sim3d_tp.cpp:847-852): narrow-band air, gas
+ * coarsening, liquid coarsening, liquid refill, each only when enabled.  Arrays hold
+ * n particles and room for cap; returns the new count, or -1 when refill would pass
+ * cap. */
+int64_t orc_particle_adaptivity(const orc_pa_domain* d, const orc_adapt_cfg* c, int64_t n, int64_t cap, double* x,
+                                double* y, double* z, double* u, double* v, double* w, uint8_t* type, double* vol,
+                                orc_adapt_stats* st);
+
 /* seeded synthetic scenes (shared construction with the HIP generator) */
 enum { ORC_SCENE_RT = 0, ORC_SCENE_BUBBLE = 1, ORC_SCENE_DAMBREAK = 2 };
 int64_t orc_scene_count(int kind, int nx, int ny, int nz);

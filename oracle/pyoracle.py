@@ -92,6 +92,96 @@ class StStats(C.Structure):
         return [getattr(self, n) for n, _ in self._fields_ if n != "pad0"]
 
 
+class PaDomain(C.Structure):
+    """ParticleCellDomain3D (src/driver/particle_adaptivity3d.h:13-48)"""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pad0", C.c_int32),
+                ("dx", C.c_double), ("ox", C.c_double), ("oy", C.c_double), ("oz", C.c_double)]
+
+
+class AdaptCfg(C.Structure):
+    """the particle adaptivity fields of SparseSim3DTP (sparse_sim3d_tp.h:29-60)"""
+    _fields_ = [("narrow_band_air", C.c_int32), ("narrow_band_air_radius", C.c_int32),
+                ("gas_particle_coarsening", C.c_int32), ("gas_particles_per_cell_target", C.c_int32),
+                ("gas_particle_coarsening_seed", C.c_uint32),
+                ("liquid_particle_coarsening", C.c_int32), ("liquid_particles_per_cell_target", C.c_int32),
+                ("liquid_particle_coarsening_seed", C.c_uint32),
+                ("liquid_particle_refill", C.c_int32), ("liquid_particle_refill_interface_only", C.c_int32),
+                ("liquid_particle_refill_interface_radius", C.c_int32),
+                ("liquid_particle_refill_max_added_per_step", C.c_int32),
+                ("liquid_refill_particles_per_cell_target", C.c_int32), ("liquid_particle_refill_seed", C.c_uint32)]
+
+    def words(self):
+        return np.frombuffer(bytes(self), np.uint32).copy()
+
+
+def adapt_cfg(**kw) -> AdaptCfg:
+    """the reference's defaults (all stages off, radius 2, targets 4, its seeds), then kw"""
+    c = AdaptCfg(narrow_band_air_radius=2, gas_particles_per_cell_target=4, gas_particle_coarsening_seed=0x9e3779b9,
+                 liquid_particles_per_cell_target=4, liquid_particle_coarsening_seed=0x7f4a7c15,
+                 liquid_particle_refill_interface_radius=1, liquid_refill_particles_per_cell_target=4,
+                 liquid_particle_refill_seed=0x6a09e667)
+    for k, v in kw.items():
+        if k not in dict(AdaptCfg._fields_):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class AdaptStats(C.Structure):
+    """NarrowBandAirResult3D, GasParticleCoarseningResult3D (gas, then liquid) and
+    LiquidParticleRefillResult3D (particle_adaptivity3d.h:50-73)"""
+    _fields_ = [(n, C.c_int32) for n in (
+        "narrow_band_air_removed", "narrow_band_air_liquid_cells", "narrow_band_air_gas_before",
+        "narrow_band_air_gas_after", "gas_removed", "gas_cells", "gas_overfull_cells", "gas_before", "gas_after",
+        "liquid_removed", "liquid_cells", "liquid_overfull_cells", "liquid_before", "liquid_after",
+        "refill_added", "refill_cells", "refill_interface_cells", "refill_underfull_cells",
+        "refill_budget_limited", "refill_before", "refill_after", "pad0")]
+
+    def values(self):
+        return [getattr(self, n) for n, _ in self._fields_ if n != "pad0"]
+
+
+def _adapt_capacity(domain, cfg, n):
+    extra = domain.nx * domain.ny * domain.nz * max(1, cfg.liquid_refill_particles_per_cell_target) \
+        if cfg.liquid_particle_refill else 0
+    return n + extra
+
+
+def _adapt_arrays(ps, cap):
+    n = len(ps)
+    out = [np.zeros(cap) for _ in range(6)]
+    for a, src in zip(out, (ps.x, ps.y, ps.z, ps.u, ps.v, ps.w)):
+        a[:n] = src
+    t = np.zeros(cap, np.uint8)
+    t[:n] = ps.type
+    vol = np.ones(cap)
+    if ps.volume is not None:
+        vol[:n] = ps.volume
+    return out, t, vol
+
+
+def particle_adaptivity(domain: PaDomain, cfg: AdaptCfg, ps):
+    """applyParticleAdaptivity on a copy of ps: (new Particles with volumes, AdaptStats)"""
+    cap = _adapt_capacity(domain, cfg, len(ps))
+    a, t, vol = _adapt_arrays(ps, cap)
+    st = AdaptStats()
+    m = lib().orc_particle_adaptivity(C.byref(domain), C.byref(cfg), len(ps), cap, *a, t, vol, C.byref(st))
+    assert m >= 0
+    return Particles(*(q[:m] for q in a), type=t[:m], volume=vol[:m]), st
+
+
+def ref_particle_adaptivity(domain: PaDomain, cfg: AdaptCfg, ps):
+    """the reference's pa3d functions on a copy of ps: (new Particles, 21 counters)"""
+    cap = _adapt_capacity(domain, cfg, len(ps))
+    a, t, vol = _adapt_arrays(ps, cap)
+    st = np.zeros(22, np.int32)
+    m = reflib().ref_pa_apply(np.array([domain.nx, domain.ny, domain.nz], np.int32),
+                              np.array([domain.dx, domain.ox, domain.oy, domain.oz]), cfg.words(), len(ps), cap,
+                              *a, t, vol, st)
+    assert m >= 0
+    return Particles(*(q[:m] for q in a), type=t[:m], volume=vol[:m]), list(st[:21])
+
+
 class MRSolveCfg(C.Structure):
     _fields_ = [("max_iterations", C.c_int32), ("use_jacobi", C.c_int32), ("use_flexible_beta", C.c_int32),
                 ("adaptive_restart", C.c_int32), ("absolute_tolerance", C.c_double),
@@ -262,6 +352,16 @@ def lib():
         L.orc_mr_advect.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _u8, C.c_double, C.c_int,
                                     C.POINTER(EscapeStats)]
         L.orc_mr_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + [_dp] * 6 + [_u8, C.c_void_p, C.c_void_p]
+        L.orc_particle_adaptivity.restype = C.c_int64
+        L.orc_particle_adaptivity.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64] + [_dp] * 6 + \
+            [_u8, _dp, C.c_void_p]
+        L.orc_mix32.restype = C.c_uint32
+        L.orc_mix32.argtypes = [C.c_uint32]
+        L.orc_particle_score.restype = C.c_uint32
+        L.orc_particle_score.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                         C.c_uint32]
+        L.orc_slot_score.restype = C.c_uint32
+        L.orc_slot_score.argtypes = [C.c_uint64, C.c_int, C.c_uint32]
         L.orc_mr_interface_diagnostics.argtypes = [C.c_void_p, C.POINTER(Phase), C.POINTER(IfaceStats)]
         L.orc_mr_surface_tension.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, C.c_double, C.c_double,
                                              C.c_int, C.POINTER(StStats)]
@@ -535,6 +635,17 @@ def reflib():
         L.ref_mr_face_valid.argtypes = [C.c_void_p, C.c_int, _u8]
         L.ref_mr_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.ref_mr_interface_diagnostics.argtypes = [C.c_void_p, _dp]
+        _u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+        L.ref_pa_apply.restype = C.c_int64
+        L.ref_pa_apply.argtypes = [_i32, _dp, _u32, C.c_int64, C.c_int64] + [_dp] * 6 + [_u8, _dp, _i32]
+        L.ref_sim_set_adapt.argtypes = [C.c_void_p, _u32]
+        L.ref_sim_particles_full.restype = C.c_int64
+        L.ref_sim_particles_full.argtypes = [C.c_void_p] + [C.c_void_p] * 8
+        L.ref_sim_get_adapt.argtypes = [C.c_void_p, _i32]
+        L.ref_mr_set_adapt.argtypes = [C.c_void_p, _u32]
+        L.ref_mr_particles_full.restype = C.c_int64
+        L.ref_mr_particles_full.argtypes = [C.c_void_p] + [C.c_void_p] * 8
+        L.ref_mr_get_adapt.argtypes = [C.c_void_p, _i32]
         L.ref_mr_surface_tension.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
         L.ref_mr_surface_flags.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]
         L.ref_mr_last_interface.argtypes = [C.c_void_p, _dp]
@@ -783,6 +894,21 @@ class RefSim:
         reflib().ref_sim_last_interface(self.h, out)
         return out[:11], out[11:]
 
+    def set_adapt(self, cfg: AdaptCfg):
+        reflib().ref_sim_set_adapt(self.h, cfg.words())
+
+    def adapt_stats(self):
+        out = np.zeros(22, np.int32)
+        reflib().ref_sim_get_adapt(self.h, out)
+        return list(out[:21])
+
+    def particles_full(self) -> Particles:
+        n = reflib().ref_sim_particles_full(self.h, *([None] * 8))
+        a = [np.zeros(max(n, 1)) for _ in range(7)]
+        t = np.zeros(max(n, 1), np.uint8)
+        reflib().ref_sim_particles_full(self.h, *[_vptr(q) for q in a[:6]], _vptr(t), _vptr(a[6]))
+        return Particles(*(q[:n] for q in a[:6]), type=t[:n], volume=a[6][:n])
+
     def set_particles(self, ps: Particles):
         self.n = len(ps)
         reflib().ref_sim_set_particles(self.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w,
@@ -915,6 +1041,21 @@ class RefMR:
         out = np.zeros(14)
         reflib().ref_mr_surface_tension(self.h, dt, strength, max_delta_speed, radius, out)
         return out
+
+    def set_adapt(self, cfg: AdaptCfg):
+        reflib().ref_mr_set_adapt(self.h, cfg.words())
+
+    def adapt_stats(self):
+        out = np.zeros(22, np.int32)
+        reflib().ref_mr_get_adapt(self.h, out)
+        return list(out[:21])
+
+    def particles_full(self) -> Particles:
+        n = reflib().ref_mr_particles_full(self.h, *([None] * 8))
+        a = [np.zeros(max(n, 1)) for _ in range(7)]
+        t = np.zeros(max(n, 1), np.uint8)
+        reflib().ref_mr_particles_full(self.h, *[_vptr(q) for q in a[:6]], _vptr(t), _vptr(a[6]))
+        return Particles(*(q[:n] for q in a[:6]), type=t[:n], volume=a[6][:n])
 
     def surface_flags(self, on, strength=0.02, max_delta_speed=0.25, radius=0):
         reflib().ref_mr_surface_flags(self.h, int(on), strength, max_delta_speed, radius)

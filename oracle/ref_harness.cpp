@@ -11,8 +11,10 @@
 // (src/driver/sparse_sim3d.h:5-15) with spP2G3D / spProjectStep3D
 // (src/driver/sparse_ops3d.h:5-9); and the multires MRSim3DTP
 // (src/driver/multires_sim3d_tp.h) with mrP2G3D_tp / mrG2P3D_tp / mrAdvect3D_tp
-// (src/driver/multires_ops3d_tp.h) and projectMR3D (src/pressure/multires_pressure3d.h).
+// (src/driver/multires_ops3d_tp.h) and projectMR3D (src/pressure/multires_pressure3d.h);
+// the particle adaptivity of src/driver/particle_adaptivity3d.h.
 #include "driver/multires_ops3d_tp.h"
+#include "driver/particle_adaptivity3d.h"
 #include "driver/multires_sim3d_tp.h"
 #include "driver/sparse_ops3d.h"
 #include "driver/sparse_ops3d_tp.h"
@@ -45,6 +47,34 @@ PhaseParams phaseOf(const double* ph) {
   PhaseParams p;
   p.rho_l = ph[0]; p.rho_g = ph[1]; p.alpha_phi = ph[2]; p.rho_tilde_0 = ph[3];
   return p;
+}
+// the adaptivity fields of SparseSim3DTP / MRSim3DTP (cfg: the 14 words of orc_adapt_cfg) and their
+// *_last results (22 counters)
+template <class S>
+void set_adapt(S* s, const uint32_t* c) {
+  s->narrow_band_air = c[0] != 0; s->narrow_band_air_radius = (int)c[1];
+  s->gas_particle_coarsening = c[2] != 0; s->gas_particles_per_cell_target = (int)c[3];
+  s->gas_particle_coarsening_seed = c[4];
+  s->liquid_particle_coarsening = c[5] != 0; s->liquid_particles_per_cell_target = (int)c[6];
+  s->liquid_particle_coarsening_seed = c[7];
+  s->liquid_particle_refill = c[8] != 0; s->liquid_particle_refill_interface_only = c[9] != 0;
+  s->liquid_particle_refill_interface_radius = (int)c[10]; s->liquid_particle_refill_max_added_per_step = (int)c[11];
+  s->liquid_refill_particles_per_cell_target = (int)c[12]; s->liquid_particle_refill_seed = c[13];
+}
+template <class S>
+void get_adapt(const S* s, int32_t* o) {
+  const int32_t v[22] = {s->narrow_band_air_removed_last, s->narrow_band_air_liquid_cells_last,
+                         s->narrow_band_air_gas_particles_before_last, s->narrow_band_air_gas_particles_after_last,
+                         s->gas_particle_coarsening_removed_last, s->gas_particle_coarsening_cells_last,
+                         s->gas_particle_coarsening_overfull_cells_last, s->gas_particle_coarsening_before_last,
+                         s->gas_particle_coarsening_after_last, s->liquid_particle_coarsening_removed_last,
+                         s->liquid_particle_coarsening_cells_last, s->liquid_particle_coarsening_overfull_cells_last,
+                         s->liquid_particle_coarsening_before_last, s->liquid_particle_coarsening_after_last,
+                         s->liquid_particle_refill_added_last, s->liquid_particle_refill_cells_last,
+                         s->liquid_particle_refill_interface_cells_last, s->liquid_particle_refill_underfull_cells_last,
+                         s->liquid_particle_refill_budget_limited_last, s->liquid_particle_refill_before_last,
+                         s->liquid_particle_refill_after_last, 0};
+  std::memcpy(o, v, sizeof(v));
 }
 }  // namespace
 
@@ -495,6 +525,62 @@ void ref_mr_last_interface(void* sp, double* out) {
   iface_out(s->interface_diagnostics_last, out);
   st_out(s->surface_tension_stats_last, out + 11);
 }
+// particle adaptivity (particle_adaptivity3d.h) on arrays, in applyParticleAdaptivity
+// order.  dom: nx, ny, nz and dx, ox, oy, oz; cfg: the 14 words of orc_adapt_cfg;
+// st: 22 counters in orc_adapt_stats order.  Returns the new count, -1 past cap.
+int64_t ref_pa_apply(const int32_t* dn, const double* dd, const uint32_t* cfg, int64_t n, int64_t cap, double* x,
+                     double* y, double* z, double* u, double* v, double* w, uint8_t* t, double* vol, int32_t* st) {
+  const pa3d::ParticleCellDomain3D dom{dn[0], dn[1], dn[2], dd[0], dd[1], dd[2], dd[3]};
+  Particles3DTP P;
+  for (int64_t p = 0; p < n; ++p) P.add({x[p], y[p], z[p]}, {u[p], v[p], w[p]}, t[p], vol[p]);
+  const auto nb = pa3d::applyNarrowBandAir(P, dom, cfg[0] != 0, (int)cfg[1]);
+  const auto gc = pa3d::applyGasParticleCoarsening(P, dom, cfg[2] != 0, (int)cfg[3], cfg[4]);
+  const auto lc = pa3d::applyTypedParticleCoarsening(P, dom, cfg[5] != 0, 0, (int)cfg[6], cfg[7]);
+  const auto rf = pa3d::applyLiquidParticleRefill(P, dom, cfg[8] != 0, (int)cfg[12], cfg[13], cfg[9] != 0,
+                                                  (int)cfg[10], (int)cfg[11]);
+  const int32_t o[22] = {nb.removed, nb.liquidCells, nb.gasBefore, nb.gasAfter,
+                         gc.removed, gc.cells, gc.overfullCells, gc.particlesBefore, gc.particlesAfter,
+                         lc.removed, lc.cells, lc.overfullCells, lc.particlesBefore, lc.particlesAfter,
+                         rf.added, rf.cells, rf.interfaceCells, rf.underfullCells, rf.budgetLimited,
+                         rf.particlesBefore, rf.particlesAfter, 0};
+  std::memcpy(st, o, sizeof(o));
+  if ((int64_t)P.size() > cap) return -1;
+  for (size_t p = 0; p < P.size(); ++p) {
+    x[p] = P.pos[p].x; y[p] = P.pos[p].y; z[p] = P.pos[p].z;
+    u[p] = P.vel[p].x; v[p] = P.vel[p].y; w[p] = P.vel[p].z;
+    t[p] = P.type[p]; vol[p] = P.volume[p];
+  }
+  return (int64_t)P.size();
+}
+
+// the sim's whole particle state (count first: adaptivity changes it)
+int64_t ref_sim_particles_full(void* sp, double* x, double* y, double* z, double* u, double* v, double* w,
+                               uint8_t* t, double* vol) {
+  const Particles3DTP& P = static_cast<SparseSim3DTP*>(sp)->particles;
+  if (x)
+    for (size_t p = 0; p < P.size(); ++p) {
+      x[p] = P.pos[p].x; y[p] = P.pos[p].y; z[p] = P.pos[p].z;
+      u[p] = P.vel[p].x; v[p] = P.vel[p].y; w[p] = P.vel[p].z;
+      t[p] = P.type[p]; vol[p] = P.volume[p];
+    }
+  return (int64_t)P.size();
+}
+void ref_sim_set_adapt(void* sp, const uint32_t* c) { set_adapt(static_cast<SparseSim3DTP*>(sp), c); }
+void ref_sim_get_adapt(void* sp, int32_t* o) { get_adapt(static_cast<SparseSim3DTP*>(sp), o); }
+int64_t ref_mr_particles_full(void* sp, double* x, double* y, double* z, double* u, double* v, double* w,
+                              uint8_t* t, double* vol) {
+  const Particles3DTP& P = mrs(sp)->particles;
+  if (x)
+    for (size_t p = 0; p < P.size(); ++p) {
+      x[p] = P.pos[p].x; y[p] = P.pos[p].y; z[p] = P.pos[p].z;
+      u[p] = P.vel[p].x; v[p] = P.vel[p].y; w[p] = P.vel[p].z;
+      t[p] = P.type[p]; vol[p] = P.volume[p];
+    }
+  return (int64_t)P.size();
+}
+void ref_mr_set_adapt(void* sp, const uint32_t* c) { set_adapt(mrs(sp), c); }
+void ref_mr_get_adapt(void* sp, int32_t* o) { get_adapt(mrs(sp), o); }
+
 void ref_mr_set(void* sp, int f, int i, int j, int k, double v) {
   auto& g = mrs(sp)->grid;
   if (f <= 5) {
