@@ -157,8 +157,8 @@ struct Ctx {
   double* red = nullptr;            // reduced values (device, 16 doubles)
   double* red_host = nullptr;       // pinned host mirror
   int* done_host = nullptr;         // pinned host mirror of sc->done
-  uint32_t* u32_dev = nullptr;      // small device scratch (pin detection, counters)
-  uint32_t* u32_host = nullptr;     // pinned mirror
+  uint32_t* u32_dev = nullptr;      // small device scratch (pin detection, counters), 64 words
+  uint32_t* u32_host = nullptr;     // pinned mirror, 64 words
   int32_t* esc_dev = nullptr;       // 8 clamp counters
   float* vmax_dev = nullptr;        // max particle speed (as float bits via atomicMax)
 
@@ -214,6 +214,16 @@ struct Ctx {
   // three normal components, curvature, three increments), allocated on first use
   double* iface = nullptr;
   unsigned long long* iface_bad = nullptr;  // least visiting-order key of a non-finite increment
+
+  // particle adaptivity scratch (per particle of the input set, per cell)
+  uint8_t* pa_alive = nullptr;
+  uint64_t* pa_key = nullptr;
+  uint32_t* pa_by_id = nullptr;     // survivors by id, then its exclusive scan (np + 1)
+  uint32_t* pa_new_id = nullptr;
+  long long pa_cap = 0;
+  uint8_t *pa_cflag = nullptr, *pa_d1 = nullptr, *pa_d2 = nullptr;  // per cell
+  uint32_t *pa_wanted = nullptr, *pa_wscan = nullptr;              // per cell, linear order (+1)
+  uint32_t* pa_cnt = nullptr;       // counters
 
   // per-kernel event timing (measurement hook)
   uint32_t ktime = 0;                // kernels (bit PF_K_*) bracketed with events

@@ -164,16 +164,7 @@ void MRSim3DTP::upload() {
   dirty_ = false;
 }
 
-void MRSim3DTP::syncParticlesToHost() {
-  const size_t n = particles.size();
-  std::vector<float> x(n), y(n), z(n), u(n), v(n), w(n);
-  check(pf_particles_download(ctx_, x.data(), y.data(), z.data(), u.data(), v.data(), w.data()),
-        "pf_particles_download");
-  for (size_t p = 0; p < n; ++p) {
-    particles.pos[p] = Vec3{x[p], y[p], z[p]};
-    particles.vel[p] = Vec3{u[p], v[p], w[p]};
-  }
-}
+void MRSim3DTP::syncParticlesToHost() { downloadParticles(ctx_, particles); }
 
 // updateDynamicRefinement, multires_sim3d_tp.cpp:577-669
 void MRSim3DTP::updateDynamicRefinement() {
@@ -256,22 +247,17 @@ void MRSim3DTP::step() {
   cfg.surface_tension_max_delta_speed = surface_tension_max_delta_speed;
   cfg.surface_tension_smoothing_radius = surface_tension_curvature_smoothing_radius;
   cfg.skip_interface_diagnostics = interface_diagnostics ? 0 : 1;
-  // updateVolumeCorrectionStats (multires_sim3d_tp.cpp:439-450) needs this step's dt
-  double stepDt = dt;
-  if (adaptive_timestep) {
-    double vmax = 0.0, cfl_dt = 0.0;
-    int32_t lim = 0;
-    check(pf_timestep(ctx_, dt, 1, adaptive_cfl, adaptive_min_dt, &stepDt, &vmax, &cfl_dt, &lim), "pf_timestep");
-  }
-  liquid_volume_current_last = 0.0;
-  for (size_t p = 0; p < particles.size(); ++p)
-    if (particles.type[p] == 0) liquid_volume_current_last += particles.volume[p] * Vp;
-  liquid_volume_error_last = liquid_volume_target - liquid_volume_current_last;
-  c_div_last = 0.0;
-  if (c_div_volume_correction && c_div_strength > 0.0 && liquid_volume_target > 0.0 && stepDt > 0.0)
-    c_div_last = c_div_strength * liquid_volume_error_last / (stepDt * liquid_volume_target);
-  cfg.c_div = c_div_last;
+  cfg.adapt = adaptCfg();
+  // applyParticleAdaptivity and updateVolumeCorrectionStats (multires_sim3d_tp.cpp:439-450)
+  // run inside the step
+  cfg.c_div_volume_correction = 1;
+  cfg.c_div_strength = c_div_volume_correction ? c_div_strength : 0.0;
+  cfg.liquid_volume_target = liquid_volume_target;
   check(pf_mr_step(ctx_, &cfg, &stats_last), "pf_mr_step");
+  storeAdaptStats(stats_last.adapt);
+  liquid_volume_current_last = stats_last.liquid_volume;
+  liquid_volume_error_last = liquid_volume_target - liquid_volume_current_last;
+  c_div_last = stats_last.c_div;
   fetchLayout();
   if (dynamic_refinement) {
     int32_t fine = 0, limited = 0;

@@ -5,8 +5,9 @@
 // Keeps the fields step() reads and the per-step stats it writes under the
 // reference's names, the layout (as MRLayout3D leaf blocks), initBubbleTankInterfaceBand,
 // updateDynamicRefinement and step().  Particles live on the device between steps
-// (see sparse_sim3d_tp_gpu.h); particle adaptivity and secondary particles are not
-// built, and surface tension is off by default, as in the reference.
+// (see sparse_sim3d_tp_gpu.h); particle adaptivity and surface tension use the
+// reference's fields (off by default, as in the reference); secondary particles are
+// not built.
 #pragma once
 
 #include <array>
@@ -41,7 +42,7 @@ struct MRPressureSolveStats3D {
   bool coarse_preconditioner_auto_disabled = false;
 };
 
-class MRSim3DTP {
+class MRSim3DTP : public ParticleAdaptivityFields {
  public:
   Particles3DTP particles;
   PhaseParams phase;

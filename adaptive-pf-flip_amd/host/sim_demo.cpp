@@ -320,6 +320,31 @@ int main() {
                 sim.interface_diagnostics_last.interface_cells, st.applied_cells, st.max_delta_speed,
                 mr.interface_diagnostics_last.interface_cells, mr.surface_tension_stats_last.applied_cells);
   }
+  {
+    // particle adaptivity through the class fields: coarsening thins the 2x2x2 seeding,
+    // refill tops underfull liquid cells back up, and the host view follows the new count
+    SparseSim3DTP sim(12, 12, 12, 1.0);
+    sim.dt = 0.01;
+    sim.initBubbleTank();
+    const size_t n0 = sim.particles.size();
+    sim.gas_particle_coarsening = true;
+    sim.liquid_particle_coarsening = true;
+    sim.liquid_particles_per_cell_target = 5;
+    sim.liquid_particle_refill = true;
+    sim.liquid_refill_particles_per_cell_target = 6;
+    sim.step();
+    sim.syncParticlesToHost();
+    const int removed = sim.gas_particle_coarsening_removed_last + sim.liquid_particle_coarsening_removed_last;
+    CHECK(removed > 0 && sim.liquid_particle_refill_added_last > 0);
+    CHECK(sim.particles.size() == n0 - (size_t)removed + (size_t)sim.liquid_particle_refill_added_last);
+    CHECK(sim.particles.volume.size() == sim.particles.size() && finite(sim));
+    double lv = 0.0;
+    for (size_t p = 0; p < sim.particles.size(); ++p)
+      if (sim.particles.type[p] == 0) lv += sim.particles.volume[p];
+    CHECK(std::fabs(lv - sim.liquid_volume_current_last) <= 1e-4 * lv);
+    std::printf("adaptivity: %zu -> %zu particles (%d removed, %d refilled), liquid volume %.3f\n", n0,
+                sim.particles.size(), removed, sim.liquid_particle_refill_added_last, lv);
+  }
   std::printf(failures ? "FAILED %d checks\n" : "all checks passed\n", failures);
   return failures ? 1 : 0;
 }

@@ -113,16 +113,73 @@ void SparseSim3DTP::upload() {
   dirty_ = false;
 }
 
-void SparseSim3DTP::syncParticlesToHost() {
-  const size_t n = particles.size();
-  std::vector<float> x(n), y(n), z(n), u(n), v(n), w(n);
-  check(pf_particles_download(ctx_, x.data(), y.data(), z.data(), u.data(), v.data(), w.data()),
-        "pf_particles_download");
-  for (size_t p = 0; p < n; ++p) {
+pf_adapt_cfg ParticleAdaptivityFields::adaptCfg() const {
+  pf_adapt_cfg c{};
+  c.narrow_band_air = narrow_band_air ? 1 : 0;
+  c.narrow_band_air_radius = narrow_band_air_radius;
+  c.gas_particle_coarsening = gas_particle_coarsening ? 1 : 0;
+  c.gas_particles_per_cell_target = gas_particles_per_cell_target;
+  c.gas_particle_coarsening_seed = gas_particle_coarsening_seed;
+  c.liquid_particle_coarsening = liquid_particle_coarsening ? 1 : 0;
+  c.liquid_particles_per_cell_target = liquid_particles_per_cell_target;
+  c.liquid_particle_coarsening_seed = liquid_particle_coarsening_seed;
+  c.liquid_particle_refill = liquid_particle_refill ? 1 : 0;
+  c.liquid_particle_refill_interface_only = liquid_particle_refill_interface_only ? 1 : 0;
+  c.liquid_particle_refill_interface_radius = liquid_particle_refill_interface_radius;
+  c.liquid_particle_refill_max_added_per_step = liquid_particle_refill_max_added_per_step;
+  c.liquid_refill_particles_per_cell_target = liquid_refill_particles_per_cell_target;
+  c.liquid_particle_refill_seed = liquid_particle_refill_seed;
+  return c;
+}
+
+void ParticleAdaptivityFields::storeAdaptStats(const pf_adapt_stats& s) {
+  narrow_band_air_removed_last = s.narrow_band_air_removed;
+  narrow_band_air_liquid_cells_last = s.narrow_band_air_liquid_cells;
+  narrow_band_air_gas_particles_before_last = s.narrow_band_air_gas_before;
+  narrow_band_air_gas_particles_after_last = s.narrow_band_air_gas_after;
+  narrow_band_air_removed_total += s.narrow_band_air_removed;
+  gas_particle_coarsening_removed_last = s.gas_removed;
+  gas_particle_coarsening_cells_last = s.gas_cells;
+  gas_particle_coarsening_overfull_cells_last = s.gas_overfull_cells;
+  gas_particle_coarsening_before_last = s.gas_before;
+  gas_particle_coarsening_after_last = s.gas_after;
+  gas_particle_coarsening_removed_total += s.gas_removed;
+  liquid_particle_coarsening_removed_last = s.liquid_removed;
+  liquid_particle_coarsening_cells_last = s.liquid_cells;
+  liquid_particle_coarsening_overfull_cells_last = s.liquid_overfull_cells;
+  liquid_particle_coarsening_before_last = s.liquid_before;
+  liquid_particle_coarsening_after_last = s.liquid_after;
+  liquid_particle_coarsening_removed_total += s.liquid_removed;
+  liquid_particle_refill_added_last = s.refill_added;
+  liquid_particle_refill_cells_last = s.refill_cells;
+  liquid_particle_refill_interface_cells_last = s.refill_interface_cells;
+  liquid_particle_refill_underfull_cells_last = s.refill_underfull_cells;
+  liquid_particle_refill_budget_limited_last = s.refill_budget_limited;
+  liquid_particle_refill_before_last = s.refill_before;
+  liquid_particle_refill_after_last = s.refill_after;
+  liquid_particle_refill_added_total += s.refill_added;
+}
+
+void downloadParticles(pf_ctx* ctx, Particles3DTP& particles) {
+  const int64_t n = pf_particle_count(ctx);
+  if (n < 0) throw std::runtime_error("pf_particle_count: no context");
+  std::vector<float> x(n), y(n), z(n), u(n), v(n), w(n), vol(n);
+  std::vector<unsigned char> t(n);
+  if (pf_particles_download(ctx, x.data(), y.data(), z.data(), u.data(), v.data(), w.data()) != PF_OK ||
+      pf_particles_download_ext(ctx, t.data(), vol.data()) != PF_OK)
+    throw std::runtime_error(std::string("pf_particles_download: ") + pf_last_error(ctx));
+  particles.pos.resize(n);
+  particles.vel.resize(n);
+  particles.type.assign(t.begin(), t.end());
+  particles.volume.resize(n);
+  for (int64_t p = 0; p < n; ++p) {
     particles.pos[p] = Vec3{x[p], y[p], z[p]};
     particles.vel[p] = Vec3{u[p], v[p], w[p]};
+    particles.volume[p] = vol[p];
   }
 }
+
+void SparseSim3DTP::syncParticlesToHost() { downloadParticles(ctx_, particles); }
 
 size_t SparseSim3DTP::activeCellBlocks() {
   int64_t n = 0;
@@ -161,20 +218,17 @@ void SparseSim3DTP::step() {
   cfg.surface_tension_max_delta_speed = surface_tension_max_delta_speed;
   cfg.surface_tension_smoothing_radius = surface_tension_curvature_smoothing_radius;
   cfg.skip_interface_diagnostics = interface_diagnostics ? 0 : 1;
-  // updateVolumeCorrectionStats (sparse_sim3d_tp.cpp:368-379) needs this step's dt
-  double stepDt = dt;
-  if (adaptive_timestep) {
-    double vmax = 0.0, cfl_dt = 0.0;
-    int32_t lim = 0;
-    check(pf_timestep(ctx_, dt, 1, adaptive_cfl, adaptive_min_dt, &stepDt, &vmax, &cfl_dt, &lim), "pf_timestep");
-  }
-  liquid_volume_current_last = liquidVolume();
-  liquid_volume_error_last = liquid_volume_target - liquid_volume_current_last;
-  c_div_last = 0.0;
-  if (c_div_volume_correction && c_div_strength > 0.0 && liquid_volume_target > 0.0 && stepDt > 0.0)
-    c_div_last = c_div_strength * liquid_volume_error_last / (stepDt * liquid_volume_target);
-  cfg.c_div = c_div_last;
+  // applyParticleAdaptivity and updateVolumeCorrectionStats run inside the step
+  // (sparse_sim3d_tp.cpp:855-862)
+  cfg.adapt = adaptCfg();
+  cfg.c_div_volume_correction = 1;
+  cfg.c_div_strength = c_div_volume_correction ? c_div_strength : 0.0;
+  cfg.liquid_volume_target = liquid_volume_target;
   check(pf_step(ctx_, &cfg, &stats_last), "pf_step");
+  storeAdaptStats(stats_last.adapt);
+  liquid_volume_current_last = stats_last.liquid_volume;
+  liquid_volume_error_last = liquid_volume_target - liquid_volume_current_last;
+  c_div_last = stats_last.c_div;
   effective_dt_last = stats_last.effective_dt;
   max_particle_speed_last = stats_last.max_particle_speed;
   cfl_limit_dt_last = stats_last.cfl_limit_dt;

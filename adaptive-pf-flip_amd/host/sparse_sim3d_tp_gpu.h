@@ -57,6 +57,43 @@ inline SurfaceTensionStats3D defaultSurfaceTensionStats() {
   return s;
 }
 
+// The particle adaptivity fields of SparseSim3DTP and MRSim3DTP (sparse_sim3d_tp.h:29-66,
+// multires_sim3d_tp.h:80-117): the controls and the *_last / *_total results under
+// the reference's names and defaults.  Both classes inherit them.
+struct ParticleAdaptivityFields {
+  bool narrow_band_air = false;
+  int narrow_band_air_radius = 2;
+  int narrow_band_air_removed_last = 0, narrow_band_air_removed_total = 0;
+  int narrow_band_air_liquid_cells_last = 0;
+  int narrow_band_air_gas_particles_before_last = 0, narrow_band_air_gas_particles_after_last = 0;
+  bool gas_particle_coarsening = false;
+  int gas_particles_per_cell_target = 4;
+  unsigned int gas_particle_coarsening_seed = 0x9e3779b9u;
+  int gas_particle_coarsening_removed_last = 0, gas_particle_coarsening_removed_total = 0;
+  int gas_particle_coarsening_cells_last = 0, gas_particle_coarsening_overfull_cells_last = 0;
+  int gas_particle_coarsening_before_last = 0, gas_particle_coarsening_after_last = 0;
+  bool liquid_particle_coarsening = false;
+  int liquid_particles_per_cell_target = 4;
+  unsigned int liquid_particle_coarsening_seed = 0x7f4a7c15u;
+  int liquid_particle_coarsening_removed_last = 0, liquid_particle_coarsening_removed_total = 0;
+  int liquid_particle_coarsening_cells_last = 0, liquid_particle_coarsening_overfull_cells_last = 0;
+  int liquid_particle_coarsening_before_last = 0, liquid_particle_coarsening_after_last = 0;
+  bool liquid_particle_refill = false;
+  bool liquid_particle_refill_interface_only = false;
+  int liquid_particle_refill_interface_radius = 1;
+  int liquid_particle_refill_max_added_per_step = 0;
+  int liquid_refill_particles_per_cell_target = 4;
+  unsigned int liquid_particle_refill_seed = 0x6a09e667u;
+  int liquid_particle_refill_added_last = 0, liquid_particle_refill_added_total = 0;
+  int liquid_particle_refill_cells_last = 0, liquid_particle_refill_interface_cells_last = 0;
+  int liquid_particle_refill_underfull_cells_last = 0, liquid_particle_refill_budget_limited_last = 0;
+  int liquid_particle_refill_before_last = 0, liquid_particle_refill_after_last = 0;
+
+  pf_adapt_cfg adaptCfg() const;
+  // the store*Stats helpers of sparse_sim3d_tp.cpp:778-845
+  void storeAdaptStats(const pf_adapt_stats& s);
+};
+
 struct PhaseParams {
   double rho_l = 1.0;
   double rho_g = 0.01;
@@ -64,7 +101,11 @@ struct PhaseParams {
   double rho_tilde_0 = 1.0;
 };
 
-class SparseSim3DTP {
+// Resizes `particles` to the context's count and fills positions, velocities, types
+// and volumes in the caller's order (after particle adaptivity the count changes).
+void downloadParticles(pf_ctx* ctx, Particles3DTP& particles);
+
+class SparseSim3DTP : public ParticleAdaptivityFields {
  public:
   Particles3DTP particles;
   PhaseParams phase;

@@ -121,6 +121,47 @@ class MRRefineCfg(C.Structure):
                 ("hysteresis_cells", C.c_int32), ("max_fine_leaves", C.c_int32), ("reserved", C.c_int32)]
 
 
+class AdaptCfg(C.Structure):
+    """pf_adapt_cfg: the particle adaptivity fields of SparseSim3DTP / MRSim3DTP
+    (sparse_sim3d_tp.h:29-60), under their names"""
+    _fields_ = [("narrow_band_air", C.c_int32), ("narrow_band_air_radius", C.c_int32),
+                ("gas_particle_coarsening", C.c_int32), ("gas_particles_per_cell_target", C.c_int32),
+                ("gas_particle_coarsening_seed", C.c_uint32),
+                ("liquid_particle_coarsening", C.c_int32), ("liquid_particles_per_cell_target", C.c_int32),
+                ("liquid_particle_coarsening_seed", C.c_uint32),
+                ("liquid_particle_refill", C.c_int32), ("liquid_particle_refill_interface_only", C.c_int32),
+                ("liquid_particle_refill_interface_radius", C.c_int32),
+                ("liquid_particle_refill_max_added_per_step", C.c_int32),
+                ("liquid_refill_particles_per_cell_target", C.c_int32), ("liquid_particle_refill_seed", C.c_uint32)]
+
+
+ADAPT_FIELDS = [n for n, _ in AdaptCfg._fields_]
+
+
+def adapt_cfg(**kw) -> "AdaptCfg":
+    """pf_adapt_defaults (the reference's defaults: every stage off), then kw"""
+    c = AdaptCfg()
+    lib().pf_adapt_defaults(C.byref(c))
+    for k, v in kw.items():
+        if k not in ADAPT_FIELDS:
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class AdaptStats(C.Structure):
+    """pf_adapt_stats: the adaptivity results (particle_adaptivity3d.h:50-73)"""
+    _fields_ = [(n, C.c_int32) for n in (
+        "narrow_band_air_removed", "narrow_band_air_liquid_cells", "narrow_band_air_gas_before",
+        "narrow_band_air_gas_after", "gas_removed", "gas_cells", "gas_overfull_cells", "gas_before", "gas_after",
+        "liquid_removed", "liquid_cells", "liquid_overfull_cells", "liquid_before", "liquid_after",
+        "refill_added", "refill_cells", "refill_interface_cells", "refill_underfull_cells",
+        "refill_budget_limited", "refill_before", "refill_after", "reserved")]
+
+    def values(self):
+        return [getattr(self, n) for n, _ in self._fields_ if n != "reserved"]
+
+
 class StepCfg(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("Vp", C.c_double),
                 ("alpha_liquid", C.c_double), ("alpha_gas", C.c_double),
@@ -130,7 +171,9 @@ class StepCfg(C.Structure):
                 ("phase", Phase), ("solve", SolveCfg),
                 ("surface_tension", C.c_int32), ("surface_tension_smoothing_radius", C.c_int32),
                 ("surface_tension_strength", C.c_double), ("surface_tension_max_delta_speed", C.c_double),
-                ("skip_interface_diagnostics", C.c_int32), ("reserved", C.c_int32)]
+                ("skip_interface_diagnostics", C.c_int32), ("reserved", C.c_int32),
+                ("c_div_volume_correction", C.c_int32), ("reserved2", C.c_int32),
+                ("c_div_strength", C.c_double), ("liquid_volume_target", C.c_double), ("adapt", AdaptCfg)]
 
 
 class MRStepCfg(C.Structure):
@@ -142,7 +185,9 @@ class MRStepCfg(C.Structure):
                 ("phase", Phase), ("solve", MRSolveCfg), ("refine", MRRefineCfg),
                 ("surface_tension", C.c_int32), ("surface_tension_smoothing_radius", C.c_int32),
                 ("surface_tension_strength", C.c_double), ("surface_tension_max_delta_speed", C.c_double),
-                ("skip_interface_diagnostics", C.c_int32), ("reserved2", C.c_int32)]
+                ("skip_interface_diagnostics", C.c_int32), ("reserved2", C.c_int32),
+                ("c_div_volume_correction", C.c_int32), ("reserved3", C.c_int32),
+                ("c_div_strength", C.c_double), ("liquid_volume_target", C.c_double), ("adapt", AdaptCfg)]
 
 
 class IfaceStats(C.Structure):
@@ -167,7 +212,8 @@ class StepStats(C.Structure):
     _fields_ = [("effective_dt", C.c_double), ("max_particle_speed", C.c_double),
                 ("cfl_limit_dt", C.c_double), ("limited", C.c_int32), ("reserved", C.c_int32),
                 ("solve", SolveStats), ("escape", EscapeStats), ("stage_ms", C.c_double * 8),
-                ("total_ms", C.c_double), ("iface", IfaceStats), ("surface", SurfaceStats)]
+                ("total_ms", C.c_double), ("iface", IfaceStats), ("surface", SurfaceStats),
+                ("liquid_volume", C.c_double), ("c_div", C.c_double), ("adapt", AdaptStats)]
 
 
 class PfError(RuntimeError):
@@ -211,6 +257,10 @@ def lib():
             "pf_apply_surface_tension": (C.c_int, [vp, C.POINTER(Phase), dbl, dbl, dbl, i32,
                                                    C.POINTER(SurfaceStats)]),
             "pf_mr_interface_diagnostics": (C.c_int, [vp, C.POINTER(Phase), C.POINTER(IfaceStats)]),
+            "pf_particle_adaptivity": (C.c_int, [vp, C.POINTER(AdaptCfg), C.POINTER(AdaptStats)]),
+            "pf_adapt_defaults": (None, [C.POINTER(AdaptCfg)]),
+            "pf_particles_download_ext": (C.c_int, [vp, vp, vp]),
+            "pf_liquid_volume": (dbl, [vp]),
             "pf_mr_apply_surface_tension": (C.c_int, [vp, C.POINTER(Phase), dbl, dbl, dbl, i32,
                                                       C.POINTER(SurfaceStats)]),
             "pf_field_dims": (C.c_int, [vp, i32, vp]),
@@ -325,8 +375,26 @@ class Context:
         self._check(lib().pf_particles_download(self.h, *[_p(q) for q in a]), "pf_particles_download")
         return a
 
+    def download_ext(self):
+        """(types, volumes) in the caller's order"""
+        n = self.count()
+        t = np.zeros(n, np.uint8)
+        vol = np.zeros(n, np.float32)
+        self._check(lib().pf_particles_download_ext(self.h, _p(t), _p(vol)), "pf_particles_download_ext")
+        return t, vol
+
     def count(self):
         return lib().pf_particle_count(self.h)
+
+    def liquid_volume(self):
+        """the summed volume of the liquid particles (liquidVolume with Vp = 1)"""
+        return lib().pf_liquid_volume(self.h)
+
+    def particle_adaptivity(self, cfg: AdaptCfg) -> AdaptStats:
+        """applyParticleAdaptivity on the context's particles"""
+        st = AdaptStats()
+        self._check(lib().pf_particle_adaptivity(self.h, C.byref(cfg), C.byref(st)), "pf_particle_adaptivity")
+        return st
 
     def scene(self, kind, seed):
         self._check(lib().pf_scene_generate(self.h, kind, seed), "pf_scene_generate")
@@ -452,6 +520,38 @@ class Context:
         return out[: n.value]
 
 
+def _init_adaptivity(sim):
+    """the particle adaptivity fields of SparseSim3DTP / MRSim3DTP (sparse_sim3d_tp.h:29-66)
+    with the reference's defaults, and their *_last / *_total results"""
+    d = adapt_cfg()
+    for n in ADAPT_FIELDS:
+        v = getattr(d, n)
+        setattr(sim, n, bool(v) if n in ("narrow_band_air", "gas_particle_coarsening", "liquid_particle_coarsening",
+                                          "liquid_particle_refill", "liquid_particle_refill_interface_only") else v)
+    sim.adapt_stats_last = AdaptStats()
+    for n in ("narrow_band_air_removed_total", "gas_particle_coarsening_removed_total",
+              "liquid_particle_coarsening_removed_total", "liquid_particle_refill_added_total"):
+        setattr(sim, n, 0)
+
+
+def _adapt_of(sim):
+    return AdaptCfg(**{n: int(getattr(sim, n)) for n in ADAPT_FIELDS})
+
+
+def _store_step_stats(sim, st):
+    """the step's *_last fields: volume correction and adaptivity (the store*Stats
+    helpers, sparse_sim3d_tp.cpp:368-379, 778-845)"""
+    sim.liquid_volume_current_last = st.liquid_volume
+    sim.liquid_volume_error_last = sim.liquid_volume_target - st.liquid_volume
+    sim.c_div_last = st.c_div
+    a = st.adapt
+    sim.adapt_stats_last = a
+    sim.narrow_band_air_removed_total += a.narrow_band_air_removed
+    sim.gas_particle_coarsening_removed_total += a.gas_removed
+    sim.liquid_particle_coarsening_removed_total += a.liquid_removed
+    sim.liquid_particle_refill_added_total += a.refill_added
+
+
 class SparseSim3DTP:
     """GPU mirror of the reference's SparseSim3DTP (src/driver/sparse_sim3d_tp.h:8-177).
 
@@ -480,19 +580,18 @@ class SparseSim3DTP:
         self.surface_tension_curvature_smoothing_radius = 0
         # False skips the per-step interface statistics (the reference always computes them)
         self.interface_diagnostics = True
+        _init_adaptivity(self)
         self.effective_dt_last = self.dt
         self.max_particle_speed_last = 0.0
         self.c_div_last = 0.0
         self.liquid_volume_current_last = 0.0
+        self.liquid_volume_error_last = 0.0
         self.interface_diagnostics_last = IfaceStats(finite=1)
         self.surface_tension_stats_last = SurfaceStats(finite=1, capillary_stable=1)
         self.stats_last = None
 
     def set_particles(self, x, y, z, u=None, v=None, w=None, type=None, volume=None):
         self.ctx.upload(x, y, z, u, v, w, type, volume)
-        t = np.asarray(type if type is not None else np.zeros(len(x)), np.uint8)
-        vol = np.ones(len(x)) if volume is None else np.asarray(volume, np.float64)
-        self._liquid_volume = float(vol[t == 0].sum()) * self.Vp
 
     def calibrate(self):
         """resetSceneState's rho_tilde_0 = calibrateRhoTilde0(phase, Vp), sparse_sim3d_tp.cpp:396"""
@@ -512,24 +611,16 @@ class SparseSim3DTP:
                        surface_tension_smoothing_radius=self.surface_tension_curvature_smoothing_radius,
                        surface_tension_strength=self.surface_tension_strength,
                        surface_tension_max_delta_speed=self.surface_tension_max_delta_speed,
-                       skip_interface_diagnostics=int(not self.interface_diagnostics))
+                       skip_interface_diagnostics=int(not self.interface_diagnostics),
+                       c_div_volume_correction=1,
+                       c_div_strength=self.c_div_strength if self.c_div_volume_correction else 0.0,
+                       liquid_volume_target=self.liquid_volume_target, adapt=_adapt_of(self))
 
     def step(self):
-        """SparseSim3DTP::step, src/driver/sparse_sim3d_tp.cpp:854-884."""
-        cfg = self.step_cfg()
-        # updateVolumeCorrectionStats, sparse_sim3d_tp.cpp:368-379 (needs the step dt)
-        if self.adaptive_timestep:
-            eff = self.ctx.timestep(self.dt, True, self.adaptive_cfl, self.adaptive_min_dt)[0]
-        else:
-            eff = self.dt
-        self.liquid_volume_current_last = self._liquid_volume
-        err = self.liquid_volume_target - self.liquid_volume_current_last
-        self.c_div_last = 0.0
-        if (self.c_div_volume_correction and self.c_div_strength > 0.0 and
-                self.liquid_volume_target > 0.0 and eff > 0.0):
-            self.c_div_last = self.c_div_strength * err / (eff * self.liquid_volume_target)
-        cfg.c_div = self.c_div_last
-        st = self.ctx.step(cfg)
+        """SparseSim3DTP::step, src/driver/sparse_sim3d_tp.cpp:854-884 (particle
+        adaptivity and updateVolumeCorrectionStats, :368-379, inside the step)."""
+        st = self.ctx.step(self.step_cfg())
+        _store_step_stats(self, st)
         self.effective_dt_last = st.effective_dt
         self.max_particle_speed_last = st.max_particle_speed
         self.interface_diagnostics_last = st.iface
@@ -774,6 +865,10 @@ class MRSim3DTP:
         self.interface_diagnostics = True
         self.interface_diagnostics_last = IfaceStats(finite=1)
         self.surface_tension_stats_last = SurfaceStats(finite=1, capillary_stable=1)
+        _init_adaptivity(self)
+        # updateVolumeCorrectionStats (multires_sim3d_tp.cpp:439-450)
+        self.c_div_volume_correction, self.c_div_strength, self.liquid_volume_target = False, 1.0, 0.0
+        self.liquid_volume_current_last = self.liquid_volume_error_last = 0.0
         self.c_div_last = 0.0
         self.timing = False
         self.stats_last = None
@@ -790,18 +885,22 @@ class MRSim3DTP:
                              self.dynamic_hysteresis_cells, self.dynamic_max_fine_leaves, 0)
         return MRStepCfg(dt=self.dt, gravity=self.gravity, Vp=self.Vp, alpha_liquid=self.alpha_liquid,
                          alpha_gas=self.alpha_gas, adaptive_cfl=self.adaptive_cfl,
-                         adaptive_min_dt=self.adaptive_min_dt, c_div=self.c_div_last,
+                         adaptive_min_dt=self.adaptive_min_dt, c_div=0.0,
                          adaptive_timestep=int(self.adaptive_timestep), advection_order=self.advection_order,
                          timing=int(self.timing), reserved=0, phase=self.phase, solve=solve, refine=refine,
                          surface_tension=int(self.surface_tension),
                          surface_tension_smoothing_radius=self.surface_tension_curvature_smoothing_radius,
                          surface_tension_strength=self.surface_tension_strength,
                          surface_tension_max_delta_speed=self.surface_tension_max_delta_speed,
-                         skip_interface_diagnostics=int(not self.interface_diagnostics))
+                         skip_interface_diagnostics=int(not self.interface_diagnostics),
+                         c_div_volume_correction=1,
+                         c_div_strength=self.c_div_strength if self.c_div_volume_correction else 0.0,
+                         liquid_volume_target=self.liquid_volume_target, adapt=_adapt_of(self))
 
     def step(self):
         """MRSim3DTP::step, src/driver/multires_sim3d_tp.cpp:671-734."""
         st = self.ctx.mr_step(self.step_cfg())
+        _store_step_stats(self, st)
         self.interface_diagnostics_last = st.iface
         self.surface_tension_stats_last = st.surface
         self.stats_last = st

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 3
+#define PF_ABI_VERSION 4
 
 enum pf_status {
   PF_OK = 0,
@@ -99,6 +99,33 @@ enum pf_model {
   PF_MODEL_SINGLE_PHASE = 1  /* SparseSim3D::step, sparse_sim3d.cpp:106-115 */
 };
 
+/* The particle adaptivity controls of SparseSim3DTP / MRSim3DTP
+ * (sparse_sim3d_tp.h:29-60, multires_sim3d_tp.h:80-111), under the same names.
+ * pf_adapt_defaults gives the reference's defaults (every stage off). */
+typedef struct pf_adapt_cfg {
+  int32_t narrow_band_air, narrow_band_air_radius;
+  int32_t gas_particle_coarsening, gas_particles_per_cell_target;
+  uint32_t gas_particle_coarsening_seed;
+  int32_t liquid_particle_coarsening, liquid_particles_per_cell_target;
+  uint32_t liquid_particle_coarsening_seed;
+  int32_t liquid_particle_refill, liquid_particle_refill_interface_only;
+  int32_t liquid_particle_refill_interface_radius, liquid_particle_refill_max_added_per_step;
+  int32_t liquid_refill_particles_per_cell_target;
+  uint32_t liquid_particle_refill_seed;
+} pf_adapt_cfg;
+
+/* NarrowBandAirResult3D, GasParticleCoarseningResult3D (gas, then liquid) and
+ * LiquidParticleRefillResult3D (particle_adaptivity3d.h:50-73): the *_last fields */
+typedef struct pf_adapt_stats {
+  int32_t narrow_band_air_removed, narrow_band_air_liquid_cells, narrow_band_air_gas_before,
+      narrow_band_air_gas_after;
+  int32_t gas_removed, gas_cells, gas_overfull_cells, gas_before, gas_after;
+  int32_t liquid_removed, liquid_cells, liquid_overfull_cells, liquid_before, liquid_after;
+  int32_t refill_added, refill_cells, refill_interface_cells, refill_underfull_cells, refill_budget_limited,
+      refill_before, refill_after;
+  int32_t reserved;
+} pf_adapt_stats;
+
 /* The SparseSim3DTP fields that step() reads (src/driver/sparse_sim3d_tp.h:12-28).
  * For PF_MODEL_SINGLE_PHASE (SparseSim3D, sparse_sim3d.h:5-15): alpha_liquid is the
  * one FLIP ratio, phase, Vp, alpha_gas and c_div are unused. */
@@ -118,6 +145,12 @@ typedef struct pf_step_cfg {
   /* 1: leave pf_step_stats.iface zero instead of computing the interface
    * diagnostics the reference computes every step (sparse_sim3d_tp.cpp:865) */
   int32_t skip_interface_diagnostics, reserved;
+  /* updateVolumeCorrectionStats (sparse_sim3d_tp.cpp:368-379): when set, the step
+   * computes c_div from the liquid volume after particle adaptivity and the step's
+   * dt, and the c_div field above is ignored */
+  int32_t c_div_volume_correction, reserved2;
+  double c_div_strength, liquid_volume_target;
+  pf_adapt_cfg adapt;        /* applyParticleAdaptivity, first in the step (:855) */
 } pf_step_cfg;
 
 enum pf_stage {
@@ -149,6 +182,8 @@ typedef struct pf_step_stats {
   double total_ms;
   pf_iface_stats iface;          /* SparseSim3DTP::interface_diagnostics_last */
   pf_surface_stats surface;      /* SparseSim3DTP::surface_tension_stats_last */
+  double liquid_volume, c_div;   /* liquid_volume_current_last, c_div_last */
+  pf_adapt_stats adapt;          /* the adaptivity *_last fields */
 } pf_step_stats;
 
 typedef struct pf_ctx pf_ctx;
@@ -168,6 +203,12 @@ int pf_particles_upload(pf_ctx* ctx, int64_t n, const float* x, const float* y, 
                         const float* u, const float* v, const float* w, const uint8_t* type,
                         const float* volume);
 int pf_particles_download(pf_ctx* ctx, float* x, float* y, float* z, float* u, float* v, float* w);
+/* particle types and volumes in the caller's order (either may be NULL; volumes are
+ * 1.0 when none were given and no adaptivity changed them) */
+int pf_particles_download_ext(pf_ctx* ctx, uint8_t* type, float* volume);
+/* liquidVolume(particles, 1.0) (sparse_sim3d_tp.cpp:353-359): the summed volume of the
+ * liquid particles */
+double pf_liquid_volume(const pf_ctx* ctx);
 int64_t pf_particle_count(const pf_ctx* ctx);
 /* Seeded synthetic scene generated on the device (same construction as the oracle's
  * scene generator): 0 Rayleigh-Taylor, 1 bubble tank, 2 dam-break. */
@@ -207,6 +248,14 @@ int pf_interface_diagnostics(pf_ctx* ctx, const pf_phase* phase, pf_iface_stats*
  * velocity increments added to the faces with positive mass; out may be NULL */
 int pf_apply_surface_tension(pf_ctx* ctx, const pf_phase* phase, double dt, double strength,
                              double max_delta_speed, int32_t smoothing_radius, pf_surface_stats* out);
+/* SparseSim3DTP::applyParticleAdaptivity, sparse_sim3d_tp.cpp:847-852 (the same in
+ * MRSim3DTP, multires_sim3d_tp.cpp:570-575): applyNarrowBandAir,
+ * applyGasParticleCoarsening, applyTypedParticleCoarsening for liquid and
+ * applyLiquidParticleRefill (particle_adaptivity3d.h:112-374) on the context's
+ * particles, each stage only when enabled.  Survivors keep their order and refilled
+ * particles follow them, so downloads return the reference's particle order. */
+int pf_particle_adaptivity(pf_ctx* ctx, const pf_adapt_cfg* cfg, pf_adapt_stats* stats);
+void pf_adapt_defaults(pf_adapt_cfg* cfg);
 /* computeAdaptiveParticleTimestep3D, src/driver/timestep3d.h:16-50 */
 int pf_timestep(pf_ctx* ctx, double requested_dt, int32_t enabled, double cfl, double min_dt,
                 double* effective_dt, double* max_speed, double* cfl_limit_dt, int32_t* limited);
@@ -332,6 +381,10 @@ typedef struct pf_mr_step_cfg {     /* the MRSim3DTP fields step() reads */
   int32_t surface_tension, surface_tension_smoothing_radius;
   double surface_tension_strength, surface_tension_max_delta_speed;
   int32_t skip_interface_diagnostics, reserved2;
+  /* volume correction and particle adaptivity, as in pf_step_cfg */
+  int32_t c_div_volume_correction, reserved3;
+  double c_div_strength, liquid_volume_target;
+  pf_adapt_cfg adapt;
 } pf_mr_step_cfg;
 
 /* MRSim3DTP's solver defaults (multires_sim3d_tp.h:34-60) in PF_MR_SOLVE_FAST mode:

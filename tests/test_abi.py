@@ -38,6 +38,9 @@ def test_struct_layouts_match_the_header(tmp_path):
                    "printf(\"%zu %zu %zu %zu %zu %zu\\n\",sizeof(pf_iface_stats),sizeof(pf_surface_stats),"
                    "offsetof(pf_step_cfg,surface_tension_strength),offsetof(pf_step_cfg,skip_interface_diagnostics),"
                    "offsetof(pf_step_stats,iface),offsetof(pf_step_stats,surface));"
+                   "printf(\"%zu %zu %zu %zu %zu %zu\\n\",sizeof(pf_adapt_cfg),sizeof(pf_adapt_stats),"
+                   "offsetof(pf_step_cfg,liquid_volume_target),offsetof(pf_step_cfg,adapt),"
+                   "offsetof(pf_step_stats,c_div),offsetof(pf_step_stats,adapt));"
                    "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -47,7 +50,9 @@ def test_struct_layouts_match_the_header(tmp_path):
             C.sizeof(G.EscapeStats), C.sizeof(G.StepCfg), C.sizeof(G.StepStats),
             G.StepCfg.phase.offset, G.StepStats.solve.offset,
             C.sizeof(G.IfaceStats), C.sizeof(G.SurfaceStats), G.StepCfg.surface_tension_strength.offset,
-            G.StepCfg.skip_interface_diagnostics.offset, G.StepStats.iface.offset, G.StepStats.surface.offset]
+            G.StepCfg.skip_interface_diagnostics.offset, G.StepStats.iface.offset, G.StepStats.surface.offset,
+            C.sizeof(G.AdaptCfg), C.sizeof(G.AdaptStats), G.StepCfg.liquid_volume_target.offset,
+            G.StepCfg.adapt.offset, G.StepStats.c_div.offset, G.StepStats.adapt.offset]
     assert got == want
 
 
@@ -61,7 +66,7 @@ def test_host_helpers_without_device():
     for raw in (0.0, 5.0, 7.0, 100.0, 900.0, 2000.0):
         assert L.pf_phi_from_raw(raw, C.byref(ph)) == O.phi(raw, oph)
     assert L.pf_beta_from_phi(0.25, C.byref(ph)) == O.beta(0.25, oph)
-    assert L.pf_abi_version() == 3
+    assert L.pf_abi_version() == 4
 
 
 def test_null_arguments_are_rejected_without_device():
@@ -124,3 +129,14 @@ def test_multires_solve_defaults_are_the_reference_defaults():
     assert (c.coarse_preconditioner_auto_disable, c.coarse_preconditioner_auto_disable_after) == (0, 4)
     with pytest.raises(TypeError):
         G.mr_solve_cfg(no_such_option=1)
+
+
+def test_adaptivity_defaults_are_the_reference_defaults():
+    # sparse_sim3d_tp.h:29-60: every stage off, radius 2, targets 4, interface radius 1
+    import pyoracle as O
+    g = G.adapt_cfg()
+    o = O.adapt_cfg()
+    assert bytes(g) == bytes(o)
+    assert g.gas_particle_coarsening_seed == 0x9E3779B9 and g.liquid_particle_refill_seed == 0x6A09E667
+    assert not (g.narrow_band_air or g.gas_particle_coarsening or g.liquid_particle_coarsening
+                or g.liquid_particle_refill)

@@ -153,7 +153,6 @@ def _sim(n, dx, ctx, oph, csf=True, diag=True):
     sim.phase = _gphase(oph)
     sim.dt = 2e-3
     sim.stop_rule, sim.cg_tol, sim.cg_iters = G.STOP_REL2, 1e-9, 20000
-    sim._liquid_volume = 0.0
     sim.surface_tension = csf
     sim.surface_tension_strength = 0.3
     sim.surface_tension_max_delta_speed = 0.1

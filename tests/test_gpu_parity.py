@@ -130,7 +130,6 @@ def test_full_step_matches_fp64_oracle(kind):
     sim.phase = _gphase(oph)
     sim.dt = 1e-3
     sim.stop_rule, sim.cg_tol, sim.cg_iters = G.STOP_REL2, 1e-9, 20000
-    sim._liquid_volume = 0.0
     g = O.Grid(n, n, n, dx, faithful=False)
     cfg = O.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95,
                     adaptive_timestep=0, advection_order=2, cg_iters=20000, stop_rule=O.STOP_REL2,
@@ -210,7 +209,6 @@ def test_solve_reaches_fp64_tolerance(kind, n):
     ph = phase()
     sim = G.SparseSim3DTP(n, n, n, 1.0 / n)
     sim.ctx = ctx
-    sim._liquid_volume = 0.0
     sim.phase = _gphase(ph)
     sim.dt = 1e-3
     sim.stop_rule, sim.cg_tol, sim.cg_iters = G.STOP_REL2, 1e-6, 50000
