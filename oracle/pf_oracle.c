@@ -1290,3 +1290,54 @@ void orc_surface_tension(orc_grid* g, const orc_phase* pp, double dt, double str
       }
   orc_st_finish(S);
 }
+
+/* ------------------------------------------------ render-cache cells */
+/* sparsePhaseCells (:557-588) needs sparseCellPhi (:279-300) over the marker's active
+ * blocks in allocation order (the pressure field's when the marker has none) */
+int orc_render_cache_sparse(const orc_grid* g, const orc_phase* pp, int64_t n, const double* x, const double* y,
+                            const double* z, const double* u, const double* v, const double* w,
+                            const uint8_t* type, const double* vol, const orc_render_meta* m, const char* path) {
+  int64_t nb = orc_active_blocks(g, ORC_MK, NULL);
+  int field = ORC_MK;
+  if (nb == 0) { field = ORC_P; nb = orc_active_blocks(g, ORC_P, NULL); }
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nb > 0 ? nb : 1));
+  orc_active_blocks(g, field, order);
+  int32_t* q = (int32_t*)malloc(sizeof(int32_t) * 5 * 64 * (size_t)(nb > 0 ? nb : 1));
+  double* pvv = (double*)malloc(sizeof(double) * 2 * 64 * (size_t)(nb > 0 ? nb : 1));
+  const int nbx = (g->nx + 3) / 4, nby = (g->ny + 3) / 4;
+  const double cv = g->dx * g->dx * g->dx;
+  double pv = 0.0;
+  int64_t nc = 0;
+  for (int64_t a = 0; a < nb; ++a) {
+    const int b = order[a], bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    for (int lz = 0; lz < 4; ++lz)
+      for (int ly = 0; ly < 4; ++ly)
+        for (int lx = 0; lx < 4; ++lx) {
+          const int i = bx * 4 + lx, j = by * 4 + ly, k = bz * 4 + lz;
+          if (i >= g->nx || j >= g->ny || k >= g->nz) continue;
+          const int mk = (int)(orc_get(g, ORC_MK, i, j, k) + 0.5);
+          double s = 0.0;
+          s += orc_phi(orc_get(g, ORC_MU, i, j, k), pp);
+          s += orc_phi(orc_get(g, ORC_MU, i + 1, j, k), pp);
+          s += orc_phi(orc_get(g, ORC_MV, i, j, k), pp);
+          s += orc_phi(orc_get(g, ORC_MV, i, j + 1, k), pp);
+          s += orc_phi(orc_get(g, ORC_MW, i, j, k), pp);
+          s += orc_phi(orc_get(g, ORC_MW, i, j, k + 1), pp);
+          const double phi = s / 6.0;
+          if (mk == 0 && phi <= 1e-8) continue;
+          const double wv = phi * cv;
+          pv += wv;
+          int32_t* r = q + 5 * nc;
+          r[0] = i; r[1] = j; r[2] = k; r[3] = 0; r[4] = mk;
+          pvv[2 * nc] = phi;
+          pvv[2 * nc + 1] = wv;
+          ++nc;
+        }
+  }
+  const int r = orc_render_frame_cells(path, "sparse3d_tp", g->nx, g->ny, g->nz, g->dx, m, q, pvv, nc, pv, n, x, y, z,
+                                       u, v, w, type, vol);
+  free(pvv);
+  free(q);
+  free(order);
+  return r;
+}

@@ -325,6 +325,32 @@ int64_t orc_particle_adaptivity(const orc_pa_domain* d, const orc_adapt_cfg* c, 
                                 double* y, double* z, double* u, double* v, double* w, uint8_t* type, double* vol,
                                 orc_adapt_stats* st);
 
+/* ---- render-cache export (pf_oracle_render.c): src/driver/render_cache3d.h */
+typedef struct orc_render_meta {
+  int32_t frame, pad0;
+  double time, dt, Vp;
+  orc_phase phase;
+  double camera[13];   /* position, target, up, fov_degrees, near_clip, far_clip, focal_length_mm */
+} orc_render_meta;
+void orc_render_default_camera(int nx, int ny, int nz, double dx, double cam[13]);
+/* writeSparseRenderCache3D: 0, -1 on invalid metadata, -2 on an I/O error */
+int orc_render_cache_sparse(const orc_grid* g, const orc_phase* pp, int64_t n, const double* x, const double* y,
+                            const double* z, const double* u, const double* v, const double* w,
+                            const uint8_t* type, const double* vol, const orc_render_meta* m, const char* path);
+/* writeMRRenderCache3D */
+int orc_render_cache_mr(const orc_mr* g, const orc_phase* pp, int64_t n, const double* x, const double* y,
+                        const double* z, const double* u, const double* v, const double* w, const uint8_t* type,
+                        const double* vol, const orc_render_meta* m, const char* path);
+int orc_render_frame_cells(const char* path, const char* kind, int nx, int ny, int nz, double dx,
+                           const orc_render_meta* m, const int32_t* cell_ijk_level_marker, const double* cell_phi_vol,
+                           int64_t ncell, double phase_vol, int64_t n, const double* x, const double* y,
+                           const double* z, const double* u, const double* v, const double* w, const uint8_t* type,
+                           const double* vol);
+/* writeRenderCacheManifest3D */
+int orc_render_manifest(const char* path, const char* kind, int nx, int ny, int nz, double dx, int64_t nframes,
+                        const int32_t* frame, const int32_t* step, const double* time, const char* const* paths,
+                        const int64_t* bytes);
+
 /* seeded synthetic scenes (shared construction with the HIP generator) */
 enum { ORC_SCENE_RT = 0, ORC_SCENE_BUBBLE = 1, ORC_SCENE_DAMBREAK = 2 };
 int64_t orc_scene_count(int kind, int nx, int ny, int nz);

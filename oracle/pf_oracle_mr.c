@@ -1343,6 +1343,44 @@ void orc_mr_surface_tension(orc_mr* g, const orc_phase* pp, double dt, double st
   orc_st_finish(S);
 }
 
+/* mrPhaseCells (render_cache3d.h:590-613): each leaf cell in leafCells order whose
+ * corner lies in the grid, sampled by mrCellPhi (:302-318) at its centre clamped to
+ * the grid; kept unless AIR with phi <= 1e-8 */
+int orc_render_cache_mr(const orc_mr* g, const orc_phase* pp, int64_t n, const double* x, const double* y,
+                        const double* z, const double* u, const double* v, const double* w, const uint8_t* type,
+                        const double* vol, const orc_render_meta* m, const char* path) {
+  const orc_iface_src s = mr_src(g);
+  const int64_t nl = (int64_t)g->nleaves * 64;
+  int32_t* q = (int32_t*)malloc(sizeof(int32_t) * 5 * (size_t)(nl > 0 ? nl : 1));
+  double* pv = (double*)malloc(sizeof(double) * 2 * (size_t)(nl > 0 ? nl : 1));
+  double total = 0.0;
+  int64_t nc = 0;
+  for (int64_t id = 0; id < nl; ++id) {
+    int x0, y0, z0, st;
+    cell_box(g, id, &x0, &y0, &z0, &st);
+    if (x0 >= g->nx || y0 >= g->ny || z0 >= g->nz) continue;
+    const int ci = x0 + st / 2 < g->nx - 1 ? x0 + st / 2 : g->nx - 1;
+    const int cj = y0 + st / 2 < g->ny - 1 ? y0 + st / 2 : g->ny - 1;
+    const int ck = z0 + st / 2 < g->nz - 1 ? z0 + st / 2 : g->nz - 1;
+    const int mk = (int)(g->mk[id] + 0.5);
+    const double phi = orc_iface_cell_phi(&s, pp, ci, cj, ck);
+    if (mk == 0 && phi <= 1e-8) continue;
+    const double cd = g->dx * (double)st;
+    const double wv = phi * cd * cd * cd;
+    total += wv;
+    int32_t* r = q + 5 * nc;
+    r[0] = x0; r[1] = y0; r[2] = z0; r[3] = g->leaves[4 * (id / 64)]; r[4] = mk;
+    pv[2 * nc] = phi;
+    pv[2 * nc + 1] = wv;
+    ++nc;
+  }
+  const int r = orc_render_frame_cells(path, "multires3d_tp", g->nx, g->ny, g->nz, g->dx, m, q, pv, nc, total, n, x,
+                                       y, z, u, v, w, type, vol);
+  free(pv);
+  free(q);
+  return r;
+}
+
 /* MRSim3DTP::step, multires_sim3d_tp.cpp:671-734 (no particle adaptivity or
  * secondary particles; interface diagnostics and optional surface tension) */
 void orc_mr_step(orc_mr* g, orc_mr_sim* sim, int64_t n, double* x, double* y, double* z, double* u, double* v,

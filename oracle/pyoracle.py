@@ -182,6 +182,41 @@ def ref_particle_adaptivity(domain: PaDomain, cfg: AdaptCfg, ps):
     return Particles(*(q[:m] for q in a), type=t[:m], volume=vol[:m]), list(st[:21])
 
 
+class RenderMeta(C.Structure):
+    """frame, time, dt, Vp, phase and camera of one render-cache frame"""
+    _fields_ = [("frame", C.c_int32), ("pad0", C.c_int32), ("time", C.c_double), ("dt", C.c_double),
+                ("Vp", C.c_double), ("phase", Phase), ("camera", C.c_double * 13)]
+
+
+def default_camera(nx, ny, nz, dx):
+    """defaultRenderCacheCamera3D (render_cache3d.h:48-62): position, target, up, fov, near, far, focal"""
+    cam = (C.c_double * 13)()
+    lib().orc_render_default_camera(nx, ny, nz, dx, cam)
+    return cam
+
+
+def render_meta(nx, ny, nz, dx, phase, frame=0, time=0.0, dt=0.0, Vp=1.0, camera=None):
+    return RenderMeta(frame, 0, time, dt, Vp, phase, camera if camera is not None else default_camera(nx, ny, nz, dx))
+
+
+def _render_particles(ps):
+    vol = ps.volume if ps.volume is not None else np.ones(len(ps))
+    return (len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, ps.type, np.ascontiguousarray(vol, np.float64))
+
+
+def render_manifest(path, kind, nx, ny, nz, dx, frames, ref=False):
+    """writeRenderCacheManifest3D; frames: (frame, step, time, path, bytes) tuples"""
+    n = len(frames)
+    fr = np.array([f[0] for f in frames], np.int32)
+    st = np.array([f[1] for f in frames], np.int32)
+    tm = np.array([f[2] for f in frames], np.float64)
+    by = np.array([f[4] for f in frames], np.int64)
+    paths = (C.c_char_p * max(n, 1))(*[f[3].encode() for f in frames])
+    L = reflib() if ref else lib()
+    fn = L.ref_render_manifest if ref else L.orc_render_manifest
+    return fn(path.encode(), kind.encode(), nx, ny, nz, dx, n, fr, st, tm, paths, by)
+
+
 class MRSolveCfg(C.Structure):
     _fields_ = [("max_iterations", C.c_int32), ("use_jacobi", C.c_int32), ("use_flexible_beta", C.c_int32),
                 ("adaptive_restart", C.c_int32), ("absolute_tolerance", C.c_double),
@@ -362,6 +397,13 @@ def lib():
                                          C.c_uint32]
         L.orc_slot_score.restype = C.c_uint32
         L.orc_slot_score.argtypes = [C.c_uint64, C.c_int, C.c_uint32]
+        L.orc_render_default_camera.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p]
+        _rp = [C.c_int64] + [_dp] * 6 + [_u8, _dp]
+        L.orc_render_cache_sparse.argtypes = [C.c_void_p, C.c_void_p] + _rp + [C.c_void_p, C.c_char_p]
+        L.orc_render_cache_mr.argtypes = [C.c_void_p, C.c_void_p] + _rp + [C.c_void_p, C.c_char_p]
+        L.orc_render_manifest.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int64,
+                                          _i32, _i32, _dp, C.c_void_p,
+                                          np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.orc_mr_interface_diagnostics.argtypes = [C.c_void_p, C.POINTER(Phase), C.POINTER(IfaceStats)]
         L.orc_mr_surface_tension.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, C.c_double, C.c_double,
                                              C.c_int, C.POINTER(StStats)]
@@ -514,6 +556,11 @@ class Grid:
         lib().orc_surface_tension(self.h, C.byref(phase), dt, strength, max_delta_speed, radius, C.byref(st))
         return st
 
+    def render_cache(self, ps, phase, meta, path):
+        """writeSparseRenderCache3D of this grid and the particles ps"""
+        return lib().orc_render_cache_sparse(self.h, C.byref(phase), *_render_particles(ps), C.byref(meta),
+                                             str(path).encode())
+
     def interface_fields(self, phase):
         """(phi, |grad phi|, curvature) per cell, arrays [nz, ny, nx]"""
         n = self.nx * self.ny * self.nz
@@ -635,6 +682,12 @@ def reflib():
         L.ref_mr_face_valid.argtypes = [C.c_void_p, C.c_int, _u8]
         L.ref_mr_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.ref_mr_interface_diagnostics.argtypes = [C.c_void_p, _dp]
+        L.ref_render_sparse.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_double]
+        L.ref_render_mr.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_double]
+        L.ref_render_manifest.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int64,
+                                          _i32, _i32, _dp, C.c_void_p,
+                                          np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+        L.ref_render_default_camera.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p]
         _u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
         L.ref_pa_apply.restype = C.c_int64
         L.ref_pa_apply.argtypes = [_i32, _dp, _u32, C.c_int64, C.c_int64] + [_dp] * 6 + [_u8, _dp, _i32]
@@ -742,6 +795,11 @@ class MRGrid:
         es = EscapeStats()
         lib().orc_mr_advect(self.h, len(ps), ps.x, ps.y, ps.z, ps.type, dt, order, C.byref(es))
         return es
+
+    def render_cache(self, ps, phase, meta, path):
+        """writeMRRenderCache3D of this grid and the particles ps"""
+        return lib().orc_render_cache_mr(self.h, C.byref(phase), *_render_particles(ps), C.byref(meta),
+                                         str(path).encode())
 
     def load_dense(self, f, a):
         """write the non-zero entries of a dense [z, y, x] array: fine faces, or for
@@ -894,6 +952,9 @@ class RefSim:
         reflib().ref_sim_last_interface(self.h, out)
         return out[:11], out[11:]
 
+    def render_cache(self, path, frame=0, time=0.0):
+        return reflib().ref_render_sparse(self.h, str(path).encode(), frame, time)
+
     def set_adapt(self, cfg: AdaptCfg):
         reflib().ref_sim_set_adapt(self.h, cfg.words())
 
@@ -1041,6 +1102,9 @@ class RefMR:
         out = np.zeros(14)
         reflib().ref_mr_surface_tension(self.h, dt, strength, max_delta_speed, radius, out)
         return out
+
+    def render_cache(self, path, frame=0, time=0.0):
+        return reflib().ref_render_mr(self.h, str(path).encode(), frame, time)
 
     def set_adapt(self, cfg: AdaptCfg):
         reflib().ref_mr_set_adapt(self.h, cfg.words())

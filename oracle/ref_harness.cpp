@@ -12,9 +12,11 @@
 // (src/driver/sparse_ops3d.h:5-9); and the multires MRSim3DTP
 // (src/driver/multires_sim3d_tp.h) with mrP2G3D_tp / mrG2P3D_tp / mrAdvect3D_tp
 // (src/driver/multires_ops3d_tp.h) and projectMR3D (src/pressure/multires_pressure3d.h);
-// the particle adaptivity of src/driver/particle_adaptivity3d.h.
+// the particle adaptivity of src/driver/particle_adaptivity3d.h; the render-cache
+// writers of src/driver/render_cache3d.h.
 #include "driver/multires_ops3d_tp.h"
 #include "driver/particle_adaptivity3d.h"
+#include "driver/render_cache3d.h"
 #include "driver/multires_sim3d_tp.h"
 #include "driver/sparse_ops3d.h"
 #include "driver/sparse_ops3d_tp.h"
@@ -580,6 +582,48 @@ int64_t ref_mr_particles_full(void* sp, double* x, double* y, double* z, double*
 }
 void ref_mr_set_adapt(void* sp, const uint32_t* c) { set_adapt(mrs(sp), c); }
 void ref_mr_get_adapt(void* sp, int32_t* o) { get_adapt(mrs(sp), o); }
+
+// writeSparseRenderCache3D / writeMRRenderCache3D with the default camera; 0, or -1
+// when the writer throws
+int ref_render_sparse(void* sp, const char* path, int frame, double time) {
+  auto* s = static_cast<SparseSim3DTP*>(sp);
+  try {
+    writeSparseRenderCache3D(*s, path, frame, time,
+                             defaultRenderCacheCamera3D(s->grid.nx, s->grid.ny, s->grid.nz, s->grid.dx));
+  } catch (const std::exception&) {
+    return -1;
+  }
+  return 0;
+}
+int ref_render_mr(void* sp, const char* path, int frame, double time) {
+  auto* s = mrs(sp);
+  try {
+    writeMRRenderCache3D(*s, path, frame, time,
+                         defaultRenderCacheCamera3D(s->layout.nx, s->layout.ny, s->layout.nz, s->layout.dx));
+  } catch (const std::exception&) {
+    return -1;
+  }
+  return 0;
+}
+// writeRenderCacheManifest3D; 0, or -1 when it throws
+int ref_render_manifest(const char* path, const char* kind, int nx, int ny, int nz, double dx, int64_t n,
+                        const int32_t* frame, const int32_t* step, const double* time, const char* const* paths,
+                        const int64_t* bytes) {
+  std::vector<RenderCacheManifestFrame3D> fr((size_t)n);
+  for (int64_t a = 0; a < n; ++a) fr[(size_t)a] = RenderCacheManifestFrame3D{frame[a], step[a], time[a], paths[a], bytes[a]};
+  try {
+    writeRenderCacheManifest3D(path, kind, nx, ny, nz, dx, fr);
+  } catch (const std::exception&) {
+    return -1;
+  }
+  return 0;
+}
+void ref_render_default_camera(int nx, int ny, int nz, double dx, double* out) {
+  const RenderCacheCamera3D c = defaultRenderCacheCamera3D(nx, ny, nz, dx);
+  const double v[13] = {c.position.x, c.position.y, c.position.z, c.target.x, c.target.y, c.target.z,
+                        c.up.x, c.up.y, c.up.z, c.fov_degrees, c.near_clip, c.far_clip, c.focal_length_mm};
+  std::memcpy(out, v, sizeof(v));
+}
 
 void ref_mr_set(void* sp, int f, int i, int j, int k, double v) {
   auto& g = mrs(sp)->grid;
