@@ -50,6 +50,7 @@ def build_host_demo(verbose: bool = False) -> str:
     cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", HOST,
            os.path.join(HOST, "sim_demo.cpp"), os.path.join(HOST, "sparse_sim3d_tp_gpu.cpp"),
            os.path.join(HOST, "sparse_sim3d_gpu.cpp"), os.path.join(HOST, "multires_sim3d_tp_gpu.cpp"),
+           os.path.join(HOST, "render_cache3d_gpu.cpp"),
            "-o", out, "-L", OUT_DIR, "-lpfflip_hip", f"-Wl,-rpath,{OUT_DIR}"]
     if verbose:
         print(" ".join(cmd), flush=True)

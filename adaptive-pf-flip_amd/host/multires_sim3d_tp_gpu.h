@@ -117,7 +117,7 @@ class MRSim3DTP : public ParticleAdaptivityFields {
 
   void markParticlesDirty() { dirty_ = true; }
   void syncParticlesToHost();
-  pf_ctx* context() { return ctx_; }
+  pf_ctx* context() const { return ctx_; }
 
  private:
   void check(int status, const char* what);

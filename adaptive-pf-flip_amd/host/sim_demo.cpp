@@ -19,12 +19,18 @@
 #include <cstdio>
 
 #include "multires_sim3d_tp_gpu.h"
+#include "render_cache3d_gpu.h"
 #include "sparse_sim3d_gpu.h"
 #include "sparse_sim3d_tp_gpu.h"
 
 using pfgpu::MRSim3DTP;
 using pfgpu::SparseSim3D;
 using pfgpu::SparseSim3DTP;
+using pfgpu::RenderCacheCamera3D;
+using pfgpu::RenderCacheManifestFrame3D;
+using pfgpu::defaultRenderCacheCamera3D;
+using pfgpu::writeRenderCacheManifest3D;
+using pfgpu::writeSparseRenderCache3D;
 
 static int failures = 0;
 #define CHECK(cond)                                                   \
@@ -344,6 +350,32 @@ int main() {
     CHECK(std::fabs(lv - sim.liquid_volume_current_last) <= 1e-4 * lv);
     std::printf("adaptivity: %zu -> %zu particles (%d removed, %d refilled), liquid volume %.3f\n", n0,
                 sim.particles.size(), removed, sim.liquid_particle_refill_added_last, lv);
+  }
+  {
+    // render-cache export (export_render_cache3d.cpp's loop): frames and a manifest
+    SparseSim3DTP sim(12, 12, 12, 1.0);
+    sim.dt = 0.01;
+    sim.initBubbleTank();
+    std::vector<RenderCacheManifestFrame3D> frames;
+    const RenderCacheCamera3D cam = defaultRenderCacheCamera3D(12, 12, 12, 1.0);
+    for (int f = 0; f < 2; ++f) {
+      sim.step();
+      const std::string path = "/tmp/pf_demo_frame_" + std::to_string(f) + ".jsonl";
+      const long long bytes = writeSparseRenderCache3D(sim, path, f, 0.01 * (f + 1), cam);
+      frames.push_back(RenderCacheManifestFrame3D{f, f + 1, 0.01 * (f + 1), path, bytes});
+      CHECK(bytes > 0);
+    }
+    writeRenderCacheManifest3D("/tmp/pf_demo_manifest.json", "sparse3d_tp", 12, 12, 12, 1.0, frames);
+    bool threw = false;
+    RenderCacheCamera3D bad = cam;
+    bad.far_clip = bad.near_clip;
+    try {
+      writeSparseRenderCache3D(sim, "/tmp/pf_demo_bad.jsonl", 0, 0.0, bad);
+    } catch (const std::invalid_argument&) {
+      threw = true;
+    }
+    CHECK(threw);
+    std::printf("render cache: 2 frames, %lld and %lld bytes\n", frames[0].bytes, frames[1].bytes);
   }
   std::printf(failures ? "FAILED %d checks\n" : "all checks passed\n", failures);
   return failures ? 1 : 0;

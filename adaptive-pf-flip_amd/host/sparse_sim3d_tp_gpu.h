@@ -172,7 +172,7 @@ class SparseSim3DTP : public ParticleAdaptivityFields {
   int ny() const { return ny_; }
   int nz() const { return nz_; }
   double dx() const { return dx_; }
-  pf_ctx* context() { return ctx_; }
+  pf_ctx* context() const { return ctx_; }
 
  private:
   void check(int status, const char* what);

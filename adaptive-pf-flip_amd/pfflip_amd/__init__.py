@@ -162,6 +162,41 @@ class AdaptStats(C.Structure):
         return [getattr(self, n) for n, _ in self._fields_ if n != "reserved"]
 
 
+class RenderCamera(C.Structure):
+    """pf_render_camera: RenderCacheCamera3D (render_cache3d.h:19-27)"""
+    _fields_ = [("position", C.c_double * 3), ("target", C.c_double * 3), ("up", C.c_double * 3),
+                ("fov_degrees", C.c_double), ("near_clip", C.c_double), ("far_clip", C.c_double),
+                ("focal_length_mm", C.c_double)]
+
+
+class RenderFrame(C.Structure):
+    """pf_render_frame: frame, time, dt (effective_dt_last), Vp, phase, camera"""
+    _fields_ = [("frame", C.c_int32), ("reserved", C.c_int32), ("time", C.c_double), ("dt", C.c_double),
+                ("Vp", C.c_double), ("phase", Phase), ("camera", RenderCamera)]
+
+
+class RenderManifestFrame(C.Structure):
+    """pf_render_manifest_frame: RenderCacheManifestFrame3D (render_cache3d.h:39-45)"""
+    _fields_ = [("frame", C.c_int32), ("step", C.c_int32), ("time", C.c_double), ("path", C.c_char_p),
+                ("bytes", C.c_int64)]
+
+
+def default_render_camera(nx, ny, nz, dx) -> RenderCamera:
+    """defaultRenderCacheCamera3D (render_cache3d.h:48-62)"""
+    cam = RenderCamera()
+    lib().pf_render_default_camera(nx, ny, nz, dx, C.byref(cam))
+    return cam
+
+
+def write_render_manifest(path, sim_kind, nx, ny, nz, dx, frames):
+    """writeRenderCacheManifest3D; frames: (frame, step, time, path, bytes) tuples"""
+    arr = (RenderManifestFrame * max(len(frames), 1))(
+        *[RenderManifestFrame(f[0], f[1], f[2], str(f[3]).encode(), f[4]) for f in frames])
+    st = lib().pf_write_render_manifest(str(path).encode(), sim_kind.encode(), nx, ny, nz, dx, arr, len(frames))
+    if st != 0:
+        raise PfError(f"pf_write_render_manifest: status {st}: invalid render cache manifest")
+
+
 class StepCfg(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("Vp", C.c_double),
                 ("alpha_liquid", C.c_double), ("alpha_gas", C.c_double),
@@ -258,6 +293,9 @@ def lib():
                                                    C.POINTER(SurfaceStats)]),
             "pf_mr_interface_diagnostics": (C.c_int, [vp, C.POINTER(Phase), C.POINTER(IfaceStats)]),
             "pf_particle_adaptivity": (C.c_int, [vp, C.POINTER(AdaptCfg), C.POINTER(AdaptStats)]),
+            "pf_render_default_camera": (None, [i32, i32, i32, dbl, C.POINTER(RenderCamera)]),
+            "pf_write_render_cache": (C.c_int, [vp, C.c_char_p, C.POINTER(RenderFrame), C.POINTER(i64)]),
+            "pf_write_render_manifest": (C.c_int, [C.c_char_p, C.c_char_p, i32, i32, i32, dbl, vp, i64]),
             "pf_adapt_defaults": (None, [C.POINTER(AdaptCfg)]),
             "pf_particles_download_ext": (C.c_int, [vp, vp, vp]),
             "pf_liquid_volume": (dbl, [vp]),
@@ -389,6 +427,13 @@ class Context:
     def liquid_volume(self):
         """the summed volume of the liquid particles (liquidVolume with Vp = 1)"""
         return lib().pf_liquid_volume(self.h)
+
+    def write_render_cache(self, path, info: RenderFrame) -> int:
+        """writeSparseRenderCache3D / writeMRRenderCache3D of this context; returns the bytes"""
+        nb = C.c_int64()
+        self._check(lib().pf_write_render_cache(self.h, str(path).encode(), C.byref(info), C.byref(nb)),
+                    "pf_write_render_cache")
+        return nb.value
 
     def particle_adaptivity(self, cfg: AdaptCfg) -> AdaptStats:
         """applyParticleAdaptivity on the context's particles"""
@@ -615,6 +660,12 @@ class SparseSim3DTP:
                        c_div_volume_correction=1,
                        c_div_strength=self.c_div_strength if self.c_div_volume_correction else 0.0,
                        liquid_volume_target=self.liquid_volume_target, adapt=_adapt_of(self))
+
+    def write_render_cache(self, path, frame, time, camera=None):
+        """writeSparseRenderCache3D(sim, path, frame, time, camera) (render_cache3d.h:617-658)"""
+        cam = camera or default_render_camera(self.nx, self.ny, self.nz, self.dx)
+        return self.ctx.write_render_cache(path, RenderFrame(frame, 0, time, self.effective_dt_last, self.Vp,
+                                                             self.phase, cam))
 
     def step(self):
         """SparseSim3DTP::step, src/driver/sparse_sim3d_tp.cpp:854-884 (particle
@@ -869,6 +920,7 @@ class MRSim3DTP:
         # updateVolumeCorrectionStats (multires_sim3d_tp.cpp:439-450)
         self.c_div_volume_correction, self.c_div_strength, self.liquid_volume_target = False, 1.0, 0.0
         self.liquid_volume_current_last = self.liquid_volume_error_last = 0.0
+        self.effective_dt_last = self.dt
         self.c_div_last = 0.0
         self.timing = False
         self.stats_last = None
@@ -897,10 +949,17 @@ class MRSim3DTP:
                          c_div_strength=self.c_div_strength if self.c_div_volume_correction else 0.0,
                          liquid_volume_target=self.liquid_volume_target, adapt=_adapt_of(self))
 
+    def write_render_cache(self, path, frame, time, camera=None):
+        """writeMRRenderCache3D(sim, path, frame, time, camera) (render_cache3d.h:660-711)"""
+        cam = camera or default_render_camera(self.nx, self.ny, self.nz, self.dx)
+        return self.ctx.write_render_cache(path, RenderFrame(frame, 0, time, self.effective_dt_last, self.Vp,
+                                                             self.phase, cam))
+
     def step(self):
         """MRSim3DTP::step, src/driver/multires_sim3d_tp.cpp:671-734."""
         st = self.ctx.mr_step(self.step_cfg())
         _store_step_stats(self, st)
+        self.effective_dt_last = st.effective_dt
         self.interface_diagnostics_last = st.iface
         self.surface_tension_stats_last = st.surface
         self.stats_last = st

@@ -284,6 +284,42 @@ double pf_calibrate_rho_tilde0(const pf_phase* phase, double Vp);
 double pf_phi_from_raw(double raw, const pf_phase* phase);
 double pf_beta_from_phi(double phi, const pf_phase* phase);
 
+/* ---- render-cache export (src/driver/render_cache3d.h) */
+/* RenderCacheCamera3D, render_cache3d.h:19-27 */
+typedef struct pf_render_camera {
+  double position[3], target[3], up[3];
+  double fov_degrees, near_clip, far_clip, focal_length_mm;
+} pf_render_camera;
+/* the arguments of writeSparseRenderCache3D / writeMRRenderCache3D besides the sim
+ * (render_cache3d.h:592-711): frame and time, the sim's effective_dt_last, Vp and
+ * phase, and the camera */
+typedef struct pf_render_frame {
+  int32_t frame, reserved;
+  double time, dt, Vp;
+  pf_phase phase;
+  pf_render_camera camera;
+} pf_render_frame;
+/* RenderCacheManifestFrame3D, render_cache3d.h:39-45 */
+typedef struct pf_render_manifest_frame {
+  int32_t frame, step;
+  double time;
+  const char* path;
+  int64_t bytes;
+} pf_render_manifest_frame;
+/* defaultRenderCacheCamera3D, render_cache3d.h:48-62 */
+void pf_render_default_camera(int32_t nx, int32_t ny, int32_t nz, double dx, pf_render_camera* out);
+/* writeSparseRenderCache3D (a context without a multires layout) or
+ * writeMRRenderCache3D (one with): the JSONL frame file of the context's grid and
+ * particles, sections in the reference's order and number format.  Sparse phase
+ * cells follow the marker blocks in the order markCells allocated them; that order
+ * is known while the particles are still in the binning the last marking used (after
+ * a step, a marking or a re-marking), else the blocks follow in id order.  *bytes
+ * (may be NULL) receives the file size. */
+int pf_write_render_cache(pf_ctx* ctx, const char* path, const pf_render_frame* frame, int64_t* bytes);
+/* writeRenderCacheManifest3D, render_cache3d.h:713-755 */
+int pf_write_render_manifest(const char* path, const char* sim_kind, int32_t nx, int32_t ny, int32_t nz, double dx,
+                             const pf_render_manifest_frame* frames, int64_t nframes);
+
 /* ---- measurement hooks (no reference counterpart: the reference times whole
  * steps with steady_clock, apps/bench_large_scale3d_tp.cpp:559-563) */
 enum pf_kernel {
