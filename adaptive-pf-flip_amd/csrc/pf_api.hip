@@ -29,6 +29,7 @@
 #include "pf_iface.hip"
 #include "pf_adapt.hip"
 #include "pf_render.hip"
+#include "pf_mg.hip"
 
 using namespace pf;
 
@@ -1129,10 +1130,219 @@ int mr_reference_solve(pf_ctx* c, const pf_mr_solve_cfg& cf, unsigned G, const V
 // single: the single-phase unit-density projection (ph unused, cdiv ignored);
 // c->mr: the multires finite-volume projection, stopping when
 // |rhs - A x|_inf < max(cfg->tol, mr_rel_tol * |rhs|_inf)
+// ---- aggregation multigrid preconditioner (pf_mg.hip)
+
+bool mg_cfg_valid(const pf_mg_cfg& q) {
+  return q.sweeps >= 1 && q.sweeps <= 16 && q.coarsest_sweeps >= 1 && q.coarsest_sweeps <= 1024 &&
+         q.omega > 0.0f && q.omega < 1.0f && q.correction > 0.0f && q.correction < 2.0f;
+}
+
+// the built-in cycle, with the PF_MG_* environment overrides (read once)
+const pf_mg_cfg& mg_default_cfg() {
+  static const pf_mg_cfg p = [] {
+    pf_mg_cfg q{2, 32, 0.9f, 1.8f};
+    pf_mg_cfg e = q;
+    if (const char* v = std::getenv("PF_MG_SWEEPS")) e.sweeps = std::atoi(v);
+    if (const char* v = std::getenv("PF_MG_COARSEST_SWEEPS")) e.coarsest_sweeps = std::atoi(v);
+    if (const char* v = std::getenv("PF_MG_OMEGA")) e.omega = (float)std::atof(v);
+    if (const char* v = std::getenv("PF_MG_CORRECTION")) e.correction = (float)std::atof(v);
+    return mg_cfg_valid(e) ? e : q;
+  }();
+  return p;
+}
+
+MgLevel mg_level0(const pf_ctx* c) {
+  MgLevel L;
+  L.d = c->dc;
+  L.nx = c->geom.nx; L.ny = c->geom.ny; L.nz = c->geom.nz;
+  L.cd = c->cd32; L.cx = c->cx32; L.cy = c->cy32; L.cz = c->cz32; L.inv = c->invd32;
+  return L;
+}
+
+MgLevel mg_level(const Ctx::MgBuf& b) {
+  MgLevel L;
+  L.d = b.d;
+  L.nx = b.nx; L.ny = b.ny; L.nz = b.nz;
+  L.cd = b.cd; L.cx = b.cx; L.cy = b.cy; L.cz = b.cz; L.inv = b.inv;
+  return L;
+}
+
+// the coarse operators of the assembled system (levels allocated on first use:
+// halving down to at most 4 cells a side)
+int mg_build(pf_ctx* c) {
+  if (c->mg.empty()) {
+    int nx = c->geom.nx, ny = c->geom.ny, nz = c->geom.nz;
+    while (std::max({nx, ny, nz}) > 4) {
+      nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+      Ctx::MgBuf b;
+      b.d = make_dims(nx, ny, nz);
+      b.nx = nx; b.ny = ny; b.nz = nz;
+      const long long n = b.d.nblocks * kBlockVol;
+      float** arrs[8] = {&b.cd, &b.cx, &b.cy, &b.cz, &b.inv, &b.x, &b.r, &b.t};
+      for (float** a : arrs) HIP_TRY(c, dalloc(a, n));
+      c->mg.push_back(b);
+    }
+    HIP_TRY(c, dalloc(&c->mg_t0, c->dc.nblocks * kBlockVol));
+  }
+  // level 1's residual is restricted from the FLUID blocks only: the aggregates of the
+  // other blocks stay zero through the solve
+  if (!c->mg.empty()) HIP_TRY(c, fill_f32(c->stream, c->mg[0].r, 0.f, c->mg[0].d.nblocks * kBlockVol));
+  MgLevel F = mg_level0(c);
+  for (auto& b : c->mg) {
+    MgLevel Cl = mg_level(b);
+    const unsigned g = std::min<unsigned>(grid_for(b.d.nblocks * kBlockVol, 256), 4096u);
+    hipLaunchKernelGGL(k_mg_coarsen, dim3(g), dim3(256), 0, c->stream, F, Cl, b.cd, b.cx, b.cy, b.cz);
+    hipLaunchKernelGGL(k_mg_inverse, dim3(g), dim3(256), 0, c->stream, Cl, b.inv);
+    LAUNCH_CHECK(c);
+    F = Cl;
+  }
+  return PF_OK;
+}
+
+// workgroups of the level-0 kernels: a multiple of 8 (the XCD-aware list walk), at
+// most 1024 (the partials of r.z)
+unsigned mg_grid0(const pf_ctx* c) { return std::min(list_grid(c->npblk), 1024u); }
+
+// z = V-cycle(r) on the assembled system; partial r.z (mg_grid0 workgroups) in
+// part[2 * kMaxPartials ...]; every kernel is skipped while *done
+int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part) {
+  struct {
+    int nu, coarsest;
+    float omega, correction;
+  } const P{c->mg_cfg.sweeps, c->mg_cfg.coarsest_sweeps, c->mg_cfg.omega, c->mg_cfg.correction};
+  const int nl = (int)c->mg.size();
+  MgWalk w0{c->pblk, c->npblk, done};
+  MgWalk wd{nullptr, 0, done};
+  const unsigned g0 = mg_grid0(c);
+  // kernels that write no partial may take more workgroups (up to 16 list blocks each)
+  const unsigned gw = list_grid(c->npblk);
+  const MgLevel L0 = mg_level0(c);
+  double* const rz = part + 2 * kMaxPartials;
+  auto gdense = [](const Ctx::MgBuf& b) {
+    return std::min<unsigned>(grid_for(b.d.nblocks * kBlockVol, 256), 2048u);
+  };
+  // level 0 by cell rows when the slabs are below 2 GiB and there is a coarse level
+  const bool rows = nl > 0 && c->dc.nblocks * kBlockVol * 4 < (1LL << 31);
+  MgRowArgs ra{};
+  ra.list = c->pblk;
+  ra.nlist = c->npblk;
+  ra.done = done;
+  ra.d = c->dc;
+  ra.cd = c->cd32; ra.cx = c->cx32; ra.cy = c->cy32; ra.cz = c->cz32; ra.inv = c->invd32;
+  ra.r = r;
+  ra.omega = P.omega;
+  ra.omega_c = P.correction;
+  if (nl > 0) ra.dc = c->mg[0].d;
+  // the level-0 kernels that write an iterate alternate between the work array and
+  // z; the first one is chosen so that the last lands in z
+  const int pre = rows ? (P.nu >= 2 ? P.nu - 1 : 1) : P.nu;
+  const int writes = pre + P.nu;
+  float* cur = nullptr;
+  float* nxt = (writes & 1) ? z : c->mg_t0;
+  auto flip = [&]() {
+    cur = nxt;
+    nxt = (cur == z) ? c->mg_t0 : z;
+  };
+  // pre-smoothing
+  if (rows && P.nu >= 2) {
+    ra.x = nullptr; ra.out = nxt; ra.part = nullptr;
+    hipLaunchKernelGGL(k_mg_row<MG_FIRST2>, dim3(gw), dim3(256), 0, c->stream, ra);
+    flip();
+    for (int s2 = 2; s2 < P.nu; ++s2) {
+      ra.x = cur; ra.out = nxt;
+      hipLaunchKernelGGL(k_mg_row<MG_SWEEP>, dim3(gw), dim3(256), 0, c->stream, ra);
+      flip();
+    }
+  } else {
+    hipLaunchKernelGGL(k_mg_first, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, nxt, P.omega);
+    flip();
+    const int more = nl > 0 ? P.nu - 1 : 2 * P.nu - 1;
+    for (int s2 = 0; s2 < more; ++s2) {
+      const bool last = nl == 0 && s2 == more - 1;
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, nxt, P.omega,
+                         last ? rz : nullptr);
+      flip();
+    }
+  }
+  if (nl == 0) {
+    LAUNCH_CHECK(c);
+    return PF_OK;
+  }
+  // restriction to level 1
+  if (rows) {
+    ra.x = cur; ra.out = c->mg[0].r; ra.part = nullptr;
+    hipLaunchKernelGGL(k_mg_row<MG_RESTRICT>, dim3(gw), dim3(256), 0, c->stream, ra);
+  } else {
+    hipLaunchKernelGGL(k_mg_restrict, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, mg_level(c->mg[0]),
+                       c->mg[0].r);
+  }
+  // coarse levels 1..nl-1 pre-smooth and restrict; the coarsest solves
+  std::vector<float*> res(nl + 1), alt(nl + 1);
+  for (int l = 1; l <= nl; ++l) {
+    Ctx::MgBuf& b = c->mg[(size_t)l - 1];
+    const MgLevel L = mg_level(b);
+    const unsigned g = gdense(b);
+    float* x = b.x;
+    float* t = b.t;
+    if (l == nl) {
+      hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(64), 0, c->stream, done, L, b.r, x, P.omega, P.coarsest);
+    } else {
+      hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, P.omega);
+      for (int s2 = 1; s2 < P.nu; ++s2) {
+        hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, P.omega, nullptr);
+        std::swap(x, t);
+      }
+      Ctx::MgBuf& nb = c->mg[(size_t)l];
+      hipLaunchKernelGGL(k_mg_restrict, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, mg_level(nb), nb.r);
+    }
+    res[l] = x;
+    alt[l] = t;
+  }
+  // coarse levels nl-1..1: prolong the correction, post-smooth
+  for (int l = nl - 1; l >= 1; --l) {
+    Ctx::MgBuf& b = c->mg[(size_t)l - 1];
+    const MgLevel F = mg_level(b);
+    const unsigned g = gdense(b);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, c->stream, wd, F, mg_level(c->mg[(size_t)l]),
+                       res[l + 1], res[l], P.correction);
+    for (int s2 = 0; s2 < P.nu; ++s2) {
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, F, b.r, res[l], alt[l], P.omega,
+                         nullptr);
+      std::swap(res[l], alt[l]);
+    }
+  }
+  // level 0: prolong and post-smooth, the last sweep into z with the partial r.z
+  if (rows) {
+    ra.x = cur; ra.out = nxt; ra.xc = res[1]; ra.part = P.nu == 1 ? rz : nullptr;
+    hipLaunchKernelGGL(k_mg_row<MG_PSWEEP>, dim3(ra.part ? g0 : gw), dim3(256), 0, c->stream, ra);
+    flip();
+    for (int s2 = 1; s2 < P.nu; ++s2) {
+      ra.x = cur; ra.out = nxt; ra.part = s2 == P.nu - 1 ? rz : nullptr;
+      hipLaunchKernelGGL(k_mg_row<MG_SWEEP>, dim3(ra.part ? g0 : gw), dim3(256), 0, c->stream, ra);
+      flip();
+    }
+  } else {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(g0), dim3(256), 0, c->stream, w0, L0, mg_level(c->mg[0]), res[1], cur,
+                       P.correction);
+    for (int s2 = 0; s2 < P.nu; ++s2) {
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, nxt, P.omega,
+                         s2 == P.nu - 1 ? rz : nullptr);
+      flip();
+    }
+  }
+  LAUNCH_CHECK(c);
+  if (cur != z) return fail(c, PF_ERR_STATE, "multigrid: cycle did not end in z");
+  return PF_OK;
+}
+
 int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cfg, double cdiv,
                 pf_solve_stats* out, bool timing, int ev_assemble, int ev_solve, int ev_vel,
                 bool single = false, double mr_rel_tol = 0.0) {
   const bool mr = c->mr;
+  if (cfg->precond < PF_PRECOND_JACOBI || cfg->precond > PF_PRECOND_MULTIGRID)
+    return fail(c, PF_ERR_ARG, "unknown preconditioner");
+  if (cfg->precond == PF_PRECOND_MULTIGRID && mr)
+    return fail(c, PF_ERR_ARG, "the multigrid preconditioner runs on the sparse path only");
   pf_solve_stats S;
   std::memset(&S, 0, sizeof(S));
   S.pinned_cell = -1;
@@ -1187,6 +1397,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const bool want_learned = cfg->precond == PF_PRECOND_LEARNED;
   if (want_learned && !c->lpc) return fail(c, PF_ERR_STATE, "precond = learned but no network attached (pf_lpc_attach)");
   if (want_learned && mr) return fail(c, PF_ERR_ARG, "the learned preconditioner runs on the sparse path only");
+  const bool want_mg = cfg->precond == PF_PRECOND_MULTIGRID;
   const bool lpc_system = c->lpc && (want_learned || c->snap_cap > 0);
   c->tiles_valid = false;
   c->nsnap = 0;
@@ -1299,7 +1510,9 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   S.pinned_cell = S.pinned ? (int64_t)c->u32_host[1] : -1;
   const int max_iters = std::max(cfg->max_iters, 0);
   const int max_updates = cfg->max_outer > 0 ? cfg->max_outer : 200;
-  const int check = cfg->check_every > 0 ? cfg->check_every : 8;
+  // (a multigrid iteration costs several products: check at least every 4)
+  const int check_base = cfg->check_every > 0 ? cfg->check_every : 8;
+  const int check = want_mg ? std::min(check_base, 4) : check_base;
   const double delta = (cfg->inner_rel_tol > 0 && cfg->inner_rel_tol < 1) ? cfg->inner_rel_tol : 0.1;
   // the multires rule: max(absolute, relative * initial max residual), with x0 = 0
   // (multires_pressure3d.cpp:858-863)
@@ -1331,8 +1544,26 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     c->snap_last_iter = iter;
     return PF_OK;
   };
-  bool learned = want_learned;
-  const unsigned GT = learned ? lpc_scatter_grid(c->ntiles) : 1u;
+  // flexible PCG around a preconditioner that is not a diagonal: the learned network
+  // or the multigrid V-cycle (z and partial r.z in slot 2, GT partials)
+  bool learned = want_learned || want_mg;
+  if (want_mg) {
+    st = mg_build(c);
+    if (st) return st;
+  }
+  const unsigned GT = want_mg ? mg_grid0(c) : want_learned ? lpc_scatter_grid(c->ntiles) : 1u;
+  auto papply = [&](const int* done) -> int {
+    if (want_mg) {
+      long long ki = kbegin(c, PF_K_MG_APPLY);
+      int s2 = mg_apply(c, c->r32, c->z32, done, c->partial);
+      kend(c, ki);
+      return s2;
+    }
+    long long ki = kbegin(c, PF_K_LPC_APPLY);
+    int s2 = lpc_apply(c, c->r32, c->z32, done, c->partial, 2);
+    kend(c, ki);
+    return s2;
+  };
   int* done_dev = &c->sc->done;
   if (refmode) {
     // the reference's FP64 iteration with its options; x in x64
@@ -1358,7 +1589,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
                          c->invd32, learned ? 1 : 0);
       LAUNCH_CHECK(c);
       if (learned) {
-        int s2 = lpc_apply(c, c->r32, c->z32, nullptr, c->partial, 2);
+        int s2 = papply(nullptr);
         if (s2) return s2;
         hipLaunchKernelGGL(k_pcg_start_flex, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, (int)GT, c->sc,
                            delta * delta, tol2_abs, budget, between);
@@ -1401,9 +1632,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
           TIMED(c, PF_K_PCG_UPDATE,
                 hipLaunchKernelGGL(k_pcg_update_flex, dim3(GS), dim3(256), 0, c->stream, vs, c->x32, c->r32, c->z32,
                                    c->pA, c->pB, c->Ap));
-          long long ki = kbegin(c, PF_K_LPC_APPLY);
-          st = lpc_apply(c, c->r32, c->z32, done_dev, c->partial, 2);
-          kend(c, ki);
+          st = papply(done_dev);
           if (st) return st;
           hipLaunchKernelGGL(k_pcg_beta_flex, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, (int)GT, c->sc);
         } else {
@@ -1425,7 +1654,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
         int& k = c->krec[r].first;
         if (k == PF_K_PCG_SPMV && kept_s++ >= ran) k = -1;
         if (k == PF_K_PCG_UPDATE && kept_u++ >= ran) k = -1;
-        if (k == PF_K_LPC_APPLY && kept_l++ >= ran) k = -1;
+        if ((k == PF_K_LPC_APPLY || k == PF_K_MG_APPLY) && kept_l++ >= ran) k = -1;
       }
       if (ran > 0) {
         st = snapshot(iter_base + c->done_host[0]);
@@ -1477,7 +1706,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
                          breakdown ? c->pA : nullptr, c->invd32, learned ? 1 : 0);
       LAUNCH_CHECK(c);
       if (learned) {
-        st = lpc_apply(c, c->r32, c->z32, nullptr, c->partial, 2);
+        st = papply(nullptr);
         if (st) return st;
       }
       if (breakdown) {
@@ -1628,6 +1857,23 @@ extern "C" {
 
 int pf_abi_version(void) { return PF_ABI_VERSION; }
 
+void pf_mg_defaults(pf_mg_cfg* cfg) {
+  if (cfg) *cfg = mg_default_cfg();
+}
+
+int pf_set_multigrid(pf_ctx* c, const pf_mg_cfg* cfg) {
+  if (!c || !cfg) return PF_ERR_ARG;
+  if (!mg_cfg_valid(*cfg)) return fail(c, PF_ERR_ARG, "invalid multigrid cycle");
+  c->mg_cfg = *cfg;
+  return PF_OK;
+}
+
+int pf_get_multigrid(const pf_ctx* c, pf_mg_cfg* cfg) {
+  if (!c || !cfg) return PF_ERR_ARG;
+  *cfg = c->mg_cfg;
+  return PF_OK;
+}
+
 const char* pf_last_error(const pf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 void* pf_stream(pf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
@@ -1707,6 +1953,7 @@ int pf_create(pf_ctx** out, const pf_grid_desc* d) {
   long long ncell_est = (long long)d->nx * d->ny * d->nz;
   if (ncell_est > (1LL << 31)) return PF_ERR_ARG;  // cell keys and ranks are 32-bit
   pf_ctx* c = new pf_ctx();
+  c->mg_cfg = mg_default_cfg();
   HIP_TRY(c, hipGetDevice(&c->device));
   HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->geom = Geom{d->nx, d->ny, d->nz, d->dx, d->ox, d->oy, d->oz};
@@ -1780,6 +2027,11 @@ int pf_destroy(pf_ctx* c) {
   dfree(c->iface); dfree(c->iface_bad);
   dfree(c->pa_alive); dfree(c->pa_key); dfree(c->pa_by_id); dfree(c->pa_new_id);
   dfree(c->pa_cflag); dfree(c->pa_d1); dfree(c->pa_d2); dfree(c->pa_wanted); dfree(c->pa_wscan); dfree(c->pa_cnt);
+  for (auto& m : c->mg) {
+    dfree(m.cd); dfree(m.cx); dfree(m.cy); dfree(m.cz); dfree(m.inv); dfree(m.x); dfree(m.r); dfree(m.t);
+  }
+  c->mg.clear();
+  dfree(c->mg_t0);
   if (c->mr_pin_host) hipHostFree(c->mr_pin_host);
   for (int a = 0; a < 3; ++a) { dfree(c->mr_valid[a]); dfree(c->mr_cf32[a]); dfree(c->mr_cf64[a]); }
   if (c->mr_bbox_host) hipHostFree(c->mr_bbox_host);

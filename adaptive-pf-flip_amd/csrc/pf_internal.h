@@ -226,6 +226,18 @@ struct Ctx {
   uint32_t *pa_wanted = nullptr, *pa_wscan = nullptr;              // per cell, linear order (+1)
   uint32_t* pa_cnt = nullptr;       // counters
 
+  // aggregation multigrid preconditioner: the coarse levels (level 0 is the assembled
+  // system) and a level-0 work array
+  struct MgBuf {
+    Dims d;
+    int nx = 0, ny = 0, nz = 0;
+    float *cd = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *inv = nullptr;
+    float *x = nullptr, *r = nullptr, *t = nullptr;
+  };
+  std::vector<MgBuf> mg;
+  float* mg_t0 = nullptr;
+  pf_mg_cfg mg_cfg{};   // the cycle (set at creation, pf_set_multigrid)
+
   // per-kernel event timing (measurement hook)
   uint32_t ktime = 0;                // kernels (bit PF_K_*) bracketed with events
   std::vector<hipEvent_t> kev;

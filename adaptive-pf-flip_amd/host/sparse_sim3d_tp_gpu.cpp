@@ -213,6 +213,7 @@ void SparseSim3DTP::step() {
   cfg.solve.max_outer = 10000;
   cfg.solve.warm_start = warm_start;
   cfg.solve.check_every = 8;
+  cfg.solve.precond = precond;
   cfg.surface_tension = surface_tension ? 1 : 0;
   cfg.surface_tension_strength = surface_tension_strength;
   cfg.surface_tension_max_delta_speed = surface_tension_max_delta_speed;

@@ -149,6 +149,8 @@ class SparseSim3DTP : public ParticleAdaptivityFields {
   // |r|_inf < cg_tol; PF_STOP_REL2 stops on ||b - Ax||_2/||b||_2 <= cg_tol in FP64
   int32_t stop_rule = PF_STOP_REF_MAXABS;
   int32_t warm_start = 0;
+  // PF_PRECOND_JACOBI (the reference's), PF_PRECOND_LEARNED or PF_PRECOND_MULTIGRID
+  int32_t precond = PF_PRECOND_JACOBI;
   bool stage_timing = false;
   pf_step_stats stats_last{};
 

@@ -24,7 +24,7 @@ STOP_REF_MAXABS, STOP_REL2 = 0, 1
 SCENE_RT, SCENE_BUBBLE, SCENE_DAMBREAK = 0, 1, 2
 STAGES = ("bin", "mark", "p2g", "grid", "assemble", "solve", "velocity", "g2p_advect")
 KERNELS = ("bin_count", "bin_scatter", "p2g", "face_ops", "assemble", "pcg_spmv", "pcg_update",
-           "residual64", "velocity", "g2p_advect", "lpc_apply", "interface", "surface")
+           "residual64", "velocity", "g2p_advect", "lpc_apply", "interface", "surface", "mg_apply")
 (K_BIN_COUNT, K_BIN_SCATTER, K_P2G, K_FACE_OPS, K_ASSEMBLE, K_PCG_SPMV, K_PCG_UPDATE,
  K_RESIDUAL64, K_VELOCITY, K_G2P_ADVECT, K_LPC_APPLY) = range(11)
 
@@ -136,6 +136,19 @@ class AdaptCfg(C.Structure):
 
 
 ADAPT_FIELDS = [n for n, _ in AdaptCfg._fields_]
+
+
+class MgCfg(C.Structure):
+    """pf_mg_cfg: the cycle of the multigrid preconditioner (PRECOND_MULTIGRID)"""
+    _fields_ = [("sweeps", C.c_int32), ("coarsest_sweeps", C.c_int32), ("omega", C.c_float),
+                ("correction", C.c_float)]
+
+
+def mg_defaults() -> MgCfg:
+    """the built-in cycle (or the PF_MG_* environment overrides)"""
+    c = MgCfg()
+    lib().pf_mg_defaults(C.byref(c))
+    return c
 
 
 def adapt_cfg(**kw) -> "AdaptCfg":
@@ -297,6 +310,9 @@ def lib():
             "pf_write_render_cache": (C.c_int, [vp, C.c_char_p, C.POINTER(RenderFrame), C.POINTER(i64)]),
             "pf_write_render_manifest": (C.c_int, [C.c_char_p, C.c_char_p, i32, i32, i32, dbl, vp, i64]),
             "pf_adapt_defaults": (None, [C.POINTER(AdaptCfg)]),
+            "pf_mg_defaults": (None, [C.POINTER(MgCfg)]),
+            "pf_set_multigrid": (C.c_int, [vp, C.POINTER(MgCfg)]),
+            "pf_get_multigrid": (C.c_int, [vp, C.POINTER(MgCfg)]),
             "pf_particles_download_ext": (C.c_int, [vp, vp, vp]),
             "pf_liquid_volume": (dbl, [vp]),
             "pf_mr_apply_surface_tension": (C.c_int, [vp, C.POINTER(Phase), dbl, dbl, dbl, i32,
@@ -359,7 +375,7 @@ def calibrate_rho_tilde0(phase: Phase, Vp: float = 1.0) -> float:
     return lib().pf_calibrate_rho_tilde0(C.byref(phase), Vp)
 
 
-PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID = 0, 1, 2
 
 
 def default_solve_cfg(stop_rule=STOP_REF_MAXABS, tol=1e-7, max_iters=600, precond=PRECOND_JACOBI) -> SolveCfg:
@@ -434,6 +450,21 @@ class Context:
         self._check(lib().pf_write_render_cache(self.h, str(path).encode(), C.byref(info), C.byref(nb)),
                     "pf_write_render_cache")
         return nb.value
+
+    def set_multigrid(self, cfg: MgCfg = None, **kw):
+        """the multigrid cycle of the following PRECOND_MULTIGRID solves (cfg, or the
+        current one with the fields in kw changed)"""
+        c = cfg if cfg is not None else self.multigrid()
+        for k, v in kw.items():
+            if k not in [n for n, _ in MgCfg._fields_]:
+                raise KeyError(k)
+            setattr(c, k, v)
+        self._check(lib().pf_set_multigrid(self.h, C.byref(c)), "pf_set_multigrid")
+
+    def multigrid(self) -> MgCfg:
+        c = MgCfg()
+        self._check(lib().pf_get_multigrid(self.h, C.byref(c)), "pf_get_multigrid")
+        return c
 
     def particle_adaptivity(self, cfg: AdaptCfg) -> AdaptStats:
         """applyParticleAdaptivity on the context's particles"""
@@ -617,6 +648,8 @@ class SparseSim3DTP:
         self.cg_iters, self.cg_tol = 600, 1e-7
         self.stop_rule = STOP_REF_MAXABS
         self.inner_rel_tol, self.warm_start, self.check_every = 0.1, 0, 8
+        # PRECOND_JACOBI (the reference's), PRECOND_LEARNED or PRECOND_MULTIGRID
+        self.precond = PRECOND_JACOBI
         self.timing = False
         # CSF surface tension (sparse_sim3d_tp.h:156-159)
         self.surface_tension = False
@@ -645,7 +678,7 @@ class SparseSim3DTP:
     def step_cfg(self) -> StepCfg:
         solve = SolveCfg(max_iters=self.cg_iters, stop_rule=self.stop_rule, tol=self.cg_tol,
                          inner_rel_tol=self.inner_rel_tol, max_outer=10000,
-                         warm_start=self.warm_start, check_every=self.check_every)
+                         warm_start=self.warm_start, check_every=self.check_every, precond=self.precond)
         return StepCfg(dt=self.dt, gravity=self.gravity, Vp=self.Vp,
                        alpha_liquid=self.alpha_liquid, alpha_gas=self.alpha_gas,
                        adaptive_cfl=self.adaptive_cfl, adaptive_min_dt=self.adaptive_min_dt,

@@ -6,8 +6,10 @@ Workload (the headline metric "step ms @256^3 two-phase"): Rayleigh-Taylor on a
 (131,096,512 FP32 particles), heavy rho = 1000 above
 y = 0.5 + 0.05 cos(2 pi x) cos(2 pi z), light rho = 1 below, zero initial
 velocity, g = -9.81, dt = 1e-3, FLIP 0.95, pressure solved until
-||b - A x||_2 / ||b||_2 <= 1e-6 measured in FP64.  Synthetic data from the seeded
-scene generator; inputs are resident in HBM before the timed region.
+||b - A x||_2 / ||b||_2 <= 1e-6 measured in FP64 (--precond: aggregation-multigrid
+PCG by default, or the reference's Jacobi PCG; both stop on that same FP64
+residual).  Synthetic data from the seeded scene generator; inputs are resident in
+HBM before the timed region.
 
 One "step" is SparseSim3DTP::step (bin, mark, P2G, save/gravity/walls, assembly,
 solve, velocity update, G2P + RK2 advection).  W untimed warm-up steps, then K
@@ -50,26 +52,39 @@ def _ensure_built():
     return G
 
 
-def step_cfg(G, n, tol, max_iters, warm):
+# per preconditioner: (pf_precond, FP32 reduction per FP64 refinement pass, iterations
+# between host checks) -- a multigrid iteration costs about four Jacobi ones, so its
+# passes go deeper and the host looks more often
+SOLVERS = {"jacobi": (0, 0.1, 16), "multigrid": (2, 0.01, 4)}
+SOLVER_NAMES = {"jacobi": "plain Jacobi PCG", "multigrid": "aggregation-multigrid PCG"}
+
+
+def solve_cfg(G, precond, tol, max_iters, warm=0):
+    pre, inner, check = SOLVERS[precond]
+    return G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=inner,
+                      max_outer=10000, warm_start=warm, check_every=check, precond=pre)
+
+
+def step_cfg(G, n, tol, max_iters, warm, precond="jacobi"):
     ph = G.Phase(1000.0, 1.0, 1.0, 1.0)
     ph.rho_tilde_0 = G.calibrate_rho_tilde0(ph, 1.0)
-    solve = G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=0.1,
-                       max_outer=10000, warm_start=warm, check_every=16)
+    solve = solve_cfg(G, precond, tol, max_iters, warm)
     return G.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95,
                      adaptive_cfl=0.5, adaptive_min_dt=1e-5, c_div=0.0, adaptive_timestep=0,
                      advection_order=2, timing=1, phase=ph, solve=solve)
 
 
-def pressure_comparison(G, ctx, cfg, tol):
-    """Pressure solve to ||b - Ax||_2/||b||_2 <= tol (FP64) on the current system,
-    plain Jacobi PCG vs the learned preconditioner inside flexible PCG, same inputs.
-    The pre-projection state (mark, P2G, save, gravity, walls) is rebuilt for each."""
+def pressure_comparison(G, ctx, cfg, tol, learned=True):
+    """Pressure solve to ||b - Ax||_2/||b||_2 <= tol (FP64) on the current system:
+    plain Jacobi PCG, multigrid PCG and (with trained weights in the package) the
+    learned preconditioner inside flexible PCG, same inputs.  The pre-projection state
+    (mark, P2G, save, gravity, walls) is rebuilt for each solve."""
     from pfflip_amd import lpc_train
-    if not os.path.exists(lpc_train.DEFAULT_WEIGHTS):
-        return {"learned_pc": None, "note": "no trained weights in the package"}
-    net = G.LearnedPreconditioner(0)
-    net.set_weights(lpc_train.load_weights())
-    net.attach(ctx)
+    net = None
+    if learned and os.path.exists(lpc_train.DEFAULT_WEIGHTS):
+        net = G.LearnedPreconditioner(0)
+        net.set_weights(lpc_train.load_weights())
+        net.attach(ctx)
     dt, ph = cfg.dt, cfg.phase
 
     def prepare():
@@ -79,25 +94,35 @@ def pressure_comparison(G, ctx, cfg, tol):
         ctx.apply_gravity(dt, cfg.gravity)
         ctx.apply_walls()
 
-    out = {}
-    for name, pre in (("learned_pc", G.PRECOND_LEARNED), ("plain_pcg", G.PRECOND_JACOBI),
-                      ("learned_pc", G.PRECOND_LEARNED)):
+    runs = [("plain_pcg", "jacobi"), ("multigrid_pcg", "multigrid"), ("multigrid_pcg", "multigrid")]
+    if net is not None:
+        # the first learned solve also allocates the network's buffers; keep the second
+        runs += [("learned_pc", G.PRECOND_LEARNED), ("learned_pc", G.PRECOND_LEARNED)]
+    out = {"learned_pc": None}
+    for name, pre in runs:
         prepare()
-        solve = G.default_solve_cfg(G.STOP_REL2, tol, 50000, pre)
-        solve.check_every = 16
+        if pre == G.PRECOND_LEARNED:
+            solve = G.default_solve_cfg(G.STOP_REL2, tol, 50000, pre)
+            solve.check_every = 16
+        else:
+            solve = solve_cfg(G, pre, tol, 50000)
         ctx.synchronize()
         t0 = time.perf_counter()
         st = ctx.project(ph, dt, solve)
         ctx.synchronize()
         ms = (time.perf_counter() - t0) * 1e3
-        # the first learned solve also allocates the network's buffers; keep the second
-        out[name] = {"ms": round(ms, 2), "iterations": st.iterations, "true_rel_residual": st.true_rel_residual,
-                     "converged": st.converged, "fallback_to_jacobi": st.fallback_used}
+        out[name] = {"ms": round(ms, 2), "iterations": st.iterations, "outer": st.outer_iterations,
+                     "true_rel_residual": st.true_rel_residual, "converged": st.converged,
+                     "fallback_to_jacobi": st.fallback_used}
         print(f"[bench] pressure solve {name}: {ms:.1f} ms, {st.iterations} it, fallback {st.fallback_used}",
               file=sys.stderr, flush=True)
-    out["speedup_learned_vs_plain"] = round(out["plain_pcg"]["ms"] / out["learned_pc"]["ms"], 3)
-    G.lib().pf_lpc_attach(ctx.h, None)
-    net.close()
+    out["speedup_multigrid_vs_plain"] = round(out["plain_pcg"]["ms"] / out["multigrid_pcg"]["ms"], 3)
+    if net is not None:
+        out["speedup_learned_vs_plain"] = round(out["plain_pcg"]["ms"] / out["learned_pc"]["ms"], 3)
+        G.lib().pf_lpc_attach(ctx.h, None)
+        net.close()
+    else:
+        out["note"] = "no trained weights in the package" if learned else "learned preconditioner skipped"
     return out
 
 
@@ -157,7 +182,10 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner in the comparison")
+    ap.add_argument("--no-pressure", action="store_true", help="skip the pressure-solver comparison")
+    ap.add_argument("--precond", choices=sorted(SOLVERS), default="multigrid",
+                    help="preconditioner of the timed steps' pressure solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -168,7 +196,7 @@ def main():
     n = args.n
     ctx = G.Context(n, n, n, 1.0 / n)
     ctx.scene(G.SCENE_RT, args.seed)
-    cfg = step_cfg(G, n, args.tol, args.max_iters, args.warm_start)
+    cfg = step_cfg(G, n, args.tol, args.max_iters, args.warm_start, args.precond)
     print(f"[bench] {n}^3 RT, {ctx.count()} particles; warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         st = ctx.step(cfg)
@@ -222,9 +250,9 @@ def main():
     ncopy = 256 << 20  # 1 GiB read + 1 GiB written: past every cache
     copy_ms = min(ctx.copy_probe(ncopy) for _ in range(3))
     copy_gbs = round(2 * 4 * ncopy / (copy_ms * 1e-3) / 1e9, 1)
-    pressure = {"plain_pcg": {"ms": stage["solve"]}, "learned_pc": None}
-    if not args.no_learned:
-        pressure = pressure_comparison(G, ctx, cfg, args.tol)
+    pressure = None
+    if not args.no_pressure:
+        pressure = pressure_comparison(G, ctx, cfg, args.tol, not args.no_learned)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (reference step on the 64^3 sample) ...", file=sys.stderr, flush=True)
@@ -245,10 +273,10 @@ def main():
         "dtype": "f32 (FP64 dots, scalars and refinement)",
         "data": "synthetic (seeded RT scene)",
         "config": {"workload": f"sparse two-phase Rayleigh-Taylor {n}^3, rho 1000:1, dt 1e-3, "
-                               f"{ctx.count()} particles, plain Jacobi PCG + FP64 refinement to "
+                               f"{ctx.count()} particles, {SOLVER_NAMES[args.precond]} + FP64 refinement to "
                                f"{args.tol:g} rel. residual",
                    "grid": n, "particles": ctx.count(), "fluid_cells": fluid,
-                   "solver": "jacobi-pcg-fp32+ir-fp64", "warm_start": bool(args.warm_start)},
+                   "solver": f"{args.precond}-pcg-fp32+ir-fp64", "warm_start": bool(args.warm_start)},
         "stage_ms": stage,
         "pressure_ms_to_1e-6": pressure,
         "solve_iterations": iters,

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 4
+#define PF_ABI_VERSION 5
 
 enum pf_status {
   PF_OK = 0,
@@ -74,9 +74,23 @@ typedef struct pf_solve_cfg {
 
 enum pf_precond {
   PF_PRECOND_JACOBI = 0,    /* the reference's diagonal preconditioner (sparse_ops3d_tp.cpp:322-327) */
-  PF_PRECOND_LEARNED = 1    /* flexible PCG with the attached learned network (pf_lpc_attach);
+  PF_PRECOND_LEARNED = 1,   /* flexible PCG with the attached learned network (pf_lpc_attach);
                                falls back to Jacobi when it stagnates (stats.fallback_used) */
+  PF_PRECOND_MULTIGRID = 2  /* a V-cycle of Galerkin aggregation multigrid (2x2x2 aggregates,
+                               damped-Jacobi smoothing; pf_set_multigrid), sparse path; no
+                               reference counterpart.  The host checks for convergence at
+                               least every 4 iterations (check_every is capped there) */
 };
+
+/* Cycle of the multigrid preconditioner (no reference counterpart).  Levels halve
+ * the grid down to at most 4 cells a side; each level smooths `sweeps` times before
+ * and after its coarse correction, the coarsest `coarsest_sweeps` times. */
+typedef struct pf_mg_cfg {
+  int32_t sweeps;           /* damped-Jacobi sweeps before and after the coarse correction, 1..16 */
+  int32_t coarsest_sweeps;  /* damped-Jacobi sweeps on the coarsest level, 1..1024 */
+  float omega;              /* Jacobi damping, in (0, 1) */
+  float correction;         /* weight of the coarse-grid correction, in (0, 2) */
+} pf_mg_cfg;
 
 /* Mirrors MRPressureSolveStats3D's core fields (src/pressure/multires_pressure3d.h:60-133) */
 typedef struct pf_solve_stats {
@@ -328,6 +342,7 @@ enum pf_kernel {
   PF_K_LPC_APPLY,      /* one application of the learned preconditioner (5 kernels) */
   PF_K_INTERFACE,      /* interface fields and diagnostics (5 kernels) */
   PF_K_SURFACE,        /* CSF increments and face updates (5 kernels) */
+  PF_K_MG_APPLY,       /* one multigrid V-cycle (PF_PRECOND_MULTIGRID) */
   PF_NKERNELS
 };
 /* The library's HIP stream (hipStream_t). */
@@ -508,6 +523,15 @@ int32_t pf_lpc_snapshot_count(const pf_ctx* ctx);
 int pf_lpc_train(pf_lpc* lpc, pf_ctx* ctx, int32_t steps, double lr, uint64_t seed, double* mean_loss);
 /* the loss of the network and of Jacobi (M = D^-1) on one recorded snapshot */
 int pf_lpc_loss(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, double* loss_learned, double* loss_jacobi);
+
+/* ---- multigrid preconditioner (no reference counterpart; PF_PRECOND_MULTIGRID) */
+/* the built-in cycle (2 sweeps, 32 coarsest sweeps, omega 0.9, correction 1.8), or
+ * the PF_MG_SWEEPS / PF_MG_COARSEST_SWEEPS / PF_MG_OMEGA / PF_MG_CORRECTION
+ * environment variables where set; every context starts with it */
+void pf_mg_defaults(pf_mg_cfg* cfg);
+/* the cycle of the following multigrid solves on ctx (PF_ERR_ARG out of range) */
+int pf_set_multigrid(pf_ctx* ctx, const pf_mg_cfg* cfg);
+int pf_get_multigrid(const pf_ctx* ctx, pf_mg_cfg* cfg);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,7 @@ def test_struct_layouts_match_the_header(tmp_path):
                    "printf(\"%zu %zu %zu %zu %zu %zu\\n\",sizeof(pf_adapt_cfg),sizeof(pf_adapt_stats),"
                    "offsetof(pf_step_cfg,liquid_volume_target),offsetof(pf_step_cfg,adapt),"
                    "offsetof(pf_step_stats,c_div),offsetof(pf_step_stats,adapt));"
+                   "printf(\"%zu %zu\\n\",sizeof(pf_mg_cfg),offsetof(pf_mg_cfg,correction));"
                    "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -52,7 +53,8 @@ def test_struct_layouts_match_the_header(tmp_path):
             C.sizeof(G.IfaceStats), C.sizeof(G.SurfaceStats), G.StepCfg.surface_tension_strength.offset,
             G.StepCfg.skip_interface_diagnostics.offset, G.StepStats.iface.offset, G.StepStats.surface.offset,
             C.sizeof(G.AdaptCfg), C.sizeof(G.AdaptStats), G.StepCfg.liquid_volume_target.offset,
-            G.StepCfg.adapt.offset, G.StepStats.c_div.offset, G.StepStats.adapt.offset]
+            G.StepCfg.adapt.offset, G.StepStats.c_div.offset, G.StepStats.adapt.offset,
+            C.sizeof(G.MgCfg), G.MgCfg.correction.offset]
     assert got == want
 
 
@@ -66,7 +68,10 @@ def test_host_helpers_without_device():
     for raw in (0.0, 5.0, 7.0, 100.0, 900.0, 2000.0):
         assert L.pf_phi_from_raw(raw, C.byref(ph)) == O.phi(raw, oph)
     assert L.pf_beta_from_phi(0.25, C.byref(ph)) == O.beta(0.25, oph)
-    assert L.pf_abi_version() == 4
+    assert L.pf_abi_version() == 5
+    # the multigrid cycle's defaults are in range
+    m = G.mg_defaults()
+    assert 1 <= m.sweeps <= 16 and 1 <= m.coarsest_sweeps <= 1024 and 0 < m.omega < 1 and 0 < m.correction < 2
 
 
 def test_null_arguments_are_rejected_without_device():
