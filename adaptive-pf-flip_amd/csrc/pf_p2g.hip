@@ -266,6 +266,7 @@ __global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
   __shared__ float s_mass[3][256];
   __shared__ uint8_t s_flag[3][256];
   __shared__ uint8_t s_ord[64 * kOrdMax];
+  __shared__ uint32_t s_ids[kOrdMax * 64];
   if (blockIdx.x >= *a.count) return;
   const uint32_t b = a.list[blockIdx.x];
   const int lane = threadIdx.x;
@@ -283,10 +284,15 @@ __global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
   const uint32_t cnt = a.start[c + 1] - s0;
   uint8_t* ord = &s_ord[lane * kOrdMax];
   if (cnt > 1 && cnt <= (uint32_t)kOrdMax) {
-    // rank of each particle by caller index (ids are unique)
+    // rank of each particle by caller index (ids are unique), from the cell's ids
+    // loaded at once into the lane's LDS column
+#pragma unroll
+    for (int t = 0; t < kOrdMax; ++t)
+      if ((uint32_t)t < cnt) s_ids[t * 64 + lane] = a.P.id[s0 + t];
     for (uint32_t t = 0; t < cnt; ++t) {
-      uint32_t id = a.P.id[s0 + t], r = 0;
-      for (uint32_t u = 0; u < cnt; ++u) r += a.P.id[s0 + u] < id;
+      const uint32_t id = s_ids[t * 64 + lane];
+      uint32_t r = 0;
+      for (uint32_t u = 0; u < cnt; ++u) r += s_ids[u * 64 + lane] < id;
       ord[r] = (uint8_t)t;
     }
   } else if (cnt == 1) {
