@@ -49,6 +49,7 @@ struct P2GArgs {
 
 constexpr int kOrdMax = 16;
 constexpr float kGuard = 1e-5f;
+constexpr float kGuardT = 1e-5f;  // on t = 1 - d^2/2.25: wider than kGuard / 2.25
 
 template <int AX>
 struct Ax {
@@ -185,31 +186,71 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
     for (int o = 0; o < A::NZ; ++o) { float d = fz - (float)(o - 1); dz2[o] = d * d; }
     float wt[A::NN];
     float wsum = 0.f;
-#pragma unroll
-    for (int oz = 0; oz < A::NZ; ++oz)
+    // two-phase: the separable form t = (1 - (dx2 + dy2)/2.25) - dz2/2.25, w = t^3 for
+    // t > 0, with rows and columns outside the domain made negative -- one
+    // subtraction per node and no branch.  A node with |t| < kGuardT (its d^2 within
+    // about 1e-5 of 2.25) sends the particle through the per-node loop below, which
+    // decides those nodes with the FP64 predicate; every other node is far enough from
+    // the edge that t > 0 and the FP64 d^2 < 2.25 agree.
+    bool slow = KT != 0;
+    if (KT == 0) {
+      float Axy[A::NX * A::NY], Bz[A::NZ];
 #pragma unroll
       for (int oy = 0; oy < A::NY; ++oy)
 #pragma unroll
-        for (int ox = 0; ox < A::NX; ++ox) {
-          const int n = ox + A::NX * (oy + A::NY * oz);
-          float d2 = dx2[ox] + dy2[oy] + dz2[oz];
-          bool inb = vx[ox] && vy[oy] && vz[oz];
-          float w;
-          bool act;
-          if (fabsf(d2 - 2.25f) < kGuard) {
-            w = exact_weight(gx - (double)(i + ox - 1), gy - (double)(j + oy - 1), gz - (double)(k + oz - 1),
-                             act);
-          } else {
-            act = d2 < 2.25f;
-            float t = 1.f - d2 * (1.f / 2.25f);
-            w = act ? t * t * t : 0.f;
+        for (int ox = 0; ox < A::NX; ++ox)
+          Axy[ox + A::NX * oy] = (vx[ox] && vy[oy]) ? 1.f - (dx2[ox] + dy2[oy]) * (1.f / 2.25f) : -4.f;
+#pragma unroll
+      for (int oz = 0; oz < A::NZ; ++oz) Bz[oz] = vz[oz] ? dz2[oz] * (1.f / 2.25f) : 4.f;
+      float tmin = 4.f;
+      uint32_t lo = 0, hi = 0;
+#pragma unroll
+      for (int oz = 0; oz < A::NZ; ++oz)
+#pragma unroll
+        for (int oy = 0; oy < A::NY; ++oy)
+#pragma unroll
+          for (int ox = 0; ox < A::NX; ++ox) {
+            const int n = ox + A::NX * (oy + A::NY * oz);
+            const float t = Axy[ox + A::NX * oy] - Bz[oz];
+            tmin = fminf(tmin, fabsf(t));
+            const float tp = fmaxf(t, 0.f);
+            const float w = tp * tp * tp;
+            wt[n] = w;
+            wsum += w;
+            const uint32_t bit = t > 0.f ? 1u : 0u;
+            if (n < 32) lo |= bit << n; else hi |= bit << (n - 32);
           }
-          act = act && inb && ((vbits >> n) & 1ull);
-          w = act ? w : 0.f;
-          wt[n] = w;
-          wsum += w;
-          if (act) fbits |= (1ull << n);
-        }
+      slow = tmin < kGuardT;
+      if (!slow) fbits |= ((uint64_t)hi << 32) | lo;
+    }
+    if (slow) {
+      wsum = 0.f;
+  #pragma unroll
+      for (int oz = 0; oz < A::NZ; ++oz)
+  #pragma unroll
+        for (int oy = 0; oy < A::NY; ++oy)
+  #pragma unroll
+          for (int ox = 0; ox < A::NX; ++ox) {
+            const int n = ox + A::NX * (oy + A::NY * oz);
+            float d2 = dx2[ox] + dy2[oy] + dz2[oz];
+            bool inb = vx[ox] && vy[oy] && vz[oz];
+            float w;
+            bool act;
+            if (fabsf(d2 - 2.25f) < kGuard) {
+              w = exact_weight(gx - (double)(i + ox - 1), gy - (double)(j + oy - 1), gz - (double)(k + oz - 1),
+                               act);
+            } else {
+              act = d2 < 2.25f;
+              float t = 1.f - d2 * (1.f / 2.25f);
+              w = act ? t * t * t : 0.f;
+            }
+            act = act && inb && ((vbits >> n) & 1ull);
+            w = act ? w : 0.f;
+            wt[n] = w;
+            wsum += w;
+            if (act) fbits |= (1ull << n);
+          }
+    }
     uint8_t ty = a.P.type[q];
     float mp = (ty == 0) ? a.mp_l : a.mp_g;
     if (a.has_vol) mp *= a.P.vol[q];
