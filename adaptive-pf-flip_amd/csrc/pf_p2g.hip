@@ -284,16 +284,33 @@ template <int AX>
 __device__ inline void flush_axis(const P2GArgs& a, int bx, int by, int bz, const float* s_mom,
                                   const float* s_mass, const uint8_t* s_flag, int lane) {
   using A = Ax<AX>;
+  constexpr int kPer = (A::RN + 63) / 64;  // region nodes per lane
   const Dims& fd = a.df[AX];
-  for (int r = lane; r < A::RN; r += 64) {
-    if (!s_flag[r]) continue;
-    int rx = r % A::RX, ry = (r / A::RX) % A::RY, rz = r / (A::RX * A::RY);
-    int ni = bx * 4 - 1 + rx, nj = by * 4 - 1 + ry, nk = bz * 4 - 1 + rz;
+  float* __restrict__ mom = a.mom[AX];
+  float* __restrict__ mass = a.mass[AX];
+  // all of the lane's nodes are read before any is written (the nodes are distinct),
+  // so the loads of one lane overlap instead of waiting on each other's stores
+  long long gi[kPer];
+  float om[kPer], oq[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int r = lane + 64 * e;
+    gi[e] = -1;
+    if (r >= A::RN || !s_flag[r]) continue;
+    const int rx = r % A::RX, ry = (r / A::RX) % A::RY, rz = r / (A::RX * A::RY);
+    const int ni = bx * 4 - 1 + rx, nj = by * 4 - 1 + ry, nk = bz * 4 - 1 + rz;
     if (ni < 0 || nj < 0 || nk < 0 || ni >= fd.nx || nj >= fd.ny || nk >= fd.nz) continue;
-    long long gi = bm_index(fd, ni, nj, nk);
-    a.mom[AX][gi] += s_mom[r];
-    a.mass[AX][gi] += s_mass[r];
-    long long blk = gi >> 6;
+    gi[e] = bm_index(fd, ni, nj, nk);
+    oq[e] = mom[gi[e]];
+    om[e] = mass[gi[e]];
+  }
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    if (gi[e] < 0) continue;
+    const int r = lane + 64 * e;
+    mom[gi[e]] = oq[e] + s_mom[r];
+    mass[gi[e]] = om[e] + s_mass[r];
+    const long long blk = gi[e] >> 6;
     a.act_vel[AX][blk] = 1;
     a.act_mass[AX][blk] = 1;
   }
