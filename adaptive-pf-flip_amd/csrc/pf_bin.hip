@@ -66,17 +66,33 @@ __global__ void k_bin_scatter(long long n, PBuf S, PBuf D, bool has_vol, const u
 }
 
 // Per-colour lists of cell blocks that hold particles (the P2G source blocks).
-// Append order is free: same-colour blocks write disjoint regions.
+// Append order is free: same-colour blocks write disjoint regions.  One atomic per
+// colour present in the wave (lanes take consecutive slots by their rank among the
+// wave's blocks of that colour): a wave of consecutive block ids holds few colours,
+// where one atomic per block would queue on 8 counters.
 __global__ void k_color_lists(Dims dc, const uint32_t* __restrict__ start, uint32_t* __restrict__ cnt,
                               uint32_t* __restrict__ lists, long long cap) {
-  long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= dc.nblocks) return;
-  if (start[(b + 1) * kBlockVol] == start[b * kBlockVol]) return;
-  int bx, by, bz;
-  block_coords(dc, b, bx, by, bz);
-  int col = color8(bx, by, bz);
-  uint32_t pos = atomicAdd(&cnt[col], 1u);
-  lists[col * cap + pos] = (uint32_t)b;
+  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool has = b < dc.nblocks && start[(b + 1) * kBlockVol] != start[b * kBlockVol];
+  int col = -1;
+  if (has) {
+    int bx, by, bz;
+    block_coords(dc, b, bx, by, bz);
+    col = color8(bx, by, bz);
+  }
+  for (int k = 0; k < 8; ++k) {
+    const unsigned long long m = __ballot(col == k);
+    if (m == 0) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[k], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (col == k) {
+      const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      lists[k * cap + base + rank] = (uint32_t)b;
+    }
+  }
 }
 
 // Return particles to the caller's order (out[id[p]] = in[p]).
