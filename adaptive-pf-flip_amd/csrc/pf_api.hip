@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -1276,41 +1277,40 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
     hipLaunchKernelGGL(k_mg_restrict, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, mg_level(c->mg[0]),
                        c->mg[0].r);
   }
-  // coarse levels 1..nl-1 pre-smooth and restrict; the coarsest solves
+  // coarse levels: level l (1..nl) solves A_l x = r_l from zero, result in res[l].
+  // (Two corrections per level -- a W-cycle -- cost more launches than they save
+  // iterations at 256^3, and twice the 1.8 over-correction stops converging.)
   std::vector<float*> res(nl + 1), alt(nl + 1);
-  for (int l = 1; l <= nl; ++l) {
+  std::function<void(int)> coarse = [&](int l) {
     Ctx::MgBuf& b = c->mg[(size_t)l - 1];
     const MgLevel L = mg_level(b);
-    const unsigned g = gdense(b);
     float* x = b.x;
     float* t = b.t;
     if (l == nl) {
       hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(64), 0, c->stream, done, L, b.r, x, P.omega, P.coarsest);
-    } else {
-      hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, P.omega);
-      for (int s2 = 1; s2 < P.nu; ++s2) {
-        hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, P.omega, nullptr);
-        std::swap(x, t);
-      }
-      Ctx::MgBuf& nb = c->mg[(size_t)l];
-      hipLaunchKernelGGL(k_mg_restrict, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, mg_level(nb), nb.r);
+      res[l] = x;
+      alt[l] = t;
+      return;
+    }
+    const unsigned g = gdense(b);
+    hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, P.omega);
+    for (int s2 = 1; s2 < P.nu; ++s2) {
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, P.omega, nullptr);
+      std::swap(x, t);
+    }
+    Ctx::MgBuf& nb = c->mg[(size_t)l];
+    hipLaunchKernelGGL(k_mg_restrict, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, mg_level(nb), nb.r);
+    coarse(l + 1);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, c->stream, wd, L, mg_level(nb), res[l + 1], x,
+                       P.correction);
+    for (int s2 = 0; s2 < P.nu; ++s2) {
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, P.omega, nullptr);
+      std::swap(x, t);
     }
     res[l] = x;
     alt[l] = t;
-  }
-  // coarse levels nl-1..1: prolong the correction, post-smooth
-  for (int l = nl - 1; l >= 1; --l) {
-    Ctx::MgBuf& b = c->mg[(size_t)l - 1];
-    const MgLevel F = mg_level(b);
-    const unsigned g = gdense(b);
-    hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, c->stream, wd, F, mg_level(c->mg[(size_t)l]),
-                       res[l + 1], res[l], P.correction);
-    for (int s2 = 0; s2 < P.nu; ++s2) {
-      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, F, b.r, res[l], alt[l], P.omega,
-                         nullptr);
-      std::swap(res[l], alt[l]);
-    }
-  }
+  };
+  coarse(1);
   // level 0: prolong and post-smooth, the last sweep into z with the partial r.z
   if (rows) {
     ra.x = cur; ra.out = nxt; ra.xc = res[1]; ra.part = P.nu == 1 ? rz : nullptr;
