@@ -17,23 +17,29 @@ __device__ inline int cell_coord(float x, double o, double dx) {
 
 // Rank of this lane among the lanes of its wave with the same key, plus one
 // atomicAdd per distinct key of the wave (a wave of cell-sorted particles covers
-// about eight cells, so this is ~8x fewer same-address atomics).
+// about eight cells, so this is ~8x fewer same-address atomics).  The groups are
+// found first (ballots only); then every group leader issues its atomic in the same
+// instruction, so the wave waits for one atomic round trip instead of one per key.
 __device__ inline uint32_t wave_aggregated_increment(uint32_t* cnt, uint32_t key) {
   const int lane = threadIdx.x & 63;
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   uint64_t todo = __ballot(1);
-  uint32_t my = 0;
+  int my_leader = lane;
+  uint32_t my_off = 0, my_count = 0;
   while (todo) {
     const int leader = __ffsll((unsigned long long)todo) - 1;
     const uint32_t k = __shfl(key, leader, 64);
     const uint64_t grp = __ballot(key == k) & todo;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&cnt[k], (uint32_t)__popcll(grp));
-    base = __shfl(base, leader, 64);
-    if (key == k && ((grp >> lane) & 1ull)) my = base + (uint32_t)__popcll(grp & below);
+    if ((grp >> lane) & 1ull) {
+      my_leader = leader;
+      my_off = (uint32_t)__popcll(grp & below);
+    }
+    if (lane == leader) my_count = (uint32_t)__popcll(grp);
     todo &= ~grp;
   }
-  return my;
+  uint32_t base = 0;
+  if (my_count) base = atomicAdd(&cnt[key], my_count);
+  return __shfl(base, my_leader, 64) + my_off;
 }
 
 __global__ void k_bin_count(long long n, PBuf P, Geom g, Dims dc, uint32_t* __restrict__ key,
