@@ -1169,9 +1169,9 @@ MgLevel mg_level(const Ctx::MgBuf& b) {
 }
 
 // the coarse operators of the assembled system (levels allocated on first use:
-// halving down to at most 4 cells a side)
+// halving down to at most 4 cells a side; a grid of at most 4 cells a side has none)
 int mg_build(pf_ctx* c) {
-  if (c->mg.empty()) {
+  if (!c->mg_t0) {
     int nx = c->geom.nx, ny = c->geom.ny, nz = c->geom.nz;
     while (std::max({nx, ny, nz}) > 4) {
       nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
@@ -1184,10 +1184,26 @@ int mg_build(pf_ctx* c) {
       c->mg.push_back(b);
     }
     HIP_TRY(c, dalloc(&c->mg_t0, c->dc.nblocks * kBlockVol));
+    for (auto& h : c->mg_h) HIP_TRY(c, dalloc(&h, c->dc.nblocks * kBlockVol));
+    HIP_TRY(c, dalloc(&c->mg_mx, 2));
+    HIP_TRY(c, dalloc(&c->mg_unscale, 2));
   }
   // level 1's residual is restricted from the FLUID blocks only: the aggregates of the
   // other blocks stay zero through the solve
   if (!c->mg.empty()) HIP_TRY(c, fill_f32(c->stream, c->mg[0].r, 0.f, c->mg[0].d.nblocks * kBlockVol));
+  // the FP16 level-0 operator of the row kernels
+  {
+    const long long n0 = c->dc.nblocks * kBlockVol;
+    HIP_TRY(c, fill_u32(c->stream, c->mg_mx, 0u, 2));
+    hipLaunchKernelGGL(k_mg_absmax, dim3(std::min<unsigned>(grid_for(n0, 256), 2048u)), dim3(256), 0, c->stream, n0,
+                       c->cd32, c->cx32, c->cy32, c->cz32, c->invd32, c->mg_mx);
+    _Float16* h[5];
+    for (int q = 0; q < 5; ++q) h[q] = reinterpret_cast<_Float16*>(c->mg_h[q]);
+    hipLaunchKernelGGL(k_mg_tohalf, dim3(std::min<unsigned>(grid_for(n0, 256), 4096u)), dim3(256), 0, c->stream, n0,
+                       c->cd32, c->cx32, c->cy32, c->cz32, c->invd32, h[0], h[1], h[2], h[3], h[4], c->mg_mx,
+                       c->mg_unscale);
+    LAUNCH_CHECK(c);
+  }
   MgLevel F = mg_level0(c);
   for (auto& b : c->mg) {
     MgLevel Cl = mg_level(b);
@@ -1229,7 +1245,12 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   ra.nlist = c->npblk;
   ra.done = done;
   ra.d = c->dc;
-  ra.cd = c->cd32; ra.cx = c->cx32; ra.cy = c->cy32; ra.cz = c->cz32; ra.inv = c->invd32;
+  ra.cd = reinterpret_cast<const _Float16*>(c->mg_h[0]);
+  ra.cx = reinterpret_cast<const _Float16*>(c->mg_h[1]);
+  ra.cy = reinterpret_cast<const _Float16*>(c->mg_h[2]);
+  ra.cz = reinterpret_cast<const _Float16*>(c->mg_h[3]);
+  ra.inv = reinterpret_cast<const _Float16*>(c->mg_h[4]);
+  ra.unscale = c->mg_unscale;
   ra.r = r;
   ra.omega = P.omega;
   ra.omega_c = P.correction;
@@ -2032,6 +2053,9 @@ int pf_destroy(pf_ctx* c) {
   }
   c->mg.clear();
   dfree(c->mg_t0);
+  for (auto& h : c->mg_h) dfree(h);
+  dfree(c->mg_mx);
+  dfree(c->mg_unscale);
   if (c->mr_pin_host) hipHostFree(c->mr_pin_host);
   for (int a = 0; a < 3; ++a) { dfree(c->mr_valid[a]); dfree(c->mr_cf32[a]); dfree(c->mr_cf64[a]); }
   if (c->mr_bbox_host) hipHostFree(c->mr_bbox_host);

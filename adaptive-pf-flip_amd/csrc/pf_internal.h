@@ -236,6 +236,11 @@ struct Ctx {
   };
   std::vector<MgBuf> mg;
   float* mg_t0 = nullptr;
+  // FP16 copies of the level-0 cd, cx, cy, cz and diagonal inverse (scaled), their
+  // maxima (float bits) and scales
+  uint16_t *mg_h[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint32_t* mg_mx = nullptr;
+  float* mg_unscale = nullptr;
   pf_mg_cfg mg_cfg{};   // the cycle (set at creation, pf_set_multigrid)
 
   // per-kernel event timing (measurement hook)

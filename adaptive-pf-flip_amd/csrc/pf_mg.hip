@@ -227,6 +227,79 @@ __global__ void __launch_bounds__(256) k_mg_inverse(MgLevel L, float* __restrict
   }
 }
 
+// ---- FP16 copies of the level-0 operator for the cycle.  The cycle only has to be
+// an approximate inverse (the solve stops on the FP64 residual), so its level-0
+// passes read the coefficients and the diagonal inverse as FP16, each group scaled
+// by a power of two that brings its maximum to [0.5, 1]: a third fewer bytes per
+// pass.  Zeros stay zero, so the cells outside the system stay out.
+
+// mx[0] = max of cd, cx, cy, cz; mx[1] = max of inv (float bits; all values >= 0)
+__global__ void __launch_bounds__(256) k_mg_absmax(long long n, const float* __restrict__ cd,
+                                                   const float* __restrict__ cx, const float* __restrict__ cy,
+                                                   const float* __restrict__ cz, const float* __restrict__ inv,
+                                                   uint32_t* mx) {
+  float ma = 0.0f, mi = 0.0f;
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+    ma = fmaxf(ma, fmaxf(fmaxf(cd[q], cx[q]), fmaxf(cy[q], cz[q])));
+    mi = fmaxf(mi, inv[q]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ma = fmaxf(ma, __shfl_xor(ma, o, 64));
+    mi = fmaxf(mi, __shfl_xor(mi, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&mx[0], __float_as_uint(ma));
+    atomicMax(&mx[1], __float_as_uint(mi));
+  }
+}
+
+// 2^e with e = the exponent that brings m to [0.5, 1) (1 for m = 0 or not finite)
+__device__ inline float mg_pow2_above(float m) {
+  if (!(m > 0.0f) || !isfinite(m)) return 1.0f;
+  int e;
+  frexpf(m, &e);
+  return ldexpf(1.0f, e);
+}
+
+__global__ void __launch_bounds__(256) k_mg_tohalf(long long n, const float* __restrict__ cd,
+                                                   const float* __restrict__ cx, const float* __restrict__ cy,
+                                                   const float* __restrict__ cz, const float* __restrict__ inv,
+                                                   _Float16* __restrict__ hcd, _Float16* __restrict__ hcx,
+                                                   _Float16* __restrict__ hcy, _Float16* __restrict__ hcz,
+                                                   _Float16* __restrict__ hinv, const uint32_t* mx,
+                                                   float* __restrict__ unscale) {
+  const float ua = mg_pow2_above(__uint_as_float(mx[0])), ui = mg_pow2_above(__uint_as_float(mx[1]));
+  const float sa = 1.0f / ua, si = 1.0f / ui;  // powers of two: exact
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    unscale[0] = ua;
+    unscale[1] = ui;
+  }
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+    hcd[q] = (_Float16)(cd[q] * sa);
+    hcx[q] = (_Float16)(cx[q] * sa);
+    hcy[q] = (_Float16)(cy[q] * sa);
+    hcz[q] = (_Float16)(cz[q] * sa);
+    hinv[q] = (_Float16)(inv[q] * si);
+  }
+}
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline float mg_h2f(uint32_t bits) { return (float)__builtin_bit_cast(_Float16, (uint16_t)bits); }
+// four FP16 values of a row at the FP32 byte offset off (masked offsets read 0)
+__device__ inline F4 mg_ldh4(__amdgpu_buffer_rsrc_t r, uint32_t off, float scale) {
+  const u32x2 q = __builtin_amdgcn_raw_buffer_load_b64(r, off >> 1, 0, 0);
+  F4 f;
+  f.v[0] = mg_h2f(q.x & 0xffffu) * scale;
+  f.v[1] = mg_h2f(q.x >> 16) * scale;
+  f.v[2] = mg_h2f(q.y & 0xffffu) * scale;
+  f.v[3] = mg_h2f(q.y >> 16) * scale;
+  return f;
+}
+__device__ inline float mg_ldh1(__amdgpu_buffer_rsrc_t r, uint32_t off, float scale) {
+  return mg_h2f(__builtin_amdgcn_raw_buffer_load_b16(r, off >> 1, 0, 0)) * scale;
+}
+
 // ---- level 0 by cell rows (arrays below 2 GiB)
 //
 // The layout of k_pcg_spmv: a lane holds one x-row of 4 cells (row = ly + 4 lz, 16
@@ -248,7 +321,9 @@ struct MgRowArgs {
   long long nlist;
   const int* done;
   Dims d, dc;                                // level 0 and level 1 slabs
-  const float *cd, *cx, *cy, *cz, *inv, *r, *x;
+  const _Float16 *cd, *cx, *cy, *cz, *inv;   // FP16 level-0 operator (scaled)
+  const float* unscale;                      // [0] coefficients, [1] diagonal inverse
+  const float *r, *x;
   float* out;                                // the new iterate, or level 1's residual
   const float* xc;                           // level 1's correction (PSWEEP)
   float omega, omega_c;
@@ -259,11 +334,12 @@ struct MgRowBufs {
   __amdgpu_buffer_rsrc_t cd, cx, cy, cz, inv, r, x, out;
 };
 
-// the stencil input u of a row / of one cell at byte offset off
+// the stencil input u of a row / of one cell at byte offset off (ui: the diagonal
+// inverse's scale)
 template <int MODE>
-__device__ inline F4 mg_u4(const MgRowBufs& B, float omega, uint32_t off) {
+__device__ inline F4 mg_u4(const MgRowBufs& B, float omega, float ui, uint32_t off) {
   if (MODE == MG_FIRST2) {
-    const F4 i4 = buf_ld4(B.inv, off), r4 = buf_ld4(B.r, off);
+    const F4 i4 = mg_ldh4(B.inv, off, ui), r4 = buf_ld4(B.r, off);
     F4 u;
 #pragma unroll
     for (int i = 0; i < 4; ++i) u.v[i] = omega * i4.v[i] * r4.v[i];
@@ -272,8 +348,8 @@ __device__ inline F4 mg_u4(const MgRowBufs& B, float omega, uint32_t off) {
   return buf_ld4(B.x, off);
 }
 template <int MODE>
-__device__ inline float mg_u1(const MgRowBufs& B, float omega, uint32_t off) {
-  if (MODE == MG_FIRST2) return omega * buf_ld(B.inv, off) * buf_ld(B.r, off);
+__device__ inline float mg_u1(const MgRowBufs& B, float omega, float ui, uint32_t off) {
+  if (MODE == MG_FIRST2) return omega * mg_ldh1(B.inv, off, ui) * buf_ld(B.r, off);
   return buf_ld(B.x, off);
 }
 
@@ -288,13 +364,14 @@ __global__ void __launch_bounds__(256) k_mg_row(MgRowArgs a) {
   if (a.done && *a.done) return;
   const int lane = threadIdx.x & 63;
   const int qb = lane >> 4, row = lane & 15, ly = row & 3, lz = row >> 2;
-  const uint32_t bytes = (uint32_t)(a.d.nblocks * kBlockVol * 4);
+  const uint32_t bytes = (uint32_t)(a.d.nblocks * kBlockVol * 4), bytes16 = bytes / 2;
   MgRowBufs B;
-  B.cd = buf_rsrc(a.cd, bytes);
-  B.cx = buf_rsrc(a.cx, bytes);
-  B.cy = buf_rsrc(a.cy, bytes);
-  B.cz = buf_rsrc(a.cz, bytes);
-  B.inv = buf_rsrc(a.inv, bytes);
+  B.cd = buf_rsrc(a.cd, bytes16);
+  B.cx = buf_rsrc(a.cx, bytes16);
+  B.cy = buf_rsrc(a.cy, bytes16);
+  B.cz = buf_rsrc(a.cz, bytes16);
+  B.inv = buf_rsrc(a.inv, bytes16);
+  const float ua = a.unscale[0], ui = a.unscale[1];
   B.r = buf_rsrc(a.r, bytes);
   B.x = buf_rsrc(a.x, bytes);
   B.out = buf_rsrc(a.out, bytes);
@@ -322,14 +399,15 @@ __global__ void __launch_bounds__(256) k_mg_row(MgRowArgs a) {
     const uint32_t oxp = fxp ? o + kBlockVol * 4 : kNoLoad, oxm = fxm ? o - kBlockVol * 4 + 12 : kNoLoad;
     const uint32_t oyp = fyp ? o + sy - 48 : kNoLoad, oym = fym ? o - sy + 48 : kNoLoad;
     const uint32_t ozp = fzp ? o + sz - 192 : kNoLoad, ozm = fzm ? o - sz + 192 : kNoLoad;
-    const F4 cd = buf_ld4(B.cd, o), cx = buf_ld4(B.cx, o), cy = buf_ld4(B.cy, o), cz = buf_ld4(B.cz, o);
-    const F4 iv = buf_ld4(B.inv, o), rr = buf_ld4(B.r, o);
-    F4 u = mg_u4<MODE>(B, om, o);
-    float uxp = mg_u1<MODE>(B, om, oxp), uxm = mg_u1<MODE>(B, om, oxm);
-    const float cxxm = buf_ld(B.cx, oxm);
-    F4 uyp = mg_u4<MODE>(B, om, oyp), uym = mg_u4<MODE>(B, om, oym);
-    F4 uzp = mg_u4<MODE>(B, om, ozp), uzm = mg_u4<MODE>(B, om, ozm);
-    const F4 cyym = buf_ld4(B.cy, oym), czzm = buf_ld4(B.cz, ozm);
+    const F4 cd = mg_ldh4(B.cd, o, ua), cx = mg_ldh4(B.cx, o, ua), cy = mg_ldh4(B.cy, o, ua);
+    const F4 cz = mg_ldh4(B.cz, o, ua);
+    const F4 iv = mg_ldh4(B.inv, o, ui), rr = buf_ld4(B.r, o);
+    F4 u = mg_u4<MODE>(B, om, ui, o);
+    float uxp = mg_u1<MODE>(B, om, ui, oxp), uxm = mg_u1<MODE>(B, om, ui, oxm);
+    const float cxxm = mg_ldh1(B.cx, oxm, ua);
+    F4 uyp = mg_u4<MODE>(B, om, ui, oyp), uym = mg_u4<MODE>(B, om, ui, oym);
+    F4 uzp = mg_u4<MODE>(B, om, ui, ozp), uzm = mg_u4<MODE>(B, om, ui, ozm);
+    const F4 cyym = mg_ldh4(B.cy, oym, ua), czzm = mg_ldh4(B.cz, ozm, ua);
     const int J = (4 * by + ly) >> 1, K = (4 * bz + lz) >> 1, I0 = 2 * bx;
     if (MODE == MG_PSWEEP) {
       if (valid) {
