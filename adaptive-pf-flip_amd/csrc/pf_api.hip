@@ -285,6 +285,9 @@ int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false,
   a.has_vol = c->has_vol;
   a.pow2 = is_pow2(c->geom.dx);
   a.inv_dx = 1.0 / c->geom.dx;
+  a.fast32 = a.pow2 && c->geom.ox == 0.0 && c->geom.oy == 0.0 && c->geom.oz == 0.0 &&
+             std::max({c->geom.nx, c->geom.ny, c->geom.nz}) < (1 << 22);
+  a.inv_dx_f = (float)a.inv_dx;
   for (int ax = 0; ax < 3; ++ax) a.valid[ax] = mr ? c->mr_valid[ax] : nullptr;
   a.mp_l = single ? 1.f : (float)(ph->rho_l * Vp);
   a.mp_g = single ? 1.f : (float)(ph->rho_g * Vp);

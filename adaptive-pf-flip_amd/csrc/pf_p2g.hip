@@ -43,6 +43,8 @@ struct P2GArgs {
   bool has_vol;
   bool pow2;              // dx is a power of two: (x - o)/dx == (x - o)*inv_dx exactly
   double inv_dx;
+  bool fast32;            // pow2, origin 0 and fewer than 2^22 cells a side (see splat_axis)
+  float inv_dx_f;
   float mp_l, mp_g;       // rho * Vp per phase
   const uint8_t* valid[3];  // KT = 2 (multires): the enumerated fine faces per axis
 };
@@ -160,10 +162,32 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
       first = false;
       q = s0 + bi;
     }
-    double px = grid_coord(a.P.x[q], a.g.ox, a), py = grid_coord(a.P.y[q], a.g.oy, a),
-           pz = grid_coord(a.P.z[q], a.g.oz, a);
-    // stencil coordinates of this axis' face grid (sparse_ops3d_tp.cpp:117-122)
-    double gx = AX == 0 ? px : px - 0.5, gy = AX == 1 ? py : py - 0.5, gz = AX == 2 ? pz : pz - 0.5;
+    // stencil coordinates of this axis' face grid (sparse_ops3d_tp.cpp:117-122) and
+    // the particle's offsets from the lane's cell
+    double gx, gy, gz;
+    float fx, fy, fz;
+    if (KT == 0 && a.fast32) {
+      // x/dx is exact in FP32 here, and each offset is either exact or one correctly
+      // rounded FP32 subtraction of exact values (sx - 0.5 is inexact only for
+      // sx < 0.25, in cell 0): the FP64 path's single rounding, bit for bit.  The
+      // FP64 coordinates (exact) are used only by the guard-band path.
+      const float sx = a.P.x[q] * a.inv_dx_f, sy = a.P.y[q] * a.inv_dx_f, sz = a.P.z[q] * a.inv_dx_f;
+      fx = (AX == 0 ? sx : sx - 0.5f) - (float)i;
+      fy = (AX == 1 ? sy : sy - 0.5f) - (float)j;
+      fz = (AX == 2 ? sz : sz - 0.5f) - (float)k;
+      gx = AX == 0 ? (double)sx : (double)sx - 0.5;
+      gy = AX == 1 ? (double)sy : (double)sy - 0.5;
+      gz = AX == 2 ? (double)sz : (double)sz - 0.5;
+    } else {
+      double px = grid_coord(a.P.x[q], a.g.ox, a), py = grid_coord(a.P.y[q], a.g.oy, a),
+             pz = grid_coord(a.P.z[q], a.g.oz, a);
+      gx = AX == 0 ? px : px - 0.5;
+      gy = AX == 1 ? py : py - 0.5;
+      gz = AX == 2 ? pz : pz - 0.5;
+      fx = (float)(gx - (double)i);
+      fy = (float)(gy - (double)j);
+      fz = (float)(gz - (double)k);
+    }
     const float vel = AX == 0 ? a.P.u[q] : (AX == 1 ? a.P.v[q] : a.P.w[q]);
     if (KT == 1) {
       float wt[A::NN];
@@ -176,7 +200,6 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
       }
       continue;
     }
-    float fx = (float)(gx - (double)i), fy = (float)(gy - (double)j), fz = (float)(gz - (double)k);
     float dx2[A::NX], dy2[A::NY], dz2[A::NZ];
 #pragma unroll
     for (int o = 0; o < A::NX; ++o) { float d = fx - (float)(o - 1); dx2[o] = d * d; }
