@@ -153,6 +153,28 @@ def test_every_cycle_shape_reaches_the_jacobi_solution(cycle):
     assert rel_err(ctx.field(G.FIELD_P), p_j) < 1e-6
 
 
+@pytest.mark.parametrize("rho_g", [0.1, 1000.0])
+def test_multigrid_at_extreme_density_ratios(rho_g):
+    # the cycle reads an FP16 copy of the level-0 operator scaled by its maximum: a
+    # liquid/gas ratio of 10^4 (and none) must still reach the FP64 tolerance and the
+    # Jacobi solution
+    n, dt = 32, 1e-3
+    print(f"[mg-ratio] rho_g={rho_g}", flush=True)
+    dx, ctx, ps, oph = _prepare(O.SCENE_RT, n, seed=13, rho_g=rho_g)
+    _to_projection(ctx, oph, dt)
+    faces = _faces(ctx)
+    st_j = ctx.project(_gphase(oph), dt, G.default_solve_cfg(G.STOP_REL2, 1e-9, 50000))
+    p_j = ctx.field(G.FIELD_P).astype(np.float64)
+    for f, a in zip((G.FIELD_U, G.FIELD_V, G.FIELD_W), faces):
+        ctx.set_field(f, a)
+    st_m = ctx.project(_gphase(oph), dt, G.default_solve_cfg(G.STOP_REL2, 1e-9, 50000, G.PRECOND_MULTIGRID))
+    print(f"iterations: jacobi {st_j.iterations}, multigrid {st_m.iterations}")
+    assert st_j.converged == 1 and st_m.converged == 1
+    assert st_m.true_rel_residual <= 1e-9 and st_m.fallback_used == 0
+    assert st_m.iterations * 2 <= st_j.iterations
+    assert rel_err(ctx.field(G.FIELD_P), p_j) < 1e-6
+
+
 def test_invalid_cycles_are_refused():
     ctx = G.Context(8, 8, 8, 1.0)
     for bad in (dict(sweeps=0), dict(sweeps=17), dict(coarsest_sweeps=0), dict(omega=1.0), dict(omega=0.0),
