@@ -1138,18 +1138,20 @@ int mr_reference_solve(pf_ctx* c, const pf_mr_solve_cfg& cf, unsigned G, const V
 
 bool mg_cfg_valid(const pf_mg_cfg& q) {
   return q.sweeps >= 1 && q.sweeps <= 16 && q.coarsest_sweeps >= 1 && q.coarsest_sweeps <= 1024 &&
-         q.omega > 0.0f && q.omega < 1.0f && q.correction > 0.0f && q.correction < 2.0f;
+         q.omega > 0.0f && q.omega < 1.0f && q.correction > 0.0f && q.correction < 2.0f &&
+         (q.chebyshev_ratio == 0.0f || (q.chebyshev_ratio > 1.0f && q.chebyshev_ratio <= 1000.0f));
 }
 
 // the built-in cycle, with the PF_MG_* environment overrides (read once)
 const pf_mg_cfg& mg_default_cfg() {
   static const pf_mg_cfg p = [] {
-    pf_mg_cfg q{2, 32, 0.9f, 1.8f};
+    pf_mg_cfg q{2, 32, 0.9f, 1.8f, 6.0f};
     pf_mg_cfg e = q;
     if (const char* v = std::getenv("PF_MG_SWEEPS")) e.sweeps = std::atoi(v);
     if (const char* v = std::getenv("PF_MG_COARSEST_SWEEPS")) e.coarsest_sweeps = std::atoi(v);
     if (const char* v = std::getenv("PF_MG_OMEGA")) e.omega = (float)std::atof(v);
     if (const char* v = std::getenv("PF_MG_CORRECTION")) e.correction = (float)std::atof(v);
+    if (const char* v = std::getenv("PF_MG_CHEB")) e.chebyshev_ratio = (float)std::atof(v);
     return mg_cfg_valid(e) ? e : q;
   }();
   return p;
@@ -1256,6 +1258,20 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   ra.unscale = c->mg_unscale;
   ra.r = r;
   ra.omega = P.omega;
+  // the weights of a smoothing pass's sweeps: all omega, or (a Chebyshev ratio r > 1)
+  // the reciprocals of the Chebyshev roots of degree nu on [lmax / r, lmax], lmax = 2
+  // (Gershgorin bound of D^-1 A), which damp that whole interval; the post-pass takes
+  // them in reverse order (they commute, so the cycle stays symmetric)
+  const float cheb = c->mg_cfg.chebyshev_ratio;
+  std::vector<float> wpre(P.nu, P.omega);
+  if (cheb > 1.0f) {
+    const double b = 2.0, lo = b / cheb;
+    for (int q = 0; q < P.nu; ++q) {
+      const double lam = 0.5 * (b + lo) + 0.5 * (b - lo) * std::cos((2.0 * q + 1.0) * M_PI / (2.0 * P.nu));
+      wpre[q] = (float)(1.0 / lam);
+    }
+  }
+  auto wpost = [&](int q) { return wpre[(size_t)(P.nu - 1 - q)]; };
   ra.omega_c = P.correction;
   if (nl > 0) ra.dc = c->mg[0].d;
   // the level-0 kernels that write an iterate alternate between the work array and
@@ -1271,20 +1287,25 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   // pre-smoothing
   if (rows && P.nu >= 2) {
     ra.x = nullptr; ra.out = nxt; ra.part = nullptr;
+    ra.omega_u = wpre[0];
+    ra.omega = wpre[1];
     hipLaunchKernelGGL(k_mg_row<MG_FIRST2>, dim3(gw), dim3(256), 0, c->stream, ra);
     flip();
     for (int s2 = 2; s2 < P.nu; ++s2) {
       ra.x = cur; ra.out = nxt;
+      ra.omega = wpre[(size_t)s2];
       hipLaunchKernelGGL(k_mg_row<MG_SWEEP>, dim3(gw), dim3(256), 0, c->stream, ra);
       flip();
     }
   } else {
-    hipLaunchKernelGGL(k_mg_first, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, nxt, P.omega);
+    hipLaunchKernelGGL(k_mg_first, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, nxt, wpre[0]);
     flip();
     const int more = nl > 0 ? P.nu - 1 : 2 * P.nu - 1;
     for (int s2 = 0; s2 < more; ++s2) {
       const bool last = nl == 0 && s2 == more - 1;
-      hipLaunchKernelGGL(k_mg_sweep, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, nxt, P.omega,
+      const int q = s2 + 1;  // sweep q of pre (q < nu) and post (q - nu) smoothing
+      const float om = q < P.nu ? wpre[(size_t)q] : wpost(q - P.nu);
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, nxt, om,
                          last ? rz : nullptr);
       flip();
     }
@@ -1317,9 +1338,10 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
       return;
     }
     const unsigned g = gdense(b);
-    hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, P.omega);
+    hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, wpre[0]);
     for (int s2 = 1; s2 < P.nu; ++s2) {
-      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, P.omega, nullptr);
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, wpre[(size_t)s2],
+                         nullptr);
       std::swap(x, t);
     }
     Ctx::MgBuf& nb = c->mg[(size_t)l];
@@ -1328,7 +1350,7 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
     hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, c->stream, wd, L, mg_level(nb), res[l + 1], x,
                        P.correction);
     for (int s2 = 0; s2 < P.nu; ++s2) {
-      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, P.omega, nullptr);
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, wpost(s2), nullptr);
       std::swap(x, t);
     }
     res[l] = x;
@@ -1338,10 +1360,12 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   // level 0: prolong and post-smooth, the last sweep into z with the partial r.z
   if (rows) {
     ra.x = cur; ra.out = nxt; ra.xc = res[1]; ra.part = P.nu == 1 ? rz : nullptr;
+    ra.omega = wpost(0);
     hipLaunchKernelGGL(k_mg_row<MG_PSWEEP>, dim3(ra.part ? g0 : gw), dim3(256), 0, c->stream, ra);
     flip();
     for (int s2 = 1; s2 < P.nu; ++s2) {
       ra.x = cur; ra.out = nxt; ra.part = s2 == P.nu - 1 ? rz : nullptr;
+      ra.omega = wpost(s2);
       hipLaunchKernelGGL(k_mg_row<MG_SWEEP>, dim3(ra.part ? g0 : gw), dim3(256), 0, c->stream, ra);
       flip();
     }
@@ -1349,7 +1373,7 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
     hipLaunchKernelGGL(k_mg_prolong, dim3(g0), dim3(256), 0, c->stream, w0, L0, mg_level(c->mg[0]), res[1], cur,
                        P.correction);
     for (int s2 = 0; s2 < P.nu; ++s2) {
-      hipLaunchKernelGGL(k_mg_sweep, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, nxt, P.omega,
+      hipLaunchKernelGGL(k_mg_sweep, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, nxt, wpost(s2),
                          s2 == P.nu - 1 ? rz : nullptr);
       flip();
     }

@@ -327,6 +327,7 @@ struct MgRowArgs {
   float* out;                                // the new iterate, or level 1's residual
   const float* xc;                           // level 1's correction (PSWEEP)
   float omega, omega_c;
+  float omega_u;                             // FIRST2: the first sweep's weight (omega: the second's)
   double* part;
 };
 
@@ -377,7 +378,7 @@ __global__ void __launch_bounds__(256) k_mg_row(MgRowArgs a) {
   B.out = buf_rsrc(a.out, bytes);
   const uint32_t sy = (uint32_t)a.d.nbx * kBlockVol * 4;
   const uint32_t sz = (uint32_t)a.d.nbx * a.d.nby * kBlockVol * 4;
-  const float om = a.omega, omc = a.omega_c;
+  const float om = a.omega, omc = a.omega_c, omu = a.omega_u;
   double rz = 0.0;
   // the XCD segments of FOR_LIST_BLOCKS, walked in slots of four list entries
   const long long seg = (a.nlist + 7) / 8, xcd = blockIdx.x & 7;
@@ -402,11 +403,11 @@ __global__ void __launch_bounds__(256) k_mg_row(MgRowArgs a) {
     const F4 cd = mg_ldh4(B.cd, o, ua), cx = mg_ldh4(B.cx, o, ua), cy = mg_ldh4(B.cy, o, ua);
     const F4 cz = mg_ldh4(B.cz, o, ua);
     const F4 iv = mg_ldh4(B.inv, o, ui), rr = buf_ld4(B.r, o);
-    F4 u = mg_u4<MODE>(B, om, ui, o);
-    float uxp = mg_u1<MODE>(B, om, ui, oxp), uxm = mg_u1<MODE>(B, om, ui, oxm);
+    F4 u = mg_u4<MODE>(B, omu, ui, o);
+    float uxp = mg_u1<MODE>(B, omu, ui, oxp), uxm = mg_u1<MODE>(B, omu, ui, oxm);
     const float cxxm = mg_ldh1(B.cx, oxm, ua);
-    F4 uyp = mg_u4<MODE>(B, om, ui, oyp), uym = mg_u4<MODE>(B, om, ui, oym);
-    F4 uzp = mg_u4<MODE>(B, om, ui, ozp), uzm = mg_u4<MODE>(B, om, ui, ozm);
+    F4 uyp = mg_u4<MODE>(B, omu, ui, oyp), uym = mg_u4<MODE>(B, omu, ui, oym);
+    F4 uzp = mg_u4<MODE>(B, omu, ui, ozp), uzm = mg_u4<MODE>(B, omu, ui, ozm);
     const F4 cyym = mg_ldh4(B.cy, oym, ua), czzm = mg_ldh4(B.cz, ozm, ua);
     const int J = (4 * by + ly) >> 1, K = (4 * bz + lz) >> 1, I0 = 2 * bx;
     if (MODE == MG_PSWEEP) {

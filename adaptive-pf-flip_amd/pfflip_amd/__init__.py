@@ -141,7 +141,7 @@ ADAPT_FIELDS = [n for n, _ in AdaptCfg._fields_]
 class MgCfg(C.Structure):
     """pf_mg_cfg: the cycle of the multigrid preconditioner (PRECOND_MULTIGRID)"""
     _fields_ = [("sweeps", C.c_int32), ("coarsest_sweeps", C.c_int32), ("omega", C.c_float),
-                ("correction", C.c_float)]
+                ("correction", C.c_float), ("chebyshev_ratio", C.c_float)]
 
 
 def mg_defaults() -> MgCfg:

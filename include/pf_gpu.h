@@ -86,10 +86,13 @@ enum pf_precond {
  * the grid down to at most 4 cells a side; each level smooths `sweeps` times before
  * and after its coarse correction, the coarsest `coarsest_sweeps` times. */
 typedef struct pf_mg_cfg {
-  int32_t sweeps;           /* damped-Jacobi sweeps before and after the coarse correction, 1..16 */
-  int32_t coarsest_sweeps;  /* damped-Jacobi sweeps on the coarsest level, 1..1024 */
+  int32_t sweeps;           /* Jacobi sweeps before and after the coarse correction, 1..16 */
+  int32_t coarsest_sweeps;  /* damped-Jacobi sweeps (omega) on the coarsest level, 1..1024 */
   float omega;              /* Jacobi damping, in (0, 1) */
   float correction;         /* weight of the coarse-grid correction, in (0, 2) */
+  float chebyshev_ratio;    /* 0: every sweep damped by omega; r in (1, 1000]: the sweeps of a
+                               smoothing pass take the reciprocals of the Chebyshev roots of
+                               degree `sweeps` on [2/r, 2] (2 bounds the spectrum of D^-1 A) */
 } pf_mg_cfg;
 
 /* Mirrors MRPressureSolveStats3D's core fields (src/pressure/multires_pressure3d.h:60-133) */
@@ -525,9 +528,10 @@ int pf_lpc_train(pf_lpc* lpc, pf_ctx* ctx, int32_t steps, double lr, uint64_t se
 int pf_lpc_loss(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, double* loss_learned, double* loss_jacobi);
 
 /* ---- multigrid preconditioner (no reference counterpart; PF_PRECOND_MULTIGRID) */
-/* the built-in cycle (2 sweeps, 32 coarsest sweeps, omega 0.9, correction 1.8), or
- * the PF_MG_SWEEPS / PF_MG_COARSEST_SWEEPS / PF_MG_OMEGA / PF_MG_CORRECTION
- * environment variables where set; every context starts with it */
+/* the built-in cycle (2 Chebyshev sweeps on [1/3, 2], 32 coarsest sweeps at omega
+ * 0.9, correction 1.8), or the PF_MG_SWEEPS / PF_MG_COARSEST_SWEEPS / PF_MG_OMEGA /
+ * PF_MG_CORRECTION / PF_MG_CHEB environment variables where set; every context
+ * starts with it */
 void pf_mg_defaults(pf_mg_cfg* cfg);
 /* the cycle of the following multigrid solves on ctx (PF_ERR_ARG out of range) */
 int pf_set_multigrid(pf_ctx* ctx, const pf_mg_cfg* cfg);

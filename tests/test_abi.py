@@ -41,7 +41,7 @@ def test_struct_layouts_match_the_header(tmp_path):
                    "printf(\"%zu %zu %zu %zu %zu %zu\\n\",sizeof(pf_adapt_cfg),sizeof(pf_adapt_stats),"
                    "offsetof(pf_step_cfg,liquid_volume_target),offsetof(pf_step_cfg,adapt),"
                    "offsetof(pf_step_stats,c_div),offsetof(pf_step_stats,adapt));"
-                   "printf(\"%zu %zu\\n\",sizeof(pf_mg_cfg),offsetof(pf_mg_cfg,correction));"
+                   "printf(\"%zu %zu\\n\",sizeof(pf_mg_cfg),offsetof(pf_mg_cfg,chebyshev_ratio));"
                    "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -54,7 +54,7 @@ def test_struct_layouts_match_the_header(tmp_path):
             G.StepCfg.skip_interface_diagnostics.offset, G.StepStats.iface.offset, G.StepStats.surface.offset,
             C.sizeof(G.AdaptCfg), C.sizeof(G.AdaptStats), G.StepCfg.liquid_volume_target.offset,
             G.StepCfg.adapt.offset, G.StepStats.c_div.offset, G.StepStats.adapt.offset,
-            C.sizeof(G.MgCfg), G.MgCfg.correction.offset]
+            C.sizeof(G.MgCfg), G.MgCfg.chebyshev_ratio.offset]
     assert got == want
 
 
@@ -72,6 +72,7 @@ def test_host_helpers_without_device():
     # the multigrid cycle's defaults are in range
     m = G.mg_defaults()
     assert 1 <= m.sweeps <= 16 and 1 <= m.coarsest_sweeps <= 1024 and 0 < m.omega < 1 and 0 < m.correction < 2
+    assert m.chebyshev_ratio == 0 or 1 < m.chebyshev_ratio <= 1000
 
 
 def test_null_arguments_are_rejected_without_device():

@@ -129,7 +129,8 @@ def test_multigrid_with_tiny_and_empty_systems():
 
 
 @pytest.mark.parametrize("cycle", [dict(sweeps=1), dict(sweeps=3), dict(sweeps=2, coarsest_sweeps=1),
-                                   dict(omega=0.6, correction=1.0), dict(sweeps=4, omega=0.7, correction=1.5)])
+                                   dict(omega=0.6, correction=1.0), dict(sweeps=4, omega=0.7, correction=1.5),
+                                   dict(chebyshev_ratio=0.0), dict(chebyshev_ratio=20.0, sweeps=3)])
 def test_every_cycle_shape_reaches_the_jacobi_solution(cycle):
     # one sweep (first sweep + fused prolongation), the fused pair of first sweeps,
     # more sweeps, a one-sweep coarsest level, no over-correction
@@ -178,7 +179,8 @@ def test_multigrid_at_extreme_density_ratios(rho_g):
 def test_invalid_cycles_are_refused():
     ctx = G.Context(8, 8, 8, 1.0)
     for bad in (dict(sweeps=0), dict(sweeps=17), dict(coarsest_sweeps=0), dict(omega=1.0), dict(omega=0.0),
-                dict(correction=2.0), dict(correction=0.0)):
+                dict(correction=2.0), dict(correction=0.0), dict(chebyshev_ratio=1.0),
+                dict(chebyshev_ratio=-3.0), dict(chebyshev_ratio=2000.0)):
         with pytest.raises(G.PfError, match="invalid multigrid cycle"):
             ctx.set_multigrid(**bad)
     assert ctx.multigrid().sweeps == G.mg_defaults().sweeps
