@@ -11,12 +11,14 @@
 // every coarse level is built from the one below by sums, in one pass per level, and
 // stays symmetric positive (semi)definite like the fine system.
 //
-// Cycle.  V-cycle from a zero guess: nu damped-Jacobi sweeps down, residual
+// Cycle.  V-cycle from a zero guess: nu Jacobi sweeps down (weights: one damping
+// factor, or the reciprocal Chebyshev roots of degree nu on [2/r, 2]), residual
 // restricted by summing the children (P^T), the coarse solve, the correction added
-// back (P), nu sweeps up; many sweeps on the coarsest level.  Symmetric in its
-// sweeps, so it is a fixed symmetric positive definite preconditioner and the
-// flexible-beta PCG around it needs no restarts.  Level 0 walks the FLUID block list;
-// coarse levels are dense block-major slabs (each 1/8 of the one below).
+// back (P, over-weighted), nu sweeps up with the same weights in reverse order; many
+// damped sweeps on the coarsest level.  The sweeps of a pass commute, so the cycle is
+// a fixed symmetric preconditioner inside the flexible-beta PCG.  Level 0 walks the
+// FLUID block list and reads an FP16 copy of the operator; coarse levels are dense
+// block-major slabs (each 1/8 of the one below) in FP32.
 #include "pf_internal.h"
 
 namespace pf {
