@@ -25,7 +25,7 @@ struct MRBlockKey3D {
   int level = 0, bx = 0, by = 0, bz = 0;
 };
 
-// the MRPressureSolveStats3D fields this build reports (multires_pressure3d.h:60-136);
+// the MRPressureSolveStats3D fields this library reports (multires_pressure3d.h:60-136);
 // the option counters are filled when the solve ran in the reference's order
 struct MRPressureSolveStats3D {
   int active_cells = 0, pinned_cell = -1, iterations = 0;

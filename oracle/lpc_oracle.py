@@ -1,8 +1,9 @@
 """FP64 oracle of the learned block preconditioner (TEST INFRASTRUCTURE ONLY).
 
-The network has no reference counterpart: it is the learned part of this build.
-Only tests/ and bench.py's baseline leg use this file, as the checker of the HIP
-kernels in adaptive-pf-flip_amd/csrc/pf_lpc.hip.
+The network has no reference counterpart: it is the learned preconditioner of
+the rows this package adds.  Only tests/ and bench.py's baseline leg use this file,
+as the FP64 statement that the HIP kernels in adaptive-pf-flip_amd/csrc/pf_lpc.hip
+are tested against.
 
 Network M(r; beta, phi), block-local on 16^3 cell tiles with a 1-cell input halo:
   input X[18,18,18,4] (z, y, x, channel): rho_hat, beta_hat, phi, 0

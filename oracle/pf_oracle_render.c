@@ -2,7 +2,7 @@
  * render-cache JSONL export (src/driver/render_cache3d.h): writeSparseRenderCache3D,
  * writeMRRenderCache3D and writeRenderCacheManifest3D, restated with stdio.  Numbers
  * print as %.17g, which is what the reference's std::setprecision(17) stream gives;
- * this build has no secondary particles, so their sections carry empty sets. */
+ * this library has no secondary particles, so their sections carry empty sets. */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
