@@ -115,6 +115,30 @@ def test_multigrid_steps_track_jacobi_steps():
     assert rel_err(b[3:], a[3:]) < 1e-4
 
 
+@pytest.mark.parametrize("kind", [O.SCENE_RT, O.SCENE_BUBBLE])
+def test_multigrid_steps_stay_converged(kind):
+    # ten adaptive-dt steps of an evolving interface: every solve reaches the FP64
+    # tolerance with the cycle (no Jacobi fallback) in a steady number of iterations
+    n = 48
+    print(f"[mg-steps] kind={kind}", flush=True)
+    ctx = G.Context(n, n, n, 1.0 / n)
+    ctx.scene(kind, 7)
+    oph = phase()
+    solve = G.default_solve_cfg(G.STOP_REL2, 1e-6, 50000, G.PRECOND_MULTIGRID)
+    solve.inner_rel_tol = 0.01
+    cfg = G.StepCfg(dt=2e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95, adaptive_cfl=0.5,
+                    adaptive_min_dt=1e-5, c_div=0.0, adaptive_timestep=1, advection_order=2, timing=0,
+                    phase=_gphase(oph), solve=solve)
+    its = []
+    for _ in range(10):
+        st = ctx.step(cfg)
+        assert st.solve.converged == 1 and st.solve.fallback_used == 0
+        assert st.solve.true_rel_residual <= 1e-6
+        its.append(st.solve.iterations)
+    print(f"iterations {its}")
+    assert max(its) <= 40
+
+
 def test_multigrid_with_tiny_and_empty_systems():
     # a grid of at most 4 cells a side has no coarse level; a grid with no FLUID cell
     # has nothing to solve
