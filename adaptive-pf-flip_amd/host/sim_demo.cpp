@@ -11,10 +11,12 @@
 //   tests/test_multires_sim3d_tp.cpp:422-439  adaptive timestep reports the effective dt
 //   tests/test_multires_sim3d_tp.cpp:558-573  c_div uses the liquid volume error
 //   tests/test_multires_sim3d_tp.cpp:575-594  high density ratio pressure converges
-// plus a step with the reference's coarse correction and coarse preconditioner on
+// plus the RT steps with the multigrid preconditioner against the Jacobi ones, and a
+// step with the reference's coarse correction and coarse preconditioner on
 // Host particle edits are followed by markParticlesDirty() (particles live on the
 // device) and device state is read back with syncParticlesToHost().
 // Exit status 0 when every check passes.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -82,6 +84,35 @@ int main() {
     CHECK(meanY(sim, 0) < heavy0);
     CHECK(maxActive > 0);
     std::printf("rt: heavy mean y %.4f -> %.4f, max active cell blocks %zu\n", heavy0, meanY(sim, 0), maxActive);
+  }
+  {
+    // the same RT steps with the reference's Jacobi preconditioner and with the
+    // multigrid one (PF_PRECOND_MULTIGRID): both stop on the same FP64 residual
+    SparseSim3DTP a(32, 32, 32, 1.0 / 32), b(32, 32, 32, 1.0 / 32);
+    int it[2] = {0, 0};
+    SparseSim3DTP* sims[2] = {&a, &b};
+    for (int q = 0; q < 2; ++q) {
+      SparseSim3DTP& sim = *sims[q];
+      sim.dt = 2e-3;
+      sim.stop_rule = PF_STOP_REL2;
+      sim.cg_tol = 1e-9;
+      sim.cg_iters = 20000;
+      sim.precond = q == 0 ? PF_PRECOND_JACOBI : PF_PRECOND_MULTIGRID;
+      sim.initRayleighTaylor();
+      for (int s = 0; s < 3; ++s) {
+        sim.step();
+        CHECK(sim.stats_last.solve.converged == 1);
+        it[q] += sim.stats_last.solve.iterations;
+      }
+      sim.syncParticlesToHost();
+    }
+    double dmax = 0.0;
+    for (size_t i = 0; i < a.particles.size(); ++i)
+      dmax = std::max(dmax, std::fabs(a.particles.pos[i].y - b.particles.pos[i].y));
+    CHECK(a.particles.size() == b.particles.size());
+    CHECK(dmax < 1e-5);
+    CHECK(it[1] * 3 <= it[0]);
+    std::printf("rt 32^3 x3: jacobi %d iterations, multigrid %d, max |dy| %.2e\n", it[0], it[1], dmax);
   }
   {
     SparseSim3DTP sim(8, 12, 8, 1.0);
