@@ -48,15 +48,17 @@ def test_single_phase_p2g_active_sets_and_fields():
     assert abs(ctx.field(G.FIELD_MV).astype(np.float64).sum() - len(ps)) < 1e-3 * len(ps)
 
 
-def test_single_phase_projection_matches_fp64_oracle():
+@pytest.mark.parametrize("precond", [0, 2])
+def test_single_phase_projection_matches_fp64_oracle(precond):
+    # precond 2: the multigrid preconditioner (same FP64 stop, same bars)
     n, dt = 24, 2e-3
     dx, ctx, ps = _setup(n)
     g = O.Grid(n, n, n, dx, faithful=False)
     ctx.mark_cells(); ctx.p2g_single(); ctx.save_velocity(); ctx.apply_gravity(dt, -9.81); ctx.apply_walls()
     g.mark_cells(ps); g.p2g_single(ps); g.apply_gravity(dt, -9.81); g.apply_walls()
-    st = ctx.project_single(dt, G.default_solve_cfg(G.STOP_REL2, 1e-9, 20000))
+    st = ctx.project_single(dt, G.default_solve_cfg(G.STOP_REL2, 1e-9, 20000, precond))
     ost = g.project_single(dt, 20000, 1e-12, O.STOP_REL2)
-    assert st.converged == 1 and st.true_rel_residual <= 1e-9
+    assert st.converged == 1 and st.true_rel_residual <= 1e-9 and st.fallback_used == 0
     assert st.pinned == 0 and ost.pinned == 0
     assert st.fluid_cells == ost.fluid_cells
     assert np.array_equal(np.sort(ctx.active_blocks(G.FIELD_P)), np.sort(g.active_blocks(O.P)))
