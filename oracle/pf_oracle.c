@@ -13,6 +13,7 @@
 #include "pf_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -393,12 +394,211 @@ static void normalize_axis(orc_grid* g, int fv, int fm, int lim_i, int lim_j, in
   }
 }
 
+/* ------------------------------------------------------------ host threads
+ * orc_set_threads(n > 1) splits the particle loops of P2G, G2P and advection over n
+ * POSIX threads so the checker finishes at 64^3-128^3 in seconds.  Every result is
+ * bit-identical to the serial loops:
+ *  - P2G: thread t owns a slab of block rows along z of each face field and walks all
+ *    particles in order, writing only its slab's nodes, so every node still receives
+ *    its contributions in particle order.  Activation records, per block, the first
+ *    (particle, stencil position) that touches it; sorting those keys rebuilds the
+ *    serial first-touch order exactly.
+ *  - G2P / advection: per-particle work on disjoint particle ranges; the clamp
+ *    counters are integer sums.
+ * The FP64 pressure solve stays serial (its dot products are order-dependent). */
+static int g_threads = 1;
+
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : (n > 256 ? 256 : n); }
+int orc_get_threads(void) { return g_threads; }
+
+typedef void (*orc_task_fn)(void* arg, int t, int nt);
+typedef struct { orc_task_fn fn; void* arg; int t, nt; } orc_task;
+
+static void* orc_task_main(void* a) {
+  orc_task* k = (orc_task*)a;
+  k->fn(k->arg, k->t, k->nt);
+  return NULL;
+}
+
+/* fn(arg, t, nt) for t = 0..nt-1 on nt threads (the caller runs t = 0) */
+static void parallel_run(int nt, orc_task_fn fn, void* arg) {
+  if (nt <= 1) { fn(arg, 0, 1); return; }
+  pthread_t th[256];
+  orc_task tk[256];
+  int started[256];
+  for (int t = 0; t < nt; ++t) {
+    tk[t].fn = fn; tk[t].arg = arg; tk[t].t = t; tk[t].nt = nt;
+    started[t] = 0;
+  }
+  for (int t = 1; t < nt; ++t) started[t] = pthread_create(&th[t], NULL, orc_task_main, &tk[t]) == 0;
+  fn(arg, 0, nt);
+  for (int t = 1; t < nt; ++t) {
+    if (started[t]) pthread_join(th[t], NULL);
+    else fn(arg, t, nt); /* thread creation failed: run that share here */
+  }
+}
+
+typedef struct {
+  orc_grid* g;
+  int64_t n;
+  const double *x, *y, *z, *u, *v, *w, *vol;
+  const uint8_t* type;
+  const orc_phase* pp;
+  double Vp;
+  int64_t* key[3]; /* first-touch key per block of the u, v, w face fields */
+} p2g_job;
+
+/* the slab of block rows [*b0, *b1) along z of a field with nbz rows, for thread t */
+static void slab_rows(int nbz, int t, int nt, int* b0, int* b1) {
+  *b0 = (int)((int64_t)nbz * t / nt);
+  *b1 = (int)((int64_t)nbz * (t + 1) / nt);
+}
+
+static void p2g_offsets(const orc_grid* g, int axis, double px, double py, double pz, double* gx, double* gy,
+                        double* gz) {
+  *gx = axis == 0 ? px : px - 0.5;
+  *gy = axis == 1 ? py : py - 0.5;
+  *gz = axis == 2 ? pz : pz - 0.5;
+  (void)g;
+}
+
+/* preactivateKernel restricted to node rows [klo, khi): the least (particle, stencil
+ * position) touching each block, as p * 216 + position */
+static void p2g_activate_task(void* a, int t, int nt) {
+  p2g_job* J = (p2g_job*)a;
+  orc_grid* g = J->g;
+  const int rad = (int)ceil(KR);
+  for (int axis = 0; axis < 3; ++axis) {
+    orc_field* f = &g->f[ORC_U + axis];
+    int b0, b1;
+    slab_rows(f->nbz, t, nt, &b0, &b1);
+    const int klo = 4 * b0, khi = 4 * b1;
+    if (klo >= khi) continue;
+    int64_t* key = J->key[axis];
+    for (int64_t p = 0; p < J->n; ++p) {
+      double px = (J->x[p] - g->ox) / g->dx, py = (J->y[p] - g->oy) / g->dx, pz = (J->z[p] - g->oz) / g->dx;
+      double gx, gy, gz;
+      p2g_offsets(g, axis, px, py, pz, &gx, &gy, &gz);
+      int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
+      if (k0 + rad + 1 < klo || k0 - rad >= khi) continue;
+      int e = 0;
+      for (int dk = -rad; dk <= rad + 1; ++dk)
+        for (int dj = -rad; dj <= rad + 1; ++dj)
+          for (int di = -rad; di <= rad + 1; ++di, ++e) {
+            int i = i0 + di, j = j0 + dj, k = k0 + dk;
+            if (k < klo || k >= khi) continue;
+            if (i < 0 || i >= f->nx || j < 0 || j >= f->ny || k < 0 || k >= f->nz) continue;
+            double ddx = gx - i, ddy = gy - j, ddz = gz - k;
+            if (kern(ddx * ddx + ddy * ddy + ddz * ddz, KR) <= 0.0) continue;
+            int64_t b = bid_of(f, i / BLK, j / BLK, k / BLK);
+            if (key[b] == INT64_MAX) key[b] = p * 216 + e;
+          }
+    }
+  }
+}
+
+static int cmp_key_bid(const void* a, const void* b) {
+  const int64_t* x = (const int64_t*)a;
+  const int64_t* y = (const int64_t*)b;
+  return x[0] < y[0] ? -1 : (x[0] > y[0] ? 1 : 0);
+}
+
+/* activation order of field f from the first-touch keys */
+static void activate_by_keys(orc_field* f, const int64_t* key) {
+  int64_t cnt = 0;
+  for (int64_t b = 0; b < f->nblocks; ++b) cnt += key[b] != INT64_MAX;
+  int64_t* kb = (int64_t*)malloc(sizeof(int64_t) * 2 * (cnt > 0 ? cnt : 1));
+  int64_t q = 0;
+  for (int64_t b = 0; b < f->nblocks; ++b)
+    if (key[b] != INT64_MAX) { kb[2 * q] = key[b]; kb[2 * q + 1] = b; ++q; }
+  qsort(kb, (size_t)cnt, 2 * sizeof(int64_t), cmp_key_bid);
+  for (int64_t a = 0; a < cnt; ++a) {
+    int64_t b = kb[2 * a + 1];
+    f->map[b] = (int32_t)f->nactive;
+    f->order[f->nactive++] = (int32_t)b;
+  }
+  free(kb);
+}
+
+/* splatKernelColor restricted to node rows [klo, khi) of the thread's slab; every
+ * written block is already active */
+static void p2g_splat_task(void* a, int t, int nt) {
+  p2g_job* J = (p2g_job*)a;
+  orc_grid* g = J->g;
+  const int rad = (int)ceil(KR);
+  for (int axis = 0; axis < 3; ++axis) {
+    orc_field* F = &g->f[ORC_U + axis];
+    orc_field* M = &g->f[ORC_MU + axis];
+    const double* vel = axis == 0 ? J->u : (axis == 1 ? J->v : J->w);
+    int b0, b1;
+    slab_rows(F->nbz, t, nt, &b0, &b1);
+    const int klo = 4 * b0, khi = 4 * b1;
+    if (klo >= khi) continue;
+    const int nx = F->nx, ny = F->ny, nz = F->nz;
+    for (int64_t p = 0; p < J->n; ++p) {
+      double px = (J->x[p] - g->ox) / g->dx, py = (J->y[p] - g->oy) / g->dx, pz = (J->z[p] - g->oz) / g->dx;
+      double gx, gy, gz;
+      p2g_offsets(g, axis, px, py, pz, &gx, &gy, &gz);
+      int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
+      if (k0 + rad + 1 < klo || k0 - rad >= khi) continue;
+      double rho = (J->type[p] == 0) ? J->pp->rho_l : J->pp->rho_g;
+      double mp = rho * J->Vp * (J->vol ? J->vol[p] : 1.0);
+      double mom = mp * vel[p];
+      double wsum = 0.0;
+      for (int dk = -rad; dk <= rad + 1; ++dk)
+        for (int dj = -rad; dj <= rad + 1; ++dj)
+          for (int di = -rad; di <= rad + 1; ++di) {
+            int i = i0 + di, j = j0 + dj, k = k0 + dk;
+            if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) continue;
+            double ddx = gx - i, ddy = gy - j, ddz = gz - k;
+            wsum += kern(ddx * ddx + ddy * ddy + ddz * ddz, KR);
+          }
+      if (wsum <= 0.0) continue;
+      for (int dk = -rad; dk <= rad + 1; ++dk)
+        for (int dj = -rad; dj <= rad + 1; ++dj)
+          for (int di = -rad; di <= rad + 1; ++di) {
+            int i = i0 + di, j = j0 + dj, k = k0 + dk;
+            if (k < klo || k >= khi) continue;
+            if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) continue;
+            double ddx = gx - i, ddy = gy - j, ddz = gz - k;
+            double wt = kern(ddx * ddx + ddy * ddy + ddz * ddz, KR) / wsum;
+            if (wt <= 0.0) continue;
+            int64_t off = bid_of(F, i / BLK, j / BLK, k / BLK) * BVOL + local_of(i, j, k);
+            F->val[off] = acc(g, F->val[off], wt * mom);
+            M->val[off] = acc(g, M->val[off], wt * mp);
+          }
+    }
+  }
+}
+
+static void p2g_threaded(orc_grid* g, int64_t n, const double* x, const double* y, const double* z,
+                         const double* u, const double* v, const double* w, const uint8_t* type,
+                         const double* vol, const orc_phase* pp, double Vp) {
+  p2g_job J = {g, n, x, y, z, u, v, w, vol, type, pp, Vp, {NULL, NULL, NULL}};
+  for (int axis = 0; axis < 3; ++axis) {
+    orc_field* f = &g->f[ORC_U + axis];
+    J.key[axis] = (int64_t*)malloc(sizeof(int64_t) * f->nblocks);
+    for (int64_t b = 0; b < f->nblocks; ++b) J.key[axis][b] = INT64_MAX;
+  }
+  parallel_run(g_threads, p2g_activate_task, &J);
+  for (int axis = 0; axis < 3; ++axis) {
+    activate_by_keys(&g->f[ORC_U + axis], J.key[axis]);  /* u, v, w and their masses: same sets */
+    activate_by_keys(&g->f[ORC_MU + axis], J.key[axis]);
+    free(J.key[axis]);
+  }
+  parallel_run(g_threads, p2g_splat_task, &J);
+}
+
 /* spP2G3D_tp, sparse_ops3d_tp.cpp:95-153 */
 void orc_p2g(orc_grid* g, int64_t n, const double* x, const double* y, const double* z,
              const double* u, const double* v, const double* w, const uint8_t* type,
              const double* vol, const orc_phase* pp, double Vp) {
   for (int f = ORC_U; f <= ORC_MW; ++f) field_clear(&g->f[f]);
   const int nx = g->nx, ny = g->ny, nz = g->nz;
+  if (g_threads > 1) {
+    p2g_threaded(g, n, x, y, z, u, v, w, type, vol, pp, Vp);
+    n = 0; /* splat done: only the normalisation below remains */
+  }
   for (int64_t p = 0; p < n; ++p) {
     double px = (x[p] - g->ox) / g->dx, py = (y[p] - g->oy) / g->dx, pz = (z[p] - g->oz) / g->dx;
     preactivate(&g->f[ORC_U], px, py - 0.5, pz - 0.5, nx + 1, ny, nz);
@@ -811,10 +1011,10 @@ static inline double sample_w(const orc_grid* g, double px, double py, double pz
 }
 
 /* spG2P3D_tp, sparse_ops3d_tp.cpp:155-172 */
-void orc_g2p(const orc_grid* g, const orc_grid* saved, int64_t n, const double* x,
-             const double* y, const double* z, double* u, double* v, double* w,
-             const uint8_t* type, double aL, double aG) {
-  for (int64_t p = 0; p < n; ++p) {
+static void g2p_range(const orc_grid* g, const orc_grid* saved, int64_t p0, int64_t p1, const double* x,
+                      const double* y, const double* z, double* u, double* v, double* w,
+                      const uint8_t* type, double aL, double aG) {
+  for (int64_t p = p0; p < p1; ++p) {
     double a = (type[p] == 0) ? aL : aG;
     double px = (x[p] - g->ox) / g->dx, py = (y[p] - g->oy) / g->dx, pz = (z[p] - g->oz) / g->dx;
     double un = sample_u(g, px, py, pz), vn = sample_v(g, px, py, pz), wn = sample_w(g, px, py, pz);
@@ -828,6 +1028,28 @@ void orc_g2p(const orc_grid* g, const orc_grid* saved, int64_t n, const double* 
   }
 }
 
+typedef struct {
+  const orc_grid *g, *saved;
+  int64_t n;
+  const double *x, *y, *z;
+  double *u, *v, *w;
+  const uint8_t* type;
+  double aL, aG;
+} g2p_job;
+
+static void g2p_task(void* a, int t, int nt) {
+  g2p_job* J = (g2p_job*)a;
+  g2p_range(J->g, J->saved, J->n * t / nt, J->n * (t + 1) / nt, J->x, J->y, J->z, J->u, J->v, J->w, J->type,
+            J->aL, J->aG);
+}
+
+void orc_g2p(const orc_grid* g, const orc_grid* saved, int64_t n, const double* x,
+             const double* y, const double* z, double* u, double* v, double* w,
+             const uint8_t* type, double aL, double aG) {
+  g2p_job J = {g, saved, n, x, y, z, u, v, w, type, aL, aG};
+  parallel_run(n >= 4096 ? g_threads : 1, g2p_task, &J);
+}
+
 /* sampleVelocity, sparse_ops3d_tp.cpp:176-183 */
 static void sample_vel(const orc_grid* g, double x, double y, double z, double out[3]) {
   double px = (x - g->ox) / g->dx, py = (y - g->oy) / g->dx, pz = (z - g->oz) / g->dx;
@@ -837,15 +1059,59 @@ static void sample_vel(const orc_grid* g, double x, double y, double z, double o
 }
 
 /* spAdvect3D_tp, advectMidpoint, advectRK3: sparse_ops3d_tp.cpp:185-232 */
+static void advect_range(const orc_grid* g, int64_t p0, int64_t p1, double* x, double* y, double* z,
+                         const uint8_t* type, double dt, int order, orc_escape_stats* es,
+                         int32_t* escaped_idx, int64_t* n_escaped);
+
+typedef struct {
+  const orc_grid* g;
+  int64_t n;
+  double *x, *y, *z;
+  const uint8_t* type;
+  double dt;
+  int order;
+  orc_escape_stats es[256];
+  int64_t ne[256];
+} advect_job;
+
+static void advect_task(void* a, int t, int nt) {
+  advect_job* J = (advect_job*)a;
+  advect_range(J->g, J->n * t / nt, J->n * (t + 1) / nt, J->x, J->y, J->z, J->type, J->dt, J->order, &J->es[t], NULL,
+               &J->ne[t]);
+}
+
 void orc_advect(const orc_grid* g, int64_t n, double* x, double* y, double* z,
                 const uint8_t* type, double dt, int order, orc_escape_stats* es,
                 int32_t* escaped_idx, int64_t* n_escaped) {
+  if (g_threads <= 1 || escaped_idx || n < 4096) {
+    advect_range(g, 0, n, x, y, z, type, dt, order, es, escaped_idx, n_escaped);
+    return;
+  }
+  advect_job* J = (advect_job*)calloc(1, sizeof(advect_job));
+  J->g = g; J->n = n; J->x = x; J->y = y; J->z = z; J->type = type; J->dt = dt; J->order = order;
+  parallel_run(g_threads, advect_task, J);
+  if (es) memset(es, 0, sizeof(*es));
+  int64_t ne = 0;
+  for (int t = 0; t < g_threads; ++t) {
+    ne += J->ne[t];
+    if (!es) continue;
+    es->clamped_liquid += J->es[t].clamped_liquid; es->clamped_gas += J->es[t].clamped_gas;
+    es->x_lo += J->es[t].x_lo; es->x_hi += J->es[t].x_hi; es->y_lo += J->es[t].y_lo;
+    es->y_hi += J->es[t].y_hi; es->z_lo += J->es[t].z_lo; es->z_hi += J->es[t].z_hi;
+  }
+  if (n_escaped) *n_escaped = ne;
+  free(J);
+}
+
+static void advect_range(const orc_grid* g, int64_t p0, int64_t p1, double* x, double* y, double* z,
+                         const uint8_t* type, double dt, int order, orc_escape_stats* es,
+                         int32_t* escaped_idx, int64_t* n_escaped) {
   double lox = g->ox + 0.5 * g->dx, hix = g->ox + (g->nx - 0.5) * g->dx;
   double loy = g->oy + 0.5 * g->dx, hiy = g->oy + (g->ny - 0.5) * g->dx;
   double loz = g->oz + 0.5 * g->dx, hiz = g->oz + (g->nz - 0.5) * g->dx;
   if (es) memset(es, 0, sizeof(*es));
   int64_t ne = 0;
-  for (int64_t p = 0; p < n; ++p) {
+  for (int64_t p = p0; p < p1; ++p) {
     double P[3] = {x[p], y[p], z[p]}, k1[3], k2[3], k3[3], nx_[3];
     sample_vel(g, P[0], P[1], P[2], k1);
     if (order == 3) {

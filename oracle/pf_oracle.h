@@ -91,6 +91,11 @@ double orc_calibrate_rho_tilde0(const orc_phase* pp, double Vp);
 
 /* step stages (src/driver/sparse_sim3d_tp.cpp, src/driver/sparse_ops3d_tp.cpp) */
 void orc_mark_cells(orc_grid* g, int64_t n, const double* x, const double* y, const double* z);
+/* host threads for the particle loops of P2G, G2P and advection (default 1); every
+ * result is bit-identical to the serial loops (first-touch activation order included) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 void orc_p2g(orc_grid* g, int64_t n, const double* x, const double* y, const double* z,
              const double* u, const double* v, const double* w, const uint8_t* type,
              const double* vol, const orc_phase* pp, double Vp);

@@ -310,6 +310,12 @@ def build(ref: bool = True) -> None:
 _lib = None
 
 
+def set_threads(n: int) -> None:
+    """Host threads for the oracle's particle loops (P2G, G2P, advection); results are
+    bit-identical to one thread."""
+    lib().orc_set_threads(int(n))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -326,6 +332,8 @@ def lib():
         L.orc_get.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.orc_field_dense.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         L.orc_active_blocks.restype = C.c_int64
         L.orc_active_blocks.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_clear_field.argtypes = [C.c_void_p, C.c_int]
