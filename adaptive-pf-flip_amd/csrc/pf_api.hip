@@ -3097,6 +3097,64 @@ int pf_active_blocks(pf_ctx* c, int32_t field, int32_t* bids, int64_t capacity, 
   return PF_OK;
 }
 
+int pf_bin_counts(pf_ctx* c, uint32_t* dense) {
+  if (!c || !dense) return PF_ERR_ARG;
+  int st = ensure_binned(c);
+  if (st) return st;
+  const long long n = (long long)c->dc.nx * c->dc.ny * c->dc.nz;
+  uint32_t* out = reinterpret_cast<uint32_t*>(c->stage_f);  // stage_cap >= cells
+  hipLaunchKernelGGL(k_counts_dense, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, c->dc, c->cell_count, out);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, hipMemcpyAsync(dense, out, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+int pf_color_lists(pf_ctx* c, int32_t* lists, int64_t capacity, int64_t counts[8]) {
+  if (!c || !counts) return PF_ERR_ARG;
+  int st = ensure_binned(c);
+  if (st) return st;
+  HIP_TRY(c, hipMemcpyAsync(c->u32_host, c->color_cnt, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  long long total = 0;
+  for (int k = 0; k < 8; ++k) {
+    counts[k] = c->u32_host[k];
+    total += counts[k];
+  }
+  if (!lists) return PF_OK;
+  if (capacity < total) return fail(c, PF_ERR_ARG, "pf_color_lists: capacity below the summed list lengths");
+  long long off = 0;
+  for (int k = 0; k < 8; ++k) {
+    if (counts[k] == 0) continue;
+    HIP_TRY(c, hipMemcpyAsync(lists + off, c->color_list + (long long)k * c->color_cap,
+                              sizeof(int32_t) * (size_t)counts[k], hipMemcpyDeviceToHost, c->stream));
+    off += counts[k];
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  off = 0;
+  for (int k = 0; k < 8; ++k) {  // append order within a colour is free: return it sorted
+    std::sort(lists + off, lists + off + counts[k]);
+    off += counts[k];
+  }
+  return PF_OK;
+}
+
+int pf_face_phi(pf_ctx* c, const pf_phase* ph, int32_t axis, double* dense) {
+  if (!c || !ph || !dense || axis < 0 || axis > 2) return PF_ERR_ARG;
+  const Dims& d = axis == 0 ? c->du : (axis == 1 ? c->dv : c->dw);
+  const long long n = (long long)d.nx * d.ny * d.nz;
+  double* out = nullptr;
+  HIP_TRY(c, dalloc(&out, n));
+  hipLaunchKernelGGL(k_face_phi, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, c->f[F_MU + axis],
+                     phase_k(ph), out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(dense, out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(out);
+  if (e != hipSuccess) return fail(c, PF_ERR_HIP, std::string("pf_face_phi: ") + hipGetErrorString(e));
+  return PF_OK;
+}
+
 double pf_calibrate_rho_tilde0(const pf_phase* ph, double Vp) { return ph ? calibrate_host(ph, Vp) : 0.0; }
 
 double pf_phi_from_raw(double rt, const pf_phase* pp) {

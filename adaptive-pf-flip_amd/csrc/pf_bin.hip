@@ -101,6 +101,17 @@ __global__ void k_color_lists(Dims dc, const uint32_t* __restrict__ start, uint3
   }
 }
 
+// Per-cell particle counts of the binning in dense linear order i + nx*(j + ny*k).
+__global__ void k_counts_dense(Dims dc, const uint32_t* __restrict__ cnt, uint32_t* __restrict__ out) {
+  const long long n = (long long)dc.nx * dc.ny * dc.nz;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = (int)(q % dc.nx);
+  const long long t = q / dc.nx;
+  const int j = (int)(t % dc.ny), k = (int)(t / dc.ny);
+  out[q] = cnt[bm_index(dc, i, j, k)];
+}
+
 // Return particles to the caller's order (out[id[p]] = in[p]).
 __global__ void k_unsort(long long n, PBuf S, float* __restrict__ x, float* __restrict__ y,
                          float* __restrict__ z, float* __restrict__ u, float* __restrict__ v,

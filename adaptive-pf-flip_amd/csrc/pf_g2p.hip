@@ -10,6 +10,7 @@
 // and advection run fused (the step), the first RK stage reuses the PIC sample at
 // the particle (same post-projection grid, same position), saving 24 taps.
 #include "pf_internal.h"
+#include "pf_speed.h"
 
 namespace pf {
 
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(256) k_g2p_advect(G2PArgs a) {
       v = (v + (k1[1] - sv[1])) * al + k1[1] * (1.f - al);
       w = (w + (k1[2] - sv[2])) * al + k1[2] * (1.f - al);
       a.P.u[p] = u; a.P.v[p] = v; a.P.w[p] = w;
-      if (isfinite(u) && isfinite(v) && isfinite(w)) speed = fmaxf(speed, sqrtf(u * u + v * v + w * w));
+      speed = fmaxf(speed, particle_speed(u, v, w));
     }
     if (a.do_advect) {
       float pos[3] = {x, y, z}, nxt[3], k2[3];
@@ -157,8 +158,7 @@ __global__ void __launch_bounds__(256) k_max_speed(long long n, PBuf P, uint32_t
   float s = 0.f;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
-    float u = P.u[p], v = P.v[p], w = P.w[p];
-    if (isfinite(u) && isfinite(v) && isfinite(w)) s = fmaxf(s, sqrtf(u * u + v * v + w * w));
+    s = fmaxf(s, particle_speed(P.u[p], P.v[p], P.w[p]));
   }
   int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   block_flush_counters(cnt, s, nullptr, vmax_bits, false, true);

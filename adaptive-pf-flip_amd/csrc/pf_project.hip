@@ -41,6 +41,18 @@ __device__ inline double beta_of_raw(double raw, const PhaseK& k) {
   return 1.0 / (phi * k.rho_l + (1.0 - phi) * k.rho_g);
 }
 
+// Face phase field phi(raw face mass) in FP64, dense linear order of the face field
+// (the phi that the assembly's beta uses, phasefield.h:10-14)
+__global__ void k_face_phi(Dims d, const float* __restrict__ mass, PhaseK ph, double* __restrict__ out) {
+  const long long n = (long long)d.nx * d.ny * d.nz;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = (int)(q % d.nx);
+  const long long t = q / d.nx;
+  const int j = (int)(t % d.ny), k = (int)(t / d.ny);
+  out[q] = phi_of_raw((double)mass[bm_index(d, i, j, k)], ph);
+}
+
 struct CellNbr {
   long long xp, xm, yp, ym, zp, zm;
 };

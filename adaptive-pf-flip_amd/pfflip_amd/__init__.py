@@ -290,6 +290,9 @@ def lib():
             "pf_particle_count": (i64, [vp]),
             "pf_scene_generate": (C.c_int, [vp, i32, C.c_uint64]),
             "pf_mark_cells": (C.c_int, [vp]),
+            "pf_bin_counts": (C.c_int, [vp, vp]),
+            "pf_color_lists": (C.c_int, [vp, vp, i64, vp]),
+            "pf_face_phi": (C.c_int, [vp, C.POINTER(Phase), i32, vp]),
             "pf_p2g": (C.c_int, [vp, C.POINTER(Phase), dbl]),
             "pf_save_velocity": (C.c_int, [vp]),
             "pf_apply_gravity": (C.c_int, [vp, dbl, dbl]),
@@ -587,6 +590,28 @@ class Context:
         ba = None if block_active is None else np.ascontiguousarray(block_active, np.uint8)
         self._check(lib().pf_field_upload(self.h, field, _p(d), _p(ba)), "pf_field_upload")
 
+    def bin_counts(self):
+        """particles per cell from the binning, (nz, ny, nx)"""
+        out = np.zeros(self.nx * self.ny * self.nz, np.uint32)
+        self._check(lib().pf_bin_counts(self.h, _p(out)), "pf_bin_counts")
+        return out.reshape(self.nz, self.ny, self.nx)
+
+    def color_lists(self):
+        """the eight P2G colour lists of source blocks, each ascending"""
+        cnt = np.zeros(8, np.int64)
+        self._check(lib().pf_color_lists(self.h, None, 0, _p(cnt)), "pf_color_lists")
+        flat = np.zeros(max(int(cnt.sum()), 1), np.int32)
+        self._check(lib().pf_color_lists(self.h, _p(flat), len(flat), _p(cnt)), "pf_color_lists")
+        off = np.concatenate([[0], np.cumsum(cnt)])
+        return [flat[off[k]:off[k + 1]] for k in range(8)]
+
+    def face_phi(self, phase, axis):
+        """FP64 phi of the current face masses of one axis, computed on the device"""
+        nx, ny, nz = self.dims(FIELD_MU + axis)
+        out = np.zeros(nx * ny * nz, np.float64)
+        self._check(lib().pf_face_phi(self.h, C.byref(phase), axis, _p(out)), "pf_face_phi")
+        return out.reshape(nz, ny, nx)
+
     def active_blocks(self, field):
         n = C.c_int64()
         self._check(lib().pf_active_blocks(self.h, field, None, 0, C.byref(n)), "pf_active_blocks")
@@ -727,6 +752,7 @@ class SparseSim3D:
         self.inner_rel_tol, self.warm_start, self.check_every = 0.1, 0, 8
         # CFL-limited dt (timestep3d.h:16-50); off in the reference class
         self.adaptive_timestep, self.adaptive_cfl, self.adaptive_min_dt = False, 1.0, 1e-5
+        self.precond = PRECOND_JACOBI
         self.timing = False
         self.stats_last = None
 
@@ -736,7 +762,7 @@ class SparseSim3D:
     def step_cfg(self) -> StepCfg:
         solve = SolveCfg(max_iters=self.cg_iters, stop_rule=self.stop_rule, tol=self.cg_tol,
                          inner_rel_tol=self.inner_rel_tol, max_outer=10000,
-                         warm_start=self.warm_start, check_every=self.check_every)
+                         warm_start=self.warm_start, check_every=self.check_every, precond=self.precond)
         return StepCfg(dt=self.dt, gravity=self.gravity, Vp=1.0, alpha_liquid=self.alpha,
                        alpha_gas=self.alpha, adaptive_cfl=self.adaptive_cfl,
                        adaptive_min_dt=self.adaptive_min_dt, c_div=0.0,

@@ -15,10 +15,21 @@ One "step" is SparseSim3DTP::step (bin, mark, P2G, save/gravity/walls, assembly,
 solve, velocity update, G2P + RK2 advection).  W untimed warm-up steps, then K
 timed steps bracketed by device synchronisation.  Prints one JSON line.
 
-The CPU baseline times the reference's own SparseSim3DTP::step (oracle/_ref,
-compiled from the reference sources) on a bounded sample: the same scene
-construction at 64^3 (1,906,624 particles), scaled by the particle count to the
-256^3 workload.  It is a reported baseline, not the target.
+The roofline object is for the step's dominant kernel (the one with the largest
+summed event time in the per-kernel table: the P2G splat), timed live with HIP events
+on the library stream inside the timed steps.
+
+The CPU baseline times the repository's C oracle in float-faithful mode (the same
+results as the reference's SparseSim3DTP::step, bit for bit, on identical state;
+8 P2G threads as the reference's 8 colour tasks, the other loops on one thread) on a
+bounded sample: the same scene construction at 64^3 (1,906,624 particles) with the
+reference's Jacobi PCG and stop rule, next to the GPU running the same 64^3 step with
+the same solver and stop rule; the 256^3 figure scales the sample by the particle
+count and is an estimate.  It is a reported baseline, not the target.
+
+Secondary lines: the 256^3 step with the reference's Jacobi preconditioner, the
+bubble tank at 128^3 and the dam-break at 256^3 (200 CFL-1 steps are the test; the
+bench times a few), each with per-stage ms and algorithmic GB/s.
 """
 import argparse
 import json
@@ -31,6 +42,17 @@ sys.path.insert(0, os.path.join(ROOT, "adaptive-pf-flip_amd"))
 
 HBM_PEAK_GBS = 8000.0
 SPMV_BYTES_PER_CELL = 32  # z, p_old, diag, 3 couplings read; p_new, Ap written (FP32)
+# algorithmic bytes per stage (FP32 = 4 B): per particle and per FLUID cell
+P2G_BYTES = (25, 24)        # particle x, y, z, u, v, w, type read; u, v, w, mu, mv, mw written
+STAGE_BYTES = {             # (per particle, per fluid cell, per grid cell)
+    "bin": (58, 0, 0),      # particle read + sorted write, key + rank
+    "mark": (0, 0, 5),      # counts read, marker written
+    "p2g": (25, 24, 0),
+    "grid": (0, 60, 0),     # 3 axes: velocity + mass read, velocity + saved written
+    "assemble": (0, 45, 0),
+    "velocity": (0, 40, 0),
+    "g2p_advect": (49, 24, 0),
+}
 
 
 def read_traffic(path):
@@ -126,10 +148,19 @@ def pressure_comparison(G, ctx, cfg, tol, learned=True):
     return out
 
 
-def cpu_baseline(n_sample, n_full, seed):
-    """Reference SparseSim3DTP::step on the n_sample^3 RT scene, scaled to n_full^3."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
+    if not os.path.exists(O.ORACLE_SO):
+        O.build(ref=False)
+    return O
+
+
+def cpu_baseline(n_sample, n_full, seed):
+    """The float-faithful C oracle's SparseSim3DTP::step on the n_sample^3 RT scene
+    (reference stop rule |r|_inf < 1e-7, 600 Jacobi PCG iterations, P2G on 8 threads,
+    the rest on one), scaled to n_full^3 by the particle count."""
+    O = _oracle()
     dx = 1.0 / n_sample
     x, y, z, t = O.scene(O.SCENE_RT, n_sample, n_sample, n_sample, dx, seed)
     ph = O.Phase(1000.0, 1.0, 1.0, 1.0)
@@ -140,33 +171,102 @@ def cpu_baseline(n_sample, n_full, seed):
                     c_div_strength=1.0, liquid_volume_target=0.0, phase=ph)
     ps = O.Particles(x, y, z, type=t)
     scale = O.lib().orc_scene_count(O.SCENE_RT, n_full, n_full, n_full) / len(x)
-    if O.ref_available():
-        rs = O.RefSim(n_sample, n_sample, n_sample, dx)
-        rs.config(cfg)
-        rs.set_particles(ps)
-        t0 = time.perf_counter()
-        out = rs.step()
-        wall = (time.perf_counter() - t0) * 1e3
-        kind = "reference"
-        ms = out[12]
-    else:
+    O.set_threads_split(8, 1)
+    try:
         g = O.Grid(n_sample, n_sample, n_sample, dx, faithful=True)
         t0 = time.perf_counter()
-        g.step(ps, cfg)
-        wall = ms = (time.perf_counter() - t0) * 1e3
-        kind = "port"
+        st = g.step(ps, cfg)
+        ms = (time.perf_counter() - t0) * 1e3
+    finally:
+        O.set_threads(1)
+    stages = {"mark": st.ms_mark, "p2g": st.ms_p2g, "save_gravity_walls": st.ms_save_gravity_walls,
+              "project": st.ms_project, "g2p": st.ms_g2p, "advect": st.ms_advect}
     return {
         "value": round(ms * scale, 1),
-        "unit": f"ms/step (estimated at {n_full}^3)",
+        "unit": f"ms/step (estimated at {n_full}^3 from the {n_sample}^3 sample)",
         "cores": 8,
-        "kind": kind,
-        "sample": (f"one SparseSim3DTP::step of the {n_sample}^3 RT scene ({len(x)} particles, "
-                   f"reference stop rule |r|_inf < 1e-7, cg_iters 600) took {ms:.0f} ms on the host; "
-                   f"scaled x{scale:.2f} by particle count; P2G uses 8 std::async colour tasks, "
-                   f"the rest one thread"),
+        "kind": "port",
+        "sample": (f"one SparseSim3DTP::step of the {n_sample}^3 RT scene ({len(x)} particles) by the "
+                   f"float-faithful C oracle (bit-exact to the reference step on identical state): "
+                   f"{ms:.0f} ms on the host, Jacobi PCG {st.solve.iterations} it, reference stop rule "
+                   f"|r|_inf < 1e-7 / 600 iterations; P2G on 8 threads (the reference's 8 colour tasks), "
+                   f"the rest on one; scaled x{scale:.2f} by particle count"),
         "sample_ms": round(ms, 1),
+        "sample_stage_ms": {k: round(v, 1) for k, v in stages.items()},
+        "sample_solve_iterations": st.solve.iterations,
         "nproc": os.cpu_count(),
     }
+
+
+def gpu_sample_pair(G, n, seed, steps=3):
+    """the same n^3 RT step on the GPU with the reference's solver and stop rule
+    (Jacobi PCG, |r|_inf < 1e-7, 600 iterations): the GPU side of the CPU pair"""
+    ctx = G.Context(n, n, n, 1.0 / n)
+    try:
+        ctx.scene(G.SCENE_RT, seed)
+        ph = G.Phase(1000.0, 1.0, 1.0, 1.0)
+        ph.rho_tilde_0 = G.calibrate_rho_tilde0(ph, 1.0)
+        solve = G.SolveCfg(max_iters=600, stop_rule=G.STOP_REF_MAXABS, tol=1e-7, inner_rel_tol=0.1,
+                           max_outer=10000, warm_start=0, check_every=16, precond=0)
+        cfg = G.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95, adaptive_cfl=0.5,
+                        adaptive_min_dt=1e-5, c_div=0.0, adaptive_timestep=0, advection_order=2, timing=0,
+                        phase=ph, solve=solve)
+        first = ctx.step(cfg)  # the state the CPU sample steps from
+        ctx.synchronize()
+        ctx2 = G.Context(n, n, n, 1.0 / n)
+        try:
+            ctx2.scene(G.SCENE_RT, seed)
+            ctx2.synchronize()
+            t0 = time.perf_counter()
+            st = ctx2.step(cfg)
+            ctx2.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+        finally:
+            ctx2.close()
+        return {"ms": round(ms, 3), "solve_iterations": st.solve.iterations,
+                "first_step_iterations": first.solve.iterations, "converged": st.solve.converged}
+    finally:
+        ctx.close()
+
+
+def stage_gbs(stage_ms, np_, nc, ncell_total):
+    out = {}
+    for k, (bp, bc, bg) in STAGE_BYTES.items():
+        ms = stage_ms.get(k, 0.0)
+        if ms > 0:
+            out[k] = round((bp * np_ + bc * nc + bg * ncell_total) / (ms * 1e-3) / 1e9, 1)
+    return out
+
+
+def run_workload(G, kind, n, seed, dt, steps, warmup, precond, adaptive=False, cap=None, label=""):
+    """ms/step, per-stage ms and algorithmic GB/s of a scene at n^3"""
+    ctx = G.Context(n, n, n, 1.0 / n)
+    try:
+        ctx.scene(kind, seed)
+        cfg = step_cfg(G, n, 1e-6, 50000, 0, precond)
+        cfg.dt = cap if cap else dt
+        if adaptive:
+            cfg.adaptive_timestep, cfg.adaptive_cfl = 1, 1.0
+        for _ in range(warmup):
+            ctx.step(cfg)
+        ctx.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            stats.append(ctx.step(cfg))
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        stage = {name: round(sum(s.stage_ms[i] for s in stats) / len(stats), 3) for i, name in enumerate(G.STAGES)}
+        nc = stats[-1].solve.fluid_cells
+        print(f"[bench] {label}: {ms:.2f} ms/step", file=sys.stderr, flush=True)
+        return {"workload": label, "ms_per_step": round(ms, 3), "steps": steps, "warmup": warmup,
+                "particles": ctx.count(), "fluid_cells": nc, "stage_ms": stage,
+                "stage_gbs": stage_gbs(stage, ctx.count(), nc, n ** 3),
+                "solve_iterations": [s.solve.iterations for s in stats],
+                "true_rel_residual_max": max(s.solve.true_rel_residual for s in stats),
+                "effective_dt": [round(s.effective_dt, 8) for s in stats]}
+    finally:
+        ctx.close()
 
 
 def main():
@@ -184,6 +284,8 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner in the comparison")
     ap.add_argument("--no-pressure", action="store_true", help="skip the pressure-solver comparison")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the secondary lines (Jacobi step, bubble 128^3, dam-break 256^3)")
     ap.add_argument("--precond", choices=sorted(SOLVERS), default="multigrid",
                     help="preconditioner of the timed steps' pressure solve")
     args = ap.parse_args()
@@ -203,10 +305,11 @@ def main():
         print(f"[bench] warmup step: {st.total_ms:.1f} ms, solve {st.solve.iterations} it", file=sys.stderr,
               flush=True)
     ctx.synchronize()
-    # inside the timed steps only the roofline kernel carries events (a pair per
-    # launch); the per-kernel table comes from one more step afterwards
+    # inside the timed steps only the roofline kernels carry events (a pair per
+    # launch: the P2G colour launches and the SpMV); the per-kernel table comes from
+    # one more step afterwards
     if not args.no_kernel_timing:
-        ctx.kernel_timing(True, ["pcg_spmv"])
+        ctx.kernel_timing(True, ["p2g", "pcg_spmv"])
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -220,11 +323,14 @@ def main():
     stage = {name: round(sum(s.stage_ms[i] for s in stats) / len(stats), 3) for i, name in enumerate(G.STAGES)}
     iters = [s.solve.iterations for s in stats]
     fluid = stats[-1].solve.fluid_cells
+    npart = ctx.count()
     kernels = {}
-    roofline = None
+    roofline = roofline_spmv = None
     if not args.no_kernel_timing:
-        nl, ms = ctx.kernel_time(G.KERNELS.index("pcg_spmv"))
-        sp = {"launches": nl, "avg_ms": round(ms / nl, 5), "total_ms": round(ms, 3)} if nl else None
+        live = {}
+        for name in ("p2g", "pcg_spmv"):
+            nl, ms = ctx.kernel_time(G.KERNELS.index(name))
+            live[name] = (nl, ms)
         print("[bench] per-kernel event times over one more step", file=sys.stderr, flush=True)
         ctx.kernel_timing(True)
         ctx.step(cfg)
@@ -233,17 +339,31 @@ def main():
             if nl:
                 kernels[name] = {"launches": nl, "avg_ms": round(ms / nl, 5), "total_ms": round(ms, 3)}
         ctx.kernel_timing(False)
-        if sp:
-            bytes_per_launch = SPMV_BYTES_PER_CELL * fluid
-            achieved = bytes_per_launch / (sp["avg_ms"] * 1e-3) / 1e9
-            traffic = None
-            tf = os.path.join(ROOT, "profiles", "pcg_spmv_rt256_traffic.md")
-            if os.path.exists(tf) and n == 256:
-                traffic = read_traffic(tf)
-            roofline = {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": round(achieved, 1),
+        dominant = max(kernels, key=lambda k: kernels[k]["total_ms"]) if kernels else None
+        nl, ms = live["p2g"]
+        if nl:
+            # one launch per colour: each carries an eighth of the step's algorithmic P2G bytes
+            avg = ms / nl
+            per_launch = (P2G_BYTES[0] * npart + P2G_BYTES[1] * fluid) / 8
+            achieved = per_launch / (avg * 1e-3) / 1e9
+            tf = os.path.join(ROOT, "profiles", "p2g_color_rt256_traffic.md")
+            roofline = {"bound": "hbm", "kernel": "k_p2g_color<0>", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": traffic, "bytes_per_launch": bytes_per_launch,
-                        "avg_launch_ms": sp["avg_ms"]}
+                        "traffic": read_traffic(tf) if (os.path.exists(tf) and n == 256) else None,
+                        "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg, 5),
+                        "launches_timed": nl, "dominant_kernel_in_step_table": dominant,
+                        "note": "bytes = (25 B/particle + 24 B/fluid cell) / 8 colour launches; the kernel is "
+                                "VALU-bound (its counter summary is in profiles/)"}
+        nl, ms = live["pcg_spmv"]
+        if nl:
+            bytes_per_launch = SPMV_BYTES_PER_CELL * fluid
+            avg = ms / nl
+            achieved = bytes_per_launch / (avg * 1e-3) / 1e9
+            tf = os.path.join(ROOT, "profiles", "pcg_spmv_rt256_traffic.md")
+            roofline_spmv = {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": round(achieved, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                             "traffic": read_traffic(tf) if (os.path.exists(tf) and n == 256) else None,
+                             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg, 5)}
     # achievable HBM bandwidth of a plain copy kernel on this box (reported beside
     # the 8 TB/s peak; also the known-size kernel for calibrating HBM counters)
     print("[bench] copy probe", file=sys.stderr, flush=True)
@@ -253,11 +373,30 @@ def main():
     pressure = None
     if not args.no_pressure:
         pressure = pressure_comparison(G, ctx, cfg, args.tol, not args.no_learned)
-    cpu = None
+    ctx.close()
+    workloads = {}
+    if not args.no_workloads:
+        workloads["rt256_jacobi"] = run_workload(
+            G, G.SCENE_RT, n, args.seed, 1e-3, 2, 1, "jacobi",
+            label=f"RT {n}^3 with the reference's Jacobi preconditioner, FP64 residual <= 1e-6")
+        workloads["bubble128"] = run_workload(
+            G, G.SCENE_BUBBLE, 128, args.seed, 5e-4, 10, 2, args.precond,
+            label="bubble tank 128^3, 1000:1, dt 5e-4")
+        workloads["dambreak256"] = run_workload(
+            G, G.SCENE_DAMBREAK, 256, args.seed, None, 20, 5, args.precond, adaptive=True, cap=5e-3,
+            label="dam-break 256^3 (liquid only), CFL 1, requested dt cap 5e-3")
+    cpu = pair = None
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline (reference step on the 64^3 sample) ...", file=sys.stderr, flush=True)
+        print(f"[bench] gpu side of the {args.cpu_sample_n}^3 pair", file=sys.stderr, flush=True)
+        pair = {"workload": f"RT {args.cpu_sample_n}^3, Jacobi PCG, |r|_inf < 1e-7, 600 iterations (the "
+                            f"reference's solver and stop rule on both sides)",
+                "gpu": gpu_sample_pair(G, args.cpu_sample_n, args.seed)}
+        print("[bench] cpu baseline (float-faithful oracle on the 64^3 sample) ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_sample_n, n, args.seed)
         print(f"[bench] cpu sample {cpu['sample_ms']} ms", file=sys.stderr, flush=True)
+        pair["cpu"] = {"ms": cpu["sample_ms"], "solve_iterations": cpu["sample_solve_iterations"],
+                       "threads": "P2G 8, rest 1"}
+        pair["speedup"] = round(cpu["sample_ms"] / pair["gpu"]["ms"], 1)
 
     line = {
         "metric": "step ms @256\u00b3 two-phase; pressure ms to 1e-6 resid, learned-PC vs plain PCG",
@@ -273,19 +412,23 @@ def main():
         "dtype": "f32 (FP64 dots, scalars and refinement)",
         "data": "synthetic (seeded RT scene)",
         "config": {"workload": f"sparse two-phase Rayleigh-Taylor {n}^3, rho 1000:1, dt 1e-3, "
-                               f"{ctx.count()} particles, {SOLVER_NAMES[args.precond]} + FP64 refinement to "
+                               f"{npart} particles, {SOLVER_NAMES[args.precond]} + FP64 refinement to "
                                f"{args.tol:g} rel. residual",
-                   "grid": n, "particles": ctx.count(), "fluid_cells": fluid,
+                   "grid": n, "particles": npart, "fluid_cells": fluid,
                    "solver": f"{args.precond}-pcg-fp32+ir-fp64", "warm_start": bool(args.warm_start)},
         "stage_ms": stage,
+        "stage_gbs": stage_gbs(stage, npart, fluid, n ** 3),
         "pressure_ms_to_1e-6": pressure,
         "solve_iterations": iters,
         "true_rel_residual": [s.solve.true_rel_residual for s in stats],
         "kernels": kernels,
         "kernels_note": "event times of every kernel over one step run after the timed steps; inside the "
-                        "timed steps only k_pcg_spmv carries events (for the roofline)",
+                        "timed steps only the P2G and SpMV launches carry events (for the rooflines)",
         "roofline": roofline,
+        "roofline_spmv": roofline_spmv,
         "copy_kernel_gbs": copy_gbs,
+        "workloads": workloads,
+        "cpu_gpu_pair": pair,
         "cpu_baseline": cpu,
     }
     if cpu:

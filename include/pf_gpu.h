@@ -293,6 +293,19 @@ int pf_field_upload(pf_ctx* ctx, int32_t field, const float* dense, const uint8_
 /* Active block ids of a field in ascending order (the set of
  * SparseBlockGrid3D::active_block_ids, sparse_block_grid3d.h:16); count returned in *n. */
 int pf_active_blocks(pf_ctx* ctx, int32_t field, int32_t* bids, int64_t capacity, int64_t* n);
+/* Particle count of every cell from the binning, dense i + nx*(j + ny*k) order: the
+ * cell of a particle is markCells' (int)((x - o)/dx) per axis
+ * (src/driver/sparse_sim3d_tp.cpp:43-46).  Bins the current particles if needed. */
+int pf_bin_counts(pf_ctx* ctx, uint32_t* dense);
+/* The P2G colour partition: for colour c = color8(bx, by, bz) (src/grid/block_color.h:7,
+ * runColor8 in src/driver/sparse_ops3d_common.h:17-28), the cell blocks holding
+ * particles, ascending, concatenated colour 0..7 into lists (capacity entries);
+ * counts[c] = length of colour c's list (lists may be NULL to query the counts). */
+int pf_color_lists(pf_ctx* ctx, int32_t* lists, int64_t capacity, int64_t counts[8]);
+/* FP64 phase field phi of the current raw face masses of one axis (0 u, 1 v, 2 w),
+ * computed on the device by the function the assembly's beta uses
+ * (phiFromRawDensity, src/physics/phasefield.h:10-14); dense over the face field. */
+int pf_face_phi(pf_ctx* ctx, const pf_phase* phase, int32_t axis, double* dense);
 
 /* ---- host-side physics helpers (no device work) */
 /* calibrateRhoTilde0, src/transfer/transfer3d_tp.cpp:29-36 */

@@ -406,9 +406,15 @@ static void normalize_axis(orc_grid* g, int fv, int fm, int lim_i, int lim_j, in
  *  - G2P / advection: per-particle work on disjoint particle ranges; the clamp
  *    counters are integer sums.
  * The FP64 pressure solve stays serial (its dot products are order-dependent). */
-static int g_threads = 1;
+static int g_threads = 1;      /* P2G */
+static int g_threads_adv = 1;  /* G2P and advection */
 
-void orc_set_threads(int n) { g_threads = n < 1 ? 1 : (n > 256 ? 256 : n); }
+static int clamp_threads(int n) { return n < 1 ? 1 : (n > 256 ? 256 : n); }
+void orc_set_threads(int n) { g_threads = g_threads_adv = clamp_threads(n); }
+void orc_set_threads_split(int p2g, int particles) {
+  g_threads = clamp_threads(p2g);
+  g_threads_adv = clamp_threads(particles);
+}
 int orc_get_threads(void) { return g_threads; }
 
 typedef void (*orc_task_fn)(void* arg, int t, int nt);
@@ -1047,7 +1053,7 @@ void orc_g2p(const orc_grid* g, const orc_grid* saved, int64_t n, const double* 
              const double* y, const double* z, double* u, double* v, double* w,
              const uint8_t* type, double aL, double aG) {
   g2p_job J = {g, saved, n, x, y, z, u, v, w, type, aL, aG};
-  parallel_run(n >= 4096 ? g_threads : 1, g2p_task, &J);
+  parallel_run(n >= 4096 ? g_threads_adv : 1, g2p_task, &J);
 }
 
 /* sampleVelocity, sparse_ops3d_tp.cpp:176-183 */
@@ -1083,16 +1089,16 @@ static void advect_task(void* a, int t, int nt) {
 void orc_advect(const orc_grid* g, int64_t n, double* x, double* y, double* z,
                 const uint8_t* type, double dt, int order, orc_escape_stats* es,
                 int32_t* escaped_idx, int64_t* n_escaped) {
-  if (g_threads <= 1 || escaped_idx || n < 4096) {
+  if (g_threads_adv <= 1 || escaped_idx || n < 4096) {
     advect_range(g, 0, n, x, y, z, type, dt, order, es, escaped_idx, n_escaped);
     return;
   }
   advect_job* J = (advect_job*)calloc(1, sizeof(advect_job));
   J->g = g; J->n = n; J->x = x; J->y = y; J->z = z; J->type = type; J->dt = dt; J->order = order;
-  parallel_run(g_threads, advect_task, J);
+  parallel_run(g_threads_adv, advect_task, J);
   if (es) memset(es, 0, sizeof(*es));
   int64_t ne = 0;
-  for (int t = 0; t < g_threads; ++t) {
+  for (int t = 0; t < g_threads_adv; ++t) {
     ne += J->ne[t];
     if (!es) continue;
     es->clamped_liquid += J->es[t].clamped_liquid; es->clamped_gas += J->es[t].clamped_gas;

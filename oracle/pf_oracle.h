@@ -94,6 +94,9 @@ void orc_mark_cells(orc_grid* g, int64_t n, const double* x, const double* y, co
 /* host threads for the particle loops of P2G, G2P and advection (default 1); every
  * result is bit-identical to the serial loops (first-touch activation order included) */
 void orc_set_threads(int n);
+/* P2G threads and G2P/advection threads separately (the reference's step runs P2G as
+ * 8 colour tasks and the particle loops on one thread) */
+void orc_set_threads_split(int p2g, int particles);
 int orc_get_threads(void);
 
 void orc_p2g(orc_grid* g, int64_t n, const double* x, const double* y, const double* z,

@@ -316,6 +316,11 @@ def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
 
 
+def set_threads_split(p2g: int, particles: int) -> None:
+    """P2G threads and G2P/advection threads separately"""
+    lib().orc_set_threads_split(int(p2g), int(particles))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -333,6 +338,7 @@ def lib():
         L.orc_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
         L.orc_field_dense.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads_split.argtypes = [C.c_int, C.c_int]
         L.orc_get_threads.restype = C.c_int
         L.orc_active_blocks.restype = C.c_int64
         L.orc_active_blocks.argtypes = [C.c_void_p, C.c_int, C.c_void_p]

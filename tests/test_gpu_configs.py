@@ -1,0 +1,320 @@
+"""The benchmark workloads at their own sizes.
+
+* RT 64^3 (1,906,624 particles, 1000:1, dt 1e-3, FLIP 0.95): the first step stage by
+  stage against the FP64 oracle on identical seeded input, with the reference's
+  Jacobi preconditioner and with the multigrid one; then 100 GPU steps (FP32, FP64
+  residual <= 1e-6) against the FP64 oracle's 100-step trajectory stored in
+  tests/golden/c0_rt64_trajectory.npz (tests/golden/make_c0_rt64_trajectory.py).
+* Bubble tank 128^3 (9,652,608 particles, water y < 0.6, air bubble r = 0.12 at
+  (0.5, 0.25, 0.5), empty headspace, 1000:1, dt 5e-4): the first step against the
+  FP64 oracle at full size; then 24 steps with the checks of the reference's
+  bubble-tank test (tests/test_sparse_sim3d_tp.cpp:749-786): particle count kept,
+  finite positions, the gas rises, the headspace stays sparse.
+* Dam-break 256^3 (31,711,392 particles, x < 0.4, y < 0.6): 200 CFL-1 steps
+  (computeAdaptiveParticleTimestep3D, src/driver/timestep3d.h:16-50, requested dt
+  cap 5e-3) on the two-phase path with liquid only and on the single-phase path.
+* RT 256^3 (131,096,512 particles): two steps with the benchmark's solver, plus the
+  size-independent P2G identities (sum of face masses = sum of particle masses,
+  sum of face momenta = sum of particle momenta: the normalised kernel is a
+  partition of unity, tests/test_sparse_transfer3d_tp.cpp:11-29).
+
+Stage bars against the FP64 oracle: markers, active-block sets and clamp counters
+bit-exact; face fields and phi <= 1e-5 norm-wise; pressure <= 1e-6; the
+post-projection divergence within 1e-5 of the pre-projection divergence norm-wise;
+positions <= 1e-6 absolute, velocities <= 1e-4 norm-wise after the step; the
+post-projection face velocities, where u and (dt/dx) beta grad p nearly cancel,
+<= 3e-5 norm-wise.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import pfflip_amd as G
+import pyoracle as O
+from _helpers import phase, rel_err
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _gphase(oph):
+    return G.Phase(oph.rho_l, oph.rho_g, oph.alpha_phi, oph.rho_tilde_0)
+
+
+def _threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _log(msg):
+    print(f"[configs] {msg}", flush=True)
+
+
+def _solve(precond, tol):
+    cfg = G.default_solve_cfg(G.STOP_REL2, tol, 100000, precond)
+    if precond == G.PRECOND_MULTIGRID:
+        cfg.inner_rel_tol, cfg.check_every = 0.01, 4
+    return cfg
+
+
+def _phi(raw, oph):
+    """Eq. 7 (phiFromRawDensity, src/physics/phasefield.h:10-14) on an array; the
+    device-phi test pins this against the oracle's own phi"""
+    rmin = np.log(oph.rho_l / oph.rho_g) * oph.rho_g * oph.rho_tilde_0
+    den = oph.alpha_phi * oph.rho_tilde_0 * oph.rho_l
+    return np.where(raw < rmin, 0.0, np.minimum(np.sqrt(np.maximum(raw - rmin, 0.0) / den), 1.0))
+
+
+def _div(U, V, W, mk, dx):
+    d = (U[:, :, 1:] - U[:, :, :-1]) + (V[:, 1:, :] - V[:, :-1, :]) + (W[1:, :, :] - W[:-1, :, :])
+    return d[mk == 1] / dx
+
+
+def _staged_first_step(kind, n, dt, seed, preconds):
+    """one step stage by stage on the GPU and in the FP64 oracle, same particles"""
+    dx = 1.0 / n
+    oph = phase()
+    gph = _gphase(oph)
+    x, y, z, t = O.scene(kind, n, n, n, dx, seed)
+    _log(f"scene {kind} n {n}: {len(x)} particles")
+    O.set_threads(_threads())
+    try:
+        t0 = time.time()
+        ps = O.Particles(x, y, z, type=t)
+        g = O.Grid(n, n, n, dx, faithful=False)
+        g.mark_cells(ps)
+        g.p2g(ps, oph, 1.0)
+        saved = g.clone()
+        g.apply_gravity(dt, -9.81)
+        g.apply_walls()
+        pre = [g.dense(f) for f in (O.U, O.V, O.W)]
+        pre_ps = ps.copy()
+        ost = g.project(oph, dt, 100000, 1e-10, 0.0, O.STOP_REL2)
+        g.g2p(saved, ps, 0.95, 0.95)
+        oes, _ = g.advect(ps, dt, 2)
+        _log(f"oracle step {time.time() - t0:.1f} s ({ost.iterations} it)")
+    finally:
+        O.set_threads(1)
+    mk = g.dense(O.MK)
+    div_pre = _div(*pre, mk, dx)
+    for pre_cond in preconds:
+        ctx = G.Context(n, n, n, dx)
+        try:
+            ctx.scene(kind, seed)
+            ctx.mark_cells()
+            ctx.p2g(gph, 1.0)
+            assert np.array_equal(ctx.field(G.FIELD_MARKER), mk)
+            # FLUID = holds a particle and is not a wall cell (the seeded jitter can round a
+            # particle onto the wall layer in FP32: it is counted but stays SOLID)
+            assert np.array_equal((ctx.bin_counts() > 0) & (mk != 2), mk == 1)
+            for f in range(6):
+                assert np.array_equal(ctx.active_blocks(f), np.sort(saved.active_blocks(f))), f
+                assert rel_err(ctx.field(f), saved.dense(f)) < 1e-5, f
+            for ax in range(3):
+                assert rel_err(ctx.face_phi(gph, ax), _phi(saved.dense(O.MU + ax), oph)) < 1e-5, ax
+            ctx.save_velocity()
+            ctx.apply_gravity(dt, -9.81)
+            ctx.apply_walls()
+            for f in range(3):
+                assert rel_err(ctx.field(f), pre[f]) < 1e-5, f
+            st = ctx.project(gph, dt, _solve(pre_cond, 1e-9))
+            _log(f"GPU solve precond {pre_cond}: {st.iterations} it, resid {st.true_rel_residual:.2e}")
+            assert st.converged == 1 and st.true_rel_residual <= 1e-9 and st.fallback_used == 0
+            assert st.fluid_cells == ost.fluid_cells and st.pinned == ost.pinned
+            assert np.array_equal(ctx.active_blocks(G.FIELD_P), np.sort(g.active_blocks(O.P)))
+            assert rel_err(ctx.field(G.FIELD_P), g.dense(O.P)) < 1e-6
+            # after the projection u - (dt/dx) beta (p_r - p_l) mostly cancels; its FP32
+            # store carries the rounding of the larger terms: 3e-5 norm-wise
+            for f in range(3):
+                assert rel_err(ctx.field(f), g.dense(f)) < 3e-5, f
+            dg = _div(*(ctx.field(f).astype(np.float64) for f in range(3)), mk, dx)
+            do = _div(g.dense(O.U), g.dense(O.V), g.dense(O.W), mk, dx)
+            assert np.linalg.norm(dg - do) <= 1e-5 * np.linalg.norm(div_pre)
+            ctx.g2p(0.95, 0.95)
+            es = ctx.advect(dt, 2)
+            assert [es.clamped_liquid, es.clamped_gas, es.y_lo, es.y_hi] == \
+                [oes.clamped_liquid, oes.clamped_gas, oes.y_lo, oes.y_hi]
+            gx, gy, gz, gu, gv, gw = ctx.download()
+            assert max(np.abs(gx - ps.x).max(), np.abs(gy - ps.y).max(), np.abs(gz - ps.z).max()) < 1e-6
+            assert rel_err(np.concatenate([gu, gv, gw]), np.concatenate([ps.u, ps.v, ps.w])) < 1e-4
+            assert np.abs(gy - pre_ps.y).max() > 0  # the step moved particles
+        finally:
+            ctx.close()
+
+
+def test_c0_rt64_first_step_matches_fp64_oracle():
+    _staged_first_step(O.SCENE_RT, 64, 1e-3, 20250601, (G.PRECOND_JACOBI, G.PRECOND_MULTIGRID))
+
+
+def test_c0_rt64_100_steps_follow_fp64_trajectory():
+    gold = np.load(os.path.join(GOLDEN, "c0_rt64_trajectory.npz"))
+    n, seed, dt = int(gold["n"]), int(gold["seed"]), float(gold["dt"])
+    summary, snaps, snap_steps, stride = gold["summary"], gold["snaps"], list(gold["snap_steps"]), int(gold["stride"])
+    ph = phase()
+    sim = G.SparseSim3DTP(n, n, n, 1.0 / n)
+    try:
+        sim.ctx.scene(O.SCENE_RT, seed)
+        assert sim.ctx.count() == int(gold["np"])
+        sim.phase = _gphase(ph)
+        sim.dt = dt
+        sim.stop_rule, sim.cg_tol, sim.cg_iters = G.STOP_REL2, 1e-6, 100000
+        sim.precond, sim.inner_rel_tol, sim.check_every = G.PRECOND_MULTIGRID, 0.01, 4
+        x0, y0, z0, *_ = sim.ctx.download()
+        _, _, _, t0 = O.scene(O.SCENE_RT, n, n, n, 1.0 / n, seed)
+        heavy, light = t0 == 0, t0 == 1
+        worst = {}
+        for s in range(1, 101):
+            st = sim.step()
+            assert st.solve.converged == 1 and st.solve.true_rel_residual <= 1e-6, s
+            assert st.solve.fallback_used == 0
+            if s in (1, 2, 5, 10, 20, 50, 100) or s in snap_steps:
+                x, y, z, u, v, w = sim.ctx.download()
+                assert len(x) == len(x0)
+                assert np.isfinite(x).all() and np.isfinite(y).all() and np.isfinite(z).all()
+                sp2 = u.astype(np.float64) ** 2 + v.astype(np.float64) ** 2 + w.astype(np.float64) ** 2
+                row = summary[s - 1]
+                got = [y[heavy].mean(), y[light].mean(), sp2[heavy].sum(), sp2[light].sum(), np.sqrt(sp2.max())]
+                err = [abs(got[0] - row[0]), abs(got[1] - row[1]), abs(got[2] - row[2]) / row[2],
+                       abs(got[3] - row[3]) / row[3], abs(got[4] - row[4]) / row[4]]
+                worst[s] = err
+                _log(f"step {s}: mean-y err {err[0]:.2e} {err[1]:.2e}, KE rel err {err[2]:.2e} {err[3]:.2e}, "
+                     f"vmax rel err {err[4]:.2e}")
+                assert err[0] < 1e-6 and err[1] < 1e-6
+                assert err[2] < 1e-3 and err[3] < 1e-3 and err[4] < 1e-3
+                if s in snap_steps:
+                    ref = snaps[snap_steps.index(s)].astype(np.float64)
+                    mine = np.stack([a[::stride] for a in (x, y, z, u, v, w)]).astype(np.float64)
+                    pos_err = np.abs(mine[:3] - ref[:3]).max()
+                    vel_err = rel_err(mine[3:], ref[3:])
+                    _log(f"step {s}: sampled particles: position err {pos_err:.2e}, velocity rel err {vel_err:.2e}")
+                    assert pos_err < 1e-5 and vel_err < 1e-3
+        # the heavy phase sinks, the light one rises (tests/test_sparse_sim3d_tp.cpp:110-148)
+        assert summary[-1][0] < summary[0][0] and y[heavy].mean() < y0[heavy].mean()
+    finally:
+        sim.ctx.close()
+
+
+def test_c1_bubble128_first_step_matches_fp64_oracle():
+    _staged_first_step(O.SCENE_BUBBLE, 128, 5e-4, 20250601, (G.PRECOND_MULTIGRID,))
+
+
+def test_c1_bubble128_24_steps_rise_and_headspace_stays_sparse():
+    n = 128
+    ph = phase()
+    sim = G.SparseSim3DTP(n, n, n, 1.0 / n)
+    try:
+        sim.ctx.scene(O.SCENE_BUBBLE, 20250601)
+        n0 = sim.ctx.count()
+        _, _, _, t = O.scene(O.SCENE_BUBBLE, n, n, n, 1.0 / n, 20250601)
+        gas = t == 1
+        x, y, z, *_ = sim.ctx.download()
+        gas0 = y[gas].mean()
+        sim.phase = _gphase(ph)
+        sim.dt = 5e-4
+        sim.stop_rule, sim.cg_tol, sim.cg_iters = G.STOP_REL2, 1e-6, 100000
+        sim.precond, sim.inner_rel_tol, sim.check_every = G.PRECOND_MULTIGRID, 0.01, 4
+        total_blocks = ((n + 3) // 4) ** 3
+        max_active = 0
+        for s in range(24):
+            st = sim.step()
+            assert st.solve.converged == 1 and st.solve.true_rel_residual <= 1e-6 and st.solve.fallback_used == 0
+            max_active = max(max_active, len(sim.ctx.active_blocks(G.FIELD_MARKER)))
+        x, y, z, *_ = sim.ctx.download()
+        assert len(x) == n0
+        assert np.isfinite(x).all() and np.isfinite(y).all() and np.isfinite(z).all()
+        rise = (y[gas].mean() - gas0) / sim.dx
+        _log(f"bubble 128^3: gas rose {rise:.3f} cells in 24 steps; max active cell blocks {max_active}/{total_blocks}")
+        assert rise > 0.05
+        assert 0 < max_active < total_blocks
+    finally:
+        sim.ctx.close()
+
+
+@pytest.mark.parametrize("model", ["two_phase_liquid_only", "single_phase"])
+def test_c2_dambreak256_200_cfl_steps(model):
+    n, cap = 256, 5e-3
+    dx = 1.0 / n
+    if model == "single_phase":
+        sim = G.SparseSim3D(n, n, n, dx)
+        sim.alpha = 0.95
+    else:
+        sim = G.SparseSim3DTP(n, n, n, dx)
+        ph = phase()
+        sim.phase = _gphase(ph)
+        sim.interface_diagnostics = False
+    try:
+        sim.ctx.scene(O.SCENE_DAMBREAK, 20250601)
+        n0 = sim.ctx.count()
+        assert n0 == 8 * 101 * 153 * 254  # filled interior cells: (i + 0.5)/n < 0.4, (j + 0.5)/n < 0.6
+        x, *_ = sim.ctx.download()
+        front0 = x.max()
+        sim.dt = cap
+        sim.adaptive_timestep, sim.adaptive_cfl, sim.adaptive_min_dt = True, 1.0, 1e-5
+        sim.stop_rule, sim.cg_tol, sim.cg_iters = G.STOP_REL2, 1e-6, 100000
+        sim.precond, sim.inner_rel_tol, sim.check_every = G.PRECOND_MULTIGRID, 0.01, 4
+        total_blocks = (n // 4) ** 3
+        t_sim, limited, max_fluid = 0.0, 0, 0
+        t0 = time.time()
+        for s in range(200):
+            st = sim.step()
+            assert st.solve.converged == 1 and st.solve.true_rel_residual <= 1e-6, s
+            assert st.effective_dt <= cap
+            if st.max_particle_speed > 0:
+                assert st.effective_dt <= max(1e-5, 1.0 * dx / st.max_particle_speed) * (1 + 1e-12)
+            limited += st.limited
+            t_sim += st.effective_dt
+            max_fluid = max(max_fluid, st.solve.fluid_cells)
+            if (s + 1) % 50 == 0:
+                _log(f"dam-break {model}: step {s + 1}, t = {t_sim:.4f} s, dt {st.effective_dt:.2e}, "
+                     f"vmax {st.max_particle_speed:.2f}, fluid cells {st.solve.fluid_cells}, "
+                     f"{time.time() - t0:.1f} s wall")
+        active = len(sim.ctx.active_blocks(G.FIELD_MARKER))
+        x, y, z, *_ = sim.ctx.download()
+        assert len(x) == n0
+        assert np.isfinite(x).all() and np.isfinite(y).all() and np.isfinite(z).all()
+        assert limited > 0  # the CFL limit took over from the requested cap
+        assert x.max() > front0 + 10 * dx  # the front ran out along x
+        assert active < total_blocks and max_fluid < n ** 3 // 2  # the grid stays sparse
+        _log(f"dam-break {model}: 200 steps, t = {t_sim:.4f} s, {limited} CFL-limited, front "
+             f"{front0:.3f} -> {x.max():.3f}, active cell blocks {active}/{total_blocks}")
+    finally:
+        sim.ctx.close()
+
+
+@pytest.mark.parametrize("precond", [G.PRECOND_MULTIGRID, G.PRECOND_JACOBI])
+def test_c3_rt256_steps_and_p2g_identities(precond):
+    n = 256
+    ph = phase()
+    gph = _gphase(ph)
+    ctx = G.Context(n, n, n, 1.0 / n)
+    try:
+        ctx.scene(O.SCENE_RT, 20250601)
+        assert ctx.count() == 131096512
+        cfg = G.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95, adaptive_cfl=0.5,
+                        adaptive_min_dt=1e-5, c_div=0.0, adaptive_timestep=0, advection_order=2, timing=0,
+                        phase=gph, solve=_solve(precond, 1e-6))
+        for s in range(2 if precond == G.PRECOND_MULTIGRID else 1):
+            st = ctx.step(cfg)
+            _log(f"RT 256^3 precond {precond} step {s + 1}: {st.solve.iterations} it, "
+                 f"resid {st.solve.true_rel_residual:.2e}")
+            assert st.solve.converged == 1 and st.solve.true_rel_residual <= 1e-6 and st.solve.fallback_used == 0
+            assert st.solve.fluid_cells == 254 ** 3 and st.solve.pinned == 1
+        x, y, z, u, v, w = ctx.download()
+        assert np.isfinite(x).all() and np.isfinite(u).all()
+        _, _, _, t = O.scene(O.SCENE_RT, n, n, n, 1.0 / n, 20250601)
+        # P2G of the stepped state: face masses and momenta sum to the particle totals
+        ctx.mark_cells()
+        ctx.p2g(gph, 1.0)
+        mp = np.where(t == 0, ph.rho_l, ph.rho_g)
+        total_m = float(mp.sum())
+        for ax, vel in enumerate((u, v, w)):
+            M = ctx.field(G.FIELD_MU + ax).astype(np.float64)
+            U = ctx.field(G.FIELD_U + ax).astype(np.float64)
+            assert abs(M.sum() - total_m) <= 1e-5 * total_m, ax
+            mom_p = float(np.dot(mp, vel.astype(np.float64)))
+            scale = float(np.dot(mp, np.abs(vel.astype(np.float64))))
+            assert abs((U * M).sum() - mom_p) <= 1e-5 * scale, ax
+    finally:
+        ctx.close()

@@ -6,6 +6,8 @@ error of the FP64 oracle; pressure solved to the FP64 residual asked for;
 post-projection divergence at the oracle's level; particle state after a step
 within the tolerances stated per test.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -179,25 +181,37 @@ def test_step_is_deterministic_and_order_independent():
         assert np.abs(p - q).max() < 1e-5 * max(1.0, np.abs(p).max())
 
 
-def test_reference_scene_steps_match_oracle():
-    # the reference's own RT test scene (tests/test_sparse_sim3d_tp.cpp:110-148:
-    # 8x12x8, dt 0.03, defaults rho_l 1 / rho_g 0.01, 12 steps): the heavy phase sinks,
-    # the particle count is kept, and the GPU follows the float-faithful oracle
-    if not O.ref_available():
-        pytest.skip("oracle/_ref not built")
-    rs = O.RefSim(8, 12, 8, 1.0)
-    ps, oph = rs.init_scene(0)
+def test_reference_scene_steps_follow_the_reference():
+    # the reference's own RT test scene (tests/test_sparse_sim3d_tp.cpp:110-148: 8x12x8,
+    # dt 0.03, defaults rho_l 1 / rho_g 0.01, 12 steps) from the stored output of the
+    # reference's SparseSim3DTP (tests/golden/make_ref_rt_scene.py): the GPU follows the
+    # reference's particles, keeps the count and the heavy phase sinks
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_rt_8x12x8.npz"))
+    init, t = gold["init"], gold["type"]
+    rho_l, rho_g, alpha, rt0 = gold["phase"]
     sim = G.SparseSim3DTP(8, 12, 8, 1.0)
-    sim.phase = _gphase(oph)
-    sim.dt = 0.03
-    sim.set_particles(ps.x.astype(np.float32), ps.y.astype(np.float32), ps.z.astype(np.float32), type=ps.type)
-    heavy0 = ps.y[ps.type == 0].mean()
-    for _ in range(12):
+    sim.phase = G.Phase(rho_l, rho_g, alpha, rt0)
+    sim.dt = float(gold["dt"])
+    f32 = [a.astype(np.float32) for a in init]
+    assert all(np.array_equal(a, b) for a, b in zip(f32, init))  # the lattice is exact in FP32
+    sim.set_particles(*f32, type=t)
+    heavy0 = init[1][t == 0].mean()
+    for s in range(1, 13):
         sim.step()
-    x, y, z, *_ = sim.ctx.download()
-    assert len(x) == len(ps)
+        x, y, z, u, v, w = sim.ctx.download()
+        my = [y[t == 0].mean(), y[t == 1].mean()]
+        err = np.abs(np.array(my) - gold["mean_y"][s - 1]).max()
+        print(f"[ref-scene] step {s}: mean-y err {err:.2e}", flush=True)
+        assert err < 1e-3
+        if s in (1, 12):
+            ref = gold["after1" if s == 1 else "after12"]
+            pos = max(np.abs(x - ref[0]).max(), np.abs(y - ref[1]).max(), np.abs(z - ref[2]).max())
+            vel = rel_err(np.concatenate([u, v, w]), np.concatenate(ref[3:]))
+            print(f"[ref-scene] step {s}: position err {pos:.2e}, velocity rel err {vel:.2e}", flush=True)
+            assert pos < (1e-4 if s == 1 else 1e-2) and vel < (1e-3 if s == 1 else 1e-2)
+    assert len(x) == init.shape[1]
     assert np.isfinite(x).all() and np.isfinite(y).all() and np.isfinite(z).all()
-    assert y[ps.type == 0].mean() < heavy0
+    assert y[t == 0].mean() < heavy0
 
 
 @pytest.mark.parametrize("kind,n", [(O.SCENE_RT, 96), (O.SCENE_BUBBLE, 64)])
