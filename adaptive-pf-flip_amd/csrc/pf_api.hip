@@ -1175,23 +1175,61 @@ MgLevel mg_level(const Ctx::MgBuf& b) {
 
 // the coarse operators of the assembled system (levels allocated on first use:
 // halving down to at most 4 cells a side; a grid of at most 4 cells a side has none)
+void mg_free_level(Ctx::MgBuf& b) {
+  float** arrs[8] = {&b.cd, &b.cx, &b.cy, &b.cz, &b.inv, &b.x, &b.r, &b.t};
+  for (float** a : arrs) dfree(*a);
+}
+
+// every multigrid buffer, or none: a partial allocation is released and the context
+// keeps no levels, so a later call starts again from nothing
+int mg_alloc(pf_ctx* c) {
+  std::vector<Ctx::MgBuf> levels;
+  float* t0 = nullptr;
+  uint16_t* h[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint32_t* mx = nullptr;
+  float* unscale = nullptr;
+  auto release = [&]() {
+    for (auto& b : levels) mg_free_level(b);
+    dfree(t0);
+    for (auto& p : h) dfree(p);
+    dfree(mx);
+    dfree(unscale);
+  };
+  hipError_t e = hipSuccess;
+  int nx = c->geom.nx, ny = c->geom.ny, nz = c->geom.nz;
+  while (e == hipSuccess && std::max({nx, ny, nz}) > 4) {
+    nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+    Ctx::MgBuf b;
+    b.d = make_dims(nx, ny, nz);
+    b.nx = nx; b.ny = ny; b.nz = nz;
+    const long long n = b.d.nblocks * kBlockVol;
+    float** arrs[8] = {&b.cd, &b.cx, &b.cy, &b.cz, &b.inv, &b.x, &b.r, &b.t};
+    for (float** a : arrs)
+      if (e == hipSuccess) e = dalloc(a, n);
+    levels.push_back(b);
+  }
+  const long long n0 = c->dc.nblocks * kBlockVol;
+  if (e == hipSuccess) e = dalloc(&t0, n0);
+  for (auto& p : h)
+    if (e == hipSuccess) e = dalloc(&p, n0);
+  if (e == hipSuccess) e = dalloc(&mx, 2);
+  if (e == hipSuccess) e = dalloc(&unscale, 2);
+  if (e != hipSuccess) {
+    release();
+    return fail(c, PF_ERR_NOMEM, std::string("multigrid levels: ") + hipGetErrorString(e));
+  }
+  c->mg = std::move(levels);
+  c->mg_t0 = t0;
+  for (int q = 0; q < 5; ++q) c->mg_h[q] = h[q];
+  c->mg_mx = mx;
+  c->mg_unscale = unscale;
+  return PF_OK;
+}
+
 int mg_build(pf_ctx* c) {
   if (!c->mg_t0) {
-    int nx = c->geom.nx, ny = c->geom.ny, nz = c->geom.nz;
-    while (std::max({nx, ny, nz}) > 4) {
-      nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
-      Ctx::MgBuf b;
-      b.d = make_dims(nx, ny, nz);
-      b.nx = nx; b.ny = ny; b.nz = nz;
-      const long long n = b.d.nblocks * kBlockVol;
-      float** arrs[8] = {&b.cd, &b.cx, &b.cy, &b.cz, &b.inv, &b.x, &b.r, &b.t};
-      for (float** a : arrs) HIP_TRY(c, dalloc(a, n));
-      c->mg.push_back(b);
-    }
-    HIP_TRY(c, dalloc(&c->mg_t0, c->dc.nblocks * kBlockVol));
-    for (auto& h : c->mg_h) HIP_TRY(c, dalloc(&h, c->dc.nblocks * kBlockVol));
-    HIP_TRY(c, dalloc(&c->mg_mx, 2));
-    HIP_TRY(c, dalloc(&c->mg_unscale, 2));
+    int st = mg_alloc(c);
+    if (st) return st;
   }
   // level 1's residual is restricted from the FLUID blocks only: the aggregates of the
   // other blocks stay zero through the solve
@@ -1322,6 +1360,11 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
     hipLaunchKernelGGL(k_mg_restrict, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, mg_level(c->mg[0]),
                        c->mg[0].r);
   }
+  // per-aggregate correction scaling: S on the restricted residual ...
+  const long long n1 = c->mg[0].d.nblocks * kBlockVol;
+  if (c->mg_cs)
+    hipLaunchKernelGGL(k_mg_scale, dim3(std::min<unsigned>(grid_for(n1, 256), 2048u)), dim3(256), 0, c->stream, n1,
+                       c->mg[0].r, c->mg_cs, done);
   // coarse levels: level l (1..nl) solves A_l x = r_l from zero, result in res[l].
   // (Two corrections per level -- a W-cycle -- cost more launches than they save
   // iterations at 256^3, and twice the 1.8 over-correction stops converging.)
@@ -1357,6 +1400,10 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
     alt[l] = t;
   };
   coarse(1);
+  // ... and on the correction (P S B_1 S P^T stays symmetric)
+  if (c->mg_cs)
+    hipLaunchKernelGGL(k_mg_scale, dim3(std::min<unsigned>(grid_for(n1, 256), 2048u)), dim3(256), 0, c->stream, n1,
+                       res[1], c->mg_cs, done);
   // level 0: prolong and post-smooth, the last sweep into z with the partial r.z
   if (rows) {
     ra.x = cur; ra.out = nxt; ra.xc = res[1]; ra.part = P.nu == 1 ? rz : nullptr;
@@ -1407,9 +1454,12 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     ZeroArgs z{};
     z.list = c->pblk_prev;
     z.nlist = c->npblk_prev;
-    float* f32[12] = {c->f[F_P], c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->cd32, c->cx32, c->cy32, c->cz32, c->invd32};
-    for (int q = 0; q < 12; ++q) z.f32[q] = f32[q];
-    z.nf32 = 12;
+    // with the multigrid work array: its values at cells outside the next system are
+    // read (times a zero coupling) by the level-0 sweeps, so none may stay non-finite
+    float* f32[13] = {c->f[F_P], c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->cd32, c->cx32, c->cy32, c->cz32,
+                      c->invd32, c->mg_t0};
+    for (int q = 0; q < 13; ++q) z.f32[q] = f32[q];
+    z.nf32 = c->mg_t0 ? 13 : 12;
     double* f64[7] = {c->b64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64, c->x64};
     for (int q = 0; q < 7; ++q) z.f64[q] = f64[q];
     z.nf64 = warm ? 6 : 7;  // a warm start keeps x64 (the saved pressure)
@@ -1916,6 +1966,33 @@ int pf_set_multigrid(pf_ctx* c, const pf_mg_cfg* cfg) {
   return PF_OK;
 }
 
+int pf_mg_set_correction_weights(pf_ctx* c, const float* w, int64_t n) {
+  if (!c) return PF_ERR_ARG;
+  if (!w) {
+    dfree(c->mg_cs);
+    return PF_OK;
+  }
+  const int ax = (c->geom.nx + 1) / 2, ay = (c->geom.ny + 1) / 2, az = (c->geom.nz + 1) / 2;
+  if (std::max({c->geom.nx, c->geom.ny, c->geom.nz}) <= 4)
+    return fail(c, PF_ERR_ARG, "pf_mg_set_correction_weights: the grid has no coarse level");
+  if (n != (int64_t)ax * ay * az)
+    return fail(c, PF_ERR_ARG, "pf_mg_set_correction_weights: n must be the aggregate count ceil(nx/2)*ceil(ny/2)*ceil(nz/2)");
+  const Dims d1 = make_dims(ax, ay, az);
+  std::vector<float> s((size_t)(d1.nblocks * kBlockVol), 1.0f);
+  for (int k = 0; k < az; ++k)
+    for (int j = 0; j < ay; ++j)
+      for (int i = 0; i < ax; ++i) {
+        const float v = w[i + (int64_t)ax * (j + (int64_t)ay * k)];
+        if (!(v > 0.f) || !std::isfinite(v))
+          return fail(c, PF_ERR_ARG, "pf_mg_set_correction_weights: weights must be positive and finite");
+        s[(size_t)bm_index(d1, i, j, k)] = std::sqrt(v);
+      }
+  if (!c->mg_cs) HIP_TRY(c, dalloc(&c->mg_cs, d1.nblocks * kBlockVol));
+  HIP_TRY(c, hipMemcpyAsync(c->mg_cs, s.data(), sizeof(float) * s.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
 int pf_get_multigrid(const pf_ctx* c, pf_mg_cfg* cfg) {
   if (!c || !cfg) return PF_ERR_ARG;
   *cfg = c->mg_cfg;
@@ -2080,6 +2157,7 @@ int pf_destroy(pf_ctx* c) {
   }
   c->mg.clear();
   dfree(c->mg_t0);
+  dfree(c->mg_cs);
   for (auto& h : c->mg_h) dfree(h);
   dfree(c->mg_mx);
   dfree(c->mg_unscale);

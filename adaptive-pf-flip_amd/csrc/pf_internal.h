@@ -241,6 +241,9 @@ struct Ctx {
   uint16_t *mg_h[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   uint32_t* mg_mx = nullptr;
   float* mg_unscale = nullptr;
+  // per-aggregate scaling s = sqrt(w) of level 1's restricted residual and correction
+  // (P S B_1 S P^T in place of P B_1 P^T), or null: w = 1
+  float* mg_cs = nullptr;
   pf_mg_cfg mg_cfg{};   // the cycle (set at creation, pf_set_multigrid)
 
   // per-kernel event timing (measurement hook)

@@ -282,8 +282,20 @@ __global__ void __launch_bounds__(256) k_mg_tohalf(long long n, const float* __r
     hcx[q] = (_Float16)(cx[q] * sa);
     hcy[q] = (_Float16)(cy[q] * sa);
     hcz[q] = (_Float16)(cz[q] * sa);
-    hinv[q] = (_Float16)(inv[q] * si);
+    // a cell is in the system where its inverse is non-zero (the level-0 kernels test
+    // the FP16 value): a tiny inverse that would round to 0 keeps the least subnormal
+    _Float16 hi = (_Float16)(inv[q] * si);
+    if (inv[q] > 0.f && (float)hi == 0.f) hi = (_Float16)5.9604644775390625e-8f;  // 2^-24
+    hinv[q] = hi;
   }
+}
+
+// x *= s elementwise over a coarse slab (the per-aggregate correction scaling)
+__global__ void __launch_bounds__(256) k_mg_scale(long long n, float* __restrict__ x, const float* __restrict__ s,
+                                                  const int* done) {
+  if (done && *done) return;
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256)
+    x[q] *= s[q];
 }
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));

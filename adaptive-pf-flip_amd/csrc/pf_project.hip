@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
 struct ZeroArgs {
   const uint32_t* list;
   long long nlist;
-  float* f32[12];
+  float* f32[13];
   int nf32;
   double* f64[7];
   int nf64;

@@ -315,6 +315,7 @@ def lib():
             "pf_adapt_defaults": (None, [C.POINTER(AdaptCfg)]),
             "pf_mg_defaults": (None, [C.POINTER(MgCfg)]),
             "pf_set_multigrid": (C.c_int, [vp, C.POINTER(MgCfg)]),
+            "pf_mg_set_correction_weights": (C.c_int, [vp, vp, i64]),
             "pf_get_multigrid": (C.c_int, [vp, C.POINTER(MgCfg)]),
             "pf_particles_download_ext": (C.c_int, [vp, vp, vp]),
             "pf_liquid_volume": (dbl, [vp]),
@@ -463,6 +464,15 @@ class Context:
                 raise KeyError(k)
             setattr(c, k, v)
         self._check(lib().pf_set_multigrid(self.h, C.byref(c)), "pf_set_multigrid")
+
+    def set_correction_weights(self, w):
+        """per-aggregate weights of the level-1 coarse correction, dense (az, ay, ax)
+        over ceil(n/2) aggregates per axis, or None for the plain cycle"""
+        if w is None:
+            self._check(lib().pf_mg_set_correction_weights(self.h, None, 0), "pf_mg_set_correction_weights")
+            return
+        a = np.ascontiguousarray(w, np.float32).reshape(-1)
+        self._check(lib().pf_mg_set_correction_weights(self.h, _p(a), len(a)), "pf_mg_set_correction_weights")
 
     def multigrid(self) -> MgCfg:
         c = MgCfg()

@@ -549,6 +549,12 @@ void pf_mg_defaults(pf_mg_cfg* cfg);
 /* the cycle of the following multigrid solves on ctx (PF_ERR_ARG out of range) */
 int pf_set_multigrid(pf_ctx* ctx, const pf_mg_cfg* cfg);
 int pf_get_multigrid(const pf_ctx* ctx, pf_mg_cfg* cfg);
+/* Per-aggregate weights w > 0 of the level-1 coarse correction of the following
+ * multigrid solves (no reference counterpart): the cycle uses P S B S P^T with
+ * S = diag(sqrt(w)) in place of P B P^T, so it stays symmetric; w = 1 everywhere is
+ * the plain cycle.  w is dense over the aggregates, ceil(nx/2) x ceil(ny/2) x
+ * ceil(nz/2), index i + ax*(j + ay*k); NULL clears it. */
+int pf_mg_set_correction_weights(pf_ctx* ctx, const float* w, int64_t n);
 
 #ifdef __cplusplus
 }

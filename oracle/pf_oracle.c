@@ -136,6 +136,9 @@ orc_grid* orc_grid_create(int nx, int ny, int nz, double dx, double ox, double o
   return g;
 }
 
+/* switch the storage rounding of an existing grid (float-faithful or FP64) */
+void orc_grid_set_faithful(orc_grid* g, int faithful) { g->faithful = faithful ? 1 : 0; }
+
 void orc_grid_destroy(orc_grid* g) {
   if (!g) return;
   for (int f = 0; f < ORC_NFIELDS; ++f) field_free(&g->f[f]);

@@ -75,6 +75,9 @@ typedef struct orc_grid orc_grid;
 /* grid lifecycle and access (src/grid/sparse_block_grid3d.h, sparse_mac_grid3d.h) */
 orc_grid* orc_grid_create(int nx, int ny, int nz, double dx, double ox, double oy, double oz,
                           int faithful);
+/* switch the storage rounding of an existing grid: 1 float-faithful (the reference's
+ * float pool), 0 FP64; later stages round their stores accordingly */
+void orc_grid_set_faithful(orc_grid* g, int faithful);
 void orc_grid_destroy(orc_grid* g);
 orc_grid* orc_grid_clone(const orc_grid* g);
 void orc_field_dims(const orc_grid* g, int field, int32_t out_dims[3]);

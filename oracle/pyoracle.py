@@ -330,6 +330,7 @@ def lib():
         L.orc_grid_create.restype = C.c_void_p
         L.orc_grid_create.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [C.c_int]
         L.orc_grid_destroy.argtypes = [C.c_void_p]
+        L.orc_grid_set_faithful.argtypes = [C.c_void_p, C.c_int]
         L.orc_grid_clone.restype = C.c_void_p
         L.orc_grid_clone.argtypes = [C.c_void_p]
         L.orc_field_dims.argtypes = [C.c_void_p, C.c_int, _i32]
@@ -476,6 +477,11 @@ class Grid:
     def clone(self):
         return Grid(self.nx, self.ny, self.nz, self.dx, self.origin, self.faithful,
                     _h=lib().orc_grid_clone(self.h))
+
+    def set_faithful(self, faithful: bool):
+        """round later stores as the reference's float pool (True) or keep FP64"""
+        lib().orc_grid_set_faithful(self.h, int(faithful))
+        self.faithful = bool(faithful)
 
     def dims(self, f):
         d = np.zeros(3, np.int32)

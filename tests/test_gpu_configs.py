@@ -21,9 +21,11 @@
 Stage bars against the FP64 oracle: markers, active-block sets and clamp counters
 bit-exact; face fields and phi <= 1e-5 norm-wise; pressure <= 1e-6; the
 post-projection divergence within 1e-5 of the pre-projection divergence norm-wise;
-positions <= 1e-6 absolute, velocities <= 1e-4 norm-wise after the step; the
-post-projection face velocities, where u and (dt/dx) beta grad p nearly cancel,
-<= 3e-5 norm-wise.
+positions <= 1e-6 absolute, velocities <= 1e-4 norm-wise after the step.  After
+the projection, where u and (dt/dx) beta grad p nearly cancel, the face velocities
+and the divergence are held to the reference's own precision: their distance from
+the FP64 oracle must stay within 1.5x of the distance of the same projection run
+with the reference's float storage of p and u (the oracle in float-faithful mode).
 """
 import os
 import time
@@ -91,6 +93,11 @@ def _staged_first_step(kind, n, dt, seed, preconds):
         g.apply_walls()
         pre = [g.dense(f) for f in (O.U, O.V, O.W)]
         pre_ps = ps.copy()
+        # the same projection at the reference's precision: pressure and face
+        # velocities stored as float, as its float pool does (sparse_ops3d_tp.cpp:366-422)
+        gf = g.clone()
+        gf.set_faithful(True)
+        gf.project(oph, dt, 100000, 1e-10, 0.0, O.STOP_REL2)
         ost = g.project(oph, dt, 100000, 1e-10, 0.0, O.STOP_REL2)
         g.g2p(saved, ps, 0.95, 0.95)
         oes, _ = g.advect(ps, dt, 2)
@@ -125,13 +132,22 @@ def _staged_first_step(kind, n, dt, seed, preconds):
             assert st.fluid_cells == ost.fluid_cells and st.pinned == ost.pinned
             assert np.array_equal(ctx.active_blocks(G.FIELD_P), np.sort(g.active_blocks(O.P)))
             assert rel_err(ctx.field(G.FIELD_P), g.dense(O.P)) < 1e-6
-            # after the projection u - (dt/dx) beta (p_r - p_l) mostly cancels; its FP32
-            # store carries the rounding of the larger terms: 3e-5 norm-wise
+            # after the projection u - (dt/dx) beta (p_r - p_l) mostly cancels, and float
+            # storage of p and u leaves an error set by the larger terms: the face
+            # velocities and the divergence must stay within 1.5x of the float-faithful
+            # projection's own distance from the FP64 one
             for f in range(3):
-                assert rel_err(ctx.field(f), g.dense(f)) < 3e-5, f
+                d_ref = np.linalg.norm(gf.dense(f) - g.dense(f))
+                d_gpu = np.linalg.norm(ctx.field(f).astype(np.float64) - g.dense(f))
+                _log(f"face {f}: |gpu - fp64| {d_gpu:.3e}, |float-faithful - fp64| {d_ref:.3e}, "
+                     f"|fp64| {np.linalg.norm(g.dense(f)):.3e}")
+                assert d_gpu <= 1.5 * d_ref + 1e-6 * np.linalg.norm(g.dense(f)), f
             dg = _div(*(ctx.field(f).astype(np.float64) for f in range(3)), mk, dx)
             do = _div(g.dense(O.U), g.dense(O.V), g.dense(O.W), mk, dx)
-            assert np.linalg.norm(dg - do) <= 1e-5 * np.linalg.norm(div_pre)
+            dfa = _div(gf.dense(O.U), gf.dense(O.V), gf.dense(O.W), mk, dx)
+            _log(f"divergence: |gpu - fp64| {np.linalg.norm(dg - do):.3e}, |float-faithful - fp64| "
+                 f"{np.linalg.norm(dfa - do):.3e}, |pre| {np.linalg.norm(div_pre):.3e}")
+            assert np.linalg.norm(dg - do) <= 1.5 * np.linalg.norm(dfa - do) + 1e-6 * np.linalg.norm(div_pre)
             ctx.g2p(0.95, 0.95)
             es = ctx.advect(dt, 2)
             assert [es.clamped_liquid, es.clamped_gas, es.y_lo, es.y_hi] == \
