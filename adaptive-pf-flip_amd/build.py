@@ -33,10 +33,14 @@ HIP_FLAGS = [
 ]
 
 
+# translation units of the library (pf_api.hip includes the kernel files it launches)
+LIB_SOURCES = ["pf_api.hip", "pf_noalloc.hip"]
+
+
 def build_lib(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "pf_api.hip"), "-o", tmp]
+    cmd = [HIPCC] + HIP_FLAGS + [os.path.join(CSRC, f) for f in LIB_SOURCES] + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -276,6 +276,16 @@ void pf_adapt_defaults(pf_adapt_cfg* cfg);
 /* computeAdaptiveParticleTimestep3D, src/driver/timestep3d.h:16-50 */
 int pf_timestep(pf_ctx* ctx, double requested_dt, int32_t enabled, double cfl, double min_dt,
                 double* effective_dt, double* max_speed, double* cfl_limit_dt, int32_t* limited);
+/* pf_timestep without a context and without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  u, v, w are caller-owned device
+ * arrays of n velocities; the reduction runs on `stream` (a hipStream_t, NULL = the
+ * default stream) through a fixed device scratch array of 512 KiB per device, so n is
+ * at most 268,435,456 (PF_ERR_ARG beyond).  The outputs are pf_timestep's.  Calls
+ * that overlap on one device share that array and race. */
+int pf_timestep_noalloc(const float* u, const float* v, const float* w, int64_t n, double dx,
+                        double requested_dt, int32_t enabled, double cfl, double min_dt, void* stream,
+                        double* eff, double* vmax, double* cfl_dt, int32_t* limited);
 /* SparseSim3DTP::step, sparse_sim3d_tp.cpp:854-884 (no particle adaptivity or
  * secondary particles; interface diagnostics and optional surface tension as the
  * reference), with G2P and advection fused;
