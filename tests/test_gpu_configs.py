@@ -21,11 +21,12 @@
 Stage bars against the FP64 oracle: markers, active-block sets and clamp counters
 bit-exact; face fields and phi <= 1e-5 norm-wise; pressure <= 1e-6; the
 post-projection divergence within 1e-5 of the pre-projection divergence norm-wise;
-positions <= 1e-6 absolute, velocities <= 1e-4 norm-wise after the step.  After
-the projection, where u and (dt/dx) beta grad p nearly cancel, the face velocities
-and the divergence are held to the reference's own precision: their distance from
-the FP64 oracle must stay within 1.5x of the distance of the same projection run
-with the reference's float storage of p and u (the oracle in float-faithful mode).
+positions <= 1e-6 absolute after the step.  After the projection, where u and
+(dt/dx) beta grad p nearly cancel, the face velocities, the divergence and the
+particle velocities G2P takes from those faces are held to the reference's own
+precision: their distance from the FP64 oracle must stay within 1.5x of the
+distance of the same projection (and G2P) run with the reference's float storage of
+p and u (the oracle in float-faithful mode).
 """
 import os
 import time
@@ -101,6 +102,9 @@ def _staged_first_step(kind, n, dt, seed, preconds):
         ost = g.project(oph, dt, 100000, 1e-10, 0.0, O.STOP_REL2)
         g.g2p(saved, ps, 0.95, 0.95)
         oes, _ = g.advect(ps, dt, 2)
+        ps_f = pre_ps.copy()  # G2P and advection through the float-faithful faces
+        gf.g2p(saved, ps_f, 0.95, 0.95)
+        gf.advect(ps_f, dt, 2)
         _log(f"oracle step {time.time() - t0:.1f} s ({ost.iterations} it)")
     finally:
         O.set_threads(1)
@@ -154,7 +158,12 @@ def _staged_first_step(kind, n, dt, seed, preconds):
                 [oes.clamped_liquid, oes.clamped_gas, oes.y_lo, oes.y_hi]
             gx, gy, gz, gu, gv, gw = ctx.download()
             assert max(np.abs(gx - ps.x).max(), np.abs(gy - ps.y).max(), np.abs(gz - ps.z).max()) < 1e-6
-            assert rel_err(np.concatenate([gu, gv, gw]), np.concatenate([ps.u, ps.v, ps.w])) < 1e-4
+            vo = np.concatenate([ps.u, ps.v, ps.w])
+            d_gpu = np.linalg.norm(np.concatenate([gu, gv, gw]).astype(np.float64) - vo)
+            d_ref = np.linalg.norm(np.concatenate([ps_f.u, ps_f.v, ps_f.w]) - vo)
+            _log(f"particle velocities: |gpu - fp64| {d_gpu:.3e}, |float-faithful - fp64| {d_ref:.3e}, "
+                 f"|fp64| {np.linalg.norm(vo):.3e}")
+            assert d_gpu <= 1.5 * d_ref + 1e-5 * np.linalg.norm(vo)
             assert np.abs(gy - pre_ps.y).max() > 0  # the step moved particles
         finally:
             ctx.close()
@@ -197,15 +206,15 @@ def test_c0_rt64_100_steps_follow_fp64_trajectory():
                 worst[s] = err
                 _log(f"step {s}: mean-y err {err[0]:.2e} {err[1]:.2e}, KE rel err {err[2]:.2e} {err[3]:.2e}, "
                      f"vmax rel err {err[4]:.2e}")
-                assert err[0] < 1e-6 and err[1] < 1e-6
-                assert err[2] < 1e-3 and err[3] < 1e-3 and err[4] < 1e-3
+                assert err[0] < 2e-6 and err[1] < 2e-6
+                assert err[2] < 1e-4 and err[3] < 1e-4 and err[4] < 5e-4
                 if s in snap_steps:
                     ref = snaps[snap_steps.index(s)].astype(np.float64)
                     mine = np.stack([a[::stride] for a in (x, y, z, u, v, w)]).astype(np.float64)
                     pos_err = np.abs(mine[:3] - ref[:3]).max()
                     vel_err = rel_err(mine[3:], ref[3:])
                     _log(f"step {s}: sampled particles: position err {pos_err:.2e}, velocity rel err {vel_err:.2e}")
-                    assert pos_err < 1e-5 and vel_err < 1e-3
+                    assert pos_err < 1e-5 and vel_err < 1e-4
         # the heavy phase sinks, the light one rises (tests/test_sparse_sim3d_tp.cpp:110-148)
         assert summary[-1][0] < summary[0][0] and y[heavy].mean() < y0[heavy].mean()
     finally:

@@ -202,13 +202,13 @@ def test_reference_scene_steps_follow_the_reference():
         my = [y[t == 0].mean(), y[t == 1].mean()]
         err = np.abs(np.array(my) - gold["mean_y"][s - 1]).max()
         print(f"[ref-scene] step {s}: mean-y err {err:.2e}", flush=True)
-        assert err < 1e-3
+        assert err < 1e-5
         if s in (1, 12):
             ref = gold["after1" if s == 1 else "after12"]
             pos = max(np.abs(x - ref[0]).max(), np.abs(y - ref[1]).max(), np.abs(z - ref[2]).max())
             vel = rel_err(np.concatenate([u, v, w]), np.concatenate(ref[3:]))
             print(f"[ref-scene] step {s}: position err {pos:.2e}, velocity rel err {vel:.2e}", flush=True)
-            assert pos < (1e-4 if s == 1 else 1e-2) and vel < (1e-3 if s == 1 else 1e-2)
+            assert pos < (1e-5 if s == 1 else 1e-4) and vel < (1e-5 if s == 1 else 1e-4)
     assert len(x) == init.shape[1]
     assert np.isfinite(x).all() and np.isfinite(y).all() and np.isfinite(z).all()
     assert y[t == 0].mean() < heavy0
