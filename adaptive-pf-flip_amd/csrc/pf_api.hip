@@ -31,6 +31,7 @@
 #include "pf_adapt.hip"
 #include "pf_render.hip"
 #include "pf_mg.hip"
+#include "pf_lpc_cw.hip"
 
 using namespace pf;
 
@@ -208,6 +209,7 @@ int pf_destroy(pf_ctx* c) {
   c->mg.clear();
   dfree(c->mg_t0);
   dfree(c->mg_cs);
+  dfree(c->mg_cs_net);
   for (auto& h : c->mg_h) dfree(h);
   dfree(c->mg_mx);
   dfree(c->mg_unscale);

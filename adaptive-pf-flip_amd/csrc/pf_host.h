@@ -22,6 +22,9 @@ struct pf_lpc {
   float* part = nullptr;
   long long part_cap = 0;
   float *zbuf = nullptr, *qbuf = nullptr, *gz = nullptr;
+  // level-1 training vectors: restricted residual, coarse solution, dL/dg, B S dL/dg, dL/ds
+  float* l1[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  long long l1_cap = 0;
   long long cell_cap = 0;
   float* stage = nullptr;
   long long stage_cap = 0;
@@ -165,8 +168,8 @@ void record(pf_ctx* c, bool timing, int slot) {
 
 
 // defined in pf_host_lpc.hip; the solver couples to them
-int lpc_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part, int slot);
-int lpc_build_tiles(pf_ctx* c);
+int lpc_cw_tiles(pf_ctx* c, const uint32_t* list, long long nlist);
+int lpc_cw_weights(pf_ctx* c, bool store);
 int lpc_features_alloc(pf_ctx* c);
 
 }  // namespace

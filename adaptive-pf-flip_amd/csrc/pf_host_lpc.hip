@@ -92,36 +92,18 @@ int lpc_backward(pf_lpc* L, hipStream_t s, long long nt) {
   return st;
 }
 
-LpcTiles lpc_tiles_args(pf_ctx* c, const float* r) {
-  LpcTiles T{};
-  T.dc = c->dc; T.g = c->geom; T.ntx = c->ntx; T.nty = c->nty; T.ntz = c->ntz;
-  T.tiles = c->tiles; T.ntiles = c->ntiles;
-  T.marker = c->marker; T.r = r; T.invd = c->invd32; T.fbeta = c->fbeta; T.fphi = c->fphi;
-  T.scale = c->lpc->scale; T.X = c->lpc->X;
-  return T;
+// level-1 extents (the 2x2x2 aggregates of the multigrid cycle)
+void lpc_level1_extents(const pf_ctx* c, int& nx1, int& ny1, int& nz1) {
+  nx1 = (c->geom.nx + 1) / 2;
+  ny1 = (c->geom.ny + 1) / 2;
+  nz1 = (c->geom.nz + 1) / 2;
 }
 
-unsigned lpc_scatter_grid(long long nt) { return (unsigned)std::max<long long>(1, std::min<long long>(nt, 1024)); }
-
-// z = M(r) on the context's stream for the last assembled system; partial r.z in
-// part[slot] (gridDim = lpc_scatter_grid); skipped while *done
-int lpc_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part, int slot) {
-  pf_lpc* L = c->lpc;
-  int st = lpc_grow(L, c->ntiles, false);
-  if (st) return fail(c, st, "learned preconditioner: " + L->err);
-  LpcTiles T = lpc_tiles_args(c, r);
-  T.done = done;
-  hipLaunchKernelGGL(k_lpc_gather, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, T);
-  st = lpc_forward(L, c->stream, c->ntiles, false, done);
-  if (st) return fail(c, st, "learned preconditioner: " + L->err);
-  hipLaunchKernelGGL(k_lpc_scatter, dim3(lpc_scatter_grid(c->ntiles)), dim3(256), 0, c->stream, T, L->o, z, part, slot);
-  LAUNCH_CHECK(c);
-  return PF_OK;
-}
-
-// 16^3 tiles holding the current fluid-block list (c->pblk, npblk blocks)
-int lpc_build_tiles(pf_ctx* c) {
-  c->ntx = (c->dc.nbx + 3) / 4; c->nty = (c->dc.nby + 3) / 4; c->ntz = (c->dc.nbz + 3) / 4;
+// 16^3 tiles of aggregates holding the FLUID blocks list[0..nlist)
+int lpc_cw_tiles(pf_ctx* c, const uint32_t* list, long long nlist) {
+  int nx1, ny1, nz1;
+  lpc_level1_extents(c, nx1, ny1, nz1);
+  c->ntx = (nx1 + 15) / 16; c->nty = (ny1 + 15) / 16; c->ntz = (nz1 + 15) / 16;
   const long long nall = (long long)c->ntx * c->nty * c->ntz;
   if (nall > c->ntiles_all) {
     dfree(c->tiles); dfree(c->tile_flags);
@@ -130,13 +112,55 @@ int lpc_build_tiles(pf_ctx* c) {
     c->ntiles_all = nall;
   }
   HIP_TRY(c, fill_u8(c->stream, c->tile_flags, 0, nall));
-  hipLaunchKernelGGL(k_lpc_tile_flags, dim3(grid_for(c->npblk, 256)), dim3(256), 0, c->stream, c->pblk, c->npblk, c->dc,
-                     c->ntx, c->nty, c->tile_flags);
-  LAUNCH_CHECK(c);
+  if (nlist > 0) {
+    hipLaunchKernelGGL(k_lpc_cw_tile_flags, dim3(grid_for(nlist, 256)), dim3(256), 0, c->stream, list, nlist, c->dc,
+                       c->ntx, c->nty, c->tile_flags);
+    LAUNCH_CHECK(c);
+  }
   HIP_TRY(c, compact_flags(*c, c->tile_flags, nall, c->tiles, c->u32_dev + 8));
   HIP_TRY(c, hipMemcpyAsync(c->u32_host + 8, c->u32_dev + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->ntiles = c->u32_host[8];
+  return PF_OK;
+}
+
+CwArgs lpc_cw_args(pf_ctx* c) {
+  CwArgs a{};
+  a.dc = c->dc;
+  a.g = c->geom;
+  a.d1 = c->mg[0].d;
+  lpc_level1_extents(c, a.nx1, a.ny1, a.nz1);
+  a.ntx = c->ntx; a.nty = c->nty;
+  a.tiles = c->tiles; a.ntiles = c->ntiles;
+  a.marker = c->marker; a.fbeta = c->fbeta; a.fphi = c->fphi;
+  a.cd = c->cd32; a.invd = c->invd32;
+  a.X = c->lpc->X;
+  return a;
+}
+
+unsigned lpc_tile_grid(long long nt) { return (unsigned)std::max<long long>(1, std::min<long long>(nt, 1024)); }
+
+// the network's per-aggregate weights of the last assembled system into c->mg_cs_net
+// (s = sqrt(w), 1 outside the tiles); store: keep the activations for a backward pass.
+// Needs the multigrid levels (mg_build) and the tiles and features of the system.
+int lpc_cw_weights(pf_ctx* c, bool store) {
+  pf_lpc* L = c->lpc;
+  if (!L) return fail(c, PF_ERR_STATE, "no network attached (pf_lpc_attach)");
+  if (c->mg.empty()) return fail(c, PF_ERR_STATE, "learned preconditioner: the grid has no coarse level");
+  const long long n1 = c->mg[0].d.nblocks * kBlockVol;
+  if (!c->mg_cs_net) HIP_TRY(c, dalloc(&c->mg_cs_net, n1));
+  HIP_TRY(c, fill_f32(c->stream, c->mg_cs_net, 1.f, n1));
+  if (c->ntiles == 0) return PF_OK;
+  int st = lpc_grow(L, c->ntiles, store);
+  if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  CwArgs a = lpc_cw_args(c);
+  hipLaunchKernelGGL(k_lpc_cw_gather, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, a);
+  LAUNCH_CHECK(c);
+  st = lpc_forward(L, c->stream, c->ntiles, store, nullptr);
+  if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  hipLaunchKernelGGL(k_lpc_cw_scatter, dim3(lpc_tile_grid(c->ntiles)), dim3(256), 0, c->stream, a, L->o,
+                     c->mg_cs_net);
+  LAUNCH_CHECK(c);
   return PF_OK;
 }
 

@@ -33,13 +33,14 @@ int pf_lpc_create(pf_lpc** out, uint64_t seed) {
   HIP_TRY(L, dalloc(&L->bf.w3f, 16 * kpad(32)));
   HIP_TRY(L, dalloc(&L->bf.w3b, 32 * kpad(16)));
   HIP_TRY(L, dalloc(&L->bf.wh, 16));
-  // He initialisation: N(0, 2 / fan_in) for the ReLU layers, N(0, 1/16) for the head
+  // He initialisation: N(0, 2 / fan_in) for the ReLU layers; the head starts at zero,
+  // so a new network outputs 0 everywhere (weights w = 1: the plain cycle)
   std::vector<float> w(kNW);
   uint64_t st = seed;
   const int off[4] = {kOffW1, kOffW2, kOffW3, kOffWH};
   const int cnt[4] = {kNW1, kNW2, kNW3, kNWH};
   const double sd[4] = {std::sqrt(2.0 / (27 * 4)), std::sqrt(2.0 / (27 * 16)), std::sqrt(2.0 / (27 * 32)),
-                        std::sqrt(1.0 / 16)};
+                        0.0};
   for (int l = 0; l < 4; ++l)
     for (int i = 0; i < cnt[l]; ++i) {
       double u1 = lpc_uniform(st), u2 = lpc_uniform(st);
@@ -59,6 +60,7 @@ int pf_lpc_destroy(pf_lpc* L) {
   dfree(L->X); dfree(L->h1); dfree(L->h2); dfree(L->h3); dfree(L->d3); dfree(L->d2); dfree(L->d1);
   dfree(L->o); dfree(L->go); dfree(L->scale); dfree(L->part);
   dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz); dfree(L->stage);
+  for (auto& p : L->l1) dfree(p);
   hipStreamDestroy(L->stream);
   delete L;
   return PF_OK;
@@ -151,10 +153,13 @@ int pf_lpc_record(pf_ctx* c, int32_t max_snapshots, int32_t every) {
 
 int32_t pf_lpc_snapshot_count(const pf_ctx* c) { return c ? c->nsnap : 0; }
 
-// loss ||r - A z||^2 / ||r||^2 of z = M(r) (learned) or z = D^-1 r for snapshot k;
-// leaves z in L->zbuf, q = r - A z in L->qbuf, ||r||^2 in *rr
+// loss ||r - A z||^2 / ||r||^2 of z = M(r) for snapshot k of the last assembled system:
+// M the multigrid cycle with the network's correction weights (learned) or the plain
+// cycle; store keeps what a backward pass needs (activations, level-1 trace).  Leaves
+// z in L->zbuf, q = r - A z in L->qbuf, ||r||^2 in *rr.
 static int lpc_loss_eval(pf_lpc* L, pf_ctx* c, int k, bool learned, bool store, double* loss, double* rr) {
   const long long ncell = c->dc.nblocks * kBlockVol;
+  const long long n1 = c->mg[0].d.nblocks * kBlockVol;
   if (ncell > L->cell_cap) {
     dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz);
     HIP_TRY(L, dalloc(&L->zbuf, ncell));
@@ -162,17 +167,29 @@ static int lpc_loss_eval(pf_lpc* L, pf_ctx* c, int k, bool learned, bool store, 
     HIP_TRY(L, dalloc(&L->gz, ncell));
     L->cell_cap = ncell;
   }
-  int st = lpc_grow(L, c->ntiles, store);
-  if (st) return fail(c, st, "learned preconditioner: " + L->err);
-  const float* r = c->snaps + (long long)k * ncell;
-  LpcTiles T = lpc_tiles_args(c, r);
-  hipLaunchKernelGGL(k_lpc_gather, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, T);
-  if (learned) {
-    st = lpc_forward(L, c->stream, c->ntiles, store, nullptr);
-    if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  if (n1 > L->l1_cap) {
+    for (auto& p : L->l1) dfree(p);
+    for (auto& p : L->l1) HIP_TRY(L, dalloc(&p, n1));
+    L->l1_cap = n1;
   }
-  hipLaunchKernelGGL(k_lpc_scatter, dim3(lpc_scatter_grid(c->ntiles)), dim3(256), 0, c->stream, T,
-                     learned ? L->o : nullptr, L->zbuf, nullptr, 0);
+  // stale values outside the system must stay finite: clear the work vectors
+  HIP_TRY(c, fill_f32(c->stream, L->zbuf, 0.f, ncell));
+  HIP_TRY(c, fill_f32(c->stream, L->qbuf, 0.f, ncell));
+  const float* r = c->snaps + (long long)k * ncell;
+  int st = PF_OK;
+  if (learned) {
+    st = lpc_cw_weights(c, store);
+    if (st) return st;
+    c->mg_cs_active = c->mg_cs_net;
+  } else {
+    c->mg_cs_active = nullptr;
+  }
+  MgTrace tr;
+  tr.y = L->l1[0];
+  tr.h = L->l1[1];
+  st = mg_apply_on(c, c->pblk_prev, c->npblk_prev, r, L->zbuf, nullptr, c->partial, store ? &tr : nullptr);
+  c->mg_cs_active = nullptr;
+  if (st) return st;
   VecArgs va{};
   va.dc = c->dc; va.list = c->pblk_prev; va.nlist = c->npblk_prev; va.sc = c->sc; va.part = c->partial;
   const unsigned G = list_grid(c->npblk_prev);
@@ -193,10 +210,14 @@ static int lpc_check_system(pf_lpc* L, pf_ctx* c) {
   if (!c->tiles_valid || c->nsnap <= 0)
     return fail(c, PF_ERR_STATE, "no recorded system: pf_lpc_record, then solve with the network attached");
   HIP_TRY(c, hipStreamSynchronize(L->stream));
+  // the cycle of the recorded system (its operator stays until the next projection)
+  int st = mg_build(c);
+  if (st) return st;
+  if (c->mg.empty()) return fail(c, PF_ERR_STATE, "learned preconditioner: the grid has no coarse level");
   return PF_OK;
 }
 
-int pf_lpc_loss(pf_lpc* L, pf_ctx* c, int32_t k, double* loss_learned, double* loss_jacobi) {
+int pf_lpc_loss(pf_lpc* L, pf_ctx* c, int32_t k, double* loss_learned, double* loss_plain) {
   int st = lpc_check_system(L, c);
   if (st) return st;
   if (k < 0 || k >= c->nsnap) return fail(c, PF_ERR_ARG, "snapshot index out of range");
@@ -205,10 +226,71 @@ int pf_lpc_loss(pf_lpc* L, pf_ctx* c, int32_t k, double* loss_learned, double* l
   if (!st) st = lpc_loss_eval(L, c, k, false, false, &b, &rr);
   if (st) return st;
   if (loss_learned) *loss_learned = a;
-  if (loss_jacobi) *loss_jacobi = b;
+  if (loss_plain) *loss_plain = b;
   return PF_OK;
 }
 
+// L->grad = d(||r - A M(r)||^2 / ||r||^2)/dw for snapshot k, with the loss (see
+// pf_lpc_cw.hip for the chain rule through the cycle)
+static int lpc_grad_eval(pf_lpc* L, pf_ctx* c, int k, double* loss_out) {
+  const long long ncell = c->dc.nblocks * kBlockVol;
+  const long long n1 = c->mg[0].d.nblocks * kBlockVol;
+  const unsigned gdn = std::min<unsigned>(grid_for(ncell, 256), 4096u);
+  const unsigned gd1 = std::min<unsigned>(grid_for(n1, 256), 2048u);
+  const std::vector<float> wpre = mg_sweep_weights(c);
+  VecArgs va{};
+  va.dc = c->dc; va.list = c->pblk_prev; va.nlist = c->npblk_prev; va.sc = c->sc; va.part = c->partial;
+  const unsigned G = list_grid(c->npblk_prev);
+  double loss = 0.0, rr = 0.0;
+  int st = lpc_loss_eval(L, c, k, true, true, &loss, &rr);
+  if (st) return st;
+  if (loss_out) *loss_out = loss;
+  HIP_TRY(c, fill_f32(c->stream, L->grad, 0.f, kNW));
+  if (!(rr > 0.0) || c->ntiles == 0) return PF_OK;
+  // dL/dz = -2 A q / ||r||^2, then through the post-smoothing (transposed sweeps
+  // I - theta A D^-1, first sweep's weight last)
+  HIP_TRY(c, fill_f32(c->stream, L->gz, 0.f, ncell));
+  hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->qbuf, nullptr, L->gz, c->cd32, c->cx32,
+                     c->cy32, c->cz32, (float)(-2.0 / rr));
+  for (size_t j = 0; j < wpre.size(); ++j) {
+    hipLaunchKernelGGL(k_lpc_dscale, dim3(gdn), dim3(256), 0, c->stream, ncell, L->gz, c->invd32, L->zbuf);
+    hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->zbuf, nullptr, L->qbuf, c->cd32,
+                       c->cx32, c->cy32, c->cz32, 1.f);
+    hipLaunchKernelGGL(k_lpc_axpy, dim3(gdn), dim3(256), 0, c->stream, ncell, L->gz, L->qbuf, -wpre[j]);
+  }
+  // dL/dg = w_c P^T (...) on level 1
+  int nx1, ny1, nz1;
+  lpc_level1_extents(c, nx1, ny1, nz1);
+  HIP_TRY(c, fill_f32(c->stream, L->l1[2], 0.f, n1));
+  hipLaunchKernelGGL(k_lpc_restrict_sum, dim3(gd1), dim3(256), 0, c->stream, c->dc, c->geom, L->gz, c->invd32,
+                     c->mg[0].d, nx1, ny1, nz1, c->mg_cfg.correction, L->l1[2]);
+  // B (S dL/dg) by the cycle's coarse levels
+  hipLaunchKernelGGL(k_lpc_mul, dim3(gd1), dim3(256), 0, c->stream, n1, c->mg_cs_net, L->l1[2], c->mg[0].r);
+  float* bw = mg_coarse(c, 1, nullptr, wpre);
+  // dL/ds = dL/dg (.) h + y (.) B S dL/dg, then dL/do in tile layout
+  hipLaunchKernelGGL(k_lpc_cw_grad, dim3(gd1), dim3(256), 0, c->stream, n1, L->l1[2], L->l1[1], L->l1[0], bw,
+                     L->l1[4]);
+  CwArgs a = lpc_cw_args(c);
+  hipLaunchKernelGGL(k_lpc_cw_gout, dim3(lpc_tile_grid(c->ntiles)), dim3(256), 0, c->stream, a, L->o, L->l1[4],
+                     c->mg_cs_net, L->go);
+  LAUNCH_CHECK(c);
+  st = lpc_backward(L, c->stream, c->ntiles);
+  if (st) return fail(c, st, "learned preconditioner: " + L->err);
+  return PF_OK;
+}
+
+int pf_lpc_grad(pf_lpc* L, pf_ctx* c, int32_t k, float* grad, double* loss) {
+  int st = lpc_check_system(L, c);
+  if (st) return st;
+  if (k < 0 || k >= c->nsnap) return fail(c, PF_ERR_ARG, "snapshot index out of range");
+  st = lpc_grad_eval(L, c, k, loss);
+  if (st) return st;
+  if (grad) HIP_TRY(c, hipMemcpyAsync(grad, L->grad, sizeof(float) * kNW, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PF_OK;
+}
+
+// one Adam step per sampled snapshot on the gradient of ||r - A M(r)||^2 / ||r||^2
 int pf_lpc_train(pf_lpc* L, pf_ctx* c, int32_t steps, double lr, uint64_t seed, double* mean_loss) {
   int st = lpc_check_system(L, c);
   if (st) return st;
@@ -217,23 +299,10 @@ int pf_lpc_train(pf_lpc* L, pf_ctx* c, int32_t steps, double lr, uint64_t seed, 
   double lsum = 0.0;
   for (int it = 0; it < steps; ++it) {
     const int k = (int)(lpc_uniform(rs) * c->nsnap) % c->nsnap;
-    double loss = 0.0, rr = 0.0;
-    st = lpc_loss_eval(L, c, k, true, true, &loss, &rr);
+    double loss = 0.0;
+    st = lpc_grad_eval(L, c, k, &loss);
     if (st) return st;
     lsum += loss;
-    if (!(rr > 0.0)) continue;
-    // dL/dz = -2 A q / ||r||^2 (A symmetric), then through z = r/diag + s * o
-    VecArgs va{};
-    va.dc = c->dc; va.list = c->pblk_prev; va.nlist = c->npblk_prev; va.sc = c->sc; va.part = c->partial;
-    const unsigned G = list_grid(c->npblk_prev);
-    hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->qbuf, nullptr, L->gz, c->cd32,
-                       c->cx32, c->cy32, c->cz32, (float)(-2.0 / rr));
-    LpcTiles T = lpc_tiles_args(c, c->snaps + (long long)k * c->dc.nblocks * kBlockVol);
-    hipLaunchKernelGGL(k_lpc_gz_to_tiles, dim3(lpc_scatter_grid(c->ntiles)), dim3(256), 0, c->stream, T, L->gz,
-                       L->go);
-    LAUNCH_CHECK(c);
-    st = lpc_backward(L, c->stream, c->ntiles);
-    if (st) return fail(c, st, "learned preconditioner: " + L->err);
     L->adam_t++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float bc1 = 1.f - std::pow(b1, (float)L->adam_t), bc2 = 1.f - std::pow(b2, (float)L->adam_t);

@@ -133,31 +133,91 @@ int mg_build(pf_ctx* c) {
 
 // workgroups of the level-0 kernels: a multiple of 8 (the XCD-aware list walk), at
 // most 1024 (the partials of r.z)
-unsigned mg_grid0(const pf_ctx* c) { return std::min(list_grid(c->npblk), 1024u); }
+unsigned mg_grid0n(long long nlist) { return std::min(list_grid(nlist), 1024u); }
+unsigned mg_grid0(const pf_ctx* c) { return mg_grid0n(c->npblk); }
 
-// z = V-cycle(r) on the assembled system; partial r.z (mg_grid0 workgroups) in
-// part[2 * kMaxPartials ...]; every kernel is skipped while *done
-int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part) {
+// the weights of a smoothing pass's sweeps: all omega, or (a Chebyshev ratio r > 1)
+// the reciprocals of the Chebyshev roots of degree nu on [lmax / r, lmax], lmax = 2
+// (Gershgorin bound of D^-1 A), which damp that whole interval; the post-pass takes
+// them in reverse order (they commute, so the cycle stays symmetric)
+std::vector<float> mg_sweep_weights(const pf_ctx* c) {
+  const int nu = c->mg_cfg.sweeps;
+  const float cheb = c->mg_cfg.chebyshev_ratio;
+  std::vector<float> w(nu, c->mg_cfg.omega);
+  if (cheb > 1.0f) {
+    const double b = 2.0, lo = b / cheb;
+    for (int q = 0; q < nu; ++q) {
+      const double lam = 0.5 * (b + lo) + 0.5 * (b - lo) * std::cos((2.0 * q + 1.0) * M_PI / (2.0 * nu));
+      w[q] = (float)(1.0 / lam);
+    }
+  }
+  return w;
+}
+
+// coarse levels: level l (1..nl) solves A_l x = r_l from zero (its buffer r holds r_l);
+// returns the buffer that holds x.  (Two corrections per level -- a W-cycle -- cost
+// more launches than they save iterations at 256^3, and twice the 1.8 over-correction
+// stops converging.)
+float* mg_coarse(pf_ctx* c, int l, const int* done, const std::vector<float>& wpre) {
+  const int nl = (int)c->mg.size();
+  const int nu = c->mg_cfg.sweeps;
+  MgWalk wd{nullptr, 0, done};
+  Ctx::MgBuf& b = c->mg[(size_t)l - 1];
+  const MgLevel L = mg_level(b);
+  float* x = b.x;
+  float* t = b.t;
+  if (l == nl) {
+    hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(64), 0, c->stream, done, L, b.r, x, c->mg_cfg.omega,
+                       c->mg_cfg.coarsest_sweeps);
+    return x;
+  }
+  const unsigned g = std::min<unsigned>(grid_for(b.d.nblocks * kBlockVol, 256), 2048u);
+  hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, wpre[0]);
+  for (int s2 = 1; s2 < nu; ++s2) {
+    hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, wpre[(size_t)s2], nullptr);
+    std::swap(x, t);
+  }
+  Ctx::MgBuf& nb = c->mg[(size_t)l];
+  hipLaunchKernelGGL(k_mg_restrict, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, mg_level(nb), nb.r);
+  float* below = mg_coarse(c, l + 1, done, wpre);
+  hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, c->stream, wd, L, mg_level(nb), below, x,
+                     c->mg_cfg.correction);
+  for (int s2 = 0; s2 < nu; ++s2) {
+    hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t,
+                       wpre[(size_t)(nu - 1 - s2)], nullptr);
+    std::swap(x, t);
+  }
+  return x;
+}
+
+// level-1 quantities of one cycle, for training the correction weights: the restricted
+// residual y before the scaling and the coarse solution B (S y) before the scaling
+struct MgTrace {
+  float* y = nullptr;
+  float* h = nullptr;
+};
+
+// z = V-cycle(r) on the assembled system whose FLUID blocks are list[0..nlist);
+// partial r.z (mg_grid0n(nlist) workgroups) in part[2 * kMaxPartials ...]; every
+// kernel is skipped while *done
+int mg_apply_on(pf_ctx* c, const uint32_t* list, long long nlist, const float* r, float* z, const int* done,
+                double* part, const MgTrace* tr) {
   struct {
     int nu, coarsest;
     float omega, correction;
   } const P{c->mg_cfg.sweeps, c->mg_cfg.coarsest_sweeps, c->mg_cfg.omega, c->mg_cfg.correction};
   const int nl = (int)c->mg.size();
-  MgWalk w0{c->pblk, c->npblk, done};
-  MgWalk wd{nullptr, 0, done};
-  const unsigned g0 = mg_grid0(c);
+  MgWalk w0{list, nlist, done};
+  const unsigned g0 = mg_grid0n(nlist);
   // kernels that write no partial may take more workgroups (up to 16 list blocks each)
-  const unsigned gw = list_grid(c->npblk);
+  const unsigned gw = list_grid(nlist);
   const MgLevel L0 = mg_level0(c);
   double* const rz = part + 2 * kMaxPartials;
-  auto gdense = [](const Ctx::MgBuf& b) {
-    return std::min<unsigned>(grid_for(b.d.nblocks * kBlockVol, 256), 2048u);
-  };
   // level 0 by cell rows when the slabs are below 2 GiB and there is a coarse level
   const bool rows = nl > 0 && c->dc.nblocks * kBlockVol * 4 < (1LL << 31);
   MgRowArgs ra{};
-  ra.list = c->pblk;
-  ra.nlist = c->npblk;
+  ra.list = list;
+  ra.nlist = nlist;
   ra.done = done;
   ra.d = c->dc;
   ra.cd = reinterpret_cast<const _Float16*>(c->mg_h[0]);
@@ -168,19 +228,7 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   ra.unscale = c->mg_unscale;
   ra.r = r;
   ra.omega = P.omega;
-  // the weights of a smoothing pass's sweeps: all omega, or (a Chebyshev ratio r > 1)
-  // the reciprocals of the Chebyshev roots of degree nu on [lmax / r, lmax], lmax = 2
-  // (Gershgorin bound of D^-1 A), which damp that whole interval; the post-pass takes
-  // them in reverse order (they commute, so the cycle stays symmetric)
-  const float cheb = c->mg_cfg.chebyshev_ratio;
-  std::vector<float> wpre(P.nu, P.omega);
-  if (cheb > 1.0f) {
-    const double b = 2.0, lo = b / cheb;
-    for (int q = 0; q < P.nu; ++q) {
-      const double lam = 0.5 * (b + lo) + 0.5 * (b - lo) * std::cos((2.0 * q + 1.0) * M_PI / (2.0 * P.nu));
-      wpre[q] = (float)(1.0 / lam);
-    }
-  }
+  const std::vector<float> wpre = mg_sweep_weights(c);
   auto wpost = [&](int q) { return wpre[(size_t)(P.nu - 1 - q)]; };
   ra.omega_c = P.correction;
   if (nl > 0) ra.dc = c->mg[0].d;
@@ -234,51 +282,21 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   }
   // per-aggregate correction scaling: S on the restricted residual ...
   const long long n1 = c->mg[0].d.nblocks * kBlockVol;
-  if (c->mg_cs)
+  if (tr && tr->y)
+    HIP_TRY(c, hipMemcpyAsync(tr->y, c->mg[0].r, sizeof(float) * (size_t)n1, hipMemcpyDeviceToDevice, c->stream));
+  if (c->mg_cs_active)
     hipLaunchKernelGGL(k_mg_scale, dim3(std::min<unsigned>(grid_for(n1, 256), 2048u)), dim3(256), 0, c->stream, n1,
-                       c->mg[0].r, c->mg_cs, done);
-  // coarse levels: level l (1..nl) solves A_l x = r_l from zero, result in res[l].
-  // (Two corrections per level -- a W-cycle -- cost more launches than they save
-  // iterations at 256^3, and twice the 1.8 over-correction stops converging.)
-  std::vector<float*> res(nl + 1), alt(nl + 1);
-  std::function<void(int)> coarse = [&](int l) {
-    Ctx::MgBuf& b = c->mg[(size_t)l - 1];
-    const MgLevel L = mg_level(b);
-    float* x = b.x;
-    float* t = b.t;
-    if (l == nl) {
-      hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(64), 0, c->stream, done, L, b.r, x, P.omega, P.coarsest);
-      res[l] = x;
-      alt[l] = t;
-      return;
-    }
-    const unsigned g = gdense(b);
-    hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, wpre[0]);
-    for (int s2 = 1; s2 < P.nu; ++s2) {
-      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, wpre[(size_t)s2],
-                         nullptr);
-      std::swap(x, t);
-    }
-    Ctx::MgBuf& nb = c->mg[(size_t)l];
-    hipLaunchKernelGGL(k_mg_restrict, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, mg_level(nb), nb.r);
-    coarse(l + 1);
-    hipLaunchKernelGGL(k_mg_prolong, dim3(g), dim3(256), 0, c->stream, wd, L, mg_level(nb), res[l + 1], x,
-                       P.correction);
-    for (int s2 = 0; s2 < P.nu; ++s2) {
-      hipLaunchKernelGGL(k_mg_sweep, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, t, wpost(s2), nullptr);
-      std::swap(x, t);
-    }
-    res[l] = x;
-    alt[l] = t;
-  };
-  coarse(1);
+                       c->mg[0].r, c->mg_cs_active, done);
+  float* const res1 = mg_coarse(c, 1, done, wpre);
+  if (tr && tr->h)
+    HIP_TRY(c, hipMemcpyAsync(tr->h, res1, sizeof(float) * (size_t)n1, hipMemcpyDeviceToDevice, c->stream));
   // ... and on the correction (P S B_1 S P^T stays symmetric)
-  if (c->mg_cs)
+  if (c->mg_cs_active)
     hipLaunchKernelGGL(k_mg_scale, dim3(std::min<unsigned>(grid_for(n1, 256), 2048u)), dim3(256), 0, c->stream, n1,
-                       res[1], c->mg_cs, done);
+                       res1, c->mg_cs_active, done);
   // level 0: prolong and post-smooth, the last sweep into z with the partial r.z
   if (rows) {
-    ra.x = cur; ra.out = nxt; ra.xc = res[1]; ra.part = P.nu == 1 ? rz : nullptr;
+    ra.x = cur; ra.out = nxt; ra.xc = res1; ra.part = P.nu == 1 ? rz : nullptr;
     ra.omega = wpost(0);
     hipLaunchKernelGGL(k_mg_row<MG_PSWEEP>, dim3(ra.part ? g0 : gw), dim3(256), 0, c->stream, ra);
     flip();
@@ -289,7 +307,7 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
       flip();
     }
   } else {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(g0), dim3(256), 0, c->stream, w0, L0, mg_level(c->mg[0]), res[1], cur,
+    hipLaunchKernelGGL(k_mg_prolong, dim3(g0), dim3(256), 0, c->stream, w0, L0, mg_level(c->mg[0]), res1, cur,
                        P.correction);
     for (int s2 = 0; s2 < P.nu; ++s2) {
       hipLaunchKernelGGL(k_mg_sweep, dim3(g0), dim3(256), 0, c->stream, w0, L0, r, cur, nxt, wpost(s2),
@@ -300,6 +318,10 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   LAUNCH_CHECK(c);
   if (cur != z) return fail(c, PF_ERR_STATE, "multigrid: cycle did not end in z");
   return PF_OK;
+}
+
+int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part) {
+  return mg_apply_on(c, c->pblk, c->npblk, r, z, done, part, nullptr);
 }
 
 // single: the single-phase unit-density projection (ph unused, cdiv ignored);
@@ -365,8 +387,9 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     if (out) *out = S;
     return PF_OK;
   }
-  // learned preconditioner: tiles and features of this system (for the solve or for
-  // recording training snapshots)
+  // learned preconditioner: the multigrid cycle with per-aggregate correction weights
+  // from the attached network; its tiles and features of this system are built for
+  // the solve or for recording training snapshots
   const bool want_learned = cfg->precond == PF_PRECOND_LEARNED;
   if (want_learned && !c->lpc) return fail(c, PF_ERR_STATE, "precond = learned but no network attached (pf_lpc_attach)");
   if (want_learned && mr) return fail(c, PF_ERR_ARG, "the learned preconditioner runs on the sparse path only");
@@ -377,7 +400,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   c->snap_last_iter = -1;
   if (lpc_system) {
     int s2 = lpc_features_alloc(c);
-    if (!s2) s2 = lpc_build_tiles(c);
+    if (!s2) s2 = lpc_cw_tiles(c, c->pblk, npb);
     if (s2) return s2;
   }
   ProjArgs pa;
@@ -485,7 +508,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const int max_updates = cfg->max_outer > 0 ? cfg->max_outer : 200;
   // (a multigrid iteration costs several products: check at least every 4)
   const int check_base = cfg->check_every > 0 ? cfg->check_every : 8;
-  const int check = want_mg ? std::min(check_base, 4) : check_base;
+  const int check = (want_mg || want_learned) ? std::min(check_base, 4) : check_base;
   const double delta = (cfg->inner_rel_tol > 0 && cfg->inner_rel_tol < 1) ? cfg->inner_rel_tol : 0.1;
   // the multires rule: max(absolute, relative * initial max residual), with x0 = 0
   // (multires_pressure3d.cpp:858-863)
@@ -517,23 +540,26 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     c->snap_last_iter = iter;
     return PF_OK;
   };
-  // flexible PCG around a preconditioner that is not a diagonal: the learned network
-  // or the multigrid V-cycle (z and partial r.z in slot 2, GT partials)
+  // flexible PCG around the multigrid V-cycle, plain or with the network's correction
+  // weights (z and partial r.z in slot 2, GT partials)
   bool learned = want_learned || want_mg;
-  if (want_mg) {
+  c->mg_cs_active = c->mg_cs;  // caller-set weights (pf_mg_set_correction_weights) or none
+  if (learned) {
     st = mg_build(c);
     if (st) return st;
   }
-  const unsigned GT = want_mg ? mg_grid0(c) : want_learned ? lpc_scatter_grid(c->ntiles) : 1u;
-  auto papply = [&](const int* done) -> int {
-    if (want_mg) {
-      long long ki = kbegin(c, PF_K_MG_APPLY);
-      int s2 = mg_apply(c, c->r32, c->z32, done, c->partial);
-      kend(c, ki);
-      return s2;
-    }
+  if (want_learned && !c->mg.empty()) {
+    // the network runs once per solve; the cycle then costs what the plain one does
     long long ki = kbegin(c, PF_K_LPC_APPLY);
-    int s2 = lpc_apply(c, c->r32, c->z32, done, c->partial, 2);
+    st = lpc_cw_weights(c, false);
+    kend(c, ki);
+    if (st) return st;
+    c->mg_cs_active = c->mg_cs_net;
+  }
+  const unsigned GT = learned ? mg_grid0(c) : 1u;
+  auto papply = [&](const int* done) -> int {
+    long long ki = kbegin(c, PF_K_MG_APPLY);
+    int s2 = mg_apply(c, c->r32, c->z32, done, c->partial);
     kend(c, ki);
     return s2;
   };

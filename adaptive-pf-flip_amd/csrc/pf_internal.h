@@ -177,7 +177,7 @@ struct Ctx {
   ::pf_lpc* lpc = nullptr;
   float* fbeta = nullptr;           // cell beta feature (mean face beta) on FLUID cells
   float* fphi = nullptr;            // cell phi feature (mean face phi) on FLUID cells
-  uint32_t* tiles = nullptr;        // 16^3 tiles holding FLUID blocks (sorted ids)
+  uint32_t* tiles = nullptr;        // 16^3 tiles of level-1 aggregates holding FLUID cells (sorted ids)
   uint8_t* tile_flags = nullptr;
   long long ntiles = 0, ntiles_all = 0;
   int ntx = 0, nty = 0, ntz = 0;
@@ -242,8 +242,12 @@ struct Ctx {
   uint32_t* mg_mx = nullptr;
   float* mg_unscale = nullptr;
   // per-aggregate scaling s = sqrt(w) of level 1's restricted residual and correction
-  // (P S B_1 S P^T in place of P B_1 P^T), or null: w = 1
+  // (P S B_1 S P^T in place of P B_1 P^T), or null: w = 1.  mg_cs holds the caller's
+  // weights, mg_cs_net the learned ones of the last assembled system; the cycle reads
+  // mg_cs_active
   float* mg_cs = nullptr;
+  float* mg_cs_net = nullptr;
+  const float* mg_cs_active = nullptr;
   pf_mg_cfg mg_cfg{};   // the cycle (set at creation, pf_set_multigrid)
 
   // per-kernel event timing (measurement hook)

@@ -1,16 +1,16 @@
-// pf_lpc.hip -- learned block preconditioner: a 3-layer 3x3x3 convolutional network
-// over 16^3 cell tiles with a 1-cell input halo, run on MFMA in BF16 with FP32
+// pf_lpc.hip -- the learned preconditioner's network: a 3-layer 3x3x3 convolutional
+// network over 16^3 tiles with a 1-cell input halo, run on MFMA in BF16 with FP32
 // accumulation, and its hand-written backward pass for training on the GPU.
 // (No reference counterpart: the reference preconditions with Jacobi only,
-// src/driver/sparse_ops3d_tp.cpp:322-327.)
+// src/driver/sparse_ops3d_tp.cpp:322-327.)  pf_lpc_cw.hip feeds it the features of
+// the multigrid cycle's aggregates and turns its output into correction weights.
 //
 // Network (weights W*[tap][ci][co], tap = dx + 3 dy + 9 dz, no biases):
-//   X[18^3][4] = (rho_hat, beta_hat, phi, 0)       rho_hat = (r / diag) / s_tile
+//   X[18^3][4]                     four input channels (pf_lpc_cw.hip)
 //   h1 = relu(conv(X, W1))        4 -> 16  (X carries the halo)
 //   h2 = relu(conv(pad0 h1, W2)) 16 -> 32  (zero padding at the tile boundary)
 //   h3 = relu(conv(pad0 h2, W3)) 32 -> 16
 //   o  = h3 . Wh                  16 -> 1
-//   z  = r / diag + s_tile * o     on FLUID cells (Jacobi plus a learned correction)
 // Activations are stored in BF16, channels last: [tile][cell][channel] with
 // cell = x + 16 (y + 16 z); the input X is [tile][18^3][4] with the halo.
 //
@@ -337,141 +337,7 @@ __global__ void __launch_bounds__(256) k_f32_to_bf16(const float* __restrict__ i
 }
 
 // ---------------------------------------------------------------- solver coupling
-
-struct LpcTiles {
-  Dims dc;
-  Geom g;
-  int ntx, nty, ntz;
-  const uint32_t* tiles;   // tile ids tx + ntx (ty + nty tz)
-  long long ntiles;
-  const uint8_t* marker;
-  const float* r;          // residual (block-major cells)
-  const float* invd;       // 1 / diag
-  const float* fbeta;      // cell beta feature
-  const float* fphi;       // cell phi feature
-  float* scale;            // per-tile RMS of r / diag over FLUID cells
-  __bf16* X;               // [tile][18^3][4]
-  const int* done;         // optional
-};
-
-__device__ inline void tile_origin(const LpcTiles& a, long long t, int& i0, int& j0, int& k0) {
-  const uint32_t id = a.tiles[t];
-  i0 = (int)(id % (uint32_t)a.ntx) * kTile;
-  j0 = (int)((id / (uint32_t)a.ntx) % (uint32_t)a.nty) * kTile;
-  k0 = (int)(id / ((uint32_t)a.ntx * (uint32_t)a.nty)) * kTile;
-}
-
-__device__ inline bool fluid_cell(const LpcTiles& a, int i, int j, int k, long long& c) {
-  if (i < 0 || j < 0 || k < 0 || i >= a.g.nx || j >= a.g.ny || k >= a.g.nz) return false;
-  c = bm_index(a.dc, i, j, k);
-  return a.marker[c] == CELL_FLUID;
-}
-
-// one workgroup per tile: s = RMS of r/diag and max beta over the tile's FLUID cells,
-// then the network input over the 18^3 halo region
-__global__ void __launch_bounds__(256) k_lpc_gather(LpcTiles a) {
-  __shared__ float s_sum[256], s_max[256];
-  __shared__ int s_cnt[256];
-  if (a.done && *a.done) return;
-  const long long t = blockIdx.x;
-  if (t >= a.ntiles) return;
-  int i0, j0, k0;
-  tile_origin(a, t, i0, j0, k0);
-  float sum = 0.f, bmax = 0.f;
-  int cnt = 0;
-  for (int q = threadIdx.x; q < kTileVol; q += 256) {
-    long long c;
-    if (fluid_cell(a, i0 + (q & 15), j0 + ((q >> 4) & 15), k0 + (q >> 8), c)) {
-      const float rho = a.r[c] * a.invd[c];
-      sum = fmaf(rho, rho, sum);
-      bmax = fmaxf(bmax, a.fbeta[c]);
-      ++cnt;
-    }
-  }
-  s_sum[threadIdx.x] = sum; s_max[threadIdx.x] = bmax; s_cnt[threadIdx.x] = cnt;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
-      s_max[threadIdx.x] = fmaxf(s_max[threadIdx.x], s_max[threadIdx.x + w]);
-      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  const float s = s_cnt[0] > 0 ? sqrtf(s_sum[0] / (float)s_cnt[0]) : 0.f;
-  const float inv_s = s > 0.f ? 1.f / s : 0.f;
-  const float inv_b = s_max[0] > 0.f ? 1.f / s_max[0] : 0.f;
-  if (threadIdx.x == 0) a.scale[t] = s;
-  __bf16* X = a.X + t * (long long)kHaloVol * 4;
-  for (int q = threadIdx.x; q < kHaloVol; q += 256) {
-    const int xh = q % kHalo, yh = (q / kHalo) % kHalo, zh = q / (kHalo * kHalo);
-    long long c;
-    bf16x4 v;
-    if (fluid_cell(a, i0 + xh - 1, j0 + yh - 1, k0 + zh - 1, c)) {
-      v[0] = (__bf16)(a.r[c] * a.invd[c] * inv_s);
-      v[1] = (__bf16)(a.fbeta[c] * inv_b);
-      v[2] = (__bf16)a.fphi[c];
-    } else {
-      v[0] = v[1] = v[2] = (__bf16)0.f;
-    }
-    v[3] = (__bf16)0.f;
-    *reinterpret_cast<bf16x4*>(X + (long long)q * 4) = v;
-  }
-}
-
-// z = r/diag + s * o on FLUID cells (0 elsewhere); partial r.z -> part[slot][workgroup]
-__global__ void __launch_bounds__(256) k_lpc_scatter(LpcTiles a, const float* __restrict__ o, float* __restrict__ z,
-                                                     double* __restrict__ part, int slot) {
-  if (a.done && *a.done) return;
-  double rz = 0.0;
-  for (long long t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-    int i0, j0, k0;
-    tile_origin(a, t, i0, j0, k0);
-    const float s = a.scale[t];
-    for (int q = threadIdx.x; q < kTileVol; q += 256) {
-      const int i = i0 + (q & 15), j = j0 + ((q >> 4) & 15), k = k0 + (q >> 8);
-      if (i >= a.g.nx || j >= a.g.ny || k >= a.g.nz) continue;
-      const long long c = bm_index(a.dc, i, j, k);
-      float zv = 0.f;
-      if (a.marker[c] == CELL_FLUID) {
-        const float rho = a.r[c] * a.invd[c];
-        zv = o ? fmaf(s, o[t * kTileVol + q], rho) : rho;  // o == null: the Jacobi part alone
-      }
-      z[c] = zv;
-      rz += (double)a.r[c] * (double)zv;
-    }
-  }
-  if (part) {
-    double v[1] = {rz};
-    const bool mx[1] = {false};
-    block_reduce_store<1>(v, mx, part + (long long)slot * kMaxPartials, kMaxPartials);
-  }
-}
-
-// go[tile][cell] = s_tile * gz[cell] on FLUID cells (the chain rule through the scatter)
-__global__ void __launch_bounds__(256) k_lpc_gz_to_tiles(LpcTiles a, const float* __restrict__ gz, float* __restrict__ go) {
-  for (long long t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-    int i0, j0, k0;
-    tile_origin(a, t, i0, j0, k0);
-    const float s = a.scale[t];
-    for (int q = threadIdx.x; q < kTileVol; q += 256) {
-      long long c;
-      float v = 0.f;
-      if (fluid_cell(a, i0 + (q & 15), j0 + ((q >> 4) & 15), k0 + (q >> 8), c)) v = s * gz[c];
-      go[t * kTileVol + q] = v;
-    }
-  }
-}
-
-// tile flags from the fluid-block list (block -> 16^3 tile)
-__global__ void __launch_bounds__(256) k_lpc_tile_flags(const uint32_t* __restrict__ list, long long n, Dims dc,
-                                                        int ntx, int nty, uint8_t* __restrict__ flags) {
-  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (q >= n) return;
-  int bx, by, bz;
-  block_coords(dc, list[q], bx, by, bz);
-  flags[(bx >> 2) + (long long)ntx * ((by >> 2) + (long long)nty * (bz >> 2))] = 1;
-}
+// (the network's inputs and outputs in the solver are built in pf_lpc_cw.hip)
 
 // y = A x over the list (conductance form, FP32); optional q = r - y with partial
 // sums q.q and r.r (slots 0, 1)

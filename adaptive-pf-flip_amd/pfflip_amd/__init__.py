@@ -358,6 +358,7 @@ def lib():
             "pf_lpc_snapshot_count": (i32, [vp]),
             "pf_lpc_train": (C.c_int, [vp, vp, i32, dbl, C.c_uint64, C.POINTER(dbl)]),
             "pf_lpc_loss": (C.c_int, [vp, vp, i32, C.POINTER(dbl), C.POINTER(dbl)]),
+            "pf_lpc_grad": (C.c_int, [vp, vp, i32, vp, C.POINTER(dbl)]),
             "pf_kernel_timing": (C.c_int, [vp, i32]),
             "pf_kernel_timing_mask": (C.c_int, [vp, C.c_uint32]),
             "pf_kernel_time": (C.c_int, [vp, i32, C.POINTER(i64), C.POINTER(dbl)]),
@@ -788,8 +789,10 @@ class SparseSim3D:
 
 
 class LearnedPreconditioner:
-    """The learned block preconditioner (a pf_lpc): conv3^3 network 4-16-32-16 plus a
-    1x1x1 head over 16^3 tiles, BF16 on MFMA; trained on the GPU."""
+    """The learned preconditioner (a pf_lpc): conv3^3 network 4-16-32-16 plus a 1x1x1
+    head, BF16 on MFMA, over 16^3 tiles of the multigrid cycle's aggregates; its
+    output weights each aggregate's coarse correction (PRECOND_LEARNED).  Trained on
+    the GPU."""
 
     def __init__(self, seed=0):
         self.h = C.c_void_p()
@@ -849,9 +852,17 @@ class LearnedPreconditioner:
         return loss.value
 
     def loss(self, ctx, k):
+        """(loss with the network's weights, loss of the plain cycle) on snapshot k"""
         a, b = C.c_double(), C.c_double()
         self._check(lib().pf_lpc_loss(self.h, ctx.h, k, C.byref(a), C.byref(b)), "pf_lpc_loss", ctx)
         return a.value, b.value
+
+    def grad(self, ctx, k):
+        """(gradient of the loss w.r.t. the flat weights, loss) on snapshot k"""
+        g = np.zeros(self.num_weights(), np.float32)
+        loss = C.c_double()
+        self._check(lib().pf_lpc_grad(self.h, ctx.h, k, _p(g), C.byref(loss)), "pf_lpc_grad", ctx)
+        return g, loss.value
 
 
 def lpc_record(ctx, max_snapshots, every=1):

@@ -1,18 +1,21 @@
-"""Train the learned block preconditioner on the GPU.
+"""Train the learned preconditioner on the GPU.
 
-Training data are residuals recorded by the solver itself: for each system, a
-two-phase Rayleigh-Taylor scene on an n^3 grid (default 128^3, 8 jittered particles
-per interior cell) with the light phase's density drawn log-uniform so that
-rho_l / rho_g lies in [1, 1000] (rho_l = 1000), one step is taken with the network
-attached and recording, and the plain Jacobi PCG's FP32 residuals (one per `every`
-iterations, up to `snapshots`) become that system's training set.  The network is
-trained with Adam on loss ||r - A M(r)||^2 / ||r||^2 using the library's own forward
-and backward kernels (pf_lpc_train).  With 10 systems x 200 snapshots this is the
-~2000 residual/system pairs the workload calls for.  The seeds and ratios are
-printed and written beside the weights.
+Training data are residuals recorded by the solver itself.  A system is one pressure
+projection of a two-phase scene on an n^3 grid (default 128^3, 8 jittered particles
+per interior cell): Rayleigh-Taylor, or (one system in four) the bubble tank with its
+empty headspace, with the light phase's density drawn log-uniform so that
+rho_l / rho_g lies in [1, 1000] (rho_l = 1000).  Each scene is stepped with the
+multigrid-preconditioned solve; from its second step on, every projection records its
+FP32 residuals (one per host check, up to `snapshots`) and the network takes `steps`
+Adam steps on them with loss ||r - A M(r)||^2 / ||r||^2, where M is the multigrid
+cycle with the network's per-aggregate correction weights (pf_lpc_train, the
+library's own forward and backward kernels).  The default 40 scenes x 8 recorded
+steps give about 2000 residual/system pairs.  Held-out systems then compare the solve
+with the trained network against the plain cycle.  Seeds, ratios and results are
+written beside the weights.
 
 Usage (on a GPU):
-  python -m pfflip_amd.lpc_train --out weights.npy [--n 128 --systems 10 ...]
+  python -m pfflip_amd.lpc_train --out weights.npy [--n 128 --scenes 40 ...]
 """
 import argparse
 import json
@@ -45,64 +48,118 @@ def system_phase(ratio):
     return ph
 
 
-def step_cfg(ph, max_iters=20000, tol=1e-6, precond=G.PRECOND_JACOBI):
-    solve = G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=0.1, max_outer=10000,
-                       warm_start=0, check_every=8, precond=precond)
-    return G.StepCfg(dt=1e-3, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95, adaptive_cfl=0.5,
+def solve_cfg(precond, tol=1e-6, max_iters=20000):
+    return G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=0.01, max_outer=10000,
+                      warm_start=0, check_every=4, precond=precond)
+
+
+def step_cfg(ph, dt, precond=G.PRECOND_MULTIGRID):
+    return G.StepCfg(dt=dt, gravity=-9.81, Vp=1.0, alpha_liquid=0.95, alpha_gas=0.95, adaptive_cfl=0.5,
                      adaptive_min_dt=1e-5, c_div=0.0, adaptive_timestep=0, advection_order=2, timing=0,
-                     model=G.MODEL_TWO_PHASE, phase=ph, solve=solve)
+                     model=G.MODEL_TWO_PHASE, phase=ph, solve=solve_cfg(precond))
 
 
-def harvest(net, n, seed, ratio, snapshots, every, warm_steps=1):
-    """A context whose last projection recorded `snapshots` residuals."""
+def compare(net, n, kind, seed, ratio, steps=2, repeats=2):
+    """iterations and solve ms of the plain cycle and of the learned one on the same
+    systems (each projection solved by both from the same pre-projection state)"""
     ctx = G.Context(n, n, n, 1.0 / n)
-    ctx.scene(G.SCENE_RT, seed)
-    ph = system_phase(ratio)
-    cfg = step_cfg(ph)
-    for _ in range(warm_steps):
-        ctx.step(cfg)
-    net.attach(ctx)
-    G.lpc_record(ctx, snapshots, every)
-    st = ctx.step(cfg)
-    return ctx, st
+    try:
+        ctx.scene(kind, seed)
+        ph = system_phase(ratio)
+        dt = 1e-3 if kind == G.SCENE_RT else 5e-4
+        net.attach(ctx)
+        out = []
+        for _ in range(steps):
+            ctx.step(step_cfg(ph, dt))
+            row = {}
+            for name, pre in (("plain", G.PRECOND_MULTIGRID), ("learned", G.PRECOND_LEARNED)):
+                best = None
+                for _ in range(repeats):
+                    ctx.mark_cells(); ctx.p2g(ph, 1.0); ctx.save_velocity(); ctx.apply_gravity(dt, -9.81)
+                    ctx.apply_walls()
+                    ctx.synchronize()
+                    t0 = time.perf_counter()
+                    st = ctx.project(ph, dt, solve_cfg(pre))
+                    ctx.synchronize()
+                    ms = (time.perf_counter() - t0) * 1e3
+                    best = ms if best is None else min(best, ms)
+                row[name] = {"iterations": st.iterations, "ms": round(best, 3), "converged": st.converged,
+                             "fallback": st.fallback_used, "residual": st.true_rel_residual}
+            out.append(row)
+        G.lib().pf_lpc_attach(ctx.h, None)
+        return out
+    finally:
+        ctx.close()
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--systems", type=int, default=10)
-    ap.add_argument("--snapshots", type=int, default=200)
-    ap.add_argument("--every", type=int, default=4)
-    ap.add_argument("--steps", type=int, default=300, help="Adam steps per system")
+    ap.add_argument("--scenes", type=int, default=40)
+    ap.add_argument("--steps-per-scene", type=int, default=8, help="recorded projections per scene")
+    ap.add_argument("--snapshots", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=24, help="Adam steps per recorded system")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=20251)
+    ap.add_argument("--init", default=None, help="start from these weights")
     args = ap.parse_args(argv)
     rng = np.random.default_rng(args.seed)
     net = G.LearnedPreconditioner(seed=args.seed)
-    log = {"n": args.n, "systems": [], "args": vars(args)}
+    if args.init:
+        net.set_weights(load_weights(args.init))
+    log = {"n": args.n, "scenes": [], "args": vars(args)}
     t0 = time.time()
-    for s in range(args.systems):
+    pairs = 0
+    for s in range(args.scenes):
         ratio = float(10.0 ** rng.uniform(0.0, 3.0))
         scene_seed = int(rng.integers(1, 2 ** 31))
-        ctx, st = harvest(net, args.n, scene_seed, ratio, args.snapshots, args.every)
-        ns = G.lpc_snapshot_count(ctx)
-        lr = args.lr * (0.5 ** (s // 4))
-        loss = net.train(ctx, args.steps, lr=lr, seed=scene_seed)
-        probe = [net.loss(ctx, k) for k in range(0, ns, max(1, ns // 8))]
-        rec = {"ratio": ratio, "scene_seed": scene_seed, "snapshots": ns, "jacobi_iterations": st.solve.iterations,
-               "train_loss": loss, "loss_learned": float(np.mean([a for a, _ in probe])),
-               "loss_jacobi": float(np.mean([b for _, b in probe]))}
-        log["systems"].append(rec)
-        print(f"[lpc_train] system {s}: ratio {ratio:.1f}, {ns} snapshots, Jacobi {st.solve.iterations} it, "
-              f"train loss {loss:.4f}, eval learned {rec['loss_learned']:.4f} vs Jacobi {rec['loss_jacobi']:.4f} "
-              f"({time.time() - t0:.0f} s)", flush=True)
-        ctx.close()
+        kind = G.SCENE_BUBBLE if s % 4 == 3 else G.SCENE_RT
+        dt = 1e-3 if kind == G.SCENE_RT else 5e-4
+        ph = system_phase(ratio)
+        ctx = G.Context(args.n, args.n, args.n, 1.0 / args.n)
+        try:
+            ctx.scene(kind, scene_seed)
+            ctx.step(step_cfg(ph, dt))
+            net.attach(ctx)
+            lr = args.lr * (0.5 ** (s // 10))
+            rec = {"kind": kind, "ratio": ratio, "scene_seed": scene_seed, "systems": []}
+            for k in range(args.steps_per_scene):
+                G.lpc_record(ctx, args.snapshots, 1)
+                st = ctx.step(step_cfg(ph, dt))
+                ns = G.lpc_snapshot_count(ctx)
+                pairs += ns
+                loss = net.train(ctx, args.steps, lr=lr, seed=scene_seed + k)
+                probe = [net.loss(ctx, q) for q in range(ns)]
+                rec["systems"].append({"snapshots": ns, "iterations": st.solve.iterations, "train_loss": loss,
+                                       "loss_learned": float(np.mean([a for a, _ in probe])),
+                                       "loss_plain": float(np.mean([b for _, b in probe]))})
+            G.lpc_record(ctx, 0, 1)
+            G.lib().pf_lpc_attach(ctx.h, None)
+        finally:
+            ctx.close()
+        last = rec["systems"][-1]
+        print(f"[lpc_train] scene {s} ({'bubble' if kind else 'RT'}, ratio {ratio:.1f}): {pairs} pairs so far; "
+              f"last system {last['iterations']} it, loss learned {last['loss_learned']:.5f} vs plain "
+              f"{last['loss_plain']:.5f} ({time.time() - t0:.0f} s)", flush=True)
+        log["scenes"].append(rec)
+    log["pairs"] = pairs
     w = net.weights()
     np.save(args.out, w)
+    # held-out systems: seeds and ratios not used above
+    evals = []
+    for kind, seed, ratio in ((G.SCENE_RT, 7, 1000.0), (G.SCENE_RT, 11, 30.0), (G.SCENE_BUBBLE, 13, 1000.0),
+                              (G.SCENE_BUBBLE, 17, 100.0)):
+        res = compare(net, args.n, kind, seed, ratio)
+        evals.append({"kind": kind, "seed": seed, "ratio": ratio, "steps": res})
+        for row in res:
+            print(f"[lpc_train] held-out {'bubble' if kind else 'RT'} ratio {ratio:g}: plain "
+                  f"{row['plain']['iterations']} it {row['plain']['ms']} ms, learned {row['learned']['iterations']} it "
+                  f"{row['learned']['ms']} ms", flush=True)
+    log["held_out"] = evals
     with open(os.path.splitext(args.out)[0] + "_train.json", "w") as f:
         json.dump(log, f, indent=1)
-    print(f"[lpc_train] wrote {args.out}", flush=True)
+    print(f"[lpc_train] wrote {args.out} ({pairs} pairs, {time.time() - t0:.0f} s)", flush=True)
 
 
 if __name__ == "__main__":

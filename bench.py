@@ -77,8 +77,9 @@ def _ensure_built():
 # per preconditioner: (pf_precond, FP32 reduction per FP64 refinement pass, iterations
 # between host checks) -- a multigrid iteration costs about four Jacobi ones, so its
 # passes go deeper and the host looks more often
-SOLVERS = {"jacobi": (0, 0.1, 16), "multigrid": (2, 0.01, 4)}
-SOLVER_NAMES = {"jacobi": "plain Jacobi PCG", "multigrid": "aggregation-multigrid PCG"}
+SOLVERS = {"jacobi": (0, 0.1, 16), "multigrid": (2, 0.01, 4), "learned": (1, 0.01, 4)}
+SOLVER_NAMES = {"jacobi": "plain Jacobi PCG", "multigrid": "aggregation-multigrid PCG",
+                "learned": "multigrid PCG with learned correction weights"}
 
 
 def solve_cfg(G, precond, tol, max_iters, warm=0):
@@ -119,15 +120,11 @@ def pressure_comparison(G, ctx, cfg, tol, learned=True):
     runs = [("plain_pcg", "jacobi"), ("multigrid_pcg", "multigrid"), ("multigrid_pcg", "multigrid")]
     if net is not None:
         # the first learned solve also allocates the network's buffers; keep the second
-        runs += [("learned_pc", G.PRECOND_LEARNED), ("learned_pc", G.PRECOND_LEARNED)]
+        runs += [("learned_pc", "learned"), ("learned_pc", "learned")]
     out = {"learned_pc": None}
     for name, pre in runs:
         prepare()
-        if pre == G.PRECOND_LEARNED:
-            solve = G.default_solve_cfg(G.STOP_REL2, tol, 50000, pre)
-            solve.check_every = 16
-        else:
-            solve = solve_cfg(G, pre, tol, 50000)
+        solve = solve_cfg(G, pre, tol, 50000)
         ctx.synchronize()
         t0 = time.perf_counter()
         st = ctx.project(ph, dt, solve)
@@ -141,6 +138,7 @@ def pressure_comparison(G, ctx, cfg, tol, learned=True):
     out["speedup_multigrid_vs_plain"] = round(out["plain_pcg"]["ms"] / out["multigrid_pcg"]["ms"], 3)
     if net is not None:
         out["speedup_learned_vs_plain"] = round(out["plain_pcg"]["ms"] / out["learned_pc"]["ms"], 3)
+        out["speedup_learned_vs_multigrid"] = round(out["multigrid_pcg"]["ms"] / out["learned_pc"]["ms"], 3)
         G.lib().pf_lpc_attach(ctx.h, None)
         net.close()
     else:
@@ -299,6 +297,12 @@ def main():
     ctx = G.Context(n, n, n, 1.0 / n)
     ctx.scene(G.SCENE_RT, args.seed)
     cfg = step_cfg(G, n, args.tol, args.max_iters, args.warm_start, args.precond)
+    step_net = None
+    if args.precond == "learned":
+        from pfflip_amd import lpc_train
+        step_net = G.LearnedPreconditioner(0)
+        step_net.set_weights(lpc_train.load_weights())
+        step_net.attach(ctx)
     print(f"[bench] {n}^3 RT, {ctx.count()} particles; warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         st = ctx.step(cfg)
@@ -374,6 +378,8 @@ def main():
     if not args.no_pressure:
         pressure = pressure_comparison(G, ctx, cfg, args.tol, not args.no_learned)
     ctx.close()
+    if step_net is not None:
+        step_net.close()
     workloads = {}
     if not args.no_workloads:
         workloads["rt256_jacobi"] = run_workload(

@@ -74,7 +74,8 @@ typedef struct pf_solve_cfg {
 
 enum pf_precond {
   PF_PRECOND_JACOBI = 0,    /* the reference's diagonal preconditioner (sparse_ops3d_tp.cpp:322-327) */
-  PF_PRECOND_LEARNED = 1,   /* flexible PCG with the attached learned network (pf_lpc_attach);
+  PF_PRECOND_LEARNED = 1,   /* the multigrid V-cycle with per-aggregate correction weights from
+                               the attached network (pf_lpc_attach), evaluated once per solve;
                                falls back to Jacobi when it stagnates (stats.fallback_used) */
   PF_PRECOND_MULTIGRID = 2  /* a V-cycle of Galerkin aggregation multigrid (2x2x2 aggregates,
                                damped-Jacobi smoothing; pf_set_multigrid), sparse path; no
@@ -517,15 +518,20 @@ int pf_mr_apply_surface_tension(pf_ctx* ctx, const pf_phase* phase, double dt, d
  * particles; interface diagnostics and optional surface tension as the reference) */
 int pf_mr_step(pf_ctx* ctx, const pf_mr_step_cfg* cfg, pf_step_stats* stats);
 
-/* ---- learned block preconditioner (no reference counterpart: the reference
+/* ---- learned preconditioner (no reference counterpart: the reference
  * preconditions with Jacobi, sparse_ops3d_tp.cpp:322-327).  A 3-layer 3x3x3
- * convolutional network over 16^3 cell tiles with a 1-cell halo (widths 16-32-16
- * and a 1x1x1 head), inputs (r / diag, beta, phi) per cell, BF16 on MFMA with FP32
- * accumulation; z = r / diag + s_tile * network.  Trained on the GPU with
- * hand-written backward kernels on residuals recorded from the solver's own runs,
- * loss ||r - A M(r)||^2 / ||r||^2, Adam. */
+ * convolutional network (widths 16-32-16 and a 1x1x1 head), BF16 on MFMA with FP32
+ * accumulation, over 16^3 tiles of the multigrid cycle's 2x2x2 aggregates with a
+ * 1-cell halo.  Its inputs per aggregate come from beta and phi of the assembled
+ * system (mean beta relative to the tile, mean phi, the beta jump inside the
+ * aggregate, the Dirichlet share of the diagonal); its output o sets the weight
+ * w = exp(clamp(o, -2, 2)) of that aggregate's coarse correction, applied
+ * symmetrically (P S B S P^T, S = diag(sqrt(w))), so the preconditioner is a fixed
+ * linear operator for the solve and the network runs once per solve.  Trained on the
+ * GPU with hand-written backward kernels on residuals recorded from the solver's own
+ * runs, loss ||r - A M(r)||^2 / ||r||^2 (exact gradient through the cycle), Adam. */
 typedef struct pf_lpc pf_lpc;
-int pf_lpc_create(pf_lpc** out, uint64_t seed);       /* He-initialised weights */
+int pf_lpc_create(pf_lpc** out, uint64_t seed);       /* He-initialised layers, zero head */
 int pf_lpc_destroy(pf_lpc* lpc);
 const char* pf_lpc_last_error(const pf_lpc* lpc);
 int64_t pf_lpc_num_weights(void);                     /* 29,392 */
@@ -547,8 +553,12 @@ int32_t pf_lpc_snapshot_count(const pf_ctx* ctx);
 /* `steps` Adam steps on the attached context's last system and its snapshots
  * (uniformly drawn from the seed); *mean_loss = mean of ||r - A M r||^2/||r||^2 */
 int pf_lpc_train(pf_lpc* lpc, pf_ctx* ctx, int32_t steps, double lr, uint64_t seed, double* mean_loss);
-/* the loss of the network and of Jacobi (M = D^-1) on one recorded snapshot */
-int pf_lpc_loss(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, double* loss_learned, double* loss_jacobi);
+/* the loss with the network's weights and with the plain cycle (w = 1) on one
+ * recorded snapshot */
+int pf_lpc_loss(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, double* loss_learned, double* loss_plain);
+/* the gradient of that loss w.r.t. the flat weights (grad[pf_lpc_num_weights()]) and
+ * the loss, on one recorded snapshot */
+int pf_lpc_grad(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, float* grad, double* loss);
 
 /* ---- multigrid preconditioner (no reference counterpart; PF_PRECOND_MULTIGRID) */
 /* the built-in cycle (2 Chebyshev sweeps on [1/3, 2], 32 coarsest sweeps at omega
