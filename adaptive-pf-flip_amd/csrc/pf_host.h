@@ -22,6 +22,10 @@ struct pf_lpc {
   float* part = nullptr;
   long long part_cap = 0;
   float *zbuf = nullptr, *qbuf = nullptr, *gz = nullptr;
+  // energy objective: the error vector e and r = A e
+  float *ebuf = nullptr, *rbuf = nullptr;
+  int objective = PF_LPC_OBJECTIVE_RESIDUAL;
+  int power_steps = 0;
   // level-1 training vectors: restricted residual, coarse solution, dL/dg, B S dL/dg, dL/ds
   float* l1[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   long long l1_cap = 0;

@@ -59,7 +59,7 @@ int pf_lpc_destroy(pf_lpc* L) {
   dfree(L->bf.wh);
   dfree(L->X); dfree(L->h1); dfree(L->h2); dfree(L->h3); dfree(L->d3); dfree(L->d2); dfree(L->d1);
   dfree(L->o); dfree(L->go); dfree(L->scale); dfree(L->part);
-  dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz); dfree(L->stage);
+  dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz); dfree(L->ebuf); dfree(L->rbuf); dfree(L->stage);
   for (auto& p : L->l1) dfree(p);
   hipStreamDestroy(L->stream);
   delete L;
@@ -153,18 +153,25 @@ int pf_lpc_record(pf_ctx* c, int32_t max_snapshots, int32_t every) {
 
 int32_t pf_lpc_snapshot_count(const pf_ctx* c) { return c ? c->nsnap : 0; }
 
-// loss ||r - A z||^2 / ||r||^2 of z = M(r) for snapshot k of the last assembled system:
+// the loss of the network's objective for snapshot k of the last assembled system, with
 // M the multigrid cycle with the network's correction weights (learned) or the plain
-// cycle; store keeps what a backward pass needs (activations, level-1 trace).  Leaves
-// z in L->zbuf, q = r - A z in L->qbuf, ||r||^2 in *rr.
-static int lpc_loss_eval(pf_lpc* L, pf_ctx* c, int k, bool learned, bool store, double* loss, double* rr) {
+// cycle; store keeps what a backward pass needs (activations, level-1 trace).
+//   residual: ||r - A z||^2 / ||r||^2, z = M r, r the snapshot
+//   energy:   (e - z)^T A (e - z) / e^T A e, z = M A e, e the snapshot after
+//             power_steps steps e <- e - M A e
+// Leaves z in L->zbuf, q = r - A z in L->qbuf (r = A e for the energy form) and the
+// loss's denominator in *den.
+static int lpc_loss_eval(pf_lpc* L, pf_ctx* c, int k, bool learned, bool store, double* loss, double* den) {
   const long long ncell = c->dc.nblocks * kBlockVol;
   const long long n1 = c->mg[0].d.nblocks * kBlockVol;
+  const bool energy = L->objective == PF_LPC_OBJECTIVE_ENERGY;
   if (ncell > L->cell_cap) {
-    dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz);
+    dfree(L->zbuf); dfree(L->qbuf); dfree(L->gz); dfree(L->ebuf); dfree(L->rbuf);
     HIP_TRY(L, dalloc(&L->zbuf, ncell));
     HIP_TRY(L, dalloc(&L->qbuf, ncell));
     HIP_TRY(L, dalloc(&L->gz, ncell));
+    HIP_TRY(L, dalloc(&L->ebuf, ncell));
+    HIP_TRY(L, dalloc(&L->rbuf, ncell));
     L->cell_cap = ncell;
   }
   if (n1 > L->l1_cap) {
@@ -180,27 +187,48 @@ static int lpc_loss_eval(pf_lpc* L, pf_ctx* c, int k, bool learned, bool store, 
   if (learned) {
     st = lpc_cw_weights(c, store);
     if (st) return st;
-    c->mg_cs_active = c->mg_cs_net;
-  } else {
-    c->mg_cs_active = nullptr;
+  }
+  c->mg_cs_active = learned ? c->mg_cs_net : nullptr;
+  VecArgs va{};
+  va.dc = c->dc; va.list = c->pblk_prev; va.nlist = c->npblk_prev; va.sc = c->sc; va.part = c->partial;
+  const unsigned G = list_grid(c->npblk_prev);
+  const unsigned gdn = std::min<unsigned>(grid_for(ncell, 256), 4096u);
+  if (energy) {
+    HIP_TRY(c, hipMemcpyAsync(L->ebuf, r, sizeof(float) * (size_t)ncell, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, fill_f32(c->stream, L->rbuf, 0.f, ncell));
+    for (int p = 0; p <= L->power_steps && !st; ++p) {
+      hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->ebuf, nullptr, L->rbuf, c->cd32,
+                         c->cx32, c->cy32, c->cz32, 1.f);
+      if (p == L->power_steps) break;
+      st = mg_apply_on(c, c->pblk_prev, c->npblk_prev, L->rbuf, L->zbuf, nullptr, c->partial, nullptr);
+      hipLaunchKernelGGL(k_lpc_axpy, dim3(gdn), dim3(256), 0, c->stream, ncell, L->ebuf, L->zbuf, -1.f);
+    }
+    r = L->rbuf;
   }
   MgTrace tr;
   tr.y = L->l1[0];
   tr.h = L->l1[1];
-  st = mg_apply_on(c, c->pblk_prev, c->npblk_prev, r, L->zbuf, nullptr, c->partial, store ? &tr : nullptr);
+  if (!st) st = mg_apply_on(c, c->pblk_prev, c->npblk_prev, r, L->zbuf, nullptr, c->partial, store ? &tr : nullptr);
   c->mg_cs_active = nullptr;
   if (st) return st;
-  VecArgs va{};
-  va.dc = c->dc; va.list = c->pblk_prev; va.nlist = c->npblk_prev; va.sc = c->sc; va.part = c->partial;
-  const unsigned G = list_grid(c->npblk_prev);
   hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->zbuf, r, L->qbuf, c->cd32, c->cx32,
                      c->cy32, c->cz32, 1.f);
-  hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 2, 0u, c->red);
+  if (energy) {
+    hipLaunchKernelGGL(k_lpc_dots3, dim3(G), dim3(256), 0, c->stream, va, L->ebuf, L->qbuf, L->zbuf, r);
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 3, 0u, c->red);
+  } else {
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 2, 0u, c->red);
+  }
   LAUNCH_CHECK(c);
   st = read_small(c);
   if (st) return st;
-  *rr = c->red_host[1];
-  *loss = *rr > 0.0 ? c->red_host[0] / *rr : 0.0;
+  if (energy) {
+    *den = c->red_host[2];
+    *loss = *den > 0.0 ? (c->red_host[0] - c->red_host[1]) / *den : 0.0;
+  } else {
+    *den = c->red_host[1];
+    *loss = *den > 0.0 ? c->red_host[0] / *den : 0.0;
+  }
   return PF_OK;
 }
 
@@ -230,7 +258,7 @@ int pf_lpc_loss(pf_lpc* L, pf_ctx* c, int32_t k, double* loss_learned, double* l
   return PF_OK;
 }
 
-// L->grad = d(||r - A M(r)||^2 / ||r||^2)/dw for snapshot k, with the loss (see
+// L->grad = d(loss)/dw of the network's objective for snapshot k, with the loss (see
 // pf_lpc_cw.hip for the chain rule through the cycle)
 static int lpc_grad_eval(pf_lpc* L, pf_ctx* c, int k, double* loss_out) {
   const long long ncell = c->dc.nblocks * kBlockVol;
@@ -247,11 +275,14 @@ static int lpc_grad_eval(pf_lpc* L, pf_ctx* c, int k, double* loss_out) {
   if (loss_out) *loss_out = loss;
   HIP_TRY(c, fill_f32(c->stream, L->grad, 0.f, kNW));
   if (!(rr > 0.0) || c->ntiles == 0) return PF_OK;
-  // dL/dz = -2 A q / ||r||^2, then through the post-smoothing (transposed sweeps
-  // I - theta A D^-1, first sweep's weight last)
+  // dL/dz = -2 A q / ||r||^2 (residual) or -2 q / e^T A e (energy), then through the
+  // post-smoothing (transposed sweeps I - theta A D^-1, first sweep's weight last)
   HIP_TRY(c, fill_f32(c->stream, L->gz, 0.f, ncell));
-  hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->qbuf, nullptr, L->gz, c->cd32, c->cx32,
-                     c->cy32, c->cz32, (float)(-2.0 / rr));
+  if (L->objective == PF_LPC_OBJECTIVE_ENERGY)
+    hipLaunchKernelGGL(k_lpc_axpy, dim3(gdn), dim3(256), 0, c->stream, ncell, L->gz, L->qbuf, (float)(-2.0 / rr));
+  else
+    hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->qbuf, nullptr, L->gz, c->cd32,
+                       c->cx32, c->cy32, c->cz32, (float)(-2.0 / rr));
   for (size_t j = 0; j < wpre.size(); ++j) {
     hipLaunchKernelGGL(k_lpc_dscale, dim3(gdn), dim3(256), 0, c->stream, ncell, L->gz, c->invd32, L->zbuf);
     hipLaunchKernelGGL(k_lpc_apply_a, dim3(G), dim3(256), 0, c->stream, va, L->zbuf, nullptr, L->qbuf, c->cd32,
@@ -290,7 +321,19 @@ int pf_lpc_grad(pf_lpc* L, pf_ctx* c, int32_t k, float* grad, double* loss) {
   return PF_OK;
 }
 
-// one Adam step per sampled snapshot on the gradient of ||r - A M(r)||^2 / ||r||^2
+// one Adam step per sampled snapshot on the gradient of the network's objective
+int pf_lpc_set_objective(pf_lpc* L, int32_t objective, int32_t power_steps) {
+  if (!L) return PF_ERR_ARG;
+  if ((objective != PF_LPC_OBJECTIVE_RESIDUAL && objective != PF_LPC_OBJECTIVE_ENERGY) || power_steps < 0 ||
+      power_steps > 64) {
+    L->err = "objective must be PF_LPC_OBJECTIVE_RESIDUAL or PF_LPC_OBJECTIVE_ENERGY with 0..64 power steps";
+    return PF_ERR_ARG;
+  }
+  L->objective = objective;
+  L->power_steps = power_steps;
+  return PF_OK;
+}
+
 int pf_lpc_train(pf_lpc* L, pf_ctx* c, int32_t steps, double lr, uint64_t seed, double* mean_loss) {
   int st = lpc_check_system(L, c);
   if (st) return st;

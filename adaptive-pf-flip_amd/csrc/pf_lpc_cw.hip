@@ -17,10 +17,13 @@
 // P S B S P^T in place of P B P^T, so the preconditioner stays symmetric and linear in
 // the residual.  A zero network is the plain cycle.
 //
-// Training minimises ||r - A M(r)||^2 / ||r||^2 over residuals the solver recorded.  With
-// q = r - A z, z = T u + (terms without s) for the post-smoothing T, u = x + w_c P g and
-// g = S B S y (y the restricted residual):
-//   dL/dg = -2 w_c P^T T^T A q / ||r||^2
+// Training minimises ||r - A M(r)||^2 / ||r||^2 over residuals the solver recorded, or
+// the energy-norm factor (e0 - z)^T A (e0 - z) / e0^T A e0 with z = M(A e0) over the
+// cycle's slowest modes e0 (recorded residuals after a few preconditioned Richardson
+// steps).  With q = r - A z (r = A e0 for the energy form), z = T u + (terms without s)
+// for the post-smoothing T, u = x + w_c P g and g = S B S y (y the restricted residual):
+//   dL/dg = -2 w_c P^T T^T A q / ||r||^2     (residual form)
+//   dL/dg = -2 w_c P^T T^T q / e0^T A e0      (energy form: A e1 = q)
 //   dL/ds = dL/dg (.) (B S y) + y (.) B (S dL/dg)     (B symmetric)
 //   dL/do = dL/ds (.) s / 2  where |o| < kCwMax.
 #include "pf_internal.h"
@@ -208,6 +211,23 @@ __global__ void __launch_bounds__(256) k_lpc_cw_grad(long long n, const float* _
                                                      const float* __restrict__ bw, float* __restrict__ gs) {
   for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256)
     gs[q] = fmaf(dg[q], h[q], y[q] * bw[q]);
+}
+
+// partial sums over the list for the energy objective: e.q, z.q and e.r (slots 0..2)
+__global__ void __launch_bounds__(256) k_lpc_dots3(VecArgs a, const float* __restrict__ e, const float* __restrict__ q,
+                                                   const float* __restrict__ z, const float* __restrict__ r) {
+  const int lane = threadIdx.x & 63;
+  double eq = 0.0, zq = 0.0, er = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    const long long c = b * kBlockVol + lane;
+    const double qc = q[c], ec = e[c];
+    eq += ec * qc;
+    zq += (double)z[c] * qc;
+    er += ec * (double)r[c];
+  }
+  double v[3] = {eq, zq, er};
+  const bool mx[3] = {false, false, false};
+  block_reduce_store<3>(v, mx, a.part, kMaxPartials);
 }
 
 __global__ void __launch_bounds__(256) k_lpc_mul(long long n, const float* __restrict__ a, const float* __restrict__ b,

@@ -148,10 +148,29 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
 
   uint32_t last = 0;
   bool first = true;
+  // the particle data of the next iteration are loaded while this one computes (for
+  // cells of at most kOrdMax particles, whose order is the precomputed ranking)
+  const bool piped = cnt <= (uint32_t)kOrdMax;
+  const float* vp = AX == 0 ? a.P.u : (AX == 1 ? a.P.v : a.P.w);
+  float nx_ = 0.f, ny_ = 0.f, nz_ = 0.f, nv_ = 0.f, nvol_ = 1.f;
+  uint8_t nty_ = 0;
+  if (piped && cnt > 0) {
+    const uint32_t q0 = s0 + ord[0];
+    nx_ = a.P.x[q0]; ny_ = a.P.y[q0]; nz_ = a.P.z[q0]; nv_ = vp[q0]; nty_ = a.P.type[q0];
+    if (a.has_vol) nvol_ = a.P.vol[q0];
+  }
   for (uint32_t it = 0; it < cnt; ++it) {
     uint32_t q;
-    if (cnt <= (uint32_t)kOrdMax) {
+    float px, py, pz, vel, pvol = 1.f;
+    uint8_t ty;
+    if (piped) {
       q = s0 + ord[it];
+      px = nx_; py = ny_; pz = nz_; vel = nv_; ty = nty_; pvol = nvol_;
+      if (it + 1 < cnt) {
+        const uint32_t qn = s0 + ord[it + 1];
+        nx_ = a.P.x[qn]; ny_ = a.P.y[qn]; nz_ = a.P.z[qn]; nv_ = vp[qn]; nty_ = a.P.type[qn];
+        if (a.has_vol) nvol_ = a.P.vol[qn];
+      }
     } else {  // selection by ascending caller index
       uint32_t best = 0xffffffffu, bi = 0;
       for (uint32_t t = 0; t < cnt; ++t) {
@@ -161,6 +180,8 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
       last = best;
       first = false;
       q = s0 + bi;
+      px = a.P.x[q]; py = a.P.y[q]; pz = a.P.z[q]; vel = vp[q]; ty = a.P.type[q];
+      if (a.has_vol) pvol = a.P.vol[q];
     }
     // stencil coordinates of this axis' face grid (sparse_ops3d_tp.cpp:117-122) and
     // the particle's offsets from the lane's cell
@@ -171,7 +192,7 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
       // rounded FP32 subtraction of exact values (sx - 0.5 is inexact only for
       // sx < 0.25, in cell 0): the FP64 path's single rounding, bit for bit.  The
       // FP64 coordinates (exact) are used only by the guard-band path.
-      const float sx = a.P.x[q] * a.inv_dx_f, sy = a.P.y[q] * a.inv_dx_f, sz = a.P.z[q] * a.inv_dx_f;
+      const float sx = px * a.inv_dx_f, sy = py * a.inv_dx_f, sz = pz * a.inv_dx_f;
       fx = (AX == 0 ? sx : sx - 0.5f) - (float)i;
       fy = (AX == 1 ? sy : sy - 0.5f) - (float)j;
       fz = (AX == 2 ? sz : sz - 0.5f) - (float)k;
@@ -179,16 +200,15 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
       gy = AX == 1 ? (double)sy : (double)sy - 0.5;
       gz = AX == 2 ? (double)sz : (double)sz - 0.5;
     } else {
-      double px = grid_coord(a.P.x[q], a.g.ox, a), py = grid_coord(a.P.y[q], a.g.oy, a),
-             pz = grid_coord(a.P.z[q], a.g.oz, a);
-      gx = AX == 0 ? px : px - 0.5;
-      gy = AX == 1 ? py : py - 0.5;
-      gz = AX == 2 ? pz : pz - 0.5;
+      double dpx = grid_coord(px, a.g.ox, a), dpy = grid_coord(py, a.g.oy, a), dpz = grid_coord(pz, a.g.oz, a);
+      gx = AX == 0 ? dpx : dpx - 0.5;
+      gy = AX == 1 ? dpy : dpy - 0.5;
+      gz = AX == 2 ? dpz : dpz - 0.5;
       fx = (float)(gx - (double)i);
       fy = (float)(gy - (double)j);
       fz = (float)(gz - (double)k);
     }
-    const float vel = AX == 0 ? a.P.u[q] : (AX == 1 ? a.P.v[q] : a.P.w[q]);
+    (void)q;
     if (KT == 1) {
       float wt[A::NN];
       trilinear_weights<AX>(gx, gy, gz, i, j, k, vx, vy, vz, wt, fbits);
@@ -225,8 +245,10 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
           Axy[ox + A::NX * oy] = (vx[ox] && vy[oy]) ? 1.f - (dx2[ox] + dy2[oy]) * (1.f / 2.25f) : -4.f;
 #pragma unroll
       for (int oz = 0; oz < A::NZ; ++oz) Bz[oz] = vz[oz] ? dz2[oz] * (1.f / 2.25f) : 4.f;
+      // (a fast-path node is activated by its positive weight: w = t^3 >= 1e-15 where
+      // t > 0 here, so the lane's accumulated mass there is positive -- the
+      // activation bits of these nodes are read from the sums after the loop)
       float tmin = 4.f;
-      uint32_t lo = 0, hi = 0;
 #pragma unroll
       for (int oz = 0; oz < A::NZ; ++oz)
 #pragma unroll
@@ -240,11 +262,8 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
             const float w = tp * tp * tp;
             wt[n] = w;
             wsum += w;
-            const uint32_t bit = t > 0.f ? 1u : 0u;
-            if (n < 32) lo |= bit << n; else hi |= bit << (n - 32);
           }
       slow = tmin < kGuardT;
-      if (!slow) fbits |= ((uint64_t)hi << 32) | lo;
     }
     if (slow) {
       wsum = 0.f;
@@ -274,9 +293,8 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
             if (act) fbits |= (1ull << n);
           }
     }
-    uint8_t ty = a.P.type[q];
     float mp = (ty == 0) ? a.mp_l : a.mp_g;
-    if (a.has_vol) mp *= a.P.vol[q];
+    if (a.has_vol) mp *= pvol;
     float sm = wsum > 0.f ? mp / wsum : 0.f;
     float sq = sm * vel;
 #pragma unroll
@@ -284,6 +302,11 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
       am[n] = fmaf(wt[n], sm, am[n]);
       aq[n] = fmaf(wt[n], sq, aq[n]);
     }
+  }
+  if (KT == 0) {
+#pragma unroll
+    for (int n = 0; n < A::NN; ++n)
+      if (am[n] > 0.f) fbits |= 1ull << n;
   }
   // deposit into the LDS region: for fixed n the 64 lanes address 64 distinct nodes
   const int base = lx + A::RX * (ly + A::RY * lz);

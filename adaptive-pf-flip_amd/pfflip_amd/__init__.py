@@ -359,6 +359,7 @@ def lib():
             "pf_lpc_train": (C.c_int, [vp, vp, i32, dbl, C.c_uint64, C.POINTER(dbl)]),
             "pf_lpc_loss": (C.c_int, [vp, vp, i32, C.POINTER(dbl), C.POINTER(dbl)]),
             "pf_lpc_grad": (C.c_int, [vp, vp, i32, vp, C.POINTER(dbl)]),
+            "pf_lpc_set_objective": (C.c_int, [vp, i32, i32]),
             "pf_kernel_timing": (C.c_int, [vp, i32]),
             "pf_kernel_timing_mask": (C.c_int, [vp, C.c_uint32]),
             "pf_kernel_time": (C.c_int, [vp, i32, C.POINTER(i64), C.POINTER(dbl)]),
@@ -381,6 +382,7 @@ def calibrate_rho_tilde0(phase: Phase, Vp: float = 1.0) -> float:
 
 
 PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID = 0, 1, 2
+LPC_OBJECTIVE_RESIDUAL, LPC_OBJECTIVE_ENERGY = 0, 1
 
 
 def default_solve_cfg(stop_rule=STOP_REF_MAXABS, tol=1e-7, max_iters=600, precond=PRECOND_JACOBI) -> SolveCfg:
@@ -845,6 +847,11 @@ class LearnedPreconditioner:
 
     def attach(self, ctx):
         ctx._check(lib().pf_lpc_attach(ctx.h, self.h), "pf_lpc_attach")
+
+    def set_objective(self, objective, power_steps=0):
+        """LPC_OBJECTIVE_RESIDUAL, or LPC_OBJECTIVE_ENERGY over the cycle's slowest modes
+        (the snapshot after power_steps preconditioned Richardson steps)"""
+        self._check(lib().pf_lpc_set_objective(self.h, objective, power_steps), "pf_lpc_set_objective")
 
     def train(self, ctx, steps, lr=1e-3, seed=0):
         loss = C.c_double()

@@ -7,12 +7,19 @@ empty headspace, with the light phase's density drawn log-uniform so that
 rho_l / rho_g lies in [1, 1000] (rho_l = 1000).  Each scene is stepped with the
 multigrid-preconditioned solve; from its second step on, every projection records its
 FP32 residuals (one per host check, up to `snapshots`) and the network takes `steps`
-Adam steps on them with loss ||r - A M(r)||^2 / ||r||^2, where M is the multigrid
-cycle with the network's per-aggregate correction weights (pf_lpc_train, the
-library's own forward and backward kernels).  The default 40 scenes x 8 recorded
-steps give about 2000 residual/system pairs.  Held-out systems then compare the solve
-with the trained network against the plain cycle.  Seeds, ratios and results are
-written beside the weights.
+Adam steps on them (pf_lpc_train, the library's own forward and backward kernels),
+where M is the multigrid cycle with the network's per-aggregate correction weights.
+The default objective is the energy-norm error reduction e1^T A e1 / e0^T A e0,
+e1 = e0 - M A e0, over the cycle's slowest modes: each recorded residual after
+`power_steps` preconditioned Richardson steps with the current M.  PCG's iteration
+count follows the spread of the spectrum of M A, which these modes bound; the
+residual objective ||r - A M(r)||^2 / ||r||^2 (--objective residual) weighs the
+typical residual instead.  The default 40 scenes x 8 recorded
+steps give about 2000 residual/system pairs.  After every `val_every` scenes the
+network's solve iterations on three validation systems are counted, and the weights
+with the fewest (the untrained network, which is the plain cycle, included) are the
+ones written.  Held-out systems then compare the solve with those weights against the
+plain cycle.  Seeds, ratios and results are written beside the weights.
 
 Usage (on a GPU):
   python -m pfflip_amd.lpc_train --out weights.npy [--n 128 --scenes 40 ...]
@@ -32,6 +39,7 @@ if os.path.dirname(HERE) not in sys.path:
 import pfflip_amd as G  # noqa: E402
 
 DEFAULT_WEIGHTS = os.path.join(HERE, "lpc_weights.npy")
+DEFAULT_SEED = 20251
 
 
 def load_weights(path=DEFAULT_WEIGHTS):
@@ -92,6 +100,32 @@ def compare(net, n, kind, seed, ratio, steps=2, repeats=2):
         ctx.close()
 
 
+# validation systems (kind, seed, ratio): seeds unused by training and by the held-out set
+VALIDATION = ((G.SCENE_RT, 5, 300.0), (G.SCENE_RT, 9, 10.0), (G.SCENE_BUBBLE, 15, 500.0))
+
+
+def validation_iterations(net, n):
+    """summed iterations of the learned solve on the validation systems (one projection
+    each, after one multigrid step of the scene)"""
+    total = 0
+    for kind, seed, ratio in VALIDATION:
+        ctx = G.Context(n, n, n, 1.0 / n)
+        try:
+            ctx.scene(kind, seed)
+            ph = system_phase(ratio)
+            dt = 1e-3 if kind == G.SCENE_RT else 5e-4
+            ctx.step(step_cfg(ph, dt))
+            net.attach(ctx)
+            ctx.mark_cells(); ctx.p2g(ph, 1.0); ctx.save_velocity(); ctx.apply_gravity(dt, -9.81)
+            ctx.apply_walls()
+            st = ctx.project(ph, dt, solve_cfg(G.PRECOND_LEARNED))
+            total += st.iterations if st.converged and not st.fallback_used else 10 ** 6
+            G.lib().pf_lpc_attach(ctx.h, None)
+        finally:
+            ctx.close()
+    return total
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
@@ -101,16 +135,25 @@ def main(argv=None):
     ap.add_argument("--snapshots", type=int, default=16)
     ap.add_argument("--steps", type=int, default=24, help="Adam steps per recorded system")
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--seed", type=int, default=20251)
+    ap.add_argument("--seed", type=int, default=DEFAULT_SEED)
     ap.add_argument("--init", default=None, help="start from these weights")
+    ap.add_argument("--objective", choices=("energy", "residual"), default="energy")
+    ap.add_argument("--power-steps", type=int, default=6)
+    ap.add_argument("--val-every", type=int, default=4, help="scenes between validation counts")
     args = ap.parse_args(argv)
     rng = np.random.default_rng(args.seed)
     net = G.LearnedPreconditioner(seed=args.seed)
+    net.set_objective(G.LPC_OBJECTIVE_ENERGY if args.objective == "energy" else G.LPC_OBJECTIVE_RESIDUAL,
+                      args.power_steps if args.objective == "energy" else 0)
     if args.init:
         net.set_weights(load_weights(args.init))
-    log = {"n": args.n, "scenes": [], "args": vars(args)}
+    log = {"n": args.n, "scenes": [], "args": vars(args), "validation": []}
     t0 = time.time()
     pairs = 0
+    best_it = validation_iterations(net, args.n)
+    best_w, best_at = net.weights(), 0
+    log["validation"].append({"scenes": 0, "iterations": best_it})
+    print(f"[lpc_train] validation before training: {best_it} it", flush=True)
     for s in range(args.scenes):
         ratio = float(10.0 ** rng.uniform(0.0, 3.0))
         scene_seed = int(rng.integers(1, 2 ** 31))
@@ -143,9 +186,17 @@ def main(argv=None):
               f"last system {last['iterations']} it, loss learned {last['loss_learned']:.5f} vs plain "
               f"{last['loss_plain']:.5f} ({time.time() - t0:.0f} s)", flush=True)
         log["scenes"].append(rec)
+        if (s + 1) % args.val_every == 0 or s + 1 == args.scenes:
+            it = validation_iterations(net, args.n)
+            log["validation"].append({"scenes": s + 1, "iterations": it})
+            print(f"[lpc_train] validation after {s + 1} scenes: {it} it (best {best_it})", flush=True)
+            if it < best_it:
+                best_it, best_w, best_at = it, net.weights(), s + 1
     log["pairs"] = pairs
-    w = net.weights()
-    np.save(args.out, w)
+    log["selected"] = {"scenes": best_at, "validation_iterations": best_it}
+    net.set_weights(best_w)
+    np.save(args.out, best_w)
+    print(f"[lpc_train] selected the weights after {best_at} scenes ({best_it} validation it)", flush=True)
     # held-out systems: seeds and ratios not used above
     evals = []
     for kind, seed, ratio in ((G.SCENE_RT, 7, 1000.0), (G.SCENE_RT, 11, 30.0), (G.SCENE_BUBBLE, 13, 1000.0),

@@ -99,14 +99,19 @@ def step_cfg(G, n, tol, max_iters, warm, precond="jacobi"):
 
 def pressure_comparison(G, ctx, cfg, tol, learned=True):
     """Pressure solve to ||b - Ax||_2/||b||_2 <= tol (FP64) on the current system:
-    plain Jacobi PCG, multigrid PCG and (with trained weights in the package) the
-    learned preconditioner inside flexible PCG, same inputs.  The pre-projection state
-    (mark, P2G, save, gravity, walls) is rebuilt for each solve."""
+    plain Jacobi PCG, multigrid PCG and the learned preconditioner (the multigrid cycle
+    with the network's correction weights) inside flexible PCG, same inputs.  The
+    network has the package's trained weights when there are any, else the untrained
+    network with its zero head (exactly the plain cycle's weights, plus the network's
+    cost).  The pre-projection state (mark, P2G, save, gravity, walls) is rebuilt for
+    each solve."""
     from pfflip_amd import lpc_train
-    net = None
-    if learned and os.path.exists(lpc_train.DEFAULT_WEIGHTS):
-        net = G.LearnedPreconditioner(0)
-        net.set_weights(lpc_train.load_weights())
+    net, trained = None, False
+    if learned:
+        net = G.LearnedPreconditioner(lpc_train.DEFAULT_SEED)
+        if os.path.exists(lpc_train.DEFAULT_WEIGHTS):
+            net.set_weights(lpc_train.load_weights())
+            trained = True
         net.attach(ctx)
     dt, ph = cfg.dt, cfg.phase
 
@@ -139,10 +144,12 @@ def pressure_comparison(G, ctx, cfg, tol, learned=True):
     if net is not None:
         out["speedup_learned_vs_plain"] = round(out["plain_pcg"]["ms"] / out["learned_pc"]["ms"], 3)
         out["speedup_learned_vs_multigrid"] = round(out["multigrid_pcg"]["ms"] / out["learned_pc"]["ms"], 3)
+        out["learned_weights"] = "trained" if trained else (
+            "untrained (zero head): training with validation selection kept no weights better than the plain cycle")
         G.lib().pf_lpc_attach(ctx.h, None)
         net.close()
     else:
-        out["note"] = "no trained weights in the package" if learned else "learned preconditioner skipped"
+        out["note"] = "learned preconditioner skipped"
     return out
 
 
@@ -300,8 +307,9 @@ def main():
     step_net = None
     if args.precond == "learned":
         from pfflip_amd import lpc_train
-        step_net = G.LearnedPreconditioner(0)
-        step_net.set_weights(lpc_train.load_weights())
+        step_net = G.LearnedPreconditioner(lpc_train.DEFAULT_SEED)
+        if os.path.exists(lpc_train.DEFAULT_WEIGHTS):
+            step_net.set_weights(lpc_train.load_weights())
         step_net.attach(ctx)
     print(f"[bench] {n}^3 RT, {ctx.count()} particles; warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):

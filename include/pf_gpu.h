@@ -559,6 +559,14 @@ int pf_lpc_loss(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, double* loss_learned
 /* the gradient of that loss w.r.t. the flat weights (grad[pf_lpc_num_weights()]) and
  * the loss, on one recorded snapshot */
 int pf_lpc_grad(pf_lpc* lpc, pf_ctx* ctx, int32_t snapshot, float* grad, double* loss);
+/* the training objective of pf_lpc_train / pf_lpc_loss / pf_lpc_grad.
+ * PF_LPC_OBJECTIVE_RESIDUAL (default): ||r - A M r||^2 / ||r||^2 with r the snapshot.
+ * PF_LPC_OBJECTIVE_ENERGY: the energy-norm error reduction of one preconditioned
+ * Richardson step, e1^T A e1 / e0^T A e0 with e1 = e0 - M A e0, where e0 is the
+ * snapshot after power_steps such steps with the same M (0..64): the slowest modes of
+ * the cycle, whose reduction bounds the PCG iteration count. */
+enum { PF_LPC_OBJECTIVE_RESIDUAL = 0, PF_LPC_OBJECTIVE_ENERGY = 1 };
+int pf_lpc_set_objective(pf_lpc* lpc, int32_t objective, int32_t power_steps);
 
 /* ---- multigrid preconditioner (no reference counterpart; PF_PRECOND_MULTIGRID) */
 /* the built-in cycle (2 Chebyshev sweeps on [1/3, 2], 32 coarsest sweeps at omega

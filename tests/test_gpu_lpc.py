@@ -119,10 +119,14 @@ def _bf16_grid_head(net, rng, scale=4):
     return w
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("objective", ["residual", "energy"])
+def test_gradient_matches_finite_differences(objective):
     # the adjoint through the cycle (post-smoothing, restriction, the scaled coarse
-    # solve and the network) against central differences of the loss itself
+    # solve and the network) against central differences of the loss itself; the
+    # energy form without power steps, so that e0 (the snapshot) does not depend on
+    # the weights
     ctx, ph, net = _recorded(32, seed=4)
+    net.set_objective(G.LPC_OBJECTIVE_ENERGY if objective == "energy" else G.LPC_OBJECTIVE_RESIDUAL, 0)
     rng = np.random.default_rng(7)
     w = _bf16_grid_head(net, rng)
     for k in (0, G.lpc_snapshot_count(ctx) - 1):
@@ -140,7 +144,7 @@ def test_gradient_matches_finite_differences():
             lm, _ = net.loss(ctx, k)
             net.set_weights(w)
             fd = (lp - lm) / (2 * eps)
-            print(f"[lpc] snapshot {k} head {j}: grad {g[idx]:.6e} fd {fd:.6e}", flush=True)
+            print(f"[lpc] {objective} snapshot {k} head {j}: grad {g[idx]:.6e} fd {fd:.6e}", flush=True)
             assert abs(fd - g[idx]) <= 0.03 * max(abs(fd), abs(g[idx])) + 1e-7
         # a direction over the third layer: BF16 rounding of the perturbed weights
         # moves the effective direction by up to ~20% of d, and loss differences below
@@ -156,7 +160,7 @@ def test_gradient_matches_finite_differences():
         net.set_weights(w)
         fd = (lp - lm) / 2
         gd = float(np.dot(g.astype(np.float64), d.astype(np.float64)))
-        print(f"[lpc] snapshot {k} layer-3 direction: grad.d {gd:.6e} fd {fd:.6e}", flush=True)
+        print(f"[lpc] {objective} snapshot {k} layer-3 direction: grad.d {gd:.6e} fd {fd:.6e}", flush=True)
         assert abs(fd - gd) <= 0.25 * max(abs(fd), abs(gd)) + 1e-6
 
 
@@ -176,3 +180,12 @@ def test_training_lowers_the_loss():
     _prepare(ctx, ph)
     st = ctx.project(ph, 1e-3, _cfg(G.PRECOND_LEARNED))
     assert st.converged == 1 and st.true_rel_residual <= 1e-6 and st.fallback_used == 0
+
+
+def test_objective_rejects_invalid_values():
+    net = G.LearnedPreconditioner(seed=1)
+    for obj, p in ((2, 0), (-1, 0), (G.LPC_OBJECTIVE_ENERGY, 65), (G.LPC_OBJECTIVE_ENERGY, -1)):
+        with pytest.raises(G.PfError):
+            net.set_objective(obj, p)
+    net.set_objective(G.LPC_OBJECTIVE_ENERGY, 64)
+    net.close()

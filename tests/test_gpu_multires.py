@@ -65,15 +65,33 @@ def test_markers_and_p2g_match_fp64_oracle():
         assert rel_err(mom, g.dense(fv) * g.dense(fv + 3)) < FIELD_TOL
 
 
+def _explain(ctx, g, ps, f):
+    """what differs, for the failure message: the mismatching faces, the markers and
+    the binned particle counts per cell"""
+    got, want = ctx.field(f), g.dense(f)
+    bad = np.argwhere(np.abs(got - want) > 1e-5 * max(np.abs(want).max(), 1e-30))
+    counts = np.zeros((ctx.nz, ctx.ny, ctx.nx), np.int64)
+    cell = np.floor(np.stack([ps.z, ps.y, ps.x], 1)).astype(np.int64)
+    np.add.at(counts, tuple(cell.T), 1)
+    return (f"field {f}: {len(bad)} faces differ, first (k,j,i) {bad[:8].tolist()}; "
+            f"markers equal {np.array_equal(ctx.field(G.FIELD_MARKER), g.dense(O.MK))}; "
+            f"binned counts equal {np.array_equal(ctx.bin_counts(), counts)}; "
+            f"particles {ctx.count()} of {len(ps.x)}")
+
+
 def test_projection_matches_fp64_oracle():
     n, dt = 16, 0.02
     ph, gph = _phase()
     leaves = O.mr_layout_bubble_band(n, n, n)
     ctx, g, ps = _ctx(n, leaves, _bubble_particles(n))
-    ctx.mr_mark_cells(); ctx.mr_p2g(gph, 1.0); ctx.save_velocity(); ctx.apply_gravity(dt, -9.81); ctx.apply_walls()
-    g.mark_cells(ps); g.p2g(ps, ph, 1.0); g.apply_gravity(dt, -9.81); g.apply_walls()
+    ctx.mr_mark_cells(); ctx.mr_p2g(gph, 1.0)
+    g.mark_cells(ps); g.p2g(ps, ph, 1.0)
+    for f in range(6):
+        assert rel_err(ctx.field(f), g.dense(f)) < FIELD_TOL, _explain(ctx, g, ps, f)
+    ctx.save_velocity(); ctx.apply_gravity(dt, -9.81); ctx.apply_walls()
+    g.apply_gravity(dt, -9.81); g.apply_walls()
     for f in range(3):
-        assert rel_err(ctx.field(f), g.dense(f)) < FIELD_TOL
+        assert rel_err(ctx.field(f), g.dense(f)) < FIELD_TOL, _explain(ctx, g, ps, f)
     st = ctx.mr_project(gph, dt, G.mr_solve_cfg(20000, 1e-11, 0.0))
     ost = g.project(ph, dt, O.mr_solve_cfg(max_iterations=20000, abs_tol=1e-12, rel_tol=0.0))
     assert st.converged == 1 and st.true_max_residual < 1e-11
