@@ -128,12 +128,16 @@ int run_p2g_splat(pf_ctx* c, const pf_phase* ph, double Vp, bool single = false,
     a.list = c->color_list + (long long)col * c->color_cap;
     a.count = c->color_cnt + col;
     const dim3 grid((unsigned)std::max<long long>(c->color_cap, 1));
+    const P2GShared sh = p2g_shared_args(a);
+    const P2GField f0 = p2g_field_args(a, 0), f1 = p2g_field_args(a, 1), f2 = p2g_field_args(a, 2);
     if (mr)
-      TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<2>, grid, dim3(64), 0, c->stream, a));
+      TIMED(c, PF_K_P2G, hipLaunchKernelGGL((k_p2g_color<2, false>), grid, dim3(64), 0, c->stream, sh, f0, f1, f2));
     else if (single)
-      TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<1>, grid, dim3(64), 0, c->stream, a));
+      TIMED(c, PF_K_P2G, hipLaunchKernelGGL((k_p2g_color<1, false>), grid, dim3(64), 0, c->stream, sh, f0, f1, f2));
+    else if (a.fast32)
+      TIMED(c, PF_K_P2G, hipLaunchKernelGGL((k_p2g_color<0, true>), grid, dim3(64), 0, c->stream, sh, f0, f1, f2));
     else
-      TIMED(c, PF_K_P2G, hipLaunchKernelGGL(k_p2g_color<0>, grid, dim3(64), 0, c->stream, a));
+      TIMED(c, PF_K_P2G, hipLaunchKernelGGL((k_p2g_color<0, false>), grid, dim3(64), 0, c->stream, sh, f0, f1, f2));
     LAUNCH_CHECK(c);
   }
   return PF_OK;

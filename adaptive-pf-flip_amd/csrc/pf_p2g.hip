@@ -9,17 +9,19 @@
 // MI355X mapping:
 // * One workgroup = one 64-lane wavefront = one 4^3 source block of cells; lane l
 //   owns cell l of the block and splats that cell's particles (they are contiguous
-//   after binning).  For a particle in cell c the non-zero weights of a face field
-//   lie in c-1..c+2 along the face axis and c-1..c+1 across it (the reference's
-//   6-wide loop box adds only zero weights), so a lane keeps its 36 node sums per
-//   axis in registers and a block's contributions stay inside a 7x6x6 node region.
-// * Region sums are accumulated in LDS: for a fixed stencil offset the 64 lanes hit
-//   64 distinct nodes, and one wave's LDS operations complete in program order, so
-//   the sum order is fixed -- no global atomics, and the result does not change
-//   from run to run.  A cell's particles are visited in ascending caller index.
+//   after binning) in ascending caller index.  For a particle in cell c the non-zero
+//   weights of a face field lie in c-1..c+2 along the face axis and c-1..c+1 across
+//   it (the reference's 6-wide loop box adds only zero weights), so a lane keeps its
+//   36 node sums per axis in registers and a block's contributions stay inside a
+//   7x6x6 node region.  The three face axes are splatted in turn in one launch.
+// * A lane's node sums go into the region in LDS one stencil offset at a time: for a
+//   fixed offset the 64 lanes hit 64 distinct nodes, and one wave's LDS operations
+//   execute in program order, so a plain read-add-write per offset sums every node in
+//   a fixed order -- no atomics there, and the result does not change from run to run.
 // * Regions of two source blocks of equal color8 (src/grid/block_color.h:7) are at
-//   least 8 cells apart and never overlap, so each of the 8 colour launches adds its
-//   regions into HBM with plain read-modify-write.
+//   least 8 cells apart and never overlap, so within one of the 8 colour launches a
+//   node takes at most one addition from a region: a no-return L2 float atomic adds it
+//   into HBM in the same order as a read-modify-write would, without the wave waiting.
 // * The activation predicate kern(d^2) > 0 is d^2 < 2.25 (exact: d^2/2.25 < 1 in
 //   correctly rounded FP64 iff d^2 < 2.25).  It is evaluated in FP32 and recomputed
 //   in FP64 inside a 1e-5 band around 2.25, so active-block sets match the FP64
@@ -49,9 +51,71 @@ struct P2GArgs {
   const uint8_t* valid[3];  // KT = 2 (multires): the enumerated fine faces per axis
 };
 
+// The launch's arguments: what all three face axes share (grid, block list, particle
+// arrays and constants) and, per face axis, its field's pointers -- each particle pointer
+// once, so the kernel's scalar registers hold one copy
+struct P2GShared {
+  Geom g;
+  Dims dc;
+  const uint32_t* start;
+  const uint32_t* list;
+  const uint32_t* count;
+  const float *x, *y, *z, *vol;
+  const uint8_t* type;
+  const uint32_t* id;
+  double inv_dx;
+  float inv_dx_f;
+  float mp_l, mp_g;
+  bool has_vol, pow2, fast32;
+};
+
+struct P2GField {
+  Dims df;
+  float* mom;
+  float* mass;
+  uint8_t* act_vel;
+  uint8_t* act_mass;
+  const float* vel;
+  const uint8_t* valid;
+};
+
+inline P2GShared p2g_shared_args(const P2GArgs& a) {
+  P2GShared r;
+  r.g = a.g;
+  r.dc = a.dc;
+  r.start = a.start;
+  r.list = a.list;
+  r.count = a.count;
+  r.x = a.P.x; r.y = a.P.y; r.z = a.P.z;
+  r.vol = a.P.vol;
+  r.type = a.P.type;
+  r.id = a.P.id;
+  r.inv_dx = a.inv_dx;
+  r.inv_dx_f = a.inv_dx_f;
+  r.mp_l = a.mp_l; r.mp_g = a.mp_g;
+  r.has_vol = a.has_vol; r.pow2 = a.pow2; r.fast32 = a.fast32;
+  return r;
+}
+
+inline P2GField p2g_field_args(const P2GArgs& a, int ax) {
+  P2GField r;
+  r.df = a.df[ax];
+  r.mom = a.mom[ax];
+  r.mass = a.mass[ax];
+  r.act_vel = a.act_vel[ax];
+  r.act_mass = a.act_mass[ax];
+  r.vel = ax == 0 ? a.P.u : (ax == 1 ? a.P.v : a.P.w);
+  r.valid = a.valid[ax];
+  return r;
+}
+
 constexpr int kOrdMax = 16;
 constexpr float kGuard = 1e-5f;
 constexpr float kGuardT = 1e-5f;  // on t = 1 - d^2/2.25: wider than kGuard / 2.25
+constexpr float kOutside = 1e4f;  // d^2 penalty of a stencil row outside the field
+// the smallest particle mass of the fast path: w * m_p / wsum >= 1e-15 * 1e-20 / 64
+// stays a normal FP32 number
+constexpr float kMinFastMass = 1e-20f;
 
 template <int AX>
 struct Ax {
@@ -65,7 +129,7 @@ struct Ax {
   static constexpr int RN = RX * RY * RZ;  // 252
 };
 
-__device__ inline double grid_coord(float x, double o, const P2GArgs& a) {
+__device__ inline double grid_coord(float x, double o, const P2GShared& a) {
   double d = (double)x - o;
   return a.pow2 ? d * a.inv_dx : d / a.g.dx;
 }
@@ -111,18 +175,76 @@ __device__ inline void trilinear_weights(double gx, double gy, double gz, int i,
       }
 }
 
-template <int AX, int KT>
-__device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_t s0, uint32_t cnt,
-                                  const uint8_t* ord, float* s_mom, float* s_mass, uint8_t* s_flag,
+// One particle's weights by the per-node path: the FP32 d^2 with the FP64 predicate and
+// weight inside the guard band (exact_weight), the multires face mask, and the
+// explicit activation bits.  dx2/dy2/dz2 carry the out-of-field penalties.
+template <int AX>
+__device__ inline float node_weights(const float* dx2, const float* dy2, const float* dz2, double gx, double gy,
+                                     double gz, int i, int j, int k, uint64_t vbits, float* wt, uint64_t& fbits) {
+  using A = Ax<AX>;
+  float wsum = 0.f;
+#pragma unroll
+  for (int oz = 0; oz < A::NZ; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < A::NY; ++oy)
+#pragma unroll
+      for (int ox = 0; ox < A::NX; ++ox) {
+        const int n = ox + A::NX * (oy + A::NY * oz);
+        float d2 = dx2[ox] + dy2[oy] + dz2[oz];
+        float w;
+        bool act;
+        if (fabsf(d2 - 2.25f) < kGuard) {
+          w = exact_weight(gx - (double)(i + ox - 1), gy - (double)(j + oy - 1), gz - (double)(k + oz - 1), act);
+        } else {
+          act = d2 < 2.25f;
+          float t = 1.f - d2 * (1.f / 2.25f);
+          w = act ? t * t * t : 0.f;
+        }
+        // (a node outside the field has d2 >= kOutside: inactive, never in the band)
+        act = act && ((vbits >> n) & 1ull);
+        w = act ? w : 0.f;
+        wt[n] = w;
+        wsum += w;
+        if (act) fbits |= (1ull << n);
+      }
+  return wsum;
+}
+
+// A lane splats its cell's particles (in ascending caller index) into its 36 node sums
+// of one face axis, then adds them into the block's LDS region.
+//
+// KT = 0 runs two loops.  The first covers cells of at most kOrdMax particles in the
+// precomputed rank order, with the next particle's data loaded while this one
+// computes, and evaluates the cubic in the separable FP32 form
+// t = (1 - (dx^2 + dy^2)/2.25) - dz^2/2.25, w = max(t, 0)^3 -- one subtraction per
+// node and no branch.  A particle with a node at |t| < kGuardT (its d^2 within about
+// 1e-5 of 2.25), or with a mass too small for its positive weights to leave a positive
+// sum, is only marked there.  The second loop takes the marked particles and every
+// particle of a fuller cell (selected by ascending caller index) through node_weights.
+// Keeping the rare paths out of the first loop keeps its registers to what it needs.
+// KT = 2 (multires, enumerated faces only) and KT = 1 (trilinear) use the second loop
+// for every particle.
+//
+// F32: origin 0, a power-of-two dx and fewer than 2^22 cells a side, where x/dx is
+// exact in FP32 and each offset from the lane's cell is either exact or one correctly
+// rounded FP32 subtraction of exact values (sx - 0.5 is inexact only for sx < 0.25, in
+// cell 0): the FP64 path's single rounding, bit for bit.  Otherwise the offsets come
+// from FP64 coordinates.
+template <int AX, int KT, bool F32>
+__device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, int j, int k, uint32_t s0,
+                                  uint32_t cnt, const uint8_t* ord, float* s_mom, float* s_mass, uint8_t* s_flag,
                                   int lx, int ly, int lz) {
   using A = Ax<AX>;
-  const Dims& fd = a.df[AX];
+  const Dims& fd = f.df;
   float am[A::NN], aq[A::NN];
   uint64_t fbits = 0;
 #pragma unroll
   for (int n = 0; n < A::NN; ++n) { am[n] = 0.f; aq[n] = 0.f; }
 
-  // node validity along each axis (field extents), fixed per lane
+  // node validity along each axis (field extents), fixed per lane, and the same as
+  // additive d^2 penalties held in vector registers: a row or column outside the field
+  // gets d^2 >= kOutside, so its t is far below zero and its weight zero without a
+  // select (lane masks would sit in scalar registers across the node loops)
   bool vx[A::NX], vy[A::NY], vz[A::NZ];
 #pragma unroll
   for (int o = 0; o < A::NX; ++o) { int c = i + o - 1; vx[o] = c >= 0 && c < fd.nx; }
@@ -130,125 +252,92 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
   for (int o = 0; o < A::NY; ++o) { int c = j + o - 1; vy[o] = c >= 0 && c < fd.ny; }
 #pragma unroll
   for (int o = 0; o < A::NZ; ++o) { int c = k + o - 1; vz[o] = c >= 0 && c < fd.nz; }
-  // multires: only the enumerated faces take (and normalise) weights
-  // (collectWeightedFaces, multires_ops3d_tp.cpp:62-75)
-  uint64_t vbits = ~0ull;
-  if (KT == 2) {
-    vbits = 0;
+  float penx[A::NX], peny[A::NY], penz[A::NZ];
 #pragma unroll
-    for (int oz = 0; oz < A::NZ; ++oz)
+  for (int o = 0; o < A::NX; ++o) penx[o] = vx[o] ? 0.f : kOutside;
 #pragma unroll
-      for (int oy = 0; oy < A::NY; ++oy)
+  for (int o = 0; o < A::NY; ++o) peny[o] = vy[o] ? 0.f : kOutside;
 #pragma unroll
-        for (int ox = 0; ox < A::NX; ++ox)
-          if (vx[ox] && vy[oy] && vz[oz] &&
-              a.valid[AX][bm_index(fd, i + ox - 1, j + oy - 1, k + oz - 1)])
-            vbits |= 1ull << (ox + A::NX * (oy + A::NY * oz));
-  }
+  for (int o = 0; o < A::NZ; ++o) penz[o] = vz[o] ? 0.f : kOutside;
+  const float* vp = f.vel;
 
-  uint32_t last = 0;
-  bool first = true;
-  // the particle data of the next iteration are loaded while this one computes (for
-  // cells of at most kOrdMax particles, whose order is the precomputed ranking)
-  const bool piped = cnt <= (uint32_t)kOrdMax;
-  const float* vp = AX == 0 ? a.P.u : (AX == 1 ? a.P.v : a.P.w);
-  float nx_ = 0.f, ny_ = 0.f, nz_ = 0.f, nv_ = 0.f, nvol_ = 1.f;
-  uint8_t nty_ = 0;
-  if (piped && cnt > 0) {
-    const uint32_t q0 = s0 + ord[0];
-    nx_ = a.P.x[q0]; ny_ = a.P.y[q0]; nz_ = a.P.z[q0]; nv_ = vp[q0]; nty_ = a.P.type[q0];
-    if (a.has_vol) nvol_ = a.P.vol[q0];
-  }
-  for (uint32_t it = 0; it < cnt; ++it) {
-    uint32_t q;
-    float px, py, pz, vel, pvol = 1.f;
-    uint8_t ty;
-    if (piped) {
-      q = s0 + ord[it];
-      px = nx_; py = ny_; pz = nz_; vel = nv_; ty = nty_; pvol = nvol_;
-      if (it + 1 < cnt) {
-        const uint32_t qn = s0 + ord[it + 1];
-        nx_ = a.P.x[qn]; ny_ = a.P.y[qn]; nz_ = a.P.z[qn]; nv_ = vp[qn]; nty_ = a.P.type[qn];
-        if (a.has_vol) nvol_ = a.P.vol[qn];
-      }
-    } else {  // selection by ascending caller index
-      uint32_t best = 0xffffffffu, bi = 0;
-      for (uint32_t t = 0; t < cnt; ++t) {
-        uint32_t id = a.P.id[s0 + t];
-        if ((first || id > last) && id <= best) { best = id; bi = t; }
-      }
-      last = best;
-      first = false;
-      q = s0 + bi;
-      px = a.P.x[q]; py = a.P.y[q]; pz = a.P.z[q]; vel = vp[q]; ty = a.P.type[q];
-      if (a.has_vol) pvol = a.P.vol[q];
-    }
-    // stencil coordinates of this axis' face grid (sparse_ops3d_tp.cpp:117-122) and
-    // the particle's offsets from the lane's cell
-    double gx, gy, gz;
-    float fx, fy, fz;
-    if (KT == 0 && a.fast32) {
-      // x/dx is exact in FP32 here, and each offset is either exact or one correctly
-      // rounded FP32 subtraction of exact values (sx - 0.5 is inexact only for
-      // sx < 0.25, in cell 0): the FP64 path's single rounding, bit for bit.  The
-      // FP64 coordinates (exact) are used only by the guard-band path.
+  // offsets of a particle from the lane's cell along this axis' face grid
+  // (sparse_ops3d_tp.cpp:117-122), and the FP64 stencil coordinates when asked
+  auto offsets = [&](float px, float py, float pz, float& fx, float& fy, float& fz) {
+    if (F32) {
       const float sx = px * a.inv_dx_f, sy = py * a.inv_dx_f, sz = pz * a.inv_dx_f;
       fx = (AX == 0 ? sx : sx - 0.5f) - (float)i;
       fy = (AX == 1 ? sy : sy - 0.5f) - (float)j;
       fz = (AX == 2 ? sz : sz - 0.5f) - (float)k;
-      gx = AX == 0 ? (double)sx : (double)sx - 0.5;
-      gy = AX == 1 ? (double)sy : (double)sy - 0.5;
-      gz = AX == 2 ? (double)sz : (double)sz - 0.5;
     } else {
-      double dpx = grid_coord(px, a.g.ox, a), dpy = grid_coord(py, a.g.oy, a), dpz = grid_coord(pz, a.g.oz, a);
-      gx = AX == 0 ? dpx : dpx - 0.5;
-      gy = AX == 1 ? dpy : dpy - 0.5;
-      gz = AX == 2 ? dpz : dpz - 0.5;
-      fx = (float)(gx - (double)i);
-      fy = (float)(gy - (double)j);
-      fz = (float)(gz - (double)k);
+      const double dpx = grid_coord(px, a.g.ox, a), dpy = grid_coord(py, a.g.oy, a), dpz = grid_coord(pz, a.g.oz, a);
+      fx = (float)((AX == 0 ? dpx : dpx - 0.5) - (double)i);
+      fy = (float)((AX == 1 ? dpy : dpy - 0.5) - (double)j);
+      fz = (float)((AX == 2 ? dpz : dpz - 0.5) - (double)k);
     }
-    (void)q;
-    if (KT == 1) {
-      float wt[A::NN];
-      trilinear_weights<AX>(gx, gy, gz, i, j, k, vx, vy, vz, wt, fbits);
-      // mass += (float)(w * 1.0) = w; momentum += (float)(w * v) with v exact in FP64
-#pragma unroll
-      for (int n = 0; n < A::NN; ++n) {
-        am[n] += wt[n];
-        aq[n] += (float)((double)wt[n] * (double)vel);
-      }
-      continue;
+  };
+  auto coords64 = [&](float px, float py, float pz, double& gx, double& gy, double& gz) {
+    double dpx, dpy, dpz;
+    if (F32) {
+      dpx = (double)(px * a.inv_dx_f); dpy = (double)(py * a.inv_dx_f); dpz = (double)(pz * a.inv_dx_f);
+    } else {
+      dpx = grid_coord(px, a.g.ox, a); dpy = grid_coord(py, a.g.oy, a); dpz = grid_coord(pz, a.g.oz, a);
     }
-    float dx2[A::NX], dy2[A::NY], dz2[A::NZ];
+    gx = AX == 0 ? dpx : dpx - 0.5;
+    gy = AX == 1 ? dpy : dpy - 0.5;
+    gz = AX == 2 ? dpz : dpz - 0.5;
+  };
+  auto particle_mass = [&](uint8_t ty, float pvol) {
+    float mp = (ty == 0) ? a.mp_l : a.mp_g;
+    if (a.has_vol) mp *= pvol;
+    return mp;
+  };
+  auto accumulate = [&](const float* wt, float wsum, float mp, float vel) {
+    const float sm = wsum > 0.f ? mp / wsum : 0.f;
+    const float sq = sm * vel;
 #pragma unroll
-    for (int o = 0; o < A::NX; ++o) { float d = fx - (float)(o - 1); dx2[o] = d * d; }
+    for (int n = 0; n < A::NN; ++n) {
+      am[n] = fmaf(wt[n], sm, am[n]);
+      aq[n] = fmaf(wt[n], sq, aq[n]);
+    }
+  };
+
+  const bool ranked = cnt <= (uint32_t)kOrdMax;
+  uint32_t marked = 0;  // ranks (bit it) left to the second loop
+  if (KT == 0 && ranked && cnt > 0) {
+    // the particle of rank r (its data for the next iteration load while this one
+    // computes)
+    auto fetch = [&](uint32_t r, float& px, float& py, float& pz, float& vel, uint8_t& ty, float& pvol) {
+      const uint32_t q = s0 + ord[r];
+      px = a.x[q]; py = a.y[q]; pz = a.z[q]; vel = vp[q]; ty = a.type[q];
+      pvol = a.has_vol ? a.vol[q] : 1.f;
+    };
+    float nx_, ny_, nz_, nv_, nvol_;
+    uint8_t nty_;
+    fetch(0, nx_, ny_, nz_, nv_, nty_, nvol_);
+    for (uint32_t it = 0; it < cnt; ++it) {
+      const float px = nx_, py = ny_, pz = nz_, vel = nv_, pvol = nvol_;
+      const uint8_t ty = nty_;
+      if (it + 1 < cnt) fetch(it + 1, nx_, ny_, nz_, nv_, nty_, nvol_);
+      float fx, fy, fz;
+      offsets(px, py, pz, fx, fy, fz);
+      float dx2[A::NX], dy2[A::NY], dz2[A::NZ];
 #pragma unroll
-    for (int o = 0; o < A::NY; ++o) { float d = fy - (float)(o - 1); dy2[o] = d * d; }
+      for (int o = 0; o < A::NX; ++o) { float d = fx - (float)(o - 1); dx2[o] = fmaf(d, d, penx[o]); }
 #pragma unroll
-    for (int o = 0; o < A::NZ; ++o) { float d = fz - (float)(o - 1); dz2[o] = d * d; }
-    float wt[A::NN];
-    float wsum = 0.f;
-    // two-phase: the separable form t = (1 - (dx2 + dy2)/2.25) - dz2/2.25, w = t^3 for
-    // t > 0, with rows and columns outside the domain made negative -- one
-    // subtraction per node and no branch.  A node with |t| < kGuardT (its d^2 within
-    // about 1e-5 of 2.25) sends the particle through the per-node loop below, which
-    // decides those nodes with the FP64 predicate; every other node is far enough from
-    // the edge that t > 0 and the FP64 d^2 < 2.25 agree.
-    bool slow = KT != 0;
-    if (KT == 0) {
+      for (int o = 0; o < A::NY; ++o) { float d = fy - (float)(o - 1); dy2[o] = fmaf(d, d, peny[o]); }
+#pragma unroll
+      for (int o = 0; o < A::NZ; ++o) { float d = fz - (float)(o - 1); dz2[o] = fmaf(d, d, penz[o]); }
+      const float mp = particle_mass(ty, pvol);
       float Axy[A::NX * A::NY], Bz[A::NZ];
 #pragma unroll
       for (int oy = 0; oy < A::NY; ++oy)
 #pragma unroll
-        for (int ox = 0; ox < A::NX; ++ox)
-          Axy[ox + A::NX * oy] = (vx[ox] && vy[oy]) ? 1.f - (dx2[ox] + dy2[oy]) * (1.f / 2.25f) : -4.f;
+        for (int ox = 0; ox < A::NX; ++ox) Axy[ox + A::NX * oy] = 1.f - (dx2[ox] + dy2[oy]) * (1.f / 2.25f);
 #pragma unroll
-      for (int oz = 0; oz < A::NZ; ++oz) Bz[oz] = vz[oz] ? dz2[oz] * (1.f / 2.25f) : 4.f;
-      // (a fast-path node is activated by its positive weight: w = t^3 >= 1e-15 where
-      // t > 0 here, so the lane's accumulated mass there is positive -- the
-      // activation bits of these nodes are read from the sums after the loop)
-      float tmin = 4.f;
+      for (int oz = 0; oz < A::NZ; ++oz) Bz[oz] = dz2[oz] * (1.f / 2.25f);
+      float wt[A::NN];
+      float wsum = 0.f, tmin = 4.f;
 #pragma unroll
       for (int oz = 0; oz < A::NZ; ++oz)
 #pragma unroll
@@ -258,57 +347,98 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
             const int n = ox + A::NX * (oy + A::NY * oz);
             const float t = Axy[ox + A::NX * oy] - Bz[oz];
             tmin = fminf(tmin, fabsf(t));
-            const float tp = fmaxf(t, 0.f);
+            // max(t, 0) as one median (t <= 1 here); fmaxf would first quiet a NaN t
+            const float tp = __builtin_amdgcn_fmed3f(t, 0.f, 1.f);
             const float w = tp * tp * tp;
             wt[n] = w;
             wsum += w;
           }
-      slow = tmin < kGuardT;
+      // (a node of a particle accumulated here is activated by its positive weight:
+      // w = t^3 >= 1e-15 where t > 0 and m_p >= kMinFastMass, so the lane's mass sum
+      // there is positive -- these activation bits are read from the sums below)
+      if (tmin < kGuardT || !(mp >= kMinFastMass)) {
+        marked |= 1u << it;
+        continue;
+      }
+      accumulate(wt, wsum, mp, vel);
     }
-    if (slow) {
-      wsum = 0.f;
-  #pragma unroll
-      for (int oz = 0; oz < A::NZ; ++oz)
-  #pragma unroll
-        for (int oy = 0; oy < A::NY; ++oy)
-  #pragma unroll
-          for (int ox = 0; ox < A::NX; ++ox) {
-            const int n = ox + A::NX * (oy + A::NY * oz);
-            float d2 = dx2[ox] + dy2[oy] + dz2[oz];
-            bool inb = vx[ox] && vy[oy] && vz[oz];
-            float w;
-            bool act;
-            if (fabsf(d2 - 2.25f) < kGuard) {
-              w = exact_weight(gx - (double)(i + ox - 1), gy - (double)(j + oy - 1), gz - (double)(k + oz - 1),
-                               act);
-            } else {
-              act = d2 < 2.25f;
-              float t = 1.f - d2 * (1.f / 2.25f);
-              w = act ? t * t * t : 0.f;
-            }
-            act = act && inb && ((vbits >> n) & 1ull);
-            w = act ? w : 0.f;
-            wt[n] = w;
-            wsum += w;
-            if (act) fbits |= (1ull << n);
-          }
-    }
-    float mp = (ty == 0) ? a.mp_l : a.mp_g;
-    if (a.has_vol) mp *= pvol;
-    float sm = wsum > 0.f ? mp / wsum : 0.f;
-    float sq = sm * vel;
-#pragma unroll
-    for (int n = 0; n < A::NN; ++n) {
-      am[n] = fmaf(wt[n], sm, am[n]);
-      aq[n] = fmaf(wt[n], sq, aq[n]);
-    }
-  }
-  if (KT == 0) {
 #pragma unroll
     for (int n = 0; n < A::NN; ++n)
       if (am[n] > 0.f) fbits |= 1ull << n;
+  } else if (ranked && cnt > 0) {
+    marked = (cnt >= 32u) ? 0xffffffffu : ((1u << cnt) - 1u);
   }
-  // deposit into the LDS region: for fixed n the 64 lanes address 64 distinct nodes
+
+  // the second loop: marked particles in rank order, or every particle of a cell of
+  // more than kOrdMax particles by selection of the next caller index
+  uint64_t vbits = ~0ull;
+  if (KT == 2) {
+    // multires: only the enumerated faces take (and normalise) weights
+    // (collectWeightedFaces, multires_ops3d_tp.cpp:62-75)
+    vbits = 0;
+#pragma unroll
+    for (int oz = 0; oz < A::NZ; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < A::NY; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < A::NX; ++ox)
+          if (vx[ox] && vy[oy] && vz[oz] && f.valid[bm_index(fd, i + ox - 1, j + oy - 1, k + oz - 1)])
+            vbits |= 1ull << (ox + A::NX * (oy + A::NY * oz));
+  }
+  const uint32_t nsecond = ranked ? (uint32_t)__popc(marked) : cnt;
+  uint32_t last = 0;
+  bool first = true;
+  for (uint32_t s2 = 0; s2 < nsecond; ++s2) {
+    uint32_t q;
+    if (ranked) {
+      const int it = __ffs(marked) - 1;
+      marked &= marked - 1u;
+      q = s0 + ord[it];
+    } else {
+      uint32_t best = 0xffffffffu, bi = 0;
+      for (uint32_t t = 0; t < cnt; ++t) {
+        const uint32_t id = a.id[s0 + t];
+        if ((first || id > last) && id <= best) { best = id; bi = t; }
+      }
+      last = best;
+      first = false;
+      q = s0 + bi;
+    }
+    const float px = a.x[q], py = a.y[q], pz = a.z[q], vel = vp[q];
+    const float pvol = a.has_vol ? a.vol[q] : 1.f;
+    const uint8_t ty = a.type[q];
+    double gx, gy, gz;
+    coords64(px, py, pz, gx, gy, gz);
+    float wt[A::NN];
+    if (KT == 1) {
+      trilinear_weights<AX>(gx, gy, gz, i, j, k, vx, vy, vz, wt, fbits);
+      // mass += (float)(w * 1.0) = w; momentum += (float)(w * v) with v exact in FP64
+#pragma unroll
+      for (int n = 0; n < A::NN; ++n) {
+        am[n] += wt[n];
+        aq[n] += (float)((double)wt[n] * (double)vel);
+      }
+      continue;
+    }
+    float fx, fy, fz;
+    offsets(px, py, pz, fx, fy, fz);
+    float dx2[A::NX], dy2[A::NY], dz2[A::NZ];
+#pragma unroll
+    for (int o = 0; o < A::NX; ++o) { float d = fx - (float)(o - 1); dx2[o] = fmaf(d, d, penx[o]); }
+#pragma unroll
+    for (int o = 0; o < A::NY; ++o) { float d = fy - (float)(o - 1); dy2[o] = fmaf(d, d, peny[o]); }
+#pragma unroll
+    for (int o = 0; o < A::NZ; ++o) { float d = fz - (float)(o - 1); dz2[o] = fmaf(d, d, penz[o]); }
+    const float wsum = node_weights<AX>(dx2, dy2, dz2, gx, gy, gz, i, j, k, vbits, wt, fbits);
+    accumulate(wt, wsum, particle_mass(ty, pvol), vel);
+  }
+  // deposit into the LDS region, one stencil offset n at a time: for fixed n the 64
+  // lanes address 64 distinct nodes, and one wave's LDS operations execute in program
+  // order, so a plain read-add-write per n (volatile: kept in order, not merged) adds a
+  // node's lane sums in n order with no LDS atomics (float atomics in LDS were half of
+  // the kernel's time)
+  volatile float* vm = s_mom;
+  volatile float* vs = s_mass;
   const int base = lx + A::RX * (ly + A::RY * lz);
 #pragma unroll
   for (int oz = 0; oz < A::NZ; ++oz)
@@ -319,65 +449,64 @@ __device__ inline void splat_axis(const P2GArgs& a, int i, int j, int k, uint32_
         const int n = ox + A::NX * (oy + A::NY * oz);
         if ((fbits >> n) & 1ull) {
           const int r = base + ox + A::RX * (oy + A::RY * oz);
-          atomicAdd(&s_mom[r], aq[n]);
-          atomicAdd(&s_mass[r], am[n]);
+          const float m0 = vm[r], w0 = vs[r];
+          vm[r] = m0 + aq[n];
+          vs[r] = w0 + am[n];
           s_flag[r] = 1;
         }
       }
 }
 
+// Adds the block's region sums into the face field and clears the region for the next
+// axis.  Regions of same-colour blocks never overlap, so each node takes at most one
+// addition per launch: a no-return L2 float atomic gives the same sum as a read-modify-
+// write, in the same order (colour by colour), without the wave waiting for the read.
 template <int AX>
-__device__ inline void flush_axis(const P2GArgs& a, int bx, int by, int bz, const float* s_mom,
-                                  const float* s_mass, const uint8_t* s_flag, int lane) {
+__device__ inline void flush_axis(const P2GField& f, int bx, int by, int bz, float* s_mom, float* s_mass,
+                                  uint8_t* s_flag, int lane) {
   using A = Ax<AX>;
   constexpr int kPer = (A::RN + 63) / 64;  // region nodes per lane
-  const Dims& fd = a.df[AX];
-  float* __restrict__ mom = a.mom[AX];
-  float* __restrict__ mass = a.mass[AX];
-  // all of the lane's nodes are read before any is written (the nodes are distinct),
-  // so the loads of one lane overlap instead of waiting on each other's stores
-  long long gi[kPer];
-  float om[kPer], oq[kPer];
+  const Dims& fd = f.df;
 #pragma unroll
   for (int e = 0; e < kPer; ++e) {
     const int r = lane + 64 * e;
-    gi[e] = -1;
-    if (r >= A::RN || !s_flag[r]) continue;
-    const int rx = r % A::RX, ry = (r / A::RX) % A::RY, rz = r / (A::RX * A::RY);
-    const int ni = bx * 4 - 1 + rx, nj = by * 4 - 1 + ry, nk = bz * 4 - 1 + rz;
-    if (ni < 0 || nj < 0 || nk < 0 || ni >= fd.nx || nj >= fd.ny || nk >= fd.nz) continue;
-    gi[e] = bm_index(fd, ni, nj, nk);
-    oq[e] = mom[gi[e]];
-    om[e] = mass[gi[e]];
-  }
-#pragma unroll
-  for (int e = 0; e < kPer; ++e) {
-    if (gi[e] < 0) continue;
-    const int r = lane + 64 * e;
-    mom[gi[e]] = oq[e] + s_mom[r];
-    mass[gi[e]] = om[e] + s_mass[r];
-    const long long blk = gi[e] >> 6;
-    a.act_vel[AX][blk] = 1;
-    a.act_mass[AX][blk] = 1;
+    if (r >= A::RN) continue;
+    if (s_flag[r]) {
+      const int rx = r % A::RX, ry = (r / A::RX) % A::RY, rz = r / (A::RX * A::RY);
+      const int ni = bx * 4 - 1 + rx, nj = by * 4 - 1 + ry, nk = bz * 4 - 1 + rz;
+      if (ni >= 0 && nj >= 0 && nk >= 0 && ni < fd.nx && nj < fd.ny && nk < fd.nz) {
+        const long long gi = bm_index(fd, ni, nj, nk);
+        unsafeAtomicAdd(&f.mom[gi], s_mom[r]);
+        unsafeAtomicAdd(&f.mass[gi], s_mass[r]);
+        const long long blk = gi >> 6;
+        f.act_vel[blk] = 1;
+        f.act_mass[blk] = 1;
+      }
+    }
+    s_mom[r] = 0.f;
+    s_mass[r] = 0.f;
+    s_flag[r] = 0;
   }
 }
 
 // KT = 0: normalized cubic (two-phase); KT = 1: trilinear, unit mass (single-phase);
-// KT = 2: normalized cubic over the enumerated multires faces
-template <int KT>
-__global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
-  __shared__ float s_mom[3][256];
-  __shared__ float s_mass[3][256];
-  __shared__ uint8_t s_flag[3][256];
-  __shared__ uint8_t s_ord[64 * kOrdMax];
+// KT = 2: normalized cubic over the enumerated multires faces.  One launch per colour:
+// a block's particles are ranked once and splatted into the three face fields in turn
+// through one LDS node region (7 KB of LDS per workgroup).
+template <int KT, bool F32>
+__global__ void __launch_bounds__(64) k_p2g_color(P2GShared a, P2GField fu, P2GField fv, P2GField fw) {
   __shared__ uint32_t s_ids[kOrdMax * 64];
+  __shared__ uint8_t s_ord[64 * kOrdMax];
+  __shared__ float s_mom[256];
+  __shared__ float s_mass[256];
+  __shared__ uint8_t s_flag[256];
   if (blockIdx.x >= *a.count) return;
   const uint32_t b = a.list[blockIdx.x];
   const int lane = threadIdx.x;
-  for (int t = lane; t < 3 * 256; t += 64) {
-    (&s_mom[0][0])[t] = 0.f;
-    (&s_mass[0][0])[t] = 0.f;
-    (&s_flag[0][0])[t] = 0;
+  for (int t = lane; t < 256; t += 64) {
+    s_mom[t] = 0.f;
+    s_mass[t] = 0.f;
+    s_flag[t] = 0;
   }
   int bx, by, bz;
   block_coords(a.dc, b, bx, by, bz);
@@ -386,13 +515,14 @@ __global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
   const long long c = (long long)b * kBlockVol + lane;
   const uint32_t s0 = a.start[c];
   const uint32_t cnt = a.start[c + 1] - s0;
-  uint8_t* ord = &s_ord[lane * kOrdMax];
-  if (cnt > 1 && cnt <= (uint32_t)kOrdMax) {
-    // rank of each particle by caller index (ids are unique), from the cell's ids
-    // loaded at once into the lane's LDS column
+  const bool rank = cnt > 1 && cnt <= (uint32_t)kOrdMax;
+  // the cell's ids at once into the lane's LDS column
 #pragma unroll
-    for (int t = 0; t < kOrdMax; ++t)
-      if ((uint32_t)t < cnt) s_ids[t * 64 + lane] = a.P.id[s0 + t];
+  for (int t = 0; t < kOrdMax; ++t)
+    if (rank && (uint32_t)t < cnt) s_ids[t * 64 + lane] = a.id[s0 + t];
+  uint8_t* ord = &s_ord[lane * kOrdMax];
+  if (rank) {
+    // rank of each particle by caller index (ids are unique)
     for (uint32_t t = 0; t < cnt; ++t) {
       const uint32_t id = s_ids[t * 64 + lane];
       uint32_t r = 0;
@@ -403,13 +533,17 @@ __global__ void __launch_bounds__(64) k_p2g_color(P2GArgs a) {
     ord[0] = 0;
   }
   __syncthreads();
-  splat_axis<0, KT>(a, i, j, k, s0, cnt, ord, s_mom[0], s_mass[0], s_flag[0], lx, ly, lz);
-  splat_axis<1, KT>(a, i, j, k, s0, cnt, ord, s_mom[1], s_mass[1], s_flag[1], lx, ly, lz);
-  splat_axis<2, KT>(a, i, j, k, s0, cnt, ord, s_mom[2], s_mass[2], s_flag[2], lx, ly, lz);
+  splat_axis<0, KT, F32>(a, fu, i, j, k, s0, cnt, ord, s_mom, s_mass, s_flag, lx, ly, lz);
   __syncthreads();
-  flush_axis<0>(a, bx, by, bz, s_mom[0], s_mass[0], s_flag[0], lane);
-  flush_axis<1>(a, bx, by, bz, s_mom[1], s_mass[1], s_flag[1], lane);
-  flush_axis<2>(a, bx, by, bz, s_mom[2], s_mass[2], s_flag[2], lane);
+  flush_axis<0>(fu, bx, by, bz, s_mom, s_mass, s_flag, lane);
+  __syncthreads();
+  splat_axis<1, KT, F32>(a, fv, i, j, k, s0, cnt, ord, s_mom, s_mass, s_flag, lx, ly, lz);
+  __syncthreads();
+  flush_axis<1>(fv, bx, by, bz, s_mom, s_mass, s_flag, lane);
+  __syncthreads();
+  splat_axis<2, KT, F32>(a, fw, i, j, k, s0, cnt, ord, s_mom, s_mass, s_flag, lx, ly, lz);
+  __syncthreads();
+  flush_axis<2>(fw, bx, by, bz, s_mom, s_mass, s_flag, lane);
 }
 
 }  // namespace pf
