@@ -63,6 +63,18 @@ def read_traffic(path):
     return None
 
 
+def read_table_value(path, label):
+    """The value column of the summary table row whose first cell starts with label."""
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 2 and cells[0].startswith(label):
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
+
+
 def _ensure_built():
     import pfflip_amd as G
     if not os.path.exists(G.LIB_PATH):
@@ -359,13 +371,17 @@ def main():
             per_launch = (P2G_BYTES[0] * npart + P2G_BYTES[1] * fluid) / 8
             achieved = per_launch / (avg * 1e-3) / 1e9
             tf = os.path.join(ROOT, "profiles", "p2g_color_rt256_traffic.md")
-            roofline = {"bound": "hbm", "kernel": "k_p2g_color<0>", "achieved": round(achieved, 1),
+            roofline = {"bound": "hbm", "kernel": "k_p2g_color<0, true>", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": read_traffic(tf) if (os.path.exists(tf) and n == 256) else None,
                         "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg, 5),
                         "launches_timed": nl, "dominant_kernel_in_step_table": dominant,
                         "note": "bytes = (25 B/particle + 24 B/fluid cell) / 8 colour launches; the kernel is "
-                                "VALU-bound (its counter summary is in profiles/)"}
+                                "VALU-bound: valu_busy and waves_per_simd are the SQ counters of the committed "
+                                "256^3 counter summary"}
+            if os.path.exists(tf) and n == 256:
+                roofline["valu_busy"] = read_table_value(tf, "VALU busy")
+                roofline["waves_per_simd"] = read_table_value(tf, "resident waves per SIMD")
         nl, ms = live["pcg_spmv"]
         if nl:
             bytes_per_launch = SPMV_BYTES_PER_CELL * fluid
