@@ -186,7 +186,7 @@ int pf_destroy(pf_ctx* c) {
   hipStreamSynchronize(c->stream);
   for (int f = 0; f < F_N; ++f) { hipFree(c->f[f]); hipFree(c->act[f]); }
   hipFree(c->marker); hipFree(c->cell_count); hipFree(c->cell_start);
-  hipFree(c->color_list); hipFree(c->color_cnt); hipFree(c->pblk); hipFree(c->pblk_prev);
+  hipFree(c->color_list); hipFree(c->color_cnt); dfree(c->crowd_list); hipFree(c->pblk); hipFree(c->pblk_prev);
   hipFree(c->blk_flag32); hipFree(c->blk_scan); hipFree(c->morton_flags);
   PBuf* bufs[2] = {&c->pa, &c->pb};
   for (PBuf* b : bufs) {

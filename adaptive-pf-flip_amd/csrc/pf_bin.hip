@@ -101,6 +101,94 @@ __global__ void k_color_lists(Dims dc, const uint32_t* __restrict__ start, uint3
   }
 }
 
+// Cells of lo < count <= hi particles, appended in any order (each is sorted on its
+// own); one atomic per wave.  cap >= np / (lo + 1) bounds the list.
+__global__ void k_crowded_cells(long long ncell, const uint32_t* __restrict__ start, uint32_t lo, uint32_t hi,
+                                uint32_t* __restrict__ list, uint32_t* __restrict__ count, long long cap) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t cnt = 0;
+  if (c < ncell) cnt = start[c + 1] - start[c];
+  const bool crowd = cnt > lo && cnt <= hi;
+  const unsigned long long m = __ballot(crowd);
+  if (m == 0) return;
+  const int leader = __ffsll(m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (crowd) {
+    const long long q = (long long)base + __popcll(m & ((1ull << lane) - 1ull));
+    if (q < cap) list[q] = (uint32_t)c;
+  }
+}
+
+// Sorts the particles of each listed cell by caller id, in place: a bitonic sort of
+// (id, slot) pairs in LDS, then every particle array permuted through LDS.  A
+// workgroup takes list entries w, w + gridDim.x, ...
+template <typename T>
+__device__ inline void permute_cell(T* a, uint32_t s0, uint32_t cnt, const uint16_t* pos, uint32_t* tmp) {
+  for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+    T v = a[s0 + t];
+    uint32_t bits = 0;
+    __builtin_memcpy(&bits, &v, sizeof(T));
+    tmp[t] = bits;
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const uint32_t bits = tmp[pos[t]];
+    T v;
+    __builtin_memcpy(&v, &bits, sizeof(T));
+    a[s0 + t] = v;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_sort_crowded(const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count,
+                                                      const uint32_t* __restrict__ start, PBuf P, bool has_vol) {
+  __shared__ uint32_t key[kSortedMax];
+  __shared__ uint16_t pos[kSortedMax];
+  __shared__ uint32_t tmp[kSortedMax];
+  const uint32_t n = *count;
+  for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+    const uint32_t c = list[e];
+    const uint32_t s0 = start[c], cnt = start[c + 1] - s0;  // kRankMax < cnt <= kSortedMax
+    uint32_t m = 1;
+    while (m < cnt) m <<= 1;
+    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+      key[t] = t < cnt ? P.id[s0 + t] : 0xffffffffu;
+      pos[t] = (uint16_t)t;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+          const uint32_t u = t ^ j;
+          if (u > t) {
+            const bool up = (t & k) == 0;
+            const uint32_t a = key[t], b = key[u];
+            if ((a > b) == up) {
+              key[t] = b; key[u] = a;
+              const uint16_t pa = pos[t];
+              pos[t] = pos[u]; pos[u] = pa;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    permute_cell(P.x, s0, cnt, pos, tmp);
+    permute_cell(P.y, s0, cnt, pos, tmp);
+    permute_cell(P.z, s0, cnt, pos, tmp);
+    permute_cell(P.u, s0, cnt, pos, tmp);
+    permute_cell(P.v, s0, cnt, pos, tmp);
+    permute_cell(P.w, s0, cnt, pos, tmp);
+    permute_cell(P.type, s0, cnt, pos, tmp);
+    permute_cell(P.id, s0, cnt, pos, tmp);
+    if (has_vol) permute_cell(P.vol, s0, cnt, pos, tmp);
+  }
+}
+
 // Per-cell particle counts of the binning in dense linear order i + nx*(j + ny*k).
 __global__ void k_counts_dense(Dims dc, const uint32_t* __restrict__ cnt, uint32_t* __restrict__ out) {
   const long long n = (long long)dc.nx * dc.ny * dc.nz;

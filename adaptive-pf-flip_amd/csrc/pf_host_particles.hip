@@ -49,6 +49,20 @@ int ensure_binned(pf_ctx* c) {
                              c->pb, c->has_vol, c->key, c->rank, c->cell_start));
     LAUNCH_CHECK(c);
     std::swap(c->pa, c->pb);
+    // cells of more than kRankMax particles (piles against walls) sorted by caller id,
+    // so that P2G walks them in order without a quadratic selection
+    const long long cap = c->np / (kRankMax + 1) + 1;
+    if (cap > c->crowd_cap) {
+      dfree(c->crowd_list);
+      HIP_TRY(c, dalloc(&c->crowd_list, cap));
+      c->crowd_cap = cap;
+    }
+    HIP_TRY(c, fill_u32(c->stream, c->u32_dev + 16, 0u, 1));
+    hipLaunchKernelGGL(k_crowded_cells, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, ncell, c->cell_start,
+                       (uint32_t)kRankMax, (uint32_t)kSortedMax, c->crowd_list, c->u32_dev + 16, c->crowd_cap);
+    hipLaunchKernelGGL(k_sort_crowded, dim3(512), dim3(256), 0, c->stream, c->crowd_list, c->u32_dev + 16,
+                       c->cell_start, c->pa, c->has_vol);
+    LAUNCH_CHECK(c);
   }
   HIP_TRY(c, fill_u32(c->stream, c->color_cnt, 0u, 8));
   hipLaunchKernelGGL(k_color_lists, dim3(grid_for(c->dc.nblocks, 256)), dim3(256), 0, c->stream, c->dc,

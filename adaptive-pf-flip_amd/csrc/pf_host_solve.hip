@@ -14,18 +14,9 @@ bool mg_cfg_valid(const pf_mg_cfg& q) {
          (q.chebyshev_ratio == 0.0f || (q.chebyshev_ratio > 1.0f && q.chebyshev_ratio <= 1000.0f));
 }
 
-// the built-in cycle, with the PF_MG_* environment overrides (read once)
+// the built-in cycle (pf_set_multigrid changes a context's)
 const pf_mg_cfg& mg_default_cfg() {
-  static const pf_mg_cfg p = [] {
-    pf_mg_cfg q{2, 32, 0.9f, 1.8f, 6.0f};
-    pf_mg_cfg e = q;
-    if (const char* v = std::getenv("PF_MG_SWEEPS")) e.sweeps = std::atoi(v);
-    if (const char* v = std::getenv("PF_MG_COARSEST_SWEEPS")) e.coarsest_sweeps = std::atoi(v);
-    if (const char* v = std::getenv("PF_MG_OMEGA")) e.omega = (float)std::atof(v);
-    if (const char* v = std::getenv("PF_MG_CORRECTION")) e.correction = (float)std::atof(v);
-    if (const char* v = std::getenv("PF_MG_CHEB")) e.chebyshev_ratio = (float)std::atof(v);
-    return mg_cfg_valid(e) ? e : q;
-  }();
+  static const pf_mg_cfg p{2, 32, 0.9f, 1.8f, 6.0f};
   return p;
 }
 
@@ -524,8 +515,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
                              : (cfg->stop_rule == PF_STOP_REL2 ? cfg->tol * cfg->tol * bnorm2 : cfg->tol * cfg->tol);
   static const bool trace = std::getenv("PF_SOLVE_TRACE") != nullptr;
   // most FP32 iterations between two residual replacements
-  const char* env_between = std::getenv("PF_SOLVE_REPLACE_EVERY");
-  const int between = env_between ? std::max(8, std::atoi(env_between)) : 200;
+  constexpr int between = 200;
   int total = 0, outer = 0;
   bool converged = met();
   if (lpc_system) c->tiles_valid = true;

@@ -100,6 +100,13 @@ struct PBuf {
 };
 
 enum { F_U = 0, F_V, F_W, F_MU, F_MV, F_MW, F_P, F_MK, F_SU, F_SV, F_SW, F_N };
+
+// Particle order inside a cell, which fixes P2G's summation order: a cell of at most
+// kRankMax particles is ranked by caller id in P2G itself; after binning, a cell of
+// kRankMax < count <= kSortedMax particles is sorted by caller id in memory
+// (k_sort_crowded); a fuller cell is walked by selection of the next id.
+constexpr int kRankMax = 32;
+constexpr int kSortedMax = 2048;
 enum { CELL_AIR = 0, CELL_FLUID = 1, CELL_SOLID = 2 };
 
 // FP64 scalars of one inner PCG solve, kept on the device so the iteration
@@ -130,6 +137,8 @@ struct Ctx {
   uint32_t* color_list = nullptr;   // 8 * color_cap
   uint32_t* color_cnt = nullptr;    // 8
   long long color_cap = 0;
+  uint32_t* crowd_list = nullptr;   // cells of kRankMax < count <= kSortedMax particles (sorted by id)
+  long long crowd_cap = 0;
   uint32_t* pblk = nullptr;         // sorted list of fluid cell blocks
   uint32_t* pblk_prev = nullptr;    // previous projection's list (zeroed on reuse)
   long long npblk = 0, npblk_prev = 0;

@@ -109,7 +109,8 @@ inline P2GField p2g_field_args(const P2GArgs& a, int ax) {
   return r;
 }
 
-constexpr int kOrdMax = 16;
+constexpr int kOrdMax = kRankMax;  // particles per cell ranked in LDS
+constexpr uint32_t kFastMax = 64;  // cells the fast loop takes (its marks fit a 64-bit mask)
 constexpr float kGuard = 1e-5f;
 constexpr float kGuardT = 1e-5f;  // on t = 1 - d^2/2.25: wider than kGuard / 2.25
 constexpr float kOutside = 1e4f;  // d^2 penalty of a stencil row outside the field
@@ -213,14 +214,15 @@ __device__ inline float node_weights(const float* dx2, const float* dy2, const f
 // A lane splats its cell's particles (in ascending caller index) into its 36 node sums
 // of one face axis, then adds them into the block's LDS region.
 //
-// KT = 0 runs two loops.  The first covers cells of at most kOrdMax particles in the
-// precomputed rank order, with the next particle's data loaded while this one
+// KT = 0 runs two loops.  The first covers cells of at most kFastMax particles in
+// caller-index order, with the next particle's data loaded while this one
 // computes, and evaluates the cubic in the separable FP32 form
 // t = (1 - (dx^2 + dy^2)/2.25) - dz^2/2.25, w = max(t, 0)^3 -- one subtraction per
 // node and no branch.  A particle with a node at |t| < kGuardT (its d^2 within about
 // 1e-5 of 2.25), or with a mass too small for its positive weights to leave a positive
 // sum, is only marked there.  The second loop takes the marked particles and every
-// particle of a fuller cell (selected by ascending caller index) through node_weights.
+// particle of a fuller cell (in memory order up to kSortedMax particles, else selected
+// by ascending caller index) through node_weights.
 // Keeping the rare paths out of the first loop keeps its registers to what it needs.
 // KT = 2 (multires, enumerated faces only) and KT = 1 (trilinear) use the second loop
 // for every particle.
@@ -302,13 +304,19 @@ __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, 
     }
   };
 
+  // the order of a cell's particles: the LDS ranking (at most kOrdMax), memory order
+  // (cells of up to kSortedMax particles were sorted by id after binning), or else
+  // selection of the next caller index
   const bool ranked = cnt <= (uint32_t)kOrdMax;
-  uint32_t marked = 0;  // ranks (bit it) left to the second loop
-  if (KT == 0 && ranked && cnt > 0) {
+  const bool sorted = !ranked && cnt <= (uint32_t)kSortedMax;
+  const bool masked = cnt <= kFastMax;  // ranked or sorted
+  auto slot = [&](uint32_t r) -> uint32_t { return ranked ? (uint32_t)ord[r] : r; };
+  uint64_t marked = 0;  // ranks (bit it) left to the second loop
+  if (KT == 0 && masked && cnt > 0) {
     // the particle of rank r (its data for the next iteration load while this one
     // computes)
     auto fetch = [&](uint32_t r, float& px, float& py, float& pz, float& vel, uint8_t& ty, float& pvol) {
-      const uint32_t q = s0 + ord[r];
+      const uint32_t q = s0 + slot(r);
       px = a.x[q]; py = a.y[q]; pz = a.z[q]; vel = vp[q]; ty = a.type[q];
       pvol = a.has_vol ? a.vol[q] : 1.f;
     };
@@ -357,7 +365,7 @@ __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, 
       // w = t^3 >= 1e-15 where t > 0 and m_p >= kMinFastMass, so the lane's mass sum
       // there is positive -- these activation bits are read from the sums below)
       if (tmin < kGuardT || !(mp >= kMinFastMass)) {
-        marked |= 1u << it;
+        marked |= 1ull << it;
         continue;
       }
       accumulate(wt, wsum, mp, vel);
@@ -365,12 +373,12 @@ __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, 
 #pragma unroll
     for (int n = 0; n < A::NN; ++n)
       if (am[n] > 0.f) fbits |= 1ull << n;
-  } else if (ranked && cnt > 0) {
-    marked = (cnt >= 32u) ? 0xffffffffu : ((1u << cnt) - 1u);
+  } else if (masked && cnt > 0) {
+    marked = (cnt >= 64u) ? ~0ull : ((1ull << cnt) - 1ull);
   }
 
-  // the second loop: marked particles in rank order, or every particle of a cell of
-  // more than kOrdMax particles by selection of the next caller index
+  // the second loop: marked particles in rank order, then every particle of a cell of
+  // more than kFastMax particles, in memory order or by selection
   uint64_t vbits = ~0ull;
   if (KT == 2) {
     // multires: only the enumerated faces take (and normalise) weights
@@ -385,15 +393,17 @@ __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, 
           if (vx[ox] && vy[oy] && vz[oz] && f.valid[bm_index(fd, i + ox - 1, j + oy - 1, k + oz - 1)])
             vbits |= 1ull << (ox + A::NX * (oy + A::NY * oz));
   }
-  const uint32_t nsecond = ranked ? (uint32_t)__popc(marked) : cnt;
+  const uint32_t nsecond = masked ? (uint32_t)__popcll(marked) : cnt;
   uint32_t last = 0;
   bool first = true;
   for (uint32_t s2 = 0; s2 < nsecond; ++s2) {
     uint32_t q;
-    if (ranked) {
-      const int it = __ffs(marked) - 1;
-      marked &= marked - 1u;
-      q = s0 + ord[it];
+    if (masked) {
+      const int it = __ffsll((unsigned long long)marked) - 1;
+      marked &= marked - 1ull;
+      q = s0 + slot((uint32_t)it);
+    } else if (sorted) {
+      q = s0 + s2;
     } else {
       uint32_t best = 0xffffffffu, bi = 0;
       for (uint32_t t = 0; t < cnt; ++t) {

@@ -145,7 +145,7 @@ class MgCfg(C.Structure):
 
 
 def mg_defaults() -> MgCfg:
-    """the built-in cycle (or the PF_MG_* environment overrides)"""
+    """the built-in cycle, which every context starts with"""
     c = MgCfg()
     lib().pf_mg_defaults(C.byref(c))
     return c

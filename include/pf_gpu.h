@@ -570,9 +570,7 @@ int pf_lpc_set_objective(pf_lpc* lpc, int32_t objective, int32_t power_steps);
 
 /* ---- multigrid preconditioner (no reference counterpart; PF_PRECOND_MULTIGRID) */
 /* the built-in cycle (2 Chebyshev sweeps on [1/3, 2], 32 coarsest sweeps at omega
- * 0.9, correction 1.8), or the PF_MG_SWEEPS / PF_MG_COARSEST_SWEEPS / PF_MG_OMEGA /
- * PF_MG_CORRECTION / PF_MG_CHEB environment variables where set; every context
- * starts with it */
+ * 0.9, correction 1.8); every context starts with it */
 void pf_mg_defaults(pf_mg_cfg* cfg);
 /* the cycle of the following multigrid solves on ctx (PF_ERR_ARG out of range) */
 int pf_set_multigrid(pf_ctx* ctx, const pf_mg_cfg* cfg);

@@ -65,6 +65,37 @@ def test_markers_and_p2g_match_fp64_oracle():
         assert rel_err(mom, g.dense(fv) * g.dense(fv + 3)) < FIELD_TOL
 
 
+def test_multires_entry_points_need_a_layout():
+    # every entry point that launches a multires kernel refuses a context without a
+    # layout (PF_ERR_STATE) rather than reading unset buffers; the state queries answer
+    n = 16
+    ph, gph = _phase()
+    ctx = G.MRContext(n, n, n, 1.0)
+    try:
+        ctx.upload(*_bubble_particles(n))
+        calls = (ctx.mr_mark_cells, lambda: ctx.mr_p2g(gph, 1.0), lambda: ctx.mr_project(gph, 0.02),
+                 lambda: ctx.face_valid(0), lambda: ctx.mr_interface_diagnostics(gph),
+                 lambda: ctx.mr_apply_surface_tension(gph, 0.02))
+        for call in calls:
+            with pytest.raises(G.PfError):
+                call()
+        assert len(ctx.layout()) == 0
+        assert ctx.mr_update_refinement(G.MRRefineCfg(0, 1, 2, 1, 0, 0)) == (0, 0)  # refinement off: no change
+        assert ctx.mr_retained_box()[0] is False
+        assert len(ctx.layout()) == 0
+        sim = G.MRSim3DTP(n, n, n, 1.0)
+        sim.ctx.close()
+        sim.ctx, sim.phase, sim.dynamic_refinement = ctx, gph, False
+        with pytest.raises(G.PfError):
+            sim.step()
+        # and the same context works once a layout is set
+        ctx.set_layout(O.mr_layout_bubble_band(n, n, n))
+        ctx.mr_mark_cells()
+        ctx.mr_p2g(gph, 1.0)
+    finally:
+        ctx.close()
+
+
 def _explain(ctx, g, ps, f):
     """what differs, for the failure message: the mismatching faces, the markers and
     the binned particle counts per cell"""

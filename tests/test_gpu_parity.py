@@ -75,6 +75,55 @@ def test_markers_active_sets_and_p2g_fields(kind):
     assert rel_err(phi(ctx.field(G.FIELD_MV).astype(np.float64)), phi(g.dense(O.MV))) < FIELD_TOL
 
 
+def test_p2g_crowded_cells_match_oracle():
+    # piles among an 8-per-cell background, in caller order shuffled: cells of 20
+    # particles (ranked in LDS), 40 and 70 (sorted by caller id after binning; 70 also
+    # passes the fast loop's 64-particle mask) and 2100 (beyond the in-memory sort: the
+    # selection walk), against the FP64 oracle; the sort moves whole particles, so the
+    # caller-order download is bit-equal to the upload; bit-identical on repeat
+    n = 16
+    dx, x, y, z, t = scene(O.SCENE_RT, n, seed=5)
+    rng = np.random.default_rng(5)
+    px, py, pz, pt = [x], [y], [z], [t]
+    for (i, j, k), cnt in (((5, 5, 5), 20), ((8, 4, 9), 40), ((3, 10, 6), 70), ((10, 10, 10), 2100)):
+        px.append(((i + rng.uniform(0.01, 0.99, cnt)) * dx).astype(np.float32))
+        py.append(((j + rng.uniform(0.01, 0.99, cnt)) * dx).astype(np.float32))
+        pz.append(((k + rng.uniform(0.01, 0.99, cnt)) * dx).astype(np.float32))
+        pt.append(rng.integers(0, 2, cnt).astype(np.uint8))
+    x, y, z, t = (np.concatenate(a) for a in (px, py, pz, pt))
+    perm = rng.permutation(len(x))
+    x, y, z, t = x[perm], y[perm], z[perm], t[perm]
+    u, v, w = (rng.normal(scale=0.3, size=len(x)).astype(np.float32) for _ in range(3))
+    oph = phase()
+    ps = O.Particles(x, y, z, u, v, w, t)
+    g = O.Grid(n, n, n, dx, faithful=False)
+    g.mark_cells(ps)
+    g.p2g(ps, oph, 1.0)
+    outs = []
+    for _ in range(2):
+        ctx = G.Context(n, n, n, dx)
+        try:
+            ctx.upload(x, y, z, u, v, w, t)
+            ctx.mark_cells()
+            ctx.p2g(_gphase(oph), 1.0)
+            assert int(ctx.bin_counts().max()) == 2100 + 8
+            assert np.array_equal(ctx.field(G.FIELD_MARKER), g.dense(O.MK))
+            fields = [ctx.field(f) for f in range(6)]
+            for f in range(6):
+                assert np.array_equal(np.sort(ctx.active_blocks(f)), np.sort(g.active_blocks(f))), f
+                assert rel_err(fields[f], g.dense(f)) < FIELD_TOL, f
+            ctx.p2g(_gphase(oph), 1.0)  # again on the same binning
+            for f in range(6):
+                assert np.array_equal(ctx.field(f), fields[f]), f
+            for a, b in zip(ctx.download(), (x, y, z, u, v, w)):
+                assert np.array_equal(a, b)
+            outs.append(fields)
+        finally:
+            ctx.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
 def _to_projection(ctx, g, ps, oph, dt):
     ctx.mark_cells()
     ctx.p2g(_gphase(oph), 1.0)
