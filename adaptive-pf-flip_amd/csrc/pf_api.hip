@@ -165,6 +165,7 @@ int pf_create(pf_ctx** out, const pf_grid_desc* d) {
   HIP_TRY(c, dalloc(&c->red, 16));
   HIP_TRY(c, hipHostMalloc((void**)&c->red_host, sizeof(double) * 16));
   HIP_TRY(c, hipHostMalloc((void**)&c->done_host, sizeof(int) * 16));
+  HIP_TRY(c, hipHostMalloc((void**)&c->sc_host, sizeof(SolveScalars)));
   HIP_TRY(c, dalloc(&c->u32_dev, 64));
   HIP_TRY(c, hipHostMalloc((void**)&c->u32_host, sizeof(uint32_t) * 64));
   HIP_TRY(c, dalloc(&c->esc_dev, 8));
@@ -218,7 +219,7 @@ int pf_destroy(pf_ctx* c) {
   if (c->mr_bbox_host) hipHostFree(c->mr_bbox_host);
   dfree(c->fbeta); dfree(c->fphi); dfree(c->tiles); dfree(c->tile_flags); dfree(c->snaps);
   hipFree(c->sc); hipFree(c->partial); hipFree(c->red);
-  hipHostFree(c->red_host); hipHostFree(c->done_host); hipHostFree(c->u32_host);
+  hipHostFree(c->red_host); hipHostFree(c->done_host); hipHostFree(c->sc_host); hipHostFree(c->u32_host);
   hipFree(c->u32_dev); hipFree(c->esc_dev); hipFree(c->vmax_dev);
   hipFree(c->scan_tiles); hipFree(c->stage_f);
   for (int e = 0; e <= PF_NSTAGES; ++e)

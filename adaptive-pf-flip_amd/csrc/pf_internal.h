@@ -167,6 +167,7 @@ struct Ctx {
   double* red = nullptr;            // reduced values (device, 16 doubles)
   double* red_host = nullptr;       // pinned host mirror
   int* done_host = nullptr;         // pinned host mirror of sc->done
+  SolveScalars* sc_host = nullptr;  // pinned host mirror of the whole struct
   uint32_t* u32_dev = nullptr;      // small device scratch (pin detection, counters), 64 words
   uint32_t* u32_host = nullptr;     // pinned mirror, 64 words
   int32_t* esc_dev = nullptr;       // 8 clamp counters
