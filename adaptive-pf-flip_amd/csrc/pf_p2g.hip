@@ -184,11 +184,12 @@ __device__ inline float node_weights(const float* dx2, const float* dy2, const f
                                      double gz, int i, int j, int k, uint64_t vbits, float* wt, uint64_t& fbits) {
   using A = Ax<AX>;
   float wsum = 0.f;
-#pragma unroll
+  // rolled: the rare exact path must not raise the kernel's register count
+#pragma unroll 1
   for (int oz = 0; oz < A::NZ; ++oz)
-#pragma unroll
+#pragma unroll 1
     for (int oy = 0; oy < A::NY; ++oy)
-#pragma unroll
+#pragma unroll 1
       for (int ox = 0; ox < A::NX; ++ox) {
         const int n = ox + A::NX * (oy + A::NY * oz);
         float d2 = dx2[ox] + dy2[oy] + dz2[oz];
