@@ -88,14 +88,21 @@ def _ensure_built():
 
 # per preconditioner: (pf_precond, FP32 reduction per FP64 refinement pass, iterations
 # between host checks) -- a multigrid iteration costs about four Jacobi ones, so its
-# passes go deeper and the host looks more often
-SOLVERS = {"jacobi": (0, 0.1, 16), "multigrid": (2, 0.01, 4), "learned": (1, 0.01, 4)}
+# passes go deeper and the host looks more often.  Measured at 256^3 (multigrid): a
+# 1e-4 reduction per pass takes 2 replacements and 20 iterations (18.4 ms), 1e-2 four
+# and 18 (19.2 ms), 1e-3 three and 26 (24.1 ms: its restart falls mid-convergence)
+SOLVERS = {"jacobi": (0, 0.1, 16), "multigrid": (2, 1e-4, 4), "learned": (1, 1e-4, 4)}
 SOLVER_NAMES = {"jacobi": "plain Jacobi PCG", "multigrid": "aggregation-multigrid PCG",
                 "learned": "multigrid PCG with learned correction weights"}
 
 
+INNER_OVERRIDE = None  # --inner-rel-tol
+
+
 def solve_cfg(G, precond, tol, max_iters, warm=0):
     pre, inner, check = SOLVERS[precond]
+    if INNER_OVERRIDE is not None and precond != "jacobi":
+        inner = INNER_OVERRIDE
     return G.SolveCfg(max_iters=max_iters, stop_rule=G.STOP_REL2, tol=tol, inner_rel_tol=inner,
                       max_outer=10000, warm_start=warm, check_every=check, precond=pre)
 
@@ -303,9 +310,13 @@ def main():
     ap.add_argument("--no-pressure", action="store_true", help="skip the pressure-solver comparison")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the secondary lines (Jacobi step, bubble 128^3, dam-break 256^3)")
+    ap.add_argument("--inner-rel-tol", type=float, default=None,
+                    help="FP32 residual reduction per FP64 refinement pass of the cycle solvers")
     ap.add_argument("--precond", choices=sorted(SOLVERS), default="multigrid",
                     help="preconditioner of the timed steps' pressure solve")
     args = ap.parse_args()
+    global INNER_OVERRIDE
+    INNER_OVERRIDE = args.inner_rel_tol
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this library has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
