@@ -194,7 +194,7 @@ int pf_destroy(pf_ctx* c) {
     hipFree(b->x); hipFree(b->y); hipFree(b->z); hipFree(b->u); hipFree(b->v); hipFree(b->w);
     hipFree(b->type); hipFree(b->id); hipFree(b->vol);
   }
-  hipFree(c->key); hipFree(c->rank);
+  hipFree(c->rank);
   float* f32[] = {c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->cd32, c->cx32, c->cy32, c->cz32, c->invd32};
   for (float* p : f32) hipFree(p);
   double* f64[] = {c->b64, c->x64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64};

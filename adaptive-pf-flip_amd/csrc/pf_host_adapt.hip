@@ -205,10 +205,8 @@ int run_adaptivity(pf_ctx* c, const pf_adapt_cfg* cfg, pf_adapt_stats* out) {
     c->pa = outb;
     st = alloc_pbuf(c, c->pb, cap);
     if (st) return st;
-    hipFree(c->key); hipFree(c->rank);
-    c->key = nullptr;
+    hipFree(c->rank);
     c->rank = nullptr;
-    HIP_TRY(c, dalloc(&c->key, cap));
     HIP_TRY(c, dalloc(&c->rank, cap));
     c->pcap = cap;
   } else {

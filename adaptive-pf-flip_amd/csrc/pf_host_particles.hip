@@ -22,8 +22,7 @@ int grow_particles(pf_ctx* c, long long n, bool vol) {
     if (vol && !b->vol) HIP_TRY(c, dalloc(&b->vol, cap));
   }
   if (n > c->pcap) {
-    hipFree(c->key); hipFree(c->rank);
-    HIP_TRY(c, dalloc(&c->key, cap));
+    hipFree(c->rank);
     HIP_TRY(c, dalloc(&c->rank, cap));
     c->pcap = cap;
   }
@@ -39,14 +38,14 @@ int ensure_binned(pf_ctx* c) {
   if (c->np > 0) {
     TIMED(c, PF_K_BIN_COUNT,
           hipLaunchKernelGGL(k_bin_count, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
-                             c->geom, c->dc, c->key, c->rank, c->cell_count));
+                             c->geom, c->dc, c->rank, c->cell_count));
     LAUNCH_CHECK(c);
   }
   HIP_TRY(c, scan_u32(*c, c->cell_count, c->cell_start, ncell));
   if (c->np > 0) {
     TIMED(c, PF_K_BIN_SCATTER,
           hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
-                             c->pb, c->has_vol, c->key, c->rank, c->cell_start));
+                             c->pb, c->has_vol, c->geom, c->dc, c->rank, c->cell_start));
     LAUNCH_CHECK(c);
     std::swap(c->pa, c->pb);
     // cells of more than kRankMax particles (piles against walls) sorted by caller id,

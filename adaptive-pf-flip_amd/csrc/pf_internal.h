@@ -153,7 +153,6 @@ struct Ctx {
   bool has_vol = false;
   bool binned = false;
   bool mark_order_valid = false;    // the particle order is the binning the last marking used
-  uint32_t* key = nullptr;
   uint32_t* rank = nullptr;
   double liquid_volume = 0.0;       // sum of volume over liquid particles
 

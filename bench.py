@@ -45,7 +45,7 @@ SPMV_BYTES_PER_CELL = 32  # z, p_old, diag, 3 couplings read; p_new, Ap written 
 # algorithmic bytes per stage (FP32 = 4 B): per particle and per FLUID cell
 P2G_BYTES = (25, 24)        # particle x, y, z, u, v, w, type read; u, v, w, mu, mv, mw written
 STAGE_BYTES = {             # (per particle, per fluid cell, per grid cell)
-    "bin": (58, 0, 0),      # particle read + sorted write, key + rank
+    "bin": (58, 0, 0),      # particle read + sorted write (rank and counts not counted)
     "mark": (0, 0, 5),      # counts read, marker written
     "p2g": (25, 24, 0),
     "grid": (0, 60, 0),     # 3 axes: velocity + mass read, velocity + saved written
