@@ -36,7 +36,7 @@ int ensure_scan_capacity(pf_ctx* c, long long n) {
 }
 
 // the liquid particle volume after a change of the particle set (FP64 partial sums
-// per workgroup, added in workgroup order)
+// per workgroup, combined in a fixed order)
 int update_liquid_volume(pf_ctx* c) {
   if (c->np == 0) {
     c->liquid_volume = 0.0;
@@ -45,7 +45,7 @@ int update_liquid_volume(pf_ctx* c) {
   const unsigned nwg = std::min<unsigned>(grid_for(c->np, 256), 1024u);
   hipLaunchKernelGGL(k_pa_liquid_volume, dim3(nwg), dim3(256), 0, c->stream, c->pa, c->np, c->has_vol, c->partial);
   const CombineOps ops{1, {0}};
-  hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(64), 0, c->stream, c->partial, (int)nwg, ops, c->red);
+  hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(kCombineThreads), 0, c->stream, c->partial, (int)nwg, ops, c->red);
   LAUNCH_CHECK(c);
   HIP_TRY(c, hipMemcpyAsync(c->red_host, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));

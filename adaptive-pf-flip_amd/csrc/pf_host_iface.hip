@@ -48,7 +48,7 @@ int iface_reduce(pf_ctx* c, const IfaceArgs& a, pf_iface_stats* out) {
   const CombineOps ops{kIfaceSlots, {0, 1, 2, 0, 0, 0, 2, 0, 2, 0}};
   const long long ki = kbegin(c, PF_K_INTERFACE);
   hipLaunchKernelGGL(k_iface_diag, dim3(nwg), dim3(256), 0, c->stream, a, c->partial);
-  hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(64), 0, c->stream, c->partial, (int)nwg, ops, c->red);
+  hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(kCombineThreads), 0, c->stream, c->partial, (int)nwg, ops, c->red);
   kend(c, ki);
   LAUNCH_CHECK(c);
   HIP_TRY(c, hipMemcpyAsync(c->red_host, c->red, sizeof(double) * kIfaceSlots, hipMemcpyDeviceToHost, c->stream));
@@ -105,7 +105,7 @@ int surface_apply(pf_ctx* c, const pf_phase* ph, const IfaceArgs& a, double dt, 
     HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->iface_bad), 0xffffffffu, 2));
     const long long ki = kbegin(c, PF_K_SURFACE);
     hipLaunchKernelGGL(k_csf_delta, dim3(nwg), dim3(256), 0, c->stream, cs, c->partial);
-    hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(64), 0, c->stream, c->partial, (int)nwg, ops, c->red);
+    hipLaunchKernelGGL(k_iface_combine, dim3(1), dim3(kCombineThreads), 0, c->stream, c->partial, (int)nwg, ops, c->red);
     const Dims* fd[3] = {&c->du, &c->dv, &c->dw};
     for (int ax = 0; ax < 3; ++ax)
       hipLaunchKernelGGL(k_csf_faces, dim3(grid_for(fd[ax]->nblocks * kBlockVol, 256)), dim3(256), 0, c->stream,

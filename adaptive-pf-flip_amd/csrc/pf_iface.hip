@@ -218,21 +218,44 @@ __global__ void __launch_bounds__(256) k_iface_diag(IfaceArgs a, double* __restr
   }
 }
 
-// combine the partials of nwg workgroups in workgroup order, one thread per slot;
-// op 0 = sum, 1 = min, 2 = max
+// the combine of per-workgroup partials: n slots, op 0 = sum, 1 = min, 2 = max
 struct CombineOps {
   int n;
   int op[16];
 };
-__global__ void k_iface_combine(const double* __restrict__ part, int nwg, CombineOps ops, double* __restrict__ out) {
-  const int t = threadIdx.x;
-  if (t >= ops.n) return;
-  double x = part[t];
-  for (int w = 1; w < nwg; ++w) {
-    const double y = part[(long long)w * ops.n + t];
-    x = ops.op[t] == 1 ? fmin(x, y) : (ops.op[t] == 2 ? fmax(x, y) : x + y);
+// Per-workgroup partials part[w * ops.n + t] combined per slot t by ops.op[t] (0 sum,
+// 1 min, 2 max) in a fixed order: thread h folds workgroups h, h + 256, ... in turn, then
+// a pairwise tree over the 256 threads.  The order does not depend on timing, so the
+// result is the same from run to run.  fmin/fmax start from NaN, which they ignore, so
+// a slot whose partials are all NaN stays NaN.
+constexpr int kCombineThreads = 256;
+__device__ inline double combine_op(int op, double x, double y) {
+  return op == 1 ? fmin(x, y) : (op == 2 ? fmax(x, y) : x + y);
+}
+__global__ void __launch_bounds__(kCombineThreads) k_iface_combine(const double* __restrict__ part, int nwg,
+                                                                   CombineOps ops, double* __restrict__ out) {
+  __shared__ double s[16][kCombineThreads];
+  const int h = threadIdx.x;
+  double x[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) x[t] = ops.op[t] == 0 ? 0.0 : NAN;
+  for (int w = h; w < nwg; w += kCombineThreads) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      if (t < ops.n) x[t] = combine_op(ops.op[t], x[t], part[(long long)w * ops.n + t]);
   }
-  out[t] = x;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) s[t][h] = x[t];
+  __syncthreads();
+  for (int half = kCombineThreads / 2; half > 0; half >>= 1) {
+    if (h < half) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        if (t < ops.n) s[t][h] = combine_op(ops.op[t], s[t][h], s[t][h + half]);
+    }
+    __syncthreads();
+  }
+  if (h < ops.n) out[h] = s[h][0];
 }
 
 struct CsfArgs {

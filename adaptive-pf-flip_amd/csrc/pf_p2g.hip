@@ -26,6 +26,8 @@
 //   correctly rounded FP64 iff d^2 < 2.25).  It is evaluated in FP32 and recomputed
 //   in FP64 inside a 1e-5 band around 2.25, so active-block sets match the FP64
 //   reference bit for bit while weights run in FP32.
+#include <type_traits>
+
 #include "pf_internal.h"
 
 namespace pf {
@@ -235,12 +237,12 @@ __device__ inline float node_weights(const float* dx2, const float* dy2, const f
 // from FP64 coordinates.
 template <int AX, int KT, bool F32>
 __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, int j, int k, uint32_t s0,
-                                  uint32_t cnt, const uint8_t* ord, float* s_mom, float* s_mass, uint8_t* s_flag,
+                                  uint32_t cnt, const uint8_t* ord, float2* s_node, uint8_t* s_flag,
                                   int lx, int ly, int lz) {
   using A = Ax<AX>;
   const Dims& fd = f.df;
   float am[A::NN], aq[A::NN];
-  uint64_t fbits = 0;
+  uint64_t fbits = 0;  // activation bits of the second loop
 #pragma unroll
   for (int n = 0; n < A::NN; ++n) { am[n] = 0.f; aq[n] = 0.f; }
 
@@ -371,9 +373,6 @@ __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, 
       }
       accumulate(wt, wsum, mp, vel);
     }
-#pragma unroll
-    for (int n = 0; n < A::NN; ++n)
-      if (am[n] > 0.f) fbits |= 1ull << n;
   } else if (masked && cnt > 0) {
     marked = (cnt >= 64u) ? ~0ull : ((1ull << cnt) - 1ull);
   }
@@ -445,27 +444,43 @@ __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, 
   }
   // deposit into the LDS region, one stencil offset n at a time: for fixed n the 64
   // lanes address 64 distinct nodes, and one wave's LDS operations execute in program
-  // order, so a plain read-add-write per n (volatile: kept in order, not merged) adds a
-  // node's lane sums in n order with no LDS atomics (float atomics in LDS were half of
-  // the kernel's time)
-  volatile float* vm = s_mom;
-  volatile float* vs = s_mass;
+  // order, so a plain read-add-write per n adds a node's lane sums in n order with no
+  // LDS atomics (float atomics in LDS were half of the kernel's time).  The empty asm
+  // with a memory clobber after each n keeps the compiler from moving offset n+1's
+  // read above offset n's write (to one lane they are different addresses; to the
+  // wave they can be the same node).  A node's (momentum, mass) pair is one float2, so
+  // each offset is one ds_read_b64 and one ds_write_b64 off the lane's base address.
+  // (A volatile pointer kept the order too, but lost the LDS address space: flat loads
+  // and stores with a wait on each.)
+  // A node takes the lane's sums where they hold mass (every positive weight of the
+  // first loop), or where the second loop set its activation bit (which it does for
+  // every positive weight, and for the zero-weight nodes that still activate); a wave
+  // with no second-loop bits skips that test.
   const int base = lx + A::RX * (ly + A::RY * lz);
+  auto deposit = [&](auto with_bits) {
 #pragma unroll
-  for (int oz = 0; oz < A::NZ; ++oz)
+    for (int oz = 0; oz < A::NZ; ++oz)
 #pragma unroll
-    for (int oy = 0; oy < A::NY; ++oy)
+      for (int oy = 0; oy < A::NY; ++oy)
 #pragma unroll
-      for (int ox = 0; ox < A::NX; ++ox) {
-        const int n = ox + A::NX * (oy + A::NY * oz);
-        if ((fbits >> n) & 1ull) {
-          const int r = base + ox + A::RX * (oy + A::RY * oz);
-          const float m0 = vm[r], w0 = vs[r];
-          vm[r] = m0 + aq[n];
-          vs[r] = w0 + am[n];
-          s_flag[r] = 1;
+        for (int ox = 0; ox < A::NX; ++ox) {
+          const int n = ox + A::NX * (oy + A::NY * oz);
+          const bool put = am[n] > 0.f || (decltype(with_bits)::value && ((fbits >> n) & 1ull));
+          if (put) {
+            const int r = base + ox + A::RX * (oy + A::RY * oz);
+            float2 v = s_node[r];
+            v.x = v.x + aq[n];
+            v.y = v.y + am[n];
+            s_node[r] = v;
+            s_flag[r] = 1;
+          }
+          __asm__ volatile("" ::: "memory");
         }
-      }
+  };
+  if (__any(fbits != 0ull))
+    deposit(std::true_type{});
+  else
+    deposit(std::false_type{});
 }
 
 // Adds the block's region sums into the face field and clears the region for the next
@@ -473,7 +488,7 @@ __device__ inline void splat_axis(const P2GShared& a, const P2GField& f, int i, 
 // addition per launch: a no-return L2 float atomic gives the same sum as a read-modify-
 // write, in the same order (colour by colour), without the wave waiting for the read.
 template <int AX>
-__device__ inline void flush_axis(const P2GField& f, int bx, int by, int bz, float* s_mom, float* s_mass,
+__device__ inline void flush_axis(const P2GField& f, int bx, int by, int bz, float2* s_node,
                                   uint8_t* s_flag, int lane) {
   using A = Ax<AX>;
   constexpr int kPer = (A::RN + 63) / 64;  // region nodes per lane
@@ -487,15 +502,15 @@ __device__ inline void flush_axis(const P2GField& f, int bx, int by, int bz, flo
       const int ni = bx * 4 - 1 + rx, nj = by * 4 - 1 + ry, nk = bz * 4 - 1 + rz;
       if (ni >= 0 && nj >= 0 && nk >= 0 && ni < fd.nx && nj < fd.ny && nk < fd.nz) {
         const long long gi = bm_index(fd, ni, nj, nk);
-        unsafeAtomicAdd(&f.mom[gi], s_mom[r]);
-        unsafeAtomicAdd(&f.mass[gi], s_mass[r]);
+        const float2 v = s_node[r];
+        unsafeAtomicAdd(&f.mom[gi], v.x);
+        unsafeAtomicAdd(&f.mass[gi], v.y);
         const long long blk = gi >> 6;
         f.act_vel[blk] = 1;
         f.act_mass[blk] = 1;
       }
     }
-    s_mom[r] = 0.f;
-    s_mass[r] = 0.f;
+    s_node[r] = make_float2(0.f, 0.f);
     s_flag[r] = 0;
   }
 }
@@ -503,20 +518,18 @@ __device__ inline void flush_axis(const P2GField& f, int bx, int by, int bz, flo
 // KT = 0: normalized cubic (two-phase); KT = 1: trilinear, unit mass (single-phase);
 // KT = 2: normalized cubic over the enumerated multires faces.  One launch per colour:
 // a block's particles are ranked once and splatted into the three face fields in turn
-// through one LDS node region (7 KB of LDS per workgroup).
+// through one LDS node region (12.3 KB of LDS per workgroup with the id ranking).
 template <int KT, bool F32>
 __global__ void __launch_bounds__(64) k_p2g_color(P2GShared a, P2GField fu, P2GField fv, P2GField fw) {
   __shared__ uint32_t s_ids[kOrdMax * 64];
   __shared__ uint8_t s_ord[64 * kOrdMax];
-  __shared__ float s_mom[256];
-  __shared__ float s_mass[256];
+  __shared__ float2 s_node[256];  // the region's (momentum, mass) sums per node
   __shared__ uint8_t s_flag[256];
   if (blockIdx.x >= *a.count) return;
   const uint32_t b = a.list[blockIdx.x];
   const int lane = threadIdx.x;
   for (int t = lane; t < 256; t += 64) {
-    s_mom[t] = 0.f;
-    s_mass[t] = 0.f;
+    s_node[t] = make_float2(0.f, 0.f);
     s_flag[t] = 0;
   }
   int bx, by, bz;
@@ -544,17 +557,17 @@ __global__ void __launch_bounds__(64) k_p2g_color(P2GShared a, P2GField fu, P2GF
     ord[0] = 0;
   }
   __syncthreads();
-  splat_axis<0, KT, F32>(a, fu, i, j, k, s0, cnt, ord, s_mom, s_mass, s_flag, lx, ly, lz);
+  splat_axis<0, KT, F32>(a, fu, i, j, k, s0, cnt, ord, s_node, s_flag, lx, ly, lz);
   __syncthreads();
-  flush_axis<0>(fu, bx, by, bz, s_mom, s_mass, s_flag, lane);
+  flush_axis<0>(fu, bx, by, bz, s_node, s_flag, lane);
   __syncthreads();
-  splat_axis<1, KT, F32>(a, fv, i, j, k, s0, cnt, ord, s_mom, s_mass, s_flag, lx, ly, lz);
+  splat_axis<1, KT, F32>(a, fv, i, j, k, s0, cnt, ord, s_node, s_flag, lx, ly, lz);
   __syncthreads();
-  flush_axis<1>(fv, bx, by, bz, s_mom, s_mass, s_flag, lane);
+  flush_axis<1>(fv, bx, by, bz, s_node, s_flag, lane);
   __syncthreads();
-  splat_axis<2, KT, F32>(a, fw, i, j, k, s0, cnt, ord, s_mom, s_mass, s_flag, lx, ly, lz);
+  splat_axis<2, KT, F32>(a, fw, i, j, k, s0, cnt, ord, s_node, s_flag, lx, ly, lz);
   __syncthreads();
-  flush_axis<2>(fw, bx, by, bz, s_mom, s_mass, s_flag, lane);
+  flush_axis<2>(fw, bx, by, bz, s_node, s_flag, lane);
 }
 
 }  // namespace pf
