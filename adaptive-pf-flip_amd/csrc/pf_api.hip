@@ -239,6 +239,7 @@ int pf_particles_upload(pf_ctx* c, int64_t n, const float* x, const float* y, co
   if (st) return st;
   c->np = n;
   c->has_vol = volume != nullptr;
+  c->vmax_valid = false;
   if (n > 0) {
     size_t fb = sizeof(float) * (size_t)n;
     HIP_TRY(c, hipMemcpyAsync(c->pa.x, x, fb, hipMemcpyHostToDevice, c->stream));
@@ -328,6 +329,7 @@ int pf_scene_generate(pf_ctx* c, int32_t kind, uint64_t seed) {
   if (st) return st;
   c->np = n;
   c->has_vol = false;
+  c->vmax_valid = false;
   if (n > 0) {
     hipLaunchKernelGGL(k_scene, dim3(grid_for(ncells, 256)), dim3(256), 0, c->stream, (int)kind, g, ie, je, seed, c->pa);
     LAUNCH_CHECK(c);
@@ -399,11 +401,16 @@ int pf_advect(pf_ctx* c, double dt, int32_t order, pf_escape_stats* stats) {
 int pf_timestep(pf_ctx* c, double requested_dt, int32_t enabled, double cfl, double min_dt, double* eff,
                 double* vmax_out, double* cfl_dt, int32_t* limited) {
   if (!c) return PF_ERR_ARG;
-  HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->vmax_dev), 0u, 1));
-  if (c->np > 0) {
-    hipLaunchKernelGGL(k_max_speed, dim3(std::min(grid_for(c->np, 256), 8192u)), dim3(256), 0, c->stream, c->np, c->pa,
-                       reinterpret_cast<uint32_t*>(c->vmax_dev));
-    LAUNCH_CHECK(c);
+  // the scan is skipped while the last G2P's own maximum (or the last scan) still
+  // describes the particles' velocities
+  if (!c->vmax_valid) {
+    HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->vmax_dev), 0u, 1));
+    if (c->np > 0) {
+      hipLaunchKernelGGL(k_max_speed, dim3(std::min(grid_for(c->np, 256), 8192u)), dim3(256), 0, c->stream, c->np,
+                         c->pa, reinterpret_cast<uint32_t*>(c->vmax_dev));
+      LAUNCH_CHECK(c);
+    }
+    c->vmax_valid = true;
   }
   float vm = 0.f;
   HIP_TRY(c, hipMemcpyAsync(&vm, c->vmax_dev, sizeof(float), hipMemcpyDeviceToHost, c->stream));

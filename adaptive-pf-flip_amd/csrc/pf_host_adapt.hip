@@ -214,6 +214,7 @@ int run_adaptivity(pf_ctx* c, const pf_adapt_cfg* cfg, pf_adapt_stats* out) {
   }
   c->np = n_new;
   c->binned = false;
+  c->vmax_valid = false;
   st = update_liquid_volume(c);
   if (st) return st;
   S.narrow_band_air_liquid_cells = (int32_t)cnt[PA_NB_LIQUID_CELLS];

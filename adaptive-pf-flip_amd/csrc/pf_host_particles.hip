@@ -179,9 +179,12 @@ int run_g2p_advect(pf_ctx* c, bool g2p, bool adv, double aL, double aG, double d
   a.do_g2p = g2p; a.do_advect = adv;
   a.esc = c->esc_dev;
   a.vmax_bits = reinterpret_cast<uint32_t*>(c->vmax_dev);
+  // the G2P reduces the max speed of the velocities it writes (the next CFL scan's value)
+  if (g2p) HIP_TRY(c, fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->vmax_dev), 0u, 1));
   TIMED(c, PF_K_G2P_ADVECT,
         hipLaunchKernelGGL(k_g2p_advect, dim3(xcd_grid(c->np, 256, 8192)), dim3(256), 0, c->stream, a));
   LAUNCH_CHECK(c);
+  if (g2p) c->vmax_valid = true;
   if (adv) c->binned = false;
   return PF_OK;
 }

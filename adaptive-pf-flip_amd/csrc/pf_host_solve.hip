@@ -100,9 +100,12 @@ int mg_build(pf_ctx* c) {
   // the FP16 level-0 operator of the row kernels
   {
     const long long n0 = c->dc.nblocks * kBlockVol;
-    HIP_TRY(c, fill_u32(c->stream, c->mg_mx, 0u, 2));
-    hipLaunchKernelGGL(k_mg_absmax, dim3(std::min<unsigned>(grid_for(n0, 256), 2048u)), dim3(256), 0, c->stream, n0,
-                       c->cd32, c->cx32, c->cy32, c->cz32, c->invd32, c->mg_mx);
+    if (!c->mg_mx_valid) {
+      HIP_TRY(c, fill_u32(c->stream, c->mg_mx, 0u, 2));
+      hipLaunchKernelGGL(k_mg_absmax, dim3(std::min<unsigned>(grid_for(n0, 256), 2048u)), dim3(256), 0, c->stream,
+                         n0, c->cd32, c->cx32, c->cy32, c->cz32, c->invd32, c->mg_mx);
+      c->mg_mx_valid = true;
+    }
     _Float16* h[5];
     for (int q = 0; q < 5; ++q) h[q] = reinterpret_cast<_Float16*>(c->mg_h[q]);
     hipLaunchKernelGGL(k_mg_tohalf, dim3(std::min<unsigned>(grid_for(n0, 256), 4096u)), dim3(256), 0, c->stream, n0,
@@ -332,6 +335,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const Geom& g = c->geom;
   const long long nb = c->dc.nblocks;
   // 1. blocks of the previous solve return to zero; the pressure field is cleared
+  c->mg_mx_valid = false;
   if (c->npblk_prev > 0) {
     const bool warm = cfg->warm_start != 0;
     if (warm) {
@@ -455,10 +459,25 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   ao.part = c->partial;
   ao.fbeta = lpc_system ? c->fbeta : nullptr;
   ao.fphi = lpc_system ? c->fphi : nullptr;
+  // the FP16 scaling of a multigrid build that may follow needs the operator's maxima:
+  // the assembly reduces them from the values it writes (cells outside the list are zero)
+  const bool want_mx = want_mg || want_learned || lpc_system;
+  ao.mx = nullptr;
+  if (want_mx) {
+    if (!c->mg_t0) {
+      int s2 = mg_alloc(c);
+      if (s2) return s2;
+    }
+    HIP_TRY(c, fill_u32(c->stream, c->mg_mx, 0u, 2));
+    ao.mx = c->mg_mx;
+  }
   TIMED(c, PF_K_ASSEMBLE, hipLaunchKernelGGL(k_assemble, dim3(G), dim3(256), 0, c->stream, pa, ao));
   LAUNCH_CHECK(c);
+  c->mg_mx_valid = want_mx;
   }
-  hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 1, 0u, c->red + 2);
+  // red[2] = sum b^2; red[3] = max |b| (sparse assembly only)
+  hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, mr ? 1 : 2, 2u,
+                     c->red + 2);
   LAUNCH_CHECK(c);
   record(c, timing, ev_assemble);
 
@@ -474,6 +493,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     mop.cd = c->cd32; mop64.cd = c->cd64; mop64.vol = c->mr_vol64;
     for (int a = 0; a < 3; ++a) { mop.cf[a] = c->mr_cf32[a]; mop64.cf[a] = c->mr_cf64[a]; }
   }
+  const double* r_src = c->r64;  // the FP64 residual the next (re)start loads
   auto residual = [&]() -> int {
     if (mr)
       TIMED(c, PF_K_RESIDUAL64,
@@ -485,12 +505,18 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->partial, (int)G, 2, 2u, c->red);
     LAUNCH_CHECK(c);
+    r_src = c->r64;
     return read_small(c);
   };
-  int st = residual();
+  // From x = 0 (no warm start) the FP64 residual is b itself: its sum of squares and max
+  // norm come from the assembly's reduction (the same partition and order, so the same
+  // bits), and the first load reads b64 -- no residual pass.
+  const bool zero_start = !mr && !cfg->warm_start;
+  int st = zero_start ? read_small(c) : residual();
   if (st) return st;
+  if (zero_start) r_src = c->b64;
   const double bnorm2 = c->red_host[2];
-  double rr = c->red_host[0], rmax = c->red_host[1];
+  double rr = zero_start ? c->red_host[2] : c->red_host[0], rmax = zero_start ? c->red_host[3] : c->red_host[1];
   S.rhs_norm2 = bnorm2;
   S.initial_max_residual = rmax;
   S.pinned = c->u32_host[0] == 0;
@@ -574,7 +600,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   if (!refmode && !converged && max_iters > 0 && std::isfinite(rmax)) {
     // (re)start: r32 from r64, x32 = 0, p = 0, z = M(r)
     auto start = [&](int budget) -> int {
-      hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, c->r64, c->x32, c->r32, c->z32, c->pA,
+      hipLaunchKernelGGL(k_pcg_load, dim3(G), dim3(256), 0, c->stream, va, r_src, c->x32, c->r32, c->z32, c->pA,
                          c->invd32, learned ? 1 : 0);
       LAUNCH_CHECK(c);
       if (learned) {

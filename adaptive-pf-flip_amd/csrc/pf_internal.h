@@ -171,6 +171,7 @@ struct Ctx {
   uint32_t* u32_host = nullptr;     // pinned mirror, 64 words
   int32_t* esc_dev = nullptr;       // 8 clamp counters
   float* vmax_dev = nullptr;        // max particle speed (as float bits via atomicMax)
+  bool vmax_valid = false;          // vmax_dev holds the max speed of the current velocities
 
   // scan scratch
   uint32_t* scan_tiles = nullptr;
@@ -249,6 +250,7 @@ struct Ctx {
   // maxima (float bits) and scales
   uint16_t *mg_h[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   uint32_t* mg_mx = nullptr;
+  bool mg_mx_valid = false;   // mg_mx holds the maxima of the current system (set by the assembly)
   float* mg_unscale = nullptr;
   // per-aggregate scaling s = sqrt(w) of level 1's restricted residual and correction
   // (P S B_1 S P^T in place of P B_1 P^T), or null: w = 1.  mg_cs holds the caller's

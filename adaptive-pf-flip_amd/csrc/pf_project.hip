@@ -253,15 +253,18 @@ struct AsmOut {
   double *b64, *cd64, *cx64, *cy64, *cz64;
   float *cd32, *cx32, *cy32, *cz32, *invd32;
   const float *u, *v, *w, *mu, *mv, *mw;
-  double* part;   // slot 0: sum b^2
+  double* part;   // slot 0: sum b^2, slot 1: max |b| (the FP64 residual of x = 0)
   float *fbeta, *fphi;  // optional learned-preconditioner features (mean face beta / phi)
+  uint32_t* mx;   // optional: max of the FP32 cd, cx, cy, cz and max of inv (float bits,
+                  // atomicMax; the values are >= 0), what the multigrid build scales by
 };
 
 // RHS and assembly, sparse_ops3d_tp.cpp:266-313
 __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
-  double bb = 0.0;
+  double bb = 0.0, bmax = 0.0;
+  float amax = 0.f, imax = 0.f;
   FOR_LIST_BLOCKS(a, b, lane) {
     const long long c = b * kBlockVol + lane;
     int bx, by, bz;
@@ -331,8 +334,11 @@ __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
     }
     o.b64[c] = rhs;
     o.cd64[c] = cd; o.cx64[c] = cx; o.cy64[c] = cy; o.cz64[c] = cz;
-    o.cd32[c] = (float)cd; o.cx32[c] = (float)cx; o.cy32[c] = (float)cy; o.cz32[c] = (float)cz;
-    o.invd32[c] = (float)inv;
+    const float cd32 = (float)cd, cx32 = (float)cx, cy32 = (float)cy, cz32 = (float)cz, inv32 = (float)inv;
+    o.cd32[c] = cd32; o.cx32[c] = cx32; o.cy32[c] = cy32; o.cz32[c] = cz32;
+    o.invd32[c] = inv32;
+    amax = fmaxf(amax, fmaxf(fmaxf(cd32, cx32), fmaxf(cy32, cz32)));
+    imax = fmaxf(imax, inv32);
     if (o.fbeta) {
       // cell features of the learned preconditioner: mean beta and mean phi over the
       // six faces (the cell phi of interface_diagnostics3d.cpp:137-148)
@@ -358,10 +364,23 @@ __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
       o.fphi[c] = fp_mean;
     }
     bb += rhs * rhs;
+    bmax = fmax(bmax, fabs(rhs));
   }
-  double v[1] = {bb};
-  const bool mx[1] = {false};
-  block_reduce_store<1>(v, mx, o.part, kMaxPartials);
+  double v[2] = {bb, bmax};
+  const bool mx[2] = {false, true};
+  block_reduce_store<2>(v, mx, o.part, kMaxPartials);
+  if (o.mx) {
+    // (fmaxf drops a NaN, as the separate pass over the slab does)
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+      amax = fmaxf(amax, __shfl_xor(amax, s, 64));
+      imax = fmaxf(imax, __shfl_xor(imax, s, 64));
+    }
+    if (lane == 0) {
+      atomicMax(&o.mx[0], __float_as_uint(amax));
+      atomicMax(&o.mx[1], __float_as_uint(imax));
+    }
+  }
 }
 
 // zero every per-cell array of the listed blocks (blocks that leave the fluid set
