@@ -76,13 +76,19 @@ void kend(pf_ctx* c, long long i) {
     kend((c), _ki);                     \
   } while (0)
 
+// A zeroed device array.  The fill goes to the null stream, which is not ordered with
+// the library's non-blocking streams, and may still be running when the call returns:
+// it is waited for here, or it could land after the first kernels on a context stream
+// have written the array (buffers allocated inside a solve are written at once).
 template <typename T>
 hipError_t dalloc(T** p, long long n) {
   *p = nullptr;
   if (n <= 0) n = 1;
   hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
   if (e != hipSuccess) return e;
-  return hipMemset(*p, 0, sizeof(T) * (size_t)n);
+  e = hipMemsetAsync(*p, 0, sizeof(T) * (size_t)n, nullptr);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(nullptr);
 }
 
 unsigned grid_for(long long work, long long per_block) {

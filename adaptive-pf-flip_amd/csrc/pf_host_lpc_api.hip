@@ -83,8 +83,9 @@ int pf_lpc_set_weights(pf_lpc* L, const float* w, int64_t n) {
     if (!std::isfinite(w[i])) { L->err = "non-finite weight"; return PF_ERR_ARG; }
   HIP_TRY(L, hipStreamSynchronize(L->stream));
   HIP_TRY(L, hipMemcpy(L->w, w, sizeof(float) * kNW, hipMemcpyHostToDevice));
-  HIP_TRY(L, hipMemset(L->m, 0, sizeof(float) * kNW));
-  HIP_TRY(L, hipMemset(L->v, 0, sizeof(float) * kNW));
+  // (on the network's stream, so the next training step sees the cleared moments)
+  HIP_TRY(L, hipMemsetAsync(L->m, 0, sizeof(float) * kNW, L->stream));
+  HIP_TRY(L, hipMemsetAsync(L->v, 0, sizeof(float) * kNW, L->stream));
   L->adam_t = 0;
   L->prepped = false;
   return PF_OK;

@@ -334,8 +334,12 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   S.pinned_cell = -1;
   const Geom& g = c->geom;
   const long long nb = c->dc.nblocks;
-  // 1. blocks of the previous solve return to zero; the pressure field is cleared
+  // 1. the blocks holding FLUID cells (flags; sorted list below)
   c->mg_mx_valid = false;
+  uint8_t* flag8 = reinterpret_cast<uint8_t*>(c->blk_flag32);
+  hipLaunchKernelGGL(k_fluid_flags, dim3(grid_for(nb, 4)), dim3(256), 0, c->stream, c->dc, c->marker, flag8);
+  LAUNCH_CHECK(c);
+  // 2. blocks of the previous solve return to zero; the pressure field is cleared
   if (c->npblk_prev > 0) {
     const bool warm = cfg->warm_start != 0;
     if (warm) {
@@ -346,23 +350,27 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     ZeroArgs z{};
     z.list = c->pblk_prev;
     z.nlist = c->npblk_prev;
-    // with the multigrid work array: its values at cells outside the next system are
-    // read (times a zero coupling) by the level-0 sweeps, so none may stay non-finite
-    float* f32[13] = {c->f[F_P], c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->cd32, c->cx32, c->cy32, c->cz32,
-                      c->invd32, c->mg_t0};
-    for (int q = 0; q < 13; ++q) z.f32[q] = f32[q];
-    z.nf32 = c->mg_t0 ? 13 : 12;
+    // Cleared on every block of the previous solve: the pressure and the iteration
+    // vectors (with the multigrid work array: its values at cells outside the next
+    // system are read, times a zero coupling, by the level-0 sweeps, so none may stay
+    // non-finite).  Only on the blocks that leave the system (sparse path): the
+    // assembly's outputs, which k_assemble writes on all 64 cells of every listed
+    // block, x64 (k_ir_start writes it there) and r64 (written before it is read).
+    int nf = 0;
+    for (float* p : {c->f[F_P], c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->mg_t0})
+      if (p) z.f32[nf++] = p;
+    z.nf32_all = nf;
+    for (float* p : {c->cd32, c->cx32, c->cy32, c->cz32, c->invd32}) z.f32[nf++] = p;
+    z.nf32 = nf;
     double* f64[7] = {c->b64, c->r64, c->cd64, c->cx64, c->cy64, c->cz64, c->x64};
     for (int q = 0; q < 7; ++q) z.f64[q] = f64[q];
     z.nf64 = warm ? 6 : 7;  // a warm start keeps x64 (the saved pressure)
+    z.keep = mr ? nullptr : flag8;
     hipLaunchKernelGGL(k_zero_blocks, dim3(grid_for(c->npblk_prev, 4)), dim3(256), 0, c->stream, z);
     LAUNCH_CHECK(c);
   }
   HIP_TRY(c, fill_u8(c->stream, c->act[F_P], 0, nb));
-  // 2. sorted list of blocks holding FLUID cells
-  uint8_t* flag8 = reinterpret_cast<uint8_t*>(c->blk_flag32);
-  hipLaunchKernelGGL(k_fluid_flags, dim3(grid_for(nb, 4)), dim3(256), 0, c->stream, c->dc, c->marker, flag8);
-  LAUNCH_CHECK(c);
+  // 3. the sorted list
   HIP_TRY(c, compact_flags_morton(*c, c->dc, flag8, c->pblk, c->u32_dev + 2));
   HIP_TRY(c, fill_u32(c->stream, c->u32_dev, 0u, 1));
   HIP_TRY(c, fill_u32(c->stream, c->u32_dev + 1, 0xffffffffu, 1));
@@ -481,7 +489,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   LAUNCH_CHECK(c);
   record(c, timing, ev_assemble);
 
-  // 3. FP64 iterative refinement around the FP32 Jacobi PCG
+  // 4. FP64 iterative refinement around the FP32 Jacobi PCG
   VecArgs va;
   va.dc = c->dc; va.list = c->pblk; va.nlist = npb; va.sc = c->sc; va.part = c->partial;
   hipLaunchKernelGGL(k_ir_start, dim3(G), dim3(256), 0, c->stream, va, c->x64, c->invd32, cfg->warm_start);
@@ -780,7 +788,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   if (mr) S.pinned_cell = S.pinned ? 0 : -1;  // index in the reference's FLUID cell order
   record(c, timing, ev_solve);
 
-  // 4. pressure store and face velocity update
+  // 5. pressure store and face velocity update
   if (mr) {
     const long long ncell = c->dc.nblocks * kBlockVol;
     hipLaunchKernelGGL(k_mr_store_pressure, dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, mr_grid(c), c->x64,

@@ -385,20 +385,29 @@ __global__ void __launch_bounds__(256) k_assemble(ProjArgs a, AsmOut o) {
 
 // zero every per-cell array of the listed blocks (blocks that leave the fluid set
 // must read as zero; see the reference's pf.clear(), sparse_ops3d_tp.cpp:240)
+// The first nf32_all FP32 arrays are cleared on every listed block; the others (and
+// every FP64 array) only on blocks that leave the system (keep[b] == 0, or every
+// block when keep is null): the next solve rewrites them on all 64 cells of its blocks
+// before reading them.
 struct ZeroArgs {
   const uint32_t* list;
   long long nlist;
   float* f32[13];
-  int nf32;
+  int nf32, nf32_all;
   double* f64[7];
   int nf64;
+  const uint8_t* keep;   // per block: 1 if it holds FLUID cells of the next system
 };
 __global__ void __launch_bounds__(256) k_zero_blocks(ZeroArgs z) {
   long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= z.nlist) return;
-  long long c = (long long)z.list[t] * kBlockVol + (threadIdx.x & 63);
-  for (int q = 0; q < z.nf32; ++q) z.f32[q][c] = 0.f;
-  for (int q = 0; q < z.nf64; ++q) z.f64[q][c] = 0.0;
+  const uint32_t b = z.list[t];
+  const long long c = (long long)b * kBlockVol + (threadIdx.x & 63);
+  const bool stays = z.keep && z.keep[b];
+  for (int q = 0; q < z.nf32; ++q)
+    if (q < z.nf32_all || !stays) z.f32[q][c] = 0.f;
+  if (!stays)
+    for (int q = 0; q < z.nf64; ++q) z.f64[q][c] = 0.0;
 }
 
 // ------------------------------------------------------------- solve kernels

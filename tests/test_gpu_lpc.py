@@ -182,6 +182,42 @@ def test_training_lowers_the_loss():
     assert st.converged == 1 and st.true_rel_residual <= 1e-6 and st.fallback_used == 0
 
 
+def test_learned_solve_against_plain_pcg():
+    """The learned preconditioner as shipped (no trained weights ship, so its network
+    has the zero head: the multigrid cycle) inside the flexible PCG + FP64 refinement,
+    against plain Jacobi PCG + FP64 refinement on the same 128^3 RT system at 1000:1:
+    the same FP64 tolerance, no fallback to Jacobi, at most a tenth of the iterations
+    and at most half the wall time (best of two solves each)."""
+    import time
+
+    n = 128
+    ctx, ph = _mg_system(n, O.SCENE_RT, seed=7)
+    net = G.LearnedPreconditioner(seed=7)
+    net.attach(ctx)
+    out = {}
+    for name, pre in (("plain", G.PRECOND_JACOBI), ("learned", G.PRECOND_LEARNED)):
+        best = None
+        for _ in range(2):
+            _prepare(ctx, ph)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            st = ctx.project(ph, 1e-3, _cfg(pre))
+            ctx.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            assert st.converged == 1 and st.true_rel_residual <= 1e-6, (name, st.true_rel_residual)
+            if best is None or ms < best[0]:
+                best = (ms, st.iterations, st.fallback_used)
+        out[name] = best
+    print(f"[lpc] 128^3 RT solve: plain PCG {out['plain'][0]:.1f} ms {out['plain'][1]} it, "
+          f"learned {out['learned'][0]:.1f} ms {out['learned'][1]} it", flush=True)
+    assert out["learned"][2] == 0
+    assert out["learned"][1] * 10 <= out["plain"][1]
+    assert out["learned"][0] * 2 <= out["plain"][0]
+    G.lib().pf_lpc_attach(ctx.h, None)
+    net.close()
+    ctx.close()
+
+
 def test_objective_rejects_invalid_values():
     net = G.LearnedPreconditioner(seed=1)
     for obj, p in ((2, 0), (-1, 0), (G.LPC_OBJECTIVE_ENERGY, 65), (G.LPC_OBJECTIVE_ENERGY, -1)):
