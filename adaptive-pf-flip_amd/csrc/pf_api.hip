@@ -221,6 +221,7 @@ int pf_destroy(pf_ctx* c) {
   hipFree(c->sc); hipFree(c->partial); hipFree(c->red);
   hipHostFree(c->red_host); hipHostFree(c->done_host); hipHostFree(c->sc_host); hipHostFree(c->u32_host);
   hipFree(c->u32_dev); hipFree(c->esc_dev); hipFree(c->vmax_dev);
+  for (auto* q : c->g2p_quad) hipFree(q);
   hipFree(c->scan_tiles); hipFree(c->stage_f);
   for (int e = 0; e <= PF_NSTAGES; ++e)
     if (c->ev[e]) hipEventDestroy(c->ev[e]);

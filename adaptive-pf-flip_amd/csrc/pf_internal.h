@@ -171,6 +171,7 @@ struct Ctx {
   uint32_t* u32_host = nullptr;     // pinned mirror, 64 words
   int32_t* esc_dev = nullptr;       // 8 clamp counters
   float* vmax_dev = nullptr;        // max particle speed (as float bits via atomicMax)
+  float4* g2p_quad[3] = {nullptr, nullptr, nullptr};  // per face (vel, saved) of it and of its +x neighbour (G2P)
   bool vmax_valid = false;          // vmax_dev holds the max speed of the current velocities
 
   // scan scratch
