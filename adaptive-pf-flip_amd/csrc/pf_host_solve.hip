@@ -41,6 +41,9 @@ MgLevel mg_level(const Ctx::MgBuf& b) {
 void mg_free_level(Ctx::MgBuf& b) {
   float** arrs[8] = {&b.cd, &b.cx, &b.cy, &b.cz, &b.inv, &b.x, &b.r, &b.t};
   for (float** a : arrs) dfree(*a);
+  for (auto& p : b.h) dfree(p);
+  dfree(b.mx);
+  dfree(b.unscale);
 }
 
 // every multigrid buffer, or none: a partial allocation is released and the context
@@ -69,6 +72,10 @@ int mg_alloc(pf_ctx* c) {
     float** arrs[8] = {&b.cd, &b.cx, &b.cy, &b.cz, &b.inv, &b.x, &b.r, &b.t};
     for (float** a : arrs)
       if (e == hipSuccess) e = dalloc(a, n);
+    for (auto& p : b.h)
+      if (e == hipSuccess) e = dalloc(&p, n);
+    if (e == hipSuccess) e = dalloc(&b.mx, 2);
+    if (e == hipSuccess) e = dalloc(&b.unscale, 2);
     levels.push_back(b);
   }
   const long long n0 = c->dc.nblocks * kBlockVol;
@@ -119,6 +126,15 @@ int mg_build(pf_ctx* c) {
     const unsigned g = std::min<unsigned>(grid_for(b.d.nblocks * kBlockVol, 256), 4096u);
     hipLaunchKernelGGL(k_mg_coarsen, dim3(g), dim3(256), 0, c->stream, F, Cl, b.cd, b.cx, b.cy, b.cz);
     hipLaunchKernelGGL(k_mg_inverse, dim3(g), dim3(256), 0, c->stream, Cl, b.inv);
+    // the FP16 copy the row kernels read (as level 0's)
+    const long long nl = b.d.nblocks * kBlockVol;
+    HIP_TRY(c, fill_u32(c->stream, b.mx, 0u, 2));
+    hipLaunchKernelGGL(k_mg_absmax, dim3(std::min<unsigned>(grid_for(nl, 256), 2048u)), dim3(256), 0, c->stream, nl,
+                       b.cd, b.cx, b.cy, b.cz, b.inv, b.mx);
+    _Float16* h[5];
+    for (int q = 0; q < 5; ++q) h[q] = reinterpret_cast<_Float16*>(b.h[q]);
+    hipLaunchKernelGGL(k_mg_tohalf, dim3(std::min<unsigned>(grid_for(nl, 256), 4096u)), dim3(256), 0, c->stream, nl,
+                       b.cd, b.cx, b.cy, b.cz, b.inv, h[0], h[1], h[2], h[3], h[4], b.mx, b.unscale);
     LAUNCH_CHECK(c);
     F = Cl;
   }
@@ -164,6 +180,59 @@ float* mg_coarse(pf_ctx* c, int l, const int* done, const std::vector<float>& wp
     hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(64), 0, c->stream, done, L, b.r, x, c->mg_cfg.omega,
                        c->mg_cfg.coarsest_sweeps);
     return x;
+  }
+  if (nu >= 2) {
+    // by cell rows, as level 0 (k_mg_row over every block of the slab, the level's
+    // FP16 operator): the two first sweeps fused, the restriction, then the
+    // prolongation fused with the first post-sweep -- four passes instead of six
+    Ctx::MgBuf& nb = c->mg[(size_t)l];
+    MgRowArgs ra{};
+    ra.list = nullptr;
+    ra.nlist = b.d.nblocks;
+    ra.done = done;
+    ra.d = b.d;
+    ra.dc = nb.d;
+    ra.cd = reinterpret_cast<const _Float16*>(b.h[0]);
+    ra.cx = reinterpret_cast<const _Float16*>(b.h[1]);
+    ra.cy = reinterpret_cast<const _Float16*>(b.h[2]);
+    ra.cz = reinterpret_cast<const _Float16*>(b.h[3]);
+    ra.inv = reinterpret_cast<const _Float16*>(b.h[4]);
+    ra.unscale = b.unscale;
+    ra.r = b.r;
+    ra.omega_c = c->mg_cfg.correction;
+    ra.part = nullptr;
+    const unsigned gw = list_grid(b.d.nblocks);
+    float* cur = nullptr;
+    float* nxt = x;
+    auto flip = [&]() {
+      cur = nxt;
+      nxt = (cur == x) ? t : x;
+    };
+    ra.x = nullptr; ra.out = nxt;
+    ra.omega_u = wpre[0];
+    ra.omega = wpre[1];
+    hipLaunchKernelGGL(k_mg_row<MG_FIRST2>, dim3(gw), dim3(256), 0, c->stream, ra);
+    flip();
+    for (int s2 = 2; s2 < nu; ++s2) {
+      ra.x = cur; ra.out = nxt;
+      ra.omega = wpre[(size_t)s2];
+      hipLaunchKernelGGL(k_mg_row<MG_SWEEP>, dim3(gw), dim3(256), 0, c->stream, ra);
+      flip();
+    }
+    ra.x = cur; ra.out = nb.r;
+    hipLaunchKernelGGL(k_mg_row<MG_RESTRICT>, dim3(gw), dim3(256), 0, c->stream, ra);
+    float* below = mg_coarse(c, l + 1, done, wpre);
+    ra.x = cur; ra.out = nxt; ra.xc = below;
+    ra.omega = wpre[(size_t)(nu - 1)];
+    hipLaunchKernelGGL(k_mg_row<MG_PSWEEP>, dim3(gw), dim3(256), 0, c->stream, ra);
+    flip();
+    for (int s2 = 1; s2 < nu; ++s2) {
+      ra.x = cur; ra.out = nxt;
+      ra.omega = wpre[(size_t)(nu - 1 - s2)];
+      hipLaunchKernelGGL(k_mg_row<MG_SWEEP>, dim3(gw), dim3(256), 0, c->stream, ra);
+      flip();
+    }
+    return cur;
   }
   const unsigned g = std::min<unsigned>(grid_for(b.d.nblocks * kBlockVol, 256), 2048u);
   hipLaunchKernelGGL(k_mg_first, dim3(g), dim3(256), 0, c->stream, wd, L, b.r, x, wpre[0]);

@@ -244,6 +244,11 @@ struct Ctx {
     int nx = 0, ny = 0, nz = 0;
     float *cd = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *inv = nullptr;
     float *x = nullptr, *r = nullptr, *t = nullptr;
+    // the level's operator as scaled FP16 for the row kernels (cd, cx, cy, cz, inv),
+    // its maxima (float bits) and scales
+    uint16_t* h[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t* mx = nullptr;
+    float* unscale = nullptr;
   };
   std::vector<MgBuf> mg;
   float* mg_t0 = nullptr;

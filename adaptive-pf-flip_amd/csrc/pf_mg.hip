@@ -18,7 +18,8 @@
 // damped sweeps on the coarsest level.  The sweeps of a pass commute, so the cycle is
 // a fixed symmetric preconditioner inside the flexible-beta PCG.  Level 0 walks the
 // FLUID block list and reads an FP16 copy of the operator; coarse levels are dense
-// block-major slabs (each 1/8 of the one below) in FP32.
+// block-major slabs (each 1/8 of the one below) in FP32, which the same row kernels
+// walk whole through their own scaled FP16 copies.
 #include "pf_internal.h"
 
 namespace pf {
@@ -331,7 +332,7 @@ __device__ inline float mg_ldh1(__amdgpu_buffer_rsrc_t r, uint32_t off, float sc
 enum { MG_FIRST2 = 0, MG_SWEEP = 1, MG_PSWEEP = 2, MG_RESTRICT = 3 };
 
 struct MgRowArgs {
-  const uint32_t* list;
+  const uint32_t* list;                      // the blocks to walk (null: the whole slab)
   long long nlist;
   const int* done;
   Dims d, dc;                                // level 0 and level 1 slabs
@@ -401,8 +402,11 @@ __global__ void __launch_bounds__(256) k_mg_row(MgRowArgs a) {
   const uint32_t last = e > 0 ? (uint32_t)(e - 1) : 0;
   for (long long sl = (long long)(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6); sl < nslot; sl += step) {
     const uint32_t t0 = wave_uniform_u32((uint32_t)(s0 + sl * 4));
-    const uint32_t b0 = list_at(a.list, t0), b1 = list_at(a.list, min(t0 + 1, last));
-    const uint32_t b2 = list_at(a.list, min(t0 + 2, last)), b3 = list_at(a.list, min(t0 + 3, last));
+    // (no list: every block of the slab, in block order)
+    const uint32_t b0 = a.list ? list_at(a.list, t0) : t0;
+    const uint32_t b1 = a.list ? list_at(a.list, min(t0 + 1, last)) : min(t0 + 1, last);
+    const uint32_t b2 = a.list ? list_at(a.list, min(t0 + 2, last)) : min(t0 + 2, last);
+    const uint32_t b3 = a.list ? list_at(a.list, min(t0 + 3, last)) : min(t0 + 3, last);
     const bool valid = (long long)t0 + qb < e;
     const uint32_t b = qb == 0 ? b0 : (qb == 1 ? b1 : (qb == 2 ? b2 : b3));
     int bx, by, bz;
