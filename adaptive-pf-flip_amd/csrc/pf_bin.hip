@@ -42,11 +42,21 @@ __device__ inline uint32_t wave_aggregated_increment(uint32_t* cnt, uint32_t key
   return __shfl(base, my_leader, 64) + my_off;
 }
 
-// the block-major index of a particle's cell
-__device__ inline uint32_t cell_key(float x, float y, float z, const Geom& g, const Dims& dc) {
-  int i = cell_coord(x, g.ox, g.dx);
-  int j = cell_coord(y, g.oy, g.dx);
-  int k = cell_coord(z, g.oz, g.dx);
+// the block-major index of a particle's cell.  fscale > 0 (origin 0 and a power-of-two
+// dx, fscale = 1/dx): x * fscale is exact in FP32 and equals the FP64 (x - 0) / dx, so
+// the index is the same without three FP64 divisions (which made the count pass
+// VALU-bound)
+__device__ inline uint32_t cell_key(float x, float y, float z, const Geom& g, const Dims& dc, float fscale) {
+  int i, j, k;
+  if (fscale > 0.f) {
+    i = (int)(x * fscale);
+    j = (int)(y * fscale);
+    k = (int)(z * fscale);
+  } else {
+    i = cell_coord(x, g.ox, g.dx);
+    j = cell_coord(y, g.oy, g.dx);
+    k = cell_coord(z, g.oz, g.dx);
+  }
   // positions are validated into the cell range on entry and clamped by advection;
   // the clamp only guards memory
   i = min(max(i, 0), g.nx - 1);
@@ -55,21 +65,21 @@ __device__ inline uint32_t cell_key(float x, float y, float z, const Geom& g, co
   return (uint32_t)bm_index(dc, i, j, k);
 }
 
-__global__ void k_bin_count(long long n, PBuf P, Geom g, Dims dc, uint32_t* __restrict__ rank,
+__global__ void k_bin_count(long long n, PBuf P, Geom g, Dims dc, float fscale, uint32_t* __restrict__ rank,
                             uint32_t* __restrict__ cnt) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  rank[p] = wave_aggregated_increment(cnt, cell_key(P.x[p], P.y[p], P.z[p], g, dc));
+  rank[p] = wave_aggregated_increment(cnt, cell_key(P.x[p], P.y[p], P.z[p], g, dc, fscale));
 }
 
 // the scatter reads the positions anyway, so it recomputes the key instead of the
 // count pass storing it (8 bytes less traffic per particle)
-__global__ void k_bin_scatter(long long n, PBuf S, PBuf D, bool has_vol, Geom g, Dims dc,
+__global__ void k_bin_scatter(long long n, PBuf S, PBuf D, bool has_vol, Geom g, Dims dc, float fscale,
                               const uint32_t* __restrict__ rank, const uint32_t* __restrict__ start) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const float x = S.x[p], y = S.y[p], z = S.z[p];
-  uint32_t d = start[cell_key(x, y, z, g, dc)] + rank[p];
+  uint32_t d = start[cell_key(x, y, z, g, dc, fscale)] + rank[p];
   D.x[d] = x; D.y[d] = y; D.z[d] = z;
   D.u[d] = S.u[p]; D.v[d] = S.v[p]; D.w[d] = S.w[p];
   D.type[d] = S.type[p];

@@ -35,17 +35,21 @@ int ensure_binned(pf_ctx* c) {
   c->mark_order_valid = false;
   const long long ncell = c->dc.nblocks * kBlockVol;
   HIP_TRY(c, fill_u32(c->stream, c->cell_count, 0u, ncell));
+  // the cell index in exact FP32 where the origin is 0 and dx a power of two
+  const Geom& g = c->geom;
+  const float fscale =
+      (g.ox == 0.0 && g.oy == 0.0 && g.oz == 0.0 && is_pow2(g.dx)) ? (float)(1.0 / g.dx) : 0.f;
   if (c->np > 0) {
     TIMED(c, PF_K_BIN_COUNT,
           hipLaunchKernelGGL(k_bin_count, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
-                             c->geom, c->dc, c->rank, c->cell_count));
+                             c->geom, c->dc, fscale, c->rank, c->cell_count));
     LAUNCH_CHECK(c);
   }
   HIP_TRY(c, scan_u32(*c, c->cell_count, c->cell_start, ncell));
   if (c->np > 0) {
     TIMED(c, PF_K_BIN_SCATTER,
           hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(c->np, 256)), dim3(256), 0, c->stream, c->np, c->pa,
-                             c->pb, c->has_vol, c->geom, c->dc, c->rank, c->cell_start));
+                             c->pb, c->has_vol, c->geom, c->dc, fscale, c->rank, c->cell_start));
     LAUNCH_CHECK(c);
     std::swap(c->pa, c->pb);
     // cells of more than kRankMax particles (piles against walls) sorted by caller id,
