@@ -243,7 +243,7 @@ __global__ void k_validate(long long n, PBuf P, Geom g, uint32_t* bad) {
 }
 
 // splitmix64 (Steele, Lea, Flood 2014)
-__device__ inline uint64_t splitmix64(uint64_t& s) {
+__host__ __device__ inline uint64_t splitmix64(uint64_t& s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;

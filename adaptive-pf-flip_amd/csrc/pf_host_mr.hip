@@ -52,6 +52,81 @@ void mr_refine(std::vector<Leaf>& leaves, int nx, int ny, int nz, int x0, int y0
   leaves.swap(next);
 }
 
+// The multires multi-bubble tank (a three-level generalisation of
+// initBubbleTankInterfaceBand, multires_sim3d_tp.cpp:457-500): water in the fine cells
+// j < ny/2 and nb bubbles drawn from the seed, radius U[0.02, 0.06] nx dx, centres
+// uniform over the positions that keep each bubble two cells inside the water.
+MrScene mr_draw_scene(const Geom& g, uint64_t seed, int nb) {
+  MrScene sc{};
+  sc.nb = nb;
+  sc.water = g.ny / 2;
+  uint64_t s = seed ^ 0x5DEECE66DULL;
+  auto uni = [&]() { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; };
+  const double L = g.nx * g.dx, m = 2.0 * g.dx;
+  const double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {g.nx * g.dx, sc.water * g.dx, g.nz * g.dx};
+  for (int b = 0; b < nb; ++b) {
+    const double r = (0.02 + 0.04 * uni()) * L;
+    double c[3];
+    for (int a = 0; a < 3; ++a) {
+      const double span = hi[a] - lo[a] - 2.0 * (r + m);
+      c[a] = span > 0.0 ? lo[a] + r + m + uni() * span : 0.5 * (lo[a] + hi[a]);
+    }
+    sc.b[b] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)r);
+  }
+  return sc;
+}
+
+// Distance in fine cells from the cell centres of the fine box [x0, x1) x [y0, y1) x
+// [z0, z1) to the nearest interface of the scene: the free surface y = water and the
+// bubble spheres (0 where the box straddles one).
+double mr_scene_distance(const Geom& g, const MrScene& sc, int x0, int y0, int z0, int x1, int y1, int z1) {
+  const double lo[3] = {x0 + 0.5, y0 + 0.5, z0 + 0.5}, hi[3] = {x1 - 0.5, y1 - 0.5, z1 - 0.5};
+  double d = std::max(0.0, std::max(lo[1] - sc.water, sc.water - hi[1]));
+  for (int b = 0; b < sc.nb; ++b) {
+    const double c[3] = {sc.b[b].x / g.dx, sc.b[b].y / g.dx, sc.b[b].z / g.dx}, r = sc.b[b].w / g.dx;
+    double near = 0.0, far = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double e = std::max(0.0, std::max(lo[a] - c[a], c[a] - hi[a]));
+      const double f = std::max(std::abs(lo[a] - c[a]), std::abs(hi[a] - c[a]));
+      near += e * e;
+      far += f * f;
+    }
+    d = std::min(d, std::max(0.0, std::max(std::sqrt(near) - r, r - std::sqrt(far))));
+  }
+  return d;
+}
+
+// Three levels by distance to the nearest interface: a level-2 leaf (16 fine cells)
+// within band + 16 cells splits into level-1 leaves, and those within band cells into
+// level-0 leaves.  A level-2 leaf never touches a level-0 one (2:1 balance): the
+// distance over cell centres is 1-Lipschitz, a level-0 leaf's level-1 parent holds a
+// centre within band, and every centre next to that parent lies within 8 sqrt(3) < 16
+// of it.
+std::vector<Leaf> mr_layout_scene(const Geom& g, const MrScene& sc, int band) {
+  std::vector<Leaf> out;
+  auto dist = [&](const Leaf& k) {
+    const int s = 4 << k[0];
+    return mr_scene_distance(g, sc, k[1] * s, k[2] * s, k[3] * s, std::min(g.nx, k[1] * s + s),
+                             std::min(g.ny, k[2] * s + s), std::min(g.nz, k[3] * s + s));
+  };
+  std::function<void(const Leaf&)> place = [&](const Leaf& k) {
+    const double lim = k[0] == 2 ? band + 16.0 : (double)band;
+    if (k[0] == 0 || dist(k) > lim) {
+      out.push_back(k);
+      return;
+    }
+    for (int c = 0; c < 8; ++c) {
+      const Leaf ch = {k[0] - 1, 2 * k[1] + (c & 1), 2 * k[2] + ((c >> 1) & 1), 2 * k[3] + (c >> 2)};
+      if (mr_block_in_domain(g.nx, g.ny, g.nz, ch)) place(ch);
+    }
+  };
+  for (int bz = 0; bz < (g.nz + 15) / 16; ++bz)
+    for (int by = 0; by < (g.ny + 15) / 16; ++by)
+      for (int bx = 0; bx < (g.nx + 15) / 16; ++bx) place({2, bx, by, bz});
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
 // layoutWithFineBox (multires_sim3d_tp.cpp:172-180) on a `level` base
 std::vector<Leaf> mr_layout_box(int nx, int ny, int nz, int level, bool valid, const int32_t* box) {
   std::vector<Leaf> lv;
@@ -314,6 +389,11 @@ int mr_reference_solve(pf_ctx* c, const pf_mr_solve_cfg& cf, unsigned G, const V
   double *Vc = nullptr, *crhs = nullptr, *cx = nullptr, *cr = nullptr, *cp = nullptr, *cap = nullptr;
   auto build_coarse = [&]() -> int {
     if (coarse_built) return PF_OK;
+    // level1ParentTuple (:39-43) throws above level 1: the coarse space has no
+    // aggregates for leaves of level 2 and up
+    for (size_t a = 0; a < c->mr_leaves.size(); a += 4)
+      if (c->mr_leaves[a] > 1)
+        return fail(c, PF_ERR_ARG, "the level-1 coarse space takes leaves of level 0 and 1 only");
     if (cg.nc * 6 > c->mrc_cap) {
       dfree(c->mrc);
       c->mrc = nullptr;

@@ -52,6 +52,73 @@ int64_t pf_mr_layout_refine(int32_t nx, int32_t ny, int32_t nz, const int32_t* i
   return (int64_t)lv.size();
 }
 
+int pf_mr_scene_bubbles(pf_ctx* c, uint64_t seed, int32_t nb, int32_t band, float* bubbles) {
+  if (!c || nb < 0 || nb > kMaxSceneBubbles || band < 1) return PF_ERR_ARG;
+  const Geom& g = c->geom;
+  if (g.ox != 0.0 || g.oy != 0.0 || g.oz != 0.0)
+    return fail(c, PF_ERR_ARG, "the multires path takes positions relative to the origin 0");
+  const MrScene sc = mr_draw_scene(g, seed, nb);
+  if (bubbles)
+    for (int b = 0; b < nb; ++b) {
+      bubbles[4 * b] = sc.b[b].x; bubbles[4 * b + 1] = sc.b[b].y;
+      bubbles[4 * b + 2] = sc.b[b].z; bubbles[4 * b + 3] = sc.b[b].w;
+    }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  int st = mr_apply_layout(c, mr_layout_scene(g, sc, band));
+  if (st) return st;
+  // particles: count per leaf, prefix on the host, then write
+  const int nl = (int)(c->mr_leaves.size() / 4);
+  const unsigned grid = (unsigned)((nl + 3) / 4);
+  uint32_t* cnt = nullptr;
+  long long* off = nullptr;
+  HIP_TRY(c, dalloc(&cnt, nl));
+  if (hipMalloc((void**)&off, sizeof(long long) * (size_t)nl) != hipSuccess) {
+    dfree(cnt);
+    return fail(c, PF_ERR_HIP, "pf_mr_scene_bubbles: allocation failed");
+  }
+  std::vector<uint32_t> hc((size_t)nl);
+  std::vector<long long> ho((size_t)nl);
+  long long n = 0;
+  auto run = [&]() -> int {
+    hipLaunchKernelGGL(k_mr_scene<true>, dim3(grid), dim3(256), 0, c->stream, c->mr_leaves_dev, nl, g, sc, seed, cnt,
+                       nullptr, c->pa);
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipMemcpyAsync(hc.data(), cnt, sizeof(uint32_t) * (size_t)nl, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < nl; ++a) {
+      ho[(size_t)a] = n;
+      n += hc[(size_t)a];
+    }
+    HIP_TRY(c, hipMemcpyAsync(off, ho.data(), sizeof(long long) * (size_t)nl, hipMemcpyHostToDevice, c->stream));
+    int s2 = grow_particles(c, n, false);
+    if (s2) return s2;
+    if (n > 0) {
+      hipLaunchKernelGGL(k_mr_scene<false>, dim3(grid), dim3(256), 0, c->stream, c->mr_leaves_dev, nl, g, sc, seed,
+                         cnt, off, c->pa);
+      LAUNCH_CHECK(c);
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return PF_OK;
+  };
+  st = run();
+  dfree(cnt);
+  dfree(off);
+  if (st) return st;
+  c->np = n;
+  c->has_vol = false;
+  c->vmax_valid = false;
+  c->binned = false;
+  std::vector<uint8_t> t((size_t)n);
+  if (n > 0) {
+    HIP_TRY(c, hipMemcpyAsync(t.data(), c->pa.type, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  long long liq = 0;
+  for (uint8_t q : t) liq += q == 0;
+  c->liquid_volume = (double)liq;
+  return PF_OK;
+}
+
 int pf_mr_face_valid(pf_ctx* c, int32_t axis, uint8_t* dense) {
   if (!c || !dense || axis < 0 || axis > 2) return PF_ERR_ARG;
   if (!c->mr) return fail(c, PF_ERR_STATE, "no multires layout (pf_mr_set_layout)");

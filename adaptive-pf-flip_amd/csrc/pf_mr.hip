@@ -322,6 +322,65 @@ __device__ inline T mr_apply_row(const MrGrid& m, const T* const* cf, int i, int
   return acc;
 }
 
+// The same row sum spread over the lanes of a 4^3 block whose cells all have level
+// L <= 2 (a block lies inside one leaf): the s^3 lanes inside a leaf cell (s = 2^L)
+// take its 6 s^2 patches in turn, q = r, r + s^3, ... (at most 6 per lane) for the
+// lane's offset r inside the cell, so a level-2 cell's 96 patches cost each lane two
+// instead of the anchor lane 96 in a row.  Patch q is axis q / 2s^2, then side and the
+// in-face position.  The loads go in three waves: every patch's coefficient and
+// neighbour level byte (independent), then the neighbour anchors' values, so a lane
+// waits on three loads in a row instead of three per patch.  The xor shuffles over
+// the offset bits (lane = x + 4 y + 16 z) leave every lane of the cell with the total.
+// Wave-uniform L; lanes with live == false add nothing.
+template <typename T, typename Val>
+__device__ inline T mr_apply_row_coop(const MrGrid& m, const T* const* cf, int i, int j, int k, int L, bool live,
+                                      T xa, Val&& val) {
+  const int s = 1 << L, s2 = s * s, s3 = s2 * s;
+  const int ax = i & ~(s - 1), ay = j & ~(s - 1), az = k & ~(s - 1);
+  const int r0 = (i - ax) + s * ((j - ay) + s * (k - az));
+  T c[6];
+  int n[6][3];
+  int lv[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const int q = r0 + t * s3;
+    c[t] = (T)0;
+    lv[t] = -1;
+    if (!live || q >= 6 * s2) continue;
+    const int axis = q / (2 * s2), rem = q - axis * 2 * s2;
+    const int side = rem & 1, w = rem >> 1, u = w % s, v = w / s;
+    int f[3] = {ax, ay, az};
+    if (axis == 0) { f[1] += u; f[2] += v; }
+    else if (axis == 1) { f[0] += u; f[2] += v; }
+    else { f[0] += u; f[1] += v; }
+    if (f[0] >= m.g.nx || f[1] >= m.g.ny || f[2] >= m.g.nz) continue;
+    n[t][0] = f[0]; n[t][1] = f[1]; n[t][2] = f[2];
+    const int lim = axis == 0 ? m.g.nx : (axis == 1 ? m.g.ny : m.g.nz);
+    if (side) {
+      n[t][axis] = f[axis] + s;
+      f[axis] = min(n[t][axis], lim);
+    } else {
+      n[t][axis] = f[axis] - 1;
+    }
+    c[t] = cf[axis][bm_index(mr_fdims(m, axis), f[0], f[1], f[2])];
+    lv[t] = mr_level(m, n[t][0], n[t][1], n[t][2]);
+  }
+  T acc = (T)0;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    if (c[t] != (T)0 && lv[t] >= 0) {
+      const int sn = 1 << lv[t];
+      acc += c[t] * (xa - val(bm_index(m.dc, n[t][0] & ~(sn - 1), n[t][1] & ~(sn - 1), n[t][2] & ~(sn - 1))));
+    }
+  }
+  for (int bit = 0; bit < L; ++bit) {
+    acc += __shfl_xor(acc, 1 << bit, 64);
+    acc += __shfl_xor(acc, 4 << bit, 64);
+    acc += __shfl_xor(acc, 16 << bit, 64);
+  }
+  return acc;
+}
+
 // multires twin of k_pcg_spmv: p_new = z + beta p_old (recomputed at the neighbour
 // anchors), Ap = S p_new, partial p.Ap
 __global__ void __launch_bounds__(256) k_mr_spmv(VecArgs a, MrOp op, const float* __restrict__ z, float* __restrict__ pA,
@@ -342,10 +401,18 @@ __global__ void __launch_bounds__(256) k_mr_spmv(VecArgs a, MrOp op, const float
       int bx, by, bz;
       block_coords(op.m.dc, b, bx, by, bz);
       const int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
-      int L;
-      if (i < op.m.g.nx && j < op.m.g.ny && k < op.m.g.nz && mr_is_anchor(op.m, i, j, k, L)) {
-        ap = op.cd[c] * pc + mr_apply_row<float>(op.m, op.cf, i, j, k, 1 << L, pc,
-                                                 [&](long long n) { return fmaf(beta, pold[n], z[n]); });
+      const bool in = i < op.m.g.nx && j < op.m.g.ny && k < op.m.g.nz;
+      auto pval = [&](long long n) { return fmaf(beta, pold[n], z[n]); };
+      const int Lb = mr_level(op.m, bx * 4, by * 4, bz * 4);
+      if (Lb >= 0 && Lb <= 2) {
+        const int s = 1 << Lb;
+        const long long anc = bm_index(op.m.dc, i & ~(s - 1), j & ~(s - 1), k & ~(s - 1));
+        const float sum = mr_apply_row_coop<float>(op.m, op.cf, i, j, k, Lb, in, in ? pval(anc) : 0.f, pval);
+        if (in && ((i | j | k) & (s - 1)) == 0) ap = op.cd[c] * pc + sum;
+      } else {
+        int L;
+        if (in && mr_is_anchor(op.m, i, j, k, L))
+          ap = op.cd[c] * pc + mr_apply_row<float>(op.m, op.cf, i, j, k, 1 << L, pc, pval);
       }
     }
     pnew[c] = pc;
@@ -366,15 +433,27 @@ __global__ void __launch_bounds__(256) k_mr_residual64(VecArgs a, MrOp64 op, con
   FOR_LIST_BLOCKS(a, b, lane) {
     const long long c = b * kBlockVol + lane;
     double res = 0.0;
-    if (op.vol[c] != 0.0) {
-      int bx, by, bz;
-      block_coords(op.m.dc, b, bx, by, bz);
-      const int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
+    int bx, by, bz;
+    block_coords(op.m.dc, b, bx, by, bz);
+    const int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4);
+    auto xval = [&](long long n) { return x[n]; };
+    const int Lb = mr_level(op.m, bx * 4, by * 4, bz * 4);
+    if (Lb >= 0 && Lb <= 2) {
+      // the cell's lanes share its patches; vol is non-zero on FLUID anchors only
+      const int s = 1 << Lb;
+      const bool in = i < op.m.g.nx && j < op.m.g.ny && k < op.m.g.nz;
+      const long long anc = bm_index(op.m.dc, i & ~(s - 1), j & ~(s - 1), k & ~(s - 1));
+      const bool live = in && op.vol[anc] != 0.0;
+      const double sum = mr_apply_row_coop<double>(op.m, op.cf, i, j, k, Lb, live, live ? x[anc] : 0.0, xval);
+      if (live && anc == c) {
+        res = b64[c] - (op.cd[c] * x[c] + sum);
+        rm = fmax(rm, fabs(res / op.vol[c]));
+      }
+    } else if (op.vol[c] != 0.0) {
       int L;
       mr_is_anchor(op.m, i, j, k, L);
       const double xc = x[c];
-      const double sx = op.cd[c] * xc + mr_apply_row<double>(op.m, op.cf, i, j, k, 1 << L, xc,
-                                                             [&](long long n) { return x[n]; });
+      const double sx = op.cd[c] * xc + mr_apply_row<double>(op.m, op.cf, i, j, k, 1 << L, xc, xval);
       res = b64[c] - sx;
       rm = fmax(rm, fabs(res / op.vol[c]));
     }
@@ -467,6 +546,65 @@ __global__ void __launch_bounds__(256) k_mr_bbox(long long n, PBuf P, Geom g, in
   }
   __syncthreads();
   if (threadIdx.x < 12) atomicMin(&out[threadIdx.x], s[threadIdx.x]);
+}
+
+// Bubbles of the multires multi-bubble tank, in domain units (centre, radius).
+constexpr int kMaxSceneBubbles = 64;
+struct MrScene {
+  int nb;
+  int water;  // fine cells j < water hold water
+  float4 b[kMaxSceneBubbles];
+};
+
+// Seeds the multires multi-bubble tank: 8 particles uniform in every leaf cell's part of
+// the interior water box [1, n-1) x [1, min(n-1, water)) x [1, n-1), one wave per leaf
+// (lane = cell of the leaf's 4^3).  COUNT stores 8 x the seeded cells per leaf; the
+// write pass places them from off[leaf] in lane order.  A particle inside a bubble is
+// gas (type 1).  Positions are (float)((c0 + U[0,1) (c1 - c0)) dx).
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_mr_scene(const int32_t* __restrict__ leaves, int nleaves, Geom g, MrScene sc,
+                                                  uint64_t seed, uint32_t* __restrict__ cnt,
+                                                  const long long* __restrict__ off, PBuf P) {
+#pragma clang fp contract(off)
+  const int lf = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (lf >= nleaves) return;  // whole waves leave together
+  const int L = leaves[4 * lf], h = 1 << L;
+  const int x0 = leaves[4 * lf + 1] * 4 * h + (lane & 3) * h;
+  const int y0 = leaves[4 * lf + 2] * 4 * h + ((lane >> 2) & 3) * h;
+  const int z0 = leaves[4 * lf + 3] * 4 * h + (lane >> 4) * h;
+  const int cx0 = max(x0, 1), cx1 = min(x0 + h, g.nx - 1);
+  const int cy0 = max(y0, 1), cy1 = min(min(y0 + h, g.ny - 1), sc.water);
+  const int cz0 = max(z0, 1), cz1 = min(z0 + h, g.nz - 1);
+  const bool seeded = cx0 < cx1 && cy0 < cy1 && cz0 < cz1;
+  const uint64_t m = __ballot(seeded);
+  if (COUNT) {
+    if (lane == 0) cnt[lf] = 8u * (uint32_t)__popcll(m);
+    return;
+  }
+  if (!seeded) return;
+  const long long base = off[lf] + 8ll * __popcll(m & ((1ull << lane) - 1ull));
+  const uint64_t cid = (uint64_t)x0 + (uint64_t)g.nx * ((uint64_t)y0 + (uint64_t)g.ny * (uint64_t)z0);
+  uint64_t s = seed * 0x9E3779B97F4A7C15ULL ^ (cid + 1) * 0xD1B54A32D192ED03ULL;
+  const double dx = g.dx;
+  for (int q = 0; q < 8; ++q) {
+    const double ux = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    const double uy = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    const double uz = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    const float fx = (float)__dmul_rn(__dadd_rn((double)cx0, __dmul_rn(ux, (double)(cx1 - cx0))), dx);
+    const float fy = (float)__dmul_rn(__dadd_rn((double)cy0, __dmul_rn(uy, (double)(cy1 - cy0))), dx);
+    const float fz = (float)__dmul_rn(__dadd_rn((double)cz0, __dmul_rn(uz, (double)(cz1 - cz0))), dx);
+    uint8_t t = 0;
+    for (int b = 0; b < sc.nb; ++b) {
+      const double ddx = (double)fx - sc.b[b].x, ddy = (double)fy - sc.b[b].y, ddz = (double)fz - sc.b[b].z;
+      if (ddx * ddx + ddy * ddy + ddz * ddz < (double)sc.b[b].w * sc.b[b].w) t = 1;
+    }
+    const long long p = base + q;
+    P.x[p] = fx; P.y[p] = fy; P.z[p] = fz;
+    P.u[p] = 0.f; P.v[p] = 0.f; P.w[p] = 0.f;
+    P.type[p] = t;
+    P.id[p] = (uint32_t)p;
+  }
 }
 
 }  // namespace pf

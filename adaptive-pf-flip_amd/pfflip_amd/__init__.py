@@ -341,6 +341,7 @@ def lib():
                                          C.POINTER(SolveStats)]),
             "pf_mr_update_refinement": (C.c_int, [vp, C.POINTER(MRRefineCfg), C.POINTER(i32), C.POINTER(i32)]),
             "pf_mr_layout_refine": (i64, [i32, i32, i32, vp, i64, vp, vp, i64]),
+            "pf_mr_scene_bubbles": (C.c_int, [vp, C.c_uint64, i32, i32, vp]),
             "pf_mr_retained_box": (C.c_int, [vp, vp, C.POINTER(i32)]),
             "pf_mr_step": (C.c_int, [vp, C.POINTER(MRStepCfg), C.POINTER(StepStats)]),
             "pf_mr_solve_defaults": (None, [C.POINTER(MRSolveCfg)]),
@@ -908,6 +909,13 @@ class MRContext(Context):
     def set_layout(self, leaves):
         lv = np.ascontiguousarray(np.asarray(leaves, np.int32).reshape(-1, 4))
         self._check(lib().pf_mr_set_layout(self.h, len(lv), _p(lv)), "pf_mr_set_layout")
+
+    def scene_bubbles(self, seed, nbubbles=16, band=6):
+        """the multires multi-bubble tank: three-level layout and particles; returns the
+        bubbles as (nbubbles, 4) centre and radius in domain units"""
+        out = np.zeros((max(nbubbles, 1), 4), np.float32)
+        self._check(lib().pf_mr_scene_bubbles(self.h, seed, nbubbles, band, _p(out)), "pf_mr_scene_bubbles")
+        return out[:nbubbles]
 
     def layout(self):
         n = C.c_int64()

@@ -29,13 +29,16 @@ count and is an estimate.  It is a reported baseline, not the target.
 
 Secondary lines: the 256^3 step with the reference's Jacobi preconditioner, the
 bubble tank at 128^3 and the dam-break at 256^3 (200 CFL-1 steps are the test; the
-bench times a few), each with per-stage ms and algorithmic GB/s.
+bench times a few), each with per-stage ms and algorithmic GB/s, and the multires
+multi-bubble tank at 512^3 effective (three levels) with per-stage ms.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adaptive-pf-flip_amd"))
@@ -293,6 +296,42 @@ def run_workload(G, kind, n, seed, dt, steps, warmup, precond, adaptive=False, c
         ctx.close()
 
 
+def run_multires(G, n, seed, dt, steps, warmup, label=""):
+    """ms/step and per-stage ms of the multires multi-bubble tank at n^3 effective: three
+    levels (level 0 within 6 cells of the free surface and of 16 bubbles, level 1 within
+    22, level 2 beyond), static layout, the fast multires solve to the reference's stop
+    rule |r|_inf < 1e-6 |r0|_inf"""
+    sim = G.MRSim3DTP(n, n, n, 1.0 / n)
+    try:
+        sim.ctx.scene_bubbles(seed, 16, 6)
+        levels = np.bincount(sim.ctx.layout()[:, 0], minlength=3)
+        ph = G.Phase(1000.0, 1.0, 1.0, 1.0)
+        ph.rho_tilde_0 = G.calibrate_rho_tilde0(ph, 1.0)
+        sim.phase, sim.dt, sim.dynamic_refinement, sim.timing = ph, dt, False, True
+        sim.cg_iters, sim.cg_tol, sim.cg_rel_tol = 50000, 0.0, 1e-6
+        for _ in range(warmup):
+            sim.step()
+        sim.ctx.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            stats.append(sim.step())
+        sim.ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        stage = {name: round(sum(s.stage_ms[i] for s in stats) / len(stats), 3) for i, name in enumerate(G.STAGES)}
+        print(f"[bench] {label}: {ms:.2f} ms/step", file=sys.stderr, flush=True)
+        return {"workload": label, "ms_per_step": round(ms, 3), "steps": steps, "warmup": warmup,
+                "leaves_per_level": [int(v) for v in levels], "particles": sim.ctx.count(),
+                "fluid_cells": stats[-1].solve.fluid_cells, "stage_ms": stage,
+                "solve_iterations": [s.solve.iterations for s in stats],
+                "converged": [s.solve.converged for s in stats],
+                "true_max_residual_rel": max(s.solve.true_max_residual / max(s.solve.initial_max_residual, 1e-300)
+                                             for s in stats),
+                "true_rel_residual_max": max(s.solve.true_rel_residual for s in stats)}
+    finally:
+        sim.ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,6 +468,9 @@ def main():
         workloads["dambreak256"] = run_workload(
             G, G.SCENE_DAMBREAK, 256, args.seed, None, 20, 5, args.precond, adaptive=True, cap=5e-3,
             label="dam-break 256^3 (liquid only), CFL 1, requested dt cap 5e-3")
+        workloads["multires512"] = run_multires(
+            G, 512, args.seed, 5e-4, 2, 1,
+            label="multires multi-bubble tank 512^3 effective, 3 levels, 16 bubbles, 1000:1, dt 5e-4")
     cpu = pair = None
     if not args.no_cpu_baseline:
         print(f"[bench] gpu side of the {args.cpu_sample_n}^3 pair", file=sys.stderr, flush=True)

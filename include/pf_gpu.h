@@ -487,6 +487,15 @@ int64_t pf_mr_layout_fine_box(int32_t nx, int32_t ny, int32_t nz, int32_t level,
  * returns the new count, writes at most capacity leaves (host only) */
 int64_t pf_mr_layout_refine(int32_t nx, int32_t ny, int32_t nz, const int32_t* in, int64_t n, const int32_t* box,
                             int32_t* leaves, int64_t capacity);
+/* The multires multi-bubble tank, a three-level generalisation of
+ * MRSim3DTP::initBubbleTankInterfaceBand (multires_sim3d_tp.cpp:457-500): water in the
+ * fine cells j < ny/2, nbubbles (at most 64) gas bubbles drawn from the seed with radius
+ * U[0.02, 0.06] nx dx, and a static layout of three levels by distance to the nearest
+ * interface (level 0 within band fine cells, level 1 within band + 16, level 2 beyond;
+ * band >= 1).  Sets the layout and seeds 8 particles per leaf cell over the interior
+ * water, gas inside the bubbles.  bubbles (may be NULL) receives 4 floats per bubble:
+ * centre and radius in domain units.  The origin must be 0. */
+int pf_mr_scene_bubbles(pf_ctx* ctx, uint64_t seed, int32_t nbubbles, int32_t band, float* bubbles);
 /* the enumerated faces of an axis (MRMacGrid3D::uFaces etc., multires_mac_grid3d.h:43-45)
  * as one byte per fine face, dense in the face field's extents */
 int pf_mr_face_valid(pf_ctx* ctx, int32_t axis, uint8_t* dense);

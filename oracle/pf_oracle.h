@@ -282,6 +282,10 @@ void orc_mr_apply_walls(orc_mr* g);
 int orc_mr_solve_cfg_valid(const orc_mr_solve_cfg* cfg);
 void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solve_cfg* cfg,
                     orc_mr_solve_stats* st);
+int64_t orc_mr_system(orc_mr* g, const orc_phase* pp, double dt, double cdiv, int64_t cap_rows, int64_t cap_nnz,
+                      int64_t* cells, int64_t* row_ptr, int64_t* col, double* val, double* rhs, int64_t* nnz,
+                      int64_t* pin);
+void orc_mr_apply_pressure(orc_mr* g, const orc_phase* pp, double dt, const double* x);
 void orc_mr_g2p(const orc_mr* g, const orc_mr* saved, int64_t n, const double* x, const double* y,
                 const double* z, double* u, double* v, double* w, const uint8_t* type, double aL, double aG);
 void orc_mr_advect(const orc_mr* g, int64_t n, double* x, double* y, double* z, const uint8_t* type,

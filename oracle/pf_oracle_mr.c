@@ -820,17 +820,55 @@ int orc_mr_solve_cfg_valid(const orc_mr_solve_cfg* c) {
            !isfinite(c->divergence_correction));
 }
 
-void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solve_cfg* cf, orc_mr_solve_stats* st) {
-  orc_mr_solve_stats local;
-  if (!st) st = &local;
-  memset(st, 0, sizeof(*st));
-  st->pinned_cell = -1;
-  st->cp_auto_disable_iteration = -1;
-  if (!orc_mr_solve_cfg_valid(cf)) {
-    st->breakdown = 1;
-    return;
+/* the face update over the enumerated faces (multires_pressure3d.cpp:1455-1524) from the
+ * row pressures x (rows numbered by idx) */
+static void mr_face_update(orc_mr* g, const orc_phase* pp, double dt, const int64_t* idx, const double* x) {
+  for (int axis = 0; axis < 3; ++axis) {
+    int fx = g->nx + (axis == 0), fy = g->ny + (axis == 1), fz = g->nz + (axis == 2);
+    for (int k = 0; k < fz; ++k)
+      for (int j = 0; j < fy; ++j)
+        for (int i = 0; i < fx; ++i) {
+          int64_t fi = fidx(g, axis, i, j, k);
+          if (!g->valid[axis][fi]) continue;
+          int li = i - (axis == 0), lj = j - (axis == 1), lk = k - (axis == 2);
+          int lm = marker_fine(g, li, lj, lk), rm = marker_fine(g, i, j, k);
+          int lf = lm == 1, rf = rm == 1;
+          if (!lf && !rf) continue;
+          if (lm == 2 || rm == 2) { g->f[axis][fi] = 0.0; continue; }
+          double dist = g->dx;
+          int64_t ref = lf ? leaf_cell(g, li, lj, lk) : leaf_cell(g, i, j, k);
+          if (ref >= 0 && idx[ref] >= 0)
+            dist = lf ? nbr_distance(g, cell_h(g, ref), i, j, k) : nbr_distance(g, cell_h(g, ref), li, lj, lk);
+          int64_t lc = leaf_cell(g, li, lj, lk), rc = leaf_cell(g, i, j, k);
+          double pl = (lf && lc >= 0 && idx[lc] >= 0) ? x[idx[lc]] : 0.0;
+          double pr = (rf && rc >= 0 && idx[rc] >= 0) ? x[idx[rc]] : 0.0;
+          double beta = beta_raw(g->f[3 + axis][fi], pp);
+          double cur = g->faithful ? (double)(float)g->f[axis][fi] : g->f[axis][fi];
+          g->f[axis][fi] = stf(g, cur - dt * beta * (pr - pl) / dist);
+        }
   }
-  /* fluidProjectionCells: leafCells order (leaves sorted, then lz, ly, lx) */
+}
+
+/* The projection system (multires_pressure3d.cpp:166-175, 783-812): the FLUID leaf
+ * cells in leafCells order (leaves sorted, then lz, ly, lx), the pin (the first of them
+ * when no FLUID cell touches AIR), and the rows diag_t x_t - sum coeff x_nbr with the
+ * divergence of the current face velocities. */
+typedef struct {
+  int64_t N, pin;
+  int64_t* idx;   /* leaf cell -> row, or -1 */
+  int64_t* cells; /* row -> leaf cell */
+  double *diag, *divg;
+  int* noff;
+  mr_term* off;   /* 6 * 64 terms per row */
+} mr_rows;
+
+static void mr_rows_free(mr_rows* R) {
+  free(R->idx); free(R->cells); free(R->diag); free(R->divg); free(R->noff); free(R->off);
+  memset(R, 0, sizeof(*R));
+}
+
+static void mr_rows_build(orc_mr* g, const orc_phase* pp, double dt, mr_rows* R) {
+  memset(R, 0, sizeof(*R));
   int64_t nl = (int64_t)g->nleaves * 64;
   int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nl > 0 ? nl : 1));
   int64_t* cells = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nl > 0 ? nl : 1));
@@ -844,13 +882,11 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
     idx[id] = N;
     cells[N++] = id;
   }
-  st->active_cells = N;
-  memset(g->p, 0, sizeof(double) * (size_t)(nl > 0 ? nl : 1));
-  if (N == 0) {
-    st->converged = 1;
-    free(idx); free(cells);
-    return;
-  }
+  R->N = N;
+  R->idx = idx;
+  R->cells = cells;
+  R->pin = -1;
+  if (N == 0) return;
   mr_patch* pt = (mr_patch*)malloc(sizeof(mr_patch) * 6 * 64);
   /* findPinCell :166-175 */
   int has_air = 0;
@@ -860,7 +896,7 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
       if (marker_fine(g, pt[q].nx, pt[q].ny, pt[q].nz) == 0) { has_air = 1; break; }
   }
   int64_t pin = has_air ? -1 : 0;
-  st->pinned_cell = pin;
+  R->pin = pin;
   /* rows :783-812 */
   double* diag = (double*)calloc((size_t)N, sizeof(double));
   double* divg = (double*)calloc((size_t)N, sizeof(double));
@@ -889,6 +925,38 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
       double vel = g->f[pt[q].axis][pt[q].fi];
       divg[t] += pt[q].sign * vel * area / V;
     }
+  }
+  free(pt);
+  R->diag = diag;
+  R->divg = divg;
+  R->noff = noff;
+  R->off = off;
+}
+
+void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solve_cfg* cf, orc_mr_solve_stats* st) {
+  orc_mr_solve_stats local;
+  if (!st) st = &local;
+  memset(st, 0, sizeof(*st));
+  st->pinned_cell = -1;
+  st->cp_auto_disable_iteration = -1;
+  if (!orc_mr_solve_cfg_valid(cf)) {
+    st->breakdown = 1;
+    return;
+  }
+  mr_rows R;
+  mr_rows_build(g, pp, dt, &R);
+  const int64_t N = R.N, pin = R.pin;
+  int64_t *idx = R.idx, *cells = R.cells;
+  double *diag = R.diag, *divg = R.divg;
+  int* noff = R.noff;
+  mr_term* off = R.off;
+  st->active_cells = N;
+  st->pinned_cell = pin;
+  memset(g->p, 0, sizeof(double) * (size_t)(g->nleaves > 0 ? (int64_t)g->nleaves * 64 : 1));
+  if (N == 0) {
+    st->converged = 1;
+    mr_rows_free(&R);
+    return;
   }
   double* x = (double*)calloc((size_t)N, sizeof(double));
   double* rhs = (double*)calloc((size_t)N, sizeof(double));
@@ -1113,33 +1181,53 @@ void orc_mr_project(orc_mr* g, const orc_phase* pp, double dt, const orc_mr_solv
 #undef APPLY
 #undef PRECOND
   for (int64_t t = 0; t < N; ++t) g->p[cells[t]] = stf(g, x[t]);
-  /* face update over the enumerated faces, :1455-1524 */
-  for (int axis = 0; axis < 3; ++axis) {
-    int fx = g->nx + (axis == 0), fy = g->ny + (axis == 1), fz = g->nz + (axis == 2);
-    for (int k = 0; k < fz; ++k)
-      for (int j = 0; j < fy; ++j)
-        for (int i = 0; i < fx; ++i) {
-          int64_t fi = fidx(g, axis, i, j, k);
-          if (!g->valid[axis][fi]) continue;
-          int li = i - (axis == 0), lj = j - (axis == 1), lk = k - (axis == 2);
-          int lm = marker_fine(g, li, lj, lk), rm = marker_fine(g, i, j, k);
-          int lf = lm == 1, rf = rm == 1;
-          if (!lf && !rf) continue;
-          if (lm == 2 || rm == 2) { g->f[axis][fi] = 0.0; continue; }
-          double dist = g->dx;
-          int64_t ref = lf ? leaf_cell(g, li, lj, lk) : leaf_cell(g, i, j, k);
-          if (ref >= 0 && idx[ref] >= 0)
-            dist = lf ? nbr_distance(g, cell_h(g, ref), i, j, k) : nbr_distance(g, cell_h(g, ref), li, lj, lk);
-          int64_t lc = leaf_cell(g, li, lj, lk), rc = leaf_cell(g, i, j, k);
-          double pl = (lf && lc >= 0 && idx[lc] >= 0) ? x[idx[lc]] : 0.0;
-          double pr = (rf && rc >= 0 && idx[rc] >= 0) ? x[idx[rc]] : 0.0;
-          double beta = beta_raw(g->f[3 + axis][fi], pp);
-          double cur = g->faithful ? (double)(float)g->f[axis][fi] : g->f[axis][fi];
-          g->f[axis][fi] = stf(g, cur - dt * beta * (pr - pl) / dist);
-        }
-  }
-  free(pt); free(idx); free(cells); free(diag); free(divg); free(noff); free(off);
+  mr_face_update(g, pp, dt, idx, x);
+  mr_rows_free(&R);
   free(x); free(rhs); free(r); free(z); free(zp); free(p); free(Ap); free(cx); free(cr);
+}
+
+/* The projection system in CSR form, for a direct solve: rows in leafCells order, each
+ * row's diagonal first, then its off-diagonal terms as assembled (the pin's column left
+ * out), and rhs_t = -div_t + cdiv (0 at the pin).  Returns the row count and stores the
+ * nonzero count and the pin row; the arrays are filled when the caps hold them. */
+int64_t orc_mr_system(orc_mr* g, const orc_phase* pp, double dt, double cdiv, int64_t cap_rows, int64_t cap_nnz,
+                      int64_t* cells, int64_t* row_ptr, int64_t* col, double* val, double* rhs, int64_t* nnz,
+                      int64_t* pin) {
+  mr_rows R;
+  mr_rows_build(g, pp, dt, &R);
+  int64_t m = 0;
+  for (int64_t t = 0; t < R.N; ++t) m += 1 + R.noff[t];
+  *nnz = m;
+  *pin = R.pin;
+  if (cap_rows >= R.N && cap_nnz >= m && row_ptr) {
+    int64_t e = 0;
+    for (int64_t t = 0; t < R.N; ++t) {
+      cells[t] = R.cells[t];
+      row_ptr[t] = e;
+      col[e] = t;
+      val[e++] = R.diag[t];
+      for (int q = 0; q < R.noff[t]; ++q) {
+        col[e] = R.off[6 * 64 * t + q].idx;
+        val[e++] = -R.off[6 * 64 * t + q].coeff;
+      }
+      rhs[t] = t == R.pin ? 0.0 : -R.divg[t] + cdiv;
+    }
+    row_ptr[R.N] = e;
+  }
+  const int64_t N = R.N;
+  mr_rows_free(&R);
+  return N;
+}
+
+/* The end of a projection from given row pressures x (rows as orc_mr_system numbers
+ * them): the leaf pressures and the face update. */
+void orc_mr_apply_pressure(orc_mr* g, const orc_phase* pp, double dt, const double* x) {
+  mr_rows R;
+  mr_rows_build(g, pp, dt, &R);
+  memset(g->p, 0, sizeof(double) * (size_t)(g->nleaves > 0 ? (int64_t)g->nleaves * 64 : 1));
+  for (int64_t t = 0; t < R.N; ++t) g->p[R.cells[t]] = stf(g, x[t]);
+  if (R.N > 0) mr_face_update(g, pp, dt, R.idx, x);
+  mr_rows_free(&R);
 }
 
 /* ---------------------------------------------------------------- step */

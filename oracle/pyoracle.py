@@ -398,6 +398,11 @@ def lib():
         L.orc_mr_apply_walls.argtypes = [C.c_void_p]
         L.orc_mr_project.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, C.POINTER(MRSolveCfg),
                                      C.POINTER(MRSolveStats)]
+        L.orc_mr_system.restype = C.c_int64
+        L.orc_mr_system.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, C.c_double, C.c_int64, C.c_int64,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.orc_mr_apply_pressure.argtypes = [C.c_void_p, C.POINTER(Phase), C.c_double, _dp]
         L.orc_mr_g2p.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + [_dp] * 6 + [_u8, C.c_double, C.c_double]
         L.orc_mr_advect.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _u8, C.c_double, C.c_int,
                                     C.POINTER(EscapeStats)]
@@ -807,6 +812,25 @@ class MRGrid:
         st = MRSolveStats()
         lib().orc_mr_project(self.h, C.byref(phase), dt, C.byref(cfg), C.byref(st))
         return st
+
+    def system(self, phase, dt, cdiv=0.0):
+        """the projection system as (cells, CSR (indptr, indices, data), rhs, pin row):
+        rows in leafCells order, A's entries as the reference assembles them"""
+        nnz, pin = C.c_int64(), C.c_int64()
+        n = lib().orc_mr_system(self.h, C.byref(phase), dt, cdiv, 0, 0, None, None, None, None, None,
+                                C.byref(nnz), C.byref(pin))
+        cells = np.zeros(max(n, 1), np.int64)
+        ptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(max(nnz.value, 1), np.int64)
+        val = np.zeros(max(nnz.value, 1))
+        rhs = np.zeros(max(n, 1))
+        lib().orc_mr_system(self.h, C.byref(phase), dt, cdiv, n, nnz.value, _vptr(cells), _vptr(ptr), _vptr(col),
+                            _vptr(val), _vptr(rhs), C.byref(nnz), C.byref(pin))
+        return cells[:n], (ptr, col[:nnz.value], val[:nnz.value]), rhs[:n], pin.value
+
+    def apply_pressure(self, phase, dt, x):
+        """finish a projection from the row pressures x: leaf pressures and face update"""
+        lib().orc_mr_apply_pressure(self.h, C.byref(phase), dt, np.ascontiguousarray(x, np.float64))
 
     def g2p(self, saved, ps, aL, aG):
         lib().orc_mr_g2p(self.h, saved.h, len(ps), ps.x, ps.y, ps.z, ps.u, ps.v, ps.w, ps.type, aL, aG)

@@ -69,3 +69,38 @@ def bubble_lattice(n, seed=3, vel=True):
     u, v, w = ((rng.normal(scale=0.3, size=len(p)) if vel else np.zeros(len(p))).astype(np.float32) for _ in range(3))
     t = p[:, 3].astype(np.uint8)
     return x, y, z, u, v, w, t
+
+
+def refine_box(nx, ny, nz, leaves, box):
+    """refineFineCellBox (multires_layout3d.h:62-97) on an (n, 4) leaf array of (level,
+    bx, by, bz): leaves above level 0 that overlap the box split into their children
+    inside the domain; sorted and unique"""
+    x0, y0, z0, x1, y1, z1 = box
+    out = set()
+    for L, bx, by, bz in np.asarray(leaves).tolist():
+        s = 4 << L
+        ov = (bx * s < x1 and min(nx, bx * s + s) > x0 and by * s < y1 and min(ny, by * s + s) > y0
+              and bz * s < z1 and min(nz, bz * s + s) > z0)
+        if L == 0 or not ov:
+            out.add((L, bx, by, bz))
+            continue
+        h = 4 << (L - 1)
+        for c in range(8):
+            ch = (L - 1, 2 * bx + (c & 1), 2 * by + ((c >> 1) & 1), 2 * bz + (c >> 2))
+            if ch[1] * h < nx and ch[2] * h < ny and ch[3] * h < nz:
+                out.add(ch)
+    return np.array(sorted(out), np.int32).reshape(-1, 4)
+
+
+def three_level_bubble_layout(n):
+    """levels 2, 1 and 0 around the bubble tank of bubble_lattice(n) (n a multiple of 16,
+    at least 64): level 2 in the bulk, level 1 in a slab around the free surface, level
+    0 in a box around the bubble (refined twice from level 2) and in a band around the
+    free surface"""
+    water = n // 2
+    lv = O.mr_layout_fine_box(n, n, n, [0, water - water // 4 + 4, 0, n, water + water // 4 + 4, n], level=2)
+    cy, r = water * 0.375, max(1.25, n * 0.1875)
+    bub = [int(n / 2 - r - 2), 0, int(n / 2 - r - 2), int(n / 2 + r + 2), int(cy + r + 2), int(n / 2 + r + 2)]
+    for box in (bub, bub, [0, water - 4, 0, n, water + 4, n]):
+        lv = refine_box(n, n, n, lv, box)
+    return lv
