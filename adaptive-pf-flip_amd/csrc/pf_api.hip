@@ -31,6 +31,7 @@
 #include "pf_adapt.hip"
 #include "pf_render.hip"
 #include "pf_mg.hip"
+#include "pf_mr_mg.hip"
 #include "pf_lpc_cw.hip"
 
 using namespace pf;

@@ -161,7 +161,7 @@ static pf_solve_cfg mr_to_solve(const pf_mr_solve_cfg* m) {
   s.inner_rel_tol = m->inner_rel_tol > 0 ? m->inner_rel_tol : 0.1;
   s.max_outer = m->max_outer > 0 ? m->max_outer : 10000;
   s.check_every = m->check_every > 0 ? m->check_every : 8;
-  s.precond = PF_PRECOND_JACOBI;
+  s.precond = m->mode == PF_MR_SOLVE_MULTIGRID ? PF_PRECOND_MULTIGRID : PF_PRECOND_JACOBI;
   return s;
 }
 
@@ -170,7 +170,7 @@ static int mr_project_with(pf_ctx* c, const pf_phase* ph, double dt, const pf_mr
                            pf_solve_stats* stats, bool timing, int e0, int e1, int e2) {
   if (cfg->max_iterations < 0 || cfg->absolute_tolerance < 0 || cfg->relative_tolerance < 0)
     return fail(c, PF_ERR_ARG, "invalid multires solve configuration");
-  if (cfg->mode != PF_MR_SOLVE_FAST && cfg->mode != PF_MR_SOLVE_REFERENCE)
+  if (cfg->mode != PF_MR_SOLVE_FAST && cfg->mode != PF_MR_SOLVE_REFERENCE && cfg->mode != PF_MR_SOLVE_MULTIGRID)
     return fail(c, PF_ERR_ARG, "unknown multires solve mode");
   if (cfg->mode == PF_MR_SOLVE_REFERENCE && (!mr_cfg_valid(*cfg) || !std::isfinite(c_div)))
     return fail(c, PF_ERR_ARG, "projectMR3D invalid solve config");

@@ -96,6 +96,8 @@ int mg_alloc(pf_ctx* c) {
   return PF_OK;
 }
 
+int mg_coarsen_from(pf_ctx* c, MgLevel F, size_t from);
+
 int mg_build(pf_ctx* c) {
   if (!c->mg_t0) {
     int st = mg_alloc(c);
@@ -120,25 +122,62 @@ int mg_build(pf_ctx* c) {
                        c->mg_unscale);
     LAUNCH_CHECK(c);
   }
-  MgLevel F = mg_level0(c);
-  for (auto& b : c->mg) {
-    MgLevel Cl = mg_level(b);
+  return mg_coarsen_from(c, mg_level0(c), 0);
+}
+
+// a level's inverse diagonal and the FP16 copy its row kernels read (as level 0's)
+int mg_level_finish(pf_ctx* c, Ctx::MgBuf& b) {
+  const long long nl = b.d.nblocks * kBlockVol;
+  const unsigned g = std::min<unsigned>(grid_for(nl, 256), 4096u);
+  hipLaunchKernelGGL(k_mg_inverse, dim3(g), dim3(256), 0, c->stream, mg_level(b), b.inv);
+  HIP_TRY(c, fill_u32(c->stream, b.mx, 0u, 2));
+  hipLaunchKernelGGL(k_mg_absmax, dim3(std::min<unsigned>(grid_for(nl, 256), 2048u)), dim3(256), 0, c->stream, nl,
+                     b.cd, b.cx, b.cy, b.cz, b.inv, b.mx);
+  _Float16* h[5];
+  for (int q = 0; q < 5; ++q) h[q] = reinterpret_cast<_Float16*>(b.h[q]);
+  hipLaunchKernelGGL(k_mg_tohalf, dim3(std::min<unsigned>(grid_for(nl, 256), 4096u)), dim3(256), 0, c->stream, nl,
+                     b.cd, b.cx, b.cy, b.cz, b.inv, h[0], h[1], h[2], h[3], h[4], b.mx, b.unscale);
+  LAUNCH_CHECK(c);
+  return PF_OK;
+}
+
+// levels c->mg[from..] by aggregation from F (the level above c->mg[from])
+int mg_coarsen_from(pf_ctx* c, MgLevel F, size_t from) {
+  for (size_t l = from; l < c->mg.size(); ++l) {
+    Ctx::MgBuf& b = c->mg[l];
+    const MgLevel Cl = mg_level(b);
     const unsigned g = std::min<unsigned>(grid_for(b.d.nblocks * kBlockVol, 256), 4096u);
     hipLaunchKernelGGL(k_mg_coarsen, dim3(g), dim3(256), 0, c->stream, F, Cl, b.cd, b.cx, b.cy, b.cz);
-    hipLaunchKernelGGL(k_mg_inverse, dim3(g), dim3(256), 0, c->stream, Cl, b.inv);
-    // the FP16 copy the row kernels read (as level 0's)
-    const long long nl = b.d.nblocks * kBlockVol;
-    HIP_TRY(c, fill_u32(c->stream, b.mx, 0u, 2));
-    hipLaunchKernelGGL(k_mg_absmax, dim3(std::min<unsigned>(grid_for(nl, 256), 2048u)), dim3(256), 0, c->stream, nl,
-                       b.cd, b.cx, b.cy, b.cz, b.inv, b.mx);
-    _Float16* h[5];
-    for (int q = 0; q < 5; ++q) h[q] = reinterpret_cast<_Float16*>(b.h[q]);
-    hipLaunchKernelGGL(k_mg_tohalf, dim3(std::min<unsigned>(grid_for(nl, 256), 4096u)), dim3(256), 0, c->stream, nl,
-                       b.cd, b.cx, b.cy, b.cz, b.inv, h[0], h[1], h[2], h[3], h[4], b.mx, b.unscale);
-    LAUNCH_CHECK(c);
+    int st = mg_level_finish(c, b);
+    if (st) return st;
     F = Cl;
   }
   return PF_OK;
+}
+
+// The multires cycle's levels: the block grid (the uniform hierarchy's second level,
+// c->mg[1]) from R S P, then the uniform levels below it.  Leaves of level 0..2 only.
+int mr_mg_build(pf_ctx* c, const MrOp& op) {
+  for (size_t a = 0; a < c->mr_leaves.size(); a += 4)
+    if (c->mr_leaves[a] > 2)
+      return fail(c, PF_ERR_ARG, "the multires multigrid takes leaves of level 0 to 2");
+  if (!c->mg_t0) {
+    int st = mg_alloc(c);
+    if (st) return st;
+  }
+  if (c->mg.size() < 2) return fail(c, PF_ERR_ARG, "the multires multigrid needs a grid above 8 cells per axis");
+  Ctx::MgBuf& b = c->mg[1];
+  const long long nl = b.d.nblocks * kBlockVol;
+  // block-grid cells of no listed block keep a zero residual through the solve
+  HIP_TRY(c, fill_f32(c->stream, b.r, 0.f, nl));
+  float** arrs[4] = {&b.cd, &b.cx, &b.cy, &b.cz};
+  for (float** q : arrs) HIP_TRY(c, fill_f32(c->stream, *q, 0.f, nl));
+  hipLaunchKernelGGL(k_mr_mg_coarse, dim3(grid_for(c->dc.nblocks, 4)), dim3(256), 0, c->stream, op, b.d, b.cd, b.cx,
+                     b.cy, b.cz);
+  LAUNCH_CHECK(c);
+  int st = mg_level_finish(c, b);
+  if (st) return st;
+  return mg_coarsen_from(c, mg_level(b), 2);
 }
 
 // workgroups of the level-0 kernels: a multiple of 8 (the XCD-aware list walk), at
@@ -387,6 +426,46 @@ int mg_apply(pf_ctx* c, const float* r, float* z, const int* done, double* part)
   return mg_apply_on(c, c->pblk, c->npblk, r, z, done, part, nullptr);
 }
 
+// z = the multires cycle on r over the listed blocks: nu sweeps on S, the block sums of
+// the residual onto the block grid, the uniform cycle from there, the prolongation and
+// nu sweeps back, the last into z with the partial r.z (mg_grid0n workgroups) in slot 2
+// of part; skipped while *done
+int mr_mg_apply(pf_ctx* c, const MrOp& op, const VecArgs& va, const float* r, float* z, const int* done,
+                double* part) {
+  const int nu = std::max(1, c->mg_cfg.sweeps);
+  std::vector<float> wpre = mg_sweep_weights(c);
+  wpre.resize((size_t)nu, c->mg_cfg.omega);
+  const unsigned g0 = mg_grid0n(va.nlist), gw = list_grid(va.nlist);
+  double* const rz = part + 2 * kMaxPartials;
+  const float* inv = c->invd32;
+  // 2 nu writes (the prolongation adds in place): the first lands in the work array
+  float* cur = nullptr;
+  float* nxt = c->mg_t0;
+  auto flip = [&]() {
+    cur = nxt;
+    nxt = (cur == z) ? c->mg_t0 : z;
+  };
+  for (int q = 0; q < nu; ++q) {
+    hipLaunchKernelGGL(k_mr_mg_sweep, dim3(gw), dim3(256), 0, c->stream, va, op, done, inv, r, cur, nxt,
+                       wpre[(size_t)q], nullptr);
+    flip();
+  }
+  Ctx::MgBuf& b = c->mg[1];
+  hipLaunchKernelGGL(k_mr_mg_restrict, dim3(gw), dim3(256), 0, c->stream, va, op, done, r, cur, b.d, b.r);
+  float* const xc = mg_coarse(c, 2, done, wpre);
+  hipLaunchKernelGGL(k_mr_mg_prolong, dim3(gw), dim3(256), 0, c->stream, va, op, done, b.d, xc, cur,
+                     c->mg_cfg.correction);
+  for (int q = 0; q < nu; ++q) {
+    const bool last = q == nu - 1;
+    hipLaunchKernelGGL(k_mr_mg_sweep, dim3(last ? g0 : gw), dim3(256), 0, c->stream, va, op, done, inv, r, cur, nxt,
+                       wpre[(size_t)(nu - 1 - q)], last ? rz : nullptr);
+    flip();
+  }
+  LAUNCH_CHECK(c);
+  if (cur != z) return fail(c, PF_ERR_STATE, "multires multigrid: cycle did not end in z");
+  return PF_OK;
+}
+
 // single: the single-phase unit-density projection (ph unused, cdiv ignored);
 // c->mr: the multires finite-volume projection, stopping when
 // |rhs - A x|_inf < max(cfg->tol, mr_rel_tol * |rhs|_inf)
@@ -396,8 +475,6 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const bool mr = c->mr;
   if (cfg->precond < PF_PRECOND_JACOBI || cfg->precond > PF_PRECOND_MULTIGRID)
     return fail(c, PF_ERR_ARG, "unknown preconditioner");
-  if (cfg->precond == PF_PRECOND_MULTIGRID && mr)
-    return fail(c, PF_ERR_ARG, "the multigrid preconditioner runs on the sparse path only");
   pf_solve_stats S;
   std::memset(&S, 0, sizeof(S));
   S.pinned_cell = -1;
@@ -638,7 +715,7 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   bool learned = want_learned || want_mg;
   c->mg_cs_active = c->mg_cs;  // caller-set weights (pf_mg_set_correction_weights) or none
   if (learned) {
-    st = mg_build(c);
+    st = mr ? mr_mg_build(c, mop) : mg_build(c);
     if (st) return st;
   }
   if (want_learned && !c->mg.empty()) {
@@ -652,7 +729,8 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   const unsigned GT = learned ? mg_grid0(c) : 1u;
   auto papply = [&](const int* done) -> int {
     long long ki = kbegin(c, PF_K_MG_APPLY);
-    int s2 = mg_apply(c, c->r32, c->z32, done, c->partial);
+    int s2 = mr ? mr_mg_apply(c, mop, va, c->r32, c->z32, done, c->partial)
+                : mg_apply(c, c->r32, c->z32, done, c->partial);
     kend(c, ki);
     return s2;
   };
@@ -698,7 +776,8 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     if (st) return st;
     // the two iteration kernels use at most 1024 workgroups, so the per-iteration
     // scalar kernels sum few partials
-    const unsigned GS = std::min(G, 1024u);
+    // (the multires SpMV waits on dependent loads per block: it takes more workgroups)
+    const unsigned GS = std::min(G, mr ? 4096u : 1024u);
     // the buffer-addressed SpMV takes 32-bit byte offsets: arrays below 2 GiB
     const bool spmv_buffers = c->dc.nblocks * kBlockVol * 4 < (1LL << 31);
     VecArgs vs = va;

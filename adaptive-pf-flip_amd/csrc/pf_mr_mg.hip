@@ -1,0 +1,128 @@
+// pf_mr_mg.hip -- the aggregation multigrid on the multires operator S = V A (no
+// reference counterpart; the reference's multires solve is Jacobi PCG).
+//
+// The first coarse space is the uniform grid of 4^3 fine blocks.  With leaves of level
+// 0..2 every leaf cell lies inside one block (a level-2 cell is a whole block), so the
+// piecewise-constant prolongation P from blocks to leaf cells and R = P^T give a
+// Galerkin operator R S P that couples face-adjacent blocks only: a 7-point
+// conductance Laplacian on the (n/4)^3 grid whose face weight is the sum of the patch
+// conductances on the plane between two blocks, and whose extra diagonal is the sum of
+// the blocks' Dirichlet parts.  That is the shape of the uniform hierarchy's level 2,
+// so the cycle continues with the uniform coarse levels from there.  The smoother is
+// weighted Jacobi (or its Chebyshev weights) on S, with the row sums spread over a
+// block's lanes as in the multires SpMV.
+
+namespace pf {
+
+// R S P on the block grid: one wave per fine block (all blocks), lane = fine cell; the
+// face weights are the conductances of the 16 fine faces on the block's + planes
+__global__ void __launch_bounds__(256) k_mr_mg_coarse(MrOp op, Dims dcc, float* __restrict__ cd,
+                                                      float* __restrict__ cx, float* __restrict__ cy,
+                                                      float* __restrict__ cz) {
+  const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= op.m.dc.nblocks) return;  // whole waves
+  int bx, by, bz;
+  block_coords(op.m.dc, b, bx, by, bz);
+  const int li = lane & 3, lj = (lane >> 2) & 3, lk = lane >> 4;
+  const int i = bx * 4 + li, j = by * 4 + lj, k = bz * 4 + lk;
+  const bool in = i < op.m.g.nx && j < op.m.g.ny && k < op.m.g.nz;
+  float d = in ? op.cd[b * kBlockVol + lane] : 0.f;
+  float wx = (in && li == 3) ? op.cf[0][bm_index(op.m.du, i + 1, j, k)] : 0.f;
+  float wy = (in && lj == 3) ? op.cf[1][bm_index(op.m.dv, i, j + 1, k)] : 0.f;
+  float wz = (in && lk == 3) ? op.cf[2][bm_index(op.m.dw, i, j, k + 1)] : 0.f;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    d += __shfl_xor(d, o, 64);
+    wx += __shfl_xor(wx, o, 64);
+    wy += __shfl_xor(wy, o, 64);
+    wz += __shfl_xor(wz, o, 64);
+  }
+  if (lane == 0) {
+    const long long q = bm_index(dcc, bx, by, bz);
+    cd[q] = d;
+    cx[q] = wx;
+    cy[q] = wy;
+    cz[q] = wz;
+  }
+}
+
+// the residual r - S x at the anchor of this lane's cell (0 on other lanes), S x by
+// the lane-spread row sum; every lane of the wave must call it (shuffles)
+__device__ inline float mr_mg_residual(const MrOp& op, const float* __restrict__ r, const float* __restrict__ x,
+                                       long long c, int i, int j, int k, int L, bool in, bool anchor) {
+  const int s = 1 << L;
+  const long long anc = bm_index(op.m.dc, i & ~(s - 1), j & ~(s - 1), k & ~(s - 1));
+  const float xa = in ? x[anc] : 0.f;
+  const float sum = mr_apply_row_coop<float>(op.m, op.cf, i, j, k, L, in, xa, [&](long long n) { return x[n]; });
+  return anchor ? r[c] - (op.cd[c] * xa + sum) : 0.f;
+}
+
+#define MR_MG_LANE                                                                     \
+  const long long c = b * kBlockVol + lane;                                            \
+  int bx, by, bz;                                                                      \
+  block_coords(op.m.dc, b, bx, by, bz);                                                \
+  const int i = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane >> 4); \
+  const bool in = i < op.m.g.nx && j < op.m.g.ny && k < op.m.g.nz;                     \
+  const int L = mr_level(op.m, bx * 4, by * 4, bz * 4);                                \
+  if (L < 0) continue; /* wave-uniform */                                              \
+  const bool anchor = in && ((i | j | k) & ((1 << L) - 1)) == 0;
+
+// one smoothing sweep on the listed blocks: xo = xi + omega D^-1 (r - S xi) on anchors,
+// xo = omega D^-1 r when xi is null, 0 on the other lanes; with part, the partial
+// r . xo (one per workgroup); skipped while *done
+__global__ void __launch_bounds__(256) k_mr_mg_sweep(VecArgs a, MrOp op, const int* done, const float* __restrict__ inv,
+                                                     const float* __restrict__ r, const float* __restrict__ xi,
+                                                     float* __restrict__ xo, float omega, double* __restrict__ part) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & 63;
+  double rz = 0.0;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    MR_MG_LANE
+    float v = 0.f;
+    if (xi) {
+      const float res = mr_mg_residual(op, r, xi, c, i, j, k, L, in, anchor);
+      if (anchor) v = fmaf(omega * inv[c], res, xi[c]);
+    } else if (anchor) {
+      v = omega * inv[c] * r[c];
+    }
+    xo[c] = v;
+    if (anchor) rz += (double)r[c] * (double)v;
+  }
+  if (part) {
+    double v[1] = {rz};
+    const bool mx[1] = {false};
+    block_reduce_store<1>(v, mx, part, kMaxPartials);
+  }
+}
+
+// R (r - S x): the block sums of the anchor residuals into the block grid's slab
+__global__ void __launch_bounds__(256) k_mr_mg_restrict(VecArgs a, MrOp op, const int* done,
+                                                        const float* __restrict__ r, const float* __restrict__ x,
+                                                        Dims dcc, float* __restrict__ rc) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & 63;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    MR_MG_LANE
+    float res = mr_mg_residual(op, r, x, c, i, j, k, L, in, anchor);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) res += __shfl_xor(res, o, 64);
+    if (lane == 0) rc[bm_index(dcc, bx, by, bz)] = res;
+  }
+}
+
+// x += omega_c P xc on the anchors of the listed blocks (in place)
+__global__ void __launch_bounds__(256) k_mr_mg_prolong(VecArgs a, MrOp op, const int* done, Dims dcc,
+                                                       const float* __restrict__ xc, float* __restrict__ x,
+                                                       float omega_c) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & 63;
+  FOR_LIST_BLOCKS(a, b, lane) {
+    MR_MG_LANE
+    if (anchor) x[c] = fmaf(omega_c, xc[bm_index(dcc, bx, by, bz)], x[c]);
+  }
+}
+
+#undef MR_MG_LANE
+
+}  // namespace pf

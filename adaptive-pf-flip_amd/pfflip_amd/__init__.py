@@ -63,7 +63,7 @@ class EscapeStats(C.Structure):
 MODEL_TWO_PHASE, MODEL_SINGLE_PHASE = 0, 1
 
 
-MR_SOLVE_FAST, MR_SOLVE_REFERENCE = 0, 1
+MR_SOLVE_FAST, MR_SOLVE_REFERENCE, MR_SOLVE_MULTIGRID = 0, 1, 2
 
 
 class MRSolveCfg(C.Structure):

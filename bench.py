@@ -296,11 +296,11 @@ def run_workload(G, kind, n, seed, dt, steps, warmup, precond, adaptive=False, c
         ctx.close()
 
 
-def run_multires(G, n, seed, dt, steps, warmup, label=""):
+def run_multires(G, n, seed, dt, steps, warmup, mode, label=""):
     """ms/step and per-stage ms of the multires multi-bubble tank at n^3 effective: three
     levels (level 0 within 6 cells of the free surface and of 16 bubbles, level 1 within
-    22, level 2 beyond), static layout, the fast multires solve to the reference's stop
-    rule |r|_inf < 1e-6 |r0|_inf"""
+    22, level 2 beyond), static layout, the multires solve in `mode` (multigrid cycle or
+    the plain Jacobi PCG) to the reference's stop rule |r|_inf < 1e-6 |r0|_inf"""
     sim = G.MRSim3DTP(n, n, n, 1.0 / n)
     try:
         sim.ctx.scene_bubbles(seed, 16, 6)
@@ -309,6 +309,7 @@ def run_multires(G, n, seed, dt, steps, warmup, label=""):
         ph.rho_tilde_0 = G.calibrate_rho_tilde0(ph, 1.0)
         sim.phase, sim.dt, sim.dynamic_refinement, sim.timing = ph, dt, False, True
         sim.cg_iters, sim.cg_tol, sim.cg_rel_tol = 50000, 0.0, 1e-6
+        sim.solve_options = {"mode": mode}
         for _ in range(warmup):
             sim.step()
         sim.ctx.synchronize()
@@ -468,9 +469,11 @@ def main():
         workloads["dambreak256"] = run_workload(
             G, G.SCENE_DAMBREAK, 256, args.seed, None, 20, 5, args.precond, adaptive=True, cap=5e-3,
             label="dam-break 256^3 (liquid only), CFL 1, requested dt cap 5e-3")
-        workloads["multires512"] = run_multires(
-            G, 512, args.seed, 5e-4, 2, 1,
-            label="multires multi-bubble tank 512^3 effective, 3 levels, 16 bubbles, 1000:1, dt 5e-4")
+        mr_label = "multires multi-bubble tank 512^3 effective, 3 levels, 16 bubbles, 1000:1, dt 5e-4"
+        workloads["multires512"] = run_multires(G, 512, args.seed, 5e-4, 3, 1, G.MR_SOLVE_MULTIGRID,
+                                                label=mr_label + ", multigrid cycle on V A")
+        workloads["multires512_jacobi"] = run_multires(G, 512, args.seed, 5e-4, 1, 1, G.MR_SOLVE_FAST,
+                                                       label=mr_label + ", plain Jacobi PCG")
     cpu = pair = None
     if not args.no_cpu_baseline:
         print(f"[bench] gpu side of the {args.cpu_sample_n}^3 pair", file=sys.stderr, flush=True)

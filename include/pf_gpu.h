@@ -395,8 +395,11 @@ int pf_copy_probe(pf_ctx* ctx, int64_t n, double* ms);
  * origin 0, as the reference's multires code assumes. */
 enum pf_mr_solve_mode {
   PF_MR_SOLVE_FAST = 0,       /* FP32 Jacobi PCG with FP64 refinement; the option fields below are ignored */
-  PF_MR_SOLVE_REFERENCE = 1   /* the reference's FP64 iteration (multires_pressure3d.cpp:846-1449) with the
+  PF_MR_SOLVE_REFERENCE = 1,  /* the reference's FP64 iteration (multires_pressure3d.cpp:846-1449) with the
                                  options below, validated as validateSolveConfig (:307-333) */
+  PF_MR_SOLVE_MULTIGRID = 2   /* no reference counterpart: FP32 flexible PCG around an aggregation V-cycle on
+                                 V A (first coarse level: the 4^3 blocks) with FP64 refinement; leaves of
+                                 level 0..2, more than 8 cells per axis; option fields ignored */
 };
 
 typedef struct pf_mr_solve_cfg {    /* MRPressureSolveConfig3D (multires_pressure3d.h:137-166) */

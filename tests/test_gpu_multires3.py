@@ -55,7 +55,8 @@ def test_layout_refine_with_level_two_leaves():
     assert np.array_equal(G.mr_layout_refine(n, n, n, once, box), refine_box(n, n, n, once, box))
 
 
-def test_three_levels_match_oracle_and_exact_projection():
+@pytest.mark.parametrize("mode", [G.MR_SOLVE_FAST, G.MR_SOLVE_MULTIGRID])
+def test_three_levels_match_oracle_and_exact_projection(mode):
     n, dt = 64, 0.02
     ctx, g, ps, ph, gph = _setup(n, dt)
     assert np.array_equal(ctx.layout(), g.leaves())
@@ -75,7 +76,7 @@ def test_three_levels_match_oracle_and_exact_projection():
     cells, A, rhs, pin, vol = _system(g, ph, dt)
     x = exact_solution(A, rhs, vol)
     g.apply_pressure(ph, dt, x)
-    st = ctx.mr_project(gph, dt, G.mr_solve_cfg(20000, 1e-11, 0.0))
+    st = ctx.mr_project(gph, dt, G.mr_solve_cfg(20000, 1e-11, 0.0, mode=mode))
     assert st.converged == 1 and st.true_max_residual < 1e-11
     assert st.fluid_cells == len(cells) and st.pinned_cell == pin
     assert rel_err(ctx.field(G.FIELD_P), g.dense(O.P)) < 1e-6
@@ -169,3 +170,30 @@ def test_multi_bubble_scene():
     assert rel_err(ctx.field(G.FIELD_P), g.dense(O.P)) < 1e-6
     for f in range(3):
         assert rel_err(ctx.field(f), g.dense(f)) < 1e-5, f
+
+
+def test_multigrid_cycle_cuts_iterations():
+    # the V-cycle on V A (block grid, then the uniform levels) against Jacobi on the
+    # same three-level system: the same solution to the solve tolerance, several times
+    # fewer iterations
+    n, dt = 64, 0.02
+    runs = {}
+    for mode in (G.MR_SOLVE_FAST, G.MR_SOLVE_MULTIGRID):
+        ctx, g, ps, ph, gph = _setup(n, dt)
+        ctx.mr_mark_cells(); ctx.mr_p2g(gph, 1.0); ctx.apply_gravity(dt, -9.81); ctx.apply_walls()
+        st = ctx.mr_project(gph, dt, G.mr_solve_cfg(20000, 0.0, 1e-8, mode=mode))
+        assert st.converged == 1
+        runs[mode] = (st.iterations, ctx.field(G.FIELD_P))
+    assert rel_err(runs[G.MR_SOLVE_MULTIGRID][1], runs[G.MR_SOLVE_FAST][1]) < 1e-6
+    assert 4 * runs[G.MR_SOLVE_MULTIGRID][0] <= runs[G.MR_SOLVE_FAST][0], runs
+
+
+def test_multigrid_refuses_level_three():
+    n = 64
+    ctx = G.MRContext(n, n, n, 1.0)
+    ctx.set_layout(O.mr_layout_fine_box(n, n, n, [0, 0, 0, n, 8, n], level=3))
+    ctx.upload(*bubble_lattice(n))
+    ph, gph = _phase()
+    ctx.mr_mark_cells(); ctx.mr_p2g(gph, 1.0)
+    with pytest.raises(G.PfError, match="level 0 to 2"):
+        ctx.mr_project(gph, 0.02, G.mr_solve_cfg(100, 0.0, 1e-6, mode=G.MR_SOLVE_MULTIGRID))
