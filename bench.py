@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner in the comparison")
     ap.add_argument("--no-pressure", action="store_true", help="skip the pressure-solver comparison")
+    ap.add_argument("--multires-only", action="store_true",
+                    help="run only the multires 512^3-effective multigrid line and print its JSON")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the secondary lines (Jacobi step, bubble 128^3, dam-break 256^3)")
     ap.add_argument("--inner-rel-tol", type=float, default=None,
@@ -363,6 +365,12 @@ def main():
         sys.exit(2)
 
     G = _ensure_built()
+    if args.multires_only:
+        # the multires multigrid line alone (for profiling it): --steps timed steps
+        r = run_multires(G, 512, args.seed, 5e-4, args.steps, args.warmup, G.MR_SOLVE_MULTIGRID,
+                         label="multires multi-bubble tank 512^3 effective, 3 levels, multigrid cycle on V A")
+        print(json.dumps(r), flush=True)
+        return
     n = args.n
     ctx = G.Context(n, n, n, 1.0 / n)
     ctx.scene(G.SCENE_RT, args.seed)
