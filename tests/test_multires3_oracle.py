@@ -105,3 +105,35 @@ def test_three_level_exact_projection_is_divergence_free():
     g.apply_pressure(ph, dt, x)
     _, _, after, _ = g.system(ph, dt)
     assert np.abs(after).max() < 1e-9 * np.abs(rhs).max()
+
+
+def test_block_grid_galerkin_operator_is_seven_point():
+    # the multires cycle's first coarse space: leaf cells of level <= 2 aggregated by
+    # the 4^3 block that holds them (P piecewise constant, R = P^T).  R (V A) P couples
+    # face-adjacent blocks only, is symmetric, keeps the constants in the null space of
+    # its Neumann part, and its face weights are the conductance sums over the
+    # block planes that the GPU cycle assembles per fine face
+    n, dt = 64, 0.02
+    g, ph = _grid(n, three_level_bubble_layout(n), dt)
+    cells, A, rhs, pin, vol = _system(g, ph, dt)
+    lv = g.leaves()[cells // 64]
+    L, loc = lv[:, 0], cells % 64
+    s = 1 << L
+    fine = np.stack([lv[:, 1] * 4 * s + (loc & 3) * s, lv[:, 2] * 4 * s + ((loc >> 2) & 3) * s,
+                     lv[:, 3] * 4 * s + (loc >> 4) * s], 1)
+    blk = fine // 4
+    nb = n // 4
+    bid = blk[:, 0] + nb * (blk[:, 1] + nb * blk[:, 2])
+    ub, col = np.unique(bid, return_inverse=True)
+    P = sp.csr_matrix((np.ones(len(cells)), (np.arange(len(cells)), col)), shape=(len(cells), len(ub)))
+    S = (sp.diags(vol) @ A).tocsr()
+    Sc = (P.T @ S @ P).tocoo()
+    coords = np.stack([ub % nb, (ub // nb) % nb, ub // (nb * nb)], 1)
+    off = Sc.row != Sc.col
+    dist = np.abs(coords[Sc.row[off]] - coords[Sc.col[off]]).sum(1)
+    assert (dist == 1).all() and (Sc.data[off] <= 0).all()
+    Scc = Sc.tocsr()
+    assert abs(Scc - Scc.T).max() <= 1e-14 * abs(Scc).max()
+    rs = np.asarray(Scc.sum(axis=1)).ravel()
+    assert np.allclose(rs, P.T @ np.asarray(S.sum(axis=1)).ravel(), rtol=0, atol=1e-12 * abs(Scc).max())
+    assert (rs >= -1e-12 * Scc.diagonal()).all()
