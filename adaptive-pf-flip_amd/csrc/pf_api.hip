@@ -202,6 +202,7 @@ int pf_destroy(pf_ctx* c) {
   for (double* p : f64) hipFree(p);
   dfree(c->mr_lvl); dfree(c->mr_vol64); dfree(c->mr_pfull); dfree(c->mr_bbox); dfree(c->mr_leaves_dev);
   dfree(c->mrf); dfree(c->mrc); dfree(c->mr_pin_dev);
+  dfree(c->mr_f01); dfree(c->mr_f2); dfree(c->mr_list01); dfree(c->mr_list2);
   dfree(c->iface); dfree(c->iface_bad);
   dfree(c->pa_alive); dfree(c->pa_key); dfree(c->pa_by_id); dfree(c->pa_new_id);
   dfree(c->pa_cflag); dfree(c->pa_d1); dfree(c->pa_d2); dfree(c->pa_wanted); dfree(c->pa_wscan); dfree(c->pa_cnt);

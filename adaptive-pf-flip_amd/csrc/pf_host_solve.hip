@@ -155,6 +155,35 @@ int mg_coarsen_from(pf_ctx* c, MgLevel F, size_t from) {
   return PF_OK;
 }
 
+// The multires fluid list split by level for the fast solves: blocks of level 0-1 (a
+// wave each) and level-2 blocks (one cell each, four to a wave).  The level-2 kernels
+// write a block's cell entry only, so the PCG's z, p and A p start as zeros.
+int mr_split_lists(pf_ctx* c, const MrGrid& m) {
+  const long long nb = c->dc.nblocks;
+  if (!c->mr_list2) {
+    HIP_TRY(c, dalloc(&c->mr_f01, nb));
+    HIP_TRY(c, dalloc(&c->mr_f2, nb));
+    HIP_TRY(c, dalloc(&c->mr_list01, nb));
+    HIP_TRY(c, dalloc(&c->mr_list2, nb));
+  }
+  HIP_TRY(c, fill_u8(c->stream, c->mr_f01, 0, nb));
+  HIP_TRY(c, fill_u8(c->stream, c->mr_f2, 0, nb));
+  hipLaunchKernelGGL(k_mr_mg_split, dim3(grid_for(c->npblk, 256)), dim3(256), 0, c->stream, c->pblk, c->npblk, m,
+                     c->mr_f01, c->mr_f2);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, compact_flags_morton(*c, c->dc, c->mr_f01, c->mr_list01, c->u32_dev + 12));
+  HIP_TRY(c, compact_flags_morton(*c, c->dc, c->mr_f2, c->mr_list2, c->u32_dev + 13));
+  HIP_TRY(c, hipMemcpyAsync(c->u32_host + 12, c->u32_dev + 12, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->mr_n01 = c->u32_host[12];
+  c->mr_n2 = c->u32_host[13];
+  if (c->mr_n01 + c->mr_n2 != c->npblk) return fail(c, PF_ERR_STATE, "multires block split lost blocks");
+  float* arrs[4] = {c->z32, c->pA, c->pB, c->Ap};
+  for (float* q : arrs) HIP_TRY(c, fill_f32(c->stream, q, 0.f, nb * kBlockVol));
+  return PF_OK;
+}
+
 // The multires cycle's levels: the block grid (the uniform hierarchy's second level,
 // c->mg[1]) from R S P, then the uniform levels below it.  Leaves of level 0..2 only.
 int mr_mg_build(pf_ctx* c, const MrOp& op) {
@@ -175,6 +204,9 @@ int mr_mg_build(pf_ctx* c, const MrOp& op) {
   hipLaunchKernelGGL(k_mr_mg_coarse, dim3(grid_for(c->dc.nblocks, 4)), dim3(256), 0, c->stream, op, b.d, b.cd, b.cx,
                      b.cy, b.cz);
   LAUNCH_CHECK(c);
+  // the level-2 kernels write the cell entry of a block only: the other entries of the
+  // cycle's work array stay zero
+  HIP_TRY(c, fill_f32(c->stream, c->mg_t0, 0.f, c->dc.nblocks * kBlockVol));
   int st = mg_level_finish(c, b);
   if (st) return st;
   return mg_coarsen_from(c, mg_level(b), 2);
@@ -435,9 +467,17 @@ int mr_mg_apply(pf_ctx* c, const MrOp& op, const VecArgs& va, const float* r, fl
   const int nu = std::max(1, c->mg_cfg.sweeps);
   std::vector<float> wpre = mg_sweep_weights(c);
   wpre.resize((size_t)nu, c->mg_cfg.omega);
-  const unsigned g0 = mg_grid0n(va.nlist), gw = list_grid(va.nlist);
+  const unsigned g0 = mg_grid0n(va.nlist);
   double* const rz = part + 2 * kMaxPartials;
   const float* inv = c->invd32;
+  // the blocks of level 0-1 (a wave each) and of level 2 (four to a wave)
+  VecArgs v01 = va;
+  v01.list = c->mr_list01;
+  v01.nlist = c->mr_n01;
+  const uint32_t* l2 = c->mr_list2;
+  const long long n2 = c->mr_n2;
+  const unsigned gw = list_grid(v01.nlist);
+  const unsigned gw2 = (unsigned)std::clamp<long long>((n2 + 15) / 16, 8, 4096);
   // 2 nu writes (the prolongation adds in place): the first lands in the work array
   float* cur = nullptr;
   float* nxt = c->mg_t0;
@@ -445,22 +485,25 @@ int mr_mg_apply(pf_ctx* c, const MrOp& op, const VecArgs& va, const float* r, fl
     cur = nxt;
     nxt = (cur == z) ? c->mg_t0 : z;
   };
-  for (int q = 0; q < nu; ++q) {
-    hipLaunchKernelGGL(k_mr_mg_sweep, dim3(gw), dim3(256), 0, c->stream, va, op, done, inv, r, cur, nxt,
-                       wpre[(size_t)q], nullptr);
+  // one sweep over both lists; the last one leaves the partial r.z in slot 2 (the
+  // level-2 kernel adds its share to the partials the other one wrote)
+  auto sweep = [&](float w, bool last) {
+    hipLaunchKernelGGL(k_mr_mg_sweep, dim3(last ? g0 : gw), dim3(256), 0, c->stream, v01, op, done, inv, r, cur, nxt, w,
+                       last ? rz : nullptr);
+    hipLaunchKernelGGL(k_mr_mg_sweep2, dim3(last ? g0 : gw2), dim3(256), 0, c->stream, l2, n2, op, done, inv, r, cur,
+                       nxt, w, last ? rz : nullptr);
     flip();
-  }
+  };
+  for (int q = 0; q < nu; ++q) sweep(wpre[(size_t)q], false);
   Ctx::MgBuf& b = c->mg[1];
-  hipLaunchKernelGGL(k_mr_mg_restrict, dim3(gw), dim3(256), 0, c->stream, va, op, done, r, cur, b.d, b.r);
+  hipLaunchKernelGGL(k_mr_mg_restrict, dim3(gw), dim3(256), 0, c->stream, v01, op, done, r, cur, b.d, b.r);
+  hipLaunchKernelGGL(k_mr_mg_restrict2, dim3(gw2), dim3(256), 0, c->stream, l2, n2, op, done, r, cur, b.d, b.r);
   float* const xc = mg_coarse(c, 2, done, wpre);
-  hipLaunchKernelGGL(k_mr_mg_prolong, dim3(gw), dim3(256), 0, c->stream, va, op, done, b.d, xc, cur,
+  hipLaunchKernelGGL(k_mr_mg_prolong, dim3(gw), dim3(256), 0, c->stream, v01, op, done, b.d, xc, cur,
                      c->mg_cfg.correction);
-  for (int q = 0; q < nu; ++q) {
-    const bool last = q == nu - 1;
-    hipLaunchKernelGGL(k_mr_mg_sweep, dim3(last ? g0 : gw), dim3(256), 0, c->stream, va, op, done, inv, r, cur, nxt,
-                       wpre[(size_t)(nu - 1 - q)], last ? rz : nullptr);
-    flip();
-  }
+  hipLaunchKernelGGL(k_mr_mg_prolong2, dim3(gw2), dim3(256), 0, c->stream, l2, n2, op, done, b.d, xc, cur,
+                     c->mg_cfg.correction);
+  for (int q = 0; q < nu; ++q) sweep(wpre[(size_t)(nu - 1 - q)], q == nu - 1);
   LAUNCH_CHECK(c);
   if (cur != z) return fail(c, PF_ERR_STATE, "multires multigrid: cycle did not end in z");
   return PF_OK;
@@ -714,6 +757,10 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
   // weights (z and partial r.z in slot 2, GT partials)
   bool learned = want_learned || want_mg;
   c->mg_cs_active = c->mg_cs;  // caller-set weights (pf_mg_set_correction_weights) or none
+  if (mr && !refmode) {
+    st = mr_split_lists(c, mop.m);
+    if (st) return st;
+  }
   if (learned) {
     st = mr ? mr_mg_build(c, mop) : mg_build(c);
     if (st) return st;
@@ -781,6 +828,10 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
     // the buffer-addressed SpMV takes 32-bit byte offsets: arrays below 2 GiB
     const bool spmv_buffers = c->dc.nblocks * kBlockVol * 4 < (1LL << 31);
     VecArgs vs = va;
+    // the multires SpMV: blocks of level 0-1 here, level-2 blocks in a second launch
+    VecArgs vs01 = va;
+    vs01.list = c->mr_list01;
+    vs01.nlist = c->mr_n01;
     int iter_before = 0, iter_base = 0;
     double best_rr = rr, prev_rr = rr;
     int no_gain = 0, learned_slow = 0;
@@ -797,7 +848,10 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
       for (int q = 0; q < q_next; ++q) {
         if (mr)
           TIMED(c, PF_K_PCG_SPMV,
-                hipLaunchKernelGGL(k_mr_spmv, dim3(GS), dim3(256), 0, c->stream, vs, mop, c->z32, c->pA, c->pB, c->Ap));
+                hipLaunchKernelGGL(k_mr_spmv, dim3(GS), dim3(256), 0, c->stream, vs01, mop, c->z32, c->pA, c->pB,
+                                   c->Ap);
+                hipLaunchKernelGGL(k_mr_spmv2, dim3(GS), dim3(256), 0, c->stream, c->mr_list2, c->mr_n2, vs, mop,
+                                   c->z32, c->pA, c->pB, c->Ap));
         else if (spmv_buffers)
         TIMED(c, PF_K_PCG_SPMV,
               hipLaunchKernelGGL(k_pcg_spmv, dim3(GS), dim3(256), 0, c->stream, vs, c->z32, c->pA, c->pB, c->Ap,

@@ -218,6 +218,11 @@ struct Ctx {
   pf_mr_solve_detail mr_detail{};
   double* mrf = nullptr;            // 15 fine FP64 vectors (ncell each)
   double* mrc = nullptr;            // 6 coarse FP64 vectors
+  // the multires cycle's block lists: levels 0-1 (a wave per block) and level 2 (four
+  // blocks per wave), split from the fluid list per solve
+  uint8_t *mr_f01 = nullptr, *mr_f2 = nullptr;
+  uint32_t *mr_list01 = nullptr, *mr_list2 = nullptr;
+  long long mr_n01 = 0, mr_n2 = 0;
   long long mrc_cap = 0;
   long long* mr_pin_dev = nullptr;  // pinned anchor (block-major index) or -1
   long long* mr_pin_host = nullptr;

@@ -331,19 +331,18 @@ __device__ inline T mr_apply_row(const MrGrid& m, const T* const* cf, int i, int
 // neighbour level byte (independent), then the neighbour anchors' values, so a lane
 // waits on three loads in a row instead of three per patch.  The xor shuffles over
 // the offset bits (lane = x + 4 y + 16 z) leave every lane of the cell with the total.
-// Wave-uniform L; lanes with live == false add nothing.
+// Wave-uniform L; lanes with live == false add nothing.  mr_row_slots is one lane's
+// share: patches r0, r0 + stride, ... (at most 6) of the cell anchored at (ax, ay, az).
 template <typename T, typename Val>
-__device__ inline T mr_apply_row_coop(const MrGrid& m, const T* const* cf, int i, int j, int k, int L, bool live,
-                                      T xa, Val&& val) {
-  const int s = 1 << L, s2 = s * s, s3 = s2 * s;
-  const int ax = i & ~(s - 1), ay = j & ~(s - 1), az = k & ~(s - 1);
-  const int r0 = (i - ax) + s * ((j - ay) + s * (k - az));
+__device__ inline T mr_row_slots(const MrGrid& m, const T* const* cf, int ax, int ay, int az, int s, int r0, int stride,
+                                 bool live, T xa, Val&& val) {
+  const int s2 = s * s;
   T c[6];
   int n[6][3];
   int lv[6];
 #pragma unroll
   for (int t = 0; t < 6; ++t) {
-    const int q = r0 + t * s3;
+    const int q = r0 + t * stride;
     c[t] = (T)0;
     lv[t] = -1;
     if (!live || q >= 6 * s2) continue;
@@ -373,6 +372,15 @@ __device__ inline T mr_apply_row_coop(const MrGrid& m, const T* const* cf, int i
       acc += c[t] * (xa - val(bm_index(m.dc, n[t][0] & ~(sn - 1), n[t][1] & ~(sn - 1), n[t][2] & ~(sn - 1))));
     }
   }
+  return acc;
+}
+
+template <typename T, typename Val>
+__device__ inline T mr_apply_row_coop(const MrGrid& m, const T* const* cf, int i, int j, int k, int L, bool live,
+                                      T xa, Val&& val) {
+  const int s = 1 << L;
+  const int ax = i & ~(s - 1), ay = j & ~(s - 1), az = k & ~(s - 1);
+  T acc = mr_row_slots<T>(m, cf, ax, ay, az, s, (i - ax) + s * ((j - ay) + s * (k - az)), s * s * s, live, xa, val);
   for (int bit = 0; bit < L; ++bit) {
     acc += __shfl_xor(acc, 1 << bit, 64);
     acc += __shfl_xor(acc, 4 << bit, 64);

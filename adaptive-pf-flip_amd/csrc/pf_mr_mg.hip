@@ -125,4 +125,136 @@ __global__ void __launch_bounds__(256) k_mr_mg_prolong(VecArgs a, MrOp op, const
 
 #undef MR_MG_LANE
 
+// splits the listed blocks by level: f01 (levels 0 and 1) and f2 (level 2)
+__global__ void __launch_bounds__(256) k_mr_mg_split(const uint32_t* __restrict__ list, long long nlist, MrGrid m,
+                                                     uint8_t* __restrict__ f01, uint8_t* __restrict__ f2) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nlist) return;
+  const uint32_t b = list[t];
+  int bx, by, bz;
+  block_coords(m.dc, b, bx, by, bz);
+  const int L = mr_level(m, bx * 4, by * 4, bz * 4);
+  f01[b] = (L == 0 || L == 1) ? 1 : 0;
+  f2[b] = L == 2 ? 1 : 0;
+}
+
+// Level-2 blocks hold one leaf cell each (anchored at the block origin), so a wave
+// takes four of them: 16 lanes per block, lane r of a block the patches r, r + 16, ...,
+// r + 80 of its 96, summed by xor shuffles inside the 16 lanes.  The cell's values sit
+// in lane 0 of the block's slab entries; the other 63 entries of a level-2 block are
+// left alone (the cycle's arrays hold zeros there).
+#define MR_MG_L2_WALK                                                                      \
+  const int lane = threadIdx.x & 63, rl = lane & 15;                                       \
+  for (long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); t * 4 < n;            \
+       t += (long long)gridDim.x * 4)
+
+#define MR_MG_L2_BLOCK                                                                     \
+  const long long e = t * 4 + (lane >> 4);                                                 \
+  const bool live = e < n;                                                                 \
+  const uint32_t b = live ? list[e] : 0u;                                                  \
+  int bx, by, bz;                                                                          \
+  block_coords(op.m.dc, b, bx, by, bz);                                                    \
+  const long long c = (long long)b * kBlockVol;
+
+__device__ inline float mr_l2_sum(float v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 8, 64);
+  return v;
+}
+
+// the level-2 sweep (see k_mr_mg_sweep); with part, the workgroup's partial r . xo is
+// added to part[blockIdx.x], which the sweep over the other blocks wrote just before
+__global__ void __launch_bounds__(256) k_mr_mg_sweep2(const uint32_t* __restrict__ list, long long n, MrOp op,
+                                                      const int* done, const float* __restrict__ inv,
+                                                      const float* __restrict__ r, const float* __restrict__ xi,
+                                                      float* __restrict__ xo, float omega, double* __restrict__ part) {
+  if (done && *done) return;
+  double rz = 0.0;
+  MR_MG_L2_WALK {
+    MR_MG_L2_BLOCK
+    const float xa = (live && xi) ? xi[c] : 0.f;
+    float sum = 0.f;
+    if (xi)
+      sum = mr_l2_sum(mr_row_slots<float>(op.m, op.cf, bx * 4, by * 4, bz * 4, 4, rl, 16, live, xa,
+                                          [&](long long q) { return xi[q]; }));
+    if (live && rl == 0) {
+      const float rc = r[c];
+      const float v = xi ? fmaf(omega * inv[c], rc - (op.cd[c] * xa + sum), xa) : omega * inv[c] * rc;
+      xo[c] = v;
+      rz += (double)rc * (double)v;
+    }
+  }
+  if (part) {
+    __shared__ double s_rz[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rz += __shfl_xor(rz, o, 64);
+    if ((threadIdx.x & 63) == 0) s_rz[threadIdx.x >> 6] = rz;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&part[blockIdx.x], s_rz[0] + s_rz[1] + s_rz[2] + s_rz[3]);
+  }
+}
+
+// the level-2 restriction: the one cell's residual to its block-grid cell
+__global__ void __launch_bounds__(256) k_mr_mg_restrict2(const uint32_t* __restrict__ list, long long n, MrOp op,
+                                                         const int* done, const float* __restrict__ r,
+                                                         const float* __restrict__ x, Dims dcc,
+                                                         float* __restrict__ rc) {
+  if (done && *done) return;
+  MR_MG_L2_WALK {
+    MR_MG_L2_BLOCK
+    const float xa = live ? x[c] : 0.f;
+    const float sum = mr_l2_sum(mr_row_slots<float>(op.m, op.cf, bx * 4, by * 4, bz * 4, 4, rl, 16, live, xa,
+                                                    [&](long long q) { return x[q]; }));
+    if (live && rl == 0) rc[bm_index(dcc, bx, by, bz)] = r[c] - (op.cd[c] * xa + sum);
+  }
+}
+
+// the level-2 prolongation: x += omega_c xc at the cell
+__global__ void __launch_bounds__(256) k_mr_mg_prolong2(const uint32_t* __restrict__ list, long long n, MrOp op,
+                                                        const int* done, Dims dcc, const float* __restrict__ xc,
+                                                        float* __restrict__ x, float omega_c) {
+  if (done && *done) return;
+  MR_MG_L2_WALK {
+    MR_MG_L2_BLOCK
+    if (live && rl == 0) x[c] = fmaf(omega_c, xc[bm_index(dcc, bx, by, bz)], x[c]);
+  }
+}
+
+// the level-2 part of the multires SpMV (k_mr_spmv on the other blocks): p_new = z +
+// beta p_old at the cell, A p there, the workgroup's partial p.Ap added to
+// a.part[blockIdx.x], which the launch over the other blocks wrote just before
+__global__ void __launch_bounds__(256) k_mr_spmv2(const uint32_t* __restrict__ list, long long n, VecArgs a, MrOp op,
+                                                  const float* __restrict__ z, float* __restrict__ pA,
+                                                  float* __restrict__ pB, float* __restrict__ Ap) {
+  if (a.sc->done) return;
+  const bool odd = a.sc->iter & 1;
+  const float* __restrict__ pold = odd ? pB : pA;
+  float* __restrict__ pnew = odd ? pA : pB;
+  const float beta = (float)a.sc->beta;
+  auto pval = [&](long long q) { return fmaf(beta, pold[q], z[q]); };
+  double pap = 0.0;
+  MR_MG_L2_WALK {
+    MR_MG_L2_BLOCK
+    const float pc = live ? pval(c) : 0.f;
+    const float sum = mr_l2_sum(mr_row_slots<float>(op.m, op.cf, bx * 4, by * 4, bz * 4, 4, rl, 16, live, pc, pval));
+    if (live && rl == 0) {
+      const float ap = op.cd[c] * pc + sum;
+      pnew[c] = pc;
+      Ap[c] = ap;
+      pap += (double)pc * (double)ap;
+    }
+  }
+  __shared__ double s_pap[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pap += __shfl_xor(pap, o, 64);
+  if ((threadIdx.x & 63) == 0) s_pap[threadIdx.x >> 6] = pap;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&a.part[blockIdx.x], s_pap[0] + s_pap[1] + s_pap[2] + s_pap[3]);
+}
+
+#undef MR_MG_L2_WALK
+#undef MR_MG_L2_BLOCK
+
 }  // namespace pf
