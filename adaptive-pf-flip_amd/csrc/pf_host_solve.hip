@@ -863,15 +863,19 @@ int run_project(pf_ctx* c, const pf_phase* ph, double dt, const pf_solve_cfg* cf
         hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, c->sc);
         if (learned) {
           TIMED(c, PF_K_PCG_UPDATE,
-                hipLaunchKernelGGL(k_pcg_update_flex, dim3(GS), dim3(256), 0, c->stream, vs, c->x32, c->r32, c->z32,
-                                   c->pA, c->pB, c->Ap));
+                hipLaunchKernelGGL(k_pcg_update_flex, dim3(GS), dim3(256), 0, c->stream, mr ? vs01 : vs, c->x32,
+                                   c->r32, c->z32, c->pA, c->pB, c->Ap);
+                if (mr) hipLaunchKernelGGL(k_mr_update2<true>, dim3(GS), dim3(256), 0, c->stream, c->mr_list2,
+                                           c->mr_n2, vs, c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->invd32));
           st = papply(done_dev);
           if (st) return st;
           hipLaunchKernelGGL(k_pcg_beta_flex, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, (int)GT, c->sc);
         } else {
           TIMED(c, PF_K_PCG_UPDATE,
-                hipLaunchKernelGGL(k_pcg_update, dim3(GS), dim3(256), 0, c->stream, vs, c->x32, c->r32, c->z32, c->pA,
-                                   c->pB, c->Ap, c->invd32));
+                hipLaunchKernelGGL(k_pcg_update, dim3(GS), dim3(256), 0, c->stream, mr ? vs01 : vs, c->x32, c->r32,
+                                   c->z32, c->pA, c->pB, c->Ap, c->invd32);
+                if (mr) hipLaunchKernelGGL(k_mr_update2<false>, dim3(GS), dim3(256), 0, c->stream, c->mr_list2,
+                                           c->mr_n2, vs, c->x32, c->r32, c->z32, c->pA, c->pB, c->Ap, c->invd32));
           hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(256), 0, c->stream, c->partial, (int)GS, c->sc);
         }
       }

@@ -257,4 +257,50 @@ __global__ void __launch_bounds__(256) k_mr_spmv2(const uint32_t* __restrict__ l
 #undef MR_MG_L2_WALK
 #undef MR_MG_L2_BLOCK
 
+// the PCG update on the level-2 blocks' cells (k_pcg_update / k_pcg_update_flex on the
+// other blocks), one lane per block: x += alpha p, r -= alpha A p, and z = D^-1 r
+// unless FLEX; the workgroup's partials r.z (FLEX: r.z of the cycle's z) and r.r are
+// added to a.part slots 0 and 1 at blockIdx.x, which the other launch wrote just before
+template <bool FLEX>
+__global__ void __launch_bounds__(256) k_mr_update2(const uint32_t* __restrict__ list, long long n, VecArgs a,
+                                                    float* __restrict__ x, float* __restrict__ r,
+                                                    float* __restrict__ z, const float* __restrict__ pA,
+                                                    const float* __restrict__ pB, const float* __restrict__ Ap,
+                                                    const float* __restrict__ invd) {
+  if (a.sc->done) return;
+  const float* __restrict__ p = (a.sc->iter & 1) ? pA : pB;
+  const float alpha = (float)a.sc->alpha;
+  double v0 = 0.0, v1 = 0.0;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const long long c = (long long)list[e] * kBlockVol;
+    const float xn = fmaf(alpha, p[c], x[c]);
+    const float rn = fmaf(-alpha, Ap[c], r[c]);
+    x[c] = xn;
+    r[c] = rn;
+    if (FLEX) {
+      v0 += (double)rn * (double)z[c];
+    } else {
+      const float zn = invd[c] * rn;
+      z[c] = zn;
+      v0 += (double)rn * (double)zn;
+    }
+    v1 += (double)rn * (double)rn;
+  }
+  __shared__ double s_v[4][2];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v0 += __shfl_xor(v0, o, 64);
+    v1 += __shfl_xor(v1, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_v[threadIdx.x >> 6][0] = v0;
+    s_v[threadIdx.x >> 6][1] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&a.part[blockIdx.x], s_v[0][0] + s_v[1][0] + s_v[2][0] + s_v[3][0]);
+    atomicAdd(&a.part[kMaxPartials + blockIdx.x], s_v[0][1] + s_v[1][1] + s_v[2][1] + s_v[3][1]);
+  }
+}
+
 }  // namespace pf
