@@ -148,6 +148,32 @@ void MRSim3DTP::initBubbleTankInterfaceBand() {
   if (dynamic_refinement) updateDynamicRefinement();
 }
 
+std::vector<float> MRSim3DTP::initMultiBubbleTank(uint64_t seed, int nbubbles, int band) {
+  pf_phase ph{phase.rho_l, phase.rho_g, phase.alpha_phi, phase.rho_tilde_0};
+  phase.rho_tilde_0 = pf_calibrate_rho_tilde0(&ph, Vp);
+  dynamic_budget_limited = false;
+  dynamic_last_fine_leaves = 0;
+  dynamic_retained_box_valid = false;
+  dynamic_refinement = false;  // the layout is static
+  particle_boundary_clamped_liquid_last = particle_boundary_clamped_gas_last = 0;
+  effective_dt_last = dt;
+  max_particle_speed_last = 0.0;
+  cfl_limit_dt_last = dt;
+  adaptive_timestep_limited_last = 0;
+  std::vector<float> bubbles(4 * (size_t)std::max(nbubbles, 0));
+  check(pf_mr_scene_bubbles(ctx_, seed, nbubbles, band, bubbles.data()), "pf_mr_scene_bubbles");
+  fetchLayout();
+  syncParticlesToHost();
+  liquid_volume_current_last = 0.0;
+  for (size_t p = 0; p < particles.size(); ++p)
+    if (particles.type[p] == 0) liquid_volume_current_last += particles.volume[p] * Vp;
+  liquid_volume_target = liquid_volume_current_last;
+  liquid_volume_error_last = 0.0;
+  c_div_last = 0.0;
+  dirty_ = false;
+  return bubbles;
+}
+
 void MRSim3DTP::upload() {
   const size_t n = particles.size();
   std::vector<float> x(n), y(n), z(n), u(n), v(n), w(n), vol(n);
@@ -236,7 +262,8 @@ void MRSim3DTP::step() {
   sc.coarse_preconditioner_auto_disable_after = cg_coarse_preconditioner_auto_disable_after;
   const bool options = !cg_jacobi_preconditioner || cg_flexible_beta || cg_relaxation_sweeps > 0 ||
                        cg_coarse_correction || cg_coarse_preconditioner;
-  sc.mode = (options || cg_reference_order) ? PF_MR_SOLVE_REFERENCE : PF_MR_SOLVE_FAST;
+  sc.mode = (options || cg_reference_order) ? PF_MR_SOLVE_REFERENCE
+                                            : (cg_multigrid ? PF_MR_SOLVE_MULTIGRID : PF_MR_SOLVE_FAST);
   cfg.refine.dynamic_refinement = dynamic_refinement ? 1 : 0;
   cfg.refine.particle_padding = dynamic_particle_padding;
   cfg.refine.gas_padding = dynamic_gas_padding;

@@ -57,6 +57,9 @@ class MRSim3DTP : public ParticleAdaptivityFields {
   double liquid_volume_error_last = 0.0, c_div_last = 0.0;
   int cg_iters = 160;
   double cg_tol = 1e-7, cg_rel_tol = 1e-6;
+  // the fast solve with the multires multigrid cycle (PF_MR_SOLVE_MULTIGRID, no
+  // reference counterpart; leaves of level 0 to 2)
+  bool cg_multigrid = false;
   // the reference's solver options (multires_sim3d_tp.h:37-60); any non-default one
   // runs the solve in the reference's FP64 order (PF_MR_SOLVE_REFERENCE)
   bool cg_jacobi_preconditioner = true, cg_flexible_beta = false, cg_adaptive_restart = true;
@@ -102,6 +105,10 @@ class MRSim3DTP : public ParticleAdaptivityFields {
   MRSim3DTP& operator=(const MRSim3DTP&) = delete;
 
   void initBubbleTankInterfaceBand();
+  // the three-level multi-bubble tank (pf_mr_scene_bubbles; no reference counterpart):
+  // static layout by distance to the interfaces, particles seeded on the device; returns
+  // the bubbles as centre x, y, z and radius per bubble
+  std::vector<float> initMultiBubbleTank(uint64_t seed, int nbubbles = 16, int band = 6);
   void updateDynamicRefinement();
   void step();
 

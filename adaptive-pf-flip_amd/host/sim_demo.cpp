@@ -408,6 +408,30 @@ int main() {
     CHECK(threw);
     std::printf("render cache: 2 frames, %lld and %lld bytes\n", frames[0].bytes, frames[1].bytes);
   }
+  {
+    // the three-level multi-bubble tank with the multires multigrid cycle (no reference
+    // counterpart): levels 0 to 2 present, the solve converges to the reference's stop
+    // rule, particles stay finite
+    MRSim3DTP sim(96, 96, 96, 1.0);
+    sim.phase.rho_l = 1.0;
+    sim.phase.rho_g = 0.01;
+    sim.dt = 0.02;
+    sim.cg_iters = 2000;
+    sim.cg_tol = 0.0;
+    sim.cg_rel_tol = 1e-6;
+    sim.cg_multigrid = true;
+    const std::vector<float> bubbles = sim.initMultiBubbleTank(7, 4, 6);
+    CHECK(bubbles.size() == 16);
+    CHECK(sim.countLevel(0) > 0 && sim.countLevel(1) > 0 && sim.countLevel(2) > 0);
+    const size_t n0 = sim.particles.size();
+    CHECK(n0 > 0);
+    sim.step();
+    CHECK(sim.stats_last.solve.converged == 1);
+    CHECK(sim.particles.size() == n0);
+    CHECK(finite(sim));
+    std::printf("multires 3 levels: %zu particles, multigrid solve %d iterations\n", n0,
+                sim.stats_last.solve.iterations);
+  }
   std::printf(failures ? "FAILED %d checks\n" : "all checks passed\n", failures);
   return failures ? 1 : 0;
 }
